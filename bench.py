@@ -1,0 +1,192 @@
+"""Benchmark: single-frame wavefront GI render on one MI355X.
+
+Workload (one step = one frame): seeded fBm heightfield terrain of 1,000,000
+triangles (1001 x 501 vertices), chunked GGX materials (roughness ~ U[0.05,1],
+metallic ~ Bernoulli(0.3)), 64 emissive triangles plus a constant
+environment of radiance 0.5; 1920 x 1080, 8 samples per pixel, 8 bounces,
+float32.  Metric: Mrays/s = (primary + extension + shadow rays traced) /
+device time of the frame (scene build excluded, image kept in HBM).
+
+Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
+this build measures one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, RAY_IN, HIT_OUT, OCC_OUT = 64, 48, 32, 16, 4
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(scene, camera, width, height, spp, bounces, target_seconds):
+    """Time the float32 C oracle on a band of image rows on all host cores."""
+    from ordinarylight_amd import cameras, packing
+    from oracle import native as oracle
+    threads = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    started = time.perf_counter()
+    orc = oracle.OracleScene(packing.pack_scene(scene))
+    build_s = time.perf_counter() - started
+    basis = cameras.camera_basis(camera)
+    proj = cameras.camera_projection(camera)
+    rows = 4
+    elapsed = 0.0
+    rays = 0
+    row0 = height // 2
+    while True:
+        r0 = max(0, row0 - rows // 2)
+        r1 = min(height, r0 + rows)
+        t0 = time.perf_counter()
+        orc.render(basis, proj, width, height, spp, 0, max_bounces=bounces, rows=(r0, r1))
+        elapsed = time.perf_counter() - t0
+        c = orc.last_counts
+        rays = int(c[0] + c[1] + c[2])
+        if elapsed >= target_seconds * 0.5 or r1 - r0 >= height:
+            break
+        rows = min(height, max(rows * 2, int(rows * target_seconds / max(elapsed, 1e-3))))
+    orc.close()
+    return {
+        "value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+        "kind": "port",
+        "sample": f"rows {r0}-{r1} of {width}x{height}, {spp} spp, {bounces} bounces "
+                  f"({rays} rays incl. shadow in {elapsed:.1f} s; oracle BVH build {build_s:.1f} s; {cpu_model()})",
+    }
+
+
+def main(argv=None):
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=5)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--width", type=int, default=1920)
+    parser.add_argument("--height", type=int, default=1080)
+    parser.add_argument("--spp", type=int, default=8)
+    parser.add_argument("--bounces", type=int, default=8)
+    parser.add_argument("--cpu-seconds", type=float, default=12.0)
+    parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--no-profile", action="store_true")
+    parser.add_argument("--leaf-size", type=int, default=4)
+    args = parser.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path shards by image tiles or jobs across GPUs; see the design notes)", file=sys.stderr)
+        return 2
+
+    from ordinarylight_amd import workloads
+    from ordinarylight_amd.renderers.hip import HipPathTracer
+
+    t0 = time.perf_counter()
+    scene, camera = workloads.terrain(0)
+    log(f"scene: {scene.triangle_count} triangles, generated in {time.perf_counter() - t0:.1f} s")
+    W, H, spp, B = args.width, args.height, args.spp, args.bounces
+    renderer = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
+                             bvh_leaf_size=args.leaf_size)
+    # Scene upload + BVH build (reported separately, outside the timed region).
+    renderer.replace_scene(scene)
+    ds = renderer._device_scene
+    log(f"bvh: {ds.info.node_count} nodes, depth {ds.info.bvh_depth}, build {ds.info.build_ms:.1f} ms")
+
+    # Work-counting pass (not timed): nodes/triangles per traversal.
+    renderer.reconfigure(work_counters=True)
+    renderer._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+    work = renderer.last_stats
+    renderer.reconfigure(work_counters=False, profiling=not args.no_profile)
+
+    for _ in range(args.warmup):
+        renderer._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+    rays = 0
+    dev_ms = 0.0
+    trace_ms = 0.0
+    trace_launches = 0
+    t_start = time.perf_counter()
+    for step in range(args.steps):
+        renderer._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+        st = renderer.last_stats
+        rays += st.primary_rays + st.extension_rays + st.shadow_rays
+        dev_ms += st.total_ms
+        trace_ms += st.trace_ms
+        trace_launches += st.trace_launches
+    wall_s = time.perf_counter() - t_start
+    ms_per_step = wall_s * 1000.0 / args.steps
+    value = rays / (wall_s * 1e6)
+
+    # PCIe-inclusive: one frame with the image copied back to the host.
+    t1 = time.perf_counter()
+    renderer.render_frame(scene, camera, W, H)
+    pcie_ms = (time.perf_counter() - t1) * 1000.0
+
+    closest_rays = work.primary_rays + work.extension_rays
+    alg_bytes = (work.closest_nodes * NODE_BYTES + work.closest_tris * TRI_BYTES
+                 + closest_rays * (RAY_IN + HIT_OUT))
+    roofline = None
+    if trace_ms > 0 and trace_launches:
+        per_launch_bytes = alg_bytes / (trace_launches / args.steps)
+        avg_ms = trace_ms / trace_launches
+        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "k_trace_closest", "avg_launch_ms": round(avg_ms, 4),
+                    "bytes_per_launch": int(per_launch_bytes)}
+    line = {
+        "metric": "Mrays/s at 1080p 8-bounce (wavefront GI, one MI355X)",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm terrain)",
+        "config": {"workload": "terrain 1M triangles, 1920x1080, 8 spp, 8 bounces, 64 emissive tris + env 0.5",
+                   "width": W, "height": H, "spp": spp, "max_bounces": B, "triangles": int(ds.info.triangle_count)},
+        "roofline": roofline,
+        "extra": {
+            "rays_per_frame": int(rays / args.steps),
+            "device_ms_per_frame": round(dev_ms / args.steps, 3),
+            "trace_ms_per_frame": round(trace_ms / args.steps, 3),
+            "pcie_inclusive_frame_ms": round(pcie_ms, 3),
+            "bvh_build_ms": round(ds.info.build_ms, 2),
+            "bvh_nodes": int(ds.info.node_count),
+            "nodes_per_closest_ray": round(work.closest_nodes / max(closest_rays, 1), 2),
+            "tris_per_closest_ray": round(work.closest_tris / max(closest_rays, 1), 2),
+            "nodes_per_shadow_ray": round(work.shadow_nodes / max(work.shadow_rays, 1), 2),
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(scene, camera, W, H, spp, B, args.cpu_seconds)
+    renderer.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

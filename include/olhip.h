@@ -1,0 +1,201 @@
+/*
+ * olhip -- C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * The reference exposes no native interface: its renderer boundary is the
+ * Python RendererProtocol (ordinarylight/renderers/base.py:41-71), driven by
+ * Renderer._render_now (ordinarylight/renderer/_core.py:757-766).  This ABI
+ * is what the drop-in renderers.hip implementation binds through ctypes;
+ * each entry point names the reference step it replaces.
+ *
+ * Conventions: every function returns OLHIP_OK (0) or a negative status
+ * whose class mirrors the reference's exceptions (ValueError, RuntimeError,
+ * MemoryError); olhip_last_error() describes the last failure of the calling
+ * thread.  Pointers are plain host pointers; nothing here takes ownership of
+ * caller memory.  One context owns one HIP stream on one device; calls on
+ * one context must not overlap.
+ */
+#ifndef OLHIP_H
+#define OLHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OLHIP_OK 0
+#define OLHIP_ERR_VALUE (-1)   /* invalid argument: ValueError/TypeError */
+#define OLHIP_ERR_RUNTIME (-2) /* unsupported or invalid state: RuntimeError */
+#define OLHIP_ERR_MEMORY (-3)  /* device allocation failed: MemoryError */
+#define OLHIP_ERR_DEVICE (-4)  /* HIP runtime error */
+
+typedef struct olhip_ctx olhip_ctx;
+typedef struct olhip_scene olhip_scene;
+
+/* Description of the last error on this thread ("" if none). */
+const char* olhip_last_error(void);
+/* ABI version (major * 100 + minor). */
+int olhip_version(void);
+/* Number of visible HIP devices. */
+int olhip_device_count(int* count);
+
+/* Renderer device state; replaces the Vulkan device/queue setup of
+ * _VulkanGlobalIlluminationEngine (targets/vulkan/api.py:844-917). */
+int olhip_create(int device, olhip_ctx** out);
+int olhip_destroy(olhip_ctx* ctx);
+
+/* Scene in world space, indexed triangles in the reference's packed
+ * triangle order (Scene.render_triangles, scene/_core.py:2755-2762).
+ * Materials use the 12 x vec4 record of Scene.triangle_material_data
+ * (scene/_core.py:2816-2924), analytic lights the 4 x vec4 record of
+ * analytic_light_data (:2578-2631), area lights the 5 x vec4 record of
+ * emissive_triangle_data (:2637-2690). */
+typedef struct olhip_scene_desc {
+  uint32_t vertex_count;
+  const float* positions;      /* [vertex_count][3] */
+  const float* normals;        /* [vertex_count][3] unit shading normals */
+  uint32_t triangle_count;
+  const uint32_t* indices;     /* [triangle_count][3] */
+  const uint32_t* material_ids;/* [triangle_count] index into materials */
+  uint32_t material_count;
+  const float* materials;      /* [material_count][12][4] */
+  uint32_t analytic_light_count;
+  const float* analytic_lights;/* [analytic_light_count][4][4] */
+  uint32_t area_light_count;
+  const float* area_lights;    /* [area_light_count][5][4] */
+  float area_light_weight;     /* Scene.emissive_light_weight */
+  uint32_t leaf_size;          /* BVH leaf size 1..16, 0 = default (4) */
+} olhip_scene_desc;
+
+typedef struct olhip_scene_info {
+  uint32_t triangle_count;
+  uint32_t node_count;
+  uint32_t bvh_depth;
+  uint32_t stack_depth;
+  double build_ms;
+  uint64_t device_bytes;
+} olhip_scene_info;
+
+/* Upload + GPU LBVH build; replaces VulkanSceneResources._build_scene
+ * (targets/vulkan/scene.py:1322-1561). */
+int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* desc, olhip_scene** out);
+int olhip_scene_destroy(olhip_scene* scene);
+int olhip_scene_info_get(const olhip_scene* scene, olhip_scene_info* out);
+
+/* Debug: copy the built BVH.  nodes: node_count * 64 bytes; tri:
+ * triangle_count * 48 bytes (leaf order).  Either pointer may be NULL. */
+int olhip_scene_bvh(const olhip_scene* scene, void* nodes, void* tri);
+
+/* Camera basis as _camera_vectors (targets/vulkan/core.py:103-121):
+ * unit forward, right/up pre-scaled by the projection half extent. */
+typedef struct olhip_camera {
+  float position[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  uint32_t projection;  /* 0 perspective, 1 orthographic, 2 panoramic */
+} olhip_camera;
+
+/* GI estimator settings: the WavefrontShadeSettings push constants
+ * (wavefront/shading.py:66-114) of the split path. */
+typedef struct olhip_gi_settings {
+  uint32_t max_bounces;               /* 1..64 */
+  uint32_t secondary_area_samples;    /* >= 1 */
+  uint32_t environment_samples;       /* 0..4 */
+  uint32_t russian_roulette_start;    /* 0 = off */
+  float russian_roulette_min_survival;
+  float secondary_nee_probability;    /* (0, 1] */
+  uint32_t unified_secondary_nee;     /* 0 or 1 */
+  uint32_t path_capacity;             /* resident paths per chunk, 0 = default */
+} olhip_gi_settings;
+
+typedef struct olhip_stats {
+  uint64_t primary_rays;
+  uint64_t extension_rays;
+  uint64_t shadow_rays;
+  uint64_t queue_overflow;
+  double total_ms;         /* device time of the render (events) */
+  double trace_ms;         /* closest-hit kernels (when profiling) */
+  double shade_ms;
+  double shadow_ms;
+  uint32_t trace_launches;
+  uint32_t shade_launches;
+  uint32_t shadow_launches;
+  uint32_t chunks;
+  /* Work counters (only with olhip_set_work_counters(ctx, 1)). */
+  uint64_t closest_nodes;  /* 64-byte nodes visited by closest-hit rays */
+  uint64_t closest_tris;   /* 48-byte triangles tested by closest-hit rays */
+  uint64_t shadow_nodes;
+  uint64_t shadow_tris;
+} olhip_stats;
+
+/* One GI frame; replaces _VulkanGlobalIlluminationEngine.render_frame /
+ * render_wavefront_outputs (targets/vulkan/api.py:1233-1451).  out_rgba is
+ * caller-owned float32 [height][width][4] (linear HDR, alpha 1), or NULL to
+ * keep the image on the device (olhip_frame_download).  frame_index is the
+ * renderer's sequence index (stored as float32 like the reference camera
+ * block).  stats may be NULL. */
+int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* scene, const olhip_camera* camera, uint32_t width,
+                    uint32_t height, uint32_t samples, uint32_t frame_index, const olhip_gi_settings* settings,
+                    float* out_rgba, olhip_stats* stats);
+/* Copy the last frame of this context to host memory. */
+int olhip_frame_download(olhip_ctx* ctx, float* out_rgba);
+/* Enable per-kernel event timing inside olhip_render_gi (0 or 1). */
+int olhip_set_profiling(olhip_ctx* ctx, int enabled);
+/* Count traversal work (nodes visited, triangles tested) in olhip_stats. */
+int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
+/* Override the traversal stack depth (0 = from the BVH depth). */
+int olhip_set_stack_depth(olhip_ctx* ctx, int depth);
+
+/* Closest hit of caller rays: origins/directions [n][4] (xyz, tmin / tmax).
+ * hits [n][4]: t, u, v, original triangle index as uint32 bits (0xFFFFFFFF
+ * on a miss).  Primary-hit products use this (core.py:4500-4536). */
+int olhip_trace(olhip_ctx* ctx, const olhip_scene* scene, const float* origins, const float* directions, uint32_t n,
+                float* hits);
+
+/* ---- reference-estimator mode (renderers/reference/path_tracer.py) ----
+ * The host owns the PCG64 stream; the device runs the per-bounce work.
+ * Sequence per frame: e0_begin; per sample { e0_sample(jx, jy);
+ * while (e0_trace(&n) and n > 0) e0_shade(deviates[n][3]); e0_sample_end };
+ * e0_end(out). */
+typedef struct olhip_e0_camera {
+  float position[3];
+  float forward[3];   /* unit, float32 as the reference computes it */
+  float right[3];
+  float up[3];
+  double k_x, k_y;    /* see olhip_e0_begin */
+  uint32_t projection;
+} olhip_e0_camera;
+
+typedef struct olhip_e0_env {
+  uint32_t mode;      /* 0 gradient sky, 1 constant, 2 image */
+  float color[3];
+  float intensity;
+  double rotation;
+  const float* image; /* [h][w][3] when mode == 2 */
+  uint32_t width, height;
+} olhip_e0_env;
+
+/* k_x, k_y: perspective aspect*tan(fov/2), tan(fov/2); orthographic
+ * aspect*size/2, size/2; panoramic radians(hfov), radians(vfov).
+ * brute_force = 1 scans triangles in index order (exact tie semantics). */
+int olhip_e0_begin(olhip_ctx* ctx, const olhip_scene* scene, const olhip_e0_camera* camera, uint32_t width,
+                   uint32_t height, const olhip_e0_env* env, uint32_t brute_force);
+int olhip_e0_sample(olhip_ctx* ctx, const float* jitter_x, const float* jitter_y);
+/* Traces the active rays, adds miss radiance, compacts the hits in ray
+ * order; *n_hit receives how many normal triples the next shade needs. */
+int olhip_e0_trace(olhip_ctx* ctx, uint32_t* n_hit, uint32_t* n_active);
+/* deviates: float32 [n_hit][3] (rng.normal(...).astype(float32)). */
+int olhip_e0_shade(olhip_ctx* ctx, const float* deviates, uint32_t n_hit);
+int olhip_e0_sample_end(olhip_ctx* ctx);
+/* out: float32 [height][width][4], alpha 0. */
+int olhip_e0_end(olhip_ctx* ctx, uint32_t samples, float* out_rgba);
+/* Primary-hit triangle of the last e0_trace call for ray i of the active
+ * list (debug): writes n_active ints (-1 on a miss). */
+int olhip_e0_last_hits(olhip_ctx* ctx, int32_t* tri, uint32_t n_active);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OLHIP_H */

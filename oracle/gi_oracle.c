@@ -1,0 +1,958 @@
+/*
+ * GI estimator oracle (O-E1) -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Sequential C restatement of the reference's split-path GI shade stage,
+ * ordinarylight/shaders/wavefront_shade_candidate.glsl main (:3106-3493),
+ * with its ray generation (wavefront_generate.comp:86-167) and host
+ * accumulation (targets/vulkan/api.py:1322-1422).  One path at a time, the
+ * next-event shadow rays traced inline exactly where the GLSL traces them
+ * (nativeOccluded, :3026), over this file's own BVH (binary, median split,
+ * double-precision boxes) -- nothing is shared with the HIP sources.
+ *
+ * Compiled twice (oracle/Makefile): OL_REAL=float is the parity oracle of
+ * the GPU (same float operation order, -ffp-contract=off); OL_REAL=double
+ * is the FP64 evaluation of the same estimator for convergence checks.
+ *
+ * Built-in untextured PBR materials only.  Deliberate deviation shared with
+ * the product: the occlusion factor mix(texture_parameters.w, 1, metallic)
+ * (:1542, :1957) reads the value the packer resolves (1.0 for untextured
+ * materials) instead of the transmission-texture binding slot (-1).
+ *
+ * Functions follow (file:line in wavefront_shade_candidate.glsl):
+ *   waveCosineHemisphere :630, rayQueryRandomState/Value :661-671,
+ *   shadeResolveSurface :957, shadeResolveEnvironmentMiss :1231,
+ *   shadeTransmitPath :1264, shadeApplyRussianRoulette :1335,
+ *   shadeBuildContinuation :1356, PBR lobes/pdf/sampling :1406-1552,
+ *   point lights :1554-1619, area lights :1621-1733, unified domain
+ *   :1735-1754/:1903-1928, secondary NEE selection :1756-1777, emission
+ *   :1779-1809, environment :1811-1894, environment NEE :1930-1959.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef OL_REAL
+#define OL_REAL float
+#endif
+#ifndef OL_SUFFIX
+#define OL_SUFFIX _f32
+#endif
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define API(name) CAT(name, OL_SUFFIX)
+
+typedef OL_REAL R;
+typedef struct {
+  R x, y, z;
+} V;
+
+static inline V v3(R x, R y, R z) {
+  V r = {x, y, z};
+  return r;
+}
+static inline V vadd(V a, V b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V vsub(V a, V b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V vneg(V a) { return v3(-a.x, -a.y, -a.z); }
+static inline V vmul(V a, V b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline V vs(V a, R s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V sv(R s, V a) { return v3(s * a.x, s * a.y, s * a.z); }
+static inline V vdiv(V a, R s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline R dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V cross(V a, V b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline R rsqrt_(R x) { return sizeof(R) == 4 ? (R)sqrtf((float)x) : (R)sqrt((double)x); }
+static inline R len(V a) { return rsqrt_(dot(a, a)); }
+static inline V norm(V a) { return vs(a, (R)1 / rsqrt_(dot(a, a))); }
+static inline R rmax(R a, R b) { return a > b ? a : (b != b ? a : b); }
+static inline R rmin(R a, R b) { return a < b ? a : (b != b ? a : b); }
+static inline R clampr(R v, R lo, R hi) { return rmin(rmax(v, lo), hi); }
+static inline R mixr(R a, R b, R t) { return a * ((R)1 - t) + b * t; }
+static inline V mixv(V a, V b, R t) {
+  return v3(a.x * ((R)1 - t) + b.x * t, a.y * ((R)1 - t) + b.y * t, a.z * ((R)1 - t) + b.z * t);
+}
+static inline V reflect_(V d, V n) { return vsub(d, sv((R)2 * dot(n, d), n)); }
+static inline R pow5(R x) { return ((x * x) * (x * x)) * x; }
+static inline R absr(R x) { return x < 0 ? -x : x; }
+
+#define PI_R ((R)3.14159265359)
+#define TWO_PI_R ((R)6.28318530718)
+
+/* Cephes sinf/cosf (S. Moshier): octant reduction with a three-part pi/4,
+ * minimax polynomials on [-pi/4, pi/4].  The double build uses libm. */
+static void sincos_r(R xr, R* so, R* co) {
+  if (sizeof(R) == 8) {
+    *so = (R)sin((double)xr);
+    *co = (R)cos((double)xr);
+    return;
+  }
+  float x = (float)xr, ss = 1.0f, cs = 1.0f;
+  if (x < 0.0f) {
+    x = -x;
+    ss = -1.0f;
+  }
+  int j = (int)(x * 1.27323954473516f);
+  float y = (float)j;
+  if (j & 1) {
+    j += 1;
+    y += 1.0f;
+  }
+  j &= 7;
+  if (j > 3) {
+    j -= 4;
+    ss = -ss;
+    cs = -cs;
+  }
+  if (j > 1) cs = -cs;
+  float z = ((x - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
+  float zz = z * z;
+  float ps = ((-1.9515295891e-4f * zz + 8.3321608736e-3f) * zz - 1.6666654611e-1f) * zz * z + z;
+  float pc = ((2.443315711809948e-5f * zz - 1.388731625493765e-3f) * zz + 4.166664568298827e-2f) * zz * zz - 0.5f * zz + 1.0f;
+  float s = (j == 1 || j == 2) ? pc : ps;
+  float c = (j == 1 || j == 2) ? ps : pc;
+  *so = (R)(ss * s);
+  *co = (R)(cs * c);
+}
+
+static uint32_t hash32(uint32_t v) {
+  v ^= v >> 16;
+  v *= 2146121005u;
+  v ^= v >> 15;
+  v *= 2221713035u;
+  v ^= v >> 16;
+  return v;
+}
+static R unit(uint32_t w) { return (R)w * (R)(1.0 / 4294967296.0); }
+static uint32_t pcg(uint32_t s) { return s * 747796405u + 2891336453u; }
+static R pcg_unit(uint32_t s) {
+  uint32_t w = ((s >> ((s >> 28) + 4u)) ^ s) * 277803737u;
+  w = (w >> 22) ^ w;
+  return unit(w);
+}
+static uint32_t brev(uint32_t v) {
+  uint32_t r = 0;
+  for (int i = 0; i < 32; ++i) r |= ((v >> i) & 1u) << (31 - i);
+  return r;
+}
+
+/* ---- scene + BVH2 ------------------------------------------------------ */
+typedef struct {
+  double mn[3], mx[3];
+  uint32_t left, right; /* leaf: right = count | 0x80000000, left = first */
+} BNode;
+
+typedef struct {
+  uint32_t V, T, M, P, L;
+  float* pos;      /* V*3 */
+  float* nrm;      /* V*3 */
+  uint32_t* idx;   /* T*3 */
+  uint32_t* mid;   /* T */
+  float* mat;      /* M*48 */
+  float* ana;      /* P*16 */
+  float* area;     /* L*20 */
+  float weight;
+  int env_valid;
+  float env[3];
+  BNode* nodes;
+  uint32_t node_count;
+  uint32_t* order; /* leaf order -> triangle */
+} Scene;
+
+static void tri_box(const Scene* s, uint32_t t, double mn[3], double mx[3]) {
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = INFINITY;
+    mx[k] = -INFINITY;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const float* p = s->pos + 3ull * s->idx[3ull * t + c];
+    for (int k = 0; k < 3; ++k) {
+      if (p[k] < mn[k]) mn[k] = p[k];
+      if (p[k] > mx[k]) mx[k] = p[k];
+    }
+  }
+}
+
+static const Scene* g_sort_scene;
+static int g_sort_axis;
+static double* g_cent;
+static int cmp_cent(const void* a, const void* b) {
+  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  double cx = g_cent[3ull * x + g_sort_axis], cy = g_cent[3ull * y + g_sort_axis];
+  if (cx < cy) return -1;
+  if (cx > cy) return 1;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static uint32_t build_rec(Scene* s, uint32_t first, uint32_t count) {
+  uint32_t id = s->node_count++;
+  BNode* n = &s->nodes[id];
+  for (int k = 0; k < 3; ++k) {
+    n->mn[k] = INFINITY;
+    n->mx[k] = -INFINITY;
+  }
+  double cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = first; i < first + count; ++i) {
+    double a[3], b[3];
+    tri_box(s, s->order[i], a, b);
+    for (int k = 0; k < 3; ++k) {
+      if (a[k] < n->mn[k]) n->mn[k] = a[k];
+      if (b[k] > n->mx[k]) n->mx[k] = b[k];
+      double c = g_cent[3ull * s->order[i] + k];
+      if (c < cmn[k]) cmn[k] = c;
+      if (c > cmx[k]) cmx[k] = c;
+    }
+  }
+  if (count <= 4) {
+    n->left = first;
+    n->right = count | 0x80000000u;
+    return id;
+  }
+  int axis = 0;
+  double ext = cmx[0] - cmn[0];
+  for (int k = 1; k < 3; ++k)
+    if (cmx[k] - cmn[k] > ext) {
+      ext = cmx[k] - cmn[k];
+      axis = k;
+    }
+  g_sort_axis = axis;
+  qsort(s->order + first, count, sizeof(uint32_t), cmp_cent);
+  uint32_t half = count / 2;
+  uint32_t l = build_rec(s, first, half);
+  uint32_t r = build_rec(s, first + half, count - half);
+  s->nodes[id].left = l;
+  s->nodes[id].right = r;
+  return id;
+}
+
+void* API(olgi_scene_new)(const float* pos, const float* nrm, uint32_t V, const uint32_t* idx, const uint32_t* mid,
+                          uint32_t T, const float* mat, uint32_t M, const float* ana, uint32_t P, const float* area,
+                          uint32_t L, float weight) {
+  Scene* s = (Scene*)calloc(1, sizeof(Scene));
+  s->V = V;
+  s->T = T;
+  s->M = M;
+  s->P = P;
+  s->L = L;
+  s->pos = (float*)malloc(12ull * V + 4);
+  s->nrm = (float*)malloc(12ull * V + 4);
+  s->idx = (uint32_t*)malloc(12ull * T + 4);
+  s->mid = (uint32_t*)malloc(4ull * T + 4);
+  s->mat = (float*)malloc(192ull * M + 4);
+  s->ana = (float*)malloc(64ull * P + 4);
+  s->area = (float*)malloc(80ull * L + 4);
+  memcpy(s->pos, pos, 12ull * V);
+  memcpy(s->nrm, nrm, 12ull * V);
+  memcpy(s->idx, idx, 12ull * T);
+  memcpy(s->mid, mid, 4ull * T);
+  memcpy(s->mat, mat, 192ull * M);
+  if (P) memcpy(s->ana, ana, 64ull * P);
+  if (L) memcpy(s->area, area, 80ull * L);
+  s->weight = weight;
+  for (uint32_t l = 0; l < P; ++l) {
+    const float* a = s->ana + 16ull * l;
+    if ((int)(a[3] + 0.5f) == 3) {
+      s->env_valid = 1;
+      s->env[0] = (1.0f * a[8]) * a[11];
+      s->env[1] = (1.0f * a[9]) * a[11];
+      s->env[2] = (1.0f * a[10]) * a[11];
+      break;
+    }
+  }
+  s->nodes = (BNode*)malloc(sizeof(BNode) * (2ull * T + 2));
+  s->order = (uint32_t*)malloc(4ull * T + 4);
+  g_cent = (double*)malloc(24ull * T + 8);
+  for (uint32_t t = 0; t < T; ++t) {
+    double a[3], b[3];
+    tri_box(s, t, a, b);
+    for (int k = 0; k < 3; ++k) g_cent[3ull * t + k] = 0.5 * (a[k] + b[k]);
+    s->order[t] = t;
+  }
+  g_sort_scene = s;
+  if (T) build_rec(s, 0, T);
+  free(g_cent);
+  g_cent = NULL;
+  return s;
+}
+
+void API(olgi_scene_free)(void* h) {
+  Scene* s = (Scene*)h;
+  if (!s) return;
+  free(s->pos);
+  free(s->nrm);
+  free(s->idx);
+  free(s->mid);
+  free(s->mat);
+  free(s->ana);
+  free(s->area);
+  free(s->nodes);
+  free(s->order);
+  free(s);
+}
+
+/* Moller-Trumbore in R, expressions as written. */
+static int tri_hit(const Scene* s, uint32_t t, V o, V d, R* tt, R* uu, R* vv) {
+  const float* a = s->pos + 3ull * s->idx[3ull * t];
+  const float* b = s->pos + 3ull * s->idx[3ull * t + 1];
+  const float* c = s->pos + 3ull * s->idx[3ull * t + 2];
+  V A = v3(a[0], a[1], a[2]);
+  V E1 = v3((R)(b[0] - a[0]), (R)(b[1] - a[1]), (R)(b[2] - a[2]));
+  V E2 = v3((R)(c[0] - a[0]), (R)(c[1] - a[1]), (R)(c[2] - a[2]));
+  V p = cross(d, E2);
+  R det = dot(E1, p);
+  if (det == (R)0) return 0;
+  R inv = (R)1 / det;
+  V sv_ = vsub(o, A);
+  R u = dot(sv_, p) * inv;
+  if (!(u >= (R)0 && u <= (R)1)) return 0;
+  V q = cross(sv_, E1);
+  R v = dot(d, q) * inv;
+  if (!(v >= (R)0 && u + v <= (R)1)) return 0;
+  *tt = dot(E2, q) * inv;
+  *uu = u;
+  *vv = v;
+  return 1;
+}
+
+static int box_hit(const BNode* n, const double o[3], const double inv[3], double tmin, double tmax) {
+  double t0 = tmin, t1 = tmax;
+  for (int k = 0; k < 3; ++k) {
+    double a = (n->mn[k] - o[k]) * inv[k], b = (n->mx[k] - o[k]) * inv[k];
+    if (a > b) {
+      double x = a;
+      a = b;
+      b = x;
+    }
+    if (a > t0) t0 = a;
+    if (b < t1) t1 = b;
+  }
+  return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
+}
+
+/* Closest hit in [tmin, tmax]: lowest t, ties to the lowest triangle index. */
+static int closest(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv, uint32_t* bi) {
+  *bt = tmax;
+  *bi = 0xFFFFFFFFu;
+  if (!s->T) return 0;
+  double od[3] = {o.x, o.y, o.z}, inv[3];
+  double dd[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (fabs(dd[k]) > 1e-300 ? dd[k] : copysign(1e-300, dd[k]));
+  uint32_t stack[256];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode* n = &s->nodes[stack[--sp]];
+    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)*bt * (1.0 + 1e-7) + 1e-9)) continue;
+    if (n->right & 0x80000000u) {
+      uint32_t cnt = n->right & 0x7FFFFFFFu;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        uint32_t t = s->order[n->left + k];
+        R tt, uu, vv;
+        if (tri_hit(s, t, o, d, &tt, &uu, &vv) && tt >= tmin &&
+            (tt < *bt || (tt == *bt && t < *bi))) {
+          *bt = tt;
+          *bu = uu;
+          *bv = vv;
+          *bi = t;
+        }
+      }
+    } else {
+      stack[sp++] = n->left;
+      stack[sp++] = n->right;
+    }
+  }
+  return *bi != 0xFFFFFFFFu;
+}
+
+static _Thread_local uint64_t g_shadow_rays;
+
+static int occluded(const Scene* s, V o, V d, R tmin, R tmax) {
+  ++g_shadow_rays;
+  if (!s->T) return 0;
+  double od[3] = {o.x, o.y, o.z}, inv[3], dd[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (fabs(dd[k]) > 1e-300 ? dd[k] : copysign(1e-300, dd[k]));
+  uint32_t stack[256];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode* n = &s->nodes[stack[--sp]];
+    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)tmax * (1.0 + 1e-7) + 1e-9)) continue;
+    if (n->right & 0x80000000u) {
+      uint32_t cnt = n->right & 0x7FFFFFFFu;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        R tt, uu, vv;
+        if (tri_hit(s, s->order[n->left + k], o, d, &tt, &uu, &vv) && tt >= tmin && tt <= tmax) return 1;
+      }
+    } else {
+      stack[sp++] = n->left;
+      stack[sp++] = n->right;
+    }
+  }
+  return 0;
+}
+
+/* ---- materials and BSDF ----------------------------------------------------- */
+typedef struct {
+  V base;
+  R rough, metal, trans, ior, adist, two_sided, occl;
+  V atten;
+  R cc, ccr, sheen_r, aniso, sub, thin, thick;
+  V sheen, subc;
+} Mat;
+
+static Mat load_mat(const Scene* s, uint32_t m) {
+  const float* r = s->mat + 48ull * m;
+  Mat x;
+  x.base = v3(r[0], r[1], r[2]);
+  x.rough = r[3];
+  x.metal = r[7];
+  x.atten = v3(r[8], r[9], r[10]);
+  x.trans = r[11];
+  x.ior = r[12];
+  x.adist = r[13];
+  x.two_sided = r[15];
+  x.occl = r[23];
+  x.cc = r[24];
+  x.ccr = r[25];
+  x.sheen_r = r[26];
+  x.aniso = r[27];
+  x.sub = r[28];
+  x.thin = r[30];
+  x.thick = r[31];
+  x.sheen = v3(r[32], r[33], r[34]);
+  x.subc = v3(r[36], r[37], r[38]);
+  return x;
+}
+static V emission_of(const Scene* s, uint32_t m) {
+  const float* r = s->mat + 48ull * m;
+  return v3(r[4], r[5], r[6]);
+}
+
+static R ggx_d(R nh, R rough) {
+  R alpha = rmax(rough * rough, (R)0.0009);
+  R a2 = alpha * alpha;
+  R den = ((nh * nh) * (a2 - (R)1)) + (R)1;
+  return a2 / rmax((PI_R * den) * den, (R)1e-6);
+}
+static R ggx_g(R nd, R rough) {
+  R alpha = rmax(rough * rough, (R)0.0009);
+  R a2 = alpha * alpha;
+  return ((R)2 * nd) / rmax(nd + rsqrt_(a2 + (((R)1 - a2) * nd) * nd), (R)1e-6);
+}
+static V f0_of(const Mat* m) { return mixv(v3((R)0.04, (R)0.04, (R)0.04), m->base, m->metal); }
+static R spec_prob(const Mat* m) {
+  V f0 = f0_of(m);
+  return clampr(rmax(f0.x, rmax(f0.y, f0.z)) + m->cc * (R)0.15, (R)0.1, (R)0.95);
+}
+
+static V eval_pbr(const Mat* m, V n, V view, V out) {
+  R nv = rmax(dot(n, view), (R)0), nl = rmax(dot(n, out), (R)0);
+  if (nv <= (R)0 || nl <= (R)0) return v3(0, 0, 0);
+  V h = norm(vadd(view, out));
+  R nh = rmax(dot(n, h), (R)0), vh = rmax(dot(view, h), (R)0);
+  R k = pow5((R)1 - clampr(vh, (R)0, (R)1));
+  V f0 = f0_of(m);
+  V fr = vadd(f0, vs(vsub(v3(1, 1, 1), f0), k));
+  R er = m->rough * ((R)1 - (R)0.25 * absr(clampr(m->aniso, (R)-1, (R)1)));
+  R D = ggx_d(nh, er);
+  R G = ggx_g(nv, er) * ggx_g(nl, er);
+  R den = rmax(((R)4 * nv) * nl, (R)1e-6);
+  V spec = vdiv(vs(vs(fr, D), G), den);
+  V diff = vdiv(vmul(vs(vsub(v3(1, 1, 1), fr), (R)1 - m->metal), m->base), PI_R);
+  diff = mixv(diff, vmul(diff, m->subc), clampr(m->sub, (R)0, (R)1));
+  R sv5 = pow5((R)1 - vh);
+  V sheen = vs(m->sheen, ((R)1 - clampr(m->sheen_r, (R)0, (R)1)) * sv5);
+  R cc = clampr(m->cc, (R)0, (R)1), cr = clampr(m->ccr, (R)0.02, (R)1);
+  R cD = ggx_d(nh, cr), cG = ggx_g(nv, cr) * ggx_g(nl, cr);
+  R cF = (R)0.04 + (R)0.96 * sv5;
+  R coat = (((cc * cD) * cG) * cF) / den;
+  R layer = (R)1 - cc * cF;
+  V dl = vs(diff, layer);
+  V sl = vadd(vs(vadd(spec, sheen), layer), v3(coat, coat, coat));
+  return vadd(dl, sl);
+}
+
+static R pdf_pbr(const Mat* m, V n, V view, V out) {
+  R nl = rmax(dot(n, out), (R)0);
+  if (nl <= (R)0) return (R)0;
+  V h = norm(vadd(view, out));
+  R nh = rmax(dot(n, h), (R)0), vh = rmax(dot(view, h), (R)1e-6);
+  R sp = (ggx_d(nh, m->rough) * nh) / ((R)4 * vh);
+  R dp = nl / PI_R;
+  return mixr(dp, sp, spec_prob(m));
+}
+
+static V tangent_of(V n) {
+  V t = cross(n, v3(0, 1, 0));
+  if (absr(n.z) < (R)0.999) t = cross(n, v3(0, 0, 1));
+  return norm(t);
+}
+static V cosine_dir(V n, R u, R v) {
+  R radius = rsqrt_(u), phi = TWO_PI_R * v, sp, cp;
+  V t = tangent_of(n), b = cross(n, t);
+  sincos_r(phi, &sp, &cp);
+  return norm(vadd(vadd(vs(t, radius * cp), vs(b, radius * sp)), vs(n, rsqrt_(rmax((R)0, (R)1 - u)))));
+}
+static V ggx_half(V n, R rough, R u, R v) {
+  R alpha = rmax(rough * rough, (R)0.0009), a2 = alpha * alpha, phi = TWO_PI_R * v;
+  R c = rsqrt_(((R)1 - u) / rmax((R)1 + (a2 - (R)1) * u, (R)1e-6));
+  R s = rsqrt_(rmax((R)0, (R)1 - c * c));
+  V t = tangent_of(n), b = cross(n, t);
+  R sp, cp;
+  sincos_r(phi, &sp, &cp);
+  return norm(vadd(vadd(vs(vs(t, s), cp), vs(vs(b, s), sp)), vs(n, c)));
+}
+static R power_h(R a, R b) {
+  R a2 = a * a, b2 = b * b;
+  return a2 / rmax(a2 + b2, (R)1e-6);
+}
+static V env_rad(const Scene* s, V d) {
+  if (!s->env_valid) return mixv(v3((R)0.018, (R)0.022, (R)0.032), v3((R)0.32, (R)0.46, (R)0.72), rmax(d.y, (R)0));
+  return v3(s->env[0], s->env[1], s->env[2]);
+}
+
+typedef struct {
+  uint32_t max_bounces, sec_area, env_samples, rr_start;
+  float rr_min, nee_prob;
+  uint32_t unified;
+} Settings;
+
+static R unified_prob(const Scene* s, const Settings* st) {
+  if (s->L == 0) return (R)0;
+  if (st->env_samples == 0) return (R)1;
+  R w = rsqrt_(rmax((R)s->weight, (R)1e-6));
+  return w / (w + (R)1);
+}
+
+/* Shadow-tested NEE sample contribution (0 when invalid or occluded). */
+static int area_sample(const Scene* s, uint32_t* rs, V hit, V n, const Mat* m, V inc, uint32_t si, uint32_t sc,
+                       R tp, V* out) {
+  if (s->L == 0) return 0;
+  *rs = pcg(*rs);
+  R sel = ((R)si + pcg_unit(*rs)) / (R)sc;
+  uint32_t lo = 0, hi = s->L - 1;
+  for (int k = 0; k < 32 && lo < hi; ++k) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (sel <= (R)s->area[20ull * mid + 16]) hi = mid; else lo = mid + 1;
+  }
+  const float* A = s->area + 20ull * lo;
+  V a = v3(A[0], A[1], A[2]), b = v3(A[4], A[5], A[6]), c = v3(A[8], A[9], A[10]);
+  *rs = pcg(*rs);
+  R ru = rsqrt_(pcg_unit(*rs));
+  *rs = pcg(*rs);
+  R vv = pcg_unit(*rs);
+  V lp = vadd(vadd(sv((R)1 - ru, a), sv(ru * ((R)1 - vv), b)), sv(ru * vv, c));
+  V off = vsub(lp, hit);
+  R d2 = dot(off, off), dl = rsqrt_(d2);
+  V dir = vdiv(off, rmax(dl, (R)1e-6));
+  R scos = rmax(dot(n, dir), (R)0);
+  V ln = norm(cross(vsub(b, a), vsub(c, a)));
+  R raw = dot(ln, vneg(dir));
+  R lc = rmax(raw, (R)0);
+  if (A[18] > 0.5f) lc = absr(raw);
+  if (scos <= (R)0 || lc <= (R)1e-6) return 0;
+  R lpdf = ((R)A[17] * d2) / rmax(lc * (R)A[15], (R)1e-6);
+  R eff = lpdf * rmax(tp, (R)1e-6);
+  if (occluded(s, vadd(hit, vs(n, (R)0.002)), dir, (R)0.001, rmax(dl - (R)0.004, (R)0.001))) return 0;
+  V view = vneg(inc);
+  R bp = pdf_pbr(m, n, view, dir);
+  R mis = power_h(eff * (R)sc, bp);
+  *out = vdiv(vs(vs(vs(vmul(eval_pbr(m, n, view, dir), v3(A[12], A[13], A[14])), scos), mis), (R)1),
+              rmax(eff, (R)1e-6));
+  return 1;
+}
+
+static int env_sample(const Scene* s, uint32_t* rs, V hit, V n, const Mat* m, V inc, uint32_t sc, R tp, V* out) {
+  *rs = pcg(*rs);
+  R u = pcg_unit(*rs);
+  *rs = pcg(*rs);
+  R v = pcg_unit(*rs);
+  V dir = cosine_dir(n, u, v);
+  R c = rmax(dot(n, dir), (R)0);
+  R pdf = c / PI_R, eff = pdf * rmax(tp, (R)1e-6);
+  if (!(c > (R)0)) return 0;
+  if (occluded(s, vadd(hit, vs(n, (R)0.002)), dir, (R)0.001, (R)1e30)) return 0;
+  R mis = power_h(eff * (R)sc, c / PI_R);
+  R mw = mixr(m->occl, (R)1, m->metal);
+  *out = vs(vdiv(vs(vs(vs(vmul(eval_pbr(m, n, vneg(inc), dir), env_rad(s, dir)), c), mis), (R)1), rmax(eff, (R)1e-6)),
+            mw);
+  return 1;
+}
+
+static int point_sample(const Scene* s, uint32_t li, V hit, V n, const Mat* m, V inc, V* out) {
+  const float* L = s->ana + 16ull * li;
+  int type = (int)(L[3] + 0.5f);
+  if (type == 3) return 0;
+  V dir;
+  R d2 = 1, dist = 10000, att = 1;
+  if (type == 1) {
+    dir = vneg(norm(v3(L[4], L[5], L[6])));
+  } else {
+    V off = vsub(v3(L[0], L[1], L[2]), hit);
+    d2 = rmax(dot(off, off), (R)1e-6);
+    dist = rsqrt_(d2);
+    dir = vdiv(off, dist);
+    if (L[7] > 0.0f && dist > (R)L[7]) return 0;
+    att = (R)1 / d2;
+    if (type == 2) {
+      R cone = dot(norm(v3(L[4], L[5], L[6])), vneg(dir));
+      R amount = clampr((cone - (R)L[13]) / rmax((R)L[12] - (R)L[13], (R)1e-6), (R)0, (R)1);
+      R spot = (amount * amount) * ((R)3 - (R)2 * amount);
+      if (spot <= (R)0) return 0;
+      att = att * spot;
+    }
+  }
+  R c = rmax(dot(n, dir), (R)0);
+  if (c <= (R)0) return 0;
+  R sd = rmax(dist - (R)0.004, (R)0.001);
+  if (type == 1) sd = 10000;
+  if (occluded(s, vadd(hit, vs(n, (R)0.002)), dir, (R)0.001, sd)) return 0;
+  V incident = vs(vs(v3(L[8], L[9], L[10]), (R)L[11]), att);
+  *out = vs(vmul(vmul(eval_pbr(m, n, vneg(inc), dir), incident), v3(1, 1, 1)), c);
+  return 1;
+}
+
+static R medium_get(const R st[16], uint32_t i) { return st[i]; }
+
+/* One path; returns radiance (rgb) as floats of R.  counts[3] += rays. */
+static void trace_path(const Scene* s, const Settings* st, uint32_t pixel, uint32_t frame, uint32_t sample, V org,
+                       V dir, R out[3], uint64_t counts[3]) {
+  V T = v3(1, 1, 1), Lr = v3(0, 0, 0);
+  R prev_pdf = 0;
+  uint32_t flags = 257u, bounce = 0;
+  uint32_t rng;
+  R stack[16];
+  stack[0] = 1;
+  {
+    uint32_t r = hash32((pixel ^ hash32(frame)) ^ hash32(sample + 1u));
+    r = hash32(r);
+    r = hash32(r);
+    rng = r;
+  }
+  V o = org, d = dir;
+  R tmin = (R)0.001, tmax = (R)1e30;
+  const uint32_t frame_sample = (frame << 8) | (sample & 255u);
+  counts[0]++;
+  for (uint32_t iter = 0; iter < st->max_bounces; ++iter) {
+    R t = 0, bu = 0, bv = 0;
+    uint32_t prim;
+    int hit = closest(s, o, d, tmin, tmax, &t, &bu, &bv, &prim);
+    V inc = d;
+    if (rmax(T.x, rmax(T.y, T.z)) < (R)0.0001) break;
+    if (!hit) {
+      V env = env_rad(s, inc);
+      R mis = 1;
+      if ((flags & 2u) && st->env_samples > 0) {
+        R lp = prev_pdf * (R)st->env_samples;
+        if (flags & 4u) lp = prev_pdf * ((R)1 - unified_prob(s, st));
+        mis = power_h(prev_pdf, lp);
+      }
+      Lr = vadd(Lr, vs(vmul(T, env), mis));
+      break;
+    }
+    V hp = vadd(o, sv(t, d));
+    const uint32_t* tv = s->idx + 3ull * prim;
+    const float *pa = s->pos + 3ull * tv[0], *pb = s->pos + 3ull * tv[1], *pc = s->pos + 3ull * tv[2];
+    V e1 = v3((R)(pb[0] - pa[0]), (R)(pb[1] - pa[1]), (R)(pb[2] - pa[2]));
+    V e2 = v3((R)(pc[0] - pa[0]), (R)(pc[1] - pa[1]), (R)(pc[2] - pa[2]));
+    V gn = norm(cross(e1, e2));
+    R wx = ((R)1 - bu) - bv, wy = bu, wz = bv;
+    const float *n0 = s->nrm + 3ull * tv[0], *n1 = s->nrm + 3ull * tv[1], *n2 = s->nrm + 3ull * tv[2];
+    V sn = norm(vadd(vadd(vs(v3(n0[0], n0[1], n0[2]), wx), vs(v3(n1[0], n1[1], n1[2]), wy)),
+                     vs(v3(n2[0], n2[1], n2[2]), wz)));
+    if (dot(sn, gn) < (R)0) sn = vneg(sn);
+    int entering = dot(inc, gn) < (R)0;
+    V n = entering ? sn : vneg(sn);
+    Mat m = load_mat(s, s->mid[prim]);
+    V em = emission_of(s, s->mid[prim]);
+    uint32_t depth = (flags >> 8) > 1u ? (flags >> 8) : 1u;
+    uint32_t rs = rng;
+    if (entering || m.two_sided > (R)0.5) {
+      R emis = 1;
+      if ((flags & 2u) && dot(em, em) > (R)0) {
+        R area = (R)0.5 * len(cross(e1, e2));
+        R raw = dot(gn, vneg(inc));
+        R lc = rmax(raw, (R)0);
+        if (m.two_sided > (R)0.5) lc = absr(raw);
+        R lum = dot(em, v3((R)0.2126, (R)0.7152, (R)0.0722));
+        R sel = (area * lum) / rmax((R)s->weight, (R)1e-6);
+        R lp = ((sel * t) * t) / rmax(lc * area, (R)1e-6);
+        R sp = lp * (R)(st->sec_area > 1u ? st->sec_area : 1u);
+        if (flags & 4u) sp = lp * unified_prob(s, st);
+        emis = power_h(prev_pdf, sp);
+      }
+      Lr = vadd(Lr, vs(vmul(T, em), emis));
+    }
+    uint32_t nb = bounce + 1u;
+    if (nb >= st->max_bounces) break;
+    R trans = m.trans, bsdf_pdf = 0;
+    V nd;
+    if (trans > (R)0.001) {
+      R cur = medium_get(stack, depth - 1u), target = 1;
+      if (entering) target = rmax(m.ior, (R)1.0001);
+      else if (depth > 1u) target = medium_get(stack, depth - 2u);
+      R eta = cur / target, ndi = dot(n, inc);
+      R k = (R)1 - eta * eta * ((R)1 - ndi * ndi);
+      V rd = k < (R)0 ? v3(0, 0, 0) : vsub(sv(eta, inc), sv(eta * ndi + rsqrt_(k), n));
+      if (dot(rd, rd) < (R)0.01) {
+        rd = reflect_(inc, n);
+      } else if (entering && depth < 16u) {
+        stack[depth] = target;
+        depth++;
+      } else if (!entering && depth > 1u) {
+        depth--;
+      }
+      R optical = entering ? (R)0 : t;
+      if (m.thin > (R)0.5) optical = m.thick;
+      R ex = optical / rmax(m.adist, (R)1e-6);
+      V ab;
+      if (sizeof(R) == 4) {
+        ab = v3((R)powf((float)rmax(m.atten.x, (R)1e-6), (float)ex), (R)powf((float)rmax(m.atten.y, (R)1e-6), (float)ex),
+                (R)powf((float)rmax(m.atten.z, (R)1e-6), (float)ex));
+      } else {
+        ab = v3((R)pow((double)rmax(m.atten.x, (R)1e-6), (double)ex), (R)pow((double)rmax(m.atten.y, (R)1e-6), (double)ex),
+                (R)pow((double)rmax(m.atten.z, (R)1e-6), (double)ex));
+      }
+      V tint = mixv(v3(1, 1, 1), m.base, m.thin);
+      T = vs(vmul(vmul(T, ab), tint), trans);
+      nd = rd;
+    } else {
+      R np = clampr((R)st->nee_prob, (R)1e-6, (R)1);
+      int do_nee = 1;
+      if (!(np >= (R)0.999999)) {
+        uint32_t scr = hash32((pixel ^ hash32(nb + 1u)) ^ hash32((frame_sample & 255u) + 1u));
+        uint32_t seq = brev(frame_sample >> 8) ^ scr;
+        do_nee = ((R)seq + (R)0.5) * (R)(1.0 / 4294967296.0) < np;
+      }
+      if (do_nee) {
+        V direct = v3(0, 0, 0), c;
+        for (uint32_t li = 0; li < 64u && li < s->P; ++li) {
+          if (point_sample(s, li, hp, n, &m, inc, &c)) direct = vadd(direct, c);
+        }
+        if (st->unified) {
+          int area_on = s->L > 0, env_on = st->env_samples > 0;
+          if (area_on || env_on) {
+            R aw = 0, ew = 0;
+            if (area_on) aw = rsqrt_(rmax((R)s->weight, (R)1e-6)) * (R)(st->sec_area > 1u ? st->sec_area : 1u);
+            if (env_on) ew = (R)(st->env_samples < 4u ? st->env_samples : 4u);
+            R ap = aw / rmax(aw + ew, (R)1e-6);
+            rs = pcg(rs);
+            if (pcg_unit(rs) < ap) {
+              if (area_sample(s, &rs, hp, n, &m, inc, 0, 1, ap, &c)) direct = vadd(direct, c);
+            } else {
+              if (env_sample(s, &rs, hp, n, &m, inc, 1, (R)1 - ap, &c)) direct = vadd(direct, c);
+            }
+          }
+        } else {
+          uint32_t ac = st->sec_area < 1u ? 1u : (st->sec_area > 16u ? 16u : st->sec_area);
+          V ad = v3(0, 0, 0), ed = v3(0, 0, 0);
+          for (uint32_t k = 0; k < ac; ++k)
+            if (area_sample(s, &rs, hp, n, &m, inc, k, ac, 1, &c)) ad = vadd(ad, c);
+          direct = vadd(direct, vdiv(ad, (R)ac));
+          uint32_t ec = st->env_samples < 4u ? st->env_samples : 4u;
+          for (uint32_t k = 0; k < ec; ++k)
+            if (env_sample(s, &rs, hp, n, &m, inc, ec, 1, &c)) ed = vadd(ed, c);
+          if (ec > 0) direct = vadd(direct, vdiv(ed, (R)ec));
+        }
+        Lr = vadd(Lr, vdiv(vmul(T, direct), rmax(np, (R)1e-6)));
+      }
+      V view = vneg(inc);
+      R prob = spec_prob(&m);
+      rs = pcg(rs);
+      int spec = pcg_unit(rs) < prob;
+      rs = pcg(rs);
+      R u1 = pcg_unit(rs);
+      rs = pcg(rs);
+      R u2 = pcg_unit(rs);
+      V ov = cosine_dir(n, u1, u2);
+      if (spec) ov = reflect_(inc, ggx_half(n, m.rough, u1, u2));
+      if (dot(n, ov) <= (R)0) {
+        rs = pcg(rs);
+        u1 = pcg_unit(rs);
+        rs = pcg(rs);
+        u2 = pcg_unit(rs);
+        ov = cosine_dir(n, u1, u2);
+      }
+      ov = norm(ov);
+      R pdf = rmax(pdf_pbr(&m, n, view, ov), (R)1e-6);
+      V w = vdiv(vs(eval_pbr(&m, n, view, ov), rmax(dot(n, ov), (R)0)), pdf);
+      w = vs(w, mixr(m.occl, (R)1, m.metal));
+      T = vmul(T, w);
+      nd = ov;
+      bsdf_pdf = pdf;
+    }
+    if (!(st->rr_start == 0 || nb < st->rr_start || trans > (R)0.001)) {
+      R surv = clampr(rmax(T.x, rmax(T.y, T.z)), (R)st->rr_min, (R)0.95);
+      rs = pcg(rs);
+      if (pcg_unit(rs) >= surv) break;
+      T = vdiv(T, surv);
+    }
+    d = norm(nd);
+    bounce = nb;
+    flags = 1u | (depth << 8);
+    if (trans <= (R)0.001) {
+      flags |= 2u;
+      if (st->unified) flags |= 4u;
+      prev_pdf = bsdf_pdf;
+    }
+    rng = rs;
+    o = vadd(hp, vs(d, (R)0.002));
+    tmin = (R)0.001;
+    tmax = (R)1e30;
+    counts[1]++;
+  }
+  out[0] = Lr.x;
+  out[1] = Lr.y;
+  out[2] = Lr.z;
+}
+
+/* Camera ray of wavefront_generate.comp for one (pixel, sample). */
+static void camera_ray(const float cam[12], uint32_t proj, uint32_t W, uint32_t H, uint32_t px, uint32_t py,
+                       uint32_t frame, uint32_t sample, V* o, V* d) {
+  uint32_t pixel = py * W + px;
+  uint32_t r = hash32((pixel ^ hash32(frame)) ^ hash32(sample + 1u));
+  r = hash32(r);
+  R jx = unit(r);
+  r = hash32(r);
+  R jy = unit(r);
+  R ndx = (((R)px + jx) / (R)W) * (R)2 - (R)1;
+  R ndy = (((R)py + jy) / (R)H) * (R)2 - (R)1;
+  R aspect = (R)W / (R)H;
+  V org = v3(cam[0], cam[1], cam[2]), f = v3(cam[3], cam[4], cam[5]), rt = v3(cam[6], cam[7], cam[8]),
+    up = v3(cam[9], cam[10], cam[11]);
+  if (proj == 1) {
+    *o = vsub(vadd(org, sv(ndx * aspect, rt)), sv(ndy, up));
+    *d = norm(f);
+  } else if (proj == 2) {
+    R yaw = ndx * len(rt), pitch = (-ndy) * len(up), sy, cy, sp, cp;
+    sincos_r(yaw, &sy, &cy);
+    sincos_r(pitch, &sp, &cp);
+    *o = org;
+    *d = norm(vadd(vadd(vs(vs(norm(f), cp), cy), vs(vs(norm(rt), cp), sy)), vs(norm(up), sp)));
+  } else {
+    *o = org;
+    *d = norm(vsub(vadd(f, sv(ndx * aspect, rt)), sv(ndy, up)));
+  }
+}
+
+/* Renders rows [row0, row1) into out (W*H*4 floats, full image layout).
+ * settings: max_bounces, sec_area, env_samples, rr_start, rr_min, nee_prob,
+ * unified.  frame is the float32-rounded frame index.  counts[3] receives
+ * primary/extension ray counts (shadow rays are not counted here). */
+int API(olgi_render)(void* h, const float cam[12], uint32_t proj, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame,
+                     const uint32_t iset[4], const float fset[2], uint32_t unified, uint32_t row0, uint32_t row1,
+                     float* out, uint64_t* counts_out) {
+  const Scene* s = (const Scene*)h;
+  Settings st = {iset[0], iset[1], iset[2], iset[3], fset[0], fset[1], unified};
+  uint64_t c0 = 0, c1 = 0, c2 = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : c0, c1, c2)
+  for (int64_t py = (int64_t)row0; py < (int64_t)row1; ++py) {
+    uint64_t counts[3] = {0, 0, 0};
+    g_shadow_rays = 0;
+    for (uint32_t px = 0; px < W; ++px) {
+      double acc[3] = {0, 0, 0};
+      for (uint32_t smp = 0; smp < spp; ++smp) {
+        V o, d;
+        camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, smp, &o, &d);
+        R rad[3];
+        trace_path(s, &st, (uint32_t)py * W + px, frame, smp, o, d, rad, counts);
+        /* float32 radiance per sample, summed in float64 (api.py:1404-1407) */
+        for (int k = 0; k < 3; ++k) acc[k] += (double)(float)rad[k];
+      }
+      float* p = out + 4ull * ((uint64_t)py * W + px);
+      for (int k = 0; k < 3; ++k) p[k] = (float)(acc[k] / (double)spp);
+      p[3] = (float)((double)spp / (double)spp);
+    }
+    c0 += counts[0];
+    c1 += counts[1];
+    c2 += g_shadow_rays;
+  }
+  if (counts_out) {
+    counts_out[0] = c0;
+    counts_out[1] = c1;
+    counts_out[2] = c2;
+  }
+  return 0;
+}
+
+/* Closest hits for caller rays: o/d [n][4] (xyz + tmin/tmax); hits [n][4]. */
+int API(olgi_trace)(void* h, const float* o, const float* d, uint32_t n, float* hits) {
+  const Scene* s = (const Scene*)h;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    R t, u = 0, v = 0;
+    uint32_t prim;
+    V oo = v3(o[4 * i], o[4 * i + 1], o[4 * i + 2]), dd = v3(d[4 * i], d[4 * i + 1], d[4 * i + 2]);
+    int hit = closest(s, oo, dd, (R)o[4 * i + 3], (R)d[4 * i + 3], &t, &u, &v, &prim);
+    hits[4 * i] = (float)t;
+    hits[4 * i + 1] = (float)u;
+    hits[4 * i + 2] = (float)v;
+    uint32_t id = hit ? prim : 0xFFFFFFFFu;
+    memcpy(&hits[4 * i + 3], &id, 4);
+  }
+  return 0;
+}
+
+/* Known-answer helpers for tests. */
+uint32_t API(olgi_wave_hash)(uint32_t v) { return hash32(v); }
+uint32_t API(olgi_pcg_step)(uint32_t s) { return pcg(s); }
+float API(olgi_pcg_value)(uint32_t s) { return (float)pcg_unit(s); }
+void API(olgi_sincos)(float x, float* s, float* c) {
+  R a, b;
+  sincos_r((R)x, &a, &b);
+  *s = (float)a;
+  *c = (float)b;
+}
+
+/* BSDF hooks for property tests: material record (48 floats), unit normal,
+ * view and outgoing directions (3 floats each). */
+void API(olgi_eval_pbr)(const float* rec, const float* n, const float* view, const float* out, float* f_out,
+                        float* pdf_out) {
+  Scene tmp;
+  memset(&tmp, 0, sizeof(tmp));
+  tmp.mat = (float*)rec;
+  Mat m = load_mat(&tmp, 0);
+  V N = v3(n[0], n[1], n[2]), Vw = v3(view[0], view[1], view[2]), O = v3(out[0], out[1], out[2]);
+  V f = eval_pbr(&m, N, Vw, O);
+  f_out[0] = (float)f.x;
+  f_out[1] = (float)f.y;
+  f_out[2] = (float)f.z;
+  *pdf_out = (float)pdf_pbr(&m, N, Vw, O);
+}
+
+/* One BSDF sample (shadeSamplePbr): outgoing direction, weight, pdf. */
+uint32_t API(olgi_sample_pbr)(const float* rec, const float* n, const float* incoming, uint32_t rs, float* out_dir,
+                              float* weight, float* pdf_out) {
+  Scene tmp;
+  memset(&tmp, 0, sizeof(tmp));
+  tmp.mat = (float*)rec;
+  Mat m = load_mat(&tmp, 0);
+  V N = v3(n[0], n[1], n[2]), I = v3(incoming[0], incoming[1], incoming[2]);
+  V view = vneg(I);
+  R prob = spec_prob(&m);
+  rs = pcg(rs);
+  int spec = pcg_unit(rs) < prob;
+  rs = pcg(rs);
+  R u1 = pcg_unit(rs);
+  rs = pcg(rs);
+  R u2 = pcg_unit(rs);
+  V ov = cosine_dir(N, u1, u2);
+  if (spec) ov = reflect_(I, ggx_half(N, m.rough, u1, u2));
+  if (dot(N, ov) <= (R)0) {
+    rs = pcg(rs);
+    u1 = pcg_unit(rs);
+    rs = pcg(rs);
+    u2 = pcg_unit(rs);
+    ov = cosine_dir(N, u1, u2);
+  }
+  ov = norm(ov);
+  R pdf = rmax(pdf_pbr(&m, N, view, ov), (R)1e-6);
+  V w = vdiv(vs(eval_pbr(&m, N, view, ov), rmax(dot(N, ov), (R)0)), pdf);
+  w = vs(w, mixr(m.occl, (R)1, m.metal));
+  out_dir[0] = (float)ov.x;
+  out_dir[1] = (float)ov.y;
+  out_dir[2] = (float)ov.z;
+  weight[0] = (float)w.x;
+  weight[1] = (float)w.y;
+  weight[2] = (float)w.z;
+  *pdf_out = (float)pdf;
+  return rs;
+}

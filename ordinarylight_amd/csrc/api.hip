@@ -1,0 +1,682 @@
+// C ABI of the olhip library (include/olhip.h): context, scene upload and
+// BVH build, GI frames, closest-hit queries and the reference-estimator mode.
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/olhip.h"
+#include "e0.h"
+#include "gi.h"
+#include "internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return OLHIP_OK;
+  } catch (const OlError& e) {
+    return fail(e.code, e.message);
+  } catch (const std::bad_alloc&) {
+    return fail(OLHIP_ERR_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(OLHIP_ERR_RUNTIME, e.what());
+  }
+}
+
+void require(bool ok, const std::string& message) {
+  if (!ok) throw OlError{OLHIP_ERR_VALUE, message};
+}
+
+template <class T>
+void upload(DevBuf& buf, const T* host, size_t count, hipStream_t s) {
+  buf.reserve(sizeof(T) * std::max<size_t>(count, 1));
+  if (count) OL_HIP_CHECK(hipMemcpyAsync(buf.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { OL_HIP_CHECK(hipEventCreate(&e)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+};
+
+float elapsed(const Event& a, const Event& b) {
+  float ms = 0.0f;
+  OL_HIP_CHECK(hipEventElapsedTime(&ms, a.e, b.e));
+  return ms;
+}
+
+}  // namespace
+
+struct olhip_scene {
+  olhip_ctx* ctx = nullptr;
+  DevBuf positions, normals, vidx, materials, analytic, area;
+  BvhOutput bvh;
+  DScene view{};
+  uint32_t tri_count = 0, vertex_count = 0;
+  bool has_transmission = false;
+  uint32_t point_count = 0;
+  std::vector<float> materials_host;
+};
+
+struct olhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int profiling = 0;
+  int counting = 0;
+  int stack_override = 0;
+  // GI frame buffers (grown on demand, reused across frames).
+  DevBuf T, L, M, stk, qo0, qo1, qd0, qd1, qp0, qp1, hit, gT, gI, eo, ed, ec, ctr, stats, work, acc, image;
+  uint32_t image_pixels = 0;
+  // Reference-estimator frame.
+  struct E0Frame {
+    DevBuf o, d, T, rad, acc, active, flags, scan, ht, htri, hitlist, hit_t, hit_tri, jx, jy, gauss, env, ctr, out;
+    E0Params params{};
+    E0State state{};
+    const olhip_scene* scene = nullptr;
+    uint32_t n_active = 0, n_hit = 0;
+    int stack = 32;
+    bool open = false;
+  } e0;
+};
+
+extern "C" {
+
+const char* olhip_last_error(void) { return g_last_error.c_str(); }
+
+int olhip_version(void) { return 100; }
+
+int olhip_device_count(int* count) {
+  return guarded([&] {
+    require(count != nullptr, "count must not be NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int olhip_create(int device, olhip_ctx** out) {
+  return guarded([&] {
+    require(out != nullptr, "out must not be NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw OlError{OLHIP_ERR_RUNTIME, "no HIP device is available"};
+    require(device >= 0 && device < n, "device index out of range");
+    OL_HIP_CHECK(hipSetDevice(device));
+    auto ctx = std::make_unique<olhip_ctx>();
+    ctx->device = device;
+    OL_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    OL_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    ctx->cu_count = prop.multiProcessorCount;
+    *out = ctx.release();
+  });
+}
+
+int olhip_destroy(olhip_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int olhip_set_profiling(olhip_ctx* ctx, int enabled) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx must not be NULL");
+    ctx->profiling = enabled ? 1 : 0;
+  });
+}
+
+int olhip_set_work_counters(olhip_ctx* ctx, int enabled) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx must not be NULL");
+    ctx->counting = enabled ? 1 : 0;
+  });
+}
+
+int olhip_set_stack_depth(olhip_ctx* ctx, int depth) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx must not be NULL");
+    const int allowed[] = {0, 16, 24, 32, 48, 64, 96, 128};
+    require(std::find(std::begin(allowed), std::end(allowed), depth) != std::end(allowed),
+            "stack depth must be one of 0, 16, 24, 32, 48, 64, 96, 128");
+    ctx->stack_override = depth;
+  });
+}
+
+int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** out) {
+  return guarded([&] {
+    require(ctx && d && out, "ctx, desc and out must not be NULL");
+    require(d->triangle_count == 0 || (d->positions && d->indices && d->material_ids), "geometry arrays missing");
+    require(d->vertex_count == 0 || d->normals, "normals missing");
+    require(d->material_count > 0 && d->materials, "at least one material is required");
+    require(d->analytic_light_count <= 64, "at most 64 analytic lights are supported");
+    require(d->analytic_light_count == 0 || d->analytic_lights, "analytic lights missing");
+    require(d->area_light_count == 0 || d->area_lights, "area lights missing");
+    require(d->leaf_size <= 16, "leaf_size must be 0..16");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    auto sc = std::make_unique<olhip_scene>();
+    sc->ctx = ctx;
+    sc->tri_count = d->triangle_count;
+    sc->vertex_count = d->vertex_count;
+    // Host-side validation and packing to 16-byte records.
+    std::vector<float4> pos(d->vertex_count), nrm(d->vertex_count);
+    for (uint32_t i = 0; i < d->vertex_count; ++i) {
+      pos[i] = make_float4(d->positions[3 * i], d->positions[3 * i + 1], d->positions[3 * i + 2], 0.0f);
+      nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
+      require(std::isfinite(pos[i].x) && std::isfinite(pos[i].y) && std::isfinite(pos[i].z),
+              "vertex positions must be finite");
+    }
+    std::vector<uint4> vidx(d->triangle_count);
+    for (uint32_t t = 0; t < d->triangle_count; ++t) {
+      uint32_t a = d->indices[3 * t], b = d->indices[3 * t + 1], c = d->indices[3 * t + 2];
+      require(a < d->vertex_count && b < d->vertex_count && c < d->vertex_count, "triangle index out of range");
+      require(d->material_ids[t] < d->material_count, "material id out of range");
+      vidx[t] = make_uint4(a, b, c, d->material_ids[t]);
+    }
+    sc->materials_host.assign(d->materials, d->materials + 48ull * d->material_count);
+    for (uint32_t m = 0; m < d->material_count; ++m)
+      if (d->materials[48ull * m + 11] > 0.001f) sc->has_transmission = true;
+    upload(sc->positions, pos.data(), pos.size(), s);
+    upload(sc->normals, nrm.data(), nrm.size(), s);
+    upload(sc->vidx, vidx.data(), vidx.size(), s);
+    upload(sc->materials, reinterpret_cast<const float4*>(d->materials), 12ull * d->material_count, s);
+    std::vector<float4> analytic(4ull * std::max<uint32_t>(d->analytic_light_count, 1), make_float4(0, 0, 0, 0));
+    if (d->analytic_light_count)
+      std::memcpy(analytic.data(), d->analytic_lights, sizeof(float) * 16ull * d->analytic_light_count);
+    upload(sc->analytic, analytic.data(), analytic.size(), s);
+    std::vector<float4> area(5ull * std::max<uint32_t>(d->area_light_count, 1), make_float4(0, 0, 0, 0));
+    if (d->area_light_count) std::memcpy(area.data(), d->area_lights, sizeof(float) * 20ull * d->area_light_count);
+    upload(sc->area, area.data(), area.size(), s);
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, d->leaf_size};
+    bvh_build(in, sc->bvh, s, nullptr);
+    DScene& v = sc->view;
+    v.nodes = sc->bvh.nodes.as<WideNode>();
+    v.tri = sc->bvh.tri.as<float4>();
+    v.tri_vidx = sc->vidx.as<uint4>();
+    v.normals = sc->normals.as<float4>();
+    v.materials = sc->materials.as<float4>();
+    v.analytic = sc->analytic.as<float4>();
+    v.area = sc->area.as<float4>();
+    v.tri_count = d->triangle_count;
+    v.node_count = sc->bvh.node_count;
+    v.material_count = d->material_count;
+    v.analytic_count = d->analytic_light_count;
+    v.area_count = d->area_light_count;
+    v.area_weight = d->area_light_weight;
+    v.env_valid = 0;
+    for (uint32_t l = 0; l < d->analytic_light_count; ++l) {
+      const float* L = d->analytic_lights + 16ull * l;
+      if ((int)(L[3] + 0.5f) == 3) {
+        require(L[12] < 0.0f, "image environments are not supported by the GI estimator");
+        v.env_valid = 1;
+        v.env_r = (1.0f * L[8]) * L[11];
+        v.env_g = (1.0f * L[9]) * L[11];
+        v.env_b = (1.0f * L[10]) * L[11];
+        break;
+      }
+    }
+    sc->point_count = d->analytic_light_count;
+    *out = sc.release();
+  });
+}
+
+int olhip_scene_destroy(olhip_scene* scene) {
+  return guarded([&] {
+    if (!scene) return;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+    delete scene;
+  });
+}
+
+int olhip_scene_info_get(const olhip_scene* scene, olhip_scene_info* out) {
+  return guarded([&] {
+    require(scene && out, "scene and out must not be NULL");
+    out->triangle_count = scene->tri_count;
+    out->node_count = scene->bvh.node_count;
+    out->bvh_depth = scene->bvh.depth;
+    out->stack_depth = (uint32_t)std::max(0, stack_depth_for(scene->bvh.depth));
+    out->build_ms = scene->bvh.build_ms;
+    out->device_bytes = scene->positions.bytes + scene->normals.bytes + scene->vidx.bytes + scene->materials.bytes +
+                        scene->analytic.bytes + scene->area.bytes + scene->bvh.nodes.bytes + scene->bvh.tri.bytes;
+  });
+}
+
+int olhip_scene_bvh(const olhip_scene* scene, void* nodes, void* tri) {
+  return guarded([&] {
+    require(scene != nullptr, "scene must not be NULL");
+    OL_HIP_CHECK(hipSetDevice(scene->ctx->device));
+    OL_HIP_CHECK(hipStreamSynchronize(scene->ctx->stream));
+    if (nodes && scene->bvh.node_count)
+      OL_HIP_CHECK(hipMemcpy(nodes, scene->bvh.nodes.ptr, 64ull * scene->bvh.node_count, hipMemcpyDeviceToHost));
+    if (tri && scene->tri_count)
+      OL_HIP_CHECK(hipMemcpy(tri, scene->bvh.tri.ptr, 48ull * scene->tri_count, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+int choose_stack(const olhip_ctx* ctx, const olhip_scene* sc) {
+  int need = stack_depth_for(std::max<uint32_t>(sc->bvh.depth, 1));
+  if (need < 0) throw OlError{OLHIP_ERR_RUNTIME, "BVH too deep for the traversal stack"};
+  if (ctx->stack_override) {
+    if (ctx->stack_override < need) throw OlError{OLHIP_ERR_VALUE, "stack depth override below the BVH requirement"};
+    return ctx->stack_override;
+  }
+  return need;
+}
+
+}  // namespace
+
+extern "C" {
+
+int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* cam, uint32_t width, uint32_t height,
+                    uint32_t samples, uint32_t frame_index, const olhip_gi_settings* set, float* out_rgba,
+                    olhip_stats* stats_out) {
+  return guarded([&] {
+    require(ctx && sc && cam && set, "ctx, scene, camera and settings must not be NULL");
+    require(sc->ctx == ctx, "scene belongs to another context");
+    require(width >= 1 && height >= 1, "width and height must be positive");
+    require((uint64_t)width * height <= (1ull << 28), "image too large");
+    require(samples >= 1 && samples <= 65536, "samples must be between 1 and 65536");
+    require(set->max_bounces >= 1 && set->max_bounces <= 64, "max_bounces must be between 1 and 64");
+    require(cam->projection <= 2, "unknown projection");
+    require(set->secondary_nee_probability > 0.0f && set->secondary_nee_probability <= 1.0f,
+            "secondary_nee_probability must be in (0, 1]");
+    require(set->russian_roulette_min_survival >= 0.0f && set->russian_roulette_min_survival <= 1.0f,
+            "russian_roulette_min_survival must be in [0, 1]");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const uint32_t np = width * height;
+    uint64_t capacity = set->path_capacity ? set->path_capacity : (1ull << 25);
+    uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, capacity / np));
+    if ((uint64_t)chunk * np > (1ull << 31)) chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 31) / np);
+    const uint64_t paths = (uint64_t)chunk * np;
+    // Buffers.
+    ctx->T.reserve(16 * paths);
+    ctx->L.reserve(16 * paths);
+    ctx->M.reserve(16 * paths);
+    if (sc->has_transmission) ctx->stk.reserve(64 * paths);
+    ctx->qo0.reserve(16 * paths);
+    ctx->qo1.reserve(16 * paths);
+    ctx->qd0.reserve(16 * paths);
+    ctx->qd1.reserve(16 * paths);
+    ctx->qp0.reserve(4 * paths);
+    ctx->qp1.reserve(4 * paths);
+    ctx->hit.reserve(16 * paths);
+    const bool two_pass = sc->point_count > (sc->view.env_valid ? 1u : 0u) || !set->unified_secondary_nee;
+    uint32_t per_hit = 1;
+    if (two_pass) per_hit = sc->point_count + (set->unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(set->secondary_area_samples, 1u), 16u) + 4u));
+    ctx->gT.reserve(16 * paths);
+    ctx->gI.reserve(16 * paths);
+    const uint64_t entries = paths * per_hit;
+    ctx->eo.reserve(16 * entries);
+    ctx->ed.reserve(16 * entries);
+    ctx->ec.reserve(16 * entries);
+    ctx->ctr.reserve(64);
+    ctx->stats.reserve(64);
+    ctx->work.reserve(64);
+    ctx->acc.reserve(sizeof(double4) * np);
+    ctx->image.reserve(sizeof(float4) * np);
+    ctx->image_pixels = np;
+
+    GiState S{};
+    S.T = ctx->T.as<float4>();
+    S.L = ctx->L.as<float4>();
+    S.M = ctx->M.as<uint4>();
+    S.stk = sc->has_transmission ? ctx->stk.as<float4>() : nullptr;
+    S.qo[0] = ctx->qo0.as<float4>();
+    S.qo[1] = ctx->qo1.as<float4>();
+    S.qd[0] = ctx->qd0.as<float4>();
+    S.qd[1] = ctx->qd1.as<float4>();
+    S.qp[0] = ctx->qp0.as<uint32_t>();
+    S.qp[1] = ctx->qp1.as<uint32_t>();
+    S.hit = ctx->hit.as<float4>();
+    S.gT = ctx->gT.as<float4>();
+    S.gI = ctx->gI.as<uint4>();
+    S.eo = ctx->eo.as<float4>();
+    S.ed = ctx->ed.as<float4>();
+    S.ec = ctx->ec.as<float4>();
+    S.ctr = ctx->ctr.as<uint32_t>();
+    S.stats = ctx->stats.as<unsigned long long>();
+    S.work = ctx->counting ? ctx->work.as<unsigned long long>() : nullptr;
+    S.qcap = (uint32_t)paths;
+    S.gcap = (uint32_t)paths;
+    S.ecap = (uint32_t)std::min<uint64_t>(entries, 0xFFFFFFFFull);
+
+    GiParams P{};
+    P.sc = sc->view;
+    for (int k = 0; k < 3; ++k) {
+      P.cam_o[k] = cam->position[k];
+      P.cam_f[k] = cam->forward[k];
+      P.cam_r[k] = cam->right[k];
+      P.cam_u[k] = cam->up[k];
+    }
+    P.projection = cam->projection;
+    P.width = width;
+    P.height = height;
+    // The reference stores the frame index as float32 in the camera block.
+    P.frame = (uint32_t)((float)frame_index + 0.5f);
+    P.max_bounces = set->max_bounces;
+    P.point_count = sc->point_count;
+    P.area_count = sc->view.area_count;
+    P.secondary_area_samples = set->secondary_area_samples;
+    P.environment_samples = set->environment_samples;
+    P.roulette_start = set->russian_roulette_start;
+    P.roulette_min_survival = set->russian_roulette_min_survival;
+    P.nee_probability = set->secondary_nee_probability;
+    P.unified = set->unified_secondary_nee ? 1u : 0u;
+    P.two_pass = two_pass ? 1u : 0u;
+
+    GiLaunch l{ctx->cu_count, choose_stack(ctx, sc), 128};
+    Event e_start, e_end;
+    std::vector<std::unique_ptr<Event>> marks;
+    double trace_ms = 0, shade_ms = 0, shadow_ms = 0;
+    uint32_t n_trace = 0, n_shade = 0, n_shadow = 0;
+    OL_HIP_CHECK(hipEventRecord(e_start.e, s));
+    gi_reset(S, s);
+    uint32_t chunks = 0;
+    for (uint32_t base = 0; base < samples; base += chunk) {
+      P.sample_base = base;
+      P.spp = std::min(chunk, samples - base);
+      OL_HIP_CHECK(hipMemsetAsync(S.ctr, 0, 8 * sizeof(uint32_t), s));
+      gi_generate(P, S, s);
+      for (uint32_t b = 0; b < set->max_bounces; ++b) {
+        const uint32_t in = b & 1u;
+        gi_bounce_prologue(S, b, s);
+        if (ctx->profiling) {
+          auto a = std::make_unique<Event>(), c = std::make_unique<Event>(), d = std::make_unique<Event>(),
+               e = std::make_unique<Event>();
+          OL_HIP_CHECK(hipEventRecord(a->e, s));
+          gi_trace_closest(l, P.sc, S, in, s);
+          OL_HIP_CHECK(hipEventRecord(c->e, s));
+          gi_shade(l, P, S, in, s);
+          OL_HIP_CHECK(hipEventRecord(d->e, s));
+          gi_trace_shadow(l, P.sc, S, s);
+          OL_HIP_CHECK(hipEventRecord(e->e, s));
+          marks.push_back(std::move(a));
+          marks.push_back(std::move(c));
+          marks.push_back(std::move(d));
+          marks.push_back(std::move(e));
+        } else {
+          gi_trace_closest(l, P.sc, S, in, s);
+          gi_shade(l, P, S, in, s);
+          gi_trace_shadow(l, P.sc, S, s);
+        }
+        ++n_trace;
+        ++n_shade;
+        ++n_shadow;
+      }
+      gi_finish(S, s);
+      gi_accumulate(P, S, ctx->acc.as<double4>(), base == 0, s);
+      ++chunks;
+    }
+    gi_final(ctx->acc.as<double4>(), np, samples, ctx->image.as<float4>(), s);
+    OL_HIP_CHECK(hipGetLastError());
+    OL_HIP_CHECK(hipEventRecord(e_end.e, s));
+    OL_HIP_CHECK(hipEventSynchronize(e_end.e));
+    uint32_t ctr_host[8];
+    unsigned long long st_host[4], work_host[4] = {0, 0, 0, 0};
+    OL_HIP_CHECK(hipMemcpy(ctr_host, S.ctr, sizeof(ctr_host), hipMemcpyDeviceToHost));
+    OL_HIP_CHECK(hipMemcpy(st_host, S.stats, sizeof(st_host), hipMemcpyDeviceToHost));
+    if (S.work) OL_HIP_CHECK(hipMemcpy(work_host, S.work, sizeof(work_host), hipMemcpyDeviceToHost));
+    if (ctr_host[5]) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
+    for (size_t k = 0; k + 3 < marks.size(); k += 4) {
+      trace_ms += elapsed(*marks[k], *marks[k + 1]);
+      shade_ms += elapsed(*marks[k + 1], *marks[k + 2]);
+      shadow_ms += elapsed(*marks[k + 2], *marks[k + 3]);
+    }
+    if (out_rgba)
+      OL_HIP_CHECK(hipMemcpy(out_rgba, ctx->image.ptr, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    if (stats_out) {
+      olhip_stats& o = *stats_out;
+      o.primary_rays = st_host[0];
+      o.extension_rays = st_host[1];
+      o.shadow_rays = st_host[2];
+      o.queue_overflow = ctr_host[4];
+      o.total_ms = elapsed(e_start, e_end);
+      o.trace_ms = trace_ms;
+      o.shade_ms = shade_ms;
+      o.shadow_ms = shadow_ms;
+      o.trace_launches = n_trace;
+      o.shade_launches = n_shade;
+      o.shadow_launches = n_shadow;
+      o.chunks = chunks;
+      o.closest_nodes = work_host[0];
+      o.closest_tris = work_host[1];
+      o.shadow_nodes = work_host[2];
+      o.shadow_tris = work_host[3];
+    }
+  });
+}
+
+int olhip_frame_download(olhip_ctx* ctx, float* out_rgba) {
+  return guarded([&] {
+    require(ctx && out_rgba, "ctx and out must not be NULL");
+    require(ctx->image_pixels > 0, "no frame has been rendered");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    OL_HIP_CHECK(hipMemcpy(out_rgba, ctx->image.ptr, sizeof(float4) * ctx->image_pixels, hipMemcpyDeviceToHost));
+  });
+}
+
+int olhip_trace(olhip_ctx* ctx, const olhip_scene* sc, const float* origins, const float* directions, uint32_t n,
+                float* hits) {
+  return guarded([&] {
+    require(ctx && sc && origins && directions && hits, "arguments must not be NULL");
+    require(sc->ctx == ctx, "scene belongs to another context");
+    if (n == 0) return;
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    DevBuf o, d, h, ov;
+    upload(o, reinterpret_cast<const float4*>(origins), n, s);
+    upload(d, reinterpret_cast<const float4*>(directions), n, s);
+    h.reserve(sizeof(float4) * n);
+    ov.reserve(4);
+    OL_HIP_CHECK(hipMemsetAsync(ov.ptr, 0, 4, s));
+    GiLaunch l{ctx->cu_count, choose_stack(ctx, sc), 128};
+    trace_rays(l, sc->view, o.as<float4>(), d.as<float4>(), n, h.as<float4>(), ov.as<uint32_t>(), nullptr, s);
+    OL_HIP_CHECK(hipGetLastError());
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    uint32_t overflow = 0;
+    OL_HIP_CHECK(hipMemcpy(&overflow, ov.ptr, 4, hipMemcpyDeviceToHost));
+    if (overflow) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
+    OL_HIP_CHECK(hipMemcpy(hits, h.ptr, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+// ---- reference-estimator mode -------------------------------------------------------------------
+int olhip_e0_begin(olhip_ctx* ctx, const olhip_scene* sc, const olhip_e0_camera* cam, uint32_t width,
+                   uint32_t height, const olhip_e0_env* env, uint32_t brute_force) {
+  return guarded([&] {
+    require(ctx && sc && cam && env, "arguments must not be NULL");
+    require(sc->ctx == ctx, "scene belongs to another context");
+    require(width >= 1 && height >= 1, "width and height must be positive");
+    require((uint64_t)width * height <= (1ull << 27), "image too large");
+    require(cam->projection <= 2, "unknown projection");
+    require(env->mode <= 2, "unknown environment mode");
+    require(env->mode != 2 || (env->image && env->width && env->height), "environment image missing");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    auto& f = ctx->e0;
+    const uint32_t n = width * height;
+    const size_t ot = cam->projection == 1 ? 8 : 4, dt = cam->projection == 1 ? 4 : 8;
+    f.o.reserve(3 * ot * n);
+    f.d.reserve(3 * dt * n);
+    f.T.reserve(3 * ot * n);
+    f.rad.reserve(3 * ot * n);
+    f.acc.reserve(12ull * n);
+    f.active.reserve(4ull * n);
+    f.flags.reserve(4ull * n);
+    f.scan.reserve(4ull * (n + 1));
+    f.ht.reserve(4ull * n);
+    f.htri.reserve(4ull * n);
+    f.hitlist.reserve(4ull * n);
+    f.hit_t.reserve(4ull * n);
+    f.hit_tri.reserve(4ull * n);
+    f.jx.reserve(4ull * n);
+    f.jy.reserve(4ull * n);
+    f.gauss.reserve(12ull * n);
+    f.ctr.reserve(16);
+    f.out.reserve(16ull * n);
+    OL_HIP_CHECK(hipMemsetAsync(f.acc.ptr, 0, 12ull * n, s));
+    OL_HIP_CHECK(hipMemsetAsync(f.ctr.ptr, 0, 16, s));
+    E0Params& p = f.params;
+    p = E0Params{};
+    p.sc = sc->view;
+    p.positions = sc->positions.as<float4>();
+    for (int k = 0; k < 3; ++k) {
+      p.pos[k] = cam->position[k];
+      p.fwd[k] = cam->forward[k];
+      p.right[k] = cam->right[k];
+      p.up[k] = cam->up[k];
+    }
+    p.k_x = cam->k_x;
+    p.k_y = cam->k_y;
+    p.projection = cam->projection;
+    p.width = width;
+    p.height = height;
+    p.brute_force = (brute_force || sc->bvh.node_count == 0) ? 1u : 0u;
+    p.env_mode = env->mode;
+    for (int k = 0; k < 3; ++k) p.env_color[k] = env->color[k];
+    p.env_intensity = env->intensity;
+    p.env_rotation = env->rotation;
+    if (env->mode == 2) {
+      upload(f.env, env->image, 3ull * env->width * env->height, s);
+      p.env_image = f.env.as<float>();
+      p.env_w = env->width;
+      p.env_h = env->height;
+    }
+    E0State& st = f.state;
+    st.o = f.o.ptr;
+    st.d = f.d.ptr;
+    st.T = f.T.ptr;
+    st.rad = f.rad.ptr;
+    st.acc = f.acc.as<float>();
+    st.active = f.active.as<uint32_t>();
+    st.flags = f.flags.as<uint32_t>();
+    st.scan = f.scan.as<uint32_t>();
+    st.ht = f.ht.as<float>();
+    st.htri = f.htri.as<int>();
+    st.hitlist = f.hitlist.as<uint32_t>();
+    st.hit_t = f.hit_t.as<float>();
+    st.hit_tri = f.hit_tri.as<int>();
+    st.ctr = f.ctr.as<uint32_t>();
+    f.scene = sc;
+    f.stack = choose_stack(ctx, sc);
+    f.n_active = 0;
+    f.n_hit = 0;
+    f.open = true;
+  });
+}
+
+int olhip_e0_sample(olhip_ctx* ctx, const float* jx, const float* jy) {
+  return guarded([&] {
+    require(ctx && jx && jy, "arguments must not be NULL");
+    auto& f = ctx->e0;
+    if (!f.open) throw OlError{OLHIP_ERR_RUNTIME, "olhip_e0_begin has not been called"};
+    hipStream_t s = ctx->stream;
+    const uint32_t n = f.params.width * f.params.height;
+    OL_HIP_CHECK(hipMemcpyAsync(f.jx.ptr, jx, 4ull * n, hipMemcpyHostToDevice, s));
+    OL_HIP_CHECK(hipMemcpyAsync(f.jy.ptr, jy, 4ull * n, hipMemcpyHostToDevice, s));
+    e0_camera(f.params, f.state, f.jx.as<float>(), f.jy.as<float>(), s);
+    OL_HIP_CHECK(hipGetLastError());
+    f.n_active = n;
+  });
+}
+
+int olhip_e0_trace(olhip_ctx* ctx, uint32_t* n_hit, uint32_t* n_active) {
+  return guarded([&] {
+    require(ctx && n_hit, "arguments must not be NULL");
+    auto& f = ctx->e0;
+    if (!f.open) throw OlError{OLHIP_ERR_RUNTIME, "olhip_e0_begin has not been called"};
+    hipStream_t s = ctx->stream;
+    if (n_active) *n_active = f.n_active;
+    e0_trace(ctx->cu_count, f.stack, f.params, f.state, f.n_active, s);
+    OL_HIP_CHECK(hipGetLastError());
+    f.n_hit = e0_compact(f.state, f.n_active, s);
+    uint32_t ctr[4];
+    OL_HIP_CHECK(hipMemcpy(ctr, f.ctr.ptr, 16, hipMemcpyDeviceToHost));
+    if (ctr[1]) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
+    *n_hit = f.n_hit;
+  });
+}
+
+int olhip_e0_last_hits(olhip_ctx* ctx, int32_t* tri, uint32_t n_active) {
+  return guarded([&] {
+    require(ctx && tri, "arguments must not be NULL");
+    auto& f = ctx->e0;
+    require(n_active <= f.params.width * f.params.height, "n_active too large");
+    OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (n_active) OL_HIP_CHECK(hipMemcpy(tri, f.htri.ptr, 4ull * n_active, hipMemcpyDeviceToHost));
+  });
+}
+
+int olhip_e0_shade(olhip_ctx* ctx, const float* deviates, uint32_t n_hit) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx must not be NULL");
+    auto& f = ctx->e0;
+    if (!f.open) throw OlError{OLHIP_ERR_RUNTIME, "olhip_e0_begin has not been called"};
+    require(n_hit == f.n_hit, "deviate count does not match the last trace");
+    require(n_hit == 0 || deviates, "deviates must not be NULL");
+    hipStream_t s = ctx->stream;
+    if (n_hit) {
+      OL_HIP_CHECK(hipMemcpyAsync(f.gauss.ptr, deviates, 12ull * n_hit, hipMemcpyHostToDevice, s));
+      e0_shade(f.params, f.state, n_hit, f.gauss.as<float>(), s);
+      OL_HIP_CHECK(hipGetLastError());
+      // The hit rays form the next active list, already in ascending order.
+      OL_HIP_CHECK(hipMemcpyAsync(f.active.ptr, f.hitlist.ptr, 4ull * n_hit, hipMemcpyDeviceToDevice, s));
+    }
+    f.n_active = n_hit;
+  });
+}
+
+int olhip_e0_sample_end(olhip_ctx* ctx) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx must not be NULL");
+    auto& f = ctx->e0;
+    if (!f.open) throw OlError{OLHIP_ERR_RUNTIME, "olhip_e0_begin has not been called"};
+    e0_accumulate(f.params, f.state, ctx->stream);
+    OL_HIP_CHECK(hipGetLastError());
+    f.n_active = 0;
+  });
+}
+
+int olhip_e0_end(olhip_ctx* ctx, uint32_t samples, float* out_rgba) {
+  return guarded([&] {
+    require(ctx && out_rgba, "arguments must not be NULL");
+    require(samples >= 1, "samples must be positive");
+    auto& f = ctx->e0;
+    if (!f.open) throw OlError{OLHIP_ERR_RUNTIME, "olhip_e0_begin has not been called"};
+    hipStream_t s = ctx->stream;
+    e0_finish(f.params, f.state, samples, f.out.as<float4>(), s);
+    OL_HIP_CHECK(hipGetLastError());
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    OL_HIP_CHECK(hipMemcpy(out_rgba, f.out.ptr, 16ull * f.params.width * f.params.height, hipMemcpyDeviceToHost));
+    f.open = false;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Host-side declarations shared by the olhip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+// Error carrying the C-ABI status class (OLHIP_ERR_*).
+struct OlError {
+  int code;
+  std::string message;
+};
+
+#define OL_HIP_CHECK(expr)                                                        \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess)                                                         \
+      throw OlError{-4, std::string(#expr) + ": " + hipGetErrorString(_e)};       \
+  } while (0)
+
+// Device buffer owned by RAII.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  // Grows (never shrinks) to at least `n` bytes; contents are not preserved.
+  void reserve(size_t n) {
+    if (n <= bytes && ptr) return;
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      throw OlError{-3, "hipMalloc of " + std::to_string(n) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    bytes = n;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---- BVH ------------------------------------------------------------------------
+// Input: world-space vertices and indexed triangles (original order).
+struct BvhInput {
+  const float4* positions;  // device, [V] (w ignored)
+  const uint4* tri_vidx;    // device, [T] (i0, i1, i2, material)
+  uint32_t tri_count;
+  uint32_t leaf_size;       // 1..16
+};
+
+// Output of a build.  All arrays are device-resident and owned by the caller.
+struct BvhOutput {
+  DevBuf nodes;      // WideNode[node_count]
+  DevBuf tri;        // float4[3 * T]: (v0, orig id bits), e1, e2 in leaf order
+  uint32_t node_count = 0;
+  uint32_t depth = 0;      // wide levels (root = level 1)
+  double build_ms = 0.0;
+};
+
+// Optional debug capture of intermediate build arrays (bit-exact checks).
+struct BvhDebug {
+  std::vector<uint32_t> morton;       // [T] per original triangle
+  std::vector<uint32_t> sorted_index; // [T] original index of sorted position
+  std::vector<uint32_t> left, right;  // [T-1] binary children (leaf refs | 0x80000000)
+  std::vector<float> node_box;        // [T-1][8] binary internal boxes (min.xyzw, max.xyzw)
+};
+
+void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t stream, BvhDebug* debug);
+
+// ---- scans ---------------------------------------------------------------------
+// Exclusive prefix sum of `n` uint32 values in place; writes the total to
+// `total` (device pointer, may be null).  Scratch is allocated internally.
+void scan_exclusive_u32(uint32_t* data, uint32_t n, uint32_t* total, hipStream_t stream);
+
+// ---- device-visible scene view ---------------------------------------------------
+struct DScene {
+  const WideNode* nodes;
+  const float4* tri;        // leaf-order traversal triangles
+  const uint4* tri_vidx;    // original order (i0, i1, i2, material)
+  const float4* normals;    // [V]
+  const float4* materials;  // [M * 12]
+  const float4* analytic;   // [P * 4]
+  const float4* area;       // [L * 5]
+  uint32_t tri_count, node_count, material_count, analytic_count, area_count;
+  float area_weight;
+  uint32_t env_valid;       // environment light present (analytic type 3)
+  float env_r, env_g, env_b;  // color * intensity
+};

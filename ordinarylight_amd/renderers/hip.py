@@ -1,0 +1,506 @@
+"""``renderers.hip``: MI355X implementations of ``RendererProtocol``.
+
+* :class:`HipPathTracer` -- the GI estimator of the reference's Vulkan
+  wavefront (``VulkanGlobalIlluminationRenderer``,
+  ``ordinarylight/renderers/gi/vulkan.py:11-20`` ->
+  ``targets/vulkan/api.py:1233-1451``): same ``(H, W, 4)`` float32 linear
+  HDR (alpha 1), same per-pixel hash sampling keyed on
+  ``(pixel, frame_index, sample)``, same settings and limits.
+* :class:`HipReferenceRenderer` -- the diffuse estimator of
+  ``CpuReferenceRenderer`` (``renderers/reference/cpu.py:24-74``) with its
+  sequential PCG64 stream seeded ``seed + frame_index`` (alpha 0).
+
+Both drive ``libolhip.so`` through :mod:`ordinarylight_amd._native`; there
+is no other compute path.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, fields, replace
+import math
+import threading
+import time
+
+import numpy as np
+
+from .. import _native as N
+from ..cameras import camera_basis, camera_projection
+from ..packing import pack_scene
+from .base import RendererImplementation, RendererImplementationInfo
+
+
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class HipRendererConfig:
+    """Settings of the GI estimator (subset of ``RendererConfig``,
+    ``targets/vulkan/api.py:26-184``, same names and defaults)."""
+
+    max_bounces: int = 5
+    samples_per_pixel: int = 1
+    area_light_samples: int = 1
+    wavefront_secondary_area_light_samples: int = 0
+    wavefront_environment_samples: int = 1
+    wavefront_secondary_nee_probability: float = 1.0
+    wavefront_unified_secondary_nee: bool = True
+    wavefront_russian_roulette: bool = False
+    wavefront_russian_roulette_start: int = 3
+    wavefront_russian_roulette_min_survival: float = 0.1
+    # Resident paths per device chunk (the reference's per-tile capacity is
+    # 131,072; one MI355X holds whole frames, so the default is 2^25).
+    wavefront_tile_capacity: int = 1 << 25
+    # The reference accepts 1..64 samples per call (api.py:1270-1272).
+    max_samples_per_call: int = 64
+    device_index: int = 0
+    bvh_leaf_size: int = 4
+    traversal_stack_depth: int = 0
+    profiling: bool = False
+    work_counters: bool = False
+
+    def __post_init__(self):
+        if not 1 <= int(self.max_bounces) <= 64:
+            raise ValueError("max_bounces must be between 1 and 64")
+        if not 1 <= int(self.samples_per_pixel) <= int(self.max_samples_per_call):
+            raise ValueError("samples_per_pixel must be between 1 and max_samples_per_call")
+        if int(self.area_light_samples) < 1 or int(self.wavefront_secondary_area_light_samples) < 0:
+            raise ValueError("area light sample counts must be positive")
+        if not 0 <= int(self.wavefront_environment_samples) <= 4:
+            raise ValueError("wavefront_environment_samples must be between 0 and 4")
+        if not 0.0 < float(self.wavefront_secondary_nee_probability) <= 1.0:
+            raise ValueError("wavefront_secondary_nee_probability must be in (0, 1]")
+        if not 0.0 <= float(self.wavefront_russian_roulette_min_survival) <= 1.0:
+            raise ValueError("wavefront_russian_roulette_min_survival must be in [0, 1]")
+        if int(self.wavefront_tile_capacity) < 1:
+            raise ValueError("wavefront_tile_capacity must be positive")
+        if not 0 <= int(self.bvh_leaf_size) <= 16:
+            raise ValueError("bvh_leaf_size must be between 0 and 16")
+
+    def gi_settings(self):
+        s = N.GiSettings()
+        s.max_bounces = int(self.max_bounces)
+        s.secondary_area_samples = int(self.wavefront_secondary_area_light_samples or self.area_light_samples)
+        s.environment_samples = int(self.wavefront_environment_samples)
+        s.russian_roulette_start = int(self.wavefront_russian_roulette_start) if self.wavefront_russian_roulette else 0
+        s.russian_roulette_min_survival = float(self.wavefront_russian_roulette_min_survival)
+        s.secondary_nee_probability = float(self.wavefront_secondary_nee_probability)
+        s.unified_secondary_nee = int(bool(self.wavefront_unified_secondary_nee))
+        s.path_capacity = int(min(self.wavefront_tile_capacity, 0xFFFFFFFF))
+        return s
+
+
+# Settings that can change between frames without rebuilding device state
+# (the reference's _HOT_SETTINGS, targets/vulkan/api.py:989-1043).
+HOT_SETTINGS = frozenset({
+    "max_bounces", "samples_per_pixel", "area_light_samples",
+    "wavefront_secondary_area_light_samples", "wavefront_environment_samples",
+    "wavefront_secondary_nee_probability", "wavefront_unified_secondary_nee",
+    "wavefront_russian_roulette", "wavefront_russian_roulette_start",
+    "wavefront_russian_roulette_min_survival", "wavefront_tile_capacity",
+    "max_samples_per_call", "traversal_stack_depth", "profiling", "work_counters",
+})
+
+
+class DeviceContext:
+    """One ``olhip_ctx``: HIP stream and frame buffers on one device."""
+
+    def __init__(self, device_index=0):
+        self._lib = N.library()
+        handle = N.ctypes.c_void_p()
+        N.check(self._lib.olhip_create(int(device_index), N.ctypes.byref(handle)))
+        self.handle = handle
+        self.device_index = int(device_index)
+
+    def close(self):
+        if self.handle:
+            N.check(self._lib.olhip_destroy(self.handle))
+            self.handle = None
+
+
+class DeviceScene:
+    """A scene resident in HBM: packed arrays + the GPU-built BVH."""
+
+    def __init__(self, context, scene, leaf_size=4):
+        self._lib = N.library()
+        self.context = context
+        started = time.perf_counter()
+        packed = pack_scene(scene)
+        self.packed = packed
+        pack_ms = (time.perf_counter() - started) * 1000.0
+        desc = N.SceneDesc()
+        desc.vertex_count = len(packed.positions)
+        desc.positions = N.ptr(packed.positions)
+        desc.normals = N.ptr(packed.normals)
+        desc.triangle_count = len(packed.indices)
+        desc.indices = N.ptr(packed.indices, N.ctypes.c_uint32)
+        desc.material_ids = N.ptr(packed.material_ids, N.ctypes.c_uint32)
+        desc.material_count = len(packed.materials)
+        desc.materials = N.ptr(packed.materials)
+        desc.analytic_light_count = packed.analytic_light_count
+        desc.analytic_lights = N.ptr(packed.analytic_lights)
+        desc.area_light_count = packed.area_light_count
+        desc.area_lights = N.ptr(packed.area_lights)
+        desc.area_light_weight = packed.area_light_weight
+        desc.leaf_size = int(leaf_size)
+        handle = N.ctypes.c_void_p()
+        started = time.perf_counter()
+        N.check(self._lib.olhip_scene_create(context.handle, N.ctypes.byref(desc), N.ctypes.byref(handle)))
+        self.handle = handle
+        upload_ms = (time.perf_counter() - started) * 1000.0
+        info = N.SceneInfo()
+        N.check(self._lib.olhip_scene_info_get(handle, N.ctypes.byref(info)))
+        self.info = info
+        self.timings = {"scene_pack_ms": pack_ms, "scene_upload_ms": upload_ms, "bvh_ms": info.build_ms}
+
+    @property
+    def triangle_count(self):
+        return int(self.info.triangle_count)
+
+    def bvh_arrays(self):
+        """(nodes uint8 [N, 64], triangles float32 [T, 3, 4]) copied from the device."""
+        nodes = np.zeros((self.info.node_count, 64), np.uint8)
+        tri = np.zeros((self.info.triangle_count, 3, 4), np.float32)
+        N.check(self._lib.olhip_scene_bvh(self.handle, nodes.ctypes.data_as(N.ctypes.c_void_p),
+                                          tri.ctypes.data_as(N.ctypes.c_void_p)))
+        return nodes, tri
+
+    def trace(self, origins, directions, tmin=0.0, tmax=1e30):
+        """Closest hits: returns (t float32, u, v, triangle uint32; 0xFFFFFFFF = miss)."""
+        n = len(origins)
+        o = np.empty((n, 4), np.float32)
+        d = np.empty((n, 4), np.float32)
+        o[:, :3] = origins
+        o[:, 3] = tmin
+        d[:, :3] = directions
+        d[:, 3] = tmax
+        hits = np.empty((n, 4), np.float32)
+        N.check(self._lib.olhip_trace(self.context.handle, self.handle, N.ptr(o), N.ptr(d), n, N.ptr(hits)))
+        return hits[:, 0], hits[:, 1], hits[:, 2], hits[:, 3].view(np.uint32)
+
+    def close(self):
+        if self.handle:
+            N.check(self._lib.olhip_scene_destroy(self.handle))
+            self.handle = None
+
+
+class _ResidentMixin:
+    """Scene residency keyed by identity and revision (``core.py:4574-4634``)."""
+
+    def _init_resident(self, device_index, leaf_size):
+        self._context = None
+        self._device_index = int(device_index)
+        self._leaf_size = int(leaf_size)
+        self._scene_key = None
+        self._device_scene = None
+        self._lock = threading.RLock()
+        self._closed = False
+
+    def _ensure_context(self):
+        if self._context is None:
+            self._context = DeviceContext(self._device_index)
+        return self._context
+
+    def _resident(self, scene):
+        key = (id(scene), getattr(scene, "revision", None))
+        if self._device_scene is None or self._scene_key != key:
+            previous = self._device_scene
+            self._device_scene = DeviceScene(self._ensure_context(), scene, self._leaf_size)
+            self._scene_key = key
+            self._scene_ref = scene
+            if previous is not None:
+                previous.close()
+        return self._device_scene
+
+    def replace_scene(self, scene):
+        """Make ``scene`` resident; the previous one stays if the upload fails."""
+        with self._lock:
+            if self._closed:
+                raise RuntimeError("renderer is closed")
+            self._resident(scene)
+
+    def close(self):
+        with self._lock:
+            if self._closed:
+                return
+            self._closed = True
+            if self._device_scene is not None:
+                self._device_scene.close()
+                self._device_scene = None
+            if self._context is not None:
+                self._context.close()
+                self._context = None
+
+
+def _check_size(width, height):
+    width, height = int(width), int(height)
+    if width < 1 or height < 1:
+        raise ValueError("width and height must be positive")
+    return width, height
+
+
+class HipPathTracer(_ResidentMixin, RendererImplementation):
+    """GI wavefront path tracer on one MI355X (drop-in for the Vulkan GI path)."""
+
+    available_outputs = ("color",)
+    implementation = RendererImplementationInfo(
+        name="hip-wavefront-gi", family="global_illumination", graphics_api="hip",
+    )
+
+    def __init__(self, config=None, **options):
+        if config is not None and options:
+            raise TypeError("pass config or renderer options, not both")
+        self.config = config or HipRendererConfig(**options)
+        if not isinstance(self.config, HipRendererConfig):
+            raise TypeError("config must be a HipRendererConfig")
+        self._init_resident(self.config.device_index, self.config.bvh_leaf_size)
+        self.device = f"HIP device {self.config.device_index}"
+        self.last_timings = {}
+        self.last_stats = None
+
+    @property
+    def capabilities(self):
+        return {
+            "renderer": self.implementation.name,
+            "outputs": self.available_outputs,
+            "features": frozenset({"offscreen_rendering", "global_illumination", "instancing",
+                                   "area_lights", "transmission", "resident_scene"}),
+            "limits": {"max_bounces": 64, "max_samples_per_pixel": int(self.config.max_samples_per_call)},
+            "device": self.device,
+        }
+
+    def reconfigure(self, **changes):
+        """Apply hot settings in place; structural changes raise RuntimeError."""
+        unknown = set(changes) - {f.name for f in fields(HipRendererConfig)}
+        if unknown:
+            raise ValueError(f"unknown settings: {sorted(unknown)}")
+        structural = set(changes) - HOT_SETTINGS
+        if structural:
+            raise RuntimeError(f"settings {sorted(structural)} require renderer recreation")
+        with self._lock:
+            self.config = replace(self.config, **changes)
+            return self.config
+
+    def _render(self, scene, camera, width, height, samples, frame_index, keep_on_device=False):
+        if self._closed:
+            raise RuntimeError("renderer is closed")
+        width, height = _check_size(width, height)
+        samples = self.config.samples_per_pixel if samples is None else int(samples)
+        if not 1 <= samples <= int(self.config.max_samples_per_call):
+            raise ValueError(f"samples must be between 1 and {self.config.max_samples_per_call}")
+        frame_index = int(frame_index)
+        if frame_index < 0:
+            raise ValueError("frame_index cannot be negative")
+        environment = getattr(scene, "environment", None)
+        if environment is not None and getattr(environment, "image", None) is not None:
+            raise RuntimeError("image environments are not supported by the HIP GI renderer")
+        started = time.perf_counter()
+        device_scene = self._resident(scene)
+        if device_scene.packed.has_textures:
+            raise RuntimeError("textured materials are not supported by the HIP GI renderer")
+        context = self._ensure_context()
+        lib = N.library()
+        N.check(lib.olhip_set_profiling(context.handle, int(bool(self.config.profiling))))
+        N.check(lib.olhip_set_stack_depth(context.handle, int(self.config.traversal_stack_depth)))
+        N.check(lib.olhip_set_work_counters(context.handle, int(bool(self.config.work_counters))))
+        position, forward, right, up = camera_basis(camera)
+        cam = N.Camera()
+        cam.position[:] = position.tolist()
+        cam.forward[:] = forward.tolist()
+        cam.right[:] = right.tolist()
+        cam.up[:] = up.tolist()
+        cam.projection = camera_projection(camera)
+        out = None if keep_on_device else np.empty((height, width, 4), np.float32)
+        stats = N.Stats()
+        settings = self.config.gi_settings()
+        N.check(lib.olhip_render_gi(context.handle, device_scene.handle, N.ctypes.byref(cam), width, height,
+                                    samples, frame_index, N.ctypes.byref(settings),
+                                    N.ptr(out) if out is not None else None, N.ctypes.byref(stats)))
+        total_ms = (time.perf_counter() - started) * 1000.0
+        rays = stats.primary_rays + stats.extension_rays + stats.shadow_rays
+        self.last_stats = stats
+        self.last_timings = {
+            "total_ms": total_ms, "gpu_ms": stats.total_ms,
+            "trace_ms": stats.trace_ms, "shade_ms": stats.shade_ms, "shadow_ms": stats.shadow_ms,
+            "primary_rays": int(stats.primary_rays), "extension_rays": int(stats.extension_rays),
+            "shadow_rays": int(stats.shadow_rays),
+            "mrays_per_second": rays / (stats.total_ms * 1e3) if stats.total_ms > 0 else 0.0,
+            **device_scene.timings,
+        }
+        return out
+
+    def render_frame(self, scene, camera, width, height, *, samples=None, frame_index=0):
+        with self._lock:
+            return self._render(scene, camera, width, height, samples, frame_index)
+
+    render_wavefront = render_frame
+
+    def render_products(self, scene, camera, width, height, *, outputs=("color",), samples=None, frame_index=0):
+        outputs = tuple(outputs)
+        unsupported = set(outputs) - set(self.available_outputs)
+        if not outputs or unsupported:
+            raise ValueError(f"unsupported render outputs: {tuple(sorted(unsupported))}")
+        with self._lock:
+            return {"color": self._render(scene, camera, width, height, samples, frame_index)}
+
+    def close(self):
+        _ResidentMixin.close(self)
+
+
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class HipReferenceConfig:
+    """``ReferenceConfig`` (``renderers/reference/cpu.py:9-21``) + device."""
+
+    samples_per_pixel: int = 1
+    max_bounces: int = 3
+    seed: int = 1
+    device_index: int = 0
+    bvh_leaf_size: int = 4
+    # Index-order triangle scan below this triangle count (exact tie order).
+    brute_force_below: int = 257
+
+    def __post_init__(self):
+        if not 1 <= int(self.samples_per_pixel) <= 4096:
+            raise ValueError("samples_per_pixel must be between 1 and 4096")
+        if not 1 <= int(self.max_bounces) <= 64:
+            raise ValueError("max_bounces must be between 1 and 64")
+        if int(self.seed) < 0:
+            raise ValueError("seed cannot be negative")
+
+
+def reference_camera(camera, width, height):
+    """float32 basis and float64 projection scalars as ``_camera_rays`` computes them."""
+    position = np.asarray(camera.position, dtype=np.float32)
+    forward = np.asarray(camera.target, dtype=np.float32) - position
+    forward /= np.linalg.norm(forward)
+    right = np.cross(forward, np.asarray(camera.up, dtype=np.float32))
+    right /= np.linalg.norm(right)
+    up = np.cross(right, forward)
+    aspect = width / height
+    projection = camera_projection(camera)
+    if projection == 2:
+        k_x = math.radians(camera.horizontal_fov_degrees)
+        k_y = math.radians(camera.vertical_fov_degrees)
+    elif projection == 1:
+        half = camera.vertical_size * 0.5
+        k_x, k_y = aspect * half, half
+    else:
+        half = math.tan(math.radians(camera.vertical_fov_degrees) / 2.0)
+        k_x, k_y = aspect * half, half
+    cam = N.E0Camera()
+    cam.position[:] = position.tolist()
+    cam.forward[:] = forward.tolist()
+    cam.right[:] = right.tolist()
+    cam.up[:] = up.tolist()
+    cam.k_x, cam.k_y = float(k_x), float(k_y)
+    cam.projection = projection
+    return cam
+
+
+class HipReferenceRenderer(_ResidentMixin, RendererImplementation):
+    """The reference's diffuse CPU estimator, run on the GPU.
+
+    The PCG64 stream is drawn on the host in the reference's order; rays,
+    intersection, sky, compaction and shading run in ``libolhip.so``.
+    """
+
+    available_outputs = ("color",)
+    implementation = RendererImplementationInfo(
+        name="hip-reference", family="reference", graphics_api="hip",
+    )
+
+    def __init__(self, config=None, **options):
+        if config is not None and options:
+            raise TypeError("pass config or reference options, not both")
+        self.config = config or HipReferenceConfig(**options)
+        if not isinstance(self.config, HipReferenceConfig):
+            raise TypeError("config must be a HipReferenceConfig")
+        self._init_resident(self.config.device_index, self.config.bvh_leaf_size)
+        self.device = f"HIP device {self.config.device_index}"
+        self.last_timings = {}
+        self.last_primary_hits = None
+
+    @property
+    def capabilities(self):
+        return {
+            "renderer": self.implementation.name,
+            "outputs": self.available_outputs,
+            "features": frozenset({"offscreen_rendering", "instancing"}),
+            "limits": {"max_bounces": 64, "max_samples_per_pixel": 4096},
+            "device": self.device,
+        }
+
+    def render_frame(self, scene, camera, width, height, *, samples=None, frame_index=0,
+                     capture_primary_hits=False):
+        with self._lock:
+            if self._closed:
+                raise RuntimeError("reference renderer is closed")
+            samples = self.config.samples_per_pixel if samples is None else int(samples)
+            width, height = int(width), int(height)
+            if width <= 0 or height <= 0:
+                raise ValueError("width and height must be positive")
+            if samples <= 0:
+                raise ValueError("samples and max_bounces must be positive")
+            if getattr(scene, "visible_volumes", ()):
+                raise RuntimeError("volumes are not supported by the HIP reference renderer")
+            started = time.perf_counter()
+            device_scene = self._resident(scene)
+            context = self._ensure_context()
+            lib = N.library()
+            cam = reference_camera(camera, width, height)
+            env = N.E0Env()
+            environment = getattr(scene, "environment", None)
+            image = None
+            if environment is None:
+                env.mode = 0
+            else:
+                env.color[:] = np.asarray(environment.color, np.float32).tolist()
+                env.intensity = float(np.float32(float(environment.intensity)))
+                env.rotation = float(environment.rotation)
+                if environment.image is None:
+                    env.mode = 1
+                else:
+                    env.mode = 2
+                    image = np.ascontiguousarray(environment.image[..., :3], np.float32)
+                    env.image = N.ptr(image)
+                    env.height, env.width = image.shape[:2]
+            brute = int(device_scene.triangle_count < int(self.config.brute_force_below))
+            N.check(lib.olhip_e0_begin(context.handle, device_scene.handle, N.ctypes.byref(cam), width, height,
+                                       N.ctypes.byref(env), brute))
+            rng = np.random.default_rng(int(self.config.seed) + int(frame_index))
+            n_hit = N.ctypes.c_uint32(0)
+            n_active = N.ctypes.c_uint32(0)
+            primary = None
+            for sample in range(samples):
+                jx = rng.random((height, width), dtype=np.float32)
+                jy = rng.random((height, width), dtype=np.float32)
+                N.check(lib.olhip_e0_sample(context.handle, N.ptr(jx), N.ptr(jy)))
+                for bounce in range(int(self.config.max_bounces)):
+                    N.check(lib.olhip_e0_trace(context.handle, N.ctypes.byref(n_hit), N.ctypes.byref(n_active)))
+                    if n_active.value == 0:
+                        break
+                    if capture_primary_hits and sample == 0 and bounce == 0:
+                        primary = np.empty(n_active.value, np.int32)
+                        N.check(lib.olhip_e0_last_hits(context.handle, N.ptr(primary, N.ctypes.c_int32),
+                                                       n_active.value))
+                    count = n_hit.value
+                    if count:
+                        deviates = rng.normal(size=(count, 3)).astype(np.float32)
+                        N.check(lib.olhip_e0_shade(context.handle, N.ptr(deviates), count))
+                    else:
+                        N.check(lib.olhip_e0_shade(context.handle, None, 0))
+                N.check(lib.olhip_e0_sample_end(context.handle))
+            out = np.empty((height, width, 4), np.float32)
+            N.check(lib.olhip_e0_end(context.handle, samples, N.ptr(out)))
+            self.last_primary_hits = primary
+            self.last_timings = {"total_ms": (time.perf_counter() - started) * 1000.0, **device_scene.timings}
+            return out
+
+    render_wavefront = render_frame
+
+    def close(self):
+        _ResidentMixin.close(self)
+
+
+__all__ = [
+    "DeviceContext", "DeviceScene", "HOT_SETTINGS", "HipPathTracer", "HipReferenceConfig",
+    "HipReferenceRenderer", "HipRendererConfig", "reference_camera",
+]

@@ -1,0 +1,176 @@
+"""Seeded procedural scenes for the renderer's benchmark and test workloads.
+
+The reference ships no room, terrain or icosphere generators; these are the
+workload definitions of this package, all deterministic in their seed:
+
+* :func:`box_room` -- closed box room of 36 triangles: 5 walls (10), two
+  boxes with a seeded rotation about +Y (24), one emissive ceiling quad (2)
+  of radiance 15.  Albedos are uniform in [0.1, 0.9]^3.  Pinhole camera.
+* :func:`terrain` -- fBm-displaced heightfield of ``(nx-1)*(nz-1)*2``
+  triangles (1001 x 501 vertices give exactly 1,000,000), split into chunk
+  meshes that each carry one GGX material (roughness ~ U[0.05, 1],
+  metallic ~ Bernoulli(0.3)), plus 64 emissive triangles and a constant
+  environment of radiance 0.5.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .cameras import PerspectiveCamera
+from .lights import EnvironmentLight
+from .scene import Material, Scene, Transform
+
+
+def _quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); normal = (p1-p0) x (p2-p0)."""
+    vertices = np.asarray((p0, p1, p2, p3), np.float32)
+    indices = np.asarray(((0, 1, 2), (0, 2, 3)), np.uint32)
+    return vertices, indices
+
+
+def _box(size, albedo_rng):
+    """Axis-aligned box centred on the origin's footprint, 12 outward triangles."""
+    sx, sy, sz = (float(v) * 0.5 for v in size)
+    corners = np.asarray([
+        (-sx, 0.0, -sz), (sx, 0.0, -sz), (sx, 0.0, sz), (-sx, 0.0, sz),
+        (-sx, 2 * sy, -sz), (sx, 2 * sy, -sz), (sx, 2 * sy, sz), (-sx, 2 * sy, sz),
+    ], np.float32)
+    faces = (
+        (0, 1, 2, 3),  # bottom (normal -y)
+        (4, 7, 6, 5),  # top (+y)
+        (0, 4, 5, 1),  # back (-z)
+        (3, 2, 6, 7),  # front (+z)
+        (0, 3, 7, 4),  # left (-x)
+        (1, 5, 6, 2),  # right (+x)
+    )
+    indices = []
+    for a, b, c, d in faces:
+        indices.extend(((a, b, c), (a, c, d)))
+    return corners, np.asarray(indices, np.uint32)
+
+
+def box_room(seed=0):
+    """Closed box room (36 triangles) and its pinhole camera."""
+    rng = np.random.default_rng(seed)
+    albedo = lambda: tuple(float(v) for v in rng.uniform(0.1, 0.9, 3))  # noqa: E731
+    scene = Scene()
+    # Room spans x in [-1, 1], y in [0, 2], z in [-1, 1]; walls face inward.
+    walls = (
+        ((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)),    # floor (+y)
+        ((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),    # ceiling (-y)
+        ((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)),  # back (+z)
+        ((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)),  # left (+x)
+        ((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)),      # right (-x)
+    )
+    for corners in walls:
+        vertices, indices = _quad(*corners)
+        scene.add_mesh(vertices, indices, Material(base_color=albedo()))
+    for size, position in (((0.55, 1.2, 0.55), (-0.38, 0.0, -0.3)),
+                           ((0.55, 0.6, 0.55), (0.4, 0.0, 0.35))):
+        vertices, indices = _box(size, rng)
+        angle = float(rng.uniform(-0.6, 0.6))
+        transform = Transform.translation(position) @ Transform.rotation((0, 1, 0), angle)
+        scene.add_mesh(vertices, indices, Material(base_color=albedo()), transform=transform)
+    light_v, light_i = _quad((-0.25, 1.99, -0.25), (0.25, 1.99, -0.25),
+                             (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25))
+    scene.add_mesh(light_v, light_i, Material(base_color=(0.8, 0.8, 0.8),
+                                              emission=(15.0, 15.0, 15.0)))
+    camera = PerspectiveCamera(position=(0.0, 1.0, 3.4), target=(0.0, 1.0, 0.0),
+                               vertical_fov_degrees=40.0)
+    return scene, camera
+
+
+def _value_noise(rng, shape_cells):
+    """Lattice of uniform values for bilinear value noise."""
+    return rng.uniform(-1.0, 1.0, size=(shape_cells[0] + 2, shape_cells[1] + 2))
+
+
+def _fbm(x, z, rng, octaves=6):
+    """Fractional Brownian motion of smoothstep-interpolated value noise."""
+    height = np.zeros_like(x, dtype=np.float64)
+    amplitude, frequency = 1.0, 1.0 / 96.0
+    for _ in range(octaves):
+        cells = (int(np.ceil(x.max() * frequency)) + 1, int(np.ceil(z.max() * frequency)) + 1)
+        lattice = _value_noise(rng, cells)
+        fx, fz = x * frequency, z * frequency
+        ix, iz = np.floor(fx).astype(np.int64), np.floor(fz).astype(np.int64)
+        tx, tz = fx - ix, fz - iz
+        tx, tz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+        v00, v10 = lattice[ix, iz], lattice[ix + 1, iz]
+        v01, v11 = lattice[ix, iz + 1], lattice[ix + 1, iz + 1]
+        height += amplitude * ((v00 * (1 - tx) + v10 * tx) * (1 - tz)
+                               + (v01 * (1 - tx) + v11 * tx) * tz)
+        amplitude *= 0.5
+        frequency *= 2.0
+    return height
+
+
+def terrain(seed=0, nx=1001, nz=501, chunk=50, emissive_quads=32, height_scale=60.0):
+    """fBm heightfield terrain with chunked GGX materials, emitters and sky."""
+    rng = np.random.default_rng(seed)
+    xs = np.arange(nx, dtype=np.float64)
+    zs = np.arange(nz, dtype=np.float64)
+    gx, gz = np.meshgrid(xs, zs, indexing="ij")
+    heights = _fbm(gx, gz, rng) * height_scale
+    positions = np.stack((gx, heights, gz), axis=-1).astype(np.float32)  # (nx, nz, 3)
+    # Area-weighted vertex normals over the whole grid (continuous at chunk seams).
+    flat = positions.reshape(-1, 3)
+    vid = np.arange(nx * nz, dtype=np.int64).reshape(nx, nz)
+    a, b = vid[:-1, :-1].ravel(), vid[1:, :-1].ravel()
+    c, d = vid[1:, 1:].ravel(), vid[:-1, 1:].ravel()
+    # (a, d, c) and (a, c, b) wind so the face normal points to +y.
+    tri_all = np.concatenate((np.stack((a, d, c), 1), np.stack((a, c, b), 1)))
+    face = np.cross(flat[tri_all[:, 1]] - flat[tri_all[:, 0]], flat[tri_all[:, 2]] - flat[tri_all[:, 0]])
+    normals = np.zeros_like(flat)
+    for corner in range(3):
+        np.add.at(normals, tri_all[:, corner], face)
+    normals /= np.maximum(np.linalg.norm(normals, axis=1, keepdims=True), 1e-12)
+    normals = normals.reshape(nx, nz, 3)
+
+    scene = Scene()
+    for x0 in range(0, nx - 1, chunk):
+        for z0 in range(0, nz - 1, chunk):
+            x1, z1 = min(x0 + chunk, nx - 1), min(z0 + chunk, nz - 1)
+            cx, cz = x1 - x0 + 1, z1 - z0 + 1
+            verts = positions[x0:x1 + 1, z0:z1 + 1].reshape(-1, 3)
+            norms = normals[x0:x1 + 1, z0:z1 + 1].reshape(-1, 3)
+            lid = np.arange(cx * cz, dtype=np.uint32).reshape(cx, cz)
+            la, lb = lid[:-1, :-1].ravel(), lid[1:, :-1].ravel()
+            lc, ld = lid[1:, 1:].ravel(), lid[:-1, 1:].ravel()
+            quads = np.stack((la, ld, lc, lb), 1)
+            indices = np.empty((len(quads) * 2, 3), np.uint32)
+            indices[0::2] = quads[:, (0, 1, 2)]
+            indices[1::2] = quads[:, (0, 2, 3)]
+            material = Material(
+                base_color=tuple(float(v) for v in rng.uniform(0.1, 0.9, 3)),
+                roughness=float(rng.uniform(0.05, 1.0)),
+                metallic=float(rng.random() < 0.3),
+            )
+            scene.add_mesh(verts, indices, material, normals=norms)
+    # 64 emissive triangles: 32 small quads hovering above the surface, facing down.
+    lv, li = [], []
+    for q in range(emissive_quads):
+        px = float(rng.uniform(0.05, 0.95) * (nx - 1))
+        pz = float(rng.uniform(0.05, 0.95) * (nz - 1))
+        ground = heights[int(px), int(pz)]
+        py = float(ground + rng.uniform(8.0, 20.0))
+        s = float(rng.uniform(2.0, 5.0))
+        base = len(lv)
+        lv.extend(((px - s, py, pz - s), (px + s, py, pz - s), (px + s, py, pz + s), (px - s, py, pz + s)))
+        li.extend(((base, base + 1, base + 2), (base, base + 2, base + 3)))
+    scene.add_mesh(np.asarray(lv, np.float32), np.asarray(li, np.uint32),
+                   Material(base_color=(0.9, 0.9, 0.9), emission=(12.0, 10.5, 8.0)))
+    scene.set_environment(EnvironmentLight(color=(1.0, 1.0, 1.0), intensity=0.5))
+    mid_height = float(np.median(heights))
+    camera = PerspectiveCamera(
+        position=(0.5 * (nx - 1), mid_height + 0.5 * height_scale + 40.0, -0.35 * (nz - 1)),
+        target=(0.5 * (nx - 1), mid_height, 0.55 * (nz - 1)),
+        vertical_fov_degrees=50.0,
+    )
+    return scene, camera
+
+
+__all__ = ["box_room", "terrain"]

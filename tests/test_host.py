@@ -1,0 +1,289 @@
+"""Host-side checks: scene packing KATs, the C ABI's exports, the renderer contract.
+
+* Packing known answers from the reference's own tests
+  (tests/test_renderer.py:149-171 of the reference).
+* Every symbol declared in include/olhip.h is exported by libolhip.so and
+  bound by ordinarylight_amd._native (no GPU call is made).
+* Renderer front end: frame-index reservation, out= reuse, reset_sequence,
+  replace_scene, reconfigure and validation (the reference's
+  tests/test_high_level_renderer.py:317-393), against a fake backend.
+"""
+
+import ctypes
+import pathlib
+import re
+import threading
+
+import numpy as np
+import pytest
+
+import ordinarylight_amd as ol
+from ordinarylight_amd import cameras, packing, workloads
+from ordinarylight_amd.scene import Material, Scene
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+
+
+# -- packing -----------------------------------------------------------------------
+def test_emissive_weights_by_area_and_power():
+    scene = Scene()
+    scene.add_mesh(((0, 0, 0), (1, 0, 0), (0, 2, 0)), ((0, 1, 2),), Material(emission=(1, 1, 1)))
+    scene.add_mesh(((0, 0, 1), (2, 0, 1), (0, 2, 1)), ((0, 1, 2),), Material(emission=(2, 2, 2)))
+    data = scene.emissive_triangle_data()
+    np.testing.assert_allclose(data[:, 4, 1], (0.2, 0.8), rtol=1e-6)
+    np.testing.assert_allclose(data[:, 4, 0], (0.2, 1.0), rtol=1e-6)
+    assert abs(scene.emissive_light_weight - 5.0) < 1e-9
+    assert scene.emissive_triangle_count == 2
+
+
+def test_two_sided_emission_flag_is_packed():
+    scene = Scene()
+    scene.add_mesh(((0, 0, 0), (1, 0, 0), (0, 1, 0)), ((0, 1, 2),),
+                   Material(emission=(1, 1, 1), emission_two_sided=True))
+    assert scene.emissive_triangle_data()[0, 4, 2] == 1.0
+    p = packing.pack_scene(scene)
+    assert p.materials[0, 3, 3] == 1.0
+
+
+def test_no_emitters_packs_placeholder_record():
+    scene, _ = workloads.box_room(0)
+    for mesh in scene.meshes:
+        mesh.material = Material()
+    assert scene.emissive_triangle_count == 0 and scene.emissive_light_weight == 0
+
+
+def test_packing_order_ids_and_materials():
+    scene, _ = workloads.box_room(0)
+    p = packing.pack_scene(scene)
+    tris, colors, emissions = scene.triangles()
+    assert np.array_equal(p.positions[p.indices], tris)
+    assert np.array_equal(p.materials[p.material_ids, 0, :3], colors)
+    assert np.array_equal(p.materials[p.material_ids, 1, :3], emissions)
+    # Occlusion factor resolved for untextured materials.
+    assert np.all(p.materials[:, 5, 3] == 1.0)
+    # IDs follow the packed triangle order.
+    ids = scene.triangle_instance_ids()
+    assert len(ids) == 36 and ids[0] == scene.meshes[0].id
+    assert p.area_light_count == 2 and abs(p.area_light_weight - 0.25 * 15.0) < 1e-6
+
+
+def test_analytic_light_records():
+    scene = Scene()
+    scene.add_point_light((1, 2, 3), (0.5, 0.5, 0.5), 4.0, range=10.0)
+    scene.add_directional_light((0, -2, 0))
+    scene.set_environment(color=(1, 1, 1), intensity=0.5)
+    data = scene.analytic_light_data()
+    assert data.shape == (3, 4, 4) and scene.analytic_light_count == 3
+    assert tuple(data[0, 0]) == (1, 2, 3, 0) and data[0, 1, 3] == 10.0
+    assert tuple(data[1, 1, :3]) == (0, -1, 0) and data[1, 0, 3] == 1
+    assert data[2, 0, 3] == 3 and data[2, 2, 3] == 0.5 and data[2, 3, 0] == -1
+
+
+def test_terrain_workload_shape():
+    scene, cam = workloads.terrain(0, nx=101, nz=51, chunk=25, emissive_quads=32)
+    assert scene.triangle_count == 100 * 50 * 2 + 64
+    assert scene.emissive_triangle_count == 64
+    assert scene.environment.intensity == 0.5
+    roughness = [m.material.roughness for m in scene.meshes[:-1]]
+    assert min(roughness) >= 0.05 and max(roughness) <= 1.0
+
+
+def test_full_terrain_has_one_million_triangles():
+    scene, _ = workloads.terrain(0)
+    assert scene.triangle_count - 64 == 1_000_000
+
+
+def test_camera_basis_matches_reference_definition():
+    cam = cameras.PerspectiveCamera((0, 1, 3), (0, 1, 0), vertical_fov_degrees=60)
+    pos, fwd, right, up = cameras.camera_basis(cam)
+    assert fwd.dtype == np.float32
+    np.testing.assert_allclose(fwd, (0, 0, -1))
+    np.testing.assert_allclose(right, (np.tan(np.radians(30)), 0, 0), rtol=1e-6)
+    np.testing.assert_allclose(up, (0, np.tan(np.radians(30)), 0), rtol=1e-6)
+
+
+# -- C ABI ---------------------------------------------------------------------------
+def test_library_exports_every_declared_symbol():
+    from ordinarylight_amd import _native
+    header = (ROOT / "include" / "olhip.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(olhip_\w+)\s*\(", header, re.M))
+    assert declared, "no declarations parsed"
+    assert declared == set(_native.EXPORTED_SYMBOLS)
+    lib = ctypes.CDLL(str(_native.LIBRARY_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.olhip_version() == 100
+
+
+def test_library_reports_errors_without_device():
+    from ordinarylight_amd import _native
+    lib = _native.library()
+    count = ctypes.c_int(-1)
+    assert lib.olhip_device_count(ctypes.byref(count)) == 0
+    if count.value == 0:
+        handle = ctypes.c_void_p()
+        assert lib.olhip_create(0, ctypes.byref(handle)) == _native.OLHIP_ERR_RUNTIME
+        assert b"no HIP device" in lib.olhip_last_error()
+    assert lib.olhip_set_profiling(None, 1) == _native.OLHIP_ERR_VALUE
+
+
+# -- renderer contract ------------------------------------------------------------------
+class FakeBackend:
+    available_outputs = ("color", "depth")
+
+    def __init__(self):
+        self.config = {"samples_per_pixel": 1}
+        self.device = "test-device"
+        self.last_timings = {}
+        self.calls = []
+        self.scene_replacements = []
+        self.setting_changes = []
+        self.closed = 0
+
+    def render_frame(self, scene, camera, width, height, *, samples=None, frame_index=0):
+        self.calls.append((scene, camera, width, height, samples, frame_index))
+        self.last_timings = {"gpu_ms": 2.5}
+        return np.full((height, width, 4), float(frame_index), np.float32)
+
+    def render_products(self, scene, camera, width, height, *, outputs, samples=None, frame_index=0):
+        out = {"color": self.render_frame(scene, camera, width, height, samples=samples, frame_index=frame_index)}
+        if "depth" in outputs:
+            out["depth"] = np.ones((height, width), np.float32)
+        return out
+
+    def replace_scene(self, scene):
+        self.scene_replacements.append(scene)
+
+    def reconfigure(self, **changes):
+        self.setting_changes.append(changes)
+        return changes
+
+    def close(self):
+        self.closed += 1
+
+
+class BlockingBackend(FakeBackend):
+    def __init__(self):
+        super().__init__()
+        self.release = threading.Event()
+        self.started = threading.Event()
+
+    def render_frame(self, *args, **kwargs):
+        self.started.set()
+        self.release.wait(5)
+        return super().render_frame(*args, **kwargs)
+
+
+def fixture():
+    return workloads.box_room(0)
+
+
+def test_renderer_advances_sequence_and_reports_statistics():
+    backend = FakeBackend()
+    r = ol.Renderer(implementation=backend)
+    scene, cam = fixture()
+    first = r.render(scene, cam, (5, 3), samples=2)
+    second = r.render(scene, cam, (5, 3))
+    assert first.shape == (3, 5, 4) and first.dtype == np.float32
+    assert np.all(first == 0.0) and np.all(second == 1.0)
+    assert backend.calls[0][2:] == (5, 3, 2, 0)
+    assert r.frame_index == 2 and r.device == "test-device"
+    assert r.last_timings == {"gpu_ms": 2.5}
+    assert r.last_statistics.as_dict() == {"frame_index": 1, "width": 5, "height": 3, "samples": None,
+                                           "total_ms": None, "gpu_ms": 2.5}
+    r.close()
+    assert backend.closed == 1
+    r.close()
+    assert backend.closed == 1
+
+
+def test_explicit_frame_and_caller_owned_output():
+    r = ol.Renderer(implementation=FakeBackend())
+    scene, cam = fixture()
+    out = np.empty((2, 4, 4), np.float32)
+    result = r.render(scene, cam, (4, 2), frame_index=7, out=out)
+    assert result is out and np.all(out == 7.0)
+    assert r.frame_index == 8
+    r.reset_sequence()
+    assert r.frame_index == 0
+    with pytest.raises(ValueError):
+        r.render(scene, cam, (4, 2), out=np.empty((3, 4, 4), np.float32))
+    with pytest.raises(TypeError):
+        r.render(scene, cam, (4, 2), out=np.empty((2, 4, 4), np.float64))
+    r.close()
+
+
+def test_replace_scene_and_reconfigure_are_ordered():
+    backend = FakeBackend()
+    r = ol.Renderer(implementation=backend)
+    a, cam = fixture()
+    b, _ = fixture()
+    r.render(a, cam, (2, 2))
+    assert r.replace_scene(b) is b
+    assert backend.scene_replacements == [b] and r.frame_index == 0 and r.last_statistics is None
+    assert r.reconfigure(samples_per_pixel=2, max_bounces=8) == {"samples_per_pixel": 2, "max_bounces": 8}
+    r.close()
+    with pytest.raises(RuntimeError):
+        r.render(a, cam, (2, 2))
+
+
+def test_frame_indices_are_reserved_at_submission_and_cancel_leaves_gaps():
+    backend = BlockingBackend()
+    r = ol.Renderer(implementation=backend)
+    scene, cam = fixture()
+    first = r.render_async(scene, cam, (2, 2))
+    backend.started.wait(5)
+    second = r.render_async(scene, cam, (2, 2))
+    third = r.render_async(scene, cam, (2, 2))
+    assert (first.frame_index, second.frame_index, third.frame_index) == (0, 1, 2)
+    assert second.cancel()
+    backend.release.set()
+    assert np.all(third.result() == 2.0)
+    assert second.cancelled() and r.frame_index == 3
+    assert [c[-1] for c in backend.calls] == [0, 2]
+    r.close()
+
+
+def test_named_products_and_validation():
+    r = ol.Renderer(implementation=FakeBackend())
+    scene, cam = fixture()
+    frame = r.render(scene, cam, (3, 2), outputs=("color", "depth"))
+    assert frame.color.shape == (2, 3, 4) and frame["depth"].shape == (2, 3)
+    assert frame.metadata["frame_index"] == 0
+    with pytest.raises(ValueError):
+        r.render(scene, cam, (3, 2), outputs=("normal",))
+    with pytest.raises(ValueError):
+        r.render(scene, cam, (3, 2), outputs=("color", "color"))
+    with pytest.raises(ValueError):
+        r.render(scene, cam, (0, 2))
+    with pytest.raises(TypeError):
+        r.render(scene, object(), (3, 2))
+    with pytest.raises(ValueError):
+        r.render(scene, cam, (3, 2), frame_index=-1)
+    r.close()
+
+
+def test_hip_renderer_configuration_validation():
+    from ordinarylight_amd.renderers.hip import HipPathTracer, HipReferenceConfig, HipRendererConfig
+    with pytest.raises(ValueError):
+        HipRendererConfig(max_bounces=0)
+    with pytest.raises(ValueError):
+        HipRendererConfig(samples_per_pixel=65)
+    with pytest.raises(ValueError):
+        HipRendererConfig(wavefront_environment_samples=5)
+    with pytest.raises(ValueError):
+        HipReferenceConfig(samples_per_pixel=0)
+    with pytest.raises(TypeError):
+        HipPathTracer(config=HipRendererConfig(), max_bounces=3)
+    r = HipPathTracer(max_bounces=3)  # no device touched until a render
+    assert r.capabilities["renderer"] == "hip-wavefront-gi"
+    assert r.implementation.graphics_api == "hip"
+    r.reconfigure(max_bounces=6)
+    assert r.config.max_bounces == 6
+    with pytest.raises(RuntimeError):
+        r.reconfigure(device_index=1)
+    r.close()
+    with pytest.raises(RuntimeError):
+        r.render_frame(None, None, 4, 4)
+    from ordinarylight_amd.renderers import RendererProtocol
+    assert isinstance(r, RendererProtocol)
