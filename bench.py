@@ -92,6 +92,8 @@ def main(argv=None):
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-profile", action="store_true")
     parser.add_argument("--leaf-size", type=int, default=4)
+    parser.add_argument("--lds-stack", type=int, default=16)
+    parser.add_argument("--block", type=int, default=128)
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -106,7 +108,8 @@ def main(argv=None):
     log(f"scene: {scene.triangle_count} triangles, generated in {time.perf_counter() - t0:.1f} s")
     W, H, spp, B = args.width, args.height, args.spp, args.bounces
     renderer = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
-                             bvh_leaf_size=args.leaf_size)
+                             bvh_leaf_size=args.leaf_size, traversal_lds_stack=args.lds_stack,
+                             traversal_block=args.block)
     # Scene upload + BVH build (reported separately, outside the timed region).
     renderer.replace_scene(scene)
     ds = renderer._device_scene
@@ -122,7 +125,7 @@ def main(argv=None):
         renderer._render(scene, camera, W, H, spp, 0, keep_on_device=True)
     rays = 0
     dev_ms = 0.0
-    trace_ms = 0.0
+    trace_ms = shade_ms = shadow_ms = 0.0
     trace_launches = 0
     t_start = time.perf_counter()
     for step in range(args.steps):
@@ -131,10 +134,19 @@ def main(argv=None):
         rays += st.primary_rays + st.extension_rays + st.shadow_rays
         dev_ms += st.total_ms
         trace_ms += st.trace_ms
+        shade_ms += st.shade_ms
+        shadow_ms += st.shadow_ms
         trace_launches += st.trace_launches
     wall_s = time.perf_counter() - t_start
     ms_per_step = wall_s * 1000.0 / args.steps
     value = rays / (wall_s * 1e6)
+
+    # Achievable streaming bandwidth on this device (1 GiB read + 1 GiB write).
+    import ctypes
+    from ordinarylight_amd import _native
+    ms = ctypes.c_double(0)
+    _native.check(_native.library().olhip_bandwidth_probe(renderer._context.handle, 1 << 30, 10, ctypes.byref(ms)))
+    copy_gbs = 2 * (1 << 30) / (ms.value * 1e-3) / 1e9
 
     # PCIe-inclusive: one frame with the image copied back to the host.
     t1 = time.perf_counter()
@@ -173,6 +185,9 @@ def main(argv=None):
             "rays_per_frame": int(rays / args.steps),
             "device_ms_per_frame": round(dev_ms / args.steps, 3),
             "trace_ms_per_frame": round(trace_ms / args.steps, 3),
+            "shade_ms_per_frame": round(shade_ms / args.steps, 3),
+            "shadow_ms_per_frame": round(shadow_ms / args.steps, 3),
+            "copy_probe_gbs": round(copy_gbs, 1),
             "pcie_inclusive_frame_ms": round(pcie_ms, 3),
             "bvh_build_ms": round(ds.info.build_ms, 2),
             "bvh_nodes": int(ds.info.node_count),

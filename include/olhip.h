@@ -138,14 +138,20 @@ typedef struct olhip_stats {
 int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* scene, const olhip_camera* camera, uint32_t width,
                     uint32_t height, uint32_t samples, uint32_t frame_index, const olhip_gi_settings* settings,
                     float* out_rgba, olhip_stats* stats);
+/* Streaming copy of `bytes` (read + write) repeated `repeats` times on the
+ * context's stream; reports milliseconds per copy.  Used to measure
+ * achievable HBM bandwidth and to calibrate hardware byte counters. */
+int olhip_bandwidth_probe(olhip_ctx* ctx, uint64_t bytes, uint32_t repeats, double* ms_per_copy);
 /* Copy the last frame of this context to host memory. */
 int olhip_frame_download(olhip_ctx* ctx, float* out_rgba);
 /* Enable per-kernel event timing inside olhip_render_gi (0 or 1). */
 int olhip_set_profiling(olhip_ctx* ctx, int enabled);
 /* Count traversal work (nodes visited, triangles tested) in olhip_stats. */
 int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
-/* Override the traversal stack depth (0 = from the BVH depth). */
-int olhip_set_stack_depth(olhip_ctx* ctx, int depth);
+/* Traversal launch shape: stack entries kept in LDS per lane (the rest of
+ * the 3 * depth + 1 entries spill to a global per-lane column) and the block
+ * size.  Supported: (8|12|24|32|48, 128) and (16, 64|128|256). */
+int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block);
 
 /* Closest hit of caller rays: origins/directions [n][4] (xyz, tmin / tmax).
  * hits [n][4]: t, u, v, original triangle index as uint32 bits (0xFFFFFFFF

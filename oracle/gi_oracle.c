@@ -324,7 +324,7 @@ static int box_hit(const BNode* n, const double o[3], const double inv[3], doubl
     if (a > t0) t0 = a;
     if (b < t1) t1 = b;
   }
-  return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
+  return t0 <= t1 * (1.0 + 1e-9) + 1e-9;
 }
 
 /* Closest hit in [tmin, tmax]: lowest t, ties to the lowest triangle index. */
@@ -340,7 +340,7 @@ static int closest(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv
   stack[sp++] = 0;
   while (sp) {
     const BNode* n = &s->nodes[stack[--sp]];
-    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)*bt * (1.0 + 1e-7) + 1e-9)) continue;
+    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)*bt * (1.0 + 1e-5) + 1e-9)) continue;
     if (n->right & 0x80000000u) {
       uint32_t cnt = n->right & 0x7FFFFFFFu;
       for (uint32_t k = 0; k < cnt; ++k) {
@@ -374,7 +374,7 @@ static int occluded(const Scene* s, V o, V d, R tmin, R tmax) {
   stack[sp++] = 0;
   while (sp) {
     const BNode* n = &s->nodes[stack[--sp]];
-    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)tmax * (1.0 + 1e-7) + 1e-9)) continue;
+    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)tmax * (1.0 + 1e-5) + 1e-9)) continue;
     if (n->right & 0x80000000u) {
       uint32_t cnt = n->right & 0x7FFFFFFFu;
       for (uint32_t k = 0; k < cnt; ++k) {

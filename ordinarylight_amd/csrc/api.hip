@@ -79,9 +79,10 @@ struct olhip_ctx {
   int cu_count = 0;
   int profiling = 0;
   int counting = 0;
-  int stack_override = 0;
+  int lds_stack = 16;
+  int trace_block = 128;
   // GI frame buffers (grown on demand, reused across frames).
-  DevBuf T, L, M, stk, qo0, qo1, qd0, qd1, qp0, qp1, hit, gT, gI, eo, ed, ec, ctr, stats, work, acc, image;
+  DevBuf T, L, M, stk, qo0, qo1, qd0, qd1, qp0, qp1, hit, gT, gI, eo, ed, ec, ctr, stats, work, acc, image, spill;
   uint32_t image_pixels = 0;
   // Reference-estimator frame.
   struct E0Frame {
@@ -151,13 +152,15 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled) {
   });
 }
 
-int olhip_set_stack_depth(olhip_ctx* ctx, int depth) {
+int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block) {
   return guarded([&] {
     require(ctx != nullptr, "ctx must not be NULL");
-    const int allowed[] = {0, 16, 24, 32, 48, 64, 96, 128};
-    require(std::find(std::begin(allowed), std::end(allowed), depth) != std::end(allowed),
-            "stack depth must be one of 0, 16, 24, 32, 48, 64, 96, 128");
-    ctx->stack_override = depth;
+    const int pairs[][2] = {{8, 128}, {12, 128}, {16, 64}, {16, 128}, {16, 256}, {24, 128}, {32, 128}, {48, 128}};
+    bool ok = false;
+    for (auto& p : pairs) ok = ok || (p[0] == lds_entries && p[1] == block);
+    require(ok, "unsupported (LDS stack entries, block size): use (8|12|24|32|48, 128) or (16, 64|128|256)");
+    ctx->lds_stack = lds_entries;
+    ctx->trace_block = block;
   });
 }
 
@@ -278,14 +281,26 @@ int olhip_scene_bvh(const olhip_scene* scene, void* nodes, void* tri) {
 
 namespace {
 
+// Fixed-size stack of the reference-estimator traversal (LDS only).
 int choose_stack(const olhip_ctx* ctx, const olhip_scene* sc) {
+  (void)ctx;
   int need = stack_depth_for(std::max<uint32_t>(sc->bvh.depth, 1));
   if (need < 0) throw OlError{OLHIP_ERR_RUNTIME, "BVH too deep for the traversal stack"};
-  if (ctx->stack_override) {
-    if (ctx->stack_override < need) throw OlError{OLHIP_ERR_VALUE, "stack depth override below the BVH requirement"};
-    return ctx->stack_override;
-  }
   return need;
+}
+
+// Two-level traversal stack: LDS part from the context, spill for the rest.
+GiLaunch make_launch(olhip_ctx* ctx, const olhip_scene* sc) {
+  GiLaunch l{};
+  l.cu_count = ctx->cu_count;
+  l.stack_depth = (int)(3u * std::max<uint32_t>(sc->bvh.depth, 1) + 1u);
+  l.lds_stack = ctx->lds_stack;
+  l.trace_block = ctx->trace_block;
+  uint64_t words = trace_spill_words(l);
+  if (words) ctx->spill.reserve(4 * words);
+  l.spill = words ? ctx->spill.as<uint32_t>() : nullptr;
+  l.spill_words = words;
+  return l;
 }
 
 }  // namespace
@@ -390,7 +405,7 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     P.unified = set->unified_secondary_nee ? 1u : 0u;
     P.two_pass = two_pass ? 1u : 0u;
 
-    GiLaunch l{ctx->cu_count, choose_stack(ctx, sc), 128};
+    GiLaunch l = make_launch(ctx, sc);
     Event e_start, e_end;
     std::vector<std::unique_ptr<Event>> marks;
     double trace_ms = 0, shade_ms = 0, shadow_ms = 0;
@@ -472,6 +487,27 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
   });
 }
 
+int olhip_bandwidth_probe(olhip_ctx* ctx, uint64_t bytes, uint32_t repeats, double* ms_per_copy) {
+  return guarded([&] {
+    require(ctx && ms_per_copy, "arguments must not be NULL");
+    require(bytes >= 16 && bytes <= (1ull << 36) && repeats >= 1, "bytes must be in [16, 2^36], repeats >= 1");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const uint64_t n16 = bytes / 16;
+    DevBuf a, b;
+    a.reserve(16 * n16);
+    b.reserve(16 * n16);
+    OL_HIP_CHECK(hipMemsetAsync(a.ptr, 0, 16 * n16, s));
+    stream_copy(a.as<float4>(), b.as<float4>(), n16, ctx->cu_count, s);
+    Event e0, e1;
+    OL_HIP_CHECK(hipEventRecord(e0.e, s));
+    for (uint32_t r = 0; r < repeats; ++r) stream_copy(a.as<float4>(), b.as<float4>(), n16, ctx->cu_count, s);
+    OL_HIP_CHECK(hipEventRecord(e1.e, s));
+    OL_HIP_CHECK(hipEventSynchronize(e1.e));
+    *ms_per_copy = elapsed(e0, e1) / repeats;
+  });
+}
+
 int olhip_frame_download(olhip_ctx* ctx, float* out_rgba) {
   return guarded([&] {
     require(ctx && out_rgba, "ctx and out must not be NULL");
@@ -496,7 +532,7 @@ int olhip_trace(olhip_ctx* ctx, const olhip_scene* sc, const float* origins, con
     h.reserve(sizeof(float4) * n);
     ov.reserve(4);
     OL_HIP_CHECK(hipMemsetAsync(ov.ptr, 0, 4, s));
-    GiLaunch l{ctx->cu_count, choose_stack(ctx, sc), 128};
+    GiLaunch l = make_launch(ctx, sc);
     trace_rays(l, sc->view, o.as<float4>(), d.as<float4>(), n, h.as<float4>(), ov.as<uint32_t>(), nullptr, s);
     OL_HIP_CHECK(hipGetLastError());
     OL_HIP_CHECK(hipStreamSynchronize(s));

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(BLOCK) k_e0_trace(E0Params P, E0State S, uint3
         load_node(P.sc.nodes, node, w0, w1, w2, w3);
         float tn[4];
         uint32_t ch[4];
-        const float cull = bt < (U)3.0e38 ? (float)bt * 1.0000020265579224f + 1e-6f : INFINITY;
+        const float cull = bt < (U)3.0e38 ? cull_distance((float)bt) + 1e-6f : INFINITY;
         uint32_t mask = ol_node_hits(w0, w1, w2, w3, rp, 0.0f, cull, tn, ch);
         uint32_t next = OL_NONE;
         for (int c = 0; c < 4; ++c) {

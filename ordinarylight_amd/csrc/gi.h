@@ -48,9 +48,15 @@ struct GiParams {
 // Host launchers (gi.hip).
 struct GiLaunch {
   int cu_count;
-  int stack_depth;  // 16, 32, 48, 64, 96, 128, 192, 256
-  int trace_block;  // 64, 128, 256
+  int stack_depth;   // total traversal stack entries needed (3 * BVH depth + 1)
+  int lds_stack;     // entries kept in LDS: 8, 12, 16, 24, 32
+  int trace_block;   // 64, 128, 256
+  uint32_t* spill;   // global spill columns for entries beyond lds_stack
+  uint64_t spill_words;
 };
+
+// Resident grid of the traversal kernels and the spill words it needs.
+uint64_t trace_spill_words(const GiLaunch& l);
 
 void gi_generate(const GiParams& p, const GiState& s, hipStream_t st);
 void gi_bounce_prologue(const GiState& s, uint32_t bounce, hipStream_t st);

@@ -18,6 +18,8 @@
 // grid-stride loops sized to the resident grid, so no host synchronisation
 // happens between bounces.
 
+#include <algorithm>
+
 #include "gi.h"
 #include "trace.h"
 
@@ -450,17 +452,32 @@ __global__ void k_reset(GiState S) {
   if (S.work && threadIdx.x < 4) S.work[threadIdx.x] = 0;
 }
 
-template <int STACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_closest(DScene sc, GiState S, uint32_t in) {
-  __shared__ uint32_t stack[STACK * BLOCK];
+// Lane-private traversal stack: LDS column + global spill column.
+template <int LSTACK, int BLOCK>
+OL_DEV TravStack<LSTACK> lane_stack(uint32_t* lds, uint32_t* spill, uint32_t spill_cap, uint32_t* overflow) {
+  TravStack<LSTACK> st;
+  st.lds = lds + threadIdx.x;
+  st.lds_stride = BLOCK;
+  st.spill = spill ? spill + (blockIdx.x * BLOCK + threadIdx.x) : nullptr;
+  st.spill_stride = gridDim.x * BLOCK;
+  st.spill_cap = spill ? spill_cap : 0u;
+  st.overflow = overflow;
+  st.sp = 0;
+  return st;
+}
+
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_closest(DScene sc, GiState S, uint32_t in, uint32_t* spill,
+                                                         uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[5]);
   const uint32_t count = S.ctr[in];
   const float4* __restrict__ qo = S.qo[in];
   const float4* __restrict__ qd = S.qd[in];
   uint32_t nv = 0, nt = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += gridDim.x * BLOCK) {
     float4 o = qo[i], d = qd[i];
-    TraceHit h = ol_trace_closest<STACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w,
-                                         stack + threadIdx.x, BLOCK, &S.ctr[5], nv, nt);
+    TraceHit h = ol_trace_closest<LSTACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, st, nv, nt);
     S.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.slot));
   }
   if (S.work && (nv | nt)) {
@@ -469,15 +486,16 @@ __global__ void __launch_bounds__(BLOCK) k_trace_closest(DScene sc, GiState S, u
   }
 }
 
-template <int STACK, int BLOCK>
+template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o4, const float4* d4, uint32_t n,
-                                                      float4* hits, uint32_t* overflow, unsigned long long* work) {
-  __shared__ uint32_t stack[STACK * BLOCK];
+                                                      float4* hits, uint32_t* overflow, unsigned long long* work,
+                                                      uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
   uint32_t nv = 0, nt = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     float4 o = o4[i], d = d4[i];
-    TraceHit h = ol_trace_closest<STACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w,
-                                         stack + threadIdx.x, BLOCK, overflow, nv, nt);
+    TraceHit h = ol_trace_closest<LSTACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, st, nv, nt);
     hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.slot == OL_NONE ? OL_NONE : h.orig));
   }
   if (work && (nv | nt)) {
@@ -486,9 +504,10 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o
   }
 }
 
-template <int STACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_shadow(DScene sc, GiState S) {
-  __shared__ uint32_t stack[STACK * BLOCK];
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_shadow(DScene sc, GiState S, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[5]);
   const uint32_t count = S.ctr[2];
   uint32_t nv = 0, nt = 0;
   for (uint32_t g = blockIdx.x * BLOCK + threadIdx.x; g < count; g += gridDim.x * BLOCK) {
@@ -499,9 +518,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_shadow(DScene sc, GiState S) {
     f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
     for (uint32_t k = 0; k < n; ++k) {
       const float4 o = S.eo[first + k], d = S.ed[first + k], c = S.ec[first + k];
-      if (ol_trace_any<STACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, stack + threadIdx.x, BLOCK,
-                              &S.ctr[5], nv, nt))
-        continue;
+      if (ol_trace_any<LSTACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, st, nv, nt)) continue;
       const uint32_t kind = (uint32_t)c.w;
       if (mode == 0u || kind == 0u) direct = direct + rgb(c);
       else if (kind == 1u) area_direct = area_direct + rgb(c);
@@ -796,51 +813,70 @@ __global__ void __launch_bounds__(256) k_final(const double4* acc, uint32_t np, 
   out[pixel] = make_float4((float)(a.x / n), (float)(a.y / n), (float)(a.z / n), (float)(a.w / n));
 }
 
-int g_max_blocks_closest = 0;
+template <class K>
+int resident_blocks(K kernel, int block) {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0) != hipSuccess || occ < 1) occ = 1;
+  return occ;
+}
 
-template <int STACK, int BLOCK>
+uint32_t spill_cap_of(const GiLaunch& l) {
+  return l.stack_depth > l.lds_stack ? (uint32_t)(l.stack_depth - l.lds_stack) : 0u;
+}
+
+void check_spill(const GiLaunch& l, uint64_t threads) {
+  if ((uint64_t)spill_cap_of(l) * threads > l.spill_words)
+    throw OlError{-2, "traversal spill buffer too small for the launch"};
+}
+
+template <int LSTACK, int BLOCK>
+uint64_t spill_words_for(const GiLaunch& l) {
+  int occ = std::max(resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK),
+                     std::max(resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK),
+                              resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK)));
+  return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
+}
+
+template <int LSTACK, int BLOCK>
 void launch_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint32_t in, hipStream_t st) {
-  static int occ = -1;
-  if (occ < 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_trace_closest<STACK, BLOCK>, BLOCK, 0) != hipSuccess || occ < 1) occ = 1;
-  }
+  static int occ = resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
-  hipLaunchKernelGGL((k_trace_closest<STACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, in);
+  check_spill(l, (uint64_t)grid.x * BLOCK);
+  hipLaunchKernelGGL((k_trace_closest<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, in, l.spill, spill_cap_of(l));
 }
 
-template <int STACK, int BLOCK>
+template <int LSTACK, int BLOCK>
 void launch_shadow(const GiLaunch& l, const DScene& sc, const GiState& s, hipStream_t st) {
-  static int occ = -1;
-  if (occ < 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_trace_shadow<STACK, BLOCK>, BLOCK, 0) != hipSuccess || occ < 1) occ = 1;
-  }
+  static int occ = resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
-  hipLaunchKernelGGL((k_trace_shadow<STACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s);
+  check_spill(l, (uint64_t)grid.x * BLOCK);
+  hipLaunchKernelGGL((k_trace_shadow<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, l.spill, spill_cap_of(l));
 }
 
-template <int STACK, int BLOCK>
+template <int LSTACK, int BLOCK>
 void launch_rays(const GiLaunch& l, const DScene& sc, const float4* o, const float4* d, uint32_t n, float4* hits,
                  uint32_t* overflow, unsigned long long* work, hipStream_t st) {
-  static int occ = -1;
-  if (occ < 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_trace_rays<STACK, BLOCK>, BLOCK, 0) != hipSuccess || occ < 1) occ = 1;
-  }
+  static int occ = resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK);
   uint32_t need = (n + BLOCK - 1) / BLOCK;
   uint32_t cap = (uint32_t)(l.cu_count * occ);
   dim3 grid(need < cap ? (need ? need : 1) : cap);
-  hipLaunchKernelGGL((k_trace_rays<STACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, o, d, n, hits, overflow, work);
+  check_spill(l, (uint64_t)grid.x * BLOCK);
+  hipLaunchKernelGGL((k_trace_rays<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, o, d, n, hits, overflow, work,
+                     l.spill, spill_cap_of(l));
 }
 
-#define OL_STACK_DISPATCH(FN, ...)                                      \
-  switch (l.stack_depth) {                                              \
-    case 16: FN<16, 128>(__VA_ARGS__); break;                           \
-    case 24: FN<24, 128>(__VA_ARGS__); break;                           \
-    case 32: FN<32, 128>(__VA_ARGS__); break;                           \
-    case 48: FN<48, 128>(__VA_ARGS__); break;                           \
-    case 64: FN<64, 64>(__VA_ARGS__); break;                            \
-    case 96: FN<96, 64>(__VA_ARGS__); break;                            \
-    case 128: FN<128, 64>(__VA_ARGS__); break;                          \
-    default: throw OlError{-2, "unsupported traversal stack depth"};    \
+// (LDS entries, block size) variants the launch autotuner chooses from.
+#define OL_STACK_DISPATCH(FN, ...)                                                  \
+  switch (l.lds_stack * 1000 + l.trace_block) {                                     \
+    case 8128: return FN<8, 128>(__VA_ARGS__);                                      \
+    case 12128: return FN<12, 128>(__VA_ARGS__);                                    \
+    case 16064: return FN<16, 64>(__VA_ARGS__);                                     \
+    case 16128: return FN<16, 128>(__VA_ARGS__);                                    \
+    case 16256: return FN<16, 256>(__VA_ARGS__);                                    \
+    case 24128: return FN<24, 128>(__VA_ARGS__);                                    \
+    case 32128: return FN<32, 128>(__VA_ARGS__);                                    \
+    case 48128: return FN<48, 128>(__VA_ARGS__);                                    \
+    default: throw OlError{-1, "unsupported traversal stack / block configuration"}; \
   }
 
 }  // namespace
@@ -852,6 +888,8 @@ int stack_depth_for(uint32_t bvh_depth) {
     if ((uint32_t)d >= need) return d;
   return -1;
 }
+
+uint64_t trace_spill_words(const GiLaunch& l) { OL_STACK_DISPATCH(spill_words_for, l); }
 
 void gi_reset(const GiState& s, hipStream_t st) { hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, s); }
 

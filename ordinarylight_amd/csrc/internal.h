@@ -83,6 +83,10 @@ void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t stream, BvhDebug*
 // `total` (device pointer, may be null).  Scratch is allocated internally.
 void scan_exclusive_u32(uint32_t* data, uint32_t n, uint32_t* total, hipStream_t stream);
 
+// Streaming device-to-device copy of n16 float4 records (bandwidth probe and
+// hardware-counter calibration).
+void stream_copy(const float4* src, float4* dst, uint64_t n16, int cu_count, hipStream_t s);
+
 // ---- device-visible scene view ---------------------------------------------------
 struct DScene {
   const WideNode* nodes;

@@ -59,7 +59,16 @@ __global__ void k_scan_total(const uint32_t* data, const uint32_t* last_in, uint
   *total = n ? data[n - 1] + *last_in : 0u;
 }
 
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = src[i];
+}
+
 }  // namespace
+
+void stream_copy(const float4* src, float4* dst, uint64_t n16, int cu_count, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)(cu_count * 8)), dim3(256), 0, s, src, dst, n16);
+}
 
 void scan_exclusive_u32(uint32_t* data, uint32_t n, uint32_t* total, hipStream_t s) {
   if (n == 0) {

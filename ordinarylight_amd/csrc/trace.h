@@ -1,9 +1,12 @@
 // Software ray traversal of the 64-byte 4-wide BVH (closest hit, any hit).
 //
-// One lane per ray (wave64).  The traversal stack lives in LDS, one column
-// per lane (entry k of lane t at stack[k * BLOCK + t], bank-conflict free),
-// with a depth chosen on the host from the BVH's level count so it cannot
-// overflow (a 4-wide level pushes at most three entries).
+// One lane per ray (wave64).  The traversal stack is two-level: the top
+// LSTACK entries live in LDS, one column per lane (entry k of lane t at
+// lds[k * BLOCK + t], bank-conflict free), deeper entries spill to a per-lane
+// column of a global buffer.  The total depth is chosen on the host from the
+// BVH's level count (a 4-wide level pushes at most three entries), so it
+// cannot overflow; the small LDS part keeps occupancy bounded by registers,
+// not LDS.
 //
 // Closest-hit semantics are those of a ray query over opaque triangles
 // (native_intersection.glsl:44-183): candidates with tmin <= t <= tmax; the
@@ -50,15 +53,14 @@ OL_DEV RayPre ray_pre(f3 o, f3 d) {
   return r;
 }
 
-// Slab test of the four quantised child boxes.  Box planes are decoded
-// relative to the ray origin, (o_node - o) + q * 2^e, one fma per plane.
-// tfar is widened by 2^-20 relative so rounding never culls a box that
-// contains a candidate hit.  Returns a 4-bit hit mask and entry distances.
+// Slab test of the four quantised child boxes.  Each plane is decoded
+// exactly as the build rounded it (origin + q * 2^e, one fma, the product is
+// exact), then t = (plane - o) * inv: two roundings relative to t, so
+// widening tfar by 2^-20 keeps every box that contains a candidate hit.
+// Returns a 4-bit hit mask and entry distances.
 OL_DEV uint32_t ol_node_hits(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3, const RayPre& r,
                              float tmin, float tmax, float tnear[4], uint32_t child[4]) {
-  const float dx = __uint_as_float(w0.x) - r.o.x;
-  const float dy = __uint_as_float(w0.y) - r.o.y;
-  const float dz = __uint_as_float(w0.z) - r.o.z;
+  const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
   const float sx = exp_scale(w0.w & 255u), sy = exp_scale((w0.w >> 8) & 255u), sz = exp_scale((w0.w >> 16) & 255u);
   const uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
   child[0] = w2.z;
@@ -69,12 +71,12 @@ OL_DEV uint32_t ol_node_hits(const uint4 w0, const uint4 w1, const uint4 w2, con
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int sh = 8 * c;
-    float lx = (__builtin_fmaf((float)((qlx >> sh) & 255u), sx, dx)) * r.inv.x;
-    float hx = (__builtin_fmaf((float)((qhx >> sh) & 255u), sx, dx)) * r.inv.x;
-    float ly = (__builtin_fmaf((float)((qly >> sh) & 255u), sy, dy)) * r.inv.y;
-    float hy = (__builtin_fmaf((float)((qhy >> sh) & 255u), sy, dy)) * r.inv.y;
-    float lz = (__builtin_fmaf((float)((qlz >> sh) & 255u), sz, dz)) * r.inv.z;
-    float hz = (__builtin_fmaf((float)((qhz >> sh) & 255u), sz, dz)) * r.inv.z;
+    float lx = (__builtin_fmaf((float)((qlx >> sh) & 255u), sx, ox) - r.o.x) * r.inv.x;
+    float hx = (__builtin_fmaf((float)((qhx >> sh) & 255u), sx, ox) - r.o.x) * r.inv.x;
+    float ly = (__builtin_fmaf((float)((qly >> sh) & 255u), sy, oy) - r.o.y) * r.inv.y;
+    float hy = (__builtin_fmaf((float)((qhy >> sh) & 255u), sy, oy) - r.o.y) * r.inv.y;
+    float lz = (__builtin_fmaf((float)((qlz >> sh) & 255u), sz, oz) - r.o.z) * r.inv.z;
+    float hz = (__builtin_fmaf((float)((qhz >> sh) & 255u), sz, oz) - r.o.z) * r.inv.z;
     float tn = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fmaxf(fminf(lz, hz), tmin));
     float tf = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmax));
     tnear[c] = tn;
@@ -84,6 +86,12 @@ OL_DEV uint32_t ol_node_hits(const uint4 w0, const uint4 w1, const uint4 w2, con
   return mask;
 }
 
+// Culling distance for a current best hit at t: boxes entered beyond it are
+// skipped.  The 2^-16 relative slack absorbs the rounding of the
+// Moller-Trumbore distance itself, so a triangle that ties the best hit
+// (equal float t, lower index) is never culled.
+OL_DEV float cull_distance(float t) { return t * 1.0000152587890625f; }
+
 OL_DEV void load_node(const WideNode* nodes, uint32_t index, uint4& w0, uint4& w1, uint4& w2, uint4& w3) {
   const uint4* p = reinterpret_cast<const uint4*>(nodes + index);
   w0 = p[0];
@@ -92,16 +100,44 @@ OL_DEV void load_node(const WideNode* nodes, uint32_t index, uint4& w0, uint4& w
   w3 = p[3];
 }
 
-// Closest hit.  `stack` points at this lane's column; `stride` = block size.
+template <int LSTACK>
+struct TravStack {
+  uint32_t* lds;       // this lane's LDS column
+  uint32_t lds_stride; // block size
+  uint32_t* spill;     // this lane's global column (may be null when spill_cap == 0)
+  uint32_t spill_stride;
+  uint32_t spill_cap;
+  uint32_t* overflow;
+  uint32_t sp;
+
+  OL_DEV void push(uint32_t v) {
+    if (sp < (uint32_t)LSTACK) {
+      lds[sp * lds_stride] = v;
+    } else if (sp - LSTACK < spill_cap) {
+      spill[(sp - LSTACK) * spill_stride] = v;
+    } else {
+      atomicOr(overflow, 1u);
+      return;
+    }
+    ++sp;
+  }
+  OL_DEV uint32_t pop() {
+    --sp;
+    return sp < (uint32_t)LSTACK ? lds[sp * lds_stride] : spill[(sp - LSTACK) * spill_stride];
+  }
+  OL_DEV bool empty() const { return sp == 0; }
+};
+
+// Closest hit.
 // nv / nt accumulate nodes visited and triangles tested (work counters).
-template <int STACK>
+template <int LSTACK>
 OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float4* __restrict__ tri,
-                                 uint32_t node_count, f3 o, f3 d, float tmin, float tmax, uint32_t* stack,
-                                 uint32_t stride, uint32_t* overflow, uint32_t& nv, uint32_t& nt) {
+                                 uint32_t node_count, f3 o, f3 d, float tmin, float tmax, TravStack<LSTACK>& st,
+                                 uint32_t& nv, uint32_t& nt) {
   TraceHit best{tmax, 0.0f, 0.0f, OL_NONE, OL_NONE};
   if (node_count == 0) return best;
   const RayPre r = ray_pre(o, d);
-  uint32_t sp = 0;
+  st.sp = 0;
   uint32_t node = 0;
   while (true) {
     uint4 w0, w1, w2, w3;
@@ -110,7 +146,7 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
     float tn[4];
     uint32_t ch[4];
     // Cull against the current best distance widened like the slab test.
-    uint32_t mask = ol_node_hits(w0, w1, w2, w3, r, tmin, best.t * 1.00000095367431640625f, tn, ch);
+    uint32_t mask = ol_node_hits(w0, w1, w2, w3, r, tmin, cull_distance(best.t), tn, ch);
     // Leaves first (order does not matter for the lexicographic minimum).
     uint32_t inner_mask = 0;
 #pragma unroll
@@ -145,8 +181,8 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
       }
     }
     if (n == 0) {
-      if (sp == 0) break;
-      node = stack[(--sp) * stride];
+      if (st.empty()) break;
+      node = st.pop();
       continue;
     }
     // Insertion sort of up to four (distance, node) pairs, ascending.
@@ -162,27 +198,20 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
       dist[j + 1] = dv;
       idx[j + 1] = iv;
     }
-    for (int i = n - 1; i >= 1; --i) {
-      if (sp < (uint32_t)STACK) {
-        stack[(sp++) * stride] = idx[i];
-      } else {
-        atomicOr(overflow, 1u);
-      }
-    }
+    for (int i = n - 1; i >= 1; --i) st.push(idx[i]);
     node = idx[0];
   }
   return best;
 }
 
 // Any hit in [tmin, tmax]: returns true as soon as one triangle qualifies.
-template <int STACK>
+template <int LSTACK>
 OL_DEV bool ol_trace_any(const WideNode* __restrict__ nodes, const float4* __restrict__ tri, uint32_t node_count,
-                         f3 o, f3 d, float tmin, float tmax, uint32_t* stack, uint32_t stride, uint32_t* overflow,
-                         uint32_t& nv, uint32_t& nt) {
+                         f3 o, f3 d, float tmin, float tmax, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& nt) {
   if (node_count == 0) return false;
   const RayPre r = ray_pre(o, d);
-  const float tcull = tmax * 1.00000095367431640625f;
-  uint32_t sp = 0;
+  const float tcull = cull_distance(tmax);
+  st.sp = 0;
   uint32_t node = 0;
   while (true) {
     uint4 w0, w1, w2, w3;
@@ -206,17 +235,15 @@ OL_DEV bool ol_trace_any(const WideNode* __restrict__ nodes, const float4* __res
         }
       } else if (next == OL_NONE) {
         next = ch[c];
-      } else if (sp < (uint32_t)STACK) {
-        stack[(sp++) * stride] = ch[c];
       } else {
-        atomicOr(overflow, 1u);
+        st.push(ch[c]);
       }
     }
     if (next != OL_NONE) {
       node = next;
       continue;
     }
-    if (sp == 0) return false;
-    node = stack[(--sp) * stride];
+    if (st.empty()) return false;
+    node = st.pop();
   }
 }

@@ -52,7 +52,9 @@ class HipRendererConfig:
     max_samples_per_call: int = 64
     device_index: int = 0
     bvh_leaf_size: int = 4
-    traversal_stack_depth: int = 0
+    # Traversal stack entries kept in LDS per lane, and the traversal block size.
+    traversal_lds_stack: int = 16
+    traversal_block: int = 128
     profiling: bool = False
     work_counters: bool = False
 
@@ -95,7 +97,7 @@ HOT_SETTINGS = frozenset({
     "wavefront_secondary_nee_probability", "wavefront_unified_secondary_nee",
     "wavefront_russian_roulette", "wavefront_russian_roulette_start",
     "wavefront_russian_roulette_min_survival", "wavefront_tile_capacity",
-    "max_samples_per_call", "traversal_stack_depth", "profiling", "work_counters",
+    "max_samples_per_call", "traversal_lds_stack", "traversal_block", "profiling", "work_counters",
 })
 
 
@@ -298,7 +300,8 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         context = self._ensure_context()
         lib = N.library()
         N.check(lib.olhip_set_profiling(context.handle, int(bool(self.config.profiling))))
-        N.check(lib.olhip_set_stack_depth(context.handle, int(self.config.traversal_stack_depth)))
+        N.check(lib.olhip_set_traversal(context.handle, int(self.config.traversal_lds_stack),
+                                        int(self.config.traversal_block)))
         N.check(lib.olhip_set_work_counters(context.handle, int(bool(self.config.work_counters))))
         position, forward, right, up = camera_basis(camera)
         cam = N.Camera()
