@@ -390,11 +390,15 @@ __global__ void __launch_bounds__(256) k_generate(GiParams P, GiState S) {
   const uint32_t rem = g - local * per_sample;
   const uint32_t sample = P.sample_base + local;
   const uint32_t tile = rem >> 6, lane = rem & 63u;
-  const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
-  const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
-  const bool valid = local < P.spp && px < W && py < H;
-  const uint32_t slot = wave_reserve(&S.ctr[0], valid);
-  if (!valid) return;
+  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+  const uint32_t lx = lane & 7u, ly = lane >> 3;
+  const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
+  if (g == 0) S.ctr[0] = P.spp * W * H;
+  if (!(local < P.spp && px < W && py < H)) return;
+  // Queue slot in 8x8 tile order without atomics: full tile rows above,
+  // full-width tiles to the left in this tile row, row-major inside the tile.
+  const uint32_t th = min(8u, H - 8u * ty), tw = min(8u, W - 8u * tx);
+  const uint32_t slot = local * (W * H) + 8u * ty * W + 8u * tx * th + ly * tw + lx;
   const uint32_t pixel = py * W + px;
   const uint32_t path = local * (W * H) + pixel;
   uint32_t rng = wave_hash((pixel ^ wave_hash(P.frame)) ^ wave_hash(sample + 1u));
