@@ -91,7 +91,7 @@ def main(argv=None):
     parser.add_argument("--cpu-seconds", type=float, default=12.0)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-profile", action="store_true")
-    parser.add_argument("--leaf-size", type=int, default=4)
+    parser.add_argument("--leaf-size", type=int, default=2)
     parser.add_argument("--lds-stack", type=int, default=16)
     parser.add_argument("--block", type=int, default=128)
     args = parser.parse_args(argv)

@@ -43,6 +43,13 @@
 #define API(name) CAT(name, OL_SUFFIX)
 
 typedef OL_REAL R;
+
+/* Float32 Moller-Trumbore accepts points a few ulps of the scene scale outside
+ * the exact triangle; node boxes are padded by this fraction of the largest
+ * coordinate so that the exact box test never culls such a hit. */
+#ifndef OL_BOX_PAD
+#define OL_BOX_PAD 1e-5
+#endif
 typedef struct {
   R x, y, z;
 } V;
@@ -155,6 +162,7 @@ typedef struct {
   BNode* nodes;
   uint32_t node_count;
   uint32_t* order; /* leaf order -> triangle */
+  double pad;      /* node boxes are grown by this much on every side */
 } Scene;
 
 static void tri_box(const Scene* s, uint32_t t, double mn[3], double mx[3]) {
@@ -200,6 +208,10 @@ static uint32_t build_rec(Scene* s, uint32_t first, uint32_t count) {
       if (c < cmn[k]) cmn[k] = c;
       if (c > cmx[k]) cmx[k] = c;
     }
+  }
+  for (int k = 0; k < 3; ++k) {
+    n->mn[k] -= s->pad;
+    n->mx[k] += s->pad;
   }
   if (count <= 4) {
     n->left = first;
@@ -266,6 +278,9 @@ void* API(olgi_scene_new)(const float* pos, const float* nrm, uint32_t V, const 
     for (int k = 0; k < 3; ++k) g_cent[3ull * t + k] = 0.5 * (a[k] + b[k]);
     s->order[t] = t;
   }
+  double scale = 0.0;
+  for (uint64_t i = 0; i < 3ull * V; ++i) scale = fmax(scale, fabs((double)s->pos[i]));
+  s->pad = OL_BOX_PAD * (1.0 + scale);
   g_sort_scene = s;
   if (T) build_rec(s, 0, T);
   free(g_cent);

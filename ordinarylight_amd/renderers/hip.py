@@ -51,7 +51,7 @@ class HipRendererConfig:
     # The reference accepts 1..64 samples per call (api.py:1270-1272).
     max_samples_per_call: int = 64
     device_index: int = 0
-    bvh_leaf_size: int = 4
+    bvh_leaf_size: int = 2
     # Traversal stack entries kept in LDS per lane, and the traversal block size.
     traversal_lds_stack: int = 16
     traversal_block: int = 128
@@ -356,7 +356,7 @@ class HipReferenceConfig:
     max_bounces: int = 3
     seed: int = 1
     device_index: int = 0
-    bvh_leaf_size: int = 4
+    bvh_leaf_size: int = 2
     # Index-order triangle scan below this triangle count (exact tie order).
     brute_force_below: int = 257
 
