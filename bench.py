@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_IN, HIT_OUT, OCC_OUT = 64, 48, 32, 16, 4
+TRAFFIC_FILE = os.path.join("profiles", "k_trace_closest_terrain1M_1080p_8spp_8b_traffic.txt")
 
 
 def log(msg):
@@ -42,6 +43,19 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
+
+
+def measured_traffic(workload):
+    """HBM bytes per closest-hit dispatch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was measured on this same workload."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            fields = dict(line.split(":", 1) for line in f if line.startswith("traffic_"))
+    except OSError:
+        return None
+    if fields.get("traffic_workload", "").strip() != workload:
+        return None
+    return int(fields["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(scene, camera, width, height, spp, bounces, target_seconds):
@@ -161,10 +175,13 @@ def main(argv=None):
         per_launch_bytes = alg_bytes / (trace_launches / args.steps)
         avg_ms = trace_ms / trace_launches
         achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        workload = f"terrain 1M triangles, {W}x{H}, {spp} spp, {B} bounces, leaf {args.leaf_size}"
+        traffic = measured_traffic(workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": "k_trace_closest", "avg_launch_ms": round(avg_ms, 4),
-                    "bytes_per_launch": int(per_launch_bytes)}
+                    "bytes_per_launch": int(per_launch_bytes),
+                    "traffic_source": TRAFFIC_FILE if traffic is not None else None}
     line = {
         "metric": "Mrays/s at 1080p 8-bounce (wavefront GI, one MI355X)",
         "value": round(value, 2),

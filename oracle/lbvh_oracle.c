@@ -7,8 +7,10 @@
  * centroids against the centroid bounds, a stable sort by code, the Karras
  * (2012) binary radix tree with index-augmented keys, bottom-up boxes and a
  * breadth-first collapse into 64-byte 4-wide nodes with 8-bit outward-rounded
- * child boxes.  All steps are integer work or exact float min/max/compare, so
- * the GPU output must match this byte for byte (tests/test_bvh_gpu.py).
+ * child boxes, each first grown by 2^-17 of the largest root-box coordinate.
+ * All steps are integer work, float min/max/compare or single correctly
+ * rounded float operations, so the GPU output must match this byte for byte
+ * (tests/test_gpu_parity.py::test_bvh_bitexact).
  *
  * The reference uses driver-built hardware BVHs (targets/vulkan/scene.py
  * :1132-1194); there is nothing in it to pin this against except the
@@ -268,6 +270,14 @@ int ol_lbvh_build(const float* positions, const uint32_t* indices, uint32_t T, u
   uint32_t* nxt = (uint32_t*)malloc(4ull * (T + 1));
   if (!cur || !nxt) return -1;
   cur[0] = n > 1 ? 0u : LEAF_BIT;
+  /* Box growth: 2^-17 of the largest root-box coordinate on every side. */
+  float pad = 0.0f;
+  {
+    const Box* rb = ref_box(&tr, cur[0]);
+    float m = 0.0f;
+    for (int a = 0; a < 3; ++a) m = fmaxf(m, fmaxf(fabsf(rb->mn[a]), fabsf(rb->mx[a])));
+    pad = m * 0x1p-17f;
+  }
   uint32_t level_count = 1, level_base = 0, depth = 0;
   while (level_count > 0) {
     ++depth;
@@ -276,11 +286,14 @@ int ol_lbvh_build(const float* positions, const uint32_t* indices, uint32_t T, u
       uint32_t kids[4];
       int nk = plan(&tr, cur[k], kids);
       float nmn[3] = {INFINITY, INFINITY, INFINITY}, nmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      float cmn[4][3], cmx[4][3];
       for (int c = 0; c < nk; ++c) {
         const Box* b = ref_box(&tr, kids[c]);
         for (int a = 0; a < 3; ++a) {
-          nmn[a] = fminf(nmn[a], b->mn[a]);
-          nmx[a] = fmaxf(nmx[a], b->mx[a]);
+          cmn[c][a] = b->mn[a] - pad;
+          cmx[c][a] = b->mx[a] + pad;
+          nmn[a] = fminf(nmn[a], cmn[c][a]);
+          nmx[a] = fmaxf(nmx[a], cmx[c][a]);
         }
       }
       uint32_t words[16];
@@ -296,11 +309,10 @@ int ol_lbvh_build(const float* positions, const uint32_t* indices, uint32_t T, u
           words[10 + c] = NONE;
           continue;
         }
-        const Box* b = ref_box(&tr, kids[c]);
         for (int a = 0; a < 3; ++a) {
           float sc = scale_of(e[a]);
-          ql[a] |= quant_down(nmn[a], sc, b->mn[a]) << (8 * c);
-          qh[a] |= quant_up(nmn[a], sc, b->mx[a]) << (8 * c);
+          ql[a] |= quant_down(nmn[a], sc, cmn[c][a]) << (8 * c);
+          qh[a] |= quant_up(nmn[a], sc, cmx[c][a]) << (8 * c);
         }
         if (ref_leafy(&tr, kids[c])) {
           words[10 + c] = LEAF_BIT | ((ref_count(&tr, kids[c]) - 1u) << 27) | ref_first(&tr, kids[c]);

@@ -329,28 +329,35 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, capacity / np));
     if ((uint64_t)chunk * np > (1ull << 31)) chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 31) / np);
     const uint64_t paths = (uint64_t)chunk * np;
+    const bool two_pass = sc->point_count > (sc->view.env_valid ? 1u : 0u) || !set->unified_secondary_nee;
+    uint32_t per_hit = 1;
+    if (two_pass) per_hit = sc->point_count + (set->unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(set->secondary_area_samples, 1u), 16u) + 4u));
+    // Striped queues: kStripes segments of qseg slots (see gi.h).
+    const uint64_t unit = (uint64_t)kStripeUnit * kStripes;
+    const uint64_t qseg = (paths + unit - 1) / unit * kStripeUnit;
+    const uint64_t qslots = qseg * kStripes;
+    const uint64_t eseg = qseg * per_hit, eslots = eseg * kStripes;
+    if (eslots > 0xFFFFFFFFull)
+      throw OlError{OLHIP_ERR_MEMORY, "shadow queue of " + std::to_string(eslots) +
+                                          " entries exceeds 2^32; lower path_capacity"};
     // Buffers.
     ctx->T.reserve(16 * paths);
     ctx->L.reserve(16 * paths);
     ctx->M.reserve(16 * paths);
     if (sc->has_transmission) ctx->stk.reserve(64 * paths);
-    ctx->qo0.reserve(16 * paths);
-    ctx->qo1.reserve(16 * paths);
-    ctx->qd0.reserve(16 * paths);
-    ctx->qd1.reserve(16 * paths);
-    ctx->qp0.reserve(4 * paths);
-    ctx->qp1.reserve(4 * paths);
-    ctx->hit.reserve(16 * paths);
-    const bool two_pass = sc->point_count > (sc->view.env_valid ? 1u : 0u) || !set->unified_secondary_nee;
-    uint32_t per_hit = 1;
-    if (two_pass) per_hit = sc->point_count + (set->unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(set->secondary_area_samples, 1u), 16u) + 4u));
-    ctx->gT.reserve(16 * paths);
-    ctx->gI.reserve(16 * paths);
-    const uint64_t entries = paths * per_hit;
-    ctx->eo.reserve(16 * entries);
-    ctx->ed.reserve(16 * entries);
-    ctx->ec.reserve(16 * entries);
-    ctx->ctr.reserve(64);
+    ctx->qo0.reserve(16 * qslots);
+    ctx->qo1.reserve(16 * qslots);
+    ctx->qd0.reserve(16 * qslots);
+    ctx->qd1.reserve(16 * qslots);
+    ctx->qp0.reserve(4 * qslots);
+    ctx->qp1.reserve(4 * qslots);
+    ctx->hit.reserve(16 * qslots);
+    ctx->gT.reserve(16 * qslots);
+    ctx->gI.reserve(16 * qslots);
+    ctx->eo.reserve(16 * eslots);
+    ctx->ed.reserve(16 * eslots);
+    ctx->ec.reserve(16 * eslots);
+    ctx->ctr.reserve(4 * CTR_WORDS);
     ctx->stats.reserve(64);
     ctx->work.reserve(64);
     ctx->acc.reserve(sizeof(double4) * np);
@@ -377,9 +384,9 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     S.ctr = ctx->ctr.as<uint32_t>();
     S.stats = ctx->stats.as<unsigned long long>();
     S.work = ctx->counting ? ctx->work.as<unsigned long long>() : nullptr;
-    S.qcap = (uint32_t)paths;
-    S.gcap = (uint32_t)paths;
-    S.ecap = (uint32_t)std::min<uint64_t>(entries, 0xFFFFFFFFull);
+    S.qseg = (uint32_t)qseg;
+    S.gseg = (uint32_t)qseg;
+    S.eseg = (uint32_t)eseg;
 
     GiParams P{};
     P.sc = sc->view;
@@ -411,12 +418,12 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     double trace_ms = 0, shade_ms = 0, shadow_ms = 0;
     uint32_t n_trace = 0, n_shade = 0, n_shadow = 0;
     OL_HIP_CHECK(hipEventRecord(e_start.e, s));
-    gi_reset(S, s);
+    gi_reset(S, true, s);
     uint32_t chunks = 0;
     for (uint32_t base = 0; base < samples; base += chunk) {
       P.sample_base = base;
       P.spp = std::min(chunk, samples - base);
-      OL_HIP_CHECK(hipMemsetAsync(S.ctr, 0, 8 * sizeof(uint32_t), s));
+      gi_reset(S, false, s);
       gi_generate(P, S, s);
       for (uint32_t b = 0; b < set->max_bounces; ++b) {
         const uint32_t in = b & 1u;
@@ -452,12 +459,12 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     OL_HIP_CHECK(hipGetLastError());
     OL_HIP_CHECK(hipEventRecord(e_end.e, s));
     OL_HIP_CHECK(hipEventSynchronize(e_end.e));
-    uint32_t ctr_host[8];
+    uint32_t ctr_host[CTR_WORDS];
     unsigned long long st_host[4], work_host[4] = {0, 0, 0, 0};
     OL_HIP_CHECK(hipMemcpy(ctr_host, S.ctr, sizeof(ctr_host), hipMemcpyDeviceToHost));
     OL_HIP_CHECK(hipMemcpy(st_host, S.stats, sizeof(st_host), hipMemcpyDeviceToHost));
     if (S.work) OL_HIP_CHECK(hipMemcpy(work_host, S.work, sizeof(work_host), hipMemcpyDeviceToHost));
-    if (ctr_host[5]) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
+    if (ctr_host[CTR_STACK]) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
     for (size_t k = 0; k + 3 < marks.size(); k += 4) {
       trace_ms += elapsed(*marks[k], *marks[k + 1]);
       shade_ms += elapsed(*marks[k + 1], *marks[k + 2]);
@@ -470,7 +477,7 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
       o.primary_rays = st_host[0];
       o.extension_rays = st_host[1];
       o.shadow_rays = st_host[2];
-      o.queue_overflow = ctr_host[4];
+      o.queue_overflow = ctr_host[CTR_OVERFLOW];
       o.total_ms = elapsed(e_start, e_end);
       o.trace_ms = trace_ms;
       o.shade_ms = shade_ms;

@@ -282,7 +282,20 @@ struct TreeView {
   const uint32_t *left, *right, *rfirst, *rlast;
   const float4 *lmin, *lmax, *bmin, *bmax;
   uint32_t leaf_size;
+  uint32_t root;  // binary root reference
 };
+
+// Float32 ray/triangle tests accept points a few ulps of the coordinate
+// magnitude outside the exact triangle (vertex- and edge-grazing rays).
+// Every wide-node box is grown by 2^-17 of the largest root-box coordinate
+// on each side before quantisation, so no such hit is culled.
+__device__ __forceinline__ float box_pad(const TreeView& t) {
+  float4 mn, mx;
+  ref_box(t.root, t.lmin, t.lmax, t.bmin, t.bmax, mn, mx);
+  const float m = fmaxf(fmaxf(fmaxf(fabsf(mn.x), fabsf(mn.y)), fmaxf(fabsf(mn.z), fabsf(mx.x))),
+                        fmaxf(fabsf(mx.y), fabsf(mx.z)));
+  return m * 0x1p-17f;
+}
 
 __device__ __forceinline__ uint32_t ref_count(const TreeView& t, uint32_t ref) {
   return (ref & OL_LEAF_BIT) ? 1u : t.rlast[ref] - t.rfirst[ref] + 1u;
@@ -376,8 +389,11 @@ __global__ void __launch_bounds__(kBlock) k_collapse_emit(TreeView t, const uint
   int nk = plan_children(t, front[k], kids);
   float4 cmn[4], cmx[4];
   float4 nmn = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), nmx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  const float pad = box_pad(t);
   for (int i = 0; i < nk; ++i) {
     ref_box(kids[i], t.lmin, t.lmax, t.bmin, t.bmax, cmn[i], cmx[i]);
+    cmn[i] = make_float4(cmn[i].x - pad, cmn[i].y - pad, cmn[i].z - pad, 0.0f);
+    cmx[i] = make_float4(cmx[i].x + pad, cmx[i].y + pad, cmx[i].z + pad, 0.0f);
     nmn = make_float4(fminf(nmn.x, cmn[i].x), fminf(nmn.y, cmn[i].y), fminf(nmn.z, cmn[i].z), 0.0f);
     nmx = make_float4(fmaxf(nmx.x, cmx[i].x), fmaxf(nmx.y, cmx[i].y), fmaxf(nmx.z, cmx[i].z), 0.0f);
   }
@@ -522,7 +538,8 @@ void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debu
 
   // Wide collapse, one breadth-first level per iteration.
   TreeView tv{left.as<uint32_t>(), right.as<uint32_t>(), rfirst.as<uint32_t>(), rlast.as<uint32_t>(),
-              lmin.as<float4>(), lmax.as<float4>(), bmin.as<float4>(), bmax.as<float4>(), leaf_size};
+              lmin.as<float4>(), lmax.as<float4>(), bmin.as<float4>(), bmax.as<float4>(), leaf_size,
+              n > 1 ? 0u : OL_LEAF_BIT};
   out.nodes.reserve(sizeof(WideNode) * (size_t)(n + 1));
   DevBuf front_a, front_b, icount;
   front_a.reserve(4ull * (n + 1));

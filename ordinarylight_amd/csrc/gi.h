@@ -3,6 +3,20 @@
 
 #include "internal.h"
 
+// Queue counters are striped: each queue (rays, shadow groups, shadow
+// entries) is cut into kStripes segments of equal capacity, each with its own
+// counter, so that slot reservations spread over kStripes addresses instead of
+// serialising on one.  A producer wave appends to the segment of stripe
+// (input index / 256) % kStripes; since every input yields at most one
+// continuation and one group, a segment of ceil(capacity / (256 * kStripes)) *
+// 256 slots never overflows.  Consumers enumerate the live entries densely
+// through the inclusive prefix of the kStripes counts.
+constexpr uint32_t kStripes = 32;
+constexpr uint32_t kStripeUnit = 256;  // producer block size: inputs per stripe step
+constexpr uint32_t CTR_Q0 = 0, CTR_Q1 = kStripes, CTR_G = 2 * kStripes, CTR_E = 3 * kStripes;
+constexpr uint32_t CTR_OVERFLOW = 4 * kStripes, CTR_STACK = 4 * kStripes + 1, CTR_WORDS = 4 * kStripes + 8;
+OL_HD constexpr uint32_t ctr_queue(uint32_t q) { return q ? CTR_Q1 : CTR_Q0; }
+
 // Per-path state, structure of arrays indexed by path id
 // p = sample * (W * H) + pixel (wavefront/_core.py:19-105 hot path state).
 struct GiState {
@@ -21,10 +35,10 @@ struct GiState {
   float4* eo;        // entry origin, tmin
   float4* ed;        // entry direction, tmax
   float4* ec;        // entry contribution rgb, kind (0 point, 1 area, 2 environment)
-  uint32_t* ctr;     // [0],[1] queue counts, [2] groups, [3] entries, [4] queue overflow, [5] stack overflow
+  uint32_t* ctr;     // CTR_WORDS: per-stripe counts of queue 0, queue 1, groups, entries; overflow flags
   unsigned long long* stats;  // [0] primary, [1] extension, [2] shadow rays, [3] shaded hits
   unsigned long long* work;   // optional: [0] closest nodes, [1] closest tris, [2] shadow nodes, [3] shadow tris
-  uint32_t qcap, gcap, ecap;
+  uint32_t qseg, gseg, eseg;  // slots per stripe segment
 };
 
 struct GiParams {
@@ -65,7 +79,7 @@ void gi_shade(const GiLaunch& l, const GiParams& p, const GiState& s, uint32_t i
 void gi_trace_shadow(const GiLaunch& l, const DScene& sc, const GiState& s, hipStream_t st);
 void gi_accumulate(const GiParams& p, const GiState& s, double4* acc, bool first, hipStream_t st);
 void gi_final(const double4* acc, uint32_t np, uint32_t samples, float4* out, hipStream_t st);
-void gi_reset(const GiState& s, hipStream_t st);
+void gi_reset(const GiState& s, bool all, hipStream_t st);
 void gi_finish(const GiState& s, hipStream_t st);
 
 // Debug / products: closest hit of caller-supplied rays.

@@ -14,7 +14,8 @@
 // visible contributions are summed in the reference's order and added before
 // the next bounce, so the radiance of every path is the same sequence of
 // float operations as the inline form.  Queue slots are reserved with one
-// atomic per wave (ballot + mbcnt).  Counts stay on the device; kernels use
+// atomic per wave (ballot + mbcnt) on one of 32 striped counters.  Counts
+// stay on the device; kernels use
 // grid-stride loops sized to the resident grid, so no host synchronisation
 // happens between bounces.
 
@@ -380,6 +381,36 @@ OL_DEV f3 refract3(f3 I, f3 N, float eta) {
   return eta * I - (eta * ndi + sqrtf(k)) * N;
 }
 
+// ---- striped queues ---------------------------------------------------------------------------
+// Inclusive prefix of the kStripes counts at `counts` into `incl` (LDS).
+// Every thread of the block calls it; wave 0 does the scan.
+OL_DEV void stripe_prefix(const uint32_t* counts, uint32_t* incl) {
+  if (threadIdx.x < 64u) {
+    uint32_t v = threadIdx.x < kStripes ? counts[threadIdx.x] : 0u;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t o = __shfl_up(v, (unsigned)off, 64);
+      if ((int)threadIdx.x >= off) v += o;
+    }
+    if (threadIdx.x < kStripes) incl[threadIdx.x] = v;
+  }
+  __syncthreads();
+}
+
+// Slot of the j-th live entry (j < incl[kStripes - 1]): stripe s holds
+// entries [incl[s-1], incl[s]) at slots s * seg + 0, 1, ...
+OL_DEV uint32_t stripe_slot(const uint32_t* incl, uint32_t seg, uint32_t j) {
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t step = kStripes / 2u; step; step >>= 1)
+    if (incl[s + step - 1u] <= j) s += step;
+  return s * seg + (j - (s ? incl[s - 1u] : 0u));
+}
+
+OL_DEV uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // ---- kernels ----------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_generate(GiParams P, GiState S) {
   const uint32_t W = P.width, H = P.height;
@@ -393,7 +424,8 @@ __global__ void __launch_bounds__(256) k_generate(GiParams P, GiState S) {
   const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
   const uint32_t lx = lane & 7u, ly = lane >> 3;
   const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
-  if (g == 0) S.ctr[0] = P.spp * W * H;
+  // The primary queue is dense: all entries in stripe 0 (slot = index).
+  if (g == 0) S.ctr[CTR_Q0] = P.spp * W * H;
   if (!(local < P.spp && px < W && py < H)) return;
   // Queue slot in 8x8 tile order without atomics: full tile rows above,
   // full-width tiles to the left in this tile row, row-major inside the tile.
@@ -436,24 +468,37 @@ __global__ void __launch_bounds__(256) k_generate(GiParams P, GiState S) {
   if (S.stk) S.stk[4u * path] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ void k_prologue(GiState S, uint32_t bounce) {
-  const uint32_t in = bounce & 1u, out = in ^ 1u;
-  S.stats[bounce == 0 ? 0 : 1] += S.ctr[in];
-  S.stats[2] += S.ctr[3];
-  S.ctr[out] = 0;
-  S.ctr[2] = 0;
-  S.ctr[3] = 0;
+// One wave: adds the input queue and the previous bounce's shadow entries to
+// the ray statistics, then empties the output queue and the shadow queues.
+__global__ void __launch_bounds__(64) k_prologue(GiState S, uint32_t bounce) {
+  const uint32_t in = bounce & 1u, out = in ^ 1u, k = threadIdx.x;
+  const uint32_t q = wave_sum(k < kStripes ? S.ctr[ctr_queue(in) + k] : 0u);
+  const uint32_t e = wave_sum(k < kStripes ? S.ctr[CTR_E + k] : 0u);
+  if (k == 0) {
+    S.stats[bounce == 0 ? 0 : 1] += q;
+    S.stats[2] += e;
+  }
+  if (k < kStripes) {
+    S.ctr[ctr_queue(out) + k] = 0u;
+    S.ctr[CTR_G + k] = 0u;
+    S.ctr[CTR_E + k] = 0u;
+  }
 }
 
-__global__ void k_finish(GiState S) {
-  S.stats[2] += S.ctr[3];
-  S.ctr[3] = 0;
+__global__ void __launch_bounds__(64) k_finish(GiState S) {
+  const uint32_t k = threadIdx.x;
+  const uint32_t e = wave_sum(k < kStripes ? S.ctr[CTR_E + k] : 0u);
+  if (k == 0) S.stats[2] += e;
+  if (k < kStripes) S.ctr[CTR_E + k] = 0u;
 }
 
-__global__ void k_reset(GiState S) {
-  if (threadIdx.x < 8) S.ctr[threadIdx.x] = 0;
-  if (threadIdx.x < 4) S.stats[threadIdx.x] = 0;
-  if (S.work && threadIdx.x < 4) S.work[threadIdx.x] = 0;
+// Whole render: counters, overflow flags, statistics.  Per chunk
+// (`all` = 0): the queue counters only; overflow flags accumulate.
+__global__ void __launch_bounds__(256) k_reset(GiState S, uint32_t all) {
+  const uint32_t k = threadIdx.x;
+  if (k < (all ? CTR_WORDS : CTR_OVERFLOW)) S.ctr[k] = 0u;
+  if (all && k < 4) S.stats[k] = 0;
+  if (all && S.work && k < 4) S.work[k] = 0;
 }
 
 // Lane-private traversal stack: LDS column + global spill column.
@@ -474,12 +519,15 @@ template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_closest(DScene sc, GiState S, uint32_t in, uint32_t* spill,
                                                          uint32_t spill_cap) {
   __shared__ uint32_t lds[LSTACK * BLOCK];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[5]);
-  const uint32_t count = S.ctr[in];
+  __shared__ uint32_t incl[kStripes];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
+  stripe_prefix(S.ctr + ctr_queue(in), incl);
+  const uint32_t count = incl[kStripes - 1u];
   const float4* __restrict__ qo = S.qo[in];
   const float4* __restrict__ qd = S.qd[in];
   uint32_t nv = 0, nt = 0;
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < count; i += gridDim.x * BLOCK) {
+  for (uint32_t j = blockIdx.x * BLOCK + threadIdx.x; j < count; j += gridDim.x * BLOCK) {
+    const uint32_t i = stripe_slot(incl, S.qseg, j);
     float4 o = qo[i], d = qd[i];
     TraceHit h = ol_trace_closest<LSTACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, st, nv, nt);
     S.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.slot));
@@ -511,10 +559,13 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o
 template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_shadow(DScene sc, GiState S, uint32_t* spill, uint32_t spill_cap) {
   __shared__ uint32_t lds[LSTACK * BLOCK];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[5]);
-  const uint32_t count = S.ctr[2];
+  __shared__ uint32_t incl[kStripes];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
+  stripe_prefix(S.ctr + CTR_G, incl);
+  const uint32_t count = incl[kStripes - 1u];
   uint32_t nv = 0, nt = 0;
-  for (uint32_t g = blockIdx.x * BLOCK + threadIdx.x; g < count; g += gridDim.x * BLOCK) {
+  for (uint32_t j = blockIdx.x * BLOCK + threadIdx.x; j < count; j += gridDim.x * BLOCK) {
+    const uint32_t g = stripe_slot(incl, S.gseg, j);
     const float4 gT = S.gT[g];
     const uint4 gI = S.gI[g];
     const uint32_t path = gI.x, first = gI.y, n = gI.z;
@@ -547,17 +598,22 @@ struct ShadeOut {
   float4 co, cd;
 };
 
-__global__ void __launch_bounds__(256) k_shade(GiParams P, GiState S, uint32_t in) {
+// Launched with blocks of kStripeUnit threads, so the output stripe
+// (j / kStripeUnit) % kStripes of dense input j is uniform per block.
+__global__ void __launch_bounds__(kStripeUnit) k_shade(GiParams P, GiState S, uint32_t in) {
   const DScene& sc = P.sc;
-  const uint32_t count = S.ctr[in];
+  __shared__ uint32_t incl[kStripes];
+  stripe_prefix(S.ctr + ctr_queue(in), incl);
+  const uint32_t count = incl[kStripes - 1u];
   const uint32_t out = in ^ 1u;
   const uint32_t stride = gridDim.x * blockDim.x;
   // Every lane runs the same number of loop trips so the wave-wide slot
   // reservations below are reached together.
   const uint32_t trips = (count + stride - 1u) / stride;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t trip = 0; trip < trips; ++trip, i += stride) {
-    const bool live = i < count;
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t trip = 0; trip < trips; ++trip, j += stride) {
+    const bool live = j < count;
+    const uint32_t so = (j / kStripeUnit) % kStripes;
     bool cont = false, nee = false;
     float4 co = make_float4(0, 0, 0, 0), cd = co;
     uint32_t path = 0, nee_count = 0, area_n = 0, env_n = 0, nee_rng = 0;
@@ -567,6 +623,7 @@ __global__ void __launch_bounds__(256) k_shade(GiParams P, GiState S, uint32_t i
     NeeSample single;
     single.valid = false;
     if (live) {
+      const uint32_t i = stripe_slot(incl, S.qseg, j);
       path = S.qp[in][i];
       const float4 ro = S.qo[in][i], rd = S.qd[in][i], hv = S.hit[i];
       float4 T = S.T[path], L = S.L[path];
@@ -747,24 +804,26 @@ __global__ void __launch_bounds__(256) k_shade(GiParams P, GiState S, uint32_t i
       S.L[path] = L;
       S.M[path] = M;
     }
-    // Continuation slot (one atomic per wave).
-    const uint32_t cslot = wave_reserve(&S.ctr[out], cont);
+    // Continuation slot in this block's stripe (one atomic per wave).
+    const uint32_t cl = wave_reserve(&S.ctr[ctr_queue(out) + so], cont);
     if (cont) {
-      if (cslot < S.qcap) {
+      if (cl < S.qseg) {
+        const uint32_t cslot = so * S.qseg + cl;
         S.qo[out][cslot] = co;
         S.qd[out][cslot] = cd;
         S.qp[out][cslot] = path;
       } else {
-        atomicAdd(&S.ctr[4], 1u);
+        atomicAdd(&S.ctr[CTR_OVERFLOW], 1u);
         S.M[path].w &= ~F_ACTIVE;
       }
     }
     // Shadow group + entries.
     const bool has_group = nee && nee_count > 0u;
-    const uint32_t gslot = wave_reserve(&S.ctr[2], has_group);
-    const uint32_t eslot = wave_reserve_n(&S.ctr[3], has_group ? nee_count : 0u);
+    const uint32_t gl = wave_reserve(&S.ctr[CTR_G + so], has_group);
+    const uint32_t el = wave_reserve_n(&S.ctr[CTR_E + so], has_group ? nee_count : 0u);
     if (has_group) {
-      if (gslot < S.gcap && eslot + nee_count <= S.ecap) {
+      if (gl < S.gseg && el + nee_count <= S.eseg) {
+        const uint32_t gslot = so * S.gseg + gl, eslot = so * S.eseg + el;
         S.gT[gslot] = mk4(nee_T, nee_p);
         S.gI[gslot] = make_uint4(path, eslot, nee_count, area_n | (env_n << 8) | ((P.unified ? 0u : 1u) << 16));
         if (!P.two_pass) {
@@ -782,7 +841,7 @@ __global__ void __launch_bounds__(256) k_shade(GiParams P, GiState S, uint32_t i
           });
         }
       } else {
-        atomicAdd(&S.ctr[4], 1u);
+        atomicAdd(&S.ctr[CTR_OVERFLOW], 1u);
       }
     }
   }
@@ -895,9 +954,12 @@ int stack_depth_for(uint32_t bvh_depth) {
 
 uint64_t trace_spill_words(const GiLaunch& l) { OL_STACK_DISPATCH(spill_words_for, l); }
 
-void gi_reset(const GiState& s, hipStream_t st) { hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, s); }
+void gi_reset(const GiState& s, bool all, hipStream_t st) {
+  static_assert(CTR_WORDS <= 256, "counter block larger than the reset kernel");
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, s, all ? 1u : 0u);
+}
 
-void gi_finish(const GiState& s, hipStream_t st) { hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, st, s); }
+void gi_finish(const GiState& s, hipStream_t st) { hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, s); }
 
 void gi_generate(const GiParams& p, const GiState& s, hipStream_t st) {
   const uint32_t tiles = ((p.width + 7u) / 8u) * ((p.height + 7u) / 8u);
@@ -906,7 +968,7 @@ void gi_generate(const GiParams& p, const GiState& s, hipStream_t st) {
 }
 
 void gi_bounce_prologue(const GiState& s, uint32_t bounce, hipStream_t st) {
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(1), 0, st, s, bounce);
+  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(64), 0, st, s, bounce);
 }
 
 void gi_trace_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint32_t in, hipStream_t st) {
@@ -923,11 +985,8 @@ void trace_rays(const GiLaunch& l, const DScene& sc, const float4* o, const floa
 }
 
 void gi_shade(const GiLaunch& l, const GiParams& p, const GiState& s, uint32_t in, hipStream_t st) {
-  static int occ = -1;
-  if (occ < 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_shade, 256, 0) != hipSuccess || occ < 1) occ = 1;
-  }
-  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, p, s, in);
+  static int occ = resident_blocks(k_shade, (int)kStripeUnit);
+  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(kStripeUnit), 0, st, p, s, in);
 }
 
 void gi_accumulate(const GiParams& p, const GiState& s, double4* acc, bool first, hipStream_t st) {
