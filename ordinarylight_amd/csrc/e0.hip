@@ -201,13 +201,13 @@ __global__ void __launch_bounds__(BLOCK) k_e0_trace(E0Params P, E0State S, uint3
       while (true) {
         uint4 w0, w1, w2, w3;
         load_node(P.sc.nodes, node, w0, w1, w2, w3);
-        float tn[4];
-        uint32_t ch[4];
+        uint32_t ch[4], key[4];
+        node_children(w2, w3, ch);
         const float cull = bt < (U)3.0e38 ? cull_distance((float)bt) + 1e-6f : INFINITY;
-        uint32_t mask = ol_node_hits(w0, w1, w2, w3, rp, 0.0f, cull, tn, ch);
+        ol_node_keys(w0, w1, w2, rp, 0.0f, cull, key, ch);
         uint32_t next = OL_NONE;
         for (int c = 0; c < 4; ++c) {
-          if (!(mask & (1u << c))) continue;
+          if (key[c] == OL_NONE) continue;
           if (child_is_leaf(ch[c])) {
             for (uint32_t k = 0; k < leaf_count(ch[c]); ++k) {
               const uint32_t slot = leaf_first(ch[c]) + k;

@@ -42,6 +42,7 @@ OL_DEV bool ol_tri_test(f3 o, f3 d, float4 a, float4 e1, float4 e2, float& t, fl
 
 struct RayPre {
   f3 o, d, inv;
+  uint32_t neg;  // bit a set when the direction's axis-a component is negative
 };
 
 OL_DEV RayPre ray_pre(f3 o, f3 d) {
@@ -50,41 +51,62 @@ OL_DEV RayPre ray_pre(f3 o, f3 d) {
   r.o = o;
   r.d = d;
   r.inv = mk3(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
+  r.neg = (r.inv.x < 0.0f ? 1u : 0u) | (r.inv.y < 0.0f ? 2u : 0u) | (r.inv.z < 0.0f ? 4u : 0u);
   return r;
 }
 
-// Slab test of the four quantised child boxes.  Each plane is decoded
-// exactly as the build rounded it (origin + q * 2^e, one fma, the product is
-// exact), then t = (plane - o) * inv: two roundings relative to t, so
-// widening tfar by 2^-20 keeps every box that contains a candidate hit.
-// Returns a 4-bit hit mask and entry distances.
-OL_DEV uint32_t ol_node_hits(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3, const RayPre& r,
-                             float tmin, float tmax, float tnear[4], uint32_t child[4]) {
-  const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
-  const float sx = exp_scale(w0.w & 255u), sy = exp_scale((w0.w >> 8) & 255u), sz = exp_scale((w0.w >> 16) & 255u);
-  const uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
-  child[0] = w2.z;
-  child[1] = w2.w;
-  child[2] = w3.x;
-  child[3] = w3.y;
-  uint32_t mask = 0;
+OL_DEV float byte_f(uint32_t w, int c) { return (float)((w >> (8 * c)) & 255u); }
+
+// Slab test of the four quantised child boxes of a node.
+//
+// Plane q of axis a lies at origin_a + q * 2^e_a; its ray distance is taken
+// as fma(q, 2^e_a * inv_a, (origin_a - o_a) * inv_a), three float32 roundings
+// of quantities no larger than the scene extent plus the origin distance.
+// The near plane of each axis is the low plane when the direction is
+// positive and the high plane otherwise, so each child costs six fma and
+// four min/max.  Every box was grown at build time by 2^-17 of the largest
+// scene coordinate, far more (in space) than these roundings, so a box that
+// contains a candidate hit is never culled.
+//
+// Returns one sort key per child: the entry distance's bits with the low two
+// bits replaced by the child's position, or OL_NONE when the child is empty
+// or missed.  Entry distances are >= tmin >= 0, so unsigned key order is
+// distance order.
+OL_DEV void ol_node_keys(const uint4 w0, const uint4 w1, const uint4 w2, const RayPre& r, float tmin, float tmax,
+                         uint32_t key[4], const uint32_t child[4]) {
+  const float ax = exp_scale(w0.w & 255u) * r.inv.x, bx = (__uint_as_float(w0.x) - r.o.x) * r.inv.x;
+  const float ay = exp_scale((w0.w >> 8) & 255u) * r.inv.y, by = (__uint_as_float(w0.y) - r.o.y) * r.inv.y;
+  const float az = exp_scale((w0.w >> 16) & 255u) * r.inv.z, bz = (__uint_as_float(w0.z) - r.o.z) * r.inv.z;
+  const uint32_t nx = (r.neg & 1u) ? w1.w : w1.x, fx = (r.neg & 1u) ? w1.x : w1.w;
+  const uint32_t ny = (r.neg & 2u) ? w2.x : w1.y, fy = (r.neg & 2u) ? w1.y : w2.x;
+  const uint32_t nz = (r.neg & 4u) ? w2.y : w1.z, fz = (r.neg & 4u) ? w1.z : w2.y;
+  const float t0 = fmaxf(tmin, 0.0f);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int sh = 8 * c;
-    float lx = (__builtin_fmaf((float)((qlx >> sh) & 255u), sx, ox) - r.o.x) * r.inv.x;
-    float hx = (__builtin_fmaf((float)((qhx >> sh) & 255u), sx, ox) - r.o.x) * r.inv.x;
-    float ly = (__builtin_fmaf((float)((qly >> sh) & 255u), sy, oy) - r.o.y) * r.inv.y;
-    float hy = (__builtin_fmaf((float)((qhy >> sh) & 255u), sy, oy) - r.o.y) * r.inv.y;
-    float lz = (__builtin_fmaf((float)((qlz >> sh) & 255u), sz, oz) - r.o.z) * r.inv.z;
-    float hz = (__builtin_fmaf((float)((qhz >> sh) & 255u), sz, oz) - r.o.z) * r.inv.z;
-    float tn = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fmaxf(fminf(lz, hz), tmin));
-    float tf = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmax));
-    tnear[c] = tn;
-    bool hit = child[c] != OL_NONE && tn <= tf * 1.00000095367431640625f;
-    mask |= hit ? (1u << c) : 0u;
+    const float tnx = __builtin_fmaf(byte_f(nx, c), ax, bx), tfx = __builtin_fmaf(byte_f(fx, c), ax, bx);
+    const float tny = __builtin_fmaf(byte_f(ny, c), ay, by), tfy = __builtin_fmaf(byte_f(fy, c), ay, by);
+    const float tnz = __builtin_fmaf(byte_f(nz, c), az, bz), tfz = __builtin_fmaf(byte_f(fz, c), az, bz);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, t0));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+    const bool hit = child[c] != OL_NONE && tn <= tf;
+    key[c] = hit ? ((__float_as_uint(tn) & ~3u) | (uint32_t)c) : OL_NONE;
   }
-  return mask;
 }
+
+OL_DEV void key_swap(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = min(a, b), hi = max(a, b);
+  a = lo;
+  b = hi;
+}
+// Ascending sort of four keys (five compare-exchanges).
+OL_DEV void sort4(uint32_t k[4]) {
+  key_swap(k[0], k[1]);
+  key_swap(k[2], k[3]);
+  key_swap(k[0], k[2]);
+  key_swap(k[1], k[3]);
+  key_swap(k[1], k[2]);
+}
+OL_DEV float key_distance(uint32_t key) { return __uint_as_float(key & ~3u); }
 
 // Culling distance for a current best hit at t: boxes entered beyond it are
 // skipped.  The 2^-16 relative slack absorbs the rounding of the
@@ -128,6 +150,14 @@ struct TravStack {
   OL_DEV bool empty() const { return sp == 0; }
 };
 
+// Children of a node in child-word order.
+OL_DEV void node_children(const uint4 w2, const uint4 w3, uint32_t ch[4]) {
+  ch[0] = w2.z;
+  ch[1] = w2.w;
+  ch[2] = w3.x;
+  ch[3] = w3.y;
+}
+
 // Closest hit.
 // nv / nt accumulate nodes visited and triangles tested (work counters).
 template <int LSTACK>
@@ -143,23 +173,22 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
     uint4 w0, w1, w2, w3;
     load_node(nodes, node, w0, w1, w2, w3);
     ++nv;
-    float tn[4];
-    uint32_t ch[4];
+    uint32_t ch[4], key[4];
+    node_children(w2, w3, ch);
     // Cull against the current best distance widened like the slab test.
-    uint32_t mask = ol_node_hits(w0, w1, w2, w3, r, tmin, cull_distance(best.t), tn, ch);
-    // Leaves first (order does not matter for the lexicographic minimum).
-    uint32_t inner_mask = 0;
+    ol_node_keys(w0, w1, w2, r, tmin, cull_distance(best.t), key, ch);
+    sort4(key);
+    // Leaves near to far (the lexicographic minimum does not depend on the
+    // order; nearer leaves shrink the culling distance sooner).
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (!(mask & (1u << c))) continue;
-      if (!child_is_leaf(ch[c])) {
-        inner_mask |= 1u << c;
-        continue;
-      }
-      const uint32_t first = leaf_first(ch[c]), cnt = leaf_count(ch[c]);
+    for (int k = 0; k < 4; ++k) {
+      if (key[k] == OL_NONE) continue;
+      const uint32_t c = ch[key[k] & 3u];
+      if (!child_is_leaf(c)) continue;
+      const uint32_t first = leaf_first(c), cnt = leaf_count(c);
       nt += cnt;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t slot = first + k;
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t slot = first + j;
         const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
         float t, u, v;
         if (ol_tri_test(o, d, a, e1, e2, t, u, v) && t >= tmin) {
@@ -168,38 +197,24 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
         }
       }
     }
-    // Inner children nearest first: push the others far to near.
-    float dist[4];
-    uint32_t idx[4];
-    int n = 0;
+    // Inner children: continue with the nearest, push the rest far to near,
+    // dropping those that now start beyond the culling distance.
+    const float cull = cull_distance(best.t);
+    uint32_t next = OL_NONE;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (inner_mask & (1u << c)) {
-        dist[n] = tn[c];
-        idx[n] = ch[c];
-        ++n;
-      }
+    for (int k = 3; k >= 0; --k) {
+      if (key[k] == OL_NONE) continue;
+      const uint32_t c = ch[key[k] & 3u];
+      if (child_is_leaf(c) || key_distance(key[k]) > cull) continue;
+      if (next != OL_NONE) st.push(next);
+      next = c;
     }
-    if (n == 0) {
-      if (st.empty()) break;
-      node = st.pop();
+    if (next != OL_NONE) {
+      node = next;
       continue;
     }
-    // Insertion sort of up to four (distance, node) pairs, ascending.
-    for (int i = 1; i < n; ++i) {
-      float dv = dist[i];
-      uint32_t iv = idx[i];
-      int j = i - 1;
-      while (j >= 0 && dist[j] > dv) {
-        dist[j + 1] = dist[j];
-        idx[j + 1] = idx[j];
-        --j;
-      }
-      dist[j + 1] = dv;
-      idx[j + 1] = iv;
-    }
-    for (int i = n - 1; i >= 1; --i) st.push(idx[i]);
-    node = idx[0];
+    if (st.empty()) break;
+    node = st.pop();
   }
   return best;
 }
@@ -217,13 +232,13 @@ OL_DEV bool ol_trace_any(const WideNode* __restrict__ nodes, const float4* __res
     uint4 w0, w1, w2, w3;
     load_node(nodes, node, w0, w1, w2, w3);
     ++nv;
-    float tn[4];
-    uint32_t ch[4];
-    uint32_t mask = ol_node_hits(w0, w1, w2, w3, r, tmin, tcull, tn, ch);
+    uint32_t ch[4], key[4];
+    node_children(w2, w3, ch);
+    ol_node_keys(w0, w1, w2, r, tmin, tcull, key, ch);
     uint32_t next = OL_NONE;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      if (!(mask & (1u << c))) continue;
+      if (key[c] == OL_NONE) continue;
       if (child_is_leaf(ch[c])) {
         const uint32_t first = leaf_first(ch[c]), cnt = leaf_count(ch[c]);
         for (uint32_t k = 0; k < cnt; ++k) {
