@@ -106,7 +106,7 @@ _SIGNATURES = {
     "olhip_frame_download": (ctypes.c_int, [ctypes.c_void_p, P_f32]),
     "olhip_bandwidth_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32, ctypes.POINTER(c_f64)]),
     "olhip_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "olhip_set_traversal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "olhip_set_traversal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "olhip_set_work_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "olhip_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P_f32, P_f32, c_u32, P_f32]),
     "olhip_e0_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(E0Camera), c_u32, c_u32,

@@ -81,6 +81,7 @@ struct olhip_ctx {
   int counting = 0;
   int lds_stack = 16;
   int trace_block = 128;
+  int trace_scheduler = 1;
   // GI frame buffers (grown on demand, reused across frames).
   DevBuf T, L, M, stk, qo0, qo1, qd0, qd1, qp0, qp1, hit, gT, gI, eo, ed, ec, ctr, stats, work, acc, image, spill;
   uint32_t image_pixels = 0;
@@ -152,15 +153,17 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled) {
   });
 }
 
-int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block) {
+int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block, int scheduler) {
   return guarded([&] {
     require(ctx != nullptr, "ctx must not be NULL");
+    require(scheduler == 0 || scheduler == 1, "scheduler must be 0 (grid-stride) or 1 (lane refill)");
     const int pairs[][2] = {{8, 128}, {12, 128}, {16, 64}, {16, 128}, {16, 256}, {24, 128}, {32, 128}, {48, 128}};
     bool ok = false;
     for (auto& p : pairs) ok = ok || (p[0] == lds_entries && p[1] == block);
     require(ok, "unsupported (LDS stack entries, block size): use (8|12|24|32|48, 128) or (16, 64|128|256)");
     ctx->lds_stack = lds_entries;
     ctx->trace_block = block;
+    ctx->trace_scheduler = scheduler;
   });
 }
 
@@ -296,6 +299,7 @@ GiLaunch make_launch(olhip_ctx* ctx, const olhip_scene* sc) {
   l.stack_depth = (int)(3u * std::max<uint32_t>(sc->bvh.depth, 1) + 1u);
   l.lds_stack = ctx->lds_stack;
   l.trace_block = ctx->trace_block;
+  l.scheduler = ctx->trace_scheduler;
   uint64_t words = trace_spill_words(l);
   if (words) ctx->spill.reserve(4 * words);
   l.spill = words ? ctx->spill.as<uint32_t>() : nullptr;

@@ -14,7 +14,10 @@
 constexpr uint32_t kStripes = 32;
 constexpr uint32_t kStripeUnit = 256;  // producer block size: inputs per stripe step
 constexpr uint32_t CTR_Q0 = 0, CTR_Q1 = kStripes, CTR_G = 2 * kStripes, CTR_E = 3 * kStripes;
-constexpr uint32_t CTR_OVERFLOW = 4 * kStripes, CTR_STACK = 4 * kStripes + 1, CTR_WORDS = 4 * kStripes + 8;
+// CTR_W: work cursors of the refilling closest-hit kernel, one per stripe of
+// its input (rays [s * n / kStripes, (s + 1) * n / kStripes)).
+constexpr uint32_t CTR_W = 4 * kStripes;
+constexpr uint32_t CTR_OVERFLOW = 5 * kStripes, CTR_STACK = 5 * kStripes + 1, CTR_WORDS = 5 * kStripes + 8;
 OL_HD constexpr uint32_t ctr_queue(uint32_t q) { return q ? CTR_Q1 : CTR_Q0; }
 
 // Per-path state, structure of arrays indexed by path id
@@ -65,6 +68,7 @@ struct GiLaunch {
   int stack_depth;   // total traversal stack entries needed (3 * BVH depth + 1)
   int lds_stack;     // entries kept in LDS: 8, 12, 16, 24, 32
   int trace_block;   // 64, 128, 256
+  int scheduler;     // closest hit: 0 one ray per lane per grid-stride trip, 1 lanes refilled from 64-ray chunks
   uint32_t* spill;   // global spill columns for entries beyond lds_stack
   uint64_t spill_words;
 };

@@ -158,65 +158,87 @@ OL_DEV void node_children(const uint4 w2, const uint4 w3, uint32_t ch[4]) {
   ch[3] = w3.y;
 }
 
-// Closest hit.
-// nv / nt accumulate nodes visited and triangles tested (work counters).
+// Closest-hit traversal state of one lane, advanced one node per step.
+struct ClosestRay {
+  RayPre r;
+  float tmin;
+  TraceHit best;
+  uint32_t node;  // next node to visit
+};
+
+OL_DEV void closest_begin(ClosestRay& q, f3 o, f3 d, float tmin, float tmax) {
+  q.r = ray_pre(o, d);
+  q.tmin = tmin;
+  q.best = TraceHit{tmax, 0.0f, 0.0f, OL_NONE, OL_NONE};
+  q.node = 0;
+}
+
+// Visits q.node: tests its leaf children (near to far), continues with the
+// nearest inner child and pushes the others far to near, dropping those
+// that start beyond the culling distance.  Returns true when the traversal
+// has finished (nothing left to visit).  nv / nt count nodes visited and
+// triangles tested (work counters).
+template <int LSTACK>
+OL_DEV bool closest_step(const WideNode* __restrict__ nodes, const float4* __restrict__ tri, ClosestRay& q,
+                         TravStack<LSTACK>& st, uint32_t& nv, uint32_t& nt) {
+  uint4 w0, w1, w2, w3;
+  load_node(nodes, q.node, w0, w1, w2, w3);
+  ++nv;
+  uint32_t ch[4], key[4];
+  node_children(w2, w3, ch);
+  // Cull against the current best distance widened like the slab test.
+  ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
+  sort4(key);
+  // The lexicographic minimum does not depend on the order; nearer leaves
+  // shrink the culling distance sooner.
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (key[k] == OL_NONE) continue;
+    const uint32_t c = ch[key[k] & 3u];
+    if (!child_is_leaf(c)) continue;
+    const uint32_t first = leaf_first(c), cnt = leaf_count(c);
+    nt += cnt;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t slot = first + j;
+      const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+      float t, u, v;
+      if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
+        const uint32_t orig = __float_as_uint(a.w);
+        if (t < q.best.t || (t == q.best.t && orig < q.best.orig)) q.best = TraceHit{t, u, v, slot, orig};
+      }
+    }
+  }
+  const float cull = cull_distance(q.best.t);
+  uint32_t next = OL_NONE;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (key[k] == OL_NONE) continue;
+    const uint32_t c = ch[key[k] & 3u];
+    if (child_is_leaf(c) || key_distance(key[k]) > cull) continue;
+    if (next != OL_NONE) st.push(next);
+    next = c;
+  }
+  if (next != OL_NONE) {
+    q.node = next;
+    return false;
+  }
+  if (st.empty()) return true;
+  q.node = st.pop();
+  return false;
+}
+
+// Closest hit of one ray, run to completion.
 template <int LSTACK>
 OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float4* __restrict__ tri,
                                  uint32_t node_count, f3 o, f3 d, float tmin, float tmax, TravStack<LSTACK>& st,
                                  uint32_t& nv, uint32_t& nt) {
-  TraceHit best{tmax, 0.0f, 0.0f, OL_NONE, OL_NONE};
-  if (node_count == 0) return best;
-  const RayPre r = ray_pre(o, d);
+  ClosestRay q;
+  closest_begin(q, o, d, tmin, tmax);
+  if (node_count == 0) return q.best;
   st.sp = 0;
-  uint32_t node = 0;
-  while (true) {
-    uint4 w0, w1, w2, w3;
-    load_node(nodes, node, w0, w1, w2, w3);
-    ++nv;
-    uint32_t ch[4], key[4];
-    node_children(w2, w3, ch);
-    // Cull against the current best distance widened like the slab test.
-    ol_node_keys(w0, w1, w2, r, tmin, cull_distance(best.t), key, ch);
-    sort4(key);
-    // Leaves near to far (the lexicographic minimum does not depend on the
-    // order; nearer leaves shrink the culling distance sooner).
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (key[k] == OL_NONE) continue;
-      const uint32_t c = ch[key[k] & 3u];
-      if (!child_is_leaf(c)) continue;
-      const uint32_t first = leaf_first(c), cnt = leaf_count(c);
-      nt += cnt;
-      for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t slot = first + j;
-        const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
-        float t, u, v;
-        if (ol_tri_test(o, d, a, e1, e2, t, u, v) && t >= tmin) {
-          const uint32_t orig = __float_as_uint(a.w);
-          if (t < best.t || (t == best.t && orig < best.orig)) best = TraceHit{t, u, v, slot, orig};
-        }
-      }
-    }
-    // Inner children: continue with the nearest, push the rest far to near,
-    // dropping those that now start beyond the culling distance.
-    const float cull = cull_distance(best.t);
-    uint32_t next = OL_NONE;
-#pragma unroll
-    for (int k = 3; k >= 0; --k) {
-      if (key[k] == OL_NONE) continue;
-      const uint32_t c = ch[key[k] & 3u];
-      if (child_is_leaf(c) || key_distance(key[k]) > cull) continue;
-      if (next != OL_NONE) st.push(next);
-      next = c;
-    }
-    if (next != OL_NONE) {
-      node = next;
-      continue;
-    }
-    if (st.empty()) break;
-    node = st.pop();
+  while (!closest_step<LSTACK>(nodes, tri, q, st, nv, nt)) {
   }
-  return best;
+  return q.best;
 }
 
 // Any hit in [tmin, tmax]: returns true as soon as one triangle qualifies.

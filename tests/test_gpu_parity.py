@@ -196,6 +196,24 @@ def test_gi_tile_capacity_invariance(hip_device):
     b.close()
 
 
+@pytest.mark.parametrize("leaf", [1, 2, 4])
+def test_gi_launch_shape_invariance(hip_device, leaf):
+    """Scheduler, LDS stack split and block size leave the image bit-identical."""
+    scene, cam = workloads.terrain(3, nx=161, nz=81, chunk=40, emissive_quads=6)
+    shapes = [(1, 16, 128), (0, 16, 128), (1, 8, 128), (1, 16, 64), (1, 32, 128)]
+    ref = None
+    for sched, lds, block in shapes:
+        log(f"launch shape leaf={leaf} scheduler={sched} lds={lds} block={block}")
+        r = HipPathTracer(max_bounces=6, samples_per_pixel=3, bvh_leaf_size=leaf, traversal_scheduler=sched,
+                          traversal_lds_stack=lds, traversal_block=block)
+        img = r.render_frame(scene, cam, 96, 64)
+        r.close()
+        if ref is None:
+            ref = img
+        else:
+            assert np.array_equal(img, ref), (sched, lds, block)
+
+
 def test_e0_room_matches_numpy_oracle(hip_device):
     scene, cam = workloads.box_room(0)
     log("e0 room")
