@@ -108,7 +108,8 @@ def main(argv=None):
     parser.add_argument("--leaf-size", type=int, default=2)
     parser.add_argument("--lds-stack", type=int, default=16)
     parser.add_argument("--block", type=int, default=128)
-    parser.add_argument("--scheduler", type=int, default=1, help="closest hit: 0 grid-stride, 1 lane refill")
+    parser.add_argument("--scheduler", type=int, default=2,
+                        help="closest hit: 0 grid-stride, 1 lane refill, 2 lane refill with split leaf steps")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "

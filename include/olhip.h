@@ -151,7 +151,8 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
 /* Traversal launch shape: stack entries kept in LDS per lane (the rest of
  * the 3 * depth + 1 entries spill to a global per-lane column), the block
  * size, and the closest-hit scheduler: 0 = one ray per lane per grid-stride
- * trip, 1 = lanes refilled from 64-ray chunks as their rays finish (default).
+ * trip, 1 = lanes refilled from 64-ray chunks as their rays finish, 2 = as 1
+ * with separate inner-node and leaf steps chosen per wave (default).
  * Supported shapes: (8|12|24|32|48, 128) and (16, 64|128|256).  Results do not
  * depend on these settings. */
 int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block, int scheduler);

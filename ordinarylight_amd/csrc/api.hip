@@ -81,7 +81,7 @@ struct olhip_ctx {
   int counting = 0;
   int lds_stack = 16;
   int trace_block = 128;
-  int trace_scheduler = 1;
+  int trace_scheduler = 2;
   // GI frame buffers (grown on demand, reused across frames).
   DevBuf T, L, M, stk, qo0, qo1, qd0, qd1, qp0, qp1, hit, gT, gI, eo, ed, ec, ctr, stats, work, acc, image, spill;
   uint32_t image_pixels = 0;
@@ -156,7 +156,8 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled) {
 int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block, int scheduler) {
   return guarded([&] {
     require(ctx != nullptr, "ctx must not be NULL");
-    require(scheduler == 0 || scheduler == 1, "scheduler must be 0 (grid-stride) or 1 (lane refill)");
+    require(scheduler >= 0 && scheduler <= 2,
+            "scheduler must be 0 (grid-stride), 1 (lane refill) or 2 (lane refill, split leaf steps)");
     const int pairs[][2] = {{8, 128}, {12, 128}, {16, 64}, {16, 128}, {16, 256}, {24, 128}, {32, 128}, {48, 128}};
     bool ok = false;
     for (auto& p : pairs) ok = ok || (p[0] == lds_entries && p[1] == block);

@@ -615,6 +615,87 @@ __global__ void __launch_bounds__(BLOCK) k_trace_closest_pool(DScene sc, GiState
   }
 }
 
+// Lane refill as in k_trace_closest_pool, with split inner / leaf steps: a
+// wave advances its lanes at leaves when at least a third of its busy lanes
+// hold one (or none can take an inner step), its lanes at inner nodes
+// otherwise.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_closest_split(DScene sc, GiState S, uint32_t in, uint32_t* spill,
+                                                               uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  __shared__ uint32_t incl[kStripes];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
+  stripe_prefix(S.ctr + ctr_queue(in), incl);
+  const uint32_t count = incl[kStripes - 1u];
+  const uint32_t ws = blockIdx.x % kStripes;
+  const uint32_t w_begin = (uint32_t)((uint64_t)count * ws / kStripes);
+  const uint32_t w_end = (uint32_t)((uint64_t)count * (ws + 1u) / kStripes);
+  uint32_t* cursor = &S.ctr[CTR_W + ws];
+  const float4* __restrict__ qo = S.qo[in];
+  const float4* __restrict__ qd = S.qd[in];
+  uint32_t pool = 0, pool_end = 0;
+  bool exhausted = false;
+  ClosestRay q;
+  q.node = OL_NONE;
+  uint32_t slot = OL_NONE;
+  bool busy = false;
+  uint32_t nv = 0, nt = 0;
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= kRefillMin) {
+      if (pool == pool_end && !exhausted) {
+        uint32_t c = 0;
+        if (lane_id() == 0) c = atomicAdd(cursor, kChunk);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        const uint32_t remaining = w_end - w_begin;
+        if (c >= remaining) {
+          exhausted = true;
+        } else {
+          pool = w_begin + c;
+          pool_end = w_begin + min(c + kChunk, remaining);
+        }
+      }
+      const uint32_t take = min(n_idle, pool_end - pool);
+      if (!busy) {
+        const uint32_t rank = lanes_below(idle);
+        if (rank < take) {
+          slot = stripe_slot(incl, S.qseg, pool + rank);
+          const float4 o = qo[slot], d = qd[slot];
+          closest_begin(q, mk3(o), mk3(d), o.w, d.w);
+          if (sc.node_count == 0) {
+            S.hit[slot] = make_float4(d.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
+          } else {
+            st.sp = 0;
+            busy = true;
+          }
+        }
+      }
+      pool += take;
+    }
+    const bool at_leaf = busy && item_is_leaf(q.node);
+    const uint64_t leafy = __ballot(at_leaf);
+    const uint64_t inner = __ballot(busy && !at_leaf);
+    if ((leafy | inner) == 0) {
+      if (exhausted && pool == pool_end) break;
+      continue;
+    }
+    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+      if (at_leaf) closest_leaf<LSTACK>(sc.tri, q, st, nt);
+    } else if (busy && !at_leaf) {
+      closest_inner<LSTACK>(sc.nodes, q, st, nv);
+    }
+    if (busy && q.node == OL_NONE) {
+      S.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
+      busy = false;
+    }
+  }
+  if (S.work && (nv | nt)) {
+    atomicAdd(&S.work[0], (unsigned long long)nv);
+    atomicAdd(&S.work[1], (unsigned long long)nt);
+  }
+}
+
 template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o4, const float4* d4, uint32_t n,
                                                       float4* hits, uint32_t* overflow, unsigned long long* work,
@@ -971,7 +1052,8 @@ void check_spill(const GiLaunch& l, uint64_t threads) {
 
 template <int LSTACK, int BLOCK>
 uint64_t spill_words_for(const GiLaunch& l) {
-  int occ = std::max(std::max(resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK),
+  int occ = std::max(std::max(std::max(resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK),
+                                       resident_blocks(k_trace_closest_split<LSTACK, BLOCK>, BLOCK)),
                               resident_blocks(k_trace_closest_pool<LSTACK, BLOCK>, BLOCK)),
                      std::max(resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK),
                               resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK)));
@@ -980,6 +1062,14 @@ uint64_t spill_words_for(const GiLaunch& l) {
 
 template <int LSTACK, int BLOCK>
 void launch_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint32_t in, hipStream_t st) {
+  if (l.scheduler == 2) {
+    static int occ = resident_blocks(k_trace_closest_split<LSTACK, BLOCK>, BLOCK);
+    dim3 grid((unsigned)(l.cu_count * occ));
+    check_spill(l, (uint64_t)grid.x * BLOCK);
+    hipLaunchKernelGGL((k_trace_closest_split<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, in, l.spill,
+                       spill_cap_of(l));
+    return;
+  }
   if (l.scheduler == 1) {
     static int occ = resident_blocks(k_trace_closest_pool<LSTACK, BLOCK>, BLOCK);
     dim3 grid((unsigned)(l.cu_count * occ));

@@ -227,6 +227,56 @@ OL_DEV bool closest_step(const WideNode* __restrict__ nodes, const float4* __res
   return false;
 }
 
+// Split-step form used by the refilling scheduler.  q.node holds the next
+// item to visit: an inner node index, a leaf word (OL_LEAF_BIT set), or
+// OL_NONE once the traversal has finished.  A wave chooses per step whether
+// its lanes at inner nodes or its lanes at leaves advance, so the triangle
+// tests of a leaf run with many lanes active instead of stalling the box
+// tests of the others.
+OL_DEV bool item_is_leaf(uint32_t item) { return child_is_leaf(item); }
+
+template <int LSTACK>
+OL_DEV void closest_next(ClosestRay& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
+
+// Inner step: continue with the nearest hit child (inner or leaf) and push the
+// others far to near.
+template <int LSTACK>
+OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, TravStack<LSTACK>& st, uint32_t& nv) {
+  uint4 w0, w1, w2, w3;
+  load_node(nodes, q.node, w0, w1, w2, w3);
+  ++nv;
+  uint32_t ch[4], key[4];
+  node_children(w2, w3, ch);
+  ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
+  sort4(key);
+  uint32_t next = OL_NONE;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (key[k] == OL_NONE) continue;
+    if (next != OL_NONE) st.push(next);
+    next = ch[key[k] & 3u];
+  }
+  if (next != OL_NONE) q.node = next;
+  else closest_next(q, st);
+}
+
+// Leaf step: test the leaf's triangles, then take the next stack entry.
+template <int LSTACK>
+OL_DEV void closest_leaf(const float4* __restrict__ tri, ClosestRay& q, TravStack<LSTACK>& st, uint32_t& nt) {
+  const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
+  nt += cnt;
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint32_t slot = first + j;
+    const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+    float t, u, v;
+    if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
+      const uint32_t orig = __float_as_uint(a.w);
+      if (t < q.best.t || (t == q.best.t && orig < q.best.orig)) q.best = TraceHit{t, u, v, slot, orig};
+    }
+  }
+  closest_next(q, st);
+}
+
 // Closest hit of one ray, run to completion.
 template <int LSTACK>
 OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float4* __restrict__ tri,

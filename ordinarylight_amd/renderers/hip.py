@@ -53,10 +53,11 @@ class HipRendererConfig:
     device_index: int = 0
     bvh_leaf_size: int = 2
     # Traversal stack entries kept in LDS per lane, the traversal block size,
-    # and the closest-hit scheduler (0 grid-stride, 1 lane refill).
+    # and the closest-hit scheduler (0 grid-stride, 1 lane refill, 2 lane
+    # refill with separate inner-node and leaf steps).
     traversal_lds_stack: int = 16
     traversal_block: int = 128
-    traversal_scheduler: int = 1
+    traversal_scheduler: int = 2
     profiling: bool = False
     work_counters: bool = False
 
@@ -77,8 +78,8 @@ class HipRendererConfig:
             raise ValueError("wavefront_tile_capacity must be positive")
         if not 0 <= int(self.bvh_leaf_size) <= 16:
             raise ValueError("bvh_leaf_size must be between 0 and 16")
-        if int(self.traversal_scheduler) not in (0, 1):
-            raise ValueError("traversal_scheduler must be 0 or 1")
+        if int(self.traversal_scheduler) not in (0, 1, 2):
+            raise ValueError("traversal_scheduler must be 0, 1 or 2")
 
     def gi_settings(self):
         s = N.GiSettings()

@@ -200,7 +200,7 @@ def test_gi_tile_capacity_invariance(hip_device):
 def test_gi_launch_shape_invariance(hip_device, leaf):
     """Scheduler, LDS stack split and block size leave the image bit-identical."""
     scene, cam = workloads.terrain(3, nx=161, nz=81, chunk=40, emissive_quads=6)
-    shapes = [(1, 16, 128), (0, 16, 128), (1, 8, 128), (1, 16, 64), (1, 32, 128)]
+    shapes = [(1, 16, 128), (0, 16, 128), (2, 16, 128), (1, 8, 128), (1, 16, 64), (1, 32, 128)]
     ref = None
     for sched, lds, block in shapes:
         log(f"launch shape leaf={leaf} scheduler={sched} lds={lds} block={block}")
