@@ -16,8 +16,9 @@ constexpr uint32_t kStripeUnit = 256;  // producer block size: inputs per stripe
 constexpr uint32_t CTR_Q0 = 0, CTR_Q1 = kStripes, CTR_G = 2 * kStripes, CTR_E = 3 * kStripes;
 // CTR_W: work cursors of the refilling closest-hit kernel, one per stripe of
 // its input (rays [s * n / kStripes, (s + 1) * n / kStripes)).
-constexpr uint32_t CTR_W = 4 * kStripes;
-constexpr uint32_t CTR_OVERFLOW = 5 * kStripes, CTR_STACK = 5 * kStripes + 1, CTR_WORDS = 5 * kStripes + 8;
+// CTR_WS: the same for the shadow kernel's groups.
+constexpr uint32_t CTR_W = 4 * kStripes, CTR_WS = 5 * kStripes;
+constexpr uint32_t CTR_OVERFLOW = 6 * kStripes, CTR_STACK = 6 * kStripes + 1, CTR_WORDS = 6 * kStripes + 8;
 OL_HD constexpr uint32_t ctr_queue(uint32_t q) { return q ? CTR_Q1 : CTR_Q0; }
 
 // Per-path state, structure of arrays indexed by path id

@@ -483,6 +483,7 @@ __global__ void __launch_bounds__(64) k_prologue(GiState S, uint32_t bounce) {
     S.ctr[CTR_G + k] = 0u;
     S.ctr[CTR_E + k] = 0u;
     S.ctr[CTR_W + k] = 0u;
+    S.ctr[CTR_WS + k] = 0u;
   }
 }
 
@@ -744,6 +745,127 @@ __global__ void __launch_bounds__(BLOCK) k_trace_shadow(DScene sc, GiState S, ui
     float4 L = S.L[path];
     f3 add = (rgb(gT) * direct) / fmaxf(gT.w, 1e-6f);
     S.L[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, L.w);
+  }
+  if (S.work && (nv | nt)) {
+    atomicAdd(&S.work[2], (unsigned long long)nv);
+    atomicAdd(&S.work[3], (unsigned long long)nt);
+  }
+}
+
+// Adds one visible shadow entry's contribution (the summation order of
+// k_trace_shadow).
+OL_DEV void shadow_add(uint32_t mode, float4 c, f3& direct, f3& area_direct, f3& env_direct) {
+  const uint32_t kind = (uint32_t)c.w;
+  if (mode == 0u || kind == 0u) direct = direct + rgb(c);
+  else if (kind == 1u) area_direct = area_direct + rgb(c);
+  else env_direct = env_direct + rgb(c);
+}
+
+OL_DEV void shadow_resolve(const GiState& S, uint32_t g, f3 direct, f3 area_direct, f3 env_direct) {
+  const float4 gT = S.gT[g];
+  const uint4 gI = S.gI[g];
+  const uint32_t path = gI.x, area_n = gI.w & 255u, env_n = (gI.w >> 8) & 255u, mode = gI.w >> 16;
+  if (mode != 0u) {
+    direct = direct + area_direct / (float)area_n;
+    if (env_n > 0u) direct = direct + env_direct / (float)env_n;
+  }
+  float4 L = S.L[path];
+  f3 add = (rgb(gT) * direct) / fmaxf(gT.w, 1e-6f);
+  S.L[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, L.w);
+}
+
+// Shadow groups with lane refill and split inner / leaf steps (the
+// scheduling of k_trace_closest_split): a lane owns one group and traces its
+// entries in order, accumulating visible contributions as k_trace_shadow does.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_shadow_split(DScene sc, GiState S, uint32_t* spill,
+                                                              uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  __shared__ uint32_t incl[kStripes];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
+  stripe_prefix(S.ctr + CTR_G, incl);
+  const uint32_t count = incl[kStripes - 1u];
+  const uint32_t ws = blockIdx.x % kStripes;
+  const uint32_t w_begin = (uint32_t)((uint64_t)count * ws / kStripes);
+  const uint32_t w_end = (uint32_t)((uint64_t)count * (ws + 1u) / kStripes);
+  uint32_t* cursor = &S.ctr[CTR_WS + ws];
+  uint32_t pool = 0, pool_end = 0;
+  bool exhausted = false;
+  AnyRay q;
+  q.node = OL_NONE;
+  q.occluded = false;
+  uint32_t g = 0, e = 0, e_end = 0, mode = 0;
+  f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
+  bool busy = false;
+  uint32_t nv = 0, nt = 0;
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= kRefillMin) {
+      if (pool == pool_end && !exhausted) {
+        uint32_t c = 0;
+        if (lane_id() == 0) c = atomicAdd(cursor, kChunk);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        const uint32_t remaining = w_end - w_begin;
+        if (c >= remaining) {
+          exhausted = true;
+        } else {
+          pool = w_begin + c;
+          pool_end = w_begin + min(c + kChunk, remaining);
+        }
+      }
+      const uint32_t take = min(n_idle, pool_end - pool);
+      if (!busy) {
+        const uint32_t rank = lanes_below(idle);
+        if (rank < take) {
+          g = stripe_slot(incl, S.gseg, pool + rank);
+          const uint4 gI = S.gI[g];
+          e = gI.y;
+          e_end = gI.y + gI.z;
+          mode = gI.w >> 16;
+          direct = mk3(0.0f, 0.0f, 0.0f);
+          area_direct = direct;
+          env_direct = direct;
+          if (sc.node_count == 0) {
+            // Nothing can occlude: every entry is visible.
+            for (; e < e_end; ++e) shadow_add(mode, S.ec[e], direct, area_direct, env_direct);
+            shadow_resolve(S, g, direct, area_direct, env_direct);
+          } else if (e < e_end) {
+            const float4 o = S.eo[e], d = S.ed[e];
+            any_begin(q, mk3(o), mk3(d), o.w, d.w);
+            st.sp = 0;
+            busy = true;
+          } else {
+            shadow_resolve(S, g, direct, area_direct, env_direct);
+          }
+        }
+      }
+      pool += take;
+    }
+    const bool at_leaf = busy && item_is_leaf(q.node);
+    const uint64_t leafy = __ballot(at_leaf);
+    const uint64_t inner = __ballot(busy && !at_leaf);
+    if ((leafy | inner) == 0) {
+      if (exhausted && pool == pool_end) break;
+      continue;
+    }
+    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+      if (at_leaf) any_leaf<LSTACK>(sc.tri, q, st, nt);
+    } else if (busy && !at_leaf) {
+      any_inner<LSTACK>(sc.nodes, q, st, nv);
+    }
+    if (busy && q.node == OL_NONE) {
+      if (!q.occluded) shadow_add(mode, S.ec[e], direct, area_direct, env_direct);
+      ++e;
+      if (e < e_end) {
+        const float4 o = S.eo[e], d = S.ed[e];
+        any_begin(q, mk3(o), mk3(d), o.w, d.w);
+        st.sp = 0;
+      } else {
+        shadow_resolve(S, g, direct, area_direct, env_direct);
+        busy = false;
+      }
+    }
   }
   if (S.work && (nv | nt)) {
     atomicAdd(&S.work[2], (unsigned long long)nv);
@@ -1055,7 +1177,8 @@ uint64_t spill_words_for(const GiLaunch& l) {
   int occ = std::max(std::max(std::max(resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK),
                                        resident_blocks(k_trace_closest_split<LSTACK, BLOCK>, BLOCK)),
                               resident_blocks(k_trace_closest_pool<LSTACK, BLOCK>, BLOCK)),
-                     std::max(resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK),
+                     std::max(std::max(resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK),
+                                       resident_blocks(k_trace_shadow_split<LSTACK, BLOCK>, BLOCK)),
                               resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK)));
   return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
@@ -1086,6 +1209,14 @@ void launch_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint3
 
 template <int LSTACK, int BLOCK>
 void launch_shadow(const GiLaunch& l, const DScene& sc, const GiState& s, hipStream_t st) {
+  if (l.scheduler == 2) {
+    static int occ2 = resident_blocks(k_trace_shadow_split<LSTACK, BLOCK>, BLOCK);
+    dim3 grid((unsigned)(l.cu_count * occ2));
+    check_spill(l, (uint64_t)grid.x * BLOCK);
+    hipLaunchKernelGGL((k_trace_shadow_split<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, l.spill,
+                       spill_cap_of(l));
+    return;
+  }
   static int occ = resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
   check_spill(l, (uint64_t)grid.x * BLOCK);

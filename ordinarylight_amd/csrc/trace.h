@@ -277,6 +277,65 @@ OL_DEV void closest_leaf(const float4* __restrict__ tri, ClosestRay& q, TravStac
   closest_next(q, st);
 }
 
+// Any-hit traversal state of one lane in split-step form (see above).
+// q.node is OL_NONE once finished; occluded tells whether a triangle in
+// [tmin, tmax] was found.
+struct AnyRay {
+  RayPre r;
+  float tmin, tmax, tcull;
+  uint32_t node;
+  bool occluded;
+};
+
+OL_DEV void any_begin(AnyRay& q, f3 o, f3 d, float tmin, float tmax) {
+  q.r = ray_pre(o, d);
+  q.tmin = tmin;
+  q.tmax = tmax;
+  q.tcull = cull_distance(tmax);
+  q.node = 0;
+  q.occluded = false;
+}
+
+template <int LSTACK>
+OL_DEV void any_next(AnyRay& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
+
+template <int LSTACK>
+OL_DEV void any_inner(const WideNode* __restrict__ nodes, AnyRay& q, TravStack<LSTACK>& st, uint32_t& nv) {
+  uint4 w0, w1, w2, w3;
+  load_node(nodes, q.node, w0, w1, w2, w3);
+  ++nv;
+  uint32_t ch[4], key[4];
+  node_children(w2, w3, ch);
+  ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
+  sort4(key);
+  uint32_t next = OL_NONE;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (key[k] == OL_NONE) continue;
+    if (next != OL_NONE) st.push(next);
+    next = ch[key[k] & 3u];
+  }
+  if (next != OL_NONE) q.node = next;
+  else any_next(q, st);
+}
+
+template <int LSTACK>
+OL_DEV void any_leaf(const float4* __restrict__ tri, AnyRay& q, TravStack<LSTACK>& st, uint32_t& nt) {
+  const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint32_t slot = first + j;
+    const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+    float t, u, v;
+    ++nt;
+    if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin && t <= q.tmax) {
+      q.occluded = true;
+      q.node = OL_NONE;
+      return;
+    }
+  }
+  any_next(q, st);
+}
+
 // Closest hit of one ray, run to completion.
 template <int LSTACK>
 OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float4* __restrict__ tri,
