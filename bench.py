@@ -108,8 +108,6 @@ def main(argv=None):
     parser.add_argument("--leaf-size", type=int, default=2)
     parser.add_argument("--lds-stack", type=int, default=16)
     parser.add_argument("--block", type=int, default=128)
-    parser.add_argument("--scheduler", type=int, default=2,
-                        help="closest hit: 0 grid-stride, 1 lane refill, 2 lane refill with split leaf steps")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -125,7 +123,7 @@ def main(argv=None):
     W, H, spp, B = args.width, args.height, args.spp, args.bounces
     renderer = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
                              bvh_leaf_size=args.leaf_size, traversal_lds_stack=args.lds_stack,
-                             traversal_block=args.block, traversal_scheduler=args.scheduler)
+                             traversal_block=args.block)
     # Scene upload + BVH build (reported separately, outside the timed region).
     renderer.replace_scene(scene)
     ds = renderer._device_scene

@@ -138,6 +138,74 @@ typedef struct olhip_stats {
 int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* scene, const olhip_camera* camera, uint32_t width,
                     uint32_t height, uint32_t samples, uint32_t frame_index, const olhip_gi_settings* settings,
                     float* out_rgba, olhip_stats* stats);
+/* ---- batched render engine (continuous batching) ----
+ * Many jobs of different scenes, sizes and sample counts share one pool of
+ * path slots and one set of ray/shadow queues; each step advances every
+ * resident path by one bounce.  Jobs are admitted first-come first-served
+ * when a slot range for their next sample chunk is free (admission by
+ * memory), so a job may join while others are in flight.  A job's image is
+ * bit-identical to olhip_render_gi of the same job: each path depends only
+ * on (pixel, sample, frame) and the float64 pixel sums run in sample order.
+ * No counterpart in the reference (its executor renders one frame at a time,
+ * targets/vulkan/core.py:2338-3215). */
+typedef struct olhip_engine olhip_engine;
+
+typedef struct olhip_engine_config {
+  uint64_t path_slots;    /* resident path pool; 0 = 2^25 */
+  uint32_t entry_factor;  /* shadow-entry slots per path slot; 0 = 2 */
+  int32_t lds_stack;      /* traversal stack entries in LDS; 0 = context setting */
+  int32_t trace_block;    /* traversal block size; 0 = context setting */
+  int32_t use_graph;      /* replay each step as a captured hipGraph (1) or launch directly (0) */
+} olhip_engine_config;
+
+typedef struct olhip_job_desc {
+  const olhip_scene* scene;
+  olhip_camera camera;
+  uint32_t width, height, samples, frame_index;
+  olhip_gi_settings settings;  /* path_capacity: most paths per sample chunk, 0 = no limit */
+} olhip_job_desc;
+
+enum {
+  OLHIP_JOB_UNKNOWN = -1,
+  OLHIP_JOB_QUEUED = 0,
+  OLHIP_JOB_RUNNING = 1,
+  OLHIP_JOB_DONE = 2,
+  OLHIP_JOB_FAILED = 3
+};
+
+typedef struct olhip_job_stats {
+  uint64_t primary_rays, extension_rays, shadow_rays;
+  uint32_t steps, chunks;
+  double submit_ms, admit_ms, done_ms;  /* engine clock: ms since the engine was created */
+} olhip_job_stats;
+
+typedef struct olhip_engine_stats {
+  uint64_t steps, jobs_done, rays, queue_overflow;
+  double busy_ms;  /* host time spent inside steps */
+  uint32_t resident_jobs, queued_jobs, max_resident_jobs, pad;
+  uint64_t path_slots, entry_slots;
+} olhip_engine_stats;
+
+int olhip_engine_create(olhip_ctx* ctx, const olhip_engine_config* config, olhip_engine** out);
+int olhip_engine_destroy(olhip_engine* engine);
+/* Queue a job (the scene must outlive it).  Thread-safe. */
+int olhip_engine_submit(olhip_engine* engine, const olhip_job_desc* job, uint64_t* job_id);
+/* Run one step; *pending = jobs not yet done.  Not with a running worker. */
+int olhip_engine_step(olhip_engine* engine, uint32_t* pending);
+/* Step until every submitted job is done.  Not with a running worker. */
+int olhip_engine_run(olhip_engine* engine);
+/* Job state (OLHIP_JOB_*).  Thread-safe. */
+int olhip_engine_poll(olhip_engine* engine, uint64_t job_id, int* state);
+/* Wait for a job (timeout_s < 0: no limit); *state = final or current state. */
+int olhip_engine_wait(olhip_engine* engine, uint64_t job_id, double timeout_s, int* state);
+/* Copy a done job's float32 [H][W][4] image (out_rgba may be NULL) and
+ * statistics, then release it. */
+int olhip_engine_result(olhip_engine* engine, uint64_t job_id, float* out_rgba, olhip_job_stats* stats);
+/* Background worker that steps whenever work is queued or resident. */
+int olhip_engine_start(olhip_engine* engine);
+int olhip_engine_stop(olhip_engine* engine);
+int olhip_engine_get_stats(olhip_engine* engine, olhip_engine_stats* out);
+
 /* Streaming copy of `bytes` (read + write) repeated `repeats` times on the
  * context's stream; reports milliseconds per copy.  Used to measure
  * achievable HBM bandwidth and to calibrate hardware byte counters. */
@@ -149,13 +217,10 @@ int olhip_set_profiling(olhip_ctx* ctx, int enabled);
 /* Count traversal work (nodes visited, triangles tested) in olhip_stats. */
 int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
 /* Traversal launch shape: stack entries kept in LDS per lane (the rest of
- * the 3 * depth + 1 entries spill to a global per-lane column), the block
- * size, and the closest-hit scheduler: 0 = one ray per lane per grid-stride
- * trip, 1 = lanes refilled from 64-ray chunks as their rays finish, 2 = as 1
- * with separate inner-node and leaf steps chosen per wave (default).
- * Supported shapes: (8|12|24|32|48, 128) and (16, 64|128|256).  Results do not
- * depend on these settings. */
-int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block, int scheduler);
+ * the 3 * depth + 1 entries spill to a global per-lane column) and the block
+ * size.  Supported: (8|12|24|32|48, 128) and (16, 64|128|256).  Results do
+ * not depend on these settings. */
+int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block);
 
 /* Closest hit of caller rays: origins/directions [n][4] (xyz, tmin / tmax).
  * hits [n][4]: t, u, v, original triangle index as uint32 bits (0xFFFFFFFF

@@ -77,6 +77,45 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class EngineConfig(ctypes.Structure):
+    _fields_ = [
+        ("path_slots", ctypes.c_uint64), ("entry_factor", c_u32), ("lds_stack", ctypes.c_int32),
+        ("trace_block", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+    ]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [
+        ("scene", ctypes.c_void_p), ("camera", Camera), ("width", c_u32), ("height", c_u32),
+        ("samples", c_u32), ("frame_index", c_u32), ("settings", GiSettings),
+    ]
+
+
+class JobStats(ctypes.Structure):
+    _fields_ = [
+        ("primary_rays", ctypes.c_uint64), ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+        ("steps", c_u32), ("chunks", c_u32), ("submit_ms", c_f64), ("admit_ms", c_f64), ("done_ms", c_f64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class EngineStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_uint64), ("jobs_done", ctypes.c_uint64), ("rays", ctypes.c_uint64),
+        ("queue_overflow", ctypes.c_uint64), ("busy_ms", c_f64), ("resident_jobs", c_u32),
+        ("queued_jobs", c_u32), ("max_resident_jobs", c_u32), ("pad", c_u32),
+        ("path_slots", ctypes.c_uint64), ("entry_slots", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
+JOB_UNKNOWN, JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = -1, 0, 1, 2, 3
+
+
 class E0Camera(ctypes.Structure):
     _fields_ = [
         ("position", c_f32 * 3), ("forward", c_f32 * 3), ("right", c_f32 * 3),
@@ -106,7 +145,20 @@ _SIGNATURES = {
     "olhip_frame_download": (ctypes.c_int, [ctypes.c_void_p, P_f32]),
     "olhip_bandwidth_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32, ctypes.POINTER(c_f64)]),
     "olhip_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "olhip_set_traversal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "olhip_set_traversal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "olhip_engine_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EngineConfig),
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "olhip_engine_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "olhip_engine_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(JobDesc),
+                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "olhip_engine_step": (ctypes.c_int, [ctypes.c_void_p, P_u32]),
+    "olhip_engine_run": (ctypes.c_int, [ctypes.c_void_p]),
+    "olhip_engine_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "olhip_engine_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_f64, ctypes.POINTER(ctypes.c_int)]),
+    "olhip_engine_result": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, P_f32, ctypes.POINTER(JobStats)]),
+    "olhip_engine_start": (ctypes.c_int, [ctypes.c_void_p]),
+    "olhip_engine_stop": (ctypes.c_int, [ctypes.c_void_p]),
+    "olhip_engine_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EngineStats)]),
     "olhip_set_work_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "olhip_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P_f32, P_f32, c_u32, P_f32]),
     "olhip_e0_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(E0Camera), c_u32, c_u32,

@@ -9,6 +9,7 @@
 
 #include "../../include/olhip.h"
 #include "e0.h"
+#include "engine.h"
 #include "gi.h"
 #include "internal.h"
 
@@ -62,41 +63,6 @@ float elapsed(const Event& a, const Event& b) {
 
 }  // namespace
 
-struct olhip_scene {
-  olhip_ctx* ctx = nullptr;
-  DevBuf positions, normals, vidx, materials, analytic, area;
-  BvhOutput bvh;
-  DScene view{};
-  uint32_t tri_count = 0, vertex_count = 0;
-  bool has_transmission = false;
-  uint32_t point_count = 0;
-  std::vector<float> materials_host;
-};
-
-struct olhip_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int cu_count = 0;
-  int profiling = 0;
-  int counting = 0;
-  int lds_stack = 16;
-  int trace_block = 128;
-  int trace_scheduler = 2;
-  // GI frame buffers (grown on demand, reused across frames).
-  DevBuf T, L, M, stk, qo0, qo1, qd0, qd1, qp0, qp1, hit, gT, gI, eo, ed, ec, ctr, stats, work, acc, image, spill;
-  uint32_t image_pixels = 0;
-  // Reference-estimator frame.
-  struct E0Frame {
-    DevBuf o, d, T, rad, acc, active, flags, scan, ht, htri, hitlist, hit_t, hit_tri, jx, jy, gauss, env, ctr, out;
-    E0Params params{};
-    E0State state{};
-    const olhip_scene* scene = nullptr;
-    uint32_t n_active = 0, n_hit = 0;
-    int stack = 32;
-    bool open = false;
-  } e0;
-};
-
 extern "C" {
 
 const char* olhip_last_error(void) { return g_last_error.c_str(); }
@@ -133,6 +99,7 @@ int olhip_destroy(olhip_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    ctx->frame_engine.reset();
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -153,18 +120,16 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled) {
   });
 }
 
-int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block, int scheduler) {
+int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block) {
   return guarded([&] {
     require(ctx != nullptr, "ctx must not be NULL");
-    require(scheduler >= 0 && scheduler <= 2,
-            "scheduler must be 0 (grid-stride), 1 (lane refill) or 2 (lane refill, split leaf steps)");
     const int pairs[][2] = {{8, 128}, {12, 128}, {16, 64}, {16, 128}, {16, 256}, {24, 128}, {32, 128}, {48, 128}};
     bool ok = false;
     for (auto& p : pairs) ok = ok || (p[0] == lds_entries && p[1] == block);
     require(ok, "unsupported (LDS stack entries, block size): use (8|12|24|32|48, 128) or (16, 64|128|256)");
+    if (ctx->lds_stack != lds_entries || ctx->trace_block != block) ctx->frame_engine.reset();
     ctx->lds_stack = lds_entries;
     ctx->trace_block = block;
-    ctx->trace_scheduler = scheduler;
   });
 }
 
@@ -300,7 +265,6 @@ GiLaunch make_launch(olhip_ctx* ctx, const olhip_scene* sc) {
   l.stack_depth = (int)(3u * std::max<uint32_t>(sc->bvh.depth, 1) + 1u);
   l.lds_stack = ctx->lds_stack;
   l.trace_block = ctx->trace_block;
-  l.scheduler = ctx->trace_scheduler;
   uint64_t words = trace_spill_words(l);
   if (words) ctx->spill.reserve(4 * words);
   l.spill = words ? ctx->spill.as<uint32_t>() : nullptr;
@@ -320,182 +284,167 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     require(sc->ctx == ctx, "scene belongs to another context");
     require(width >= 1 && height >= 1, "width and height must be positive");
     require((uint64_t)width * height <= (1ull << 28), "image too large");
-    require(samples >= 1 && samples <= 65536, "samples must be between 1 and 65536");
-    require(set->max_bounces >= 1 && set->max_bounces <= 64, "max_bounces must be between 1 and 64");
-    require(cam->projection <= 2, "unknown projection");
-    require(set->secondary_nee_probability > 0.0f && set->secondary_nee_probability <= 1.0f,
-            "secondary_nee_probability must be in (0, 1]");
-    require(set->russian_roulette_min_survival >= 0.0f && set->russian_roulette_min_survival <= 1.0f,
-            "russian_roulette_min_survival must be in [0, 1]");
     OL_HIP_CHECK(hipSetDevice(ctx->device));
-    hipStream_t s = ctx->stream;
-    const uint32_t np = width * height;
-    uint64_t capacity = set->path_capacity ? set->path_capacity : (1ull << 25);
-    uint32_t chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, capacity / np));
-    if ((uint64_t)chunk * np > (1ull << 31)) chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 31) / np);
-    const uint64_t paths = (uint64_t)chunk * np;
-    const bool two_pass = sc->point_count > (sc->view.env_valid ? 1u : 0u) || !set->unified_secondary_nee;
-    uint32_t per_hit = 1;
-    if (two_pass) per_hit = sc->point_count + (set->unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(set->secondary_area_samples, 1u), 16u) + 4u));
-    // Striped queues: kStripes segments of qseg slots (see gi.h).
-    const uint64_t unit = (uint64_t)kStripeUnit * kStripes;
-    const uint64_t qseg = (paths + unit - 1) / unit * kStripeUnit;
-    const uint64_t qslots = qseg * kStripes;
-    const uint64_t eseg = qseg * per_hit, eslots = eseg * kStripes;
-    if (eslots > 0xFFFFFFFFull)
-      throw OlError{OLHIP_ERR_MEMORY, "shadow queue of " + std::to_string(eslots) +
-                                          " entries exceeds 2^32; lower path_capacity"};
-    // Buffers.
-    ctx->T.reserve(16 * paths);
-    ctx->L.reserve(16 * paths);
-    ctx->M.reserve(16 * paths);
-    if (sc->has_transmission) ctx->stk.reserve(64 * paths);
-    ctx->qo0.reserve(16 * qslots);
-    ctx->qo1.reserve(16 * qslots);
-    ctx->qd0.reserve(16 * qslots);
-    ctx->qd1.reserve(16 * qslots);
-    ctx->qp0.reserve(4 * qslots);
-    ctx->qp1.reserve(4 * qslots);
-    ctx->hit.reserve(16 * qslots);
-    ctx->gT.reserve(16 * qslots);
-    ctx->gI.reserve(16 * qslots);
-    ctx->eo.reserve(16 * eslots);
-    ctx->ed.reserve(16 * eslots);
-    ctx->ec.reserve(16 * eslots);
-    ctx->ctr.reserve(4 * CTR_WORDS);
-    ctx->stats.reserve(64);
-    ctx->work.reserve(64);
-    ctx->acc.reserve(sizeof(double4) * np);
-    ctx->image.reserve(sizeof(float4) * np);
-    ctx->image_pixels = np;
-
-    GiState S{};
-    S.T = ctx->T.as<float4>();
-    S.L = ctx->L.as<float4>();
-    S.M = ctx->M.as<uint4>();
-    S.stk = sc->has_transmission ? ctx->stk.as<float4>() : nullptr;
-    S.qo[0] = ctx->qo0.as<float4>();
-    S.qo[1] = ctx->qo1.as<float4>();
-    S.qd[0] = ctx->qd0.as<float4>();
-    S.qd[1] = ctx->qd1.as<float4>();
-    S.qp[0] = ctx->qp0.as<uint32_t>();
-    S.qp[1] = ctx->qp1.as<uint32_t>();
-    S.hit = ctx->hit.as<float4>();
-    S.gT = ctx->gT.as<float4>();
-    S.gI = ctx->gI.as<uint4>();
-    S.eo = ctx->eo.as<float4>();
-    S.ed = ctx->ed.as<float4>();
-    S.ec = ctx->ec.as<float4>();
-    S.ctr = ctx->ctr.as<uint32_t>();
-    S.stats = ctx->stats.as<unsigned long long>();
-    S.work = ctx->counting ? ctx->work.as<unsigned long long>() : nullptr;
-    S.qseg = (uint32_t)qseg;
-    S.gseg = (uint32_t)qseg;
-    S.eseg = (uint32_t)eseg;
-
-    GiParams P{};
-    P.sc = sc->view;
-    for (int k = 0; k < 3; ++k) {
-      P.cam_o[k] = cam->position[k];
-      P.cam_f[k] = cam->forward[k];
-      P.cam_r[k] = cam->right[k];
-      P.cam_u[k] = cam->up[k];
+    // A one-job engine sized for this frame (grown when a larger frame comes).
+    const uint64_t np = (uint64_t)width * height;
+    const uint32_t per_hit = std::max<uint32_t>(2, job_entries_per_hit(sc, *set));
+    // Shadow entries are bounded to 2^28 (12 GB): many NEE samples per hit
+    // shrink the sample chunk instead.
+    const uint64_t capacity = std::min<uint64_t>(set->path_capacity ? set->path_capacity : (1ull << 25),
+                                                 (1ull << 28) / per_hit);
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(samples, capacity / np));
+    const uint64_t need = std::max<uint64_t>(4096, chunk * np + 2ull * kChunk * kJobStripes);
+    if (!ctx->frame_engine || ctx->frame_engine_slots < need || ctx->frame_engine_entries < per_hit) {
+      ctx->frame_engine.reset();
+      olhip_engine_config cfg{};
+      cfg.path_slots = need;
+      cfg.entry_factor = per_hit;
+      cfg.use_graph = 1;
+      ctx->frame_engine = std::make_unique<Engine>(ctx, cfg);
+      ctx->frame_engine_slots = need;
+      ctx->frame_engine_entries = per_hit;
     }
-    P.projection = cam->projection;
-    P.width = width;
-    P.height = height;
-    // The reference stores the frame index as float32 in the camera block.
-    P.frame = (uint32_t)((float)frame_index + 0.5f);
-    P.max_bounces = set->max_bounces;
-    P.point_count = sc->point_count;
-    P.area_count = sc->view.area_count;
-    P.secondary_area_samples = set->secondary_area_samples;
-    P.environment_samples = set->environment_samples;
-    P.roulette_start = set->russian_roulette_start;
-    P.roulette_min_survival = set->russian_roulette_min_survival;
-    P.nee_probability = set->secondary_nee_probability;
-    P.unified = set->unified_secondary_nee ? 1u : 0u;
-    P.two_pass = two_pass ? 1u : 0u;
-
-    GiLaunch l = make_launch(ctx, sc);
+    Engine& eng = *ctx->frame_engine;
+    eng.set_profiling(ctx->profiling != 0);
+    eng.set_work_counters(ctx->counting != 0);
+    eng.trace_ms = eng.shade_ms = eng.shadow_ms = 0.0;
+    eng.trace_launches = 0;
+    olhip_job_desc d{};
+    d.scene = sc;
+    d.camera = *cam;
+    d.width = width;
+    d.height = height;
+    d.samples = samples;
+    d.frame_index = frame_index;
+    d.settings = *set;
+    d.settings.path_capacity = (uint32_t)std::min<uint64_t>(capacity, 0xFFFFFFFFull);
+    unsigned long long work0[4] = {0, 0, 0, 0};
+    if (ctx->counting) eng.read_work(work0);
     Event e_start, e_end;
-    std::vector<std::unique_ptr<Event>> marks;
-    double trace_ms = 0, shade_ms = 0, shadow_ms = 0;
-    uint32_t n_trace = 0, n_shade = 0, n_shadow = 0;
-    OL_HIP_CHECK(hipEventRecord(e_start.e, s));
-    gi_reset(S, true, s);
-    uint32_t chunks = 0;
-    for (uint32_t base = 0; base < samples; base += chunk) {
-      P.sample_base = base;
-      P.spp = std::min(chunk, samples - base);
-      gi_reset(S, false, s);
-      gi_generate(P, S, s);
-      for (uint32_t b = 0; b < set->max_bounces; ++b) {
-        const uint32_t in = b & 1u;
-        gi_bounce_prologue(S, b, s);
-        if (ctx->profiling) {
-          auto a = std::make_unique<Event>(), c = std::make_unique<Event>(), d = std::make_unique<Event>(),
-               e = std::make_unique<Event>();
-          OL_HIP_CHECK(hipEventRecord(a->e, s));
-          gi_trace_closest(l, P.sc, S, in, s);
-          OL_HIP_CHECK(hipEventRecord(c->e, s));
-          gi_shade(l, P, S, in, s);
-          OL_HIP_CHECK(hipEventRecord(d->e, s));
-          gi_trace_shadow(l, P.sc, S, s);
-          OL_HIP_CHECK(hipEventRecord(e->e, s));
-          marks.push_back(std::move(a));
-          marks.push_back(std::move(c));
-          marks.push_back(std::move(d));
-          marks.push_back(std::move(e));
-        } else {
-          gi_trace_closest(l, P.sc, S, in, s);
-          gi_shade(l, P, S, in, s);
-          gi_trace_shadow(l, P.sc, S, s);
-        }
-        ++n_trace;
-        ++n_shade;
-        ++n_shadow;
-      }
-      gi_finish(S, s);
-      gi_accumulate(P, S, ctx->acc.as<double4>(), base == 0, s);
-      ++chunks;
-    }
-    gi_final(ctx->acc.as<double4>(), np, samples, ctx->image.as<float4>(), s);
-    OL_HIP_CHECK(hipGetLastError());
-    OL_HIP_CHECK(hipEventRecord(e_end.e, s));
+    OL_HIP_CHECK(hipEventRecord(e_start.e, ctx->stream));
+    const uint64_t id = eng.submit(d);
+    eng.run_until_idle();
+    OL_HIP_CHECK(hipEventRecord(e_end.e, ctx->stream));
     OL_HIP_CHECK(hipEventSynchronize(e_end.e));
-    uint32_t ctr_host[CTR_WORDS];
-    unsigned long long st_host[4], work_host[4] = {0, 0, 0, 0};
-    OL_HIP_CHECK(hipMemcpy(ctr_host, S.ctr, sizeof(ctr_host), hipMemcpyDeviceToHost));
-    OL_HIP_CHECK(hipMemcpy(st_host, S.stats, sizeof(st_host), hipMemcpyDeviceToHost));
-    if (S.work) OL_HIP_CHECK(hipMemcpy(work_host, S.work, sizeof(work_host), hipMemcpyDeviceToHost));
-    if (ctr_host[CTR_STACK]) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
-    for (size_t k = 0; k + 3 < marks.size(); k += 4) {
-      trace_ms += elapsed(*marks[k], *marks[k + 1]);
-      shade_ms += elapsed(*marks[k + 1], *marks[k + 2]);
-      shadow_ms += elapsed(*marks[k + 2], *marks[k + 3]);
+    olhip_job_stats js{};
+    if (eng.state(id) != OLHIP_JOB_DONE) {
+      eng.take_result(id, nullptr, nullptr);  // throws with the failure reason
+      throw OlError{OLHIP_ERR_RUNTIME, "render did not complete"};
     }
-    if (out_rgba)
-      OL_HIP_CHECK(hipMemcpy(out_rgba, ctx->image.ptr, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    ctx->image.reserve(sizeof(float4) * np);
+    OL_HIP_CHECK(hipMemcpyAsync(ctx->image.ptr, eng.device_image(id), sizeof(float4) * np, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->image_pixels = (uint32_t)np;
+    eng.take_result(id, out_rgba, &js);
     if (stats_out) {
       olhip_stats& o = *stats_out;
-      o.primary_rays = st_host[0];
-      o.extension_rays = st_host[1];
-      o.shadow_rays = st_host[2];
-      o.queue_overflow = ctr_host[CTR_OVERFLOW];
+      std::memset(&o, 0, sizeof(o));
+      o.primary_rays = js.primary_rays;
+      o.extension_rays = js.extension_rays;
+      o.shadow_rays = js.shadow_rays;
+      olhip_engine_stats es{};
+      eng.engine_stats(&es);
+      o.queue_overflow = es.queue_overflow;
       o.total_ms = elapsed(e_start, e_end);
-      o.trace_ms = trace_ms;
-      o.shade_ms = shade_ms;
-      o.shadow_ms = shadow_ms;
-      o.trace_launches = n_trace;
-      o.shade_launches = n_shade;
-      o.shadow_launches = n_shadow;
-      o.chunks = chunks;
-      o.closest_nodes = work_host[0];
-      o.closest_tris = work_host[1];
-      o.shadow_nodes = work_host[2];
-      o.shadow_tris = work_host[3];
+      o.trace_ms = eng.trace_ms;
+      o.shade_ms = eng.shade_ms;
+      o.shadow_ms = eng.shadow_ms;
+      o.trace_launches = eng.trace_launches;
+      o.shade_launches = eng.trace_launches;
+      o.shadow_launches = eng.trace_launches;
+      o.chunks = js.chunks;
+      if (ctx->counting) {
+        unsigned long long w[4];
+        eng.read_work(w);
+        o.closest_nodes = w[0] - work0[0];
+        o.closest_tris = w[1] - work0[1];
+        o.shadow_nodes = w[2] - work0[2];
+        o.shadow_tris = w[3] - work0[3];
+      }
     }
+  });
+}
+
+// ---- batched engine ------------------------------------------------------------------------------
+int olhip_engine_create(olhip_ctx* ctx, const olhip_engine_config* config, olhip_engine** out) {
+  return guarded([&] {
+    require(ctx && out, "ctx and out must not be NULL");
+    olhip_engine_config cfg{};
+    if (config) cfg = *config;
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    auto e = std::make_unique<olhip_engine>();
+    e->engine = std::make_unique<Engine>(ctx, cfg);
+    *out = e.release();
+  });
+}
+
+int olhip_engine_destroy(olhip_engine* engine) {
+  return guarded([&] { delete engine; });
+}
+
+int olhip_engine_submit(olhip_engine* engine, const olhip_job_desc* job, uint64_t* job_id) {
+  return guarded([&] {
+    require(engine && job && job_id, "arguments must not be NULL");
+    *job_id = engine->engine->submit(*job);
+  });
+}
+
+int olhip_engine_step(olhip_engine* engine, uint32_t* pending) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    const uint32_t p = engine->engine->step();
+    if (pending) *pending = p;
+  });
+}
+
+int olhip_engine_run(olhip_engine* engine) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    engine->engine->run_until_idle();
+  });
+}
+
+int olhip_engine_poll(olhip_engine* engine, uint64_t job_id, int* state) {
+  return guarded([&] {
+    require(engine && state, "arguments must not be NULL");
+    *state = engine->engine->state(job_id);
+  });
+}
+
+int olhip_engine_wait(olhip_engine* engine, uint64_t job_id, double timeout_s, int* state) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    const int st = engine->engine->wait(job_id, timeout_s);
+    if (state) *state = st;
+  });
+}
+
+int olhip_engine_result(olhip_engine* engine, uint64_t job_id, float* out_rgba, olhip_job_stats* stats) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    engine->engine->take_result(job_id, out_rgba, stats);
+  });
+}
+
+int olhip_engine_start(olhip_engine* engine) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    engine->engine->start_worker();
+  });
+}
+
+int olhip_engine_stop(olhip_engine* engine) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    engine->engine->stop_worker();
+  });
+}
+
+int olhip_engine_get_stats(olhip_engine* engine, olhip_engine_stats* out) {
+  return guarded([&] {
+    require(engine && out, "arguments must not be NULL");
+    engine->engine->engine_stats(out);
   });
 }
 

@@ -1,0 +1,188 @@
+// Library handles and the batched render engine (host side).
+//
+// The engine keeps a pool of path slots, shadow-entry slots and kMaxJobs
+// device job records.  Jobs are admitted first-come first-served when a
+// contiguous slot range for their next sample chunk is free (admission by
+// memory), advance together one wavefront step at a time, and release
+// their ranges when their paths have terminated.  A step is a fixed
+// sequence of launches, captured once per queue parity as a hipGraph and
+// replayed.  A worker thread can drive steps while other threads submit and
+// wait (serving).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/olhip.h"
+#include "e0.h"
+#include "gi.h"
+#include "internal.h"
+
+struct olhip_scene {
+  olhip_ctx* ctx = nullptr;
+  DevBuf positions, normals, vidx, materials, analytic, area;
+  BvhOutput bvh;
+  DScene view{};
+  uint32_t tri_count = 0, vertex_count = 0;
+  bool has_transmission = false;
+  uint32_t point_count = 0;
+  std::vector<float> materials_host;
+};
+
+// Shadow entries one shaded hit of this job may emit (1 for unified NEE
+// without point lights).
+uint32_t job_entries_per_hit(const olhip_scene* sc, const olhip_gi_settings& s);
+
+// First-fit allocator of [0, size) ranges.
+class RangeAllocator {
+ public:
+  void reset(uint64_t size);
+  // Start of a free range of `n`, or UINT64_MAX.
+  uint64_t allocate(uint64_t n);
+  void release(uint64_t begin, uint64_t n);
+  uint64_t largest_free() const;
+  uint64_t size() const { return size_; }
+
+ private:
+  std::map<uint64_t, uint64_t> free_;  // begin -> length
+  uint64_t size_ = 0;
+};
+
+class Engine {
+ public:
+  Engine(olhip_ctx* ctx, const olhip_engine_config& cfg);
+  ~Engine();
+
+  uint64_t submit(const olhip_job_desc& job);
+  // One step.  Returns the number of jobs not yet done (queued, resident or
+  // accumulating).  Callers hold no lock; the engine serialises steps.
+  uint32_t step();
+  void run_until_idle();
+  int state(uint64_t id);
+  // Copies the result (optional) and statistics, then forgets the job.
+  void take_result(uint64_t id, float* out_rgba, olhip_job_stats* stats, bool keep = false);
+  void engine_stats(olhip_engine_stats* out);
+  // Device image of a done job (valid until take_result).
+  const float4* device_image(uint64_t id);
+
+  void start_worker();
+  void stop_worker();
+  // Blocks until the job is done or failed, or the timeout (seconds, < 0 =
+  // none) passes.  Returns the job state.
+  int wait(uint64_t id, double timeout_s);
+
+  olhip_ctx* ctx() const { return ctx_; }
+  const olhip_engine_config& config() const { return cfg_; }
+  // Per-step kernel timing (events around the traversal and shade launches).
+  void set_profiling(bool on) { profiling_ = on; }
+  void set_work_counters(bool on);
+  void read_work(unsigned long long out[4]);
+  double trace_ms = 0, shade_ms = 0, shadow_ms = 0;
+  uint32_t trace_launches = 0;
+
+ private:
+  struct Job {
+    uint64_t id = 0;
+    olhip_job_desc desc{};
+    uint32_t per_hit = 1;
+    bool transmits = false;
+    uint32_t next_sample = 0;   // first sample not yet admitted
+    int slot = -1;              // device job slot of the resident chunk
+    uint32_t chunk_spp = 0, chunk_base_sample = 0;
+    uint64_t region = 0, span = 0, ebase = 0, espan = 0;
+    bool first_chunk = true;
+    uint32_t steps_in_chunk = 0;
+    uint32_t passed = 0;        // admission rounds in which this job did not fit
+    int status = OLHIP_JOB_QUEUED;
+    DevBuf acc, image;
+    olhip_job_stats stats{};
+    std::string error;
+  };
+
+  void ensure_buffers();
+  void capture_graphs();
+  void launch_step(uint32_t in, bool timed);
+  bool admit(Job& j, int slot);
+  GiLaunch make_launch();
+  double now_ms() const;
+
+  olhip_ctx* ctx_;
+  olhip_engine_config cfg_;
+  std::mutex mu_;                 // jobs, queue, counters
+  std::mutex step_mu_;            // one step at a time
+  std::condition_variable cv_;    // job state changes / new work
+  std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs_;
+  std::deque<uint64_t> queue_;    // waiting for admission (FIFO)
+  std::vector<uint64_t> slot_job_;  // kMaxJobs: job id per device slot (0 = free)
+  std::vector<AccItem> pending_acc_;
+  std::vector<uint64_t> pending_acc_jobs_;  // jobs whose final accumulation is pending
+  RangeAllocator paths_, entries_;
+  uint64_t next_id_ = 1;
+  uint32_t parity_ = 0;           // input queue of the next step
+  uint64_t steps_ = 0;
+  int max_depth_ = 1;
+  bool stk_needed_ = false;
+
+  // Device state.
+  DevBuf T_, L_, M_, stk_, qo_[2], qd_[2], qp_[2], hit_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
+      list_dev_, spill_;
+  uint64_t slots_ = 0, eslots_ = 0;
+  Wave wave_{};
+  GiLaunch launch_{};
+  bool counting_ = false;
+  bool profiling_ = false;
+  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  bool graph_valid_ = false;
+  // Pinned host staging.
+  JobDev* jobs_host_ = nullptr;
+  StepList* list_host_ = nullptr;
+  uint32_t* report_host_ = nullptr;
+  hipEvent_t done_ = nullptr;
+  double t0_ms_ = 0;
+  // Engine totals.
+  olhip_engine_stats totals_{};
+
+  std::thread worker_;
+  std::atomic<bool> stop_{false};
+  bool worker_running_ = false;
+};
+
+struct olhip_engine {
+  std::unique_ptr<Engine> engine;
+};
+
+struct olhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int profiling = 0;
+  int counting = 0;
+  int lds_stack = 16;
+  int trace_block = 128;
+  // Engine of the single-frame entry point (created on first use).
+  std::unique_ptr<Engine> frame_engine;
+  uint64_t frame_engine_slots = 0;
+  uint32_t frame_engine_entries = 0;
+  DevBuf image;  // last single frame (olhip_frame_download)
+  uint32_t image_pixels = 0;
+  DevBuf spill;  // olhip_trace
+  // Reference-estimator frame.
+  struct E0Frame {
+    DevBuf o, d, T, rad, acc, active, flags, scan, ht, htri, hitlist, hit_t, hit_tri, jx, jy, gauss, env, ctr, out;
+    E0Params params{};
+    E0State state{};
+    const olhip_scene* scene = nullptr;
+    uint32_t n_active = 0, n_hit = 0;
+    int stack = 32;
+    bool open = false;
+  } e0;
+};

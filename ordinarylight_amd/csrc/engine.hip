@@ -1,0 +1,634 @@
+// Batched render engine: admission, sample chunking, step launch (direct or
+// captured hipGraph), per-job reports and the serving worker thread.
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "engine.h"
+
+// ---- RangeAllocator -------------------------------------------------------------------
+void RangeAllocator::reset(uint64_t size) {
+  free_.clear();
+  size_ = size;
+  if (size) free_[0] = size;
+}
+
+uint64_t RangeAllocator::allocate(uint64_t n) {
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < n) continue;
+    const uint64_t begin = it->first, len = it->second;
+    free_.erase(it);
+    if (len > n) free_[begin + n] = len - n;
+    return begin;
+  }
+  return UINT64_MAX;
+}
+
+void RangeAllocator::release(uint64_t begin, uint64_t n) {
+  if (n == 0) return;
+  auto next = free_.lower_bound(begin);
+  uint64_t b = begin, len = n;
+  if (next != free_.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == b) {
+      b = prev->first;
+      len += prev->second;
+      free_.erase(prev);
+    }
+  }
+  next = free_.lower_bound(begin);
+  if (next != free_.end() && b + len == next->first) {
+    len += next->second;
+    free_.erase(next);
+  }
+  free_[b] = len;
+}
+
+uint64_t RangeAllocator::largest_free() const {
+  uint64_t m = 0;
+  for (const auto& kv : free_) m = std::max(m, kv.second);
+  return m;
+}
+
+// ---- helpers ----------------------------------------------------------------------------
+namespace {
+
+uint32_t stripes_for(uint64_t n) {
+  return (uint32_t)std::min<uint64_t>(kJobStripes, std::max<uint64_t>(1, (n + 65535) / 65536));
+}
+
+// Slots per segment: a job's shade chunks number at most n / 64 + stripes,
+// so (ceil(n / (64 * stripes)) + 1) * 64 slots per segment cannot overflow.
+uint32_t seg_for(uint64_t n, uint32_t stripes) {
+  return (uint32_t)(((n + (uint64_t)kChunk * stripes - 1) / ((uint64_t)kChunk * stripes) + 1) * kChunk);
+}
+
+}  // namespace
+
+uint32_t job_entries_per_hit(const olhip_scene* sc, const olhip_gi_settings& s) {
+  const bool two_pass = sc->point_count > (sc->view.env_valid ? 1u : 0u) || !s.unified_secondary_nee;
+  if (!two_pass) return 1;
+  return sc->point_count +
+         (s.unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(s.secondary_area_samples, 1u), 16u) + 4u));
+}
+
+namespace {
+
+void require_arg(bool ok, const char* message) {
+  if (!ok) throw OlError{OLHIP_ERR_VALUE, message};
+}
+
+template <class T>
+T* pinned(size_t count) {
+  void* p = nullptr;
+  OL_HIP_CHECK(hipHostMalloc(&p, sizeof(T) * count, hipHostMallocDefault));
+  std::memset(p, 0, sizeof(T) * count);
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+// ---- Engine -------------------------------------------------------------------------------
+Engine::Engine(olhip_ctx* ctx, const olhip_engine_config& cfg) : ctx_(ctx), cfg_(cfg) {
+  if (cfg_.path_slots == 0) cfg_.path_slots = 1ull << 25;
+  if (cfg_.entry_factor == 0) cfg_.entry_factor = 2;
+  if (cfg_.lds_stack == 0) cfg_.lds_stack = ctx->lds_stack;
+  if (cfg_.trace_block == 0) cfg_.trace_block = ctx->trace_block;
+  require_arg(cfg_.path_slots >= 4096 && cfg_.path_slots < (1ull << 32), "path_slots must be in [4096, 2^32)");
+  require_arg(cfg_.entry_factor <= 128 && cfg_.path_slots * cfg_.entry_factor < (1ull << 32),
+              "entry_factor * path_slots must be below 2^32");
+  slots_ = cfg_.path_slots;
+  eslots_ = cfg_.path_slots * cfg_.entry_factor;
+  OL_HIP_CHECK(hipSetDevice(ctx_->device));
+  slot_job_.assign(kMaxJobs, 0);
+  paths_.reset(slots_);
+  entries_.reset(eslots_);
+  jobs_host_ = pinned<JobDev>(kMaxJobs);
+  list_host_ = pinned<StepList>(1);
+  report_host_ = pinned<uint32_t>(C_WORDS - C_OVERFLOW);
+  OL_HIP_CHECK(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+  t0_ms_ = 0;
+  t0_ms_ = now_ms();
+  ctr_.reserve(4ull * C_WORDS);
+  work_.reserve(64);
+  jobs_dev_.reserve(sizeof(JobDev) * kMaxJobs);
+  list_dev_.reserve(sizeof(StepList));
+  ensure_buffers();
+  launch_ = make_launch();
+  wave_clear(wave_, ctx_->stream);
+  OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+  totals_.path_slots = slots_;
+  totals_.entry_slots = eslots_;
+}
+
+Engine::~Engine() {
+  stop_worker();
+  (void)hipSetDevice(ctx_->device);
+  (void)hipStreamSynchronize(ctx_->stream);
+  for (auto& g : graph_)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (done_) (void)hipEventDestroy(done_);
+  if (jobs_host_) (void)hipHostFree(jobs_host_);
+  if (list_host_) (void)hipHostFree(list_host_);
+  if (report_host_) (void)hipHostFree(report_host_);
+}
+
+double Engine::now_ms() const {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count() - t0_ms_;
+}
+
+void Engine::ensure_buffers() {
+  T_.reserve(16 * slots_);
+  L_.reserve(16 * slots_);
+  M_.reserve(16 * slots_);
+  for (int q = 0; q < 2; ++q) {
+    qo_[q].reserve(16 * slots_);
+    qd_[q].reserve(16 * slots_);
+    qp_[q].reserve(4 * slots_);
+  }
+  hit_.reserve(16 * slots_);
+  gT_.reserve(16 * slots_);
+  gI_.reserve(16 * slots_);
+  eo_.reserve(16 * eslots_);
+  ed_.reserve(16 * eslots_);
+  ec_.reserve(16 * eslots_);
+  if (stk_needed_) stk_.reserve(64 * slots_);
+  Wave w{};
+  w.jobs = jobs_dev_.as<JobDev>();
+  w.list = list_dev_.as<StepList>();
+  w.T = T_.as<float4>();
+  w.L = L_.as<float4>();
+  w.M = M_.as<uint4>();
+  w.stk = stk_needed_ ? stk_.as<float4>() : nullptr;
+  for (int q = 0; q < 2; ++q) {
+    w.qo[q] = qo_[q].as<float4>();
+    w.qd[q] = qd_[q].as<float4>();
+    w.qp[q] = qp_[q].as<uint32_t>();
+  }
+  w.hit = hit_.as<float4>();
+  w.gT = gT_.as<float4>();
+  w.gI = gI_.as<uint4>();
+  w.eo = eo_.as<float4>();
+  w.ed = ed_.as<float4>();
+  w.ec = ec_.as<float4>();
+  w.ctr = ctr_.as<uint32_t>();
+  w.work = counting_ ? work_.as<unsigned long long>() : nullptr;
+  wave_ = w;
+  graph_valid_ = false;
+}
+
+GiLaunch Engine::make_launch() {
+  GiLaunch l{};
+  l.cu_count = ctx_->cu_count;
+  l.stack_depth = (int)(3u * (uint32_t)std::max(max_depth_, 1) + 1u);
+  l.lds_stack = cfg_.lds_stack;
+  l.trace_block = cfg_.trace_block;
+  const uint64_t words = trace_spill_words(l);
+  if (words) spill_.reserve(4 * words);
+  l.spill = words ? spill_.as<uint32_t>() : nullptr;
+  l.spill_words = words;
+  graph_valid_ = false;
+  return l;
+}
+
+void Engine::set_work_counters(bool on) {
+  std::lock_guard<std::mutex> g(step_mu_);
+  if (on == counting_) return;
+  counting_ = on;
+  OL_HIP_CHECK(hipMemsetAsync(work_.ptr, 0, 32, ctx_->stream));
+  ensure_buffers();
+}
+
+void Engine::read_work(unsigned long long out[4]) {
+  OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+  OL_HIP_CHECK(hipMemcpy(out, work_.ptr, 32, hipMemcpyDeviceToHost));
+}
+
+uint64_t Engine::submit(const olhip_job_desc& d) {
+  require_arg(d.scene != nullptr, "job scene must not be NULL");
+  require_arg(d.scene->ctx == ctx_, "scene belongs to another context");
+  require_arg(d.width >= 1 && d.height >= 1, "width and height must be positive");
+  require_arg((uint64_t)d.width * d.height <= (1ull << 28), "image too large");
+  require_arg(d.samples >= 1 && d.samples <= 65536, "samples must be between 1 and 65536");
+  const olhip_gi_settings& s = d.settings;
+  require_arg(s.max_bounces >= 1 && s.max_bounces <= 64, "max_bounces must be between 1 and 64");
+  require_arg(d.camera.projection <= 2, "unknown projection");
+  require_arg(s.secondary_nee_probability > 0.0f && s.secondary_nee_probability <= 1.0f,
+              "secondary_nee_probability must be in (0, 1]");
+  require_arg(s.russian_roulette_min_survival >= 0.0f && s.russian_roulette_min_survival <= 1.0f,
+              "russian_roulette_min_survival must be in [0, 1]");
+  require_arg(s.environment_samples <= 4, "environment_samples must be between 0 and 4");
+  auto j = std::make_unique<Job>();
+  j->desc = d;
+  j->per_hit = job_entries_per_hit(d.scene, s);
+  j->transmits = d.scene->has_transmission;
+  // One sample chunk must fit an empty engine.
+  const uint64_t n = (uint64_t)d.width * d.height;
+  const uint32_t S = stripes_for(n);
+  const uint64_t span = (uint64_t)S * seg_for(n, S);
+  if (span > slots_ || span * j->per_hit > eslots_)
+    throw OlError{OLHIP_ERR_MEMORY, "one sample of this job (" + std::to_string(n) +
+                                        " paths) exceeds the engine's path or shadow-entry slots"};
+  if (stripes_for(n) > kJobStripes) throw OlError{OLHIP_ERR_VALUE, "job too large"};
+  std::lock_guard<std::mutex> g(mu_);
+  j->id = next_id_++;
+  j->stats.submit_ms = now_ms();
+  const uint64_t id = j->id;
+  queue_.push_back(id);
+  jobs_[id] = std::move(j);
+  cv_.notify_all();
+  return id;
+}
+
+// Admits the job's next sample chunk into device slot `slot` if a slot range
+// is free.  Called with mu_ held.
+bool Engine::admit(Job& j, int slot) {
+  const olhip_job_desc& d = j.desc;
+  const uint64_t np = (uint64_t)d.width * d.height;
+  uint64_t spp = d.samples - j.next_sample;
+  if (d.settings.path_capacity) spp = std::min<uint64_t>(spp, std::max<uint64_t>(1, d.settings.path_capacity / np));
+  const uint64_t free_p = paths_.largest_free(), free_e = entries_.largest_free();
+  while (spp >= 1) {
+    const uint64_t n = spp * np;
+    const uint32_t S = stripes_for(n);
+    const uint32_t seg = seg_for(n, S);
+    const uint64_t span = (uint64_t)S * seg, espan = span * j.per_hit;
+    if (span <= free_p && espan <= free_e) {
+      const uint64_t base = paths_.allocate(span);
+      const uint64_t ebase = entries_.allocate(espan);
+      if (base == UINT64_MAX || ebase == UINT64_MAX) throw OlError{OLHIP_ERR_RUNTIME, "slot allocator inconsistency"};
+      j.region = base;
+      j.span = span;
+      j.ebase = ebase;
+      j.espan = espan;
+      j.chunk_spp = (uint32_t)spp;
+      j.chunk_base_sample = j.next_sample;
+      j.next_sample += (uint32_t)spp;
+      j.slot = slot;
+      j.steps_in_chunk = 0;
+      JobDev& J = jobs_host_[slot];
+      std::memset(&J, 0, sizeof(J));
+      GiParams& P = J.P;
+      P.sc = d.scene->view;
+      for (int k = 0; k < 3; ++k) {
+        P.cam_o[k] = d.camera.position[k];
+        P.cam_f[k] = d.camera.forward[k];
+        P.cam_r[k] = d.camera.right[k];
+        P.cam_u[k] = d.camera.up[k];
+      }
+      P.projection = d.camera.projection;
+      P.width = d.width;
+      P.height = d.height;
+      P.spp = (uint32_t)spp;
+      // The reference stores the frame index as float32 in the camera block.
+      P.frame = (uint32_t)((float)d.frame_index + 0.5f);
+      P.sample_base = j.chunk_base_sample;
+      P.max_bounces = d.settings.max_bounces;
+      P.point_count = d.scene->point_count;
+      P.area_count = d.scene->view.area_count;
+      P.secondary_area_samples = d.settings.secondary_area_samples;
+      P.environment_samples = d.settings.environment_samples;
+      P.roulette_start = d.settings.russian_roulette_start;
+      P.roulette_min_survival = d.settings.russian_roulette_min_survival;
+      P.nee_probability = d.settings.secondary_nee_probability;
+      P.unified = d.settings.unified_secondary_nee ? 1u : 0u;
+      const bool two_pass = d.scene->point_count > (d.scene->view.env_valid ? 1u : 0u) || !P.unified;
+      P.two_pass = two_pass ? 1u : 0u;
+      J.base = (uint32_t)base;
+      J.n_paths = (uint32_t)n;
+      J.stripes = S;
+      J.seg = seg;
+      J.ebase = (uint32_t)ebase;
+      J.eseg = seg * j.per_hit;
+      if (j.first_chunk) {
+        j.acc.reserve(sizeof(double4) * np);
+        j.image.reserve(sizeof(float4) * np);
+        j.stats.admit_ms = now_ms();
+      }
+      j.status = OLHIP_JOB_RUNNING;
+      ++j.stats.chunks;
+      slot_job_[slot] = j.id;
+      if (j.transmits && !stk_needed_) {
+        stk_needed_ = true;
+        ensure_buffers();
+      }
+      const int depth = (int)d.scene->bvh.depth;
+      if (depth > max_depth_) {
+        max_depth_ = depth;
+        launch_ = make_launch();
+      }
+      return true;
+    }
+    // Shrink towards what the largest free range can hold.
+    const uint64_t fit = std::min(free_p * spp / std::max<uint64_t>(span, 1), free_e * spp / std::max<uint64_t>(espan, 1));
+    spp = std::min(spp - 1, fit);
+  }
+  return false;
+}
+
+void Engine::launch_step(uint32_t in, bool timed) {
+  hipStream_t s = ctx_->stream;
+  wave_accumulate(wave_, ctx_->cu_count, s);
+  wave_generate(wave_, in, ctx_->cu_count, s);
+  if (!timed) {
+    wave_trace(launch_, wave_, in, s);
+    wave_shade(launch_, wave_, in, s);
+    wave_shadow(launch_, wave_, s);
+  } else {
+    hipEvent_t e[4];
+    for (auto& x : e) OL_HIP_CHECK(hipEventCreate(&x));
+    OL_HIP_CHECK(hipEventRecord(e[0], s));
+    wave_trace(launch_, wave_, in, s);
+    OL_HIP_CHECK(hipEventRecord(e[1], s));
+    wave_shade(launch_, wave_, in, s);
+    OL_HIP_CHECK(hipEventRecord(e[2], s));
+    wave_shadow(launch_, wave_, s);
+    OL_HIP_CHECK(hipEventRecord(e[3], s));
+    OL_HIP_CHECK(hipEventSynchronize(e[3]));
+    float a = 0, b = 0, c = 0;
+    OL_HIP_CHECK(hipEventElapsedTime(&a, e[0], e[1]));
+    OL_HIP_CHECK(hipEventElapsedTime(&b, e[1], e[2]));
+    OL_HIP_CHECK(hipEventElapsedTime(&c, e[2], e[3]));
+    trace_ms += a;
+    shade_ms += b;
+    shadow_ms += c;
+    ++trace_launches;
+    for (auto& x : e) (void)hipEventDestroy(x);
+  }
+  wave_epilogue(wave_, in, s);
+}
+
+void Engine::capture_graphs() {
+  hipStream_t s = ctx_->stream;
+  for (uint32_t in = 0; in < 2; ++in) {
+    if (graph_[in]) {
+      OL_HIP_CHECK(hipGraphExecDestroy(graph_[in]));
+      graph_[in] = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    OL_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_step(in, false);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    OL_HIP_CHECK(hipStreamEndCapture(s, &g));
+    OL_HIP_CHECK(hipGraphInstantiate(&graph_[in], g, nullptr, nullptr, 0));
+    OL_HIP_CHECK(hipGraphDestroy(g));
+  }
+  graph_valid_ = true;
+}
+
+uint32_t Engine::step() {
+  std::lock_guard<std::mutex> step_guard(step_mu_);
+  OL_HIP_CHECK(hipSetDevice(ctx_->device));
+  const double t_begin = now_ms();
+  bool any_resident = false;
+  std::vector<uint64_t> finals;  // jobs whose last chunk is accumulated in this step
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    StepList& Ls = *list_host_;
+    Ls.n_gen = 0;
+    Ls.gen_prefix[0] = 0;
+    // Admission: first-fit over the queue; a job passed over for 8 steps
+    // blocks the ones behind it so it cannot starve.
+    for (auto it = queue_.begin(); it != queue_.end();) {
+      int slot = -1;
+      for (uint32_t k = 0; k < kMaxJobs; ++k)
+        if (slot_job_[k] == 0) {
+          slot = (int)k;
+          break;
+        }
+      if (slot < 0) break;
+      Job& j = *jobs_[*it];
+      if (admit(j, slot)) {
+        Ls.gen_job[Ls.n_gen] = (uint32_t)slot;
+        Ls.gen_prefix[Ls.n_gen + 1] = Ls.gen_prefix[Ls.n_gen] + jobs_host_[slot].n_paths;
+        ++Ls.n_gen;
+        it = queue_.erase(it);
+        continue;
+      }
+      if (++j.passed > 8) break;
+      ++it;
+    }
+    Ls.n_acc = (uint32_t)pending_acc_.size();
+    Ls.acc_prefix[0] = 0;
+    for (uint32_t k = 0; k < Ls.n_acc; ++k) {
+      Ls.acc[k] = pending_acc_[k];
+      Ls.acc_prefix[k + 1] = Ls.acc_prefix[k] + pending_acc_[k].np;
+    }
+    finals = pending_acc_jobs_;
+    pending_acc_.clear();
+    pending_acc_jobs_.clear();
+    for (uint32_t k = 0; k < kMaxJobs; ++k) any_resident = any_resident || slot_job_[k] != 0;
+    totals_.resident_jobs = 0;
+    for (uint32_t k = 0; k < kMaxJobs; ++k) totals_.resident_jobs += slot_job_[k] != 0;
+    totals_.max_resident_jobs = std::max(totals_.max_resident_jobs, totals_.resident_jobs);
+    totals_.queued_jobs = (uint32_t)queue_.size();
+  }
+  const StepList& Ls = *list_host_;
+  if (!any_resident && Ls.n_acc == 0) return 0;
+  hipStream_t s = ctx_->stream;
+  OL_HIP_CHECK(hipMemcpyAsync(jobs_dev_.ptr, jobs_host_, sizeof(JobDev) * kMaxJobs, hipMemcpyHostToDevice, s));
+  OL_HIP_CHECK(hipMemcpyAsync(list_dev_.ptr, list_host_, sizeof(StepList), hipMemcpyHostToDevice, s));
+  const uint32_t in = parity_;
+  if (!any_resident) {
+    // Only accumulation left: no traversal launches.
+    wave_accumulate(wave_, ctx_->cu_count, s);
+  } else if (profiling_ || !cfg_.use_graph || steps_ == 0) {
+    launch_step(in, profiling_);
+  } else {
+    if (!graph_valid_) capture_graphs();
+    OL_HIP_CHECK(hipGraphLaunch(graph_[in], s));
+  }
+  OL_HIP_CHECK(hipGetLastError());
+  OL_HIP_CHECK(hipMemcpyAsync(report_host_, ctr_.as<uint32_t>() + C_OVERFLOW, 4ull * (C_WORDS - C_OVERFLOW),
+                              hipMemcpyDeviceToHost, s));
+  OL_HIP_CHECK(hipEventRecord(done_, s));
+  OL_HIP_CHECK(hipEventSynchronize(done_));
+  uint32_t pending = 0;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    const double t = now_ms();
+    for (uint64_t id : finals) {
+      auto it = jobs_.find(id);
+      if (it == jobs_.end()) continue;
+      it->second->status = OLHIP_JOB_DONE;
+      it->second->stats.done_ms = t;
+      ++totals_.jobs_done;
+    }
+    if (any_resident) {
+      ++steps_;
+      const uint32_t* rep = report_host_ + (C_REPORT - C_OVERFLOW);
+      const bool stack_overflow = report_host_[C_STACK - C_OVERFLOW] != 0;
+      totals_.queue_overflow = report_host_[0];
+      for (uint32_t k = 0; k < kMaxJobs; ++k) {
+        if (slot_job_[k] == 0) continue;
+        Job& j = *jobs_[slot_job_[k]];
+        const uint32_t rays = rep[4 * k], shadow = rep[4 * k + 1], live = rep[4 * k + 2];
+        if (j.steps_in_chunk == 0) j.stats.primary_rays += rays;
+        else j.stats.extension_rays += rays;
+        j.stats.shadow_rays += shadow;
+        totals_.rays += (uint64_t)rays + shadow;
+        ++j.steps_in_chunk;
+        ++j.stats.steps;
+        if (stack_overflow) {
+          j.status = OLHIP_JOB_FAILED;
+          j.error = "traversal stack overflow";
+        }
+        if (live == 0 || stack_overflow) {
+          // Chunk finished: accumulate next step, release its slots now (the
+          // accumulation precedes any generate that reuses them).
+          AccItem a{};
+          a.base = (uint32_t)j.region;
+          a.np = j.desc.width * j.desc.height;
+          a.spp = j.chunk_spp;
+          a.first_chunk = j.first_chunk ? 1u : 0u;
+          const bool last = j.next_sample >= j.desc.samples;
+          a.final_samples = last ? j.desc.samples : 0u;
+          a.acc = j.acc.as<double4>();
+          a.image = j.image.as<float4>();
+          if (!stack_overflow) {
+            pending_acc_.push_back(a);
+            if (last) pending_acc_jobs_.push_back(j.id);
+          }
+          paths_.release(j.region, j.span);
+          entries_.release(j.ebase, j.espan);
+          slot_job_[k] = 0;
+          j.slot = -1;
+          j.first_chunk = false;
+          if (!last && !stack_overflow) queue_.push_front(j.id);
+        }
+      }
+      parity_ ^= 1u;
+      if (stack_overflow) {
+        OL_HIP_CHECK(hipMemsetAsync(ctr_.as<uint32_t>() + C_STACK, 0, 4, ctx_->stream));
+        OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+      }
+    }
+    totals_.steps = steps_;
+    totals_.busy_ms += now_ms() - t_begin;
+    for (const auto& kv : jobs_)
+      if (kv.second->status == OLHIP_JOB_QUEUED || kv.second->status == OLHIP_JOB_RUNNING) ++pending;
+    cv_.notify_all();
+  }
+  return pending;
+}
+
+void Engine::run_until_idle() {
+  if (worker_running_) throw OlError{OLHIP_ERR_RUNTIME, "the engine's worker thread is running"};
+  while (step() > 0) {
+  }
+}
+
+int Engine::state(uint64_t id) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = jobs_.find(id);
+  return it == jobs_.end() ? OLHIP_JOB_UNKNOWN : it->second->status;
+}
+
+int Engine::wait(uint64_t id, double timeout_s) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto done = [&] {
+    auto it = jobs_.find(id);
+    return it == jobs_.end() || it->second->status == OLHIP_JOB_DONE || it->second->status == OLHIP_JOB_FAILED;
+  };
+  if (timeout_s < 0) cv_.wait(lk, done);
+  else cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+  auto it = jobs_.find(id);
+  return it == jobs_.end() ? OLHIP_JOB_UNKNOWN : it->second->status;
+}
+
+const float4* Engine::device_image(uint64_t id) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end() || it->second->status != OLHIP_JOB_DONE) return nullptr;
+  return it->second->image.as<float4>();
+}
+
+void Engine::take_result(uint64_t id, float* out_rgba, olhip_job_stats* stats, bool keep) {
+  std::unique_ptr<Job> j;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = jobs_.find(id);
+    if (it == jobs_.end()) throw OlError{OLHIP_ERR_VALUE, "unknown job id"};
+    if (it->second->status == OLHIP_JOB_FAILED) {
+      std::string e = it->second->error;
+      jobs_.erase(it);
+      throw OlError{OLHIP_ERR_RUNTIME, "job failed: " + e};
+    }
+    if (it->second->status != OLHIP_JOB_DONE) throw OlError{OLHIP_ERR_RUNTIME, "job is not done"};
+    if (stats) *stats = it->second->stats;
+    if (keep) {
+      if (out_rgba) {
+        OL_HIP_CHECK(hipSetDevice(ctx_->device));
+        OL_HIP_CHECK(hipMemcpy(out_rgba, it->second->image.ptr,
+                               sizeof(float4) * it->second->desc.width * it->second->desc.height,
+                               hipMemcpyDeviceToHost));
+      }
+      return;
+    }
+    j = std::move(it->second);
+    jobs_.erase(it);
+  }
+  OL_HIP_CHECK(hipSetDevice(ctx_->device));
+  if (out_rgba)
+    OL_HIP_CHECK(hipMemcpy(out_rgba, j->image.ptr, sizeof(float4) * j->desc.width * j->desc.height,
+                           hipMemcpyDeviceToHost));
+}
+
+void Engine::engine_stats(olhip_engine_stats* out) {
+  std::lock_guard<std::mutex> g(mu_);
+  *out = totals_;
+}
+
+void Engine::start_worker() {
+  if (worker_running_) return;
+  stop_ = false;
+  worker_running_ = true;
+  worker_ = std::thread([this] {
+    (void)hipSetDevice(ctx_->device);
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] {
+          if (stop_) return true;
+          if (!queue_.empty() || !pending_acc_.empty()) return true;
+          for (uint32_t k = 0; k < kMaxJobs; ++k)
+            if (slot_job_[k]) return true;
+          return false;
+        });
+        if (stop_) break;
+      }
+      try {
+        step();
+      } catch (const OlError& e) {
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto& kv : jobs_)
+          if (kv.second->status == OLHIP_JOB_QUEUED || kv.second->status == OLHIP_JOB_RUNNING) {
+            kv.second->status = OLHIP_JOB_FAILED;
+            kv.second->error = e.message;
+          }
+        queue_.clear();
+        std::fill(slot_job_.begin(), slot_job_.end(), 0);
+        cv_.notify_all();
+        break;
+      }
+    }
+  });
+}
+
+void Engine::stop_worker() {
+  if (!worker_running_) return;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+    cv_.notify_all();
+  }
+  worker_.join();
+  worker_running_ = false;
+}

@@ -1,50 +1,18 @@
-// Kernel parameter blocks of the GI wavefront (host and device).
+// Batched GI wavefront: per-job parameters, the device job table, the shared
+// path/queue state and the host launchers (gi.hip).
+//
+// Every render is a job: a scene, a camera, an image size and a range of
+// samples.  Resident jobs share one pool of path slots and one set of ray,
+// hit and shadow queues; every kernel serves all resident jobs at once, and a
+// job may join at any step (continuous batching).  A single frame is the
+// one-job case.
 #pragma once
 
 #include "internal.h"
 
-// Queue counters are striped: each queue (rays, shadow groups, shadow
-// entries) is cut into kStripes segments of equal capacity, each with its own
-// counter, so that slot reservations spread over kStripes addresses instead of
-// serialising on one.  A producer wave appends to the segment of stripe
-// (input index / 256) % kStripes; since every input yields at most one
-// continuation and one group, a segment of ceil(capacity / (256 * kStripes)) *
-// 256 slots never overflows.  Consumers enumerate the live entries densely
-// through the inclusive prefix of the kStripes counts.
-constexpr uint32_t kStripes = 32;
-constexpr uint32_t kStripeUnit = 256;  // producer block size: inputs per stripe step
-constexpr uint32_t CTR_Q0 = 0, CTR_Q1 = kStripes, CTR_G = 2 * kStripes, CTR_E = 3 * kStripes;
-// CTR_W: work cursors of the refilling closest-hit kernel, one per stripe of
-// its input (rays [s * n / kStripes, (s + 1) * n / kStripes)).
-// CTR_WS: the same for the shadow kernel's groups.
-constexpr uint32_t CTR_W = 4 * kStripes, CTR_WS = 5 * kStripes;
-constexpr uint32_t CTR_OVERFLOW = 6 * kStripes, CTR_STACK = 6 * kStripes + 1, CTR_WORDS = 6 * kStripes + 8;
-OL_HD constexpr uint32_t ctr_queue(uint32_t q) { return q ? CTR_Q1 : CTR_Q0; }
-
-// Per-path state, structure of arrays indexed by path id
-// p = sample * (W * H) + pixel (wavefront/_core.py:19-105 hot path state).
-struct GiState {
-  float4* T;    // throughput rgb, bounce index
-  float4* L;    // radiance rgb, previous BSDF pdf
-  uint4* M;     // pixel, frame << 8 | sample, rng state, flags | medium depth << 8
-  float4* stk;  // 4 x float4 medium IOR stack per path (null without transmission)
-  // Ray queues (ping-pong), structure of arrays.
-  float4* qo[2];     // origin, tmin
-  float4* qd[2];     // direction, tmax
-  uint32_t* qp[2];   // path id
-  float4* hit;       // (t, u, v, leaf slot bits) per input-queue entry
-  // Shadow work: one group per shaded hit with next-event estimation.
-  float4* gT;        // throughput rgb, NEE selection probability
-  uint4* gI;         // path, first entry, entry count, area count | env count << 8 | mode << 16
-  float4* eo;        // entry origin, tmin
-  float4* ed;        // entry direction, tmax
-  float4* ec;        // entry contribution rgb, kind (0 point, 1 area, 2 environment)
-  uint32_t* ctr;     // CTR_WORDS: per-stripe counts of queue 0, queue 1, groups, entries; overflow flags
-  unsigned long long* stats;  // [0] primary, [1] extension, [2] shadow rays, [3] shaded hits
-  unsigned long long* work;   // optional: [0] closest nodes, [1] closest tris, [2] shadow nodes, [3] shadow tris
-  uint32_t qseg, gseg, eseg;  // slots per stripe segment
-};
-
+// Per-job estimator parameters (the reference's camera block, tile constants
+// and shade push constants: targets/vulkan/core.py:4452-4456,
+// wavefront_shade_candidate.glsl:482-498).
 struct GiParams {
   DScene sc;
   float cam_o[3], cam_f[3], cam_r[3], cam_u[3];
@@ -63,31 +31,107 @@ struct GiParams {
   uint32_t two_pass;          // NEE may emit more than one shadow ray
 };
 
-// Host launchers (gi.hip).
+// ---- batching limits and queue layout ------------------------------------------
+// A job's queues are cut into up to kJobStripes segments with one counter
+// each, so that slot reservations (one atomic per wave) spread over many
+// addresses: MI355X serialises device-scope atomics on one address.  A
+// (job, segment) pair is a run; consumers enumerate the runs' live entries in
+// 64-entry chunks through a prefix over all runs, so each chunk lies in one
+// job and a wave that takes a chunk is job-uniform.
+constexpr uint32_t kMaxJobs = 64;
+constexpr uint32_t kJobStripes = 32;
+constexpr uint32_t kRuns = kMaxJobs * kJobStripes;
+constexpr uint32_t kChunk = 64;
+constexpr uint32_t kWorkStripes = 32;  // work cursors of the refilling traversal kernels
+
+// Counter block (uint32 words).
+constexpr uint32_t C_Q0 = 0, C_Q1 = kRuns, C_G = 2 * kRuns, C_E = 3 * kRuns;
+constexpr uint32_t C_RP = 4 * kRuns;             // kRuns + 1: chunk prefix of the input ray queue
+constexpr uint32_t C_GP = C_RP + kRuns + 1;      // kRuns + 1: chunk prefix of the shadow groups
+constexpr uint32_t C_W = C_GP + kRuns + 1;       // kWorkStripes closest-hit cursors
+constexpr uint32_t C_WS = C_W + kWorkStripes;    // kWorkStripes shadow cursors
+constexpr uint32_t C_OVERFLOW = C_WS + kWorkStripes, C_STACK = C_OVERFLOW + 1;
+constexpr uint32_t C_REPORT = C_OVERFLOW + 8;    // kMaxJobs x 4: rays traced, shadow rays, live paths, shaded hits
+constexpr uint32_t C_WORDS = C_REPORT + 4 * kMaxJobs;
+OL_HD constexpr uint32_t c_queue(uint32_t q) { return q ? C_Q1 : C_Q0; }
+
+// Device job record (read wave-uniformly by the kernels).
+struct JobDev {
+  GiParams P;
+  uint32_t base;     // first path slot of the job's region; path = base + sample * W * H + pixel
+  uint32_t n_paths;  // spp * W * H
+  uint32_t stripes;  // queue segments, 1..kJobStripes
+  uint32_t seg;      // slots per segment: segment s holds slots base + s * seg ...
+  uint32_t ebase;    // first shadow-entry slot
+  uint32_t eseg;     // shadow-entry slots per segment
+  uint32_t pad[2];
+};
+
+// Accumulation of one finished sample chunk into its job's float64 sums.
+struct AccItem {
+  uint32_t base, np, spp, first_chunk;
+  uint32_t final_samples;  // > 0: also write the image divided by this count
+  uint32_t pad;
+  double4* acc;
+  float4* image;
+};
+
+// Per-step work lists written by the host before each step.
+struct StepList {
+  uint32_t n_gen, n_acc;
+  uint32_t gen_job[kMaxJobs];          // job slots whose primary rays are generated this step
+  uint32_t gen_prefix[kMaxJobs + 1];   // exclusive prefix of their n_paths
+  uint32_t acc_prefix[kMaxJobs + 1];   // exclusive prefix of the accumulation items' pixel counts
+  AccItem acc[kMaxJobs];
+};
+
+// Device pointers of an engine (kernel argument).
+struct Wave {
+  const JobDev* jobs;
+  const StepList* list;
+  float4* T;    // throughput rgb, bounce index
+  float4* L;    // radiance rgb, previous BSDF pdf
+  uint4* M;     // pixel, frame << 8 | sample, rng state, flags | medium depth << 8
+  float4* stk;  // 4 x float4 medium IOR stack per path (null when no resident job transmits)
+  float4* qo[2];
+  float4* qd[2];
+  uint32_t* qp[2];
+  float4* hit;  // (t, u, v, leaf slot bits) per ray-queue slot
+  float4* gT;   // shadow group: throughput rgb, NEE selection probability
+  uint4* gI;    // path, first entry, entry count, area count | env count << 8 | mode << 16
+  float4* eo;   // shadow entry: origin, tmin
+  float4* ed;   // direction, tmax
+  float4* ec;   // contribution rgb, kind (0 point, 1 area, 2 environment)
+  uint32_t* ctr;
+  unsigned long long* work;  // optional: closest nodes, closest tris, shadow nodes, shadow tris
+};
+
+// Traversal launch shape.
 struct GiLaunch {
   int cu_count;
   int stack_depth;   // total traversal stack entries needed (3 * BVH depth + 1)
-  int lds_stack;     // entries kept in LDS: 8, 12, 16, 24, 32
+  int lds_stack;     // entries kept in LDS: 8, 12, 16, 24, 32, 48
   int trace_block;   // 64, 128, 256
-  int scheduler;     // closest hit: 0 one ray per lane per grid-stride trip, 1 lanes refilled from 64-ray chunks
   uint32_t* spill;   // global spill columns for entries beyond lds_stack
   uint64_t spill_words;
 };
 
-// Resident grid of the traversal kernels and the spill words it needs.
 uint64_t trace_spill_words(const GiLaunch& l);
+int stack_depth_for(uint32_t bvh_depth);
 
-void gi_generate(const GiParams& p, const GiState& s, hipStream_t st);
-void gi_bounce_prologue(const GiState& s, uint32_t bounce, hipStream_t st);
-void gi_trace_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint32_t in, hipStream_t st);
-void gi_shade(const GiLaunch& l, const GiParams& p, const GiState& s, uint32_t in, hipStream_t st);
-void gi_trace_shadow(const GiLaunch& l, const DScene& sc, const GiState& s, hipStream_t st);
-void gi_accumulate(const GiParams& p, const GiState& s, double4* acc, bool first, hipStream_t st);
-void gi_final(const double4* acc, uint32_t np, uint32_t samples, float4* out, hipStream_t st);
-void gi_reset(const GiState& s, bool all, hipStream_t st);
-void gi_finish(const GiState& s, hipStream_t st);
+// One step of the batched wavefront: accumulate finished chunks, generate
+// newly admitted jobs, then trace / shade / shadow every resident path once,
+// and report per-job counts.  `in` is the parity of the input ray queue.
+// With `marks`, events are recorded around the closest-hit, shade and shadow
+// launches (profiling).
+void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
+void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
+void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
+void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
+void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st);
+void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
+void wave_clear(const Wave& w, hipStream_t st);
 
-// Debug / products: closest hit of caller-supplied rays.
+// Closest hit of caller-supplied rays (primary-hit products, debugging).
 void trace_rays(const GiLaunch& l, const DScene& sc, const float4* o, const float4* d, uint32_t n, float4* hits,
                 uint32_t* overflow, unsigned long long* work, hipStream_t st);
-int stack_depth_for(uint32_t bvh_depth);

@@ -1,509 +1,215 @@
-// GI wavefront kernels: generate -> [trace closest -> shade -> trace shadow]
-// x bounces -> resolve.
+// Batched GI wavefront kernels: per step, accumulate finished chunks ->
+// generate admitted jobs -> closest hit -> shade -> shadow rays -> report.
 //
 // The estimator is the reference's production split-path shade stage
 // (shaders/wavefront_shade_candidate.glsl main :3106-3493) for built-in PBR
-// materials without textures, custom programs or volumes: emission with MIS,
-// environment miss with MIS, point/spot/directional lights, unified or
-// per-domain area/environment next-event estimation, PBR/GGX BSDF sampling,
-// deterministic transmission with a 16-deep IOR stack, Russian roulette and
-// continuation.  Every expression keeps the GLSL evaluation order.
+// materials without textures, custom programs or volumes; the device
+// functions it uses are in shade.h.
 //
-// MI355X structure: the shadow rays of next-event estimation are deferred to
-// their own any-hit kernel instead of being traced inside shade; each hit's
-// visible contributions are summed in the reference's order and added before
-// the next bounce, so the radiance of every path is the same sequence of
-// float operations as the inline form.  Queue slots are reserved with one
-// atomic per wave (ballot + mbcnt) on one of 32 striped counters.  Counts
-// stay on the device; kernels use
-// grid-stride loops sized to the resident grid, so no host synchronisation
-// happens between bounces.
+// MI355X structure:
+// * All resident jobs share the path slots and queues; every kernel serves
+//   every job, so small jobs fill the GPU together (continuous batching).
+// * Queues are cut into (job, segment) runs with one counter each; a plan
+//   kernel turns the run counts into a prefix of 64-entry chunks, and a
+//   wave takes whole chunks, so its job is uniform and the job's parameters
+//   are scalar loads.
+// * The next-event-estimation shadow rays are deferred to their own any-hit
+//   kernel; each hit's visible contributions are summed in the reference's
+//   order and added before the next bounce, so each path's radiance is the
+//   same sequence of float operations as the inline form.
+// * The traversal kernels keep their grid resident: waves take chunks from
+//   striped cursors, refill lanes whose ray has finished, and choose per step
+//   between advancing lanes at inner nodes and lanes at leaves.
+// No host synchronisation happens inside a step; the host reads the per-job
+// report after it.
 
 #include <algorithm>
 
 #include "gi.h"
+#include "shade.h"
 #include "trace.h"
 
 namespace {
 
-constexpr uint32_t F_ACTIVE = 1u, F_PREV_DIFFUSE = 2u, F_PREV_UNIFIED = 4u, F_CAPTURE = 8u;
+using namespace olshade;
 
-// ---- materials ------------------------------------------------------------------------
-struct Mat {
-  float4 br;  // base colour, roughness
-  float4 em;  // emission, metallic
-  float4 at;  // attenuation colour, transmission
-  float4 id;  // ior, attenuation distance, program, two-sided emission
-  float4 tp;  // normal scale, occlusion binding, occlusion strength, resolved occlusion factor
-  float4 a0;  // clearcoat, clearcoat roughness, sheen roughness, anisotropy
-  float4 a1;  // subsurface, radius, thin walled, thickness
-  float4 sh;  // sheen colour
-  float4 ss;  // subsurface colour
-};
+constexpr uint32_t kRefillMin = 16;
 
-OL_DEV Mat load_mat(const float4* m, uint32_t id) {
-  const float4* r = m + 12u * id;
-  Mat x;
-  x.br = r[0];
-  x.em = r[1];
-  x.at = r[2];
-  x.id = r[3];
-  x.tp = r[5];
-  x.a0 = r[6];
-  x.a1 = r[7];
-  x.sh = r[8];
-  x.ss = r[9];
-  return x;
-}
-
-OL_DEV f3 rgb(float4 v) { return mk3(v.x, v.y, v.z); }
-OL_DEV float pow5(float x) { return ((x * x) * (x * x)) * x; }
-
-OL_DEV f3 cosine_hemisphere(f3 n, float u, float v) {
-  float radius = sqrtf(u);
-  float phi = OL_TWO_PI * v;
-  f3 tangent = cross3(n, mk3(0.0f, 1.0f, 0.0f));
-  if (fabsf(n.z) < 0.999f) tangent = cross3(n, mk3(0.0f, 0.0f, 1.0f));
-  tangent = normalize3(tangent);
-  f3 bitangent = cross3(n, tangent);
-  float sp, cp;
-  ol_sincos(phi, &sp, &cp);
-  return normalize3(((tangent * (radius * cp)) + (bitangent * (radius * sp))) + (n * sqrtf(fmaxf(0.0f, 1.0f - u))));
-}
-
-OL_DEV f3 fresnel_schlick(f3 f0, float c) {
-  float k = pow5(1.0f - clampf(c, 0.0f, 1.0f));
-  return f0 + (mk3(1.0f, 1.0f, 1.0f) - f0) * k;
-}
-
-OL_DEV float ggx_d(float nh, float roughness) {
-  float alpha = fmaxf(roughness * roughness, 0.0009f);
-  float a2 = alpha * alpha;
-  float den = ((nh * nh) * (a2 - 1.0f)) + 1.0f;
-  return a2 / fmaxf((OL_PI * den) * den, 1e-6f);
-}
-
-OL_DEV float ggx_g1(float nd, float roughness) {
-  float alpha = fmaxf(roughness * roughness, 0.0009f);
-  float a2 = alpha * alpha;
-  return (2.0f * nd) / fmaxf(nd + sqrtf(a2 + ((1.0f - a2) * nd) * nd), 1e-6f);
-}
-
-OL_DEV f3 f0_of(const Mat& m) { return mix3(mk3(0.04f, 0.04f, 0.04f), rgb(m.br), m.em.w); }
-
-OL_DEV float spec_probability(const Mat& m) {
-  f3 f0 = f0_of(m);
-  return clampf(fmaxf(f0.x, fmaxf(f0.y, f0.z)) + m.a0.x * 0.15f, 0.1f, 0.95f);
-}
-
-// shadeEvaluatePbrLobes (:1438-1468); returns diffuse + specular.
-OL_DEV f3 eval_pbr(const Mat& m, f3 n, f3 view, f3 out) {
-  float nv = fmaxf(dot3(n, view), 0.0f);
-  float nl = fmaxf(dot3(n, out), 0.0f);
-  if (nv <= 0.0f || nl <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
-  f3 h = normalize3(view + out);
-  float nh = fmaxf(dot3(n, h), 0.0f);
-  float vh = fmaxf(dot3(view, h), 0.0f);
-  float metallic = m.em.w;
-  f3 base = rgb(m.br);
-  f3 fr = fresnel_schlick(f0_of(m), vh);
-  float eff_r = m.br.w * (1.0f - 0.25f * fabsf(clampf(m.a0.w, -1.0f, 1.0f)));
-  float D = ggx_d(nh, eff_r);
-  float G = ggx_g1(nv, eff_r) * ggx_g1(nl, eff_r);
-  float den = fmaxf((4.0f * nv) * nl, 1e-6f);
-  f3 spec = ((fr * D) * G) / den;
-  f3 diffuse = (((mk3(1.0f, 1.0f, 1.0f) - fr) * (1.0f - metallic)) * base) / OL_PI;
-  float subsurface = clampf(m.a1.x, 0.0f, 1.0f);
-  diffuse = mix3(diffuse, diffuse * rgb(m.ss), subsurface);
-  float sv = pow5(1.0f - vh);
-  f3 sheen = rgb(m.sh) * ((1.0f - clampf(m.a0.z, 0.0f, 1.0f)) * sv);
-  float clearcoat = clampf(m.a0.x, 0.0f, 1.0f);
-  float coat_r = clampf(m.a0.y, 0.02f, 1.0f);
-  float cD = ggx_d(nh, coat_r);
-  float cG = ggx_g1(nv, coat_r) * ggx_g1(nl, coat_r);
-  float cF = 0.04f + 0.96f * sv;
-  float coat = (((clearcoat * cD) * cG) * cF) / den;
-  float layer = 1.0f - clearcoat * cF;
-  f3 d = diffuse * layer;
-  f3 s = ((spec + sheen) * layer) + mk3(coat, coat, coat);
-  return d + s;
-}
-
-OL_DEV float pdf_pbr(const Mat& m, f3 n, f3 view, f3 out) {
-  float nl = fmaxf(dot3(n, out), 0.0f);
-  if (nl <= 0.0f) return 0.0f;
-  f3 h = normalize3(view + out);
-  float nh = fmaxf(dot3(n, h), 0.0f);
-  float vh = fmaxf(dot3(view, h), 1e-6f);
-  float sp = (ggx_d(nh, m.br.w) * nh) / (4.0f * vh);
-  float dp = nl / OL_PI;
-  return mixf(dp, sp, spec_probability(m));
-}
-
-OL_DEV f3 ggx_half(f3 n, float roughness, float u, float v) {
-  float alpha = fmaxf(roughness * roughness, 0.0009f);
-  float a2 = alpha * alpha;
-  float phi = OL_TWO_PI * v;
-  float c = sqrtf((1.0f - u) / fmaxf(1.0f + (a2 - 1.0f) * u, 1e-6f));
-  float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
-  f3 tangent = cross3(n, mk3(0.0f, 1.0f, 0.0f));
-  if (fabsf(n.z) < 0.999f) tangent = cross3(n, mk3(0.0f, 0.0f, 1.0f));
-  tangent = normalize3(tangent);
-  f3 bitangent = cross3(n, tangent);
-  float sp, cp;
-  ol_sincos(phi, &sp, &cp);
-  return normalize3((((tangent * s) * cp) + ((bitangent * s) * sp)) + (n * c));
-}
-
-OL_DEV float power_heuristic(float a, float b) {
-  float a2 = a * a, b2 = b * b;
-  return a2 / fmaxf(a2 + b2, 1e-6f);
-}
-
-OL_DEV f3 analytic_environment(f3 d) {
-  float sky = fmaxf(d.y, 0.0f);
-  return mix3(mk3(0.018f, 0.022f, 0.032f), mk3(0.32f, 0.46f, 0.72f), sky);
-}
-
-OL_DEV f3 environment_radiance(const DScene& sc, f3 d) {
-  if (!sc.env_valid) return analytic_environment(d);
-  return mk3(sc.env_r, sc.env_g, sc.env_b);
-}
-
-OL_DEV float unified_area_probability(const GiParams& P) {
-  if (P.area_count == 0u) return 0.0f;
-  if (P.environment_samples == 0u) return 1.0f;
-  float w = sqrtf(fmaxf(P.sc.area_weight, 1e-6f));
-  return w / (w + 1.0f);
-}
-
-OL_DEV bool select_secondary_nee(float probability, uint32_t pixel, uint32_t frame_sample, uint32_t bounce) {
-  if (probability >= 0.999999f) return true;
-  uint32_t frame = frame_sample >> 8, sample = frame_sample & 255u;
-  uint32_t scramble = wave_hash((pixel ^ wave_hash(bounce + 1u)) ^ wave_hash(sample + 1u));
-  uint32_t sequence = bit_reverse32(frame) ^ scramble;
-  float selector = ((float)sequence + 0.5f) * (1.0f / 4294967296.0f);
-  return selector < probability;
-}
-
-// ---- next-event samples ---------------------------------------------------------------
-struct NeeSample {
-  f3 origin, dir;
-  float tmax;
-  f3 c;        // unoccluded contribution (lobe contribution of the reference)
-  uint32_t kind;
-  bool valid;
-};
-
-OL_DEV NeeSample point_sample(const DScene& sc, uint32_t li, f3 hit, f3 n, const Mat& m, f3 incoming) {
-  NeeSample s;
-  s.valid = false;
-  s.kind = 0;
-  const float4* L = sc.analytic + 4u * li;
-  float4 pt = L[0], dr = L[1], ci = L[2], sp = L[3];
-  int type = (int)(pt.w + 0.5f);
-  if (type == 3) return s;
-  f3 dir = mk3(0.0f, 0.0f, 0.0f);
-  float dist2 = 1.0f, dist = 10000.0f, att = 1.0f;
-  if (type == 1) {
-    dir = -normalize3(rgb(dr));
-  } else {
-    f3 off = rgb(pt) - hit;
-    dist2 = fmaxf(dot3(off, off), 1e-6f);
-    dist = sqrtf(dist2);
-    dir = off / dist;
-    if (dr.w > 0.0f && dist > dr.w) return s;
-    att = 1.0f / dist2;
-    if (type == 2) {
-      float cone = dot3(normalize3(rgb(dr)), -dir);
-      float amount = clampf((cone - sp.y) / fmaxf(sp.x - sp.y, 1e-6f), 0.0f, 1.0f);
-      float spot = (amount * amount) * (3.0f - 2.0f * amount);
-      if (spot <= 0.0f) return s;
-      att = att * spot;
-    }
+// ---- chunk plan ----------------------------------------------------------------------
+// Exclusive prefix of ceil(count / kChunk) over all runs; out[kRuns] = total.
+__global__ void __launch_bounds__(1024) k_plan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ out) {
+  static_assert(kRuns == 2 * 1024, "two runs per thread");
+  __shared__ uint32_t wave_tot[16];
+  const uint32_t t = threadIdx.x;
+  const uint32_t a = (counts[2 * t] + kChunk - 1u) / kChunk, b = (counts[2 * t + 1] + kChunk - 1u) / kChunk;
+  const uint32_t sum = a + b;
+  uint32_t incl = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(incl, (unsigned)off, 64);
+    if ((int)lane_id() >= off) incl += o;
   }
-  float cosine = fmaxf(dot3(n, dir), 0.0f);
-  if (cosine <= 0.0f) return s;
-  s.origin = hit + n * 0.002f;
-  s.tmax = fmaxf(dist - 0.004f, 0.001f);
-  if (type == 1) s.tmax = 10000.0f;
-  f3 incident = (rgb(ci) * ci.w) * att;
-  s.dir = dir;
-  // shadePointLightContribution with unit volume transmittance.
-  s.c = ((eval_pbr(m, n, -incoming, dir) * incident) * mk3(1.0f, 1.0f, 1.0f)) * cosine;
-  s.valid = true;
-  return s;
-}
-
-OL_DEV uint32_t select_area_light(const DScene& sc, float selection) {
-  uint32_t lower = 0, upper = sc.area_count - 1u;
-  for (int step = 0; step < 32; ++step) {
-    if (lower >= upper) break;
-    uint32_t middle = lower + (upper - lower) / 2u;
-    if (selection <= sc.area[5u * middle + 4u].x) upper = middle; else lower = middle + 1u;
-  }
-  return lower;
-}
-
-OL_DEV NeeSample area_sample(const DScene& sc, uint32_t& rs, f3 hit, f3 n, const Mat& m, f3 incoming,
-                             uint32_t sample_index, uint32_t sample_count, float technique) {
-  NeeSample s;
-  s.valid = false;
-  s.kind = 1;
-  if (sc.area_count == 0u) return s;
-  rs = pcg_step(rs);
-  float selection = ((float)sample_index + pcg_value(rs)) / (float)sample_count;
-  uint32_t li = select_area_light(sc, selection);
-  const float4* A = sc.area + 5u * li;
-  f3 a = rgb(A[0]), b = rgb(A[1]), c = rgb(A[2]);
-  float4 ea = A[3], dist = A[4];
-  rs = pcg_step(rs);
-  float ru = sqrtf(pcg_value(rs));
-  rs = pcg_step(rs);
-  float vv = pcg_value(rs);
-  f3 lp = (((1.0f - ru) * a) + ((ru * (1.0f - vv)) * b)) + ((ru * vv) * c);
-  f3 off = lp - hit;
-  float d2 = dot3(off, off);
-  float dl = sqrtf(d2);
-  f3 dir = off / fmaxf(dl, 1e-6f);
-  float sc_cos = fmaxf(dot3(n, dir), 0.0f);
-  f3 ln = normalize3(cross3(b - a, c - a));
-  float raw = dot3(ln, -dir);
-  float lc = fmaxf(raw, 0.0f);
-  if (dist.z > 0.5f) lc = fabsf(raw);
-  if (sc_cos <= 0.0f || lc <= 1e-6f) return s;
-  float light_pdf = (dist.y * d2) / fmaxf(lc * ea.w, 1e-6f);
-  float eff = light_pdf * fmaxf(technique, 1e-6f);
-  s.origin = hit + n * 0.002f;
-  s.dir = dir;
-  s.tmax = fmaxf(dl - 0.004f, 0.001f);
-  // shadeAreaLightContribution with unit volume transmittance.
-  f3 view = -incoming;
-  float bp = pdf_pbr(m, n, view, dir);
-  float mis = power_heuristic(eff * (float)sample_count, bp);
-  s.c = ((((eval_pbr(m, n, view, dir) * rgb(ea)) * sc_cos) * mis) * mk3(1.0f, 1.0f, 1.0f)) / fmaxf(eff, 1e-6f);
-  s.valid = true;
-  return s;
-}
-
-OL_DEV NeeSample env_sample(const DScene& sc, uint32_t& rs, f3 hit, f3 n, const Mat& m, f3 incoming,
-                            uint32_t sample_count, float technique) {
-  NeeSample s;
-  s.kind = 2;
-  rs = pcg_step(rs);
-  float u = pcg_value(rs);
-  rs = pcg_step(rs);
-  float v = pcg_value(rs);
-  f3 dir = cosine_hemisphere(n, u, v);
-  float cosine = fmaxf(dot3(n, dir), 0.0f);
-  float pdf = cosine / OL_PI;
-  float eff = pdf * fmaxf(technique, 1e-6f);
-  s.origin = hit + n * 0.002f;
-  s.dir = dir;
-  s.tmax = 1e30f;
-  s.valid = cosine > 0.0f;
-  if (!s.valid) return s;
-  // shadeEnvironmentContribution with unit volume transmittance.
-  float pdf2 = cosine / OL_PI;
-  float mis = power_heuristic(eff * (float)sample_count, pdf2);
-  float mw = mixf(m.tp.w, 1.0f, m.em.w);
-  s.c = (((((eval_pbr(m, n, -incoming, dir) * environment_radiance(sc, dir)) * cosine) * mis) * mk3(1.0f, 1.0f, 1.0f)) /
-         fmaxf(eff, 1e-6f)) * mw;
-  return s;
-}
-
-// Visits every next-event sample of one hit in the reference's order and
-// calls emit(sample) for the valid ones.  Returns counts for the divisors.
-template <class Emit>
-OL_DEV void for_each_nee(const GiParams& P, uint32_t& rs, f3 hit, f3 n, const Mat& m, f3 incoming, uint32_t& area_n,
-                         uint32_t& env_n, Emit&& emit) {
-  const DScene& sc = P.sc;
-  for (uint32_t li = 0; li < 64u; ++li) {
-    if (li >= P.point_count) break;
-    NeeSample s = point_sample(sc, li, hit, n, m, incoming);
-    if (s.valid) emit(s);
-  }
-  area_n = 0;
-  env_n = 0;
-  if (P.unified) {
-    bool area_on = P.area_count > 0u, env_on = P.environment_samples > 0u;
-    if (!area_on && !env_on) return;
-    float aw = 0.0f;
-    if (area_on) aw = sqrtf(fmaxf(sc.area_weight, 1e-6f)) * (float)(P.secondary_area_samples > 1u ? P.secondary_area_samples : 1u);
-    float ew = 0.0f;
-    if (env_on) ew = (float)(P.environment_samples < 4u ? P.environment_samples : 4u);
-    float ap = aw / fmaxf(aw + ew, 1e-6f);
-    rs = pcg_step(rs);
-    bool area_sel = pcg_value(rs) < ap;
-    if (area_sel) {
-      NeeSample s = area_sample(sc, rs, hit, n, m, incoming, 0u, 1u, ap);
-      if (s.valid) emit(s);
-    } else {
-      NeeSample s = env_sample(sc, rs, hit, n, m, incoming, 1u, 1.0f - ap);
-      if (s.valid) emit(s);
-    }
-    return;
-  }
-  uint32_t ac = P.secondary_area_samples < 1u ? 1u : (P.secondary_area_samples > 16u ? 16u : P.secondary_area_samples);
-  area_n = ac;
-  for (uint32_t k = 0; k < 16u; ++k) {
-    if (k >= ac) break;
-    NeeSample s = area_sample(sc, rs, hit, n, m, incoming, k, ac, 1.0f);
-    if (s.valid) emit(s);
-  }
-  uint32_t ec = P.environment_samples < 4u ? P.environment_samples : 4u;
-  env_n = ec;
-  for (uint32_t k = 0; k < 4u; ++k) {
-    if (k >= ec) break;
-    NeeSample s = env_sample(sc, rs, hit, n, m, incoming, ec, 1.0f);
-    if (s.valid) emit(s);
-  }
-}
-
-// ---- medium stack -----------------------------------------------------------------------------
-OL_DEV float stack_get(const float4* st, uint32_t i) {
-  float4 v = st[i >> 2];
-  uint32_t k = i & 3u;
-  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
-}
-OL_DEV void stack_set(float4* st, uint32_t i, float value) {
-  float4 v = st[i >> 2];
-  uint32_t k = i & 3u;
-  if (k == 0) v.x = value; else if (k == 1) v.y = value; else if (k == 2) v.z = value; else v.w = value;
-  st[i >> 2] = v;
-}
-
-// GLSL refract(I, N, eta).
-OL_DEV f3 refract3(f3 I, f3 N, float eta) {
-  float ndi = dot3(N, I);
-  float k = 1.0f - eta * eta * (1.0f - ndi * ndi);
-  if (k < 0.0f) return mk3(0.0f, 0.0f, 0.0f);
-  return eta * I - (eta * ndi + sqrtf(k)) * N;
-}
-
-// ---- striped queues ---------------------------------------------------------------------------
-// Inclusive prefix of the kStripes counts at `counts` into `incl` (LDS).
-// Every thread of the block calls it; wave 0 does the scan.
-OL_DEV void stripe_prefix(const uint32_t* counts, uint32_t* incl) {
-  if (threadIdx.x < 64u) {
-    uint32_t v = threadIdx.x < kStripes ? counts[threadIdx.x] : 0u;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t o = __shfl_up(v, (unsigned)off, 64);
-      if ((int)threadIdx.x >= off) v += o;
-    }
-    if (threadIdx.x < kStripes) incl[threadIdx.x] = v;
-  }
+  const uint32_t wave = t / 64u;
+  if (lane_id() == 63u) wave_tot[wave] = incl;
   __syncthreads();
+  uint32_t base = 0, total = 0;
+  for (uint32_t w = 0; w < 16u; ++w) {
+    if (w < wave) base += wave_tot[w];
+    total += wave_tot[w];
+  }
+  const uint32_t excl = base + incl - sum;
+  out[2 * t] = excl;
+  out[2 * t + 1] = excl + a;
+  if (t == 0) out[kRuns] = total;
 }
 
-// Slot of the j-th live entry (j < incl[kStripes - 1]): stripe s holds
-// entries [incl[s-1], incl[s]) at slots s * seg + 0, 1, ...
-OL_DEV uint32_t stripe_slot(const uint32_t* incl, uint32_t seg, uint32_t j) {
-  uint32_t s = 0;
+// Run holding chunk c: the largest r with prefix[r] <= c.
+OL_DEV uint32_t find_run(const uint32_t* __restrict__ prefix, uint32_t c) {
+  uint32_t r = 0;
 #pragma unroll
-  for (uint32_t step = kStripes / 2u; step; step >>= 1)
-    if (incl[s + step - 1u] <= j) s += step;
-  return s * seg + (j - (s ? incl[s - 1u] : 0u));
+  for (uint32_t step = kRuns / 2u; step; step >>= 1)
+    if (prefix[r + step] <= c) r += step;
+  return r;
 }
+
+// A chunk located in its run: job slot, segment and the first queue slot.
+struct ChunkLoc {
+  uint32_t job, stripe, first, count;  // count: live entries in this chunk (1..64)
+  uint32_t chunk_in_job;               // chunk index counted from the job's first run
+};
+
+OL_DEV ChunkLoc locate_chunk(const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ counts,
+                             const JobDev* __restrict__ jobs, uint32_t c) {
+  ChunkLoc k;
+  const uint32_t r = find_run(prefix, c);
+  k.job = r / kJobStripes;
+  k.stripe = r % kJobStripes;
+  const uint32_t off = (c - prefix[r]) * kChunk;
+  k.count = min(kChunk, counts[r] - off);
+  k.first = jobs[k.job].base + k.stripe * jobs[k.job].seg + off;
+  k.chunk_in_job = c - prefix[k.job * kJobStripes];
+  return k;
+}
+
+OL_DEV uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 OL_DEV uint32_t wave_sum(uint32_t v) {
   for (int off = 32; off; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
-// ---- kernels ----------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_generate(GiParams P, GiState S) {
-  const uint32_t W = P.width, H = P.height;
-  const uint32_t tiles_x = (W + 7u) / 8u, tiles_y = (H + 7u) / 8u;
-  const uint32_t per_sample = tiles_x * tiles_y * 64u;
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t local = g / per_sample;
-  const uint32_t rem = g - local * per_sample;
-  const uint32_t sample = P.sample_base + local;
-  const uint32_t tile = rem >> 6, lane = rem & 63u;
-  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-  const uint32_t lx = lane & 7u, ly = lane >> 3;
-  const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
-  // The primary queue is dense: all entries in stripe 0 (slot = index).
-  if (g == 0) S.ctr[CTR_Q0] = P.spp * W * H;
-  if (!(local < P.spp && px < W && py < H)) return;
-  // Queue slot in 8x8 tile order without atomics: full tile rows above,
-  // full-width tiles to the left in this tile row, row-major inside the tile.
-  const uint32_t th = min(8u, H - 8u * ty), tw = min(8u, W - 8u * tx);
-  const uint32_t slot = local * (W * H) + 8u * ty * W + 8u * tx * th + ly * tw + lx;
-  const uint32_t pixel = py * W + px;
-  const uint32_t path = local * (W * H) + pixel;
-  uint32_t rng = wave_hash((pixel ^ wave_hash(P.frame)) ^ wave_hash(sample + 1u));
-  rng = wave_hash(rng);
-  const float jx = u32_to_unit(rng);
-  rng = wave_hash(rng);
-  const float jy = u32_to_unit(rng);
-  const float ndx = (((float)px + jx) / (float)W) * 2.0f - 1.0f;
-  const float ndy = (((float)py + jy) / (float)H) * 2.0f - 1.0f;
-  const float aspect = (float)W / (float)H;
-  f3 org = mk3(P.cam_o[0], P.cam_o[1], P.cam_o[2]);
-  const f3 fw = mk3(P.cam_f[0], P.cam_f[1], P.cam_f[2]);
-  const f3 rt = mk3(P.cam_r[0], P.cam_r[1], P.cam_r[2]);
-  const f3 up = mk3(P.cam_u[0], P.cam_u[1], P.cam_u[2]);
-  f3 dir;
-  if (P.projection == 1u) {
-    org = (org + ((ndx * aspect) * rt)) - (ndy * up);
-    dir = normalize3(fw);
-  } else if (P.projection == 2u) {
-    float yaw = ndx * length3(rt);
-    float pitch = (-ndy) * length3(up);
-    float sy, cy, sp, cp;
-    ol_sincos(yaw, &sy, &cy);
-    ol_sincos(pitch, &sp, &cp);
-    dir = normalize3((((normalize3(fw) * cp) * cy) + ((normalize3(rt) * cp) * sy)) + (normalize3(up) * sp));
-  } else {
-    dir = normalize3((fw + ((ndx * aspect) * rt)) - (ndy * up));
-  }
-  S.qo[0][slot] = mk4(org, 0.001f);
-  S.qd[0][slot] = mk4(dir, 1e30f);
-  S.qp[0][slot] = path;
-  S.T[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  S.L[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  S.M[path] = make_uint4(pixel, (P.frame << 8) | (sample & 255u), rng, 257u);
-  if (S.stk) S.stk[4u * path] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+// ---- accumulate + generate (host lists) ------------------------------------------------------
+// Index of the list item owning global thread index g (prefix[n] = total).
+OL_DEV uint32_t list_item(const uint32_t* prefix, uint32_t n, uint32_t g) {
+  uint32_t k = 0;
+  while (k + 1u < n && prefix[k + 1u] <= g) ++k;
+  return k;
 }
 
-// One wave: adds the input queue and the previous bounce's shadow entries to
-// the ray statistics, then empties the output queue and the shadow queues.
-__global__ void __launch_bounds__(64) k_prologue(GiState S, uint32_t bounce) {
-  const uint32_t in = bounce & 1u, out = in ^ 1u, k = threadIdx.x;
-  const uint32_t q = wave_sum(k < kStripes ? S.ctr[ctr_queue(in) + k] : 0u);
-  const uint32_t e = wave_sum(k < kStripes ? S.ctr[CTR_E + k] : 0u);
-  if (k == 0) {
-    S.stats[bounce == 0 ? 0 : 1] += q;
-    S.stats[2] += e;
-  }
-  if (k < kStripes) {
-    S.ctr[ctr_queue(out) + k] = 0u;
-    S.ctr[CTR_G + k] = 0u;
-    S.ctr[CTR_E + k] = 0u;
-    S.ctr[CTR_W + k] = 0u;
-    S.ctr[CTR_WS + k] = 0u;
+// Adds the float32 radiance of a finished chunk's samples to its job's
+// float64 pixel sums in sample order, then writes the image after the last
+// chunk (render_wavefront_outputs, api.py:1404-1422).
+__global__ void __launch_bounds__(256) k_accumulate(Wave W) {
+  const StepList& Ls = *W.list;
+  const uint32_t n = Ls.n_acc;
+  if (n == 0) return;
+  const uint32_t total = Ls.acc_prefix[n];
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const uint32_t k = list_item(Ls.acc_prefix, n, g);
+    const AccItem& it = Ls.acc[k];
+    const uint32_t pixel = g - Ls.acc_prefix[k];
+    double4 a = it.first_chunk ? make_double4(0.0, 0.0, 0.0, 0.0) : it.acc[pixel];
+    for (uint32_t s = 0; s < it.spp; ++s) {
+      const float4 l = W.L[it.base + s * it.np + pixel];
+      if (it.first_chunk && s == 0) {
+        a = make_double4((double)l.x, (double)l.y, (double)l.z, 1.0);
+      } else {
+        a.x += (double)l.x;
+        a.y += (double)l.y;
+        a.z += (double)l.z;
+        a.w += 1.0;
+      }
+    }
+    it.acc[pixel] = a;
+    if (it.final_samples) {
+      const double ns = (double)it.final_samples;
+      it.image[pixel] = make_float4((float)(a.x / ns), (float)(a.y / ns), (float)(a.z / ns), (float)(a.w / ns));
+    }
   }
 }
 
-__global__ void __launch_bounds__(64) k_finish(GiState S) {
-  const uint32_t k = threadIdx.x;
-  const uint32_t e = wave_sum(k < kStripes ? S.ctr[CTR_E + k] : 0u);
-  if (k == 0) S.stats[2] += e;
-  if (k < kStripes) S.ctr[CTR_E + k] = 0u;
+// Camera rays and path initialisation of the listed jobs
+// (wavefront_generate.comp:86-167).  Queue slots follow 8x8 pixel tiles, so a
+// wave's rays are coherent; the job's queue segments are filled in order
+// (slot base + i for the i-th ray in tile order).
+__global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
+  const StepList& Ls = *W.list;
+  const uint32_t n = Ls.n_gen;
+  if (n == 0) return;
+  const uint32_t total = Ls.gen_prefix[n];
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const uint32_t k = list_item(Ls.gen_prefix, n, g);
+    const uint32_t job = Ls.gen_job[k];
+    const JobDev& J = W.jobs[job];
+    const GiParams& P = J.P;
+    const uint32_t i = g - Ls.gen_prefix[k];
+    const uint32_t Wd = P.width, H = P.height, np = Wd * H;
+    if (i == 0) {
+      for (uint32_t s = 0; s < J.stripes; ++s) {
+        const uint32_t lo = s * J.seg;
+        W.ctr[c_queue(in) + job * kJobStripes + s] = J.n_paths > lo ? min(J.seg, J.n_paths - lo) : 0u;
+      }
+    }
+    // i -> (sample, 8x8 tile, position in tile) in tile order.
+    const uint32_t local = i / np, rem = i - local * np;
+    const uint32_t ty = rem / (8u * Wd);            // tile row
+    const uint32_t th = min(8u, H - 8u * ty);       // rows in this tile row
+    const uint32_t in_row = rem - ty * 8u * Wd;     // index inside the tile row
+    const uint32_t tx = in_row / (8u * th);         // full-height tiles to the left are 8 wide
+    const uint32_t tw = min(8u, Wd - 8u * tx);
+    const uint32_t in_tile = in_row - tx * 8u * th;
+    const uint32_t ly = in_tile / tw, lx = in_tile - ly * tw;
+    const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
+    const uint32_t sample = P.sample_base + local;
+    const uint32_t pixel = py * Wd + px;
+    const uint32_t path = J.base + local * np + pixel;
+    const uint32_t slot = J.base + i;
+    uint32_t rng = wave_hash((pixel ^ wave_hash(P.frame)) ^ wave_hash(sample + 1u));
+    rng = wave_hash(rng);
+    const float jx = u32_to_unit(rng);
+    rng = wave_hash(rng);
+    const float jy = u32_to_unit(rng);
+    const float ndx = (((float)px + jx) / (float)Wd) * 2.0f - 1.0f;
+    const float ndy = (((float)py + jy) / (float)H) * 2.0f - 1.0f;
+    const float aspect = (float)Wd / (float)H;
+    f3 org = mk3(P.cam_o[0], P.cam_o[1], P.cam_o[2]);
+    const f3 fw = mk3(P.cam_f[0], P.cam_f[1], P.cam_f[2]);
+    const f3 rt = mk3(P.cam_r[0], P.cam_r[1], P.cam_r[2]);
+    const f3 up = mk3(P.cam_u[0], P.cam_u[1], P.cam_u[2]);
+    f3 dir;
+    if (P.projection == 1u) {
+      org = (org + ((ndx * aspect) * rt)) - (ndy * up);
+      dir = normalize3(fw);
+    } else if (P.projection == 2u) {
+      float yaw = ndx * length3(rt);
+      float pitch = (-ndy) * length3(up);
+      float sy, cy, sp, cp;
+      ol_sincos(yaw, &sy, &cy);
+      ol_sincos(pitch, &sp, &cp);
+      dir = normalize3((((normalize3(fw) * cp) * cy) + ((normalize3(rt) * cp) * sy)) + (normalize3(up) * sp));
+    } else {
+      dir = normalize3((fw + ((ndx * aspect) * rt)) - (ndy * up));
+    }
+    W.qo[in][slot] = mk4(org, 0.001f);
+    W.qd[in][slot] = mk4(dir, 1e30f);
+    W.qp[in][slot] = path;
+    W.T[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    W.L[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.M[path] = make_uint4(pixel, (P.frame << 8) | (sample & 255u), rng, 257u);
+    if (W.stk) W.stk[4u * path] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
-// Whole render: counters, overflow flags, statistics.  Per chunk
-// (`all` = 0): the queue counters only; overflow flags accumulate.
-__global__ void __launch_bounds__(256) k_reset(GiState S, uint32_t all) {
-  const uint32_t k = threadIdx.x;
-  if (k < (all ? CTR_WORDS : CTR_OVERFLOW)) S.ctr[k] = 0u;
-  if (all && k < 4) S.stats[k] = 0;
-  if (all && S.work && k < 4) S.work[k] = 0;
-}
-
-// Lane-private traversal stack: LDS column + global spill column.
+// ---- traversal ------------------------------------------------------------------------------
 template <int LSTACK, int BLOCK>
 OL_DEV TravStack<LSTACK> lane_stack(uint32_t* lds, uint32_t* spill, uint32_t spill_cap, uint32_t* overflow) {
   TravStack<LSTACK> st;
@@ -517,127 +223,55 @@ OL_DEV TravStack<LSTACK> lane_stack(uint32_t* lds, uint32_t* spill, uint32_t spi
   return st;
 }
 
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_closest(DScene sc, GiState S, uint32_t in, uint32_t* spill,
-                                                         uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  __shared__ uint32_t incl[kStripes];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
-  stripe_prefix(S.ctr + ctr_queue(in), incl);
-  const uint32_t count = incl[kStripes - 1u];
-  const float4* __restrict__ qo = S.qo[in];
-  const float4* __restrict__ qd = S.qd[in];
-  uint32_t nv = 0, nt = 0;
-  for (uint32_t j = blockIdx.x * BLOCK + threadIdx.x; j < count; j += gridDim.x * BLOCK) {
-    const uint32_t i = stripe_slot(incl, S.qseg, j);
-    float4 o = qo[i], d = qd[i];
-    TraceHit h = ol_trace_closest<LSTACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, st, nv, nt);
-    S.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.slot));
+// Wave-uniform pool of rays not yet started: `n` entries from queue slot
+// `slot` of one chunk, all in one job.
+struct Pool {
+  uint32_t slot, n;
+  const WideNode* nodes;
+  const float4* tri;
+  uint32_t node_count;
+  bool exhausted;
+};
+
+// Takes the next chunk of this block's work stripe when the pool is empty.
+OL_DEV void pool_refill(Pool& p, const Wave& W, const uint32_t* prefix, const uint32_t* counts, uint32_t* cursor,
+                        uint32_t c_begin, uint32_t c_end) {
+  if (p.n != 0 || p.exhausted) return;
+  uint32_t c = 0;
+  if (lane_id() == 0) c = atomicAdd(cursor, 1u);
+  c = uniform(__shfl(c, 0, 64));
+  if (c >= c_end - c_begin) {
+    p.exhausted = true;
+    return;
   }
-  if (S.work && (nv | nt)) {
-    atomicAdd(&S.work[0], (unsigned long long)nv);
-    atomicAdd(&S.work[1], (unsigned long long)nt);
-  }
+  const ChunkLoc k = locate_chunk(prefix, counts, W.jobs, c_begin + c);
+  const DScene& sc = W.jobs[uniform(k.job)].P.sc;
+  p.slot = uniform(k.first);
+  p.n = uniform(k.count);
+  p.nodes = sc.nodes;
+  p.tri = sc.tri;
+  p.node_count = sc.node_count;
 }
 
-// Closest hit with lane refill.  Block b works on stripe b % kStripes of the
-// dense input; its waves take 64-ray chunks of that stripe from a global
-// cursor (one atomic per chunk) and hand rays to lanes whose traversal has
-// finished whenever at least kRefillMin lanes are idle, so a wave no longer
-// waits for its slowest ray before starting new ones.  Results are the same
-// as k_trace_closest's: each ray's traversal is independent of scheduling.
-constexpr uint32_t kChunk = 64, kRefillMin = 16;
-
+// Closest hit for the input ray queue: refilled lanes, split inner / leaf steps.
 template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_closest_pool(DScene sc, GiState S, uint32_t in, uint32_t* spill,
-                                                              uint32_t spill_cap) {
+__global__ void __launch_bounds__(BLOCK) k_closest(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
   __shared__ uint32_t lds[LSTACK * BLOCK];
-  __shared__ uint32_t incl[kStripes];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
-  stripe_prefix(S.ctr + ctr_queue(in), incl);
-  const uint32_t count = incl[kStripes - 1u];
-  const uint32_t ws = blockIdx.x % kStripes;
-  const uint32_t w_begin = (uint32_t)((uint64_t)count * ws / kStripes);
-  const uint32_t w_end = (uint32_t)((uint64_t)count * (ws + 1u) / kStripes);
-  uint32_t* cursor = &S.ctr[CTR_W + ws];
-  const float4* __restrict__ qo = S.qo[in];
-  const float4* __restrict__ qd = S.qd[in];
-  uint32_t pool = 0, pool_end = 0;  // wave-uniform: dense rays [pool, pool_end) not yet started
-  bool exhausted = false;            // wave-uniform: the stripe has no chunks left
-  ClosestRay q;
-  uint32_t slot = OL_NONE;
-  bool busy = false;
-  uint32_t nv = 0, nt = 0;
-  while (true) {
-    const uint64_t idle = __ballot(!busy);
-    const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (n_idle >= kRefillMin) {
-      if (pool == pool_end && !exhausted) {
-        uint32_t c = 0;
-        if (lane_id() == 0) c = atomicAdd(cursor, kChunk);
-        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
-        const uint32_t remaining = w_end - w_begin;
-        if (c >= remaining) {
-          exhausted = true;
-        } else {
-          pool = w_begin + c;
-          pool_end = w_begin + min(c + kChunk, remaining);
-        }
-      }
-      const uint32_t take = min(n_idle, pool_end - pool);
-      if (!busy) {
-        const uint32_t rank = lanes_below(idle);
-        if (rank < take) {
-          slot = stripe_slot(incl, S.qseg, pool + rank);
-          const float4 o = qo[slot], d = qd[slot];
-          closest_begin(q, mk3(o), mk3(d), o.w, d.w);
-          if (sc.node_count == 0) {
-            S.hit[slot] = make_float4(d.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
-          } else {
-            st.sp = 0;
-            busy = true;
-          }
-        }
-      }
-      pool += take;
-    }
-    if (__ballot(busy) == 0) {
-      if (exhausted && pool == pool_end) break;
-      continue;
-    }
-    if (busy && closest_step<LSTACK>(sc.nodes, sc.tri, q, st, nv, nt)) {
-      S.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
-      busy = false;
-    }
-  }
-  if (S.work && (nv | nt)) {
-    atomicAdd(&S.work[0], (unsigned long long)nv);
-    atomicAdd(&S.work[1], (unsigned long long)nt);
-  }
-}
-
-// Lane refill as in k_trace_closest_pool, with split inner / leaf steps: a
-// wave advances its lanes at leaves when at least a third of its busy lanes
-// hold one (or none can take an inner step), its lanes at inner nodes
-// otherwise.
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_closest_split(DScene sc, GiState S, uint32_t in, uint32_t* spill,
-                                                               uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  __shared__ uint32_t incl[kStripes];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
-  stripe_prefix(S.ctr + ctr_queue(in), incl);
-  const uint32_t count = incl[kStripes - 1u];
-  const uint32_t ws = blockIdx.x % kStripes;
-  const uint32_t w_begin = (uint32_t)((uint64_t)count * ws / kStripes);
-  const uint32_t w_end = (uint32_t)((uint64_t)count * (ws + 1u) / kStripes);
-  uint32_t* cursor = &S.ctr[CTR_W + ws];
-  const float4* __restrict__ qo = S.qo[in];
-  const float4* __restrict__ qd = S.qd[in];
-  uint32_t pool = 0, pool_end = 0;
-  bool exhausted = false;
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  const uint32_t* prefix = W.ctr + C_RP;
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t chunks = prefix[kRuns];
+  const uint32_t ws = blockIdx.x % kWorkStripes;
+  const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
+  const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
+  uint32_t* cursor = &W.ctr[C_W + ws];
+  const float4* __restrict__ qo = W.qo[in];
+  const float4* __restrict__ qd = W.qd[in];
+  Pool pool{0u, 0u, nullptr, nullptr, 0u, false};
   ClosestRay q;
   q.node = OL_NONE;
+  const WideNode* nodes = nullptr;
+  const float4* tri = nullptr;
   uint32_t slot = OL_NONE;
   bool busy = false;
   uint32_t nv = 0, nt = 0;
@@ -645,55 +279,159 @@ __global__ void __launch_bounds__(BLOCK) k_trace_closest_split(DScene sc, GiStat
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= kRefillMin) {
-      if (pool == pool_end && !exhausted) {
-        uint32_t c = 0;
-        if (lane_id() == 0) c = atomicAdd(cursor, kChunk);
-        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
-        const uint32_t remaining = w_end - w_begin;
-        if (c >= remaining) {
-          exhausted = true;
-        } else {
-          pool = w_begin + c;
-          pool_end = w_begin + min(c + kChunk, remaining);
-        }
-      }
-      const uint32_t take = min(n_idle, pool_end - pool);
+      pool_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      const uint32_t take = min(n_idle, pool.n);
       if (!busy) {
         const uint32_t rank = lanes_below(idle);
         if (rank < take) {
-          slot = stripe_slot(incl, S.qseg, pool + rank);
+          slot = pool.slot + rank;
           const float4 o = qo[slot], d = qd[slot];
           closest_begin(q, mk3(o), mk3(d), o.w, d.w);
-          if (sc.node_count == 0) {
-            S.hit[slot] = make_float4(d.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
+          if (pool.node_count == 0) {
+            W.hit[slot] = make_float4(d.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
           } else {
+            nodes = pool.nodes;
+            tri = pool.tri;
             st.sp = 0;
             busy = true;
           }
         }
       }
-      pool += take;
+      pool.slot += take;
+      pool.n -= take;
     }
     const bool at_leaf = busy && item_is_leaf(q.node);
     const uint64_t leafy = __ballot(at_leaf);
     const uint64_t inner = __ballot(busy && !at_leaf);
     if ((leafy | inner) == 0) {
-      if (exhausted && pool == pool_end) break;
+      if (pool.exhausted && pool.n == 0) break;
       continue;
     }
     if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
-      if (at_leaf) closest_leaf<LSTACK>(sc.tri, q, st, nt);
+      if (at_leaf) closest_leaf<LSTACK>(tri, q, st, nt);
     } else if (busy && !at_leaf) {
-      closest_inner<LSTACK>(sc.nodes, q, st, nv);
+      closest_inner<LSTACK>(nodes, q, st, nv);
     }
     if (busy && q.node == OL_NONE) {
-      S.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
+      W.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
       busy = false;
     }
   }
-  if (S.work && (nv | nt)) {
-    atomicAdd(&S.work[0], (unsigned long long)nv);
-    atomicAdd(&S.work[1], (unsigned long long)nt);
+  if (W.work && (nv | nt)) {
+    atomicAdd(&W.work[0], (unsigned long long)nv);
+    atomicAdd(&W.work[1], (unsigned long long)nt);
+  }
+}
+
+// Adds one visible shadow entry's contribution.
+OL_DEV void shadow_add(uint32_t mode, float4 c, f3& direct, f3& area_direct, f3& env_direct) {
+  const uint32_t kind = (uint32_t)c.w;
+  if (mode == 0u || kind == 0u) direct = direct + rgb(c);
+  else if (kind == 1u) area_direct = area_direct + rgb(c);
+  else env_direct = env_direct + rgb(c);
+}
+
+// Per-domain averages, then L += T * direct / p_nee.
+OL_DEV void shadow_resolve(const Wave& W, uint32_t g, f3 direct, f3 area_direct, f3 env_direct) {
+  const float4 gT = W.gT[g];
+  const uint4 gI = W.gI[g];
+  const uint32_t path = gI.x, area_n = gI.w & 255u, env_n = (gI.w >> 8) & 255u, mode = gI.w >> 16;
+  if (mode != 0u) {
+    direct = direct + area_direct / (float)area_n;
+    if (env_n > 0u) direct = direct + env_direct / (float)env_n;
+  }
+  float4 L = W.L[path];
+  f3 add = (rgb(gT) * direct) / fmaxf(gT.w, 1e-6f);
+  W.L[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, L.w);
+}
+
+// Shadow groups: a lane owns one group and traces its entries in order
+// (any hit), summing the visible contributions as the inline form does.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_shadow(Wave W, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  const uint32_t* prefix = W.ctr + C_GP;
+  const uint32_t* counts = W.ctr + C_G;
+  const uint32_t chunks = prefix[kRuns];
+  const uint32_t ws = blockIdx.x % kWorkStripes;
+  const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
+  const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
+  uint32_t* cursor = &W.ctr[C_WS + ws];
+  Pool pool{0u, 0u, nullptr, nullptr, 0u, false};
+  AnyRay q;
+  q.node = OL_NONE;
+  q.occluded = false;
+  const WideNode* nodes = nullptr;
+  const float4* tri = nullptr;
+  uint32_t g = 0, e = 0, e_end = 0, mode = 0;
+  f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
+  bool busy = false;
+  uint32_t nv = 0, nt = 0;
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= kRefillMin) {
+      pool_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      const uint32_t take = min(n_idle, pool.n);
+      if (!busy) {
+        const uint32_t rank = lanes_below(idle);
+        if (rank < take) {
+          g = pool.slot + rank;
+          const uint4 gI = W.gI[g];
+          e = gI.y;
+          e_end = gI.y + gI.z;
+          mode = gI.w >> 16;
+          direct = mk3(0.0f, 0.0f, 0.0f);
+          area_direct = direct;
+          env_direct = direct;
+          if (pool.node_count == 0) {
+            // Nothing can occlude: every entry is visible.
+            for (; e < e_end; ++e) shadow_add(mode, W.ec[e], direct, area_direct, env_direct);
+            shadow_resolve(W, g, direct, area_direct, env_direct);
+          } else if (e < e_end) {
+            const float4 o = W.eo[e], d = W.ed[e];
+            any_begin(q, mk3(o), mk3(d), o.w, d.w);
+            nodes = pool.nodes;
+            tri = pool.tri;
+            st.sp = 0;
+            busy = true;
+          } else {
+            shadow_resolve(W, g, direct, area_direct, env_direct);
+          }
+        }
+      }
+      pool.slot += take;
+      pool.n -= take;
+    }
+    const bool at_leaf = busy && item_is_leaf(q.node);
+    const uint64_t leafy = __ballot(at_leaf);
+    const uint64_t inner = __ballot(busy && !at_leaf);
+    if ((leafy | inner) == 0) {
+      if (pool.exhausted && pool.n == 0) break;
+      continue;
+    }
+    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+      if (at_leaf) any_leaf<LSTACK>(tri, q, st, nt);
+    } else if (busy && !at_leaf) {
+      any_inner<LSTACK>(nodes, q, st, nv);
+    }
+    if (busy && q.node == OL_NONE) {
+      if (!q.occluded) shadow_add(mode, W.ec[e], direct, area_direct, env_direct);
+      ++e;
+      if (e < e_end) {
+        const float4 o = W.eo[e], d = W.ed[e];
+        any_begin(q, mk3(o), mk3(d), o.w, d.w);
+        st.sp = 0;
+      } else {
+        shadow_resolve(W, g, direct, area_direct, env_direct);
+        busy = false;
+      }
+    }
+  }
+  if (W.work && (nv | nt)) {
+    atomicAdd(&W.work[2], (unsigned long long)nv);
+    atomicAdd(&W.work[3], (unsigned long long)nt);
   }
 }
 
@@ -715,185 +453,28 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o
   }
 }
 
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_shadow(DScene sc, GiState S, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  __shared__ uint32_t incl[kStripes];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
-  stripe_prefix(S.ctr + CTR_G, incl);
-  const uint32_t count = incl[kStripes - 1u];
-  uint32_t nv = 0, nt = 0;
-  for (uint32_t j = blockIdx.x * BLOCK + threadIdx.x; j < count; j += gridDim.x * BLOCK) {
-    const uint32_t g = stripe_slot(incl, S.gseg, j);
-    const float4 gT = S.gT[g];
-    const uint4 gI = S.gI[g];
-    const uint32_t path = gI.x, first = gI.y, n = gI.z;
-    const uint32_t area_n = gI.w & 255u, env_n = (gI.w >> 8) & 255u, mode = gI.w >> 16;
-    f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
-    for (uint32_t k = 0; k < n; ++k) {
-      const float4 o = S.eo[first + k], d = S.ed[first + k], c = S.ec[first + k];
-      if (ol_trace_any<LSTACK>(sc.nodes, sc.tri, sc.node_count, mk3(o), mk3(d), o.w, d.w, st, nv, nt)) continue;
-      const uint32_t kind = (uint32_t)c.w;
-      if (mode == 0u || kind == 0u) direct = direct + rgb(c);
-      else if (kind == 1u) area_direct = area_direct + rgb(c);
-      else env_direct = env_direct + rgb(c);
-    }
-    if (mode != 0u) {
-      direct = direct + area_direct / (float)area_n;
-      if (env_n > 0u) direct = direct + env_direct / (float)env_n;
-    }
-    float4 L = S.L[path];
-    f3 add = (rgb(gT) * direct) / fmaxf(gT.w, 1e-6f);
-    S.L[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, L.w);
-  }
-  if (S.work && (nv | nt)) {
-    atomicAdd(&S.work[2], (unsigned long long)nv);
-    atomicAdd(&S.work[3], (unsigned long long)nt);
-  }
-}
-
-// Adds one visible shadow entry's contribution (the summation order of
-// k_trace_shadow).
-OL_DEV void shadow_add(uint32_t mode, float4 c, f3& direct, f3& area_direct, f3& env_direct) {
-  const uint32_t kind = (uint32_t)c.w;
-  if (mode == 0u || kind == 0u) direct = direct + rgb(c);
-  else if (kind == 1u) area_direct = area_direct + rgb(c);
-  else env_direct = env_direct + rgb(c);
-}
-
-OL_DEV void shadow_resolve(const GiState& S, uint32_t g, f3 direct, f3 area_direct, f3 env_direct) {
-  const float4 gT = S.gT[g];
-  const uint4 gI = S.gI[g];
-  const uint32_t path = gI.x, area_n = gI.w & 255u, env_n = (gI.w >> 8) & 255u, mode = gI.w >> 16;
-  if (mode != 0u) {
-    direct = direct + area_direct / (float)area_n;
-    if (env_n > 0u) direct = direct + env_direct / (float)env_n;
-  }
-  float4 L = S.L[path];
-  f3 add = (rgb(gT) * direct) / fmaxf(gT.w, 1e-6f);
-  S.L[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, L.w);
-}
-
-// Shadow groups with lane refill and split inner / leaf steps (the
-// scheduling of k_trace_closest_split): a lane owns one group and traces its
-// entries in order, accumulating visible contributions as k_trace_shadow does.
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_shadow_split(DScene sc, GiState S, uint32_t* spill,
-                                                              uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  __shared__ uint32_t incl[kStripes];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &S.ctr[CTR_STACK]);
-  stripe_prefix(S.ctr + CTR_G, incl);
-  const uint32_t count = incl[kStripes - 1u];
-  const uint32_t ws = blockIdx.x % kStripes;
-  const uint32_t w_begin = (uint32_t)((uint64_t)count * ws / kStripes);
-  const uint32_t w_end = (uint32_t)((uint64_t)count * (ws + 1u) / kStripes);
-  uint32_t* cursor = &S.ctr[CTR_WS + ws];
-  uint32_t pool = 0, pool_end = 0;
-  bool exhausted = false;
-  AnyRay q;
-  q.node = OL_NONE;
-  q.occluded = false;
-  uint32_t g = 0, e = 0, e_end = 0, mode = 0;
-  f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
-  bool busy = false;
-  uint32_t nv = 0, nt = 0;
-  while (true) {
-    const uint64_t idle = __ballot(!busy);
-    const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (n_idle >= kRefillMin) {
-      if (pool == pool_end && !exhausted) {
-        uint32_t c = 0;
-        if (lane_id() == 0) c = atomicAdd(cursor, kChunk);
-        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
-        const uint32_t remaining = w_end - w_begin;
-        if (c >= remaining) {
-          exhausted = true;
-        } else {
-          pool = w_begin + c;
-          pool_end = w_begin + min(c + kChunk, remaining);
-        }
-      }
-      const uint32_t take = min(n_idle, pool_end - pool);
-      if (!busy) {
-        const uint32_t rank = lanes_below(idle);
-        if (rank < take) {
-          g = stripe_slot(incl, S.gseg, pool + rank);
-          const uint4 gI = S.gI[g];
-          e = gI.y;
-          e_end = gI.y + gI.z;
-          mode = gI.w >> 16;
-          direct = mk3(0.0f, 0.0f, 0.0f);
-          area_direct = direct;
-          env_direct = direct;
-          if (sc.node_count == 0) {
-            // Nothing can occlude: every entry is visible.
-            for (; e < e_end; ++e) shadow_add(mode, S.ec[e], direct, area_direct, env_direct);
-            shadow_resolve(S, g, direct, area_direct, env_direct);
-          } else if (e < e_end) {
-            const float4 o = S.eo[e], d = S.ed[e];
-            any_begin(q, mk3(o), mk3(d), o.w, d.w);
-            st.sp = 0;
-            busy = true;
-          } else {
-            shadow_resolve(S, g, direct, area_direct, env_direct);
-          }
-        }
-      }
-      pool += take;
-    }
-    const bool at_leaf = busy && item_is_leaf(q.node);
-    const uint64_t leafy = __ballot(at_leaf);
-    const uint64_t inner = __ballot(busy && !at_leaf);
-    if ((leafy | inner) == 0) {
-      if (exhausted && pool == pool_end) break;
-      continue;
-    }
-    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
-      if (at_leaf) any_leaf<LSTACK>(sc.tri, q, st, nt);
-    } else if (busy && !at_leaf) {
-      any_inner<LSTACK>(sc.nodes, q, st, nv);
-    }
-    if (busy && q.node == OL_NONE) {
-      if (!q.occluded) shadow_add(mode, S.ec[e], direct, area_direct, env_direct);
-      ++e;
-      if (e < e_end) {
-        const float4 o = S.eo[e], d = S.ed[e];
-        any_begin(q, mk3(o), mk3(d), o.w, d.w);
-        st.sp = 0;
-      } else {
-        shadow_resolve(S, g, direct, area_direct, env_direct);
-        busy = false;
-      }
-    }
-  }
-  if (S.work && (nv | nt)) {
-    atomicAdd(&S.work[2], (unsigned long long)nv);
-    atomicAdd(&S.work[3], (unsigned long long)nt);
-  }
-}
-
-struct ShadeOut {
-  bool cont;
-  float4 co, cd;
-};
-
-// Launched with blocks of kStripeUnit threads, so the output stripe
-// (j / kStripeUnit) % kStripes of dense input j is uniform per block.
-__global__ void __launch_bounds__(kStripeUnit) k_shade(GiParams P, GiState S, uint32_t in) {
-  const DScene& sc = P.sc;
-  __shared__ uint32_t incl[kStripes];
-  stripe_prefix(S.ctr + ctr_queue(in), incl);
-  const uint32_t count = incl[kStripes - 1u];
+// ---- shade -----------------------------------------------------------------------------------
+// One wave per 64-ray chunk (grid-stride over chunks), so the job and its
+// parameters are wave-uniform.  Outputs go to the job's segment
+// (chunk index within the job) % stripes: a job's chunks number at most
+// n_paths / 64 + stripes, so a segment of (ceil(n_paths / (64 * stripes)) + 1)
+// * 64 slots cannot overflow.
+__global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
   const uint32_t out = in ^ 1u;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  // Every lane runs the same number of loop trips so the wave-wide slot
-  // reservations below are reached together.
-  const uint32_t trips = (count + stride - 1u) / stride;
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t trip = 0; trip < trips; ++trip, j += stride) {
-    const bool live = j < count;
-    const uint32_t so = (j / kStripeUnit) % kStripes;
+  const uint32_t* prefix = W.ctr + C_RP;
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t chunks = prefix[kRuns];
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
+    const ChunkLoc k = locate_chunk(prefix, counts, W.jobs, uniform(c));
+    const uint32_t job = uniform(k.job);
+    const JobDev& J = W.jobs[job];
+    const GiParams& P = J.P;
+    const DScene& sc = P.sc;
+    const uint32_t so = uniform(k.chunk_in_job) % J.stripes;
+    const uint32_t run = job * kJobStripes + so;
+    const bool live = lane_id() < k.count;
+    const uint32_t i = k.first + lane_id();
     bool cont = false, nee = false;
     float4 co = make_float4(0, 0, 0, 0), cd = co;
     uint32_t path = 0, nee_count = 0, area_n = 0, env_n = 0, nee_rng = 0;
@@ -903,11 +484,10 @@ __global__ void __launch_bounds__(kStripeUnit) k_shade(GiParams P, GiState S, ui
     NeeSample single;
     single.valid = false;
     if (live) {
-      const uint32_t i = stripe_slot(incl, S.qseg, j);
-      path = S.qp[in][i];
-      const float4 ro = S.qo[in][i], rd = S.qd[in][i], hv = S.hit[i];
-      float4 T = S.T[path], L = S.L[path];
-      uint4 M = S.M[path];
+      path = W.qp[in][i];
+      const float4 ro = W.qo[in][i], rd = W.qd[in][i], hv = W.hit[i];
+      float4 T = W.T[path], L = W.L[path];
+      uint4 M = W.M[path];
       incoming = rgb(rd);
       const uint32_t slot = __float_as_uint(hv.w);
       do {
@@ -981,7 +561,7 @@ __global__ void __launch_bounds__(kStripeUnit) k_shade(GiParams P, GiState S, ui
         float bsdf_pdf = 0.0f;
         if (transmission > 0.001f) {
           // shadeTransmitPath
-          float4* st = S.stk + 4u * path;
+          float4* st = W.stk + 4u * path;
           float cur = stack_get(st, depth - 1u);
           float target = 1.0f;
           if (entering) target = fmaxf(mat.id.x, 1.0001f);
@@ -1080,82 +660,86 @@ __global__ void __launch_bounds__(kStripeUnit) k_shade(GiParams P, GiState S, ui
         co = mk4(hitp + dir * 0.002f, 0.001f);
         cd = mk4(dir, 1e30f);
       } while (false);
-      S.T[path] = T;
-      S.L[path] = L;
-      S.M[path] = M;
+      W.T[path] = T;
+      W.L[path] = L;
+      W.M[path] = M;
     }
-    // Continuation slot in this block's stripe (one atomic per wave).
-    const uint32_t cl = wave_reserve(&S.ctr[ctr_queue(out) + so], cont);
+    // Continuation slot in the job's output segment (one atomic per wave).
+    const uint32_t cl = wave_reserve(&W.ctr[c_queue(out) + run], cont);
     if (cont) {
-      if (cl < S.qseg) {
-        const uint32_t cslot = so * S.qseg + cl;
-        S.qo[out][cslot] = co;
-        S.qd[out][cslot] = cd;
-        S.qp[out][cslot] = path;
+      if (cl < J.seg) {
+        const uint32_t cslot = J.base + so * J.seg + cl;
+        W.qo[out][cslot] = co;
+        W.qd[out][cslot] = cd;
+        W.qp[out][cslot] = path;
       } else {
-        atomicAdd(&S.ctr[CTR_OVERFLOW], 1u);
-        S.M[path].w &= ~F_ACTIVE;
+        atomicAdd(&W.ctr[C_OVERFLOW], 1u);
+        W.M[path].w &= ~F_ACTIVE;
       }
     }
     // Shadow group + entries.
     const bool has_group = nee && nee_count > 0u;
-    const uint32_t gl = wave_reserve(&S.ctr[CTR_G + so], has_group);
-    const uint32_t el = wave_reserve_n(&S.ctr[CTR_E + so], has_group ? nee_count : 0u);
+    const uint32_t gl = wave_reserve(&W.ctr[C_G + run], has_group);
+    const uint32_t el = wave_reserve_n(&W.ctr[C_E + run], has_group ? nee_count : 0u);
     if (has_group) {
-      if (gl < S.gseg && el + nee_count <= S.eseg) {
-        const uint32_t gslot = so * S.gseg + gl, eslot = so * S.eseg + el;
-        S.gT[gslot] = mk4(nee_T, nee_p);
-        S.gI[gslot] = make_uint4(path, eslot, nee_count, area_n | (env_n << 8) | ((P.unified ? 0u : 1u) << 16));
+      if (gl < J.seg && el + nee_count <= J.eseg) {
+        const uint32_t gslot = J.base + so * J.seg + gl, eslot = J.ebase + so * J.eseg + el;
+        W.gT[gslot] = mk4(nee_T, nee_p);
+        W.gI[gslot] = make_uint4(path, eslot, nee_count, area_n | (env_n << 8) | ((P.unified ? 0u : 1u) << 16));
         if (!P.two_pass) {
-          S.eo[eslot] = mk4(single.origin, 0.001f);
-          S.ed[eslot] = mk4(single.dir, single.tmax);
-          S.ec[eslot] = mk4(single.c, (float)single.kind);
+          W.eo[eslot] = mk4(single.origin, 0.001f);
+          W.ed[eslot] = mk4(single.dir, single.tmax);
+          W.ec[eslot] = mk4(single.c, (float)single.kind);
         } else {
-          uint32_t k = 0, an, en;
+          uint32_t kk = 0, an, en;
           uint32_t rs2 = nee_rng;
           for_each_nee(P, rs2, hitp, nrm, mat, incoming, an, en, [&](const NeeSample& s) {
-            S.eo[eslot + k] = mk4(s.origin, 0.001f);
-            S.ed[eslot + k] = mk4(s.dir, s.tmax);
-            S.ec[eslot + k] = mk4(s.c, (float)s.kind);
-            ++k;
+            W.eo[eslot + kk] = mk4(s.origin, 0.001f);
+            W.ed[eslot + kk] = mk4(s.dir, s.tmax);
+            W.ec[eslot + kk] = mk4(s.c, (float)s.kind);
+            ++kk;
           });
         }
       } else {
-        atomicAdd(&S.ctr[CTR_OVERFLOW], 1u);
+        atomicAdd(&W.ctr[C_OVERFLOW], 1u);
       }
     }
   }
 }
 
-// Adds the float32 radiance of this chunk's samples to the float64 pixel
-// sums in sample order (render_wavefront_outputs, api.py:1404-1407).
-__global__ void __launch_bounds__(256) k_accumulate(GiParams P, GiState S, double4* acc, uint32_t first_chunk) {
-  const uint32_t np = P.width * P.height;
-  const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pixel >= np) return;
-  double4 a = first_chunk ? make_double4(0.0, 0.0, 0.0, 0.0) : acc[pixel];
-  for (uint32_t s = 0; s < P.spp; ++s) {
-    const float4 l = S.L[(size_t)s * np + pixel];
-    if (first_chunk && s == 0) {
-      a = make_double4((double)l.x, (double)l.y, (double)l.z, 1.0);
-    } else {
-      a.x += (double)l.x;
-      a.y += (double)l.y;
-      a.z += (double)l.z;
-      a.w += 1.0;
-    }
+// ---- step epilogue -----------------------------------------------------------------------------
+// Per-job report (rays traced from the input queue, shadow rays, live paths
+// in the output queue, shaded hits = groups) and reset of every consumed
+// counter, so the input queue can serve as the next step's output.
+__global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
+  __shared__ uint32_t rep[4 * kMaxJobs];
+  const uint32_t t = threadIdx.x;
+  if (t < 4 * kMaxJobs) rep[t] = 0u;
+  __syncthreads();
+  for (uint32_t r = t; r < kRuns; r += blockDim.x) {
+    const uint32_t job = r / kJobStripes;
+    const uint32_t a = W.ctr[c_queue(in) + r], e = W.ctr[C_E + r], l = W.ctr[c_queue(in ^ 1u) + r],
+                   g = W.ctr[C_G + r];
+    if (a) atomicAdd(&rep[4 * job + 0], a);
+    if (e) atomicAdd(&rep[4 * job + 1], e);
+    if (l) atomicAdd(&rep[4 * job + 2], l);
+    if (g) atomicAdd(&rep[4 * job + 3], g);
+    W.ctr[c_queue(in) + r] = 0u;
+    W.ctr[C_G + r] = 0u;
+    W.ctr[C_E + r] = 0u;
   }
-  acc[pixel] = a;
+  if (t < 2 * kWorkStripes) W.ctr[C_W + t] = 0u;
+  __syncthreads();
+  if (t < 4 * kMaxJobs) W.ctr[C_REPORT + t] = rep[t];
 }
 
-__global__ void __launch_bounds__(256) k_final(const double4* acc, uint32_t np, uint32_t samples, float4* out) {
-  const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pixel >= np) return;
-  const double n = (double)samples;
-  const double4 a = acc[pixel];
-  out[pixel] = make_float4((float)(a.x / n), (float)(a.y / n), (float)(a.z / n), (float)(a.w / n));
+// Zeroes the whole counter block (engine start).
+__global__ void __launch_bounds__(1024) k_clear(Wave W) {
+  for (uint32_t i = threadIdx.x; i < C_WORDS; i += blockDim.x) W.ctr[i] = 0u;
+  if (W.work && threadIdx.x < 4) W.work[threadIdx.x] = 0ull;
 }
 
+// ---- launch helpers ------------------------------------------------------------------------------
 template <class K>
 int resident_blocks(K kernel, int block) {
   int occ = 0;
@@ -1174,53 +758,26 @@ void check_spill(const GiLaunch& l, uint64_t threads) {
 
 template <int LSTACK, int BLOCK>
 uint64_t spill_words_for(const GiLaunch& l) {
-  int occ = std::max(std::max(std::max(resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK),
-                                       resident_blocks(k_trace_closest_split<LSTACK, BLOCK>, BLOCK)),
-                              resident_blocks(k_trace_closest_pool<LSTACK, BLOCK>, BLOCK)),
-                     std::max(std::max(resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK),
-                                       resident_blocks(k_trace_shadow_split<LSTACK, BLOCK>, BLOCK)),
-                              resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK)));
+  int occ = std::max(std::max(resident_blocks(k_closest<LSTACK, BLOCK>, BLOCK),
+                              resident_blocks(k_shadow<LSTACK, BLOCK>, BLOCK)),
+                     resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK));
   return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
 
 template <int LSTACK, int BLOCK>
-void launch_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint32_t in, hipStream_t st) {
-  if (l.scheduler == 2) {
-    static int occ = resident_blocks(k_trace_closest_split<LSTACK, BLOCK>, BLOCK);
-    dim3 grid((unsigned)(l.cu_count * occ));
-    check_spill(l, (uint64_t)grid.x * BLOCK);
-    hipLaunchKernelGGL((k_trace_closest_split<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, in, l.spill,
-                       spill_cap_of(l));
-    return;
-  }
-  if (l.scheduler == 1) {
-    static int occ = resident_blocks(k_trace_closest_pool<LSTACK, BLOCK>, BLOCK);
-    dim3 grid((unsigned)(l.cu_count * occ));
-    check_spill(l, (uint64_t)grid.x * BLOCK);
-    hipLaunchKernelGGL((k_trace_closest_pool<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, in, l.spill,
-                       spill_cap_of(l));
-    return;
-  }
-  static int occ = resident_blocks(k_trace_closest<LSTACK, BLOCK>, BLOCK);
+void launch_closest(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  static int occ = resident_blocks(k_closest<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
   check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_trace_closest<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, in, l.spill, spill_cap_of(l));
+  hipLaunchKernelGGL((k_closest<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
-void launch_shadow(const GiLaunch& l, const DScene& sc, const GiState& s, hipStream_t st) {
-  if (l.scheduler == 2) {
-    static int occ2 = resident_blocks(k_trace_shadow_split<LSTACK, BLOCK>, BLOCK);
-    dim3 grid((unsigned)(l.cu_count * occ2));
-    check_spill(l, (uint64_t)grid.x * BLOCK);
-    hipLaunchKernelGGL((k_trace_shadow_split<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, l.spill,
-                       spill_cap_of(l));
-    return;
-  }
-  static int occ = resident_blocks(k_trace_shadow<LSTACK, BLOCK>, BLOCK);
+void launch_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
+  static int occ = resident_blocks(k_shadow<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
   check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_trace_shadow<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, s, l.spill, spill_cap_of(l));
+  hipLaunchKernelGGL((k_shadow<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
@@ -1261,46 +818,36 @@ int stack_depth_for(uint32_t bvh_depth) {
 
 uint64_t trace_spill_words(const GiLaunch& l) { OL_STACK_DISPATCH(spill_words_for, l); }
 
-void gi_reset(const GiState& s, bool all, hipStream_t st) {
-  static_assert(CTR_WORDS <= 256, "counter block larger than the reset kernel");
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, s, all ? 1u : 0u);
+void wave_clear(const Wave& w, hipStream_t st) { hipLaunchKernelGGL(k_clear, dim3(1), dim3(1024), 0, st, w); }
+
+void wave_accumulate(const Wave& w, int cu_count, hipStream_t st) {
+  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)(cu_count * 4)), dim3(256), 0, st, w);
 }
 
-void gi_finish(const GiState& s, hipStream_t st) { hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, s); }
-
-void gi_generate(const GiParams& p, const GiState& s, hipStream_t st) {
-  const uint32_t tiles = ((p.width + 7u) / 8u) * ((p.height + 7u) / 8u);
-  const uint64_t threads = (uint64_t)tiles * 64u * p.spp;
-  hipLaunchKernelGGL(k_generate, dim3((unsigned)((threads + 255u) / 256u)), dim3(256), 0, st, p, s);
+void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st) {
+  hipLaunchKernelGGL(k_generate, dim3((unsigned)(cu_count * 8)), dim3(256), 0, st, w, in);
 }
 
-void gi_bounce_prologue(const GiState& s, uint32_t bounce, hipStream_t st) {
-  hipLaunchKernelGGL(k_prologue, dim3(1), dim3(64), 0, st, s, bounce);
+void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP);
+  OL_STACK_DISPATCH(launch_closest, l, w, in, st);
 }
 
-void gi_trace_closest(const GiLaunch& l, const DScene& sc, const GiState& s, uint32_t in, hipStream_t st) {
-  OL_STACK_DISPATCH(launch_closest, l, sc, s, in, st);
+void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  static int occ = resident_blocks(k_shade, 256);
+  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in);
 }
 
-void gi_trace_shadow(const GiLaunch& l, const DScene& sc, const GiState& s, hipStream_t st) {
-  OL_STACK_DISPATCH(launch_shadow, l, sc, s, st);
+void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP);
+  OL_STACK_DISPATCH(launch_shadow, l, w, st);
+}
+
+void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(1024), 0, st, w, in);
 }
 
 void trace_rays(const GiLaunch& l, const DScene& sc, const float4* o, const float4* d, uint32_t n, float4* hits,
                 uint32_t* overflow, unsigned long long* work, hipStream_t st) {
   OL_STACK_DISPATCH(launch_rays, l, sc, o, d, n, hits, overflow, work, st);
-}
-
-void gi_shade(const GiLaunch& l, const GiParams& p, const GiState& s, uint32_t in, hipStream_t st) {
-  static int occ = resident_blocks(k_shade, (int)kStripeUnit);
-  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(kStripeUnit), 0, st, p, s, in);
-}
-
-void gi_accumulate(const GiParams& p, const GiState& s, double4* acc, bool first, hipStream_t st) {
-  const uint32_t np = p.width * p.height;
-  hipLaunchKernelGGL(k_accumulate, dim3((np + 255u) / 256u), dim3(256), 0, st, p, s, acc, first ? 1u : 0u);
-}
-
-void gi_final(const double4* acc, uint32_t np, uint32_t samples, float4* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_final, dim3((np + 255u) / 256u), dim3(256), 0, st, acc, np, samples, out);
 }

@@ -307,13 +307,14 @@ OL_DEV void any_inner(const WideNode* __restrict__ nodes, AnyRay& q, TravStack<L
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
-  sort4(key);
+  // Any hit ends at the first occluder found, so no distance order: children
+  // are taken in build order (the first one continues).
   uint32_t next = OL_NONE;
 #pragma unroll
   for (int k = 3; k >= 0; --k) {
     if (key[k] == OL_NONE) continue;
     if (next != OL_NONE) st.push(next);
-    next = ch[key[k] & 3u];
+    next = ch[k];
   }
   if (next != OL_NONE) q.node = next;
   else any_next(q, st);

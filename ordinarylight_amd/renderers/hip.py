@@ -52,12 +52,9 @@ class HipRendererConfig:
     max_samples_per_call: int = 64
     device_index: int = 0
     bvh_leaf_size: int = 2
-    # Traversal stack entries kept in LDS per lane, the traversal block size,
-    # and the closest-hit scheduler (0 grid-stride, 1 lane refill, 2 lane
-    # refill with separate inner-node and leaf steps).
+    # Traversal stack entries kept in LDS per lane, and the traversal block size.
     traversal_lds_stack: int = 16
     traversal_block: int = 128
-    traversal_scheduler: int = 2
     profiling: bool = False
     work_counters: bool = False
 
@@ -78,8 +75,6 @@ class HipRendererConfig:
             raise ValueError("wavefront_tile_capacity must be positive")
         if not 0 <= int(self.bvh_leaf_size) <= 16:
             raise ValueError("bvh_leaf_size must be between 0 and 16")
-        if int(self.traversal_scheduler) not in (0, 1, 2):
-            raise ValueError("traversal_scheduler must be 0, 1 or 2")
 
     def gi_settings(self):
         s = N.GiSettings()
@@ -102,8 +97,7 @@ HOT_SETTINGS = frozenset({
     "wavefront_secondary_nee_probability", "wavefront_unified_secondary_nee",
     "wavefront_russian_roulette", "wavefront_russian_roulette_start",
     "wavefront_russian_roulette_min_survival", "wavefront_tile_capacity",
-    "max_samples_per_call", "traversal_lds_stack", "traversal_block", "traversal_scheduler", "profiling",
-    "work_counters",
+    "max_samples_per_call", "traversal_lds_stack", "traversal_block", "profiling", "work_counters",
 })
 
 
@@ -307,7 +301,7 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         lib = N.library()
         N.check(lib.olhip_set_profiling(context.handle, int(bool(self.config.profiling))))
         N.check(lib.olhip_set_traversal(context.handle, int(self.config.traversal_lds_stack),
-                                        int(self.config.traversal_block), int(self.config.traversal_scheduler)))
+                                        int(self.config.traversal_block)))
         N.check(lib.olhip_set_work_counters(context.handle, int(bool(self.config.work_counters))))
         position, forward, right, up = camera_basis(camera)
         cam = N.Camera()
