@@ -1,14 +1,21 @@
-"""Benchmark: single-frame wavefront GI render on one MI355X.
+"""Benchmark: wavefront GI on one MI355X -- plain frames, batched jobs, served jobs.
 
-Workload (one step = one frame): seeded fBm heightfield terrain of 1,000,000
-triangles (1001 x 501 vertices), chunked GGX materials (roughness ~ U[0.05,1],
-metallic ~ Bernoulli(0.3)), 64 emissive triangles plus a constant
-environment of radiance 0.5; 1920 x 1080, 8 samples per pixel, 8 bounces,
-float32.  Metric: Mrays/s = (primary + extension + shadow rays traced) /
-device time of the frame (scene build excluded, image kept in HBM).
+1. Plain (``value``; one step = one frame): seeded fBm heightfield terrain of
+   1,000,000 triangles (1001 x 501 vertices), chunked GGX materials
+   (roughness ~ U[0.05,1], metallic ~ Bernoulli(0.3)), 64 emissive triangles
+   plus a constant environment of radiance 0.5; 1920 x 1080, 8 samples per
+   pixel, 8 bounces, float32.  Mrays/s = (primary + extension + shadow rays
+   traced) / wall time of the timed frames (scene build excluded, image kept
+   in HBM).
+2. Batched (``batched``): 64 jobs, square images with log-uniform edge in
+   [128, 1024], eight resident scenes of 10k-2M triangles, spp in
+   {1,2,4,8,16}, bounces in {4,8}; one engine run (autotuned capacity and
+   launch shape) against the same jobs rendered back-to-back by the plain path.
+3. Served (``served``): Poisson arrivals of such jobs at a fixed fraction of
+   the batched throughput; continuous-batching server against a plain
+   one-at-a-time server on the same arrivals; p50/p99 latency and jobs/s.
 
-Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
-this build measures one GPU.
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
 
 from __future__ import annotations
@@ -93,6 +100,117 @@ def cpu_baseline(scene, camera, width, height, spp, bounces, target_seconds):
     }
 
 
+def build_mixed_scenes(context, keys=None):
+    """The eight resident scenes of the batched/served workloads."""
+    from ordinarylight_amd import workloads
+    from ordinarylight_amd.renderers.hip import DeviceScene
+    scenes = {}
+    for key, _, _ in workloads.MIXED_SCENES:
+        if keys is not None and key not in keys:
+            continue
+        scene, camera = workloads.mixed_scene(key)
+        scenes[key] = (DeviceScene(context, scene, 2), camera)
+    return scenes
+
+
+def make_jobs(specs, scenes):
+    from ordinarylight_amd.batch import Job
+    return [Job(scenes[j["scene"]][0], scenes[j["scene"]][1], j["width"], j["height"], j["samples"],
+                0, None, j["max_bounces"]) for j in specs]
+
+
+def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
+    """64 mixed jobs: plain back-to-back vs one batched engine run."""
+    from ordinarylight_amd import batch, workloads
+    jobs = make_jobs(workloads.mixed_jobs(seed, n_jobs), scenes)
+    rays = 0
+    # Warm-up of both paths (frame-engine growth, graph capture, allocations).
+    for j in sorted(jobs, key=lambda j: -j.width * j.height * j.samples)[:2]:
+        batch.plain_render(j, copy_image=False)
+    t0 = time.perf_counter()
+    for j in jobs:
+        _, st = batch.plain_render(j, copy_image=False)
+        rays += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+    plain_s = time.perf_counter() - t0
+    if autotune_on:
+        settings, timings = batch.autotune(context, jobs)
+        tuned = {f"{k.path_slots}/{k.lds_stack}/{k.trace_block}": round(v * 1e3, 2) for k, v in timings.items()}
+    else:
+        settings, tuned = batch.EngineSettings(), {}
+    with batch.BatchEngine(context, settings) as engine:
+        batch.run_batch(engine, jobs)  # warm-up
+        batched_s, results = batch.run_batch(engine, jobs)
+        est = engine.stats()
+    batched_rays = sum(st["primary_rays"] + st["extension_rays"] + st["shadow_rays"] for _, st in results)
+    return settings, {
+        "jobs": n_jobs,
+        "jobs_per_s": round(n_jobs / batched_s, 2),
+        "plain_jobs_per_s": round(n_jobs / plain_s, 2),
+        "speedup_vs_plain": round(plain_s / batched_s, 3),
+        "batched_ms": round(batched_s * 1e3, 2),
+        "plain_ms": round(plain_s * 1e3, 2),
+        "mrays_per_s": round(batched_rays / batched_s / 1e6, 1),
+        "plain_mrays_per_s": round(rays / plain_s / 1e6, 1),
+        "max_resident_jobs": int(est["max_resident_jobs"]),
+        "settings": {"path_slots": settings.path_slots, "lds_stack": settings.lds_stack,
+                     "trace_block": settings.trace_block},
+        "autotune_ms": tuned,
+    }
+
+
+def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8):
+    """Poisson arrivals of mixed jobs: continuous-batching server vs a plain
+    one-job-at-a-time server on the same arrival sequence."""
+    from ordinarylight_amd import batch, workloads
+    specs = workloads.mixed_jobs(seed, n_jobs)
+    jobs = make_jobs(specs, scenes)
+    arrivals = batch.poisson_arrivals(rate, n_jobs, seed)
+    engine = batch.BatchEngine(context, settings)
+    engine.start()
+    ids = []
+    t0 = time.perf_counter()
+    for j, at in zip(jobs, arrivals):
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append((engine.submit(j), time.perf_counter()))
+    lat, results = [], []
+    first_submit, last_done = None, None
+    for k, (jid, _) in enumerate(ids):
+        state = engine.wait(jid, 600.0)
+        if state != 2:
+            raise RuntimeError(f"served job {jid} ended in state {state}")
+        img, st = engine.result(jid, copy_image=k < check)
+        lat.append(st["done_ms"] - st["submit_ms"])
+        first_submit = st["submit_ms"] if first_submit is None else min(first_submit, st["submit_ms"])
+        last_done = st["done_ms"] if last_done is None else max(last_done, st["done_ms"])
+        if k < check:
+            results.append(img)
+    engine.close()
+    served = batch.latency_percentiles(lat)
+    equal = all(np.array_equal(img, batch.plain_render(j)[0]) for img, j in zip(results, jobs[:check]))
+    # Plain server: jobs run one at a time in arrival order.
+    plain_lat = []
+    t0 = time.perf_counter()
+    clock = 0.0
+    for j, at in zip(jobs, arrivals):
+        now = time.perf_counter() - t0
+        if now < at:
+            time.sleep(at - now)
+        batch.plain_render(j, copy_image=False)
+        clock = time.perf_counter() - t0
+        plain_lat.append((clock - at) * 1e3)
+    plain = batch.latency_percentiles(plain_lat)
+    return {
+        "jobs": n_jobs, "rate_per_s": rate,
+        "jobs_per_s": round(n_jobs / ((last_done - first_submit) * 1e-3), 2),
+        "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
+        "plain_jobs_per_s": round(n_jobs / clock, 2),
+        "plain_p50_ms": round(plain["p50_ms"], 2), "plain_p99_ms": round(plain["p99_ms"], 2),
+        "results_equal_plain": bool(equal), "checked_jobs": check,
+    }
+
+
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -108,6 +226,11 @@ def main(argv=None):
     parser.add_argument("--leaf-size", type=int, default=2)
     parser.add_argument("--lds-stack", type=int, default=16)
     parser.add_argument("--block", type=int, default=128)
+    parser.add_argument("--batch-jobs", type=int, default=64, help="batched workload size (0: skip)")
+    parser.add_argument("--serve-jobs", type=int, default=96, help="served workload size (0: skip)")
+    parser.add_argument("--serve-load", type=float, default=0.7,
+                        help="Poisson arrival rate as a fraction of the batched jobs/s")
+    parser.add_argument("--no-autotune", action="store_true")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -183,7 +306,7 @@ def main(argv=None):
                     "bytes_per_launch": int(per_launch_bytes),
                     "traffic_source": TRAFFIC_FILE if traffic is not None else None}
     line = {
-        "metric": "Mrays/s at 1080p 8-bounce (wavefront GI, one MI355X)",
+        "metric": "Mrays/s at 1080p 8-bounce; batched jobs/s + p50/p99 latency vs plain GPU",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -216,6 +339,27 @@ def main(argv=None):
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(scene, camera, W, H, spp, B, args.cpu_seconds)
     renderer.close()
+
+    if args.batch_jobs > 0 or args.serve_jobs > 0:
+        from ordinarylight_amd.renderers.hip import DeviceContext
+        context = DeviceContext(0)
+        t0 = time.perf_counter()
+        scenes = build_mixed_scenes(context)
+        log(f"mixed scenes: {len(scenes)} resident in {time.perf_counter() - t0:.1f} s")
+        settings = None
+        if args.batch_jobs > 0:
+            settings, line["batched"] = batched_measurement(context, scenes, args.batch_jobs, 0,
+                                                            not args.no_autotune)
+            log(f"batched: {line['batched']}")
+        if args.serve_jobs > 0:
+            from ordinarylight_amd.batch import EngineSettings
+            rate = args.serve_load * (line["batched"]["jobs_per_s"] if "batched" in line else 20.0)
+            line["served"] = served_measurement(context, scenes, settings or EngineSettings(), args.serve_jobs,
+                                                round(rate, 2), 1)
+            log(f"served: {line['served']}")
+        for ds, _ in scenes.values():
+            ds.close()
+        context.close()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -158,11 +158,20 @@ typedef struct olhip_engine_config {
   int32_t use_graph;      /* replay each step as a captured hipGraph (1) or launch directly (0) */
 } olhip_engine_config;
 
+/* Named products beside the color image (render_wavefront_outputs,
+ * targets/vulkan/api.py:1252-1451): sample 0's primary hit (depth = ray
+ * distance or +inf, unflipped geometric normal, original triangle index or
+ * 0xFFFFFFFF) and the unbiased per-pixel luminance variance over samples. */
+#define OLHIP_PRODUCT_PRIMARY 1u
+#define OLHIP_PRODUCT_VARIANCE 2u
+
 typedef struct olhip_job_desc {
   const olhip_scene* scene;
   olhip_camera camera;
   uint32_t width, height, samples, frame_index;
   olhip_gi_settings settings;  /* path_capacity: most paths per sample chunk, 0 = no limit */
+  uint32_t products;           /* OLHIP_PRODUCT_* bits */
+  uint32_t pad;
 } olhip_job_desc;
 
 enum {
@@ -201,10 +210,23 @@ int olhip_engine_wait(olhip_engine* engine, uint64_t job_id, double timeout_s, i
 /* Copy a done job's float32 [H][W][4] image (out_rgba may be NULL) and
  * statistics, then release it. */
 int olhip_engine_result(olhip_engine* engine, uint64_t job_id, float* out_rgba, olhip_job_stats* stats);
+/* Copy a done job's products before olhip_engine_result: depth [H][W],
+ * normal [H][W][3], primitive [H][W] (needs OLHIP_PRODUCT_PRIMARY) and
+ * variance [H][W] (needs OLHIP_PRODUCT_VARIANCE); any pointer may be NULL. */
+int olhip_engine_products(olhip_engine* engine, uint64_t job_id, float* depth, float* normal, uint32_t* primitive,
+                          float* variance);
 /* Background worker that steps whenever work is queued or resident. */
 int olhip_engine_start(olhip_engine* engine);
 int olhip_engine_stop(olhip_engine* engine);
 int olhip_engine_get_stats(olhip_engine* engine, olhip_engine_stats* out);
+
+/* olhip_render_gi with named products: out_rgba (may be NULL), depth,
+ * normal, primitive, variance as for olhip_engine_products (NULL = not
+ * requested). */
+int olhip_render_products(olhip_ctx* ctx, const olhip_scene* scene, const olhip_camera* camera, uint32_t width,
+                          uint32_t height, uint32_t samples, uint32_t frame_index, const olhip_gi_settings* settings,
+                          float* out_rgba, float* depth, float* normal, uint32_t* primitive, float* variance,
+                          olhip_stats* stats);
 
 /* Streaming copy of `bytes` (read + write) repeated `repeats` times on the
  * context's stream; reports milliseconds per copy.  Used to measure

@@ -888,6 +888,68 @@ int API(olgi_render)(void* h, const float cam[12], uint32_t proj, uint32_t W, ui
   return 0;
 }
 
+/* Named products (targets/vulkan/api.py:1252-1451, core.py:4500-4536):
+ * sample 0's primary ray -> closest hit: depth = t (or +inf), geometric
+ * normal normalize(cross(b - a, c - a)) unflipped (wavefront_intersect.comp
+ * :317-356), triangle index (or 0xFFFFFFFF); and the unbiased per-pixel
+ * variance of the float32 sample luminance 0.2126 r + 0.7152 g + 0.0722 b
+ * (float32 arithmetic, float64 sums).  Any output may be NULL. */
+int API(olgi_render_products)(void* h, const float cam[12], uint32_t proj, uint32_t W, uint32_t H, uint32_t spp,
+                              uint32_t frame, const uint32_t iset[4], const float fset[2], uint32_t unified,
+                              float* depth, float* normal, uint32_t* prim, float* variance) {
+  const Scene* s = (const Scene*)h;
+  Settings st = {iset[0], iset[1], iset[2], iset[3], fset[0], fset[1], unified};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t py = 0; py < (int64_t)H; ++py) {
+    uint64_t counts[3] = {0, 0, 0};
+    for (uint32_t px = 0; px < W; ++px) {
+      const uint64_t pix = (uint64_t)py * W + px;
+      if (depth || normal || prim) {
+        V o, d;
+        camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, 0, &o, &d);
+        R t, u, v;
+        uint32_t tri;
+        const int hit = closest(s, o, d, (R)0.001, (R)1e30, &t, &u, &v, &tri);
+        V n = v3(0, 0, 0);
+        if (hit) {
+          const float* a = s->pos + 3ull * s->idx[3ull * tri];
+          const float* b = s->pos + 3ull * s->idx[3ull * tri + 1];
+          const float* c = s->pos + 3ull * s->idx[3ull * tri + 2];
+          V e1 = v3((R)(b[0] - a[0]), (R)(b[1] - a[1]), (R)(b[2] - a[2]));
+          V e2 = v3((R)(c[0] - a[0]), (R)(c[1] - a[1]), (R)(c[2] - a[2]));
+          n = norm(cross(e1, e2));
+        }
+        if (depth) depth[pix] = hit ? (float)t : INFINITY;
+        if (normal) {
+          normal[3 * pix] = (float)n.x;
+          normal[3 * pix + 1] = (float)n.y;
+          normal[3 * pix + 2] = (float)n.z;
+        }
+        if (prim) prim[pix] = hit ? tri : 0xFFFFFFFFu;
+      }
+      if (variance) {
+        double sum = 0.0, sq = 0.0;
+        for (uint32_t smp = 0; smp < spp; ++smp) {
+          V o, d;
+          camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, smp, &o, &d);
+          R rad[3];
+          trace_path(s, &st, (uint32_t)pix, frame, smp, o, d, rad, counts);
+          const float y = ((float)rad[0] * 0.2126f + (float)rad[1] * 0.7152f) + (float)rad[2] * 0.0722f;
+          sum += (double)y;
+          sq += (double)y * (double)y;
+        }
+        double var = 0.0;
+        if (spp > 1) {
+          var = (sq - sum * sum / (double)spp) / (double)(spp - 1);
+          if (var < 0.0) var = 0.0;
+        }
+        variance[pix] = (float)var;
+      }
+    }
+  }
+  return 0;
+}
+
 /* Closest hits for caller rays: o/d [n][4] (xyz + tmin/tmax); hits [n][4]. */
 int API(olgi_trace)(void* h, const float* o, const float* d, uint32_t n, float* hits) {
   const Scene* s = (const Scene*)h;

@@ -45,6 +45,9 @@ def lib():
                 ctypes.c_void_p, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                 ctypes.c_uint32, P_u32, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P_f32, P_u64]
             getattr(L, "olgi_trace" + sfx).argtypes = [ctypes.c_void_p, P_f32, P_f32, ctypes.c_uint32, P_f32]
+            getattr(L, "olgi_render_products" + sfx).argtypes = [
+                ctypes.c_void_p, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                ctypes.c_uint32, P_u32, P_f32, ctypes.c_uint32, P_f32, P_f32, P_u32, P_f32]
             getattr(L, "olgi_wave_hash" + sfx).restype = ctypes.c_uint32
             getattr(L, "olgi_wave_hash" + sfx).argtypes = [ctypes.c_uint32]
             getattr(L, "olgi_pcg_step" + sfx).restype = ctypes.c_uint32
@@ -106,6 +109,22 @@ class OracleScene:
             self.handle, _p(cam), projection, width, height, samples, frame, _p(iset, ctypes.c_uint32),
             _p(fset), int(bool(unified)), r0, r1, _p(out), _p(counts, ctypes.c_uint64))
         self.last_counts = counts
+        return out
+
+    def products(self, camera_basis, projection, width, height, samples, frame_index, *, max_bounces=5,
+                 secondary_area_samples=1, environment_samples=1, roulette_start=0, roulette_min_survival=0.1,
+                 nee_probability=1.0, unified=True):
+        """dict of depth (H, W) float32, normal (H, W, 3), primitive (H, W) uint32, variance (H, W)."""
+        cam = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float32) for v in camera_basis]), np.float32)
+        frame = int(np.float32(frame_index) + np.float32(0.5))
+        iset = np.asarray((max_bounces, secondary_area_samples, environment_samples, roulette_start), np.uint32)
+        fset = np.asarray((roulette_min_survival, nee_probability), np.float32)
+        out = {"depth": np.zeros((height, width), np.float32), "normal": np.zeros((height, width, 3), np.float32),
+               "primitive": np.zeros((height, width), np.uint32), "variance": np.zeros((height, width), np.float32)}
+        getattr(lib(), "olgi_render_products" + self.sfx)(
+            self.handle, _p(cam), projection, width, height, samples, frame, _p(iset, ctypes.c_uint32), _p(fset),
+            int(bool(unified)), _p(out["depth"]), _p(out["normal"]), _p(out["primitive"], ctypes.c_uint32),
+            _p(out["variance"]))
         return out
 
     def trace(self, origins, directions, tmin=0.0, tmax=1e30):

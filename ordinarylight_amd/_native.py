@@ -87,7 +87,7 @@ class EngineConfig(ctypes.Structure):
 class JobDesc(ctypes.Structure):
     _fields_ = [
         ("scene", ctypes.c_void_p), ("camera", Camera), ("width", c_u32), ("height", c_u32),
-        ("samples", c_u32), ("frame_index", c_u32), ("settings", GiSettings),
+        ("samples", c_u32), ("frame_index", c_u32), ("settings", GiSettings), ("products", c_u32), ("pad", c_u32),
     ]
 
 
@@ -114,6 +114,7 @@ class EngineStats(ctypes.Structure):
 
 
 JOB_UNKNOWN, JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = -1, 0, 1, 2, 3
+PRODUCT_PRIMARY, PRODUCT_VARIANCE = 1, 2
 
 
 class E0Camera(ctypes.Structure):
@@ -156,6 +157,10 @@ _SIGNATURES = {
     "olhip_engine_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
     "olhip_engine_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_f64, ctypes.POINTER(ctypes.c_int)]),
     "olhip_engine_result": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, P_f32, ctypes.POINTER(JobStats)]),
+    "olhip_engine_products": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, P_f32, P_f32, P_u32, P_f32]),
+    "olhip_render_products": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Camera), c_u32, c_u32,
+                                             c_u32, c_u32, ctypes.POINTER(GiSettings), P_f32, P_f32, P_f32, P_u32,
+                                             P_f32, ctypes.POINTER(Stats)]),
     "olhip_engine_start": (ctypes.c_int, [ctypes.c_void_p]),
     "olhip_engine_stop": (ctypes.c_int, [ctypes.c_void_p]),
     "olhip_engine_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EngineStats)]),

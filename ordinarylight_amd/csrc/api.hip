@@ -279,6 +279,14 @@ extern "C" {
 int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* cam, uint32_t width, uint32_t height,
                     uint32_t samples, uint32_t frame_index, const olhip_gi_settings* set, float* out_rgba,
                     olhip_stats* stats_out) {
+  return olhip_render_products(ctx, sc, cam, width, height, samples, frame_index, set, out_rgba, nullptr, nullptr,
+                               nullptr, nullptr, stats_out);
+}
+
+int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* cam, uint32_t width,
+                          uint32_t height, uint32_t samples, uint32_t frame_index, const olhip_gi_settings* set,
+                          float* out_rgba, float* depth, float* normal, uint32_t* primitive, float* variance,
+                          olhip_stats* stats_out) {
   return guarded([&] {
     require(ctx && sc && cam && set, "ctx, scene, camera and settings must not be NULL");
     require(sc->ctx == ctx, "scene belongs to another context");
@@ -318,6 +326,7 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
     d.frame_index = frame_index;
     d.settings = *set;
     d.settings.path_capacity = (uint32_t)std::min<uint64_t>(capacity, 0xFFFFFFFFull);
+    d.products = ((depth || normal || primitive) ? OLHIP_PRODUCT_PRIMARY : 0u) | (variance ? OLHIP_PRODUCT_VARIANCE : 0u);
     unsigned long long work0[4] = {0, 0, 0, 0};
     if (ctx->counting) eng.read_work(work0);
     Event e_start, e_end;
@@ -336,6 +345,7 @@ int olhip_render_gi(olhip_ctx* ctx, const olhip_scene* sc, const olhip_camera* c
                                 ctx->stream));
     OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->image_pixels = (uint32_t)np;
+    if (d.products) eng.products(id, depth, normal, primitive, variance);
     eng.take_result(id, out_rgba, &js);
     if (stats_out) {
       olhip_stats& o = *stats_out;
@@ -424,6 +434,14 @@ int olhip_engine_result(olhip_engine* engine, uint64_t job_id, float* out_rgba, 
   return guarded([&] {
     require(engine != nullptr, "engine must not be NULL");
     engine->engine->take_result(job_id, out_rgba, stats);
+  });
+}
+
+int olhip_engine_products(olhip_engine* engine, uint64_t job_id, float* depth, float* normal, uint32_t* primitive,
+                          float* variance) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be NULL");
+    engine->engine->products(job_id, depth, normal, primitive, variance);
   });
 }
 

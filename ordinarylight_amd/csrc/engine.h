@@ -70,6 +70,8 @@ class Engine {
   int state(uint64_t id);
   // Copies the result (optional) and statistics, then forgets the job.
   void take_result(uint64_t id, float* out_rgba, olhip_job_stats* stats, bool keep = false);
+  // Copies a done job's products (any pointer may be null).
+  void products(uint64_t id, float* depth, float* normal, uint32_t* primitive, float* variance);
   void engine_stats(olhip_engine_stats* out);
   // Device image of a done job (valid until take_result).
   const float4* device_image(uint64_t id);
@@ -103,7 +105,7 @@ class Engine {
     uint32_t steps_in_chunk = 0;
     uint32_t passed = 0;        // admission rounds in which this job did not fit
     int status = OLHIP_JOB_QUEUED;
-    DevBuf acc, image;
+    DevBuf acc, image, geo, prim, lum, variance;
     olhip_job_stats stats{};
     std::string error;
   };
@@ -147,6 +149,7 @@ class Engine {
   StepList* list_host_ = nullptr;
   uint32_t* report_host_ = nullptr;
   hipEvent_t done_ = nullptr;
+  hipEvent_t prof_ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   double t0_ms_ = 0;
   // Engine totals.
   olhip_engine_stats totals_{};

@@ -129,6 +129,8 @@ Engine::~Engine() {
   for (auto& g : graph_)
     if (g) (void)hipGraphExecDestroy(g);
   if (done_) (void)hipEventDestroy(done_);
+  for (auto& x : prof_ev_)
+    if (x) (void)hipEventDestroy(x);
   if (jobs_host_) (void)hipHostFree(jobs_host_);
   if (list_host_) (void)hipHostFree(list_host_);
   if (report_host_) (void)hipHostFree(report_host_);
@@ -220,6 +222,7 @@ uint64_t Engine::submit(const olhip_job_desc& d) {
   require_arg(s.russian_roulette_min_survival >= 0.0f && s.russian_roulette_min_survival <= 1.0f,
               "russian_roulette_min_survival must be in [0, 1]");
   require_arg(s.environment_samples <= 4, "environment_samples must be between 0 and 4");
+  require_arg((d.products & ~(OLHIP_PRODUCT_PRIMARY | OLHIP_PRODUCT_VARIANCE)) == 0, "unknown product bits");
   auto j = std::make_unique<Job>();
   j->desc = d;
   j->per_hit = job_entries_per_hit(d.scene, s);
@@ -305,8 +308,18 @@ bool Engine::admit(Job& j, int slot) {
       if (j.first_chunk) {
         j.acc.reserve(sizeof(double4) * np);
         j.image.reserve(sizeof(float4) * np);
+        if (d.products & OLHIP_PRODUCT_PRIMARY) {
+          j.geo.reserve(sizeof(float4) * np);
+          j.prim.reserve(sizeof(uint32_t) * np);
+        }
+        if (d.products & OLHIP_PRODUCT_VARIANCE) {
+          j.lum.reserve(sizeof(double2) * np);
+          j.variance.reserve(sizeof(float) * np);
+        }
         j.stats.admit_ms = now_ms();
       }
+      J.geo = (d.products & OLHIP_PRODUCT_PRIMARY) ? j.geo.as<float4>() : nullptr;
+      J.prim = (d.products & OLHIP_PRODUCT_PRIMARY) ? j.prim.as<uint32_t>() : nullptr;
       j.status = OLHIP_JOB_RUNNING;
       ++j.stats.chunks;
       slot_job_[slot] = j.id;
@@ -337,25 +350,16 @@ void Engine::launch_step(uint32_t in, bool timed) {
     wave_shade(launch_, wave_, in, s);
     wave_shadow(launch_, wave_, s);
   } else {
-    hipEvent_t e[4];
-    for (auto& x : e) OL_HIP_CHECK(hipEventCreate(&x));
-    OL_HIP_CHECK(hipEventRecord(e[0], s));
+    // Read after the step's completion (no synchronisation inside the step).
+    for (auto& x : prof_ev_)
+      if (!x) OL_HIP_CHECK(hipEventCreate(&x));
+    OL_HIP_CHECK(hipEventRecord(prof_ev_[0], s));
     wave_trace(launch_, wave_, in, s);
-    OL_HIP_CHECK(hipEventRecord(e[1], s));
+    OL_HIP_CHECK(hipEventRecord(prof_ev_[1], s));
     wave_shade(launch_, wave_, in, s);
-    OL_HIP_CHECK(hipEventRecord(e[2], s));
+    OL_HIP_CHECK(hipEventRecord(prof_ev_[2], s));
     wave_shadow(launch_, wave_, s);
-    OL_HIP_CHECK(hipEventRecord(e[3], s));
-    OL_HIP_CHECK(hipEventSynchronize(e[3]));
-    float a = 0, b = 0, c = 0;
-    OL_HIP_CHECK(hipEventElapsedTime(&a, e[0], e[1]));
-    OL_HIP_CHECK(hipEventElapsedTime(&b, e[1], e[2]));
-    OL_HIP_CHECK(hipEventElapsedTime(&c, e[2], e[3]));
-    trace_ms += a;
-    shade_ms += b;
-    shadow_ms += c;
-    ++trace_launches;
-    for (auto& x : e) (void)hipEventDestroy(x);
+    OL_HIP_CHECK(hipEventRecord(prof_ev_[3], s));
   }
   wave_epilogue(wave_, in, s);
 }
@@ -436,6 +440,7 @@ uint32_t Engine::step() {
   OL_HIP_CHECK(hipMemcpyAsync(jobs_dev_.ptr, jobs_host_, sizeof(JobDev) * kMaxJobs, hipMemcpyHostToDevice, s));
   OL_HIP_CHECK(hipMemcpyAsync(list_dev_.ptr, list_host_, sizeof(StepList), hipMemcpyHostToDevice, s));
   const uint32_t in = parity_;
+  const bool timed = any_resident && profiling_;
   if (!any_resident) {
     // Only accumulation left: no traversal launches.
     wave_accumulate(wave_, ctx_->cu_count, s);
@@ -450,6 +455,16 @@ uint32_t Engine::step() {
                               hipMemcpyDeviceToHost, s));
   OL_HIP_CHECK(hipEventRecord(done_, s));
   OL_HIP_CHECK(hipEventSynchronize(done_));
+  if (timed) {
+    float a = 0, b = 0, c = 0;
+    OL_HIP_CHECK(hipEventElapsedTime(&a, prof_ev_[0], prof_ev_[1]));
+    OL_HIP_CHECK(hipEventElapsedTime(&b, prof_ev_[1], prof_ev_[2]));
+    OL_HIP_CHECK(hipEventElapsedTime(&c, prof_ev_[2], prof_ev_[3]));
+    trace_ms += a;
+    shade_ms += b;
+    shadow_ms += c;
+    ++trace_launches;
+  }
   uint32_t pending = 0;
   {
     std::lock_guard<std::mutex> g(mu_);
@@ -492,6 +507,8 @@ uint32_t Engine::step() {
           a.final_samples = last ? j.desc.samples : 0u;
           a.acc = j.acc.as<double4>();
           a.image = j.image.as<float4>();
+          a.lum = (j.desc.products & OLHIP_PRODUCT_VARIANCE) ? j.lum.as<double2>() : nullptr;
+          a.variance = (j.desc.products & OLHIP_PRODUCT_VARIANCE) ? j.variance.as<float>() : nullptr;
           if (!stack_overflow) {
             pending_acc_.push_back(a);
             if (last) pending_acc_jobs_.push_back(j.id);
@@ -579,6 +596,37 @@ void Engine::take_result(uint64_t id, float* out_rgba, olhip_job_stats* stats, b
   if (out_rgba)
     OL_HIP_CHECK(hipMemcpy(out_rgba, j->image.ptr, sizeof(float4) * j->desc.width * j->desc.height,
                            hipMemcpyDeviceToHost));
+}
+
+void Engine::products(uint64_t id, float* depth, float* normal, uint32_t* primitive, float* variance) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) throw OlError{OLHIP_ERR_VALUE, "unknown job id"};
+  const Job& j = *it->second;
+  if (j.status != OLHIP_JOB_DONE) throw OlError{OLHIP_ERR_RUNTIME, "job is not done"};
+  const size_t np = (size_t)j.desc.width * j.desc.height;
+  OL_HIP_CHECK(hipSetDevice(ctx_->device));
+  if (depth || normal) {
+    if (!(j.desc.products & OLHIP_PRODUCT_PRIMARY)) throw OlError{OLHIP_ERR_VALUE, "primary-hit products not requested"};
+    std::vector<float4> g4(np);
+    OL_HIP_CHECK(hipMemcpy(g4.data(), j.geo.ptr, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < np; ++i) {
+      if (depth) depth[i] = g4[i].x;
+      if (normal) {
+        normal[3 * i] = g4[i].y;
+        normal[3 * i + 1] = g4[i].z;
+        normal[3 * i + 2] = g4[i].w;
+      }
+    }
+  }
+  if (primitive) {
+    if (!(j.desc.products & OLHIP_PRODUCT_PRIMARY)) throw OlError{OLHIP_ERR_VALUE, "primary-hit products not requested"};
+    OL_HIP_CHECK(hipMemcpy(primitive, j.prim.ptr, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
+  }
+  if (variance) {
+    if (!(j.desc.products & OLHIP_PRODUCT_VARIANCE)) throw OlError{OLHIP_ERR_VALUE, "variance not requested"};
+    OL_HIP_CHECK(hipMemcpy(variance, j.variance.ptr, sizeof(float) * np, hipMemcpyDeviceToHost));
+  }
 }
 
 void Engine::engine_stats(olhip_engine_stats* out) {

@@ -65,6 +65,11 @@ struct JobDev {
   uint32_t ebase;    // first shadow-entry slot
   uint32_t eseg;     // shadow-entry slots per segment
   uint32_t pad[2];
+  // Primary-hit products of sample 0 (null unless requested): per pixel
+  // (t or +inf, geometric normal) and the original triangle index (or
+  // OL_NONE) (wavefront_intersect.comp:317-356, targets/vulkan/core.py:4500-4536).
+  float4* geo;
+  uint32_t* prim;
 };
 
 // Accumulation of one finished sample chunk into its job's float64 sums.
@@ -74,6 +79,11 @@ struct AccItem {
   uint32_t pad;
   double4* acc;
   float4* image;
+  // Variance product (null unless requested): per pixel float64 sums of the
+  // float32 sample luminance and its square; with final_samples, the
+  // unbiased variance (render_wavefront_outputs, api.py:1408-1432).
+  double2* lum;
+  float* variance;
 };
 
 // Per-step work lists written by the host before each step.

@@ -134,6 +134,28 @@ __global__ void __launch_bounds__(256) k_accumulate(Wave W) {
       const double ns = (double)it.final_samples;
       it.image[pixel] = make_float4((float)(a.x / ns), (float)(a.y / ns), (float)(a.z / ns), (float)(a.w / ns));
     }
+    if (it.lum) {
+      // Luminance of each float32 sample (float32 products and sums, as the
+      // reference's NumPy expression), summed with its square in float64.
+      double2 m = it.first_chunk ? make_double2(0.0, 0.0) : it.lum[pixel];
+      for (uint32_t s = 0; s < it.spp; ++s) {
+        const float4 l = W.L[it.base + s * it.np + pixel];
+        const double y = (double)((l.x * 0.2126f + l.y * 0.7152f) + l.z * 0.0722f);
+        if (it.first_chunk && s == 0) {
+          m = make_double2(y, y * y);
+        } else {
+          m.x += y;
+          m.y += y * y;
+        }
+      }
+      it.lum[pixel] = m;
+      if (it.final_samples) {
+        const double n = (double)it.final_samples;
+        double v = 0.0;
+        if (it.final_samples > 1) v = fmax((m.y - m.x * m.x / n) / (n - 1.0), 0.0);
+        it.variance[pixel] = (float)v;
+      }
+    }
   }
 }
 
@@ -495,7 +517,14 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
           M.w &= ~F_ACTIVE;
           break;
         }
+        // Sample 0's primary hit for the named products.
+        const bool record_primary = J.geo != nullptr && T.w == 0.0f && P.sample_base == 0u &&
+                                    path - J.base < P.width * P.height;
         if (slot == OL_NONE) {
+          if (record_primary) {
+            J.geo[M.x] = make_float4(__builtin_huge_valf(), 0.0f, 0.0f, 0.0f);
+            J.prim[M.x] = OL_NONE;
+          }
           // shadeResolveEnvironmentMiss
           f3 env = environment_radiance(sc, incoming);
           float mis = 1.0f;
@@ -518,6 +547,10 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
         const uint32_t prim = __float_as_uint(ta.w);
         hitp = mk3(ro.x + t * rd.x, ro.y + t * rd.y, ro.z + t * rd.z);
         const f3 gn = normalize3(cross3(mk3(te1), mk3(te2)));
+        if (record_primary) {
+          J.geo[M.x] = make_float4(t, gn.x, gn.y, gn.z);
+          J.prim[M.x] = prim;
+        }
         const uint4 vi = sc.tri_vidx[prim];
         const float wx = (1.0f - bu) - bv, wy = bu, wz = bv;
         const f3 n0 = rgb(sc.normals[vi.x]), n1 = rgb(sc.normals[vi.y]), n2 = rgb(sc.normals[vi.z]);

@@ -241,7 +241,7 @@ def _check_size(width, height):
 class HipPathTracer(_ResidentMixin, RendererImplementation):
     """GI wavefront path tracer on one MI355X (drop-in for the Vulkan GI path)."""
 
-    available_outputs = ("color",)
+    available_outputs = ("color", "variance", "depth", "normal", "instance_id", "object_id", "material_id")
     implementation = RendererImplementationInfo(
         name="hip-wavefront-gi", family="global_illumination", graphics_api="hip",
     )
@@ -280,7 +280,7 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
             self.config = replace(self.config, **changes)
             return self.config
 
-    def _render(self, scene, camera, width, height, samples, frame_index, keep_on_device=False):
+    def _render(self, scene, camera, width, height, samples, frame_index, keep_on_device=False, products=None):
         if self._closed:
             raise RuntimeError("renderer is closed")
         width, height = _check_size(width, height)
@@ -313,9 +313,17 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         out = None if keep_on_device else np.empty((height, width, 4), np.float32)
         stats = N.Stats()
         settings = self.config.gi_settings()
-        N.check(lib.olhip_render_gi(context.handle, device_scene.handle, N.ctypes.byref(cam), width, height,
-                                    samples, frame_index, N.ctypes.byref(settings),
-                                    N.ptr(out) if out is not None else None, N.ctypes.byref(stats)))
+        if products is None:
+            N.check(lib.olhip_render_gi(context.handle, device_scene.handle, N.ctypes.byref(cam), width, height,
+                                        samples, frame_index, N.ctypes.byref(settings),
+                                        N.ptr(out) if out is not None else None, N.ctypes.byref(stats)))
+        else:
+            N.check(lib.olhip_render_products(
+                context.handle, device_scene.handle, N.ctypes.byref(cam), width, height, samples, frame_index,
+                N.ctypes.byref(settings), N.ptr(out) if out is not None else None,
+                *(N.ptr(products[k], N.ctypes.c_uint32 if k == "primitive" else N.ctypes.c_float)
+                  if k in products else None for k in ("depth", "normal", "primitive", "variance")),
+                N.ctypes.byref(stats)))
         total_ms = (time.perf_counter() - started) * 1000.0
         rays = stats.primary_rays + stats.extension_rays + stats.shadow_rays
         self.last_stats = stats
@@ -340,8 +348,38 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         unsupported = set(outputs) - set(self.available_outputs)
         if not outputs or unsupported:
             raise ValueError(f"unsupported render outputs: {tuple(sorted(unsupported))}")
+        width, height = _check_size(width, height)
+        ids = {"instance_id", "object_id", "material_id"} & set(outputs)
+        buffers = {}
+        if {"depth"} & set(outputs):
+            buffers["depth"] = np.empty((height, width), np.float32)
+        if "normal" in outputs:
+            buffers["normal"] = np.empty((height, width, 3), np.float32)
+        if ids:
+            buffers["primitive"] = np.empty((height, width), np.uint32)
+        if "variance" in outputs:
+            buffers["variance"] = np.empty((height, width), np.float32)
         with self._lock:
-            return {"color": self._render(scene, camera, width, height, samples, frame_index)}
+            color = self._render(scene, camera, width, height, samples, frame_index, products=buffers)
+        result = {}
+        if "color" in outputs:
+            result["color"] = color
+        for name in ("depth", "normal", "variance"):
+            if name in outputs:
+                result[name] = buffers[name]
+        if ids:
+            # Primitive -> id maps on the host, as the reference does
+            # (targets/vulkan/api.py:1360-1386); misses stay 0xFFFFFFFF.
+            prim = buffers["primitive"]
+            hit = prim != np.uint32(0xFFFFFFFF)
+            maps = {"instance_id": scene.triangle_instance_ids, "object_id": scene.triangle_instance_ids,
+                    "material_id": scene.triangle_material_ids}
+            for name in ids:
+                ids_out = np.full((height, width), np.uint32(0xFFFFFFFF), np.uint32)
+                table = np.asarray(maps[name](), np.uint32)
+                ids_out[hit] = table[prim[hit]]
+                result[name] = ids_out
+        return {name: result[name] for name in outputs}
 
     def close(self):
         _ResidentMixin.close(self)

@@ -7,10 +7,13 @@ workload definitions of this package, all deterministic in their seed:
   boxes with a seeded rotation about +Y (24), one emissive ceiling quad (2)
   of radiance 15.  Albedos are uniform in [0.1, 0.9]^3.  Pinhole camera.
 * :func:`terrain` -- fBm-displaced heightfield of ``(nx-1)*(nz-1)*2``
-  triangles (1001 x 501 vertices give exactly 1,000,000), split into chunk
+  triangles (1001 x 501 vertices give 1,000,000), split into chunk
   meshes that each carry one GGX material (roughness ~ U[0.05, 1],
   metallic ~ Bernoulli(0.3)), plus 64 emissive triangles and a constant
   environment of radiance 0.5.
+* :func:`tessellated_room`, :func:`mixed_scene`, :func:`mixed_jobs` -- the
+  batched/served workload: eight scenes of 10k to 2M triangles from the two
+  generators and a seeded list of jobs over them.
 """
 
 from __future__ import annotations
@@ -173,4 +176,98 @@ def terrain(seed=0, nx=1001, nz=501, chunk=50, emissive_quads=32, height_scale=6
     return scene, camera
 
 
-__all__ = ["box_room", "terrain"]
+def _tessellate(corners, n):
+    """Planar quad (p0, p1, p2, p3) as an n x n grid: 2 n^2 triangles wound like _quad."""
+    p0, p1, p2, p3 = (np.asarray(c, np.float64) for c in corners)
+    t = np.linspace(0.0, 1.0, n + 1)
+    u, v = np.meshgrid(t, t, indexing="ij")
+    u, v = u[..., None], v[..., None]
+    grid = (1 - u) * (1 - v) * p0 + u * (1 - v) * p1 + u * v * p2 + (1 - u) * v * p3
+    vid = np.arange((n + 1) * (n + 1), dtype=np.uint32).reshape(n + 1, n + 1)
+    a, b = vid[:-1, :-1].ravel(), vid[1:, :-1].ravel()
+    c, d = vid[1:, 1:].ravel(), vid[:-1, 1:].ravel()
+    indices = np.empty((2 * n * n, 3), np.uint32)
+    indices[0::2] = np.stack((a, b, c), 1)
+    indices[1::2] = np.stack((a, c, d), 1)
+    return grid.reshape(-1, 3).astype(np.float32), indices
+
+
+def tessellated_room(seed=0, target_triangles=10_000):
+    """:func:`box_room` with its five walls tessellated into n x n grids so the
+    room holds about ``target_triangles`` (same materials, boxes, light and
+    camera as the 36-triangle room of the same seed)."""
+    rng = np.random.default_rng(seed)
+    albedo = lambda: tuple(float(v) for v in rng.uniform(0.1, 0.9, 3))  # noqa: E731
+    n = max(1, int(round(math.sqrt(max(target_triangles - 26, 10) / 10.0))))
+    scene = Scene()
+    walls = (
+        ((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)),
+        ((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),
+        ((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)),
+        ((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)),
+        ((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)),
+    )
+    for corners in walls:
+        vertices, indices = _tessellate(corners, n)
+        scene.add_mesh(vertices, indices, Material(base_color=albedo()))
+    for size, position in (((0.55, 1.2, 0.55), (-0.38, 0.0, -0.3)),
+                           ((0.55, 0.6, 0.55), (0.4, 0.0, 0.35))):
+        vertices, indices = _box(size, rng)
+        angle = float(rng.uniform(-0.6, 0.6))
+        transform = Transform.translation(position) @ Transform.rotation((0, 1, 0), angle)
+        scene.add_mesh(vertices, indices, Material(base_color=albedo()), transform=transform)
+    light_v, light_i = _quad((-0.25, 1.99, -0.25), (0.25, 1.99, -0.25),
+                             (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25))
+    scene.add_mesh(light_v, light_i, Material(base_color=(0.8, 0.8, 0.8),
+                                              emission=(15.0, 15.0, 15.0)))
+    camera = PerspectiveCamera(position=(0.0, 1.0, 3.4), target=(0.0, 1.0, 0.0),
+                               vertical_fov_degrees=40.0)
+    return scene, camera
+
+
+# Resident scenes of the batched and served workloads: three rooms and five
+# terrains spanning 10k to 2M triangles (terrain with (2k+1) x (k+1) vertices
+# has 4 k^2 triangles).
+MIXED_SCENES = (
+    ("room_10k", "room", 10_000),
+    ("room_40k", "room", 40_000),
+    ("room_160k", "room", 160_000),
+    ("terrain_100k", "terrain", 158),
+    ("terrain_250k", "terrain", 250),
+    ("terrain_500k", "terrain", 354),
+    ("terrain_1m", "terrain", 500),
+    ("terrain_2m", "terrain", 707),
+)
+
+
+def mixed_scene(key, seed=0):
+    """One scene of :data:`MIXED_SCENES` by key."""
+    for name, kind, size in MIXED_SCENES:
+        if name == key:
+            if kind == "room":
+                return tessellated_room(seed, size)
+            return terrain(seed, nx=2 * size + 1, nz=size + 1, chunk=50, emissive_quads=32)
+    raise KeyError(key)
+
+
+def mixed_jobs(seed=0, n_jobs=64, scene_keys=None, min_edge=128, max_edge=1024):
+    """Job list of the batched workload: square images whose edge is
+    log-uniform in [min_edge, max_edge], a scene drawn uniformly from
+    ``scene_keys`` (default: all of :data:`MIXED_SCENES`), samples per pixel
+    uniform in {1, 2, 4, 8, 16} and max_bounces in {4, 8}."""
+    rng = np.random.default_rng(seed)
+    keys = list(scene_keys) if scene_keys is not None else [name for name, _, _ in MIXED_SCENES]
+    jobs = []
+    for _ in range(n_jobs):
+        edge = int(round(math.exp(rng.uniform(math.log(min_edge), math.log(max_edge)))))
+        jobs.append({
+            "scene": keys[int(rng.integers(len(keys)))],
+            "width": edge,
+            "height": edge,
+            "samples": int(rng.choice((1, 2, 4, 8, 16))),
+            "max_bounces": int(rng.choice((4, 8))),
+        })
+    return jobs
+
+
+__all__ = ["box_room", "terrain", "tessellated_room", "MIXED_SCENES", "mixed_scene", "mixed_jobs"]
