@@ -14,6 +14,9 @@
 3. Served (``served``): Poisson arrivals of such jobs at a fixed fraction of
    the batched throughput; continuous-batching server against a plain
    one-at-a-time server on the same arrivals; p50/p99 latency and jobs/s.
+4. Instanced (``c3``): a two-level scene of 10,000 instances of five
+   20,480-triangle icospheres (~205M effective triangles), 20 % glass,
+   100k emitters; 1920 x 1080, 4 spp, 16 bounces.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
@@ -211,6 +214,37 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8):
     }
 
 
+def c3_measurement(context, steps=2):
+    """Instanced two-level scene: 5 jittered icospheres (20,480 triangles each)
+    placed 10,000 times (~205M effective triangles), 20 % glass, 100k small
+    emitters; 1920x1080, 4 spp, 16 bounces, float32."""
+    from ordinarylight_amd import workloads
+    from ordinarylight_amd.batch import Job, plain_render
+    from ordinarylight_amd.renderers.hip import DeviceScene, HipRendererConfig
+    t0 = time.perf_counter()
+    scene, camera = workloads.instanced_spheres(0)
+    gen_s = time.perf_counter() - t0
+    ds = DeviceScene(context, scene, 2, instanced=True)
+    job = Job(ds, camera, 1920, 1080, 4, 0, HipRendererConfig(max_bounces=16, samples_per_pixel=4))
+    plain_render(job, copy_image=False)  # warm-up
+    rays, wall = 0, 0.0
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        _, st = plain_render(job, copy_image=False)
+        wall += time.perf_counter() - t1
+        rays += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+    out = {
+        "workload": "5 icospheres x 10,000 instances (204,900,000 triangles), 20% glass, 100k emitters, "
+                    "1920x1080, 4 spp, 16 bounces",
+        "mrays_per_s": round(rays / wall / 1e6, 1), "ms_per_frame": round(wall * 1e3 / steps, 2),
+        "rays_per_frame": int(rays / steps), "scene_generation_s": round(gen_s, 2),
+        "scene_build_ms": round(ds.info.build_ms, 2), "effective_triangles": int(ds.triangle_count),
+        "direct_lighting": "next-event estimation (light CDF)",
+    }
+    ds.close()
+    return out
+
+
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -231,6 +265,7 @@ def main(argv=None):
     parser.add_argument("--serve-load", type=float, default=0.7,
                         help="Poisson arrival rate as a fraction of the batched jobs/s")
     parser.add_argument("--no-autotune", action="store_true")
+    parser.add_argument("--no-c3", action="store_true", help="skip the instanced two-level workload")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -339,6 +374,13 @@ def main(argv=None):
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(scene, camera, W, H, spp, B, args.cpu_seconds)
     renderer.close()
+
+    if not args.no_c3:
+        from ordinarylight_amd.renderers.hip import DeviceContext
+        context = DeviceContext(0)
+        line["c3"] = c3_measurement(context)
+        log(f"c3: {line['c3']}")
+        context.close()
 
     if args.batch_jobs > 0 or args.serve_jobs > 0:
         from ordinarylight_amd.renderers.hip import DeviceContext

@@ -67,6 +67,40 @@ typedef struct olhip_scene_desc {
   uint32_t leaf_size;          /* BVH leaf size 1..16, 0 = default (4) */
 } olhip_scene_desc;
 
+/* Two-level scene: object-space mesh resources, each with its own
+ * bottom-level BVH, placed by instances under affine transforms; a top-level
+ * BVH over the instances' world bounds.  Replaces the BLAS/TLAS build of
+ * VulkanSceneResources (targets/vulkan/scene.py:1132-1205, 1322-1561).
+ * Triangle order (ids, tie-breaks, products) is the flattened packed order:
+ * instance k's triangles start at instance_triangle_offset[k]. */
+typedef struct olhip_instanced_desc {
+  uint32_t resource_count;
+  const uint32_t* vertex_offsets;      /* [resource_count + 1] into positions / normals */
+  const uint32_t* triangle_offsets;    /* [resource_count + 1] into indices */
+  const float* positions;              /* object space [vertex_offsets[R]][3] */
+  const float* normals;                /* object space unit normals, same layout */
+  const uint32_t* indices;             /* [triangle_offsets[R]][3], local to the resource */
+  uint32_t instance_count;
+  const uint32_t* instance_resource;   /* [I] */
+  const uint32_t* instance_material;   /* [I] index into materials */
+  const uint32_t* instance_triangle_offset; /* [I] */
+  const uint32_t* instance_id;         /* [I] scene instance ids */
+  const float* world_to_object;        /* [I][3][4] affine rows */
+  const float* normal_matrix;          /* [I][3][3] inverse transpose of the linear part */
+  const float* determinant;            /* [I] determinant of the linear part */
+  const float* world_bounds;           /* [I][6] min xyz, max xyz (rounded outward) */
+  uint32_t material_count;
+  const float* materials;              /* [material_count][12][4] */
+  uint32_t analytic_light_count;
+  const float* analytic_lights;
+  uint32_t area_light_count;
+  const float* area_lights;            /* world-space emissive triangles [L][5][4] */
+  float area_light_weight;
+  uint32_t leaf_size;                  /* bottom-level leaf size 1..16, 0 = 4 */
+} olhip_instanced_desc;
+
+int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* desc, olhip_scene** out);
+
 typedef struct olhip_scene_info {
   uint32_t triangle_count;
   uint32_t node_count;
@@ -107,6 +141,19 @@ typedef struct olhip_gi_settings {
   float secondary_nee_probability;    /* (0, 1] */
   uint32_t unified_secondary_nee;     /* 0 or 1 */
   uint32_t path_capacity;             /* resident paths per chunk, 0 = default */
+  /* ReSTIR DI for area lights at primary hits (wavefront_restir.glsl,
+   * wavefront_primary_impl.glsl:1539-1700; settings api.py:60-75):
+   * candidates per pixel (0 = off, <= 64), spatial neighbours (0 = no
+   * spatial reuse, <= 8) and their pixel radius (<= 32), the represented-
+   * sample limit of reused reservoirs (1..64), and pairwise (1) or canonical
+   * (0) spatial merges.  The samples of such a job run as successive
+   * one-sample passes, each reusing the previous pass's reservoirs as the
+   * reference reuses the previous frame's. */
+  uint32_t restir_candidates;
+  uint32_t restir_spatial_neighbors;
+  uint32_t restir_spatial_radius;
+  uint32_t restir_history_limit;
+  uint32_t restir_pairwise_mis;
 } olhip_gi_settings;
 
 typedef struct olhip_stats {

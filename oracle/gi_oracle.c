@@ -625,11 +625,371 @@ static int point_sample(const Scene* s, uint32_t li, V hit, V n, const Mat* m, V
   return 1;
 }
 
+/* ---- ReSTIR DI at primary hits ----------------------------------------------
+ * Reservoir arithmetic of wavefront_restir.glsl:51-180 (update :51, merge
+ * :68, canonical :83, pairwise :94, normalisation :124, limit :134, stored
+ * form :150-180), candidates of transport_v1/lighting.glsl:441-552 and the
+ * primary reuse loop of wavefront_primary_impl.glsl:1539-1700 (reprojection
+ * :350-365, history position :367-378, surface test :382-393, spatial
+ * offsets :319-331).  Halves are IEEE binary16 with round-to-nearest-even,
+ * as packHalf2x16. */
+#define OL_NONE_ID 0xFFFFFFFFu
+
+uint16_t API(olgi_f2h)(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+  const uint32_t e = ax >> 23, m = (ax & 0x7FFFFFu) | 0x800000u;
+  if (e > 142u) return (uint16_t)(sign | 0x7C00u);
+  if (e < 113u) { /* half subnormal: round(value * 2^24) */
+    if (e < 102u) return (uint16_t)sign;
+    const uint32_t sh = 126u - e, q = m >> sh, rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+    return (uint16_t)(sign | (q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u)));
+  }
+  uint32_t hbits = ((e - 112u) << 10) | ((ax & 0x7FFFFFu) >> 13);
+  const uint32_t rem = ax & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (hbits & 1u))) hbits++;
+  return (uint16_t)(sign | hbits);
+}
+float API(olgi_h2f)(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 31u, m = h & 1023u;
+  float v;
+  if (e == 0u) v = ldexpf((float)m, -24);
+  else if (e == 31u) v = m ? NAN : INFINITY;
+  else v = ldexpf((float)(m | 1024u), (int)e - 25);
+  uint32_t b;
+  memcpy(&b, &v, 4);
+  b |= sign;
+  memcpy(&v, &b, 4);
+  return v;
+}
+static uint32_t pack_h2(R a, R b) {
+  return (uint32_t)API(olgi_f2h)((float)a) | ((uint32_t)API(olgi_f2h)((float)b) << 16);
+}
+static R h_lo(uint32_t v) { return (R)API(olgi_h2f)((uint16_t)(v & 0xFFFFu)); }
+static R h_hi(uint32_t v) { return (R)API(olgi_h2f)((uint16_t)(v >> 16)); }
+
+typedef struct {
+  uint32_t light, bary; /* light (OL_NONE_ID: empty), half2 barycentrics */
+  R w_sum;
+  uint32_t tc;          /* half2 (target, represented samples) */
+} Rsv;
+
+static Rsv rsv_empty(void) {
+  Rsv r = {OL_NONE_ID, 0u, (R)0, 0u};
+  return r;
+}
+static int rsv_update(Rsv* r, uint32_t light, R b1, R b2, R target, R weight, R represented, R random) {
+  weight = rmax(weight, (R)0);
+  const R w_sum = r->w_sum + weight;
+  R t = h_lo(r->tc);
+  const R count = h_hi(r->tc) + represented;
+  const int sel = weight > (R)0 && (r->light == OL_NONE_ID || random * w_sum < weight);
+  if (sel) {
+    r->light = light;
+    r->bary = pack_h2(clampr(b1, 0, 1), clampr(b2, 0, 1));
+    t = rmin(rmax(target, (R)0), (R)65504);
+  }
+  r->w_sum = w_sum;
+  r->tc = pack_h2(t, count);
+  return sel;
+}
+static void rsv_merge(Rsv* d, const Rsv* src, R target_here, R random) {
+  const R st = h_lo(src->tc), sc = h_hi(src->tc);
+  if (src->light == OL_NONE_ID || sc <= (R)0) return;
+  R w = 0;
+  if (st > (R)0) w = (rmax(target_here, (R)0) * src->w_sum) / st;
+  rsv_update(d, src->light, h_lo(src->bary), h_hi(src->bary), target_here, w, sc, random);
+}
+static void rsv_merge_canonical(Rsv* d, const Rsv* src, R target_here, R random) {
+  const R st = h_lo(src->tc), sc = h_hi(src->tc);
+  if (src->light == OL_NONE_ID || st <= (R)0 || sc <= (R)0) return;
+  const R w = (rmax(target_here, (R)0) * src->w_sum) / (st * sc);
+  rsv_update(d, src->light, h_lo(src->bary), h_hi(src->bary), target_here, w, 1, random);
+}
+static void rsv_merge_pairwise(Rsv* d, const Rsv* src, R target_here, R target_there, R random) {
+  const R st = h_lo(src->tc), sc = h_hi(src->tc);
+  if (src->light == OL_NONE_ID || st <= (R)0 || sc <= (R)0) return;
+  const R total = rmax(target_here, (R)0) + rmax(target_there, (R)0);
+  R balance = 0;
+  if (total > (R)0) balance = ((R)2 * rmax(target_there, (R)0)) / total;
+  const R w = ((rmax(target_here, (R)0) * src->w_sum) / (st * sc)) * balance;
+  rsv_update(d, src->light, h_lo(src->bary), h_hi(src->bary), target_here, w, 1, random);
+}
+static R rsv_normalization(const Rsv* r) {
+  const R t = h_lo(r->tc), c = h_hi(r->tc);
+  if (r->light == OL_NONE_ID || t <= (R)0 || c <= (R)0) return 0;
+  return r->w_sum / (c * t);
+}
+static Rsv rsv_limit(Rsv r, R maximum) {
+  const R t = h_lo(r.tc), c = h_hi(r.tc);
+  maximum = rmax(maximum, (R)0);
+  if (c > maximum && c > (R)0) {
+    const R scale = maximum / c;
+    r.w_sum = r.w_sum * scale;
+    r.tc = pack_h2(t, maximum);
+  }
+  return r;
+}
+/* Stored form: light | count << 25, half2 barycentrics, half2 (w_sum, target). */
+static void rsv_store(const Rsv* r, uint32_t out[4]) {
+  const R t = h_lo(r->tc), c = h_hi(r->tc);
+  uint32_t light = 33554431u;
+  if (r->light != OL_NONE_ID) light = r->light < 33554430u ? r->light : 33554430u;
+  const R rc = sizeof(R) == 4 ? (R)rintf((float)c) : (R)rint((double)c);
+  const uint32_t count = (uint32_t)rmin(rmax(rc, (R)0), (R)127);
+  const R w = rmin(rmax(r->w_sum, (R)0), (R)65504);
+  out[0] = light | (count << 25);
+  out[1] = r->bary;
+  out[2] = pack_h2(w, t);
+  out[3] = 0u;
+}
+static Rsv rsv_load(const uint32_t w[4]) {
+  uint32_t light = w[0] & 33554431u;
+  const uint32_t count = w[0] >> 25;
+  if (light == 33554431u) light = OL_NONE_ID;
+  Rsv r = {light, w[1], h_lo(w[2]), pack_h2(h_hi(w[2]), (R)count)};
+  return r;
+}
+
+/* Reservoir entry points for the tests: a reservoir in register form is
+ * (light, half2 bary, w_sum float bits, half2 (target, count)). */
+static Rsv rsv_from(const uint32_t w[4]) {
+  float f;
+  memcpy(&f, &w[2], 4);
+  Rsv r = {w[0], w[1], (R)f, w[3]};
+  return r;
+}
+static void rsv_to(const Rsv* r, uint32_t w[4]) {
+  const float f = (float)r->w_sum;
+  w[0] = r->light;
+  w[1] = r->bary;
+  memcpy(&w[2], &f, 4);
+  w[3] = r->tc;
+}
+/* n updates from an empty reservoir; selected[i] = 1 when candidate i was taken. */
+void API(olgi_rsv_updates)(uint32_t n, const uint32_t* light, const float* b1, const float* b2, const float* target,
+                           const float* weight, const float* represented, const float* random, uint32_t out[4],
+                           uint8_t* selected) {
+  Rsv r = rsv_empty();
+  for (uint32_t i = 0; i < n; ++i) {
+    const int s = rsv_update(&r, light[i], b1[i], b2[i], target[i], weight[i], represented[i], random[i]);
+    if (selected) selected[i] = (uint8_t)s;
+  }
+  rsv_to(&r, out);
+}
+/* mode 0: temporal merge, 1: pairwise, 2: canonical; 3: limit(src, target_here). */
+void API(olgi_rsv_merge)(uint32_t dst[4], const uint32_t src[4], float target_here, float target_there, float random,
+                         uint32_t mode) {
+  Rsv d = rsv_from(dst), s = rsv_from(src);
+  if (mode == 0u) rsv_merge(&d, &s, target_here, random);
+  else if (mode == 1u) rsv_merge_pairwise(&d, &s, target_here, target_there, random);
+  else if (mode == 2u) rsv_merge_canonical(&d, &s, target_here, random);
+  else d = rsv_limit(s, target_here);
+  rsv_to(&d, dst);
+}
+float API(olgi_rsv_normalization)(const uint32_t r[4]) {
+  Rsv x = rsv_from(r);
+  return (float)rsv_normalization(&x);
+}
+void API(olgi_rsv_store)(const uint32_t r[4], uint32_t stored[4]) {
+  Rsv x = rsv_from(r);
+  rsv_store(&x, stored);
+}
+void API(olgi_rsv_load)(const uint32_t stored[4], uint32_t r[4]) {
+  Rsv x = rsv_load(stored);
+  rsv_to(&x, r);
+}
+
+/* evaluateAreaLightCandidate: MIS-weighted NEE contribution of the point at
+ * (b1, b2) on area light `li` (technique probability 1). */
+static V cand_contribution(const Scene* s, uint32_t li, R b1, R b2, V hit, V n, V inc, const Mat* m,
+                           uint32_t count, V* dir, R* dist) {
+  *dir = n;
+  *dist = 0;
+  if (li >= s->L) return v3(0, 0, 0);
+  const float* A = s->area + 20ull * li;
+  V a = v3(A[0], A[1], A[2]), b = v3(A[4], A[5], A[6]), c = v3(A[8], A[9], A[10]);
+  const R aw = rmax(((R)1 - b1) - b2, (R)0);
+  V p = vadd(vadd(vs(a, aw), vs(b, b1)), vs(c, b2));
+  V off = vsub(p, hit);
+  const R d2 = dot(off, off);
+  *dist = rsqrt_(d2);
+  *dir = vdiv(off, rmax(*dist, (R)1e-6));
+  const R cs = rmax(dot(n, *dir), (R)0);
+  V ln = norm(cross(vsub(b, a), vsub(c, a)));
+  const R raw = dot(ln, vneg(*dir));
+  const R cl = A[18] > 0.5f ? absr(raw) : rmax(raw, (R)0);
+  if (cs <= (R)0 || cl <= (R)1e-6) return v3(0, 0, 0);
+  const R eff = (((R)A[17] * d2) / rmax(cl * (R)A[15], (R)1e-6)) * rmax((R)1, (R)1e-6);
+  const R bp = pdf_pbr(m, n, vneg(inc), *dir);
+  const R first = eff * (R)count, fs = first * first;
+  const R mis = fs / rmax(fs + bp * bp, (R)1e-6);
+  return vdiv(vs(vs(vs(vmul(eval_pbr(m, n, vneg(inc), *dir), v3(A[12], A[13], A[14])), cs), mis), (R)1),
+              rmax(eff, (R)1e-6));
+}
+static R cand_target(V c) { return rmax(dot(c, v3((R)0.2126, (R)0.7152, (R)0.0722)), (R)0); }
+
+/* ReSTIR settings and the reservoir/surface buffers of one pass.  Reservoirs
+ * are 4 words per pixel; surfaces 8 floats per pixel (shading normal, primary
+ * distance or -1; material id bits). */
+typedef struct {
+  uint32_t candidates, spatial, radius, pairwise, history;
+  R limit;
+  const float* cam;
+  uint32_t proj, W, H, frame, pass;
+  const uint32_t* res_in;
+  const float* surf_in;
+  uint32_t* res_out;
+  float* surf_out;
+} Restir;
+
+static int restir_reproject(const Restir* rc, V p, int* px, int* py) {
+  const float* c = rc->cam;
+  V off = vsub(p, v3(c[0], c[1], c[2]));
+  V fw = v3(c[3], c[4], c[5]), rt = v3(c[6], c[7], c[8]), up = v3(c[9], c[10], c[11]);
+  const R depth = dot(off, fw), scale = len(up), aspect = (R)rc->W / (R)rc->H;
+  if (depth <= (R)0.0001 || scale <= (R)0.0001) return 0;
+  const R ps = rc->proj == 1u ? (R)1 : depth;
+  const R nx = dot(off, norm(rt)) / ((ps * aspect) * scale);
+  const R ny = (-dot(off, norm(up))) / (ps * scale);
+  R fx = ((nx * (R)0.5 + (R)0.5) * (R)rc->W) - (R)0.5;
+  R fy = ((ny * (R)0.5 + (R)0.5) * (R)rc->H) - (R)0.5;
+  fx = sizeof(R) == 4 ? (R)rintf((float)fx) : (R)rint((double)fx);
+  fy = sizeof(R) == 4 ? (R)rintf((float)fy) : (R)rint((double)fy);
+  if (!(fx >= (R)0 && fy >= (R)0 && fx < (R)rc->W && fy < (R)rc->H)) return 0;
+  *px = (int)fx;
+  *py = (int)fy;
+  return 1;
+}
+static V restir_previous_position(const Restir* rc, int px, int py, R dist) {
+  const float* c = rc->cam;
+  const R ndx = (((R)px + (R)0.5) / (R)rc->W) * (R)2 - (R)1;
+  const R ndy = (((R)py + (R)0.5) / (R)rc->H) * (R)2 - (R)1;
+  const R aspect = (R)rc->W / (R)rc->H;
+  V org = v3(c[0], c[1], c[2]), fw = v3(c[3], c[4], c[5]), rt = v3(c[6], c[7], c[8]), up = v3(c[9], c[10], c[11]);
+  if (rc->proj == 1u) return vadd(vsub(vadd(org, vs(rt, ndx * aspect)), vs(up, ndy)), vs(norm(fw), dist));
+  return vadd(org, vs(norm(vsub(vadd(fw, vs(rt, ndx * aspect)), vs(up, ndy))), dist));
+}
+static int restir_surface_matches(const Restir* rc, int hx, int hy, V p, V n, R dist, uint32_t mat_id, int center) {
+  const float* sf = rc->surf_in + 8ull * ((uint64_t)hy * rc->W + (uint64_t)hx);
+  uint32_t old_mat;
+  memcpy(&old_mat, sf + 4, 4);
+  V delta = vsub(restir_previous_position(rc, hx, hy, (R)sf[3]), p);
+  const R tol = rmax((R)0.03, dist * (R)0.01);
+  const R err = center ? len(delta) : absr(dot(delta, norm(n)));
+  const R agree = dot(norm(v3(sf[0], sf[1], sf[2])), norm(n));
+  return sf[3] >= 0.0f && err <= tol && agree > (R)0.9 && old_mat == mat_id;
+}
+static void restir_clear_pixel(const Restir* rc, uint32_t pixel) {
+  Rsv e = rsv_empty();
+  rsv_store(&e, rc->res_out + 4ull * pixel);
+  float* sf = rc->surf_out + 8ull * pixel;
+  const uint32_t none = OL_NONE_ID;
+  sf[0] = sf[1] = sf[2] = 0.0f;
+  sf[3] = -1.0f;
+  memcpy(sf + 4, &none, 4);
+  sf[5] = sf[6] = sf[7] = 0.0f;
+}
+
+/* Candidates, temporal and spatial merges of the previous pass's reservoirs,
+ * store; returns 1 with the selected point's shadow ray and its contribution
+ * times the reservoir normalisation (unoccluded) when a light was selected. */
+static int restir_primary(const Scene* s, const Restir* rc, uint32_t* rs, V hit, V n, const Mat* m, V inc, R dist,
+                          uint32_t pixel, uint32_t mat_id, V* c_out, V* dir_out, R* tmax_out) {
+  const uint32_t count = rc->candidates;
+  Rsv r = rsv_empty();
+  for (uint32_t k = 0; k < count; ++k) {
+    uint32_t light = OL_NONE_ID;
+    R b1 = 0, b2 = 0, target = 0;
+    if (s->L > 0) {
+      *rs = pcg(*rs);
+      const R sel = ((R)k + pcg_unit(*rs)) / (R)count;
+      uint32_t lo = 0, hi = s->L - 1;
+      for (int st = 0; st < 32 && lo < hi; ++st) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (sel <= (R)s->area[20ull * mid + 16]) hi = mid; else lo = mid + 1;
+      }
+      *rs = pcg(*rs);
+      const R ru = rsqrt_(pcg_unit(*rs));
+      *rs = pcg(*rs);
+      const R vv = pcg_unit(*rs);
+      light = lo;
+      b1 = ru * ((R)1 - vv);
+      b2 = ru * vv;
+      V d;
+      R dd;
+      target = cand_target(cand_contribution(s, light, b1, b2, hit, n, inc, m, count, &d, &dd));
+    }
+    *rs = pcg(*rs);
+    rsv_update(&r, light, b1, b2, target, target, 1, pcg_unit(*rs));
+  }
+  int px, py;
+  if (rc->history && restir_reproject(rc, hit, &px, &py)) {
+    const uint32_t rot = hash32(pixel ^ (rc->frame + rc->pass)) & 7u;
+    static const int ox[8] = {1, -1, 0, 0, 1, -1, 1, -1}, oy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+    for (uint32_t ri = 0; ri <= rc->spatial; ++ri) {
+      int hx = px, hy = py;
+      if (ri > 0u) {
+        const uint32_t idx = ((ri - 1u) + rot) & 7u;
+        hx += ox[idx] * (int)rc->radius;
+        hy += oy[idx] * (int)rc->radius;
+      }
+      const int inside = hx >= 0 && hy >= 0 && hx < (int)rc->W && hy < (int)rc->H;
+      Rsv h = rsv_empty();
+      int present = 0, valid = 0;
+      if (inside) {
+        const uint64_t hidx = (uint64_t)hy * rc->W + (uint64_t)hx;
+        h = rsv_load(rc->res_in + 4ull * hidx);
+        present = h.light != OL_NONE_ID;
+        if (present) valid = restir_surface_matches(rc, hx, hy, hit, n, dist, mat_id, ri == 0u);
+      }
+      if (valid) {
+        const R remaining = rmax(rc->limit - h_hi(r.tc), (R)0);
+        h = rsv_limit(h, rc->spatial > 0u ? rmin(remaining, (R)1) : remaining);
+        if (h_hi(h.tc) <= (R)0) h = rsv_empty();
+      } else {
+        h = rsv_empty();
+      }
+      if (h.light != OL_NONE_ID) {
+        V d;
+        R dd;
+        const R ht = cand_target(cand_contribution(s, h.light, h_lo(h.bary), h_hi(h.bary), hit, n, inc, m, count, &d, &dd));
+        *rs = pcg(*rs);
+        const R u = pcg_unit(*rs);
+        if (ri == 0u) rsv_merge(&r, &h, ht, u);
+        else if (rc->pairwise) rsv_merge_pairwise(&r, &h, ht, h_lo(h.tc), u);
+        else rsv_merge_canonical(&r, &h, ht, u);
+      } else if (ri == 0u && !present) {
+        break;
+      }
+    }
+  }
+  rsv_store(&r, rc->res_out + 4ull * pixel);
+  float* sf = rc->surf_out + 8ull * pixel;
+  sf[0] = (float)n.x;
+  sf[1] = (float)n.y;
+  sf[2] = (float)n.z;
+  sf[3] = (float)dist;
+  memcpy(sf + 4, &mat_id, 4);
+  sf[5] = sf[6] = sf[7] = 0.0f;
+  if (r.light == OL_NONE_ID) return 0;
+  V d;
+  R dd;
+  V c = cand_contribution(s, r.light, h_lo(r.bary), h_hi(r.bary), hit, n, inc, m, count, &d, &dd);
+  if (dd <= (R)0) return 0;
+  *c_out = vs(c, rsv_normalization(&r));
+  *dir_out = d;
+  *tmax_out = rmax(dd - (R)0.004, (R)0.001);
+  return 1;
+}
+
 static R medium_get(const R st[16], uint32_t i) { return st[i]; }
 
 /* One path; returns radiance (rgb) as floats of R.  counts[3] += rays. */
-static void trace_path(const Scene* s, const Settings* st, uint32_t pixel, uint32_t frame, uint32_t sample, V org,
-                       V dir, R out[3], uint64_t counts[3]) {
+static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uint32_t pixel, uint32_t frame,
+                       uint32_t sample, V org, V dir, R out[3], uint64_t counts[3]) {
   V T = v3(1, 1, 1), Lr = v3(0, 0, 0);
   R prev_pdf = 0;
   uint32_t flags = 257u, bounce = 0;
@@ -646,6 +1006,8 @@ static void trace_path(const Scene* s, const Settings* st, uint32_t pixel, uint3
   R tmin = (R)0.001, tmax = (R)1e30;
   const uint32_t frame_sample = (frame << 8) | (sample & 255u);
   counts[0]++;
+  /* This pass's history for the pixel: empty unless ReSTIR runs below. */
+  if (rc) restir_clear_pixel(rc, pixel);
   for (uint32_t iter = 0; iter < st->max_bounces; ++iter) {
     R t = 0, bu = 0, bv = 0;
     uint32_t prim;
@@ -699,6 +1061,7 @@ static void trace_path(const Scene* s, const Settings* st, uint32_t pixel, uint3
     uint32_t nb = bounce + 1u;
     if (nb >= st->max_bounces) break;
     R trans = m.trans, bsdf_pdf = 0;
+    int used_restir = 0;
     V nd;
     if (trans > (R)0.001) {
       R cur = medium_get(stack, depth - 1u), target = 1;
@@ -742,7 +1105,21 @@ static void trace_path(const Scene* s, const Settings* st, uint32_t pixel, uint3
         for (uint32_t li = 0; li < 64u && li < s->P; ++li) {
           if (point_sample(s, li, hp, n, &m, inc, &c)) direct = vadd(direct, c);
         }
-        if (st->unified) {
+        if (rc && bounce == 0u) {
+          /* Primary hit: the ReSTIR area-light sample, then environment
+           * samples (wavefront_primary_impl.glsl:1527-1785). */
+          V rdir;
+          R rtmax;
+          used_restir = 1;
+          if (restir_primary(s, rc, &rs, hp, n, &m, inc, t, pixel, s->mid[prim], &c, &rdir, &rtmax) &&
+              !occluded(s, vadd(hp, vs(n, (R)0.002)), rdir, (R)0.001, rtmax))
+            direct = vadd(direct, c);
+          uint32_t ec = st->env_samples < 4u ? st->env_samples : 4u;
+          V ed = v3(0, 0, 0);
+          for (uint32_t k = 0; k < ec; ++k)
+            if (env_sample(s, &rs, hp, n, &m, inc, ec, 1, &c)) ed = vadd(ed, c);
+          if (ec > 0) direct = vadd(direct, vdiv(ed, (R)ec));
+        } else if (st->unified) {
           int area_on = s->L > 0, env_on = st->env_samples > 0;
           if (area_on || env_on) {
             R aw = 0, ew = 0;
@@ -805,7 +1182,7 @@ static void trace_path(const Scene* s, const Settings* st, uint32_t pixel, uint3
     flags = 1u | (depth << 8);
     if (trans <= (R)0.001) {
       flags |= 2u;
-      if (st->unified) flags |= 4u;
+      if (st->unified && !used_restir) flags |= 4u;
       prev_pdf = bsdf_pdf;
     }
     rng = rs;
@@ -868,7 +1245,7 @@ int API(olgi_render)(void* h, const float cam[12], uint32_t proj, uint32_t W, ui
         V o, d;
         camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, smp, &o, &d);
         R rad[3];
-        trace_path(s, &st, (uint32_t)py * W + px, frame, smp, o, d, rad, counts);
+        trace_path(s, &st, NULL, (uint32_t)py * W + px, frame, smp, o, d, rad, counts);
         /* float32 radiance per sample, summed in float64 (api.py:1404-1407) */
         for (int k = 0; k < 3; ++k) acc[k] += (double)(float)rad[k];
       }
@@ -933,7 +1310,7 @@ int API(olgi_render_products)(void* h, const float cam[12], uint32_t proj, uint3
           V o, d;
           camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, smp, &o, &d);
           R rad[3];
-          trace_path(s, &st, (uint32_t)pix, frame, smp, o, d, rad, counts);
+          trace_path(s, &st, NULL, (uint32_t)pix, frame, smp, o, d, rad, counts);
           const float y = ((float)rad[0] * 0.2126f + (float)rad[1] * 0.7152f) + (float)rad[2] * 0.0722f;
           sum += (double)y;
           sq += (double)y * (double)y;
@@ -946,6 +1323,76 @@ int API(olgi_render_products)(void* h, const float cam[12], uint32_t proj, uint3
         variance[pix] = (float)var;
       }
     }
+  }
+  return 0;
+}
+
+/* ReSTIR DI render: samples run as successive passes over the whole image,
+ * pass s reusing pass s - 1's reservoirs (double-buffered).  rset:
+ * candidates, spatial neighbours (0 = none), radius, pairwise (0/1),
+ * history limit.  Other arguments as olgi_render (whole image). */
+int API(olgi_render_restir)(void* h, const float cam[12], uint32_t proj, uint32_t W, uint32_t H, uint32_t spp,
+                            uint32_t frame, const uint32_t iset[4], const float fset[2], uint32_t unified,
+                            const uint32_t rset[5], float* out, uint64_t* counts_out) {
+  const Scene* s = (const Scene*)h;
+  Settings st = {iset[0], iset[1], iset[2], iset[3], fset[0], fset[1], unified};
+  const uint64_t np = (uint64_t)W * H;
+  uint32_t* res = (uint32_t*)calloc(8 * np, sizeof(uint32_t));
+  float* surf = (float*)calloc(16 * np, sizeof(float));
+  double* acc = (double*)calloc(3 * np, sizeof(double));
+  if (!res || !surf || !acc) {
+    free(res);
+    free(surf);
+    free(acc);
+    return -1;
+  }
+  uint64_t c0 = 0, c1 = 0, c2 = 0;
+  for (uint32_t pass = 0; pass < spp; ++pass) {
+    Restir rc;
+    rc.candidates = rset[0];
+    rc.spatial = rset[1];
+    rc.radius = rset[2];
+    rc.pairwise = rset[3];
+    rc.limit = (R)rset[4];
+    rc.history = pass > 0u;
+    rc.cam = cam;
+    rc.proj = proj;
+    rc.W = W;
+    rc.H = H;
+    rc.frame = frame;
+    rc.pass = pass;
+    rc.res_out = res + 4 * np * (pass & 1u);
+    rc.res_in = res + 4 * np * ((pass + 1u) & 1u);
+    rc.surf_out = surf + 8 * np * (pass & 1u);
+    rc.surf_in = surf + 8 * np * ((pass + 1u) & 1u);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : c0, c1, c2)
+    for (int64_t py = 0; py < (int64_t)H; ++py) {
+      uint64_t counts[3] = {0, 0, 0};
+      g_shadow_rays = 0;
+      for (uint32_t px = 0; px < W; ++px) {
+        V o, d;
+        camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, pass, &o, &d);
+        R rad[3];
+        const uint64_t pix = (uint64_t)py * W + px;
+        trace_path(s, &st, &rc, (uint32_t)pix, frame, pass, o, d, rad, counts);
+        for (int k = 0; k < 3; ++k) acc[3 * pix + k] += (double)(float)rad[k];
+      }
+      c0 += counts[0];
+      c1 += counts[1];
+      c2 += g_shadow_rays;
+    }
+  }
+  for (uint64_t i = 0; i < np; ++i) {
+    for (int k = 0; k < 3; ++k) out[4 * i + k] = (float)(acc[3 * i + k] / (double)spp);
+    out[4 * i + 3] = 1.0f;
+  }
+  free(res);
+  free(surf);
+  free(acc);
+  if (counts_out) {
+    counts_out[0] = c0;
+    counts_out[1] = c1;
+    counts_out[2] = c2;
   }
   return 0;
 }

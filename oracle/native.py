@@ -48,6 +48,22 @@ def lib():
             getattr(L, "olgi_render_products" + sfx).argtypes = [
                 ctypes.c_void_p, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                 ctypes.c_uint32, P_u32, P_f32, ctypes.c_uint32, P_f32, P_f32, P_u32, P_f32]
+            getattr(L, "olgi_render_restir" + sfx).argtypes = [
+                ctypes.c_void_p, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                ctypes.c_uint32, P_u32, P_f32, ctypes.c_uint32, P_u32, P_f32, P_u64]
+            getattr(L, "olgi_f2h" + sfx).restype = ctypes.c_uint16
+            getattr(L, "olgi_f2h" + sfx).argtypes = [ctypes.c_float]
+            getattr(L, "olgi_h2f" + sfx).restype = ctypes.c_float
+            getattr(L, "olgi_h2f" + sfx).argtypes = [ctypes.c_uint16]
+            getattr(L, "olgi_rsv_updates" + sfx).argtypes = [
+                ctypes.c_uint32, P_u32, P_f32, P_f32, P_f32, P_f32, P_f32, P_f32, P_u32,
+                ctypes.POINTER(ctypes.c_uint8)]
+            getattr(L, "olgi_rsv_merge" + sfx).argtypes = [
+                P_u32, P_u32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]
+            getattr(L, "olgi_rsv_normalization" + sfx).restype = ctypes.c_float
+            getattr(L, "olgi_rsv_normalization" + sfx).argtypes = [P_u32]
+            getattr(L, "olgi_rsv_store" + sfx).argtypes = [P_u32, P_u32]
+            getattr(L, "olgi_rsv_load" + sfx).argtypes = [P_u32, P_u32]
             getattr(L, "olgi_wave_hash" + sfx).restype = ctypes.c_uint32
             getattr(L, "olgi_wave_hash" + sfx).argtypes = [ctypes.c_uint32]
             getattr(L, "olgi_pcg_step" + sfx).restype = ctypes.c_uint32
@@ -111,6 +127,28 @@ class OracleScene:
         self.last_counts = counts
         return out
 
+    def render_restir(self, camera_basis, projection, width, height, samples, frame_index, *, candidates,
+                      spatial_neighbors=0, spatial_radius=4, pairwise=True, history_limit=20, max_bounces=5,
+                      secondary_area_samples=1, environment_samples=1, roulette_start=0, roulette_min_survival=0.1,
+                      nee_probability=1.0, unified=True):
+        """Float32 (H, W, 4) image of the GI estimator with ReSTIR DI at
+        primary hits; samples run as successive passes over the image."""
+        cam = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float32) for v in camera_basis]), np.float32)
+        frame = int(np.float32(frame_index) + np.float32(0.5))
+        iset = np.asarray((max_bounces, secondary_area_samples, environment_samples, roulette_start), np.uint32)
+        fset = np.asarray((roulette_min_survival, nee_probability), np.float32)
+        rset = np.asarray((candidates, spatial_neighbors, spatial_radius, int(bool(pairwise)), history_limit),
+                          np.uint32)
+        out = np.zeros((height, width, 4), np.float32)
+        counts = np.zeros(3, np.uint64)
+        rc = getattr(lib(), "olgi_render_restir" + self.sfx)(
+            self.handle, _p(cam), projection, width, height, samples, frame, _p(iset, ctypes.c_uint32),
+            _p(fset), int(bool(unified)), _p(rset, ctypes.c_uint32), _p(out), _p(counts, ctypes.c_uint64))
+        if rc != 0:
+            raise MemoryError("oracle ReSTIR buffers")
+        self.last_counts = counts
+        return out
+
     def products(self, camera_basis, projection, width, height, samples, frame_index, *, max_bounces=5,
                  secondary_area_samples=1, environment_samples=1, roulette_start=0, roulette_min_survival=0.1,
                  nee_probability=1.0, unified=True):
@@ -162,6 +200,83 @@ def lbvh_build(positions, indices, leaf_size=4):
     n = int(count[0])
     return {"morton": morton, "sorted": order, "left": left[:T - 1], "right": right[:T - 1],
             "box": box[:T - 1], "nodes": nodes[:n], "tri": tri, "depth": int(depth[0])}
+
+
+def float_to_half(x):
+    """IEEE binary16 bits of float32 x, round to nearest even (packHalf2x16)."""
+    return int(lib().olgi_f2h_f32(ctypes.c_float(x)))
+
+
+def half_to_float(h):
+    return float(lib().olgi_h2f_f32(int(h)))
+
+
+class Reservoir:
+    """A ReSTIR DI reservoir in the oracle's register form."""
+
+    def __init__(self, words=None):
+        self.words = np.array(words if words is not None else (0xFFFFFFFF, 0, 0, 0), np.uint32)
+
+    @property
+    def light(self):
+        return int(self.words[0])
+
+    @property
+    def w_sum(self):
+        return float(self.words[2:3].view(np.float32)[0])
+
+    @property
+    def target(self):
+        return half_to_float(int(self.words[3]) & 0xFFFF)
+
+    @property
+    def count(self):
+        return half_to_float(int(self.words[3]) >> 16)
+
+    @property
+    def barycentrics(self):
+        return half_to_float(int(self.words[1]) & 0xFFFF), half_to_float(int(self.words[1]) >> 16)
+
+    def normalization(self):
+        return float(lib().olgi_rsv_normalization_f32(_p(self.words, ctypes.c_uint32)))
+
+    def merge(self, src, target_here, random, *, mode="temporal", target_there=0.0):
+        code = {"temporal": 0, "pairwise": 1, "canonical": 2}[mode]
+        lib().olgi_rsv_merge_f32(_p(self.words, ctypes.c_uint32), _p(src.words, ctypes.c_uint32),
+                                 target_here, target_there, random, code)
+        return self
+
+    def limited(self, maximum):
+        out = Reservoir()
+        lib().olgi_rsv_merge_f32(_p(out.words, ctypes.c_uint32), _p(self.words, ctypes.c_uint32),
+                                 maximum, 0.0, 0.0, 3)
+        return out
+
+    def stored(self):
+        out = np.zeros(4, np.uint32)
+        lib().olgi_rsv_store_f32(_p(self.words, ctypes.c_uint32), _p(out, ctypes.c_uint32))
+        return out
+
+    @staticmethod
+    def loaded(stored):
+        out = Reservoir()
+        stored = np.ascontiguousarray(stored, np.uint32)
+        lib().olgi_rsv_load_f32(_p(stored, ctypes.c_uint32), _p(out.words, ctypes.c_uint32))
+        return out
+
+
+def reservoir_updates(lights, targets, weights, randoms, *, b1=None, b2=None, represented=None):
+    """Streams candidates into an empty reservoir; returns (Reservoir, selected flags)."""
+    n = len(lights)
+    f = lambda a, d: np.ascontiguousarray(np.full(n, d, np.float32) if a is None else a, np.float32)
+    lights = np.ascontiguousarray(lights, np.uint32)
+    b1, b2, rep = f(b1, 0.25), f(b2, 0.5), f(represented, 1.0)
+    targets, weights, randoms = f(targets, 0), f(weights, 0), f(randoms, 0)
+    out = Reservoir()
+    sel = np.zeros(n, np.uint8)
+    lib().olgi_rsv_updates_f32(n, _p(lights, ctypes.c_uint32), _p(b1), _p(b2), _p(targets), _p(weights), _p(rep),
+                               _p(randoms), _p(out.words, ctypes.c_uint32), _p(sel, ctypes.c_uint8))
+    return out, sel.astype(bool)
 
 
 def wave_hash(v):

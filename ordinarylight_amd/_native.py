@@ -40,6 +40,20 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+class InstancedDesc(ctypes.Structure):
+    _fields_ = [
+        ("resource_count", c_u32), ("vertex_offsets", P_u32), ("triangle_offsets", P_u32),
+        ("positions", P_f32), ("normals", P_f32), ("indices", P_u32),
+        ("instance_count", c_u32), ("instance_resource", P_u32), ("instance_material", P_u32),
+        ("instance_triangle_offset", P_u32), ("instance_id", P_u32), ("world_to_object", P_f32),
+        ("normal_matrix", P_f32), ("determinant", P_f32), ("world_bounds", P_f32),
+        ("material_count", c_u32), ("materials", P_f32),
+        ("analytic_light_count", c_u32), ("analytic_lights", P_f32),
+        ("area_light_count", c_u32), ("area_lights", P_f32),
+        ("area_light_weight", c_f32), ("leaf_size", c_u32),
+    ]
+
+
 class SceneInfo(ctypes.Structure):
     _fields_ = [
         ("triangle_count", c_u32), ("node_count", c_u32), ("bvh_depth", c_u32),
@@ -60,6 +74,8 @@ class GiSettings(ctypes.Structure):
         ("environment_samples", c_u32), ("russian_roulette_start", c_u32),
         ("russian_roulette_min_survival", c_f32), ("secondary_nee_probability", c_f32),
         ("unified_secondary_nee", c_u32), ("path_capacity", c_u32),
+        ("restir_candidates", c_u32), ("restir_spatial_neighbors", c_u32), ("restir_spatial_radius", c_u32),
+        ("restir_history_limit", c_u32), ("restir_pairwise_mis", c_u32),
     ]
 
 
@@ -138,6 +154,8 @@ _SIGNATURES = {
     "olhip_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "olhip_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "olhip_scene_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "olhip_scene_create_instanced": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(InstancedDesc),
+                                                    ctypes.POINTER(ctypes.c_void_p)]),
     "olhip_scene_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "olhip_scene_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneInfo)]),
     "olhip_scene_bvh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -212,6 +212,166 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
   });
 }
 
+namespace {
+
+// Lights, materials and environment of either scene kind.
+void upload_shading(olhip_scene* sc, uint32_t material_count, const float* materials, uint32_t analytic_count,
+                    const float* analytic_lights, uint32_t area_count, const float* area_lights, float area_weight,
+                    hipStream_t s) {
+  require(material_count > 0 && materials, "at least one material is required");
+  require(analytic_count <= 64, "at most 64 analytic lights are supported");
+  require(analytic_count == 0 || analytic_lights, "analytic lights missing");
+  require(area_count == 0 || area_lights, "area lights missing");
+  sc->materials_host.assign(materials, materials + 48ull * material_count);
+  for (uint32_t m = 0; m < material_count; ++m)
+    if (materials[48ull * m + 11] > 0.001f) sc->has_transmission = true;
+  upload(sc->materials, reinterpret_cast<const float4*>(materials), 12ull * material_count, s);
+  std::vector<float4> analytic(4ull * std::max<uint32_t>(analytic_count, 1), make_float4(0, 0, 0, 0));
+  if (analytic_count) std::memcpy(analytic.data(), analytic_lights, sizeof(float) * 16ull * analytic_count);
+  upload(sc->analytic, analytic.data(), analytic.size(), s);
+  std::vector<float4> area(5ull * std::max<uint32_t>(area_count, 1), make_float4(0, 0, 0, 0));
+  if (area_count) std::memcpy(area.data(), area_lights, sizeof(float) * 20ull * area_count);
+  upload(sc->area, area.data(), area.size(), s);
+  DScene& v = sc->view;
+  v.materials = sc->materials.as<float4>();
+  v.analytic = sc->analytic.as<float4>();
+  v.area = sc->area.as<float4>();
+  v.material_count = material_count;
+  v.analytic_count = analytic_count;
+  v.area_count = area_count;
+  v.area_weight = area_weight;
+  v.env_valid = 0;
+  for (uint32_t l = 0; l < analytic_count; ++l) {
+    const float* L = analytic_lights + 16ull * l;
+    if ((int)(L[3] + 0.5f) == 3) {
+      require(L[12] < 0.0f, "image environments are not supported by the GI estimator");
+      v.env_valid = 1;
+      v.env_r = (1.0f * L[8]) * L[11];
+      v.env_g = (1.0f * L[9]) * L[11];
+      v.env_b = (1.0f * L[10]) * L[11];
+      break;
+    }
+  }
+  sc->point_count = analytic_count;
+}
+
+}  // namespace
+
+int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, olhip_scene** out) {
+  return guarded([&] {
+    require(ctx && d && out, "ctx, desc and out must not be NULL");
+    require(d->resource_count >= 1 && d->vertex_offsets && d->triangle_offsets, "resources missing");
+    require(d->instance_count >= 1 && d->instance_resource && d->instance_material && d->instance_triangle_offset &&
+                d->instance_id && d->world_to_object && d->normal_matrix && d->determinant && d->world_bounds,
+            "instance arrays missing");
+    require(d->leaf_size <= 16, "leaf_size must be 0..16");
+    const uint32_t R = d->resource_count, V = d->vertex_offsets[R], T = d->triangle_offsets[R];
+    require(V == 0 || (d->positions && d->normals), "geometry arrays missing");
+    require(T == 0 || d->indices, "indices missing");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    auto sc = std::make_unique<olhip_scene>();
+    sc->ctx = ctx;
+    sc->vertex_count = V;
+    std::vector<float4> pos(V), nrm(V);
+    for (uint32_t i = 0; i < V; ++i) {
+      pos[i] = make_float4(d->positions[3 * i], d->positions[3 * i + 1], d->positions[3 * i + 2], 0.0f);
+      nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
+      require(std::isfinite(pos[i].x) && std::isfinite(pos[i].y) && std::isfinite(pos[i].z),
+              "vertex positions must be finite");
+    }
+    std::vector<uint4> vidx(T);
+    for (uint32_t r = 0; r < R; ++r) {
+      require(d->vertex_offsets[r] <= d->vertex_offsets[r + 1] && d->triangle_offsets[r] <= d->triangle_offsets[r + 1],
+              "resource offsets must be non-decreasing");
+      const uint32_t nv = d->vertex_offsets[r + 1] - d->vertex_offsets[r];
+      for (uint32_t t = d->triangle_offsets[r]; t < d->triangle_offsets[r + 1]; ++t) {
+        const uint32_t a = d->indices[3 * t], b = d->indices[3 * t + 1], c = d->indices[3 * t + 2];
+        require(a < nv && b < nv && c < nv, "triangle index out of range of its resource");
+        vidx[t] = make_uint4(a, b, c, 0u);
+      }
+    }
+    upload(sc->positions, pos.data(), pos.size(), s);
+    upload(sc->normals, nrm.data(), nrm.size(), s);
+    upload(sc->vidx, vidx.data(), vidx.size(), s);
+    upload_shading(sc.get(), d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
+                   d->area_light_count, d->area_lights, d->area_light_weight, s);
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    // Bottom level: one BVH per resource, object space.
+    std::vector<BlasDev> blas(R);
+    uint32_t blas_depth = 0;
+    for (uint32_t r = 0; r < R; ++r) {
+      auto out_r = std::make_unique<BvhOutput>();
+      const uint32_t vo = d->vertex_offsets[r], to = d->triangle_offsets[r];
+      const uint32_t nt = d->triangle_offsets[r + 1] - to;
+      if (nt) {
+        BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, d->leaf_size};
+        bvh_build(in, *out_r, s, nullptr);
+      }
+      blas[r] = BlasDev{out_r->nodes.as<WideNode>(), out_r->tri.as<float4>(), sc->vidx.as<uint4>() + to,
+                        sc->normals.as<float4>() + vo, out_r->node_count, nt, 0u, 0u};
+      blas_depth = std::max(blas_depth, out_r->depth);
+      sc->blas.push_back(std::move(out_r));
+    }
+    // Instances and the top level over their world bounds (a box is the
+    // bounds of the degenerate triangle (min, max, min)).
+    const uint32_t I = d->instance_count;
+    std::vector<InstanceDev> inst(I);
+    std::vector<float4> tpos(2ull * I);
+    std::vector<uint4> tvidx(I);
+    uint64_t tri_total = 0;
+    for (uint32_t k = 0; k < I; ++k) {
+      require(d->instance_resource[k] < R, "instance resource out of range");
+      require(d->instance_material[k] < d->material_count, "instance material out of range");
+      InstanceDev& x = inst[k];
+      for (int row = 0; row < 3; ++row) {
+        const float* w = d->world_to_object + 12ull * k + 4 * row;
+        const float* n = d->normal_matrix + 9ull * k + 3 * row;
+        x.w2o[row] = make_float4(w[0], w[1], w[2], w[3]);
+        x.nrm[row] = make_float4(n[0], n[1], n[2], 0.0f);
+      }
+      x.nrm[0].w = d->determinant[k];
+      x.blas = d->instance_resource[k];
+      x.material = d->instance_material[k];
+      x.tri_offset = d->instance_triangle_offset[k];
+      x.id = d->instance_id[k];
+      const float* b = d->world_bounds + 6ull * k;
+      for (int a = 0; a < 6; ++a) require(std::isfinite(b[a]), "instance bounds must be finite");
+      tpos[2 * k] = make_float4(b[0], b[1], b[2], 0.0f);
+      tpos[2 * k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
+      tvidx[k] = make_uint4(2 * k, 2 * k + 1, 2 * k, 0u);
+      tri_total += d->triangle_offsets[x.blas + 1] - d->triangle_offsets[x.blas];
+    }
+    upload(sc->tlas_pos, tpos.data(), tpos.size(), s);
+    upload(sc->tlas_vidx, tvidx.data(), tvidx.size(), s);
+    upload(sc->inst_table, inst.data(), inst.size(), s);
+    upload(sc->blas_table, blas.data(), blas.size(), s);
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u};
+    bvh_build(tin, sc->bvh, s, nullptr);
+    sc->tlas_depth = sc->bvh.depth;
+    sc->blas_depth = blas_depth;
+    // Stack sizing: both levels plus the sentinel.
+    sc->bvh.depth = sc->tlas_depth + blas_depth + 1u;
+    sc->bvh.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    sc->instance_count = I;
+    sc->tri_count = (uint32_t)std::min<uint64_t>(tri_total, 0xFFFFFFFFull);
+    DScene& v = sc->view;
+    v.nodes = sc->bvh.nodes.as<WideNode>();
+    v.tri = sc->bvh.tri.as<float4>();
+    v.tri_vidx = nullptr;
+    v.normals = nullptr;
+    v.tri_count = sc->tri_count;
+    v.node_count = sc->bvh.node_count;
+    v.instanced = 1u;
+    v.instance_count = I;
+    v.inst = sc->inst_table.as<InstanceDev>();
+    v.blas = sc->blas_table.as<BlasDev>();
+    *out = sc.release();
+  });
+}
+
 int olhip_scene_destroy(olhip_scene* scene) {
   return guarded([&] {
     if (!scene) return;
@@ -533,6 +693,7 @@ int olhip_e0_begin(olhip_ctx* ctx, const olhip_scene* sc, const olhip_e0_camera*
     require(cam->projection <= 2, "unknown projection");
     require(env->mode <= 2, "unknown environment mode");
     require(env->mode != 2 || (env->image && env->width && env->height), "environment image missing");
+    require(!sc->view.instanced, "the reference estimator takes flat scenes");
     OL_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     auto& f = ctx->e0;

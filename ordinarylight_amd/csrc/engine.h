@@ -30,7 +30,11 @@
 struct olhip_scene {
   olhip_ctx* ctx = nullptr;
   DevBuf positions, normals, vidx, materials, analytic, area;
-  BvhOutput bvh;
+  BvhOutput bvh;  // the scene's BVH, or the top-level BVH over instances
+  // Two-level scenes: bottom-level BVHs per resource, instance and BLAS tables.
+  std::vector<std::unique_ptr<BvhOutput>> blas;
+  DevBuf tlas_pos, tlas_vidx, inst_table, blas_table;
+  uint32_t instance_count = 0, tlas_depth = 0, blas_depth = 0;
   DScene view{};
   uint32_t tri_count = 0, vertex_count = 0;
   bool has_transmission = false;
@@ -105,7 +109,7 @@ class Engine {
     uint32_t steps_in_chunk = 0;
     uint32_t passed = 0;        // admission rounds in which this job did not fit
     int status = OLHIP_JOB_QUEUED;
-    DevBuf acc, image, geo, prim, lum, variance;
+    DevBuf acc, image, geo, prim, lum, variance, res, surf;
     olhip_job_stats stats{};
     std::string error;
   };
@@ -135,7 +139,7 @@ class Engine {
   bool stk_needed_ = false;
 
   // Device state.
-  DevBuf T_, L_, M_, stk_, qo_[2], qd_[2], qp_[2], hit_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
+  DevBuf T_, L_, M_, stk_, qo_[2], qd_[2], qp_[2], hit_, hit_inst_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
       list_dev_, spill_;
   uint64_t slots_ = 0, eslots_ = 0;
   Wave wave_{};

@@ -68,9 +68,14 @@ uint32_t seg_for(uint64_t n, uint32_t stripes) {
 
 uint32_t job_entries_per_hit(const olhip_scene* sc, const olhip_gi_settings& s) {
   const bool two_pass = sc->point_count > (sc->view.env_valid ? 1u : 0u) || !s.unified_secondary_nee;
-  if (!two_pass) return 1;
-  return sc->point_count +
-         (s.unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(s.secondary_area_samples, 1u), 16u) + 4u));
+  uint32_t n = 1;
+  if (two_pass)
+    n = sc->point_count +
+        (s.unified_secondary_nee ? 1u : (std::min<uint32_t>(std::max<uint32_t>(s.secondary_area_samples, 1u), 16u) + 4u));
+  // ReSTIR primary hits: point lights, the reservoir's sample and up to four
+  // environment samples.
+  if (s.restir_candidates) n = std::max(n, sc->point_count + 1u + std::min<uint32_t>(s.environment_samples, 4u));
+  return n;
 }
 
 namespace {
@@ -151,6 +156,7 @@ void Engine::ensure_buffers() {
     qp_[q].reserve(4 * slots_);
   }
   hit_.reserve(16 * slots_);
+  hit_inst_.reserve(4 * slots_);
   gT_.reserve(16 * slots_);
   gI_.reserve(16 * slots_);
   eo_.reserve(16 * eslots_);
@@ -170,6 +176,7 @@ void Engine::ensure_buffers() {
     w.qp[q] = qp_[q].as<uint32_t>();
   }
   w.hit = hit_.as<float4>();
+  w.hit_inst = hit_inst_.as<uint32_t>();
   w.gT = gT_.as<float4>();
   w.gI = gI_.as<uint4>();
   w.eo = eo_.as<float4>();
@@ -223,6 +230,15 @@ uint64_t Engine::submit(const olhip_job_desc& d) {
               "russian_roulette_min_survival must be in [0, 1]");
   require_arg(s.environment_samples <= 4, "environment_samples must be between 0 and 4");
   require_arg((d.products & ~(OLHIP_PRODUCT_PRIMARY | OLHIP_PRODUCT_VARIANCE)) == 0, "unknown product bits");
+  if (s.restir_candidates) {
+    require_arg(s.restir_candidates <= 64, "restir_candidates must be at most 64");
+    require_arg(s.restir_spatial_neighbors <= 8, "restir_spatial_neighbors must be at most 8");
+    require_arg(s.restir_spatial_neighbors == 0 || (s.restir_spatial_radius >= 1 && s.restir_spatial_radius <= 32),
+                "restir_spatial_radius must be between 1 and 32");
+    require_arg(s.restir_history_limit >= 1 && s.restir_history_limit <= 64,
+                "restir_history_limit must be between 1 and 64");
+    require_arg(s.restir_pairwise_mis <= 1, "restir_pairwise_mis must be 0 or 1");
+  }
   auto j = std::make_unique<Job>();
   j->desc = d;
   j->per_hit = job_entries_per_hit(d.scene, s);
@@ -252,6 +268,8 @@ bool Engine::admit(Job& j, int slot) {
   const uint64_t np = (uint64_t)d.width * d.height;
   uint64_t spp = d.samples - j.next_sample;
   if (d.settings.path_capacity) spp = std::min<uint64_t>(spp, std::max<uint64_t>(1, d.settings.path_capacity / np));
+  // ReSTIR: one sample per pass, each reusing the previous pass's reservoirs.
+  if (d.settings.restir_candidates) spp = std::min<uint64_t>(spp, 1);
   const uint64_t free_p = paths_.largest_free(), free_e = entries_.largest_free();
   while (spp >= 1) {
     const uint64_t n = spp * np;
@@ -317,6 +335,23 @@ bool Engine::admit(Job& j, int slot) {
           j.variance.reserve(sizeof(float) * np);
         }
         j.stats.admit_ms = now_ms();
+      }
+      if (d.settings.restir_candidates) {
+        if (j.first_chunk) {
+          j.res.reserve(sizeof(uint4) * 2 * np);
+          j.surf.reserve(sizeof(float4) * 4 * np);
+        }
+        const uint32_t pass = j.chunk_base_sample;
+        P.restir = d.settings.restir_candidates;
+        P.restir_spatial = d.settings.restir_spatial_neighbors;
+        P.restir_radius = d.settings.restir_spatial_radius;
+        P.restir_pairwise = d.settings.restir_pairwise_mis;
+        P.restir_limit = (float)d.settings.restir_history_limit;
+        P.restir_history = pass > 0 ? 1u : 0u;
+        J.res_out = j.res.as<uint4>() + (pass & 1u) * np;
+        J.res_in = j.res.as<uint4>() + ((pass + 1u) & 1u) * np;
+        J.surf_out = j.surf.as<float4>() + (pass & 1u) * 2 * np;
+        J.surf_in = j.surf.as<float4>() + ((pass + 1u) & 1u) * 2 * np;
       }
       J.geo = (d.products & OLHIP_PRODUCT_PRIMARY) ? j.geo.as<float4>() : nullptr;
       J.prim = (d.products & OLHIP_PRODUCT_PRIMARY) ? j.prim.as<uint32_t>() : nullptr;

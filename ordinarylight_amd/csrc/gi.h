@@ -29,6 +29,11 @@ struct GiParams {
   float nee_probability;
   uint32_t unified;
   uint32_t two_pass;          // NEE may emit more than one shadow ray
+  // ReSTIR DI at primary hits: candidates (0 = off), spatial neighbours, radius (pixels),
+  // pairwise spatial merges, whether the previous pass's reservoirs exist,
+  // and the represented-sample limit.
+  uint32_t restir, restir_spatial, restir_radius, restir_pairwise, restir_history;
+  float restir_limit;
 };
 
 // ---- batching limits and queue layout ------------------------------------------
@@ -46,11 +51,18 @@ constexpr uint32_t kWorkStripes = 32;  // work cursors of the refilling traversa
 
 // Counter block (uint32 words).
 constexpr uint32_t C_Q0 = 0, C_Q1 = kRuns, C_G = 2 * kRuns, C_E = 3 * kRuns;
-constexpr uint32_t C_RP = 4 * kRuns;             // kRuns + 1: chunk prefix of the input ray queue
-constexpr uint32_t C_GP = C_RP + kRuns + 1;      // kRuns + 1: chunk prefix of the shadow groups
-constexpr uint32_t C_W = C_GP + kRuns + 1;       // kWorkStripes closest-hit cursors
-constexpr uint32_t C_WS = C_W + kWorkStripes;    // kWorkStripes shadow cursors
-constexpr uint32_t C_OVERFLOW = C_WS + kWorkStripes, C_STACK = C_OVERFLOW + 1;
+// Chunk prefixes are kept separately for jobs of flat scenes (C_RP, C_GP)
+// and of two-level scenes (C_RP2, C_GP2), so each traversal kernel variant
+// sees one kind only.
+constexpr uint32_t C_RP = 4 * kRuns;             // kRuns + 1: chunk prefix of the input ray queue (flat)
+constexpr uint32_t C_GP = C_RP + kRuns + 1;      // kRuns + 1: chunk prefix of the shadow groups (flat)
+constexpr uint32_t C_RP2 = C_GP + kRuns + 1;     // the same for two-level scenes
+constexpr uint32_t C_GP2 = C_RP2 + kRuns + 1;
+constexpr uint32_t C_W = C_GP2 + kRuns + 1;      // kWorkStripes closest-hit cursors (flat)
+constexpr uint32_t C_WS = C_W + kWorkStripes;    // kWorkStripes shadow cursors (flat)
+constexpr uint32_t C_W2 = C_WS + kWorkStripes;   // closest-hit cursors (two-level)
+constexpr uint32_t C_WS2 = C_W2 + kWorkStripes;  // shadow cursors (two-level)
+constexpr uint32_t C_OVERFLOW = C_WS2 + kWorkStripes, C_STACK = C_OVERFLOW + 1;
 constexpr uint32_t C_REPORT = C_OVERFLOW + 8;    // kMaxJobs x 4: rays traced, shadow rays, live paths, shaded hits
 constexpr uint32_t C_WORDS = C_REPORT + 4 * kMaxJobs;
 OL_HD constexpr uint32_t c_queue(uint32_t q) { return q ? C_Q1 : C_Q0; }
@@ -70,6 +82,13 @@ struct JobDev {
   // OL_NONE) (wavefront_intersect.comp:317-356, targets/vulkan/core.py:4500-4536).
   float4* geo;
   uint32_t* prim;
+  // ReSTIR DI (null unless enabled): this pass's and the previous pass's
+  // stored reservoirs and primary surfaces (2 x float4 per pixel: position,
+  // distance; shading normal, material id bits).
+  uint4* res_out;
+  const uint4* res_in;
+  float4* surf_out;
+  const float4* surf_in;
 };
 
 // Accumulation of one finished sample chunk into its job's float64 sums.
@@ -107,6 +126,7 @@ struct Wave {
   float4* qd[2];
   uint32_t* qp[2];
   float4* hit;  // (t, u, v, leaf slot bits) per ray-queue slot
+  uint32_t* hit_inst;  // instance of the hit (two-level scenes) per ray-queue slot
   float4* gT;   // shadow group: throughput rgb, NEE selection probability
   uint4* gI;    // path, first entry, entry count, area count | env count << 8 | mode << 16
   float4* eo;   // shadow entry: origin, tmin
@@ -137,6 +157,7 @@ int stack_depth_for(uint32_t bvh_depth);
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
 void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
+void wave_trace_kernels(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
 void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st);
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);

@@ -27,6 +27,7 @@
 
 #include "gi.h"
 #include "shade.h"
+#include "restir.h"
 #include "trace.h"
 
 namespace {
@@ -36,12 +37,16 @@ using namespace olshade;
 constexpr uint32_t kRefillMin = 16;
 
 // ---- chunk plan ----------------------------------------------------------------------
-// Exclusive prefix of ceil(count / kChunk) over all runs; out[kRuns] = total.
-__global__ void __launch_bounds__(1024) k_plan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ out) {
-  static_assert(kRuns == 2 * 1024, "two runs per thread");
+// Exclusive prefix of ceil(count / kChunk) over the runs of jobs whose scene
+// is flat (kind 0) or two-level (kind 1); out[kRuns] = total.
+__global__ void __launch_bounds__(1024) k_plan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
+                                               const JobDev* __restrict__ jobs, uint32_t kind) {
+  static_assert(kRuns == 2 * 1024 && kJobStripes % 2 == 0, "two runs of one job per thread");
   __shared__ uint32_t wave_tot[16];
   const uint32_t t = threadIdx.x;
-  const uint32_t a = (counts[2 * t] + kChunk - 1u) / kChunk, b = (counts[2 * t + 1] + kChunk - 1u) / kChunk;
+  const bool mine = (jobs[(2 * t) / kJobStripes].P.sc.instanced != 0u) == (kind != 0u);
+  const uint32_t a = mine ? (counts[2 * t] + kChunk - 1u) / kChunk : 0u;
+  const uint32_t b = mine ? (counts[2 * t + 1] + kChunk - 1u) / kChunk : 0u;
   const uint32_t sum = a + b;
   uint32_t incl = sum;
   for (int off = 1; off < 64; off <<= 1) {
@@ -283,6 +288,7 @@ __global__ void __launch_bounds__(BLOCK) k_closest(Wave W, uint32_t in, uint32_t
   const uint32_t* prefix = W.ctr + C_RP;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
+  if (chunks == 0) return;
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
@@ -359,7 +365,7 @@ OL_DEV void shadow_resolve(const Wave& W, uint32_t g, f3 direct, f3 area_direct,
   const uint4 gI = W.gI[g];
   const uint32_t path = gI.x, area_n = gI.w & 255u, env_n = (gI.w >> 8) & 255u, mode = gI.w >> 16;
   if (mode != 0u) {
-    direct = direct + area_direct / (float)area_n;
+    if (area_n > 0u) direct = direct + area_direct / (float)area_n;
     if (env_n > 0u) direct = direct + env_direct / (float)env_n;
   }
   float4 L = W.L[path];
@@ -376,6 +382,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(Wave W, uint32_t* spill, uint3
   const uint32_t* prefix = W.ctr + C_GP;
   const uint32_t* counts = W.ctr + C_G;
   const uint32_t chunks = prefix[kRuns];
+  if (chunks == 0) return;
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
@@ -457,6 +464,186 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(Wave W, uint32_t* spill, uint3
   }
 }
 
+// Two-level pool: rays of one chunk, all of one job (hence one scene view).
+struct Pool2 {
+  uint32_t slot, n, job;
+  bool exhausted;
+};
+
+OL_DEV void pool2_refill(Pool2& p, const Wave& W, const uint32_t* prefix, const uint32_t* counts, uint32_t* cursor,
+                         uint32_t c_begin, uint32_t c_end) {
+  if (p.n != 0 || p.exhausted) return;
+  uint32_t c = 0;
+  if (lane_id() == 0) c = atomicAdd(cursor, 1u);
+  c = uniform(__shfl(c, 0, 64));
+  if (c >= c_end - c_begin) {
+    p.exhausted = true;
+    return;
+  }
+  const ChunkLoc k = locate_chunk(prefix, counts, W.jobs, c_begin + c);
+  p.slot = uniform(k.first);
+  p.n = uniform(k.count);
+  p.job = uniform(k.job);
+}
+
+// Closest hit in two-level scenes (top-level BVH over instances): the
+// scheduling of k_closest with the instance-entering inner step.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_closest2(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  const uint32_t* prefix = W.ctr + C_RP2;
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t chunks = prefix[kRuns];
+  if (chunks == 0) return;
+  const uint32_t ws = blockIdx.x % kWorkStripes;
+  const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
+  const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
+  uint32_t* cursor = &W.ctr[C_W2 + ws];
+  const float4* __restrict__ qo = W.qo[in];
+  const float4* __restrict__ qd = W.qd[in];
+  Pool2 pool{0u, 0u, 0u, false};
+  Closest2 q;
+  q.node = OL_NONE;
+  q.inst = OL_NONE;
+  uint32_t job = 0, slot = OL_NONE;
+  bool busy = false;
+  uint32_t nv = 0, nt = 0;
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= kRefillMin) {
+      pool2_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      const uint32_t take = min(n_idle, pool.n);
+      if (!busy) {
+        const uint32_t rank = lanes_below(idle);
+        if (rank < take) {
+          slot = pool.slot + rank;
+          job = pool.job;
+          const float4 o = qo[slot], d = qd[slot];
+          closest2_begin(q, mk3(o), mk3(d), o.w, d.w);
+          if (W.jobs[job].P.sc.node_count == 0) {
+            W.hit[slot] = make_float4(d.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
+          } else {
+            st.sp = 0;
+            busy = true;
+          }
+        }
+      }
+      pool.slot += take;
+      pool.n -= take;
+    }
+    const bool at_leaf = busy && closest2_at_leaf(q);
+    const uint64_t leafy = __ballot(at_leaf);
+    const uint64_t inner = __ballot(busy && !at_leaf);
+    if ((leafy | inner) == 0) {
+      if (pool.exhausted && pool.n == 0) break;
+      continue;
+    }
+    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+      if (at_leaf) closest2_leaf<LSTACK>(q, st, nt);
+    } else if (busy && !at_leaf) {
+      closest2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv);
+    }
+    if (busy && q.node == OL_NONE) {
+      W.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
+      W.hit_inst[slot] = q.best_inst;
+      busy = false;
+    }
+  }
+  if (W.work && (nv | nt)) {
+    atomicAdd(&W.work[0], (unsigned long long)nv);
+    atomicAdd(&W.work[1], (unsigned long long)nt);
+  }
+}
+
+// Shadow groups in two-level scenes.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_shadow2(Wave W, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  const uint32_t* prefix = W.ctr + C_GP2;
+  const uint32_t* counts = W.ctr + C_G;
+  const uint32_t chunks = prefix[kRuns];
+  if (chunks == 0) return;
+  const uint32_t ws = blockIdx.x % kWorkStripes;
+  const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
+  const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
+  uint32_t* cursor = &W.ctr[C_WS2 + ws];
+  Pool2 pool{0u, 0u, 0u, false};
+  Any2 q;
+  q.node = OL_NONE;
+  q.inst = OL_NONE;
+  q.occluded = false;
+  uint32_t job = 0, g = 0, e = 0, e_end = 0, mode = 0;
+  f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
+  bool busy = false;
+  uint32_t nv = 0, nt = 0;
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= kRefillMin) {
+      pool2_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      const uint32_t take = min(n_idle, pool.n);
+      if (!busy) {
+        const uint32_t rank = lanes_below(idle);
+        if (rank < take) {
+          g = pool.slot + rank;
+          job = pool.job;
+          const uint4 gI = W.gI[g];
+          e = gI.y;
+          e_end = gI.y + gI.z;
+          mode = gI.w >> 16;
+          direct = mk3(0.0f, 0.0f, 0.0f);
+          area_direct = direct;
+          env_direct = direct;
+          if (W.jobs[job].P.sc.node_count == 0) {
+            for (; e < e_end; ++e) shadow_add(mode, W.ec[e], direct, area_direct, env_direct);
+            shadow_resolve(W, g, direct, area_direct, env_direct);
+          } else if (e < e_end) {
+            const float4 o = W.eo[e], d = W.ed[e];
+            any2_begin(q, mk3(o), mk3(d), o.w, d.w);
+            st.sp = 0;
+            busy = true;
+          } else {
+            shadow_resolve(W, g, direct, area_direct, env_direct);
+          }
+        }
+      }
+      pool.slot += take;
+      pool.n -= take;
+    }
+    const bool at_leaf = busy && any2_at_leaf(q);
+    const uint64_t leafy = __ballot(at_leaf);
+    const uint64_t inner = __ballot(busy && !at_leaf);
+    if ((leafy | inner) == 0) {
+      if (pool.exhausted && pool.n == 0) break;
+      continue;
+    }
+    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+      if (at_leaf) any2_leaf<LSTACK>(q, st, nt);
+    } else if (busy && !at_leaf) {
+      any2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv);
+    }
+    if (busy && q.node == OL_NONE) {
+      if (!q.occluded) shadow_add(mode, W.ec[e], direct, area_direct, env_direct);
+      ++e;
+      if (e < e_end) {
+        const float4 o = W.eo[e], d = W.ed[e];
+        any2_begin(q, mk3(o), mk3(d), o.w, d.w);
+        st.sp = 0;
+      } else {
+        shadow_resolve(W, g, direct, area_direct, env_direct);
+        busy = false;
+      }
+    }
+  }
+  if (W.work && (nv | nt)) {
+    atomicAdd(&W.work[2], (unsigned long long)nv);
+    atomicAdd(&W.work[3], (unsigned long long)nt);
+  }
+}
+
 template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o4, const float4* d4, uint32_t n,
                                                       float4* hits, uint32_t* overflow, unsigned long long* work,
@@ -475,6 +662,189 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o
   }
 }
 
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_rays2(DScene sc, const float4* o4, const float4* d4, uint32_t n,
+                                                       float4* hits, uint32_t* overflow, unsigned long long* work,
+                                                       uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
+  uint32_t nv = 0, nt = 0;
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    const float4 o = o4[i], d = d4[i];
+    Closest2 q;
+    closest2_begin(q, mk3(o), mk3(d), o.w, d.w);
+    st.sp = 0;
+    if (sc.node_count == 0) q.node = OL_NONE;
+    while (q.node != OL_NONE) {
+      if (closest2_at_leaf(q)) closest2_leaf<LSTACK>(q, st, nt);
+      else closest2_inner<LSTACK>(sc, q, st, nv);
+    }
+    hits[i] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.orig));
+  }
+  if (work && (nv | nt)) {
+    atomicAdd(&work[0], (unsigned long long)nv);
+    atomicAdd(&work[1], (unsigned long long)nt);
+  }
+}
+
+// ---- ReSTIR DI at primary hits -----------------------------------------------------------------
+// Pixel of a world point through the job's camera (reprojectRestir,
+// wavefront_primary_impl.glsl:350-365, with the current camera as the
+// previous one: passes of one job share it).
+OL_DEV bool restir_reproject(const GiParams& P, f3 p, int& px, int& py) {
+  const f3 off = p - mk3(P.cam_o[0], P.cam_o[1], P.cam_o[2]);
+  const f3 fw = mk3(P.cam_f[0], P.cam_f[1], P.cam_f[2]);
+  const f3 rt = mk3(P.cam_r[0], P.cam_r[1], P.cam_r[2]);
+  const f3 up = mk3(P.cam_u[0], P.cam_u[1], P.cam_u[2]);
+  const float depth = dot3(off, fw);
+  const float scale = length3(up);
+  const float aspect = (float)P.width / (float)P.height;
+  if (depth <= 0.0001f || scale <= 0.0001f) return false;
+  const float ps = P.projection == 1u ? 1.0f : depth;
+  const float nx = dot3(off, normalize3(rt)) / ((ps * aspect) * scale);
+  const float ny = (-dot3(off, normalize3(up))) / (ps * scale);
+  const float fx = rintf(((nx * 0.5f + 0.5f) * (float)P.width) - 0.5f);
+  const float fy = rintf(((ny * 0.5f + 0.5f) * (float)P.height) - 0.5f);
+  if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)P.width && fy < (float)P.height)) return false;
+  px = (int)fx;
+  py = (int)fy;
+  return true;
+}
+
+// restirPreviousWorldPosition (:367-378): the point at `dist` along the
+// history pixel's centre ray.
+OL_DEV f3 restir_previous_position(const GiParams& P, int px, int py, float dist) {
+  const float ndx = (((float)px + 0.5f) / (float)P.width) * 2.0f - 1.0f;
+  const float ndy = (((float)py + 0.5f) / (float)P.height) * 2.0f - 1.0f;
+  const float aspect = (float)P.width / (float)P.height;
+  const f3 org = mk3(P.cam_o[0], P.cam_o[1], P.cam_o[2]);
+  const f3 fw = mk3(P.cam_f[0], P.cam_f[1], P.cam_f[2]);
+  const f3 rt = mk3(P.cam_r[0], P.cam_r[1], P.cam_r[2]);
+  const f3 up = mk3(P.cam_u[0], P.cam_u[1], P.cam_u[2]);
+  if (P.projection == 1u) return ((org + rt * (ndx * aspect)) - up * ndy) + normalize3(fw) * dist;
+  return org + normalize3((fw + rt * (ndx * aspect)) - up * ndy) * dist;
+}
+
+// restirHistorySurfaceCompatible (:382-393) against the stored surface of
+// the history pixel: (shading normal, primary distance or -1), material id.
+OL_DEV bool restir_surface_matches(const GiParams& P, const float4* s, int hx, int hy, f3 p, f3 n, float dist,
+                                   uint32_t mat_id, bool temporal_center) {
+  const float4 a = s[0];
+  const uint32_t old_mat = __float_as_uint(s[1].x);
+  const f3 delta = restir_previous_position(P, hx, hy, a.w) - p;
+  const float tol = fmaxf(0.03f, dist * 0.01f);
+  const float err = temporal_center ? length3(delta) : fabsf(dot3(delta, normalize3(n)));
+  const float agree = dot3(normalize3(rgb(a)), normalize3(n));
+  return a.w >= 0.0f && err <= tol && agree > 0.9f && old_mat == mat_id;
+}
+
+// restirSpatialOffset (:319-331)
+OL_DEV void restir_offset(uint32_t index, int radius, int& dx, int& dy) {
+  const int ox[8] = {1, -1, 0, 0, 1, -1, 1, -1}, oy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+  dx = ox[index & 7u] * radius;
+  dy = oy[index & 7u] * radius;
+}
+
+// Candidates, then merges of the previous pass's reservoirs at the
+// reprojected pixel (temporal) and its spatial neighbours (pairwise), as the
+// primary ReSTIR loop (:1539-1700); stores this pass's reservoir and returns
+// the selected light point as one shadow entry weighted by the reservoir's
+// normalisation.  Incompatible history surfaces are skipped.
+OL_DEV void restir_primary(const GiParams& P, const JobDev& J, uint32_t& rs, f3 hit, f3 n, const Mat& mat, f3 incoming,
+                           float dist, uint32_t pixel, uint32_t mat_id, NeeSample& out) {
+  const DScene& sc = P.sc;
+  const uint32_t count = P.restir;
+  Reservoir r = empty_reservoir();
+  for (uint32_t k = 0; k < count; ++k) {
+    uint32_t light;
+    float b1, b2, target;
+    candidate_generate(sc, rs, hit, n, incoming, mat, k, count, light, b1, b2, target);
+    rs = pcg_step(rs);
+    reservoir_update(r, light, b1, b2, target, target, 1.0f, pcg_value(rs));
+  }
+  int px, py;
+  if (P.restir_history && restir_reproject(P, hit, px, py)) {
+    const uint32_t spatial = P.restir_spatial;
+    // Pass s plays the role of the reference's frame frame_index + s.
+    const uint32_t rotation = wave_hash(pixel ^ (P.frame + P.sample_base)) & 7u;
+    for (uint32_t ri = 0; ri <= spatial; ++ri) {
+      int hx = px, hy = py;
+      if (ri > 0u) {
+        int dx, dy;
+        restir_offset((ri - 1u) + rotation, (int)P.restir_radius, dx, dy);
+        hx += dx;
+        hy += dy;
+      }
+      const bool inside = hx >= 0 && hy >= 0 && hx < (int)P.width && hy < (int)P.height;
+      Reservoir h = empty_reservoir();
+      bool present = false, valid = false;
+      if (inside) {
+        const uint32_t hidx = (uint32_t)hy * P.width + (uint32_t)hx;
+        h = reservoir_load(J.res_in[hidx]);
+        present = h.light != OL_NONE;
+        if (present) valid = restir_surface_matches(P, J.surf_in + 2u * hidx, hx, hy, hit, n, dist, mat_id, ri == 0u);
+      }
+      if (valid) {
+        const float remaining = fmaxf(P.restir_limit - half_hi(r.tc), 0.0f);
+        h = reservoir_limit(h, spatial > 0u ? fminf(remaining, 1.0f) : remaining);
+        if (half_hi(h.tc) <= 0.0f) h = empty_reservoir();
+      } else {
+        h = empty_reservoir();
+      }
+      if (h.light != OL_NONE) {
+        f3 dir;
+        float d;
+        const float ht = candidate_target(candidate_contribution(sc, h.light, half_lo(h.bary), half_hi(h.bary), hit, n,
+                                                                 incoming, mat, count, dir, d));
+        rs = pcg_step(rs);
+        const float u = pcg_value(rs);
+        if (ri == 0u) reservoir_merge(r, h, ht, u);
+        else if (P.restir_pairwise) reservoir_merge_pairwise(r, h, ht, half_lo(h.tc), u);
+        else reservoir_merge_canonical(r, h, ht, u);
+      } else if (ri == 0u && !present) {
+        break;
+      }
+    }
+  }
+  J.res_out[pixel] = reservoir_store(r);
+  J.surf_out[2u * pixel] = mk4(n, dist);
+  J.surf_out[2u * pixel + 1u] = make_float4(__uint_as_float(mat_id), 0.0f, 0.0f, 0.0f);
+  out.valid = false;
+  out.kind = 0u;
+  if (r.light == OL_NONE) return;
+  f3 dir;
+  float d;
+  const f3 c = candidate_contribution(sc, r.light, half_lo(r.bary), half_hi(r.bary), hit, n, incoming, mat, count, dir,
+                                      d);
+  if (d <= 0.0f) return;
+  out.origin = hit + n * 0.002f;
+  out.dir = dir;
+  out.tmax = fmaxf(d - 0.004f, 0.001f);
+  out.c = c * reservoir_normalization(r);
+  out.valid = true;
+}
+
+// Next-event samples of a ReSTIR primary hit in the reference's order: point
+// lights, the reservoir's light point (`chosen`, counted like a point light),
+// then min(environment_samples, 4) environment samples drawn from `rs`.
+template <class Emit>
+OL_DEV void restir_nee(const GiParams& P, const NeeSample& chosen, uint32_t& rs, f3 hit, f3 n, const Mat& m,
+                       f3 incoming, uint32_t& env_n, Emit&& emit) {
+  for (uint32_t li = 0; li < 64u; ++li) {
+    if (li >= P.point_count) break;
+    NeeSample s = point_sample(P.sc, li, hit, n, m, incoming);
+    if (s.valid) emit(s);
+  }
+  if (chosen.valid) emit(chosen);
+  const uint32_t ec = P.environment_samples < 4u ? P.environment_samples : 4u;
+  env_n = ec;
+  for (uint32_t k = 0; k < 4u; ++k) {
+    if (k >= ec) break;
+    NeeSample s = env_sample(P.sc, rs, hit, n, m, incoming, ec, 1.0f);
+    if (s.valid) emit(s);
+  }
+}
+
 // ---- shade -----------------------------------------------------------------------------------
 // One wave per 64-ray chunk (grid-stride over chunks), so the job and its
 // parameters are wave-uniform.  Outputs go to the job's segment
@@ -483,12 +853,14 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o
 // * 64 slots cannot overflow.
 __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
   const uint32_t out = in ^ 1u;
-  const uint32_t* prefix = W.ctr + C_RP;
   const uint32_t* counts = W.ctr + c_queue(in);
-  const uint32_t chunks = prefix[kRuns];
+  const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
+  const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
   for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
-    const ChunkLoc k = locate_chunk(prefix, counts, W.jobs, uniform(c));
+    const uint32_t cu = uniform(c);
+    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, counts, W.jobs, cu)
+                                        : locate_chunk(W.ctr + C_RP2, counts, W.jobs, cu - flat_chunks);
     const uint32_t job = uniform(k.job);
     const JobDev& J = W.jobs[job];
     const GiParams& P = J.P;
@@ -497,7 +869,7 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
     const uint32_t run = job * kJobStripes + so;
     const bool live = lane_id() < k.count;
     const uint32_t i = k.first + lane_id();
-    bool cont = false, nee = false;
+    bool cont = false, nee = false, used_restir = false;
     float4 co = make_float4(0, 0, 0, 0), cd = co;
     uint32_t path = 0, nee_count = 0, area_n = 0, env_n = 0, nee_rng = 0;
     f3 nee_T = mk3(0, 0, 0), hitp = nee_T, nrm = nee_T, incoming = nee_T;
@@ -510,6 +882,12 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
       const float4 ro = W.qo[in][i], rd = W.qd[in][i], hv = W.hit[i];
       float4 T = W.T[path], L = W.L[path];
       uint4 M = W.M[path];
+      if (P.restir && T.w == 0.0f) {
+        // This pass's history for the pixel: empty unless ReSTIR runs below.
+        J.res_out[M.x] = reservoir_store(empty_reservoir());
+        J.surf_out[2u * M.x] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        J.surf_out[2u * M.x + 1u] = make_float4(__uint_as_float(OL_NONE), 0.0f, 0.0f, 0.0f);
+      }
       incoming = rgb(rd);
       const uint32_t slot = __float_as_uint(hv.w);
       do {
@@ -543,22 +921,52 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
         }
         // Hit reconstruction (shadeLoadHit) and surface (shadeResolveSurface).
         const float t = hv.x, bu = hv.y, bv = hv.z;
-        const float4 ta = sc.tri[3 * slot], te1 = sc.tri[3 * slot + 1], te2 = sc.tri[3 * slot + 2];
-        const uint32_t prim = __float_as_uint(ta.w);
+        float4 ta, te1, te2;
+        uint32_t prim;
+        uint4 vi;
+        f3 n0, n1, n2, cw;  // vertex normals and the world-space edge cross product
+        uint32_t mat_id;
+        if (!sc.instanced) {
+          ta = sc.tri[3 * slot];
+          te1 = sc.tri[3 * slot + 1];
+          te2 = sc.tri[3 * slot + 2];
+          prim = __float_as_uint(ta.w);
+          vi = sc.tri_vidx[prim];
+          n0 = rgb(sc.normals[vi.x]);
+          n1 = rgb(sc.normals[vi.y]);
+          n2 = rgb(sc.normals[vi.z]);
+          cw = cross3(mk3(te1), mk3(te2));
+          mat_id = vi.w;
+        } else {
+          // Object-space triangle of the instance: normals and the edge cross
+          // product go to world space through the normal matrix; the cross
+          // product of world edges is det(A) * N * (e1 x e2).
+          const InstanceDev& I = sc.inst[W.hit_inst[i]];
+          const BlasDev& B = sc.blas[I.blas];
+          ta = B.tri[3 * slot];
+          te1 = B.tri[3 * slot + 1];
+          te2 = B.tri[3 * slot + 2];
+          const uint32_t local = __float_as_uint(ta.w);
+          prim = I.tri_offset + local;
+          vi = B.vidx[local];
+          n0 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.x])));
+          n1 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.y])));
+          n2 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.z])));
+          cw = xform_vector(I.nrm, cross3(mk3(te1), mk3(te2))) * I.nrm[0].w;
+          mat_id = I.material;
+        }
         hitp = mk3(ro.x + t * rd.x, ro.y + t * rd.y, ro.z + t * rd.z);
-        const f3 gn = normalize3(cross3(mk3(te1), mk3(te2)));
+        const f3 gn = normalize3(cw);
         if (record_primary) {
           J.geo[M.x] = make_float4(t, gn.x, gn.y, gn.z);
           J.prim[M.x] = prim;
         }
-        const uint4 vi = sc.tri_vidx[prim];
         const float wx = (1.0f - bu) - bv, wy = bu, wz = bv;
-        const f3 n0 = rgb(sc.normals[vi.x]), n1 = rgb(sc.normals[vi.y]), n2 = rgb(sc.normals[vi.z]);
         f3 sn = normalize3(((n0 * wx) + (n1 * wy)) + (n2 * wz));
         if (dot3(sn, gn) < 0.0f) sn = -sn;
         const bool entering = dot3(incoming, gn) < 0.0f;
         nrm = entering ? sn : -sn;
-        mat = load_mat(sc.materials, vi.w);
+        mat = load_mat(sc.materials, mat_id);
         uint32_t rs = M.z;
         uint32_t depth = (M.w >> 8) > 1u ? (M.w >> 8) : 1u;
         const uint32_t bounce = (uint32_t)T.w;
@@ -567,7 +975,7 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
           float emis = 1.0f;
           f3 e = rgb(mat.em);
           if ((M.w & F_PREV_DIFFUSE) && dot3(e, e) > 0.0f) {
-            float area = 0.5f * length3(cross3(mk3(te1), mk3(te2)));
+            float area = 0.5f * length3(cw);
             float raw = dot3(gn, -incoming);
             float lc = fmaxf(raw, 0.0f);
             if (mat.id.w > 0.5f) lc = fabsf(raw);
@@ -576,6 +984,7 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
             float lpdf = ((sel * t) * t) / fmaxf(lc * area, 1e-6f);
             float sampled = lpdf * (float)(P.secondary_area_samples > 1u ? P.secondary_area_samples : 1u);
             if (M.w & F_PREV_UNIFIED) sampled = lpdf * unified_area_probability(P);
+            // After ReSTIR the light technique's MIS counted its candidates.
             emis = power_heuristic(L.w, sampled);
           }
           f3 add = (rgb(T) * e) * emis;
@@ -626,7 +1035,14 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
             nee_T = rgb(T);
             nee_p = nee_prob;
             nee_rng = rs;
-            if (!P.two_pass) {
+            if (P.restir && bounce == 0u) {
+              // Primary hit: point lights, the ReSTIR area-light sample, then
+              // the environment samples (wavefront_primary_impl.glsl:1527-1785).
+              restir_primary(P, J, rs, hitp, nrm, mat, incoming, t, M.x, mat_id, single);
+              nee_rng = rs;
+              restir_nee(P, single, rs, hitp, nrm, mat, incoming, env_n, [&](const NeeSample&) { ++nee_count; });
+              used_restir = true;
+            } else if (!P.two_pass) {
               // At most one sample: evaluate once and keep it in registers.
               for_each_nee(P, rs, hitp, nrm, mat, incoming, area_n, env_n, [&](const NeeSample& s) {
                 single = s;
@@ -685,7 +1101,9 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
         M.w = (F_ACTIVE | (depth << 8)) | capture;
         if (transmission <= 0.001f) {
           M.w |= F_PREV_DIFFUSE;
-          if (P.unified) M.w |= F_PREV_UNIFIED;
+          // After ReSTIR the next hit weighs emission against per-sample
+          // area NEE (continuation flags of wavefront_primary_impl.glsl:1837).
+          if (P.unified && !used_restir) M.w |= F_PREV_UNIFIED;
           L.w = bsdf_pdf;
         }
         M.z = rs;
@@ -718,8 +1136,19 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
       if (gl < J.seg && el + nee_count <= J.eseg) {
         const uint32_t gslot = J.base + so * J.seg + gl, eslot = J.ebase + so * J.eseg + el;
         W.gT[gslot] = mk4(nee_T, nee_p);
-        W.gI[gslot] = make_uint4(path, eslot, nee_count, area_n | (env_n << 8) | ((P.unified ? 0u : 1u) << 16));
-        if (!P.two_pass) {
+        W.gI[gslot] = make_uint4(path, eslot, nee_count,
+                                 area_n | (env_n << 8) | ((P.unified && !used_restir ? 0u : 1u) << 16));
+        if (used_restir) {
+          uint32_t kk = 0;
+          uint32_t rs2 = nee_rng;
+          uint32_t en;
+          restir_nee(P, single, rs2, hitp, nrm, mat, incoming, en, [&](const NeeSample& s) {
+            W.eo[eslot + kk] = mk4(s.origin, 0.001f);
+            W.ed[eslot + kk] = mk4(s.dir, s.tmax);
+            W.ec[eslot + kk] = mk4(s.c, (float)s.kind);
+            ++kk;
+          });
+        } else if (!P.two_pass) {
           W.eo[eslot] = mk4(single.origin, 0.001f);
           W.ed[eslot] = mk4(single.dir, single.tmax);
           W.ec[eslot] = mk4(single.c, (float)single.kind);
@@ -761,7 +1190,7 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
     W.ctr[C_G + r] = 0u;
     W.ctr[C_E + r] = 0u;
   }
-  if (t < 2 * kWorkStripes) W.ctr[C_W + t] = 0u;
+  if (t < 4 * kWorkStripes) W.ctr[C_W + t] = 0u;
   __syncthreads();
   if (t < 4 * kMaxJobs) W.ctr[C_REPORT + t] = rep[t];
 }
@@ -791,9 +1220,12 @@ void check_spill(const GiLaunch& l, uint64_t threads) {
 
 template <int LSTACK, int BLOCK>
 uint64_t spill_words_for(const GiLaunch& l) {
-  int occ = std::max(std::max(resident_blocks(k_closest<LSTACK, BLOCK>, BLOCK),
-                              resident_blocks(k_shadow<LSTACK, BLOCK>, BLOCK)),
-                     resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK));
+  int occ = std::max(std::max(std::max(resident_blocks(k_closest<LSTACK, BLOCK>, BLOCK),
+                                       resident_blocks(k_shadow<LSTACK, BLOCK>, BLOCK)),
+                              std::max(resident_blocks(k_closest2<LSTACK, BLOCK>, BLOCK),
+                                       resident_blocks(k_shadow2<LSTACK, BLOCK>, BLOCK))),
+                     std::max(resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK),
+                              resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK)));
   return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
 
@@ -803,6 +1235,22 @@ void launch_closest(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t s
   dim3 grid((unsigned)(l.cu_count * occ));
   check_spill(l, (uint64_t)grid.x * BLOCK);
   hipLaunchKernelGGL((k_closest<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
+}
+
+template <int LSTACK, int BLOCK>
+void launch_closest2(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  static int occ = resident_blocks(k_closest2<LSTACK, BLOCK>, BLOCK);
+  dim3 grid((unsigned)(l.cu_count * occ));
+  check_spill(l, (uint64_t)grid.x * BLOCK);
+  hipLaunchKernelGGL((k_closest2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
+}
+
+template <int LSTACK, int BLOCK>
+void launch_shadow2(const GiLaunch& l, const Wave& w, hipStream_t st) {
+  static int occ = resident_blocks(k_shadow2<LSTACK, BLOCK>, BLOCK);
+  dim3 grid((unsigned)(l.cu_count * occ));
+  check_spill(l, (uint64_t)grid.x * BLOCK);
+  hipLaunchKernelGGL((k_shadow2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
@@ -816,13 +1264,18 @@ void launch_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
 template <int LSTACK, int BLOCK>
 void launch_rays(const GiLaunch& l, const DScene& sc, const float4* o, const float4* d, uint32_t n, float4* hits,
                  uint32_t* overflow, unsigned long long* work, hipStream_t st) {
-  static int occ = resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK);
+  static int occ = std::min(resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK),
+                            resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK));
   uint32_t need = (n + BLOCK - 1) / BLOCK;
   uint32_t cap = (uint32_t)(l.cu_count * occ);
   dim3 grid(need < cap ? (need ? need : 1) : cap);
   check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_trace_rays<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, o, d, n, hits, overflow, work,
-                     l.spill, spill_cap_of(l));
+  if (sc.instanced)
+    hipLaunchKernelGGL((k_trace_rays2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, o, d, n, hits, overflow, work,
+                       l.spill, spill_cap_of(l));
+  else
+    hipLaunchKernelGGL((k_trace_rays<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, sc, o, d, n, hits, overflow, work,
+                       l.spill, spill_cap_of(l));
 }
 
 // (LDS entries, block size) variants the launch autotuner chooses from.
@@ -862,8 +1315,14 @@ void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st) {
 }
 
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP);
-  OL_STACK_DISPATCH(launch_closest, l, w, in, st);
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP, w.jobs, 0u);
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP2, w.jobs, 1u);
+  wave_trace_kernels(l, w, in, st);
+}
+
+void wave_trace_kernels(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  [&] { OL_STACK_DISPATCH(launch_closest, l, w, in, st); }();
+  OL_STACK_DISPATCH(launch_closest2, l, w, in, st);
 }
 
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
@@ -872,8 +1331,10 @@ void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
 }
 
 void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP);
-  OL_STACK_DISPATCH(launch_shadow, l, w, st);
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP, w.jobs, 0u);
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP2, w.jobs, 1u);
+  [&] { OL_STACK_DISPATCH(launch_shadow, l, w, st); }();
+  OL_STACK_DISPATCH(launch_shadow2, l, w, st);
 }
 
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {

@@ -88,6 +88,25 @@ void scan_exclusive_u32(uint32_t* data, uint32_t n, uint32_t* total, hipStream_t
 void stream_copy(const float4* src, float4* dst, uint64_t n16, int cu_count, hipStream_t s);
 
 // ---- device-visible scene view ---------------------------------------------------
+// One placement of a bottom-level BVH in a two-level scene.
+struct InstanceDev {
+  float4 w2o[3];        // world -> object affine rows: o = (dot(row.xyz, p) + row.w)
+  float4 nrm[3];        // object -> world normal matrix rows (inverse transpose of the linear part)
+  uint32_t blas;        // bottom-level structure
+  uint32_t material;    // material record index
+  uint32_t tri_offset;  // first triangle of this instance in packed (reference) triangle order
+  uint32_t id;          // scene instance id
+};
+
+// Bottom-level BVH of one mesh resource, object space.
+struct BlasDev {
+  const WideNode* nodes;
+  const float4* tri;      // leaf-order (v0, local id bits), e1, e2
+  const uint4* vidx;      // local triangle -> (i0, i1, i2, unused), indices into `normals`
+  const float4* normals;  // object-space vertex normals of this resource
+  uint32_t node_count, tri_count, pad0, pad1;
+};
+
 struct DScene {
   const WideNode* nodes;
   const float4* tri;        // leaf-order traversal triangles
@@ -100,4 +119,10 @@ struct DScene {
   float area_weight;
   uint32_t env_valid;       // environment light present (analytic type 3)
   float env_r, env_g, env_b;  // color * intensity
+  // Two-level scenes: `nodes` is the top-level BVH over instances (leaf
+  // records hold the instance index in v0.w), `tri_vidx` and `normals` are
+  // unused.
+  uint32_t instanced, instance_count;
+  const InstanceDev* inst;
+  const BlasDev* blas;
 };

@@ -394,3 +394,207 @@ OL_DEV bool ol_trace_any(const WideNode* __restrict__ nodes, const float4* __res
     node = st.pop();
   }
 }
+
+// ---- two-level traversal (top-level BVH over instances of bottom-level BVHs) -----------
+// The stack is shared by both levels: entering an instance pushes
+// OL_SENTINEL, and popping it returns to the top level.  The ray is
+// transformed into object space without renormalising the direction, so
+// distances along it are the world distances and hits of different
+// instances compare directly.  Ties resolve to the lowest packed-order
+// triangle index (instance triangle offset + local index), the order of the
+// flattened scene.
+constexpr uint32_t OL_SENTINEL = 0x7FFFFFFFu;
+
+OL_DEV f3 xform_point(const float4* rows, f3 p) {
+  return mk3(((rows[0].x * p.x + rows[0].y * p.y) + rows[0].z * p.z) + rows[0].w,
+             ((rows[1].x * p.x + rows[1].y * p.y) + rows[1].z * p.z) + rows[1].w,
+             ((rows[2].x * p.x + rows[2].y * p.y) + rows[2].z * p.z) + rows[2].w);
+}
+OL_DEV f3 xform_vector(const float4* rows, f3 v) {
+  return mk3((rows[0].x * v.x + rows[0].y * v.y) + rows[0].z * v.z,
+             (rows[1].x * v.x + rows[1].y * v.y) + rows[1].z * v.z,
+             (rows[2].x * v.x + rows[2].y * v.y) + rows[2].z * v.z);
+}
+
+struct Closest2 {
+  RayPre w;                // world ray
+  RayPre r;                // current ray: world at the top level, object space inside an instance
+  float tmin;
+  TraceHit best;           // best.orig = packed-order triangle index
+  uint32_t best_inst;
+  uint32_t node;           // next item
+  uint32_t inst;           // current instance, OL_NONE at the top level
+  uint32_t offset;         // current instance's triangle offset
+  const WideNode* bnodes;  // current bottom-level BVH
+  const float4* btri;
+};
+
+OL_DEV void closest2_begin(Closest2& q, f3 o, f3 d, float tmin, float tmax) {
+  q.w = ray_pre(o, d);
+  q.r = q.w;
+  q.tmin = tmin;
+  q.best = TraceHit{tmax, 0.0f, 0.0f, OL_NONE, OL_NONE};
+  q.best_inst = OL_NONE;
+  q.node = 0;
+  q.inst = OL_NONE;
+  q.offset = 0;
+  q.bnodes = nullptr;
+  q.btri = nullptr;
+}
+
+// Leaf step: only bottom-level leaves (triangle tests in object space).
+OL_DEV bool closest2_at_leaf(const Closest2& q) { return q.inst != OL_NONE && child_is_leaf(q.node); }
+
+template <int LSTACK>
+OL_DEV void closest2_next(Closest2& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
+
+// Inner step: leave an instance (sentinel), enter one (top-level leaf), or
+// test a node's boxes at the current level.
+template <int LSTACK>
+OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv) {
+  if (q.node == OL_SENTINEL) {
+    q.inst = OL_NONE;
+    q.r = q.w;
+    closest2_next(q, st);
+    return;
+  }
+  if (q.inst == OL_NONE && child_is_leaf(q.node)) {
+    const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
+    const InstanceDev& I = sc.inst[k];
+    const BlasDev& B = sc.blas[I.blas];
+    if (B.node_count == 0) {
+      closest2_next(q, st);
+      return;
+    }
+    st.push(OL_SENTINEL);
+    q.inst = k;
+    q.offset = I.tri_offset;
+    q.bnodes = B.nodes;
+    q.btri = B.tri;
+    q.r = ray_pre(xform_point(I.w2o, q.w.o), xform_vector(I.w2o, q.w.d));
+    q.node = 0;
+    return;
+  }
+  const WideNode* nodes = q.inst == OL_NONE ? sc.nodes : q.bnodes;
+  uint4 w0, w1, w2, w3;
+  load_node(nodes, q.node, w0, w1, w2, w3);
+  ++nv;
+  uint32_t ch[4], key[4];
+  node_children(w2, w3, ch);
+  ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
+  sort4(key);
+  uint32_t next = OL_NONE;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (key[k] == OL_NONE) continue;
+    if (next != OL_NONE) st.push(next);
+    next = ch[key[k] & 3u];
+  }
+  if (next != OL_NONE) q.node = next;
+  else closest2_next(q, st);
+}
+
+template <int LSTACK>
+OL_DEV void closest2_leaf(Closest2& q, TravStack<LSTACK>& st, uint32_t& nt) {
+  const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
+  nt += cnt;
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint32_t slot = first + j;
+    const float4 a = q.btri[3 * slot], e1 = q.btri[3 * slot + 1], e2 = q.btri[3 * slot + 2];
+    float t, u, v;
+    if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
+      const uint32_t orig = q.offset + __float_as_uint(a.w);
+      if (t < q.best.t || (t == q.best.t && orig < q.best.orig)) {
+        q.best = TraceHit{t, u, v, slot, orig};
+        q.best_inst = q.inst;
+      }
+    }
+  }
+  closest2_next(q, st);
+}
+
+struct Any2 {
+  RayPre w, r;
+  float tmin, tmax, tcull;
+  uint32_t node, inst;
+  const WideNode* bnodes;
+  const float4* btri;
+  bool occluded;
+};
+
+OL_DEV void any2_begin(Any2& q, f3 o, f3 d, float tmin, float tmax) {
+  q.w = ray_pre(o, d);
+  q.r = q.w;
+  q.tmin = tmin;
+  q.tmax = tmax;
+  q.tcull = cull_distance(tmax);
+  q.node = 0;
+  q.inst = OL_NONE;
+  q.bnodes = nullptr;
+  q.btri = nullptr;
+  q.occluded = false;
+}
+
+OL_DEV bool any2_at_leaf(const Any2& q) { return q.inst != OL_NONE && child_is_leaf(q.node); }
+
+template <int LSTACK>
+OL_DEV void any2_next(Any2& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
+
+template <int LSTACK>
+OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& nv) {
+  if (q.node == OL_SENTINEL) {
+    q.inst = OL_NONE;
+    q.r = q.w;
+    any2_next(q, st);
+    return;
+  }
+  if (q.inst == OL_NONE && child_is_leaf(q.node)) {
+    const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
+    const InstanceDev& I = sc.inst[k];
+    const BlasDev& B = sc.blas[I.blas];
+    if (B.node_count == 0) {
+      any2_next(q, st);
+      return;
+    }
+    st.push(OL_SENTINEL);
+    q.inst = k;
+    q.bnodes = B.nodes;
+    q.btri = B.tri;
+    q.r = ray_pre(xform_point(I.w2o, q.w.o), xform_vector(I.w2o, q.w.d));
+    q.node = 0;
+    return;
+  }
+  const WideNode* nodes = q.inst == OL_NONE ? sc.nodes : q.bnodes;
+  uint4 w0, w1, w2, w3;
+  load_node(nodes, q.node, w0, w1, w2, w3);
+  ++nv;
+  uint32_t ch[4], key[4];
+  node_children(w2, w3, ch);
+  ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
+  uint32_t next = OL_NONE;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (key[k] == OL_NONE) continue;
+    if (next != OL_NONE) st.push(next);
+    next = ch[k];
+  }
+  if (next != OL_NONE) q.node = next;
+  else any2_next(q, st);
+}
+
+template <int LSTACK>
+OL_DEV void any2_leaf(Any2& q, TravStack<LSTACK>& st, uint32_t& nt) {
+  const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint32_t slot = first + j;
+    const float4 a = q.btri[3 * slot], e1 = q.btri[3 * slot + 1], e2 = q.btri[3 * slot + 2];
+    float t, u, v;
+    ++nt;
+    if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin && t <= q.tmax) {
+      q.occluded = true;
+      q.node = OL_NONE;
+      return;
+    }
+  }
+  any2_next(q, st);
+}

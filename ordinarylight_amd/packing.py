@@ -113,4 +113,129 @@ def pack_scene(scene):
     )
 
 
-__all__ = ["PackedScene", "pack_scene"]
+@dataclass
+class PackedInstancedScene:
+    """Two-level layout: object-space resources and their placements."""
+
+    vertex_offsets: np.ndarray      # uint32 [R + 1]
+    triangle_offsets: np.ndarray    # uint32 [R + 1]
+    positions: np.ndarray           # float32 [V, 3] object space
+    normals: np.ndarray             # float32 [V, 3]
+    indices: np.ndarray             # uint32 [T, 3] local to each resource
+    instance_resource: np.ndarray   # uint32 [I]
+    instance_material: np.ndarray   # uint32 [I]
+    instance_triangle_offset: np.ndarray  # uint32 [I] first triangle in packed order
+    instance_id: np.ndarray         # uint32 [I]
+    world_to_object: np.ndarray     # float32 [I, 3, 4]
+    normal_matrix: np.ndarray       # float32 [I, 3, 3]
+    determinant: np.ndarray         # float32 [I]
+    world_bounds: np.ndarray        # float32 [I, 6]
+    materials: np.ndarray
+    analytic_lights: np.ndarray
+    analytic_light_count: int
+    area_lights: np.ndarray
+    area_light_count: int
+    area_light_weight: float
+    environment: object
+    has_textures: bool
+
+    @property
+    def triangle_count(self):
+        return int(self.instance_triangle_offset[-1] + (self.triangle_offsets[self.instance_resource[-1] + 1]
+                                                        - self.triangle_offsets[self.instance_resource[-1]]))
+
+
+def shares_resources(scene):
+    """True when some mesh resource is placed by more than one visible mesh."""
+    seen = set()
+    for mesh in _meshes(scene):
+        resource = getattr(mesh, "resource", None)
+        if resource is None or not len(mesh.indices):
+            continue
+        if id(resource) in seen:
+            return True
+        seen.add(id(resource))
+    return False
+
+
+def pack_instanced_scene(scene):
+    """Pack ``scene`` as resources + instances for ``olhip_scene_create_instanced``.
+
+    Triangle order stays the flattened packed order (instance k's triangles
+    start at ``instance_triangle_offset[k]``), so ids and tie-breaks agree
+    with :func:`pack_scene` and ``Scene.triangle_instance_ids()``.
+    """
+    meshes = [m for m in _meshes(scene) if len(m.indices)]
+    if not meshes:
+        raise ValueError("a two-level scene needs at least one mesh")
+    resources, res_index = [], {}
+    materials, material_index = [], {}
+    textured = False
+    inst_res, inst_mat, inst_off, inst_id, mats = [], [], [], [], []
+    offset = 0
+    for mesh in meshes:
+        resource = mesh.resource
+        if id(resource) not in res_index:
+            res_index[id(resource)] = len(resources)
+            resources.append(resource)
+        material = mesh.material
+        if id(material) not in material_index:
+            material_index[id(material)] = len(materials)
+            if any(getattr(material, name, None) is not None for name in _TEXTURE_FIELDS):
+                textured = True
+            record = material_record(material)
+            record[5, 3] = 1.0  # resolved occlusion factor of an untextured material
+            materials.append(record)
+        inst_res.append(res_index[id(resource)])
+        inst_mat.append(material_index[id(material)])
+        inst_off.append(offset)
+        inst_id.append(int(mesh.id) if mesh.id is not None else len(inst_id))
+        mats.append(np.asarray(mesh.transform.matrix, np.float64))
+        offset += len(mesh.indices)
+    vcount = np.array([len(r.vertices) for r in resources], np.int64)
+    tcount = np.array([len(r.indices) for r in resources], np.int64)
+    vertex_offsets = np.concatenate(([0], np.cumsum(vcount))).astype(np.uint32)
+    triangle_offsets = np.concatenate(([0], np.cumsum(tcount))).astype(np.uint32)
+    positions = np.ascontiguousarray(np.concatenate([r.vertices for r in resources]), np.float32)
+    normals = np.ascontiguousarray(np.concatenate([r.normals for r in resources]), np.float32)
+    indices = np.ascontiguousarray(np.concatenate([r.indices for r in resources]), np.uint32)
+    M = np.stack(mats)                        # [I, 4, 4] object -> world
+    inv = np.linalg.inv(M)                    # world -> object
+    A = M[:, :3, :3]
+    normal_matrix = np.transpose(np.linalg.inv(A), (0, 2, 1))
+    det = np.linalg.det(A)
+    # World bounds: the 8 corners of each resource's object box, transformed
+    # in float64 and rounded outward to float32.
+    lo = np.stack([r.vertices.min(0) for r in resources]).astype(np.float64)
+    hi = np.stack([r.vertices.max(0) for r in resources]).astype(np.float64)
+    ri = np.asarray(inst_res)
+    corners = np.stack([np.where(np.array(bits, bool), hi[ri], lo[ri])
+                        for bits in np.ndindex(2, 2, 2)], axis=1)      # [I, 8, 3]
+    world = corners @ np.transpose(A, (0, 2, 1)) + M[:, None, :3, 3]
+    wmin = np.nextafter(world.min(1).astype(np.float32), np.float32(-np.inf))
+    wmax = np.nextafter(world.max(1).astype(np.float32), np.float32(np.inf))
+    if not materials:
+        raise ValueError("no materials")
+    analytic_count = int(getattr(scene, "analytic_light_count", 0))
+    area_count = int(getattr(scene, "emissive_triangle_count", 0))
+    return PackedInstancedScene(
+        vertex_offsets=vertex_offsets, triangle_offsets=triangle_offsets,
+        positions=positions, normals=normals, indices=indices,
+        instance_resource=np.asarray(inst_res, np.uint32), instance_material=np.asarray(inst_mat, np.uint32),
+        instance_triangle_offset=np.asarray(inst_off, np.uint32), instance_id=np.asarray(inst_id, np.uint32),
+        world_to_object=np.ascontiguousarray(inv[:, :3, :4], np.float32),
+        normal_matrix=np.ascontiguousarray(normal_matrix, np.float32),
+        determinant=np.ascontiguousarray(det, np.float32),
+        world_bounds=np.ascontiguousarray(np.concatenate((wmin, wmax), 1), np.float32),
+        materials=np.ascontiguousarray(np.stack(materials), np.float32),
+        analytic_lights=np.ascontiguousarray(scene.analytic_light_data(), np.float32),
+        analytic_light_count=analytic_count,
+        area_lights=np.ascontiguousarray(scene.emissive_triangle_data(), np.float32),
+        area_light_count=area_count,
+        area_light_weight=float(scene.emissive_light_weight),
+        environment=getattr(scene, "environment", None),
+        has_textures=textured,
+    )
+
+
+__all__ = ["PackedScene", "PackedInstancedScene", "pack_scene", "pack_instanced_scene", "shares_resources"]

@@ -225,6 +225,82 @@ def tessellated_room(seed=0, target_triangles=10_000):
     return scene, camera
 
 
+def icosphere(subdivisions=5, seed=0, jitter=0.02):
+    """Unit icosphere with 20 * 4^subdivisions triangles; vertices moved
+    radially by a seeded relative jitter.  Returns (vertices, indices)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = np.asarray(verts, np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.asarray(faces, np.int64)
+    for _ in range(subdivisions):
+        edges = np.concatenate((f[:, (0, 1)], f[:, (1, 2)], f[:, (2, 0)]))
+        key = np.sort(edges, axis=1)
+        uniq, inverse = np.unique(key, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = inverse.reshape(3, -1).T + len(v)  # midpoints of (01, 12, 20)
+        v = np.concatenate((v, mid))
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        f = np.concatenate((np.stack((a, m[:, 0], m[:, 2]), 1), np.stack((b, m[:, 1], m[:, 0]), 1),
+                            np.stack((c, m[:, 2], m[:, 1]), 1), np.stack((m[:, 0], m[:, 1], m[:, 2]), 1)))
+    rng = np.random.default_rng(seed)
+    v *= 1.0 + jitter * rng.uniform(-1.0, 1.0, (len(v), 1))
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def _random_rotation(rng):
+    """Uniform random rotation (Shoemake's quaternion method), 3x3."""
+    u1, u2, u3 = rng.random(3)
+    q = (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+         math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_spheres(seed=0, meshes=5, subdivisions=5, instances=10_000, box=120.0, dielectric=0.2,
+                      emitters=100_000, emitter_size=0.35):
+    """Two-level scene: ``meshes`` jittered icospheres (20 * 4^subdivisions
+    triangles each) placed ``instances`` times under random affine
+    transforms (uniform rotation, scale in [0.5, 2], translation in a cube of
+    side ``box``); a ``dielectric`` fraction of the instances is glass
+    (transmission 1, IOR 1.5), the rest GGX; ``emitters`` small emissive
+    triangles scattered through the cube."""
+    rng = np.random.default_rng(seed)
+    scene = Scene()
+    resources = []
+    for k in range(meshes):
+        v, f = icosphere(subdivisions, seed=seed * 1000 + k)
+        resources.append(scene.create_mesh(v, f, Material()))
+    glass = Material(base_color=(0.95, 0.97, 1.0), transmission=1.0, ior=1.5, roughness=0.0)
+    palette = [Material(base_color=tuple(float(x) for x in rng.uniform(0.1, 0.9, 3)),
+                        roughness=float(rng.uniform(0.05, 1.0)), metallic=float(rng.random() < 0.3))
+               for _ in range(64)]
+    half = 0.5 * box
+    for _ in range(instances):
+        m = np.eye(4)
+        m[:3, :3] = _random_rotation(rng) * rng.uniform(0.5, 2.0)
+        m[:3, 3] = rng.uniform(-half, half, 3)
+        material = glass if rng.random() < dielectric else palette[int(rng.integers(len(palette)))]
+        scene.add_instance(resources[int(rng.integers(meshes))], transform=Transform(m), material=material)
+    if emitters:
+        centers = rng.uniform(-half, half, (emitters, 1, 3))
+        offsets = rng.normal(0.0, emitter_size, (emitters, 3, 3))
+        ev = (centers + offsets).reshape(-1, 3).astype(np.float32)
+        ei = np.arange(3 * emitters, dtype=np.uint32).reshape(-1, 3)
+        scene.add_mesh(ev, ei, Material(base_color=(0.9, 0.9, 0.9), emission=(14.0, 12.0, 9.0),
+                                        emission_two_sided=True))
+    camera = PerspectiveCamera(position=(0.0, 0.0, -0.75 * box), target=(0.0, 0.0, 0.0),
+                               vertical_fov_degrees=55.0)
+    return scene, camera
+
+
 # Resident scenes of the batched and served workloads: three rooms and five
 # terrains spanning 10k to 2M triangles (terrain with (2k+1) x (k+1) vertices
 # has 4 k^2 triangles).
@@ -270,4 +346,5 @@ def mixed_jobs(seed=0, n_jobs=64, scene_keys=None, min_edge=128, max_edge=1024):
     return jobs
 
 
-__all__ = ["box_room", "terrain", "tessellated_room", "MIXED_SCENES", "mixed_scene", "mixed_jobs"]
+__all__ = ["box_room", "terrain", "tessellated_room", "MIXED_SCENES", "mixed_scene", "mixed_jobs", "icosphere",
+           "instanced_spheres"]

@@ -44,7 +44,10 @@ def scenes(ctx):
     v, i = workloads._box((0.4, 0.4, 0.4), None)
     lit.add_mesh(v, i, glass, transform=Transform.translation((0.1, 0.7, 0.3)))
     lit.add_point_light((0.3, 1.6, 0.4), (1.0, 0.9, 0.8), 2.0)
+    inst, inst_cam = workloads.instanced_spheres(3, meshes=2, subdivisions=2, instances=40, box=10.0,
+                                                 emitters=30, emitter_size=0.3)
     out = {
+        "inst": (DeviceScene(ctx, inst, 2, instanced=True), inst_cam),
         "room": (DeviceScene(ctx, room, 2), room_cam),
         "troom": (DeviceScene(ctx, troom, 2), troom_cam),
         "terrain": (DeviceScene(ctx, terr, 2), terr_cam),
@@ -65,6 +68,7 @@ def make_jobs(scenes):
         ("room", 20, 90, 1, 8, 9, {"wavefront_environment_samples": 2}),
         ("lit", 64, 64, 4, 6, 3, {}),
         ("troom", 128, 96, 1, 4, 0, {}),
+        ("inst", 72, 40, 3, 8, 4, {}),
     ]
     jobs = []
     for key, w, h, spp, bounces, frame, extra in spec:
