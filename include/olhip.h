@@ -291,6 +291,15 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
  * not depend on these settings. */
 int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block);
 
+/* Primary rays of a pixel tile, exactly as a render generates them for
+ * (pixel, frame_index, sample) (wavefront_generate.comp:86-167): origins /
+ * directions [tile_height][tile_width][4] (xyz, tmin / tmax), row-major from
+ * (x0, y0).  With olhip_trace this is the GPU pick of
+ * targets/vulkan/api.py:936-987 (a 1x1 tile at frame 0, sample 0). */
+int olhip_camera_rays(olhip_ctx* ctx, const olhip_camera* cam, uint32_t width, uint32_t height, uint32_t frame_index,
+                      uint32_t sample, uint32_t x0, uint32_t y0, uint32_t tile_width, uint32_t tile_height,
+                      float* origins, float* directions);
+
 /* Closest hit of caller rays: origins/directions [n][4] (xyz, tmin / tmax).
  * hits [n][4]: t, u, v, original triangle index as uint32 bits (0xFFFFFFFF
  * on a miss).  Primary-hit products use this (core.py:4500-4536). */

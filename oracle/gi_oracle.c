@@ -1225,6 +1225,25 @@ static void camera_ray(const float cam[12], uint32_t proj, uint32_t W, uint32_t 
   }
 }
 
+/* Camera rays of a pixel tile, row-major: o/d [n][4] (xyz, tmin / tmax). */
+void API(olgi_camera_rays)(const float cam[12], uint32_t proj, uint32_t W, uint32_t H, uint32_t frame,
+                           uint32_t sample, uint32_t x0, uint32_t y0, uint32_t tw, uint32_t th, float* o, float* d) {
+  for (uint32_t ly = 0; ly < th; ++ly)
+    for (uint32_t lx = 0; lx < tw; ++lx) {
+      V oo, dd;
+      camera_ray(cam, proj, W, H, x0 + lx, y0 + ly, frame, sample, &oo, &dd);
+      const uint64_t i = (uint64_t)ly * tw + lx;
+      o[4 * i] = (float)oo.x;
+      o[4 * i + 1] = (float)oo.y;
+      o[4 * i + 2] = (float)oo.z;
+      o[4 * i + 3] = 0.001f;
+      d[4 * i] = (float)dd.x;
+      d[4 * i + 1] = (float)dd.y;
+      d[4 * i + 2] = (float)dd.z;
+      d[4 * i + 3] = 1e30f;
+    }
+}
+
 /* Renders rows [row0, row1) into out (W*H*4 floats, full image layout).
  * settings: max_bounces, sec_area, env_samples, rr_start, rr_min, nee_prob,
  * unified.  frame is the float32-rounded frame index.  counts[3] receives

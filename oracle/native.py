@@ -51,6 +51,9 @@ def lib():
             getattr(L, "olgi_render_restir" + sfx).argtypes = [
                 ctypes.c_void_p, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                 ctypes.c_uint32, P_u32, P_f32, ctypes.c_uint32, P_u32, P_f32, P_u64]
+            getattr(L, "olgi_camera_rays" + sfx).argtypes = [
+                P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P_f32, P_f32]
             getattr(L, "olgi_f2h" + sfx).restype = ctypes.c_uint16
             getattr(L, "olgi_f2h" + sfx).argtypes = [ctypes.c_float]
             getattr(L, "olgi_h2f" + sfx).restype = ctypes.c_float
@@ -200,6 +203,17 @@ def lbvh_build(positions, indices, leaf_size=4):
     n = int(count[0])
     return {"morton": morton, "sorted": order, "left": left[:T - 1], "right": right[:T - 1],
             "box": box[:T - 1], "nodes": nodes[:n], "tri": tri, "depth": int(depth[0])}
+
+
+def camera_rays(camera_basis, projection, width, height, frame_index=0, sample=0, tile=None):
+    """Primary rays of a pixel tile (x0, y0, w, h), as the oracle's renders generate them."""
+    cam = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float32) for v in camera_basis]), np.float32)
+    x0, y0, tw, th = tile if tile is not None else (0, 0, width, height)
+    frame = int(np.float32(frame_index) + np.float32(0.5))
+    o = np.zeros((tw * th, 4), np.float32)
+    d = np.zeros((tw * th, 4), np.float32)
+    lib().olgi_camera_rays_f32(_p(cam), projection, width, height, frame, sample, x0, y0, tw, th, _p(o), _p(d))
+    return o, d
 
 
 def float_to_half(x):

@@ -657,6 +657,42 @@ int olhip_frame_download(olhip_ctx* ctx, float* out_rgba) {
   });
 }
 
+int olhip_camera_rays(olhip_ctx* ctx, const olhip_camera* cam, uint32_t width, uint32_t height, uint32_t frame_index,
+                      uint32_t sample, uint32_t x0, uint32_t y0, uint32_t tile_width, uint32_t tile_height,
+                      float* origins, float* directions) {
+  return guarded([&] {
+    require(ctx && cam && origins && directions, "arguments must not be NULL");
+    require(width >= 1 && height >= 1, "width and height must be positive");
+    require(cam->projection <= 2, "unknown projection");
+    require(tile_width >= 1 && tile_height >= 1, "tile must not be empty");
+    require((uint64_t)x0 + tile_width <= width && (uint64_t)y0 + tile_height <= height, "tile outside the image");
+    require((uint64_t)tile_width * tile_height <= (1ull << 26), "tile too large");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    GiParams P;
+    std::memset(&P, 0, sizeof(P));
+    for (int k = 0; k < 3; ++k) {
+      P.cam_o[k] = cam->position[k];
+      P.cam_f[k] = cam->forward[k];
+      P.cam_r[k] = cam->right[k];
+      P.cam_u[k] = cam->up[k];
+    }
+    P.projection = cam->projection;
+    P.width = width;
+    P.height = height;
+    P.frame = (uint32_t)((float)frame_index + 0.5f);
+    const uint32_t n = tile_width * tile_height;
+    DevBuf o, d;
+    o.reserve(sizeof(float4) * n);
+    d.reserve(sizeof(float4) * n);
+    camera_rays(P, x0, y0, tile_width, tile_height, sample, o.as<float4>(), d.as<float4>(), s);
+    OL_HIP_CHECK(hipGetLastError());
+    OL_HIP_CHECK(hipMemcpyAsync(origins, o.ptr, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+    OL_HIP_CHECK(hipMemcpyAsync(directions, d.ptr, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int olhip_trace(olhip_ctx* ctx, const olhip_scene* sc, const float* origins, const float* directions, uint32_t n,
                 float* hits) {
   return guarded([&] {

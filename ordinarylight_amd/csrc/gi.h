@@ -83,8 +83,8 @@ struct JobDev {
   float4* geo;
   uint32_t* prim;
   // ReSTIR DI (null unless enabled): this pass's and the previous pass's
-  // stored reservoirs and primary surfaces (2 x float4 per pixel: position,
-  // distance; shading normal, material id bits).
+  // stored reservoirs and primary surfaces (2 x float4 per pixel: shading
+  // normal and primary distance, or -1 without a ReSTIR hit; material id bits).
   uint4* res_out;
   const uint4* res_in;
   float4* surf_out;
@@ -164,5 +164,8 @@ void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
 void wave_clear(const Wave& w, hipStream_t st);
 
 // Closest hit of caller-supplied rays (primary-hit products, debugging).
+// Primary rays of a pixel tile (row-major) for the camera, size and frame in P.
+void camera_rays(const GiParams& P, uint32_t x0, uint32_t y0, uint32_t tw, uint32_t th, uint32_t sample, float4* o,
+                 float4* d, hipStream_t st);
 void trace_rays(const GiLaunch& l, const DScene& sc, const float4* o, const float4* d, uint32_t n, float4* hits,
                 uint32_t* overflow, unsigned long long* work, hipStream_t st);

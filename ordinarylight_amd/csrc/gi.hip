@@ -164,6 +164,53 @@ __global__ void __launch_bounds__(256) k_accumulate(Wave W) {
   }
 }
 
+// Camera ray of one (pixel, sample) with the hash jitter of
+// wavefront_generate.comp:115-149; rng is the path's initial state.
+OL_DEV void primary_ray(const GiParams& P, uint32_t px, uint32_t py, uint32_t sample, f3& org, f3& dir,
+                        uint32_t& rng) {
+  const uint32_t Wd = P.width, H = P.height;
+  const uint32_t pixel = py * Wd + px;
+  rng = wave_hash((pixel ^ wave_hash(P.frame)) ^ wave_hash(sample + 1u));
+  rng = wave_hash(rng);
+  const float jx = u32_to_unit(rng);
+  rng = wave_hash(rng);
+  const float jy = u32_to_unit(rng);
+  const float ndx = (((float)px + jx) / (float)Wd) * 2.0f - 1.0f;
+  const float ndy = (((float)py + jy) / (float)H) * 2.0f - 1.0f;
+  const float aspect = (float)Wd / (float)H;
+  org = mk3(P.cam_o[0], P.cam_o[1], P.cam_o[2]);
+  const f3 fw = mk3(P.cam_f[0], P.cam_f[1], P.cam_f[2]);
+  const f3 rt = mk3(P.cam_r[0], P.cam_r[1], P.cam_r[2]);
+  const f3 up = mk3(P.cam_u[0], P.cam_u[1], P.cam_u[2]);
+  if (P.projection == 1u) {
+    org = (org + ((ndx * aspect) * rt)) - (ndy * up);
+    dir = normalize3(fw);
+  } else if (P.projection == 2u) {
+    float yaw = ndx * length3(rt);
+    float pitch = (-ndy) * length3(up);
+    float sy, cy, sp, cp;
+    ol_sincos(yaw, &sy, &cy);
+    ol_sincos(pitch, &sp, &cp);
+    dir = normalize3((((normalize3(fw) * cp) * cy) + ((normalize3(rt) * cp) * sy)) + (normalize3(up) * sp));
+  } else {
+    dir = normalize3((fw + ((ndx * aspect) * rt)) - (ndy * up));
+  }
+}
+
+// Primary rays of a pixel tile [x0, x0 + tw) x [y0, y0 + th), row-major
+// (picking and tests): origin/tmin, direction/tmax as the queues hold them.
+__global__ void __launch_bounds__(256) k_camera_rays(GiParams P, uint32_t x0, uint32_t y0, uint32_t tw, uint32_t th,
+                                                     uint32_t sample, float4* o, float4* d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= tw * th) return;
+  const uint32_t ly = i / tw, lx = i - ly * tw;
+  f3 org, dir;
+  uint32_t rng;
+  primary_ray(P, x0 + lx, y0 + ly, sample, org, dir, rng);
+  o[i] = mk4(org, 0.001f);
+  d[i] = mk4(dir, 1e30f);
+}
+
 // Camera rays and path initialisation of the listed jobs
 // (wavefront_generate.comp:86-167).  Queue slots follow 8x8 pixel tiles, so a
 // wave's rays are coherent; the job's queue segments are filled in order
@@ -200,32 +247,9 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
     const uint32_t pixel = py * Wd + px;
     const uint32_t path = J.base + local * np + pixel;
     const uint32_t slot = J.base + i;
-    uint32_t rng = wave_hash((pixel ^ wave_hash(P.frame)) ^ wave_hash(sample + 1u));
-    rng = wave_hash(rng);
-    const float jx = u32_to_unit(rng);
-    rng = wave_hash(rng);
-    const float jy = u32_to_unit(rng);
-    const float ndx = (((float)px + jx) / (float)Wd) * 2.0f - 1.0f;
-    const float ndy = (((float)py + jy) / (float)H) * 2.0f - 1.0f;
-    const float aspect = (float)Wd / (float)H;
-    f3 org = mk3(P.cam_o[0], P.cam_o[1], P.cam_o[2]);
-    const f3 fw = mk3(P.cam_f[0], P.cam_f[1], P.cam_f[2]);
-    const f3 rt = mk3(P.cam_r[0], P.cam_r[1], P.cam_r[2]);
-    const f3 up = mk3(P.cam_u[0], P.cam_u[1], P.cam_u[2]);
-    f3 dir;
-    if (P.projection == 1u) {
-      org = (org + ((ndx * aspect) * rt)) - (ndy * up);
-      dir = normalize3(fw);
-    } else if (P.projection == 2u) {
-      float yaw = ndx * length3(rt);
-      float pitch = (-ndy) * length3(up);
-      float sy, cy, sp, cp;
-      ol_sincos(yaw, &sy, &cy);
-      ol_sincos(pitch, &sp, &cp);
-      dir = normalize3((((normalize3(fw) * cp) * cy) + ((normalize3(rt) * cp) * sy)) + (normalize3(up) * sp));
-    } else {
-      dir = normalize3((fw + ((ndx * aspect) * rt)) - (ndy * up));
-    }
+    f3 org, dir;
+    uint32_t rng;
+    primary_ray(P, px, py, sample, org, dir, rng);
     W.qo[in][slot] = mk4(org, 0.001f);
     W.qd[in][slot] = mk4(dir, 1e30f);
     W.qp[in][slot] = path;
@@ -1339,6 +1363,12 @@ void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
 
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {
   hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(1024), 0, st, w, in);
+}
+
+void camera_rays(const GiParams& P, uint32_t x0, uint32_t y0, uint32_t tw, uint32_t th, uint32_t sample, float4* o,
+                 float4* d, hipStream_t st) {
+  const uint32_t n = tw * th;
+  hipLaunchKernelGGL(k_camera_rays, dim3((n + 255u) / 256u), dim3(256), 0, st, P, x0, y0, tw, th, sample, o, d);
 }
 
 void trace_rays(const GiLaunch& l, const DScene& sc, const float4* o, const float4* d, uint32_t n, float4* hits,

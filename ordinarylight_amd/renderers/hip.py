@@ -26,6 +26,7 @@ import numpy as np
 from .. import _native as N
 from ..cameras import camera_basis, camera_projection
 from ..packing import pack_instanced_scene, pack_scene, shares_resources
+from ..selection import PickOptions, PickResult, ViewportMapping
 from .base import RendererImplementation, RendererImplementationInfo
 
 
@@ -139,6 +140,25 @@ class DeviceContext:
         N.check(self._lib.olhip_create(int(device_index), N.ctypes.byref(handle)))
         self.handle = handle
         self.device_index = int(device_index)
+
+    def camera_rays(self, camera, width, height, frame_index=0, sample=0, tile=None):
+        """Primary rays of a pixel tile ``(x0, y0, w, h)`` (default: the whole
+        image) exactly as a render generates them: float32 origins
+        ``[n, 4]`` (xyz, tmin) and directions ``[n, 4]`` (xyz, tmax), row-major."""
+        x0, y0, tw, th = tile if tile is not None else (0, 0, width, height)
+        position, forward, right, up = camera_basis(camera)
+        cam = N.Camera()
+        cam.position[:] = position.tolist()
+        cam.forward[:] = forward.tolist()
+        cam.right[:] = right.tolist()
+        cam.up[:] = up.tolist()
+        cam.projection = camera_projection(camera)
+        o = np.empty((tw * th, 4), np.float32)
+        d = np.empty((tw * th, 4), np.float32)
+        N.check(N.library().olhip_camera_rays(self.handle, N.ctypes.byref(cam), int(width), int(height),
+                                              int(frame_index), int(sample), int(x0), int(y0), int(tw), int(th),
+                                              N.ptr(o), N.ptr(d)))
+        return o, d
 
     def close(self):
         if self.handle:
@@ -343,7 +363,7 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
             "renderer": self.implementation.name,
             "outputs": self.available_outputs,
             "features": frozenset({"offscreen_rendering", "global_illumination", "instancing",
-                                   "area_lights", "transmission", "resident_scene"}),
+                                   "area_lights", "transmission", "resident_scene", "picking", "restir_di"}),
             "limits": {"max_bounces": 64, "max_samples_per_pixel": int(self.config.max_samples_per_call)},
             "device": self.device,
         }
@@ -460,6 +480,45 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
                 ids_out[hit] = table[prim[hit]]
                 result[name] = ids_out
         return {name: result[name] for name in outputs}
+
+    def pick(self, scene, camera, viewport_size, pixel, *, options=None, mapping=None):
+        """Object under a pixel: the pixel's primary ray of frame 0, sample 0
+        (the jittered ray a render traces) against the resident BVH, as
+        ``targets/vulkan/api.py:936-987``.  None on a miss."""
+        options = PickOptions() if options is None else options
+        if not isinstance(options, PickOptions):
+            raise TypeError("options must be PickOptions")
+        if options != PickOptions():
+            raise NotImplementedError("pick policies other than the default are not supported on the HIP renderer")
+        if mapping is not None:
+            if not isinstance(mapping, ViewportMapping):
+                raise TypeError("mapping must be ViewportMapping")
+            pixel = mapping.map_pixel(pixel)
+            if pixel is None:
+                return None
+            viewport_size = mapping.render_size
+        width, height = _check_size(*viewport_size)
+        px, py = (float(v) for v in pixel)
+        if not (math.isfinite(px) and math.isfinite(py)):
+            raise ValueError("pixel must contain two finite coordinates")
+        x = min(max(int(math.floor(px)), 0), width - 1)
+        y = min(max(int(math.floor(py)), 0), height - 1)
+        with self._lock:
+            if self._closed:
+                raise RuntimeError("renderer is closed")
+            device_scene = self._resident(scene)
+            o, d = self._ensure_context().camera_rays(camera, width, height, 0, 0, (x, y, 1, 1))
+            t, u, v, prim = device_scene.trace(o[:, :3], d[:, :3], float(o[0, 3]), float(d[0, 3]))
+        primitive = int(prim[0])
+        if primitive == 0xFFFFFFFF:
+            return None
+        object_id = int(np.asarray(scene.triangle_instance_ids())[primitive])
+        start, _ = scene.object_triangle_range(object_id)
+        distance = np.float32(t[0])
+        position = o[0, :3] + distance * d[0, :3]
+        return PickResult(scene.get_instance(object_id), object_id, float(distance),
+                          tuple(float(c) for c in position), primitive, primitive - int(start),
+                          (float(np.float32(1.0) - u[0] - v[0]), float(u[0]), float(v[0])))
 
     def close(self):
         _ResidentMixin.close(self)

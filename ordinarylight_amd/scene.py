@@ -495,6 +495,25 @@ class Scene:
 
     triangle_object_ids = triangle_instance_ids
 
+    def get_instance(self, reference):
+        """The instance with this stable id (or the instance itself)."""
+        object_id = int(reference.id if hasattr(reference, "id") else reference)
+        for mesh in self.meshes:
+            if mesh.id == object_id:
+                return mesh
+        raise KeyError(f"scene has no instance with id {object_id}")
+
+    def object_triangle_range(self, reference):
+        """Half-open range of the object's triangles in packed order."""
+        object_id = int(reference.id if hasattr(reference, "id") else reference)
+        offset = 0
+        for mesh in self.render_meshes:
+            count = len(mesh.indices)
+            if mesh.id == object_id:
+                return offset, offset + count
+            offset += count
+        raise KeyError(f"scene has no visible object with id {object_id}")
+
     def triangle_material_ids(self):
         values = [np.full(len(m.indices), self.material_id(m.material), np.uint32)
                   for m in self.render_meshes if len(m.indices)]
