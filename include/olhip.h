@@ -291,6 +291,28 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
  * not depend on these settings. */
 int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block);
 
+/* Output formats of olhip_frame_output. */
+#define OLHIP_OUTPUT_RGBA8 0u
+#define OLHIP_OUTPUT_NV12 1u
+#define OLHIP_OUTPUT_P010 2u
+
+/* Display output of the last frame rendered with olhip_render_gi (kept on the
+ * device whether or not it was copied out); replaces the presentation passes
+ * wavefront_tone_map.comp, rgba_to_nv12.comp and hdr_to_p010.comp behind
+ * render_gpu_frame (targets/vulkan/api.py:1074-1115).  Colour is
+ * sRGB(ACES(max(hdr, 0) * exposure)).
+ *   RGBA8: uint8 [height][width][4], alpha 255; size must equal the frame's.
+ *   NV12:  luma plane (pitch = width bytes) then interleaved CbCr plane of
+ *          the 2x2 averages of the RGBA8 pixels, BT.709 studio range.
+ *   P010:  the same with 10-bit values in the high bits of 16-bit words
+ *          (pitch = 2 * width bytes), from the half-float HDR image scaled
+ *          to width x height by nearest neighbour.
+ * NV12 and P010 need width % 4 == 0 and even height.  out is host memory, or
+ * device memory of ctx's device when out_on_device is 1 (a GPU-resident
+ * frame, no host copy). */
+int olhip_frame_output(olhip_ctx* ctx, uint32_t format, float exposure, uint32_t width, uint32_t height, void* out,
+                       uint32_t out_on_device);
+
 /* Primary rays of a pixel tile, exactly as a render generates them for
  * (pixel, frame_index, sample) (wavefront_generate.comp:86-167): origins /
  * directions [tile_height][tile_width][4] (xyz, tmin / tmax), row-major from

@@ -18,7 +18,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIBRARY = HERE / "libolhip.so"
-SOURCES = ("bvh_build.hip", "scan.hip", "gi.hip", "engine.hip", "e0.hip", "api.hip")
+SOURCES = ("bvh_build.hip", "scan.hip", "gi.hip", "engine.hip", "e0.hip", "output.hip", "api.hip")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OLHIP_ARCH", "gfx950")

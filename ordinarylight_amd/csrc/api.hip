@@ -12,6 +12,7 @@
 #include "engine.h"
 #include "gi.h"
 #include "internal.h"
+#include "output.h"
 
 namespace {
 
@@ -505,6 +506,8 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
                                 ctx->stream));
     OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->image_pixels = (uint32_t)np;
+    ctx->image_width = width;
+    ctx->image_height = height;
     if (d.products) eng.products(id, depth, normal, primitive, variance);
     eng.take_result(id, out_rgba, &js);
     if (stats_out) {
@@ -654,6 +657,36 @@ int olhip_frame_download(olhip_ctx* ctx, float* out_rgba) {
     OL_HIP_CHECK(hipSetDevice(ctx->device));
     OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     OL_HIP_CHECK(hipMemcpy(out_rgba, ctx->image.ptr, sizeof(float4) * ctx->image_pixels, hipMemcpyDeviceToHost));
+  });
+}
+
+int olhip_frame_output(olhip_ctx* ctx, uint32_t format, float exposure, uint32_t width, uint32_t height, void* out,
+                       uint32_t out_on_device) {
+  return guarded([&] {
+    require(ctx && out, "ctx and out must not be NULL");
+    require(ctx->image_pixels > 0, "no frame has been rendered");
+    require(format <= OLHIP_OUTPUT_P010, "unknown output format");
+    require(std::isfinite(exposure) && exposure > 0.0f, "exposure must be positive");
+    require(width >= 1 && height >= 1, "width and height must be positive");
+    if (format != OLHIP_OUTPUT_P010)
+      require(width == ctx->image_width && height == ctx->image_height, "output size must equal the frame size");
+    if (format != OLHIP_OUTPUT_RGBA8)
+      require(width % 4 == 0 && height % 2 == 0, "NV12/P010 output requires width divisible by 4 and even height");
+    require((uint64_t)width * height <= (1ull << 28), "output too large");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t bytes = output_bytes(format, width, height);
+    DevBuf staging;
+    uint32_t* dst = static_cast<uint32_t*>(out);
+    if (!out_on_device) {
+      staging.reserve(bytes);
+      dst = staging.as<uint32_t>();
+    }
+    launch_output(format, ctx->image.as<float4>(), ctx->image_width, ctx->image_height, width, height, exposure, dst,
+                  s);
+    OL_HIP_CHECK(hipGetLastError());
+    if (!out_on_device) OL_HIP_CHECK(hipMemcpyAsync(out, staging.ptr, bytes, hipMemcpyDeviceToHost, s));
+    OL_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
 

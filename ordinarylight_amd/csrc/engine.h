@@ -179,8 +179,8 @@ struct olhip_ctx {
   std::unique_ptr<Engine> frame_engine;
   uint64_t frame_engine_slots = 0;
   uint32_t frame_engine_entries = 0;
-  DevBuf image;  // last single frame (olhip_frame_download)
-  uint32_t image_pixels = 0;
+  DevBuf image;  // last single frame (olhip_frame_download, olhip_frame_output)
+  uint32_t image_pixels = 0, image_width = 0, image_height = 0;
   DevBuf spill;  // olhip_trace
   // Reference-estimator frame.
   struct E0Frame {

@@ -58,6 +58,8 @@ class HipRendererConfig:
     wavefront_restir_spatial_neighbors: int = 4
     wavefront_restir_spatial_radius: int = 4
     wavefront_restir_pairwise_mis: bool = True
+    # Exposure of the display outputs (api.py:107).
+    wavefront_exposure: float = 1.0
     # Resident paths per device chunk (the reference's per-tile capacity is
     # 131,072; one MI355X holds whole frames, so the default is 2^25).
     wavefront_tile_capacity: int = 1 << 25
@@ -88,6 +90,8 @@ class HipRendererConfig:
             raise ValueError("wavefront_tile_capacity must be positive")
         if not 0 <= int(self.bvh_leaf_size) <= 16:
             raise ValueError("bvh_leaf_size must be between 0 and 16")
+        if not (math.isfinite(float(self.wavefront_exposure)) and float(self.wavefront_exposure) > 0.0):
+            raise ValueError("wavefront_exposure must be positive")
         if not 1 <= int(self.wavefront_restir_history_limit) <= 64:
             raise ValueError("wavefront_restir_history_limit must be between 1 and 64")
         if not 1 <= int(self.wavefront_restir_candidates) <= 64:
@@ -128,7 +132,12 @@ HOT_SETTINGS = frozenset({
     "wavefront_russian_roulette", "wavefront_russian_roulette_start",
     "wavefront_russian_roulette_min_survival", "wavefront_tile_capacity",
     "max_samples_per_call", "traversal_lds_stack", "traversal_block", "profiling", "work_counters",
+    "wavefront_exposure", "wavefront_restir_di", "wavefront_restir_candidates", "wavefront_restir_history_limit",
+    "wavefront_restir_spatial_reuse", "wavefront_restir_spatial_neighbors", "wavefront_restir_spatial_radius",
+    "wavefront_restir_pairwise_mis",
 })
+
+OUTPUT_FORMATS = {"rgba8": 0, "nv12": 1, "p010": 2}
 
 
 class DeviceContext:
@@ -480,6 +489,40 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
                 ids_out[hit] = table[prim[hit]]
                 result[name] = ids_out
         return {name: result[name] for name in outputs}
+
+    def render_output(self, scene, camera, width, height, *, pixel_format="rgba8", samples=None, frame_index=0,
+                      output_size=None, device_out=None):
+        """Display frame of a render (the reference's GPU-resident output,
+        ``targets/vulkan/api.py:1074-1115``): tone-mapped RGBA8
+        ``uint8 [H, W, 4]``, NV12 ``uint8 [H * 3 / 2, W]`` or P010
+        ``uint16 [H * 3 / 2, W]``.  ``output_size`` scales P010 only.  With
+        ``device_out`` (an address of device memory on the renderer's device,
+        e.g. a torch tensor's ``data_ptr()``), the frame is written there and
+        nothing returns to the host."""
+        pixel_format = str(pixel_format).lower()
+        if pixel_format not in OUTPUT_FORMATS:
+            raise ValueError("pixel_format must be 'rgba8', 'nv12', or 'p010'")
+        width, height = _check_size(width, height)
+        ow, oh = _check_size(*(output_size or (width, height)))
+        if pixel_format != "p010" and (ow, oh) != (width, height):
+            raise ValueError("only P010 output can be scaled")
+        if pixel_format != "rgba8" and (ow % 4 or oh % 2):
+            raise ValueError(f"{pixel_format.upper()} output requires width divisible by 4 and even height")
+        with self._lock:
+            self._render(scene, camera, width, height, samples, frame_index, keep_on_device=True)
+            lib = N.library()
+            fmt = OUTPUT_FORMATS[pixel_format]
+            if device_out is not None:
+                N.check(lib.olhip_frame_output(self._ensure_context().handle, fmt, float(self.config.wavefront_exposure),
+                                               ow, oh, N.ctypes.c_void_p(int(device_out)), 1))
+                return None
+            if pixel_format == "rgba8":
+                out = np.empty((oh, ow, 4), np.uint8)
+            else:
+                out = np.empty((oh * 3 // 2, ow), np.uint8 if pixel_format == "nv12" else np.uint16)
+            N.check(lib.olhip_frame_output(self._ensure_context().handle, fmt, float(self.config.wavefront_exposure),
+                                           ow, oh, out.ctypes.data_as(N.ctypes.c_void_p), 0))
+            return out
 
     def pick(self, scene, camera, viewport_size, pixel, *, options=None, mapping=None):
         """Object under a pixel: the pixel's primary ray of frame 0, sample 0
