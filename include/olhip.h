@@ -101,6 +101,18 @@ typedef struct olhip_instanced_desc {
 
 int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* desc, olhip_scene** out);
 
+/* In-place update of a resident scene with the same triangles (vertex
+ * indices; for two-level scenes the same resources and each instance's
+ * resource): material ids may change; new vertex positions and normals (optional: NULL positions
+ * keeps them) refit the BVH over the kept leaf order, as the reference's
+ * acceleration-structure update (targets/vulkan/scene.py:1217-1275); new
+ * instance transforms rebuild the top level; materials and lights are
+ * replaced.  Replaces _try_update_window_scene (targets/vulkan/core.py
+ * :4696-4870).  A different structure is rejected (OLHIP_ERR_VALUE).  Jobs
+ * that use the scene must not be queued or running during an update. */
+int olhip_scene_update(olhip_ctx* ctx, olhip_scene* scene, const olhip_scene_desc* desc);
+int olhip_scene_update_instanced(olhip_ctx* ctx, olhip_scene* scene, const olhip_instanced_desc* desc);
+
 typedef struct olhip_scene_info {
   uint32_t triangle_count;
   uint32_t node_count;

@@ -62,6 +62,13 @@ float elapsed(const Event& a, const Event& b) {
   return ms;
 }
 
+constexpr uint64_t kFnvBasis = 1469598103934665603ull;
+uint64_t fnv1a(const void* data, size_t bytes, uint64_t h) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  for (size_t i = 0; i < bytes; ++i) h = (h ^ p[i]) * 1099511628211ull;
+  return h;
+}
+
 }  // namespace
 
 extern "C" {
@@ -180,8 +187,10 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     if (d->area_light_count) std::memcpy(area.data(), d->area_lights, sizeof(float) * 20ull * d->area_light_count);
     upload(sc->area, area.data(), area.size(), s);
     OL_HIP_CHECK(hipStreamSynchronize(s));
+    sc->structure_hash = fnv1a(d->indices, 12ull * d->triangle_count, kFnvBasis);
+    sc->leaf_size = d->leaf_size;
     BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, d->leaf_size};
-    bvh_build(in, sc->bvh, s, nullptr);
+    bvh_build(in, sc->bvh, s, nullptr, true);
     DScene& v = sc->view;
     v.nodes = sc->bvh.nodes.as<WideNode>();
     v.tri = sc->bvh.tri.as<float4>();
@@ -258,6 +267,76 @@ void upload_shading(olhip_scene* sc, uint32_t material_count, const float* mater
 
 }  // namespace
 
+namespace {
+
+uint64_t instanced_hash(const olhip_instanced_desc* d, const std::vector<uint4>& vidx) {
+  const uint32_t R = d->resource_count, I = d->instance_count;
+  uint64_t h = fnv1a(vidx.data(), sizeof(uint4) * vidx.size(), kFnvBasis);
+  h = fnv1a(d->vertex_offsets, 4ull * (R + 1), h);
+  h = fnv1a(d->triangle_offsets, 4ull * (R + 1), h);
+  h = fnv1a(&I, 4, h);
+  h = fnv1a(d->instance_resource, 4ull * I, h);
+  h = fnv1a(d->instance_triangle_offset, 4ull * I, h);
+  return h;
+}
+
+// Instance records and the top-level BVH over the instances' world bounds
+// (a box is the bounds of the degenerate triangle (min, max, min)); sets the
+// scene view.  The bottom levels must be built.
+void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t s) {
+  const uint32_t R = d->resource_count, I = d->instance_count;
+  std::vector<InstanceDev> inst(I);
+  std::vector<float4> tpos(2ull * I);
+  std::vector<uint4> tvidx(I);
+  uint64_t tri_total = 0;
+  for (uint32_t k = 0; k < I; ++k) {
+    require(d->instance_resource[k] < R, "instance resource out of range");
+    require(d->instance_material[k] < d->material_count, "instance material out of range");
+    InstanceDev& x = inst[k];
+    for (int row = 0; row < 3; ++row) {
+      const float* w = d->world_to_object + 12ull * k + 4 * row;
+      const float* n = d->normal_matrix + 9ull * k + 3 * row;
+      x.w2o[row] = make_float4(w[0], w[1], w[2], w[3]);
+      x.nrm[row] = make_float4(n[0], n[1], n[2], 0.0f);
+    }
+    x.nrm[0].w = d->determinant[k];
+    x.blas = d->instance_resource[k];
+    x.material = d->instance_material[k];
+    x.tri_offset = d->instance_triangle_offset[k];
+    x.id = d->instance_id[k];
+    const float* b = d->world_bounds + 6ull * k;
+    for (int a = 0; a < 6; ++a) require(std::isfinite(b[a]), "instance bounds must be finite");
+    tpos[2 * k] = make_float4(b[0], b[1], b[2], 0.0f);
+    tpos[2 * k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
+    tvidx[k] = make_uint4(2 * k, 2 * k + 1, 2 * k, 0u);
+    tri_total += d->triangle_offsets[x.blas + 1] - d->triangle_offsets[x.blas];
+  }
+  upload(sc->tlas_pos, tpos.data(), tpos.size(), s);
+  upload(sc->tlas_vidx, tvidx.data(), tvidx.size(), s);
+  upload(sc->inst_table, inst.data(), inst.size(), s);
+  OL_HIP_CHECK(hipStreamSynchronize(s));
+  BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u};
+  bvh_build(tin, sc->bvh, s, nullptr);
+  sc->tlas_depth = sc->bvh.depth;
+  // Stack sizing: both levels plus the sentinel.
+  sc->bvh.depth = sc->tlas_depth + sc->blas_depth + 1u;
+  sc->instance_count = I;
+  sc->tri_count = (uint32_t)std::min<uint64_t>(tri_total, 0xFFFFFFFFull);
+  DScene& v = sc->view;
+  v.nodes = sc->bvh.nodes.as<WideNode>();
+  v.tri = sc->bvh.tri.as<float4>();
+  v.tri_vidx = nullptr;
+  v.normals = nullptr;
+  v.tri_count = sc->tri_count;
+  v.node_count = sc->bvh.node_count;
+  v.instanced = 1u;
+  v.instance_count = I;
+  v.inst = sc->inst_table.as<InstanceDev>();
+  v.blas = sc->blas_table.as<BlasDev>();
+}
+
+}  // namespace
+
 int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, olhip_scene** out) {
   return guarded([&] {
     require(ctx && d && out, "ctx, desc and out must not be NULL");
@@ -297,6 +376,10 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, 
     upload(sc->vidx, vidx.data(), vidx.size(), s);
     upload_shading(sc.get(), d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
                    d->area_light_count, d->area_lights, d->area_light_weight, s);
+    sc->leaf_size = d->leaf_size;
+    sc->vertex_offsets.assign(d->vertex_offsets, d->vertex_offsets + R + 1);
+    sc->triangle_offsets.assign(d->triangle_offsets, d->triangle_offsets + R + 1);
+    sc->structure_hash = instanced_hash(d, vidx);
     OL_HIP_CHECK(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
     // Bottom level: one BVH per resource, object space.
@@ -308,68 +391,136 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, 
       const uint32_t nt = d->triangle_offsets[r + 1] - to;
       if (nt) {
         BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, d->leaf_size};
-        bvh_build(in, *out_r, s, nullptr);
+        bvh_build(in, *out_r, s, nullptr, true);
       }
       blas[r] = BlasDev{out_r->nodes.as<WideNode>(), out_r->tri.as<float4>(), sc->vidx.as<uint4>() + to,
                         sc->normals.as<float4>() + vo, out_r->node_count, nt, 0u, 0u};
       blas_depth = std::max(blas_depth, out_r->depth);
       sc->blas.push_back(std::move(out_r));
     }
-    // Instances and the top level over their world bounds (a box is the
-    // bounds of the degenerate triangle (min, max, min)).
-    const uint32_t I = d->instance_count;
-    std::vector<InstanceDev> inst(I);
-    std::vector<float4> tpos(2ull * I);
-    std::vector<uint4> tvidx(I);
-    uint64_t tri_total = 0;
-    for (uint32_t k = 0; k < I; ++k) {
-      require(d->instance_resource[k] < R, "instance resource out of range");
-      require(d->instance_material[k] < d->material_count, "instance material out of range");
-      InstanceDev& x = inst[k];
-      for (int row = 0; row < 3; ++row) {
-        const float* w = d->world_to_object + 12ull * k + 4 * row;
-        const float* n = d->normal_matrix + 9ull * k + 3 * row;
-        x.w2o[row] = make_float4(w[0], w[1], w[2], w[3]);
-        x.nrm[row] = make_float4(n[0], n[1], n[2], 0.0f);
-      }
-      x.nrm[0].w = d->determinant[k];
-      x.blas = d->instance_resource[k];
-      x.material = d->instance_material[k];
-      x.tri_offset = d->instance_triangle_offset[k];
-      x.id = d->instance_id[k];
-      const float* b = d->world_bounds + 6ull * k;
-      for (int a = 0; a < 6; ++a) require(std::isfinite(b[a]), "instance bounds must be finite");
-      tpos[2 * k] = make_float4(b[0], b[1], b[2], 0.0f);
-      tpos[2 * k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
-      tvidx[k] = make_uint4(2 * k, 2 * k + 1, 2 * k, 0u);
-      tri_total += d->triangle_offsets[x.blas + 1] - d->triangle_offsets[x.blas];
-    }
-    upload(sc->tlas_pos, tpos.data(), tpos.size(), s);
-    upload(sc->tlas_vidx, tvidx.data(), tvidx.size(), s);
-    upload(sc->inst_table, inst.data(), inst.size(), s);
     upload(sc->blas_table, blas.data(), blas.size(), s);
-    OL_HIP_CHECK(hipStreamSynchronize(s));
-    BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u};
-    bvh_build(tin, sc->bvh, s, nullptr);
-    sc->tlas_depth = sc->bvh.depth;
     sc->blas_depth = blas_depth;
-    // Stack sizing: both levels plus the sentinel.
-    sc->bvh.depth = sc->tlas_depth + blas_depth + 1u;
+    build_top_level(sc.get(), d, s);
     sc->bvh.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    sc->instance_count = I;
-    sc->tri_count = (uint32_t)std::min<uint64_t>(tri_total, 0xFFFFFFFFull);
-    DScene& v = sc->view;
-    v.nodes = sc->bvh.nodes.as<WideNode>();
-    v.tri = sc->bvh.tri.as<float4>();
-    v.tri_vidx = nullptr;
-    v.normals = nullptr;
-    v.tri_count = sc->tri_count;
-    v.node_count = sc->bvh.node_count;
-    v.instanced = 1u;
-    v.instance_count = I;
-    v.inst = sc->inst_table.as<InstanceDev>();
-    v.blas = sc->blas_table.as<BlasDev>();
     *out = sc.release();
+  });
+}
+
+int olhip_scene_update(olhip_ctx* ctx, olhip_scene* sc, const olhip_scene_desc* d) {
+  return guarded([&] {
+    require(ctx && sc && d, "ctx, scene and desc must not be NULL");
+    require(sc->ctx == ctx, "scene belongs to another context");
+    require(!sc->view.instanced, "two-level scenes are updated with olhip_scene_update_instanced");
+    require(d->vertex_count == sc->vertex_count && d->triangle_count == sc->tri_count,
+            "an update keeps the vertex and triangle counts (create a new scene instead)");
+    require(d->triangle_count == 0 || (d->indices && d->material_ids), "indices and material ids are required");
+    require(d->material_count > 0 && d->materials, "at least one material is required");
+    std::vector<uint4> vidx(d->triangle_count);
+    for (uint32_t t = 0; t < d->triangle_count; ++t) {
+      require(d->material_ids[t] < d->material_count, "material id out of range");
+      vidx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2], d->material_ids[t]);
+    }
+    require(fnv1a(d->indices, 12ull * d->triangle_count, kFnvBasis) == sc->structure_hash,
+            "an update keeps every triangle's vertex indices (create a new scene instead)");
+    std::vector<float4> pos, nrm;
+    if (d->positions) {
+      require(d->vertex_count == 0 || d->normals, "normals missing");
+      pos.resize(d->vertex_count);
+      nrm.resize(d->vertex_count);
+      for (uint32_t i = 0; i < d->vertex_count; ++i) {
+        pos[i] = make_float4(d->positions[3 * i], d->positions[3 * i + 1], d->positions[3 * i + 2], 0.0f);
+        nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
+        require(std::isfinite(pos[i].x) && std::isfinite(pos[i].y) && std::isfinite(pos[i].z),
+                "vertex positions must be finite");
+      }
+    }
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    // Work already queued on the old contents finishes first.
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    upload(sc->vidx, vidx.data(), vidx.size(), s);  // material ids may change
+    if (d->positions) {
+      upload(sc->positions, pos.data(), pos.size(), s);
+      upload(sc->normals, nrm.data(), nrm.size(), s);
+      OL_HIP_CHECK(hipStreamSynchronize(s));
+      BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, sc->leaf_size};
+      bvh_refit(in, sc->bvh, s);
+      DScene& v = sc->view;
+      v.nodes = sc->bvh.nodes.as<WideNode>();
+      v.tri = sc->bvh.tri.as<float4>();
+      v.normals = sc->normals.as<float4>();
+      v.node_count = sc->bvh.node_count;
+    }
+    sc->has_transmission = false;
+    upload_shading(sc, d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
+                   d->area_light_count, d->area_lights, d->area_light_weight, s);
+    sc->point_count = d->analytic_light_count;
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int olhip_scene_update_instanced(olhip_ctx* ctx, olhip_scene* sc, const olhip_instanced_desc* d) {
+  return guarded([&] {
+    require(ctx && sc && d, "ctx, scene and desc must not be NULL");
+    require(sc->ctx == ctx, "scene belongs to another context");
+    require(sc->view.instanced, "flat scenes are updated with olhip_scene_update");
+    const uint32_t R = d->resource_count;
+    require(R == sc->blas.size() && d->vertex_offsets && d->triangle_offsets,
+            "an update keeps the mesh resources (create a new scene instead)");
+    require(d->instance_count == sc->instance_count && d->instance_resource && d->instance_material &&
+                d->instance_triangle_offset && d->instance_id && d->world_to_object && d->normal_matrix &&
+                d->determinant && d->world_bounds,
+            "an update keeps the instance count (create a new scene instead)");
+    for (uint32_t r = 0; r <= R; ++r)
+      require(d->vertex_offsets[r] == sc->vertex_offsets[r] && d->triangle_offsets[r] == sc->triangle_offsets[r],
+              "an update keeps the resources' vertex and triangle counts");
+    const uint32_t V = d->vertex_offsets[R], T = d->triangle_offsets[R];
+    require(T == 0 || d->indices, "indices missing");
+    std::vector<uint4> vidx(T);
+    for (uint32_t t = 0; t < T; ++t) vidx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2], 0u);
+    require(instanced_hash(d, vidx) == sc->structure_hash,
+            "an update keeps the triangles and each instance's resource (create a new scene instead)");
+    std::vector<float4> pos, nrm;
+    if (d->positions) {
+      require(V == 0 || d->normals, "normals missing");
+      pos.resize(V);
+      nrm.resize(V);
+      for (uint32_t i = 0; i < V; ++i) {
+        pos[i] = make_float4(d->positions[3 * i], d->positions[3 * i + 1], d->positions[3 * i + 2], 0.0f);
+        nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
+        require(std::isfinite(pos[i].x) && std::isfinite(pos[i].y) && std::isfinite(pos[i].z),
+                "vertex positions must be finite");
+      }
+    }
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    if (d->positions) {
+      upload(sc->positions, pos.data(), pos.size(), s);
+      upload(sc->normals, nrm.data(), nrm.size(), s);
+      OL_HIP_CHECK(hipStreamSynchronize(s));
+      std::vector<BlasDev> blas(R);
+      uint32_t blas_depth = 0;
+      for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t vo = d->vertex_offsets[r], to = d->triangle_offsets[r];
+        const uint32_t nt = d->triangle_offsets[r + 1] - to;
+        BvhOutput& b = *sc->blas[r];
+        if (nt) {
+          BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, sc->leaf_size};
+          bvh_refit(in, b, s);
+        }
+        blas[r] = BlasDev{b.nodes.as<WideNode>(), b.tri.as<float4>(), sc->vidx.as<uint4>() + to,
+                          sc->normals.as<float4>() + vo, b.node_count, nt, 0u, 0u};
+        blas_depth = std::max(blas_depth, b.depth);
+      }
+      upload(sc->blas_table, blas.data(), blas.size(), s);
+      sc->blas_depth = blas_depth;
+    }
+    sc->has_transmission = false;
+    upload_shading(sc, d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
+                   d->area_light_count, d->area_lights, d->area_light_weight, s);
+    sc->point_count = d->analytic_light_count;
+    build_top_level(sc, d, s);
   });
 }
 

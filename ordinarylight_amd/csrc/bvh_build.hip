@@ -453,78 +453,46 @@ inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
 }  // namespace
 
-void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debug) {
-  auto t0 = std::chrono::steady_clock::now();
-  const uint32_t n = in.tri_count;
-  if (n == 0) {
-    out.node_count = 0;
-    out.depth = 0;
-    out.nodes.reserve(64);
-    out.tri.reserve(48);
-    return;
-  }
-  if (n > OL_LEAF_FIRST_MASK) throw OlError{-1, "too many triangles for one BVH (max 134217727)"};
-  const uint32_t leaf_size = in.leaf_size == 0 ? 4u : (in.leaf_size > OL_LEAF_MAX ? OL_LEAF_MAX : in.leaf_size);
+namespace {
 
-  DevBuf tmin, tmax, cb, codes, vals, codes_tmp, vals_tmp;
+// Boxes of every leaf and binary node for the current vertex positions, the
+// wide collapse, and the leaf-order triangles (steps 5-7 of the build).
+void build_boxes(const BvhInput& in, const BvhTopology& tp, BvhOutput& out, hipStream_t s, BvhDebug* debug) {
+  const uint32_t n = in.tri_count;
+  const uint32_t leaf_size = tp.leaf_size;
+  DevBuf tmin, tmax, cb;
   tmin.reserve(sizeof(float4) * n);
   tmax.reserve(sizeof(float4) * n);
   cb.reserve(sizeof(int) * 8);
-  codes.reserve(sizeof(uint32_t) * n);
-  vals.reserve(sizeof(uint32_t) * n);
-  codes_tmp.reserve(sizeof(uint32_t) * n);
-  vals_tmp.reserve(sizeof(uint32_t) * n);
-
   hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, cb.as<int>());
   hipLaunchKernelGGL(k_tri_bounds, grid_for(n), dim3(kBlock), 0, s, in.positions, in.tri_vidx, n, tmin.as<float4>(),
                      tmax.as<float4>(), cb.as<int>());
-  hipLaunchKernelGGL(k_morton, grid_for(n), dim3(kBlock), 0, s, tmin.as<float4>(), tmax.as<float4>(), n, cb.as<int>(),
-                     codes.as<uint32_t>(), vals.as<uint32_t>());
-  if (debug) {
-    OL_HIP_CHECK(hipStreamSynchronize(s));
-    debug->morton.resize(n);
-    OL_HIP_CHECK(hipMemcpy(debug->morton.data(), codes.ptr, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-  }
-  radix_sort_pairs(codes.as<uint32_t>(), vals.as<uint32_t>(), codes_tmp.as<uint32_t>(), vals_tmp.as<uint32_t>(), n, s,
-                   32);
-  codes_tmp.release();
-  vals_tmp.release();
-
-  // Binary tree.
   const uint32_t ni = n > 1 ? n - 1 : 1;
-  DevBuf left, right, rfirst, rlast, par_int, par_leaf, lmin, lmax, bmin, bmax, flags;
-  left.reserve(4ull * ni);
-  right.reserve(4ull * ni);
-  rfirst.reserve(4ull * ni);
-  rlast.reserve(4ull * ni);
-  par_int.reserve(4ull * ni);
-  par_leaf.reserve(4ull * n);
+  DevBuf lmin, lmax, bmin, bmax, flags;
   lmin.reserve(sizeof(float4) * n);
   lmax.reserve(sizeof(float4) * n);
   bmin.reserve(sizeof(float4) * ni);
   bmax.reserve(sizeof(float4) * ni);
   flags.reserve(4ull * ni);
-  hipLaunchKernelGGL(k_leaf_boxes, grid_for(n), dim3(kBlock), 0, s, vals.as<uint32_t>(), n, tmin.as<float4>(),
+  hipLaunchKernelGGL(k_leaf_boxes, grid_for(n), dim3(kBlock), 0, s, tp.sorted.as<uint32_t>(), n, tmin.as<float4>(),
                      tmax.as<float4>(), lmin.as<float4>(), lmax.as<float4>());
   if (n > 1) {
     OL_HIP_CHECK(hipMemsetAsync(flags.ptr, 0, 4ull * ni, s));
-    hipLaunchKernelGGL(k_karras, grid_for(n - 1), dim3(kBlock), 0, s, codes.as<uint32_t>(), (int)n,
-                       left.as<uint32_t>(), right.as<uint32_t>(), rfirst.as<uint32_t>(), rlast.as<uint32_t>(),
-                       par_int.as<uint32_t>(), par_leaf.as<uint32_t>());
-    hipLaunchKernelGGL(k_bottom_up, grid_for(n), dim3(kBlock), 0, s, n, left.as<uint32_t>(), right.as<uint32_t>(),
-                       par_int.as<uint32_t>(), par_leaf.as<uint32_t>(), lmin.as<float4>(), lmax.as<float4>(),
-                       bmin.as<float4>(), bmax.as<float4>(), flags.as<uint32_t>());
+    hipLaunchKernelGGL(k_bottom_up, grid_for(n), dim3(kBlock), 0, s, n, tp.left.as<uint32_t>(),
+                       tp.right.as<uint32_t>(), tp.parent_int.as<uint32_t>(), tp.parent_leaf.as<uint32_t>(),
+                       lmin.as<float4>(), lmax.as<float4>(), bmin.as<float4>(), bmax.as<float4>(),
+                       flags.as<uint32_t>());
   }
   if (debug) {
     OL_HIP_CHECK(hipStreamSynchronize(s));
     debug->sorted_index.resize(n);
-    OL_HIP_CHECK(hipMemcpy(debug->sorted_index.data(), vals.ptr, 4ull * n, hipMemcpyDeviceToHost));
+    OL_HIP_CHECK(hipMemcpy(debug->sorted_index.data(), tp.sorted.ptr, 4ull * n, hipMemcpyDeviceToHost));
     if (n > 1) {
       debug->left.resize(n - 1);
       debug->right.resize(n - 1);
       debug->node_box.resize(8ull * (n - 1));
-      OL_HIP_CHECK(hipMemcpy(debug->left.data(), left.ptr, 4ull * (n - 1), hipMemcpyDeviceToHost));
-      OL_HIP_CHECK(hipMemcpy(debug->right.data(), right.ptr, 4ull * (n - 1), hipMemcpyDeviceToHost));
+      OL_HIP_CHECK(hipMemcpy(debug->left.data(), tp.left.ptr, 4ull * (n - 1), hipMemcpyDeviceToHost));
+      OL_HIP_CHECK(hipMemcpy(debug->right.data(), tp.right.ptr, 4ull * (n - 1), hipMemcpyDeviceToHost));
       std::vector<float4> a(n - 1), b(n - 1);
       OL_HIP_CHECK(hipMemcpy(a.data(), bmin.ptr, sizeof(float4) * (n - 1), hipMemcpyDeviceToHost));
       OL_HIP_CHECK(hipMemcpy(b.data(), bmax.ptr, sizeof(float4) * (n - 1), hipMemcpyDeviceToHost));
@@ -537,7 +505,7 @@ void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debu
   }
 
   // Wide collapse, one breadth-first level per iteration.
-  TreeView tv{left.as<uint32_t>(), right.as<uint32_t>(), rfirst.as<uint32_t>(), rlast.as<uint32_t>(),
+  TreeView tv{tp.left.as<uint32_t>(), tp.right.as<uint32_t>(), tp.rfirst.as<uint32_t>(), tp.rlast.as<uint32_t>(),
               lmin.as<float4>(), lmax.as<float4>(), bmin.as<float4>(), bmax.as<float4>(), leaf_size,
               n > 1 ? 0u : OL_LEAF_BIT};
   out.nodes.reserve(sizeof(WideNode) * (size_t)(n + 1));
@@ -569,9 +537,78 @@ void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debu
   out.depth = depth;
 
   out.tri.reserve(sizeof(float4) * 3ull * n);
-  hipLaunchKernelGGL(k_reorder, grid_for(n), dim3(kBlock), 0, s, vals.as<uint32_t>(), n, in.positions, in.tri_vidx,
-                     out.tri.as<float4>());
+  hipLaunchKernelGGL(k_reorder, grid_for(n), dim3(kBlock), 0, s, tp.sorted.as<uint32_t>(), n, in.positions,
+                     in.tri_vidx, out.tri.as<float4>());
   OL_HIP_CHECK(hipGetLastError());
   OL_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debug, bool keep_topology) {
+  auto t0 = std::chrono::steady_clock::now();
+  const uint32_t n = in.tri_count;
+  out.topology.reset();
+  if (n == 0) {
+    out.node_count = 0;
+    out.depth = 0;
+    out.nodes.reserve(64);
+    out.tri.reserve(48);
+    return;
+  }
+  if (n > OL_LEAF_FIRST_MASK) throw OlError{-1, "too many triangles for one BVH (max 134217727)"};
+  auto tp = std::make_unique<BvhTopology>();
+  tp->tri_count = n;
+  tp->leaf_size = in.leaf_size == 0 ? 4u : (in.leaf_size > OL_LEAF_MAX ? OL_LEAF_MAX : in.leaf_size);
+
+  // Steps 1-4: bounds, Morton codes, radix sort, Karras hierarchy.
+  DevBuf tmin, tmax, cb, codes, codes_tmp, vals_tmp;
+  tmin.reserve(sizeof(float4) * n);
+  tmax.reserve(sizeof(float4) * n);
+  cb.reserve(sizeof(int) * 8);
+  codes.reserve(sizeof(uint32_t) * n);
+  tp->sorted.reserve(sizeof(uint32_t) * n);
+  codes_tmp.reserve(sizeof(uint32_t) * n);
+  vals_tmp.reserve(sizeof(uint32_t) * n);
+  hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, cb.as<int>());
+  hipLaunchKernelGGL(k_tri_bounds, grid_for(n), dim3(kBlock), 0, s, in.positions, in.tri_vidx, n, tmin.as<float4>(),
+                     tmax.as<float4>(), cb.as<int>());
+  hipLaunchKernelGGL(k_morton, grid_for(n), dim3(kBlock), 0, s, tmin.as<float4>(), tmax.as<float4>(), n, cb.as<int>(),
+                     codes.as<uint32_t>(), tp->sorted.as<uint32_t>());
+  if (debug) {
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    debug->morton.resize(n);
+    OL_HIP_CHECK(hipMemcpy(debug->morton.data(), codes.ptr, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  }
+  radix_sort_pairs(codes.as<uint32_t>(), tp->sorted.as<uint32_t>(), codes_tmp.as<uint32_t>(),
+                   vals_tmp.as<uint32_t>(), n, s, 32);
+  codes_tmp.release();
+  vals_tmp.release();
+  tmin.release();
+  tmax.release();
+  const uint32_t ni = n > 1 ? n - 1 : 1;
+  tp->left.reserve(4ull * ni);
+  tp->right.reserve(4ull * ni);
+  tp->rfirst.reserve(4ull * ni);
+  tp->rlast.reserve(4ull * ni);
+  tp->parent_int.reserve(4ull * ni);
+  tp->parent_leaf.reserve(4ull * n);
+  if (n > 1)
+    hipLaunchKernelGGL(k_karras, grid_for(n - 1), dim3(kBlock), 0, s, codes.as<uint32_t>(), (int)n,
+                       tp->left.as<uint32_t>(), tp->right.as<uint32_t>(), tp->rfirst.as<uint32_t>(),
+                       tp->rlast.as<uint32_t>(), tp->parent_int.as<uint32_t>(), tp->parent_leaf.as<uint32_t>());
+  // Steps 5-7.
+  build_boxes(in, *tp, out, s, debug);
+  if (keep_topology) out.topology = std::move(tp);
+  out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void bvh_refit(const BvhInput& in, BvhOutput& out, hipStream_t s) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (in.tri_count == 0) return;
+  if (!out.topology) throw OlError{-1, "the BVH was built without a kept topology"};
+  const BvhTopology& tp = *out.topology;
+  if (tp.tri_count != in.tri_count) throw OlError{-1, "refit needs the triangle count of the build"};
+  build_boxes(in, tp, out, s, nullptr);
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

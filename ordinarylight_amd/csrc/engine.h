@@ -40,6 +40,11 @@ struct olhip_scene {
   bool has_transmission = false;
   uint32_t point_count = 0;
   std::vector<float> materials_host;
+  // Fingerprint of the triangle structure (indices, material ids, resource
+  // and instance layout): updates must keep it.
+  uint64_t structure_hash = 0;
+  uint32_t leaf_size = 0;
+  std::vector<uint32_t> vertex_offsets, triangle_offsets;  // two-level: per resource
 };
 
 // Shadow entries one shaded hit of this job may emit (1 for unified NEE

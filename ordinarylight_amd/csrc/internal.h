@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,16 @@ struct BvhInput {
   uint32_t leaf_size;       // 1..16
 };
 
+// Binary LBVH topology of a build (Morton order and Karras hierarchy), kept
+// so that a refit can recompute boxes for moved vertices without re-sorting.
+struct BvhTopology {
+  uint32_t tri_count = 0, leaf_size = 0;
+  DevBuf sorted;                  // [T] original triangle of each leaf position
+  DevBuf left, right;             // [T-1] children (leaf refs | OL_LEAF_BIT)
+  DevBuf rfirst, rlast;           // [T-1] leaf range of each internal node
+  DevBuf parent_int, parent_leaf; // parents of internal nodes / leaves
+};
+
 // Output of a build.  All arrays are device-resident and owned by the caller.
 struct BvhOutput {
   DevBuf nodes;      // WideNode[node_count]
@@ -66,6 +77,7 @@ struct BvhOutput {
   uint32_t node_count = 0;
   uint32_t depth = 0;      // wide levels (root = level 1)
   double build_ms = 0.0;
+  std::unique_ptr<BvhTopology> topology;  // kept when the build asked for it (refit)
 };
 
 // Optional debug capture of intermediate build arrays (bit-exact checks).
@@ -76,7 +88,13 @@ struct BvhDebug {
   std::vector<float> node_box;        // [T-1][8] binary internal boxes (min.xyzw, max.xyzw)
 };
 
-void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t stream, BvhDebug* debug);
+// keep_topology: retain the binary topology in out.topology for bvh_refit.
+void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t stream, BvhDebug* debug, bool keep_topology = false);
+// Recomputes the boxes, the wide nodes and the leaf triangles of a kept
+// topology for new vertex positions (same triangles, same indices), as an
+// acceleration-structure update does: the leaf order stays, boxes follow the
+// geometry.  in.tri_count and in.leaf_size must match the build.
+void bvh_refit(const BvhInput& in, BvhOutput& out, hipStream_t stream);
 
 // ---- scans ---------------------------------------------------------------------
 // Exclusive prefix sum of `n` uint32 values in place; writes the total to

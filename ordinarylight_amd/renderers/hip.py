@@ -175,6 +175,57 @@ class DeviceContext:
             self.handle = None
 
 
+def _scene_desc(packed, leaf_size):
+    desc = N.SceneDesc()
+    desc.vertex_count = len(packed.positions)
+    desc.positions = N.ptr(packed.positions)
+    desc.normals = N.ptr(packed.normals)
+    desc.triangle_count = len(packed.indices)
+    desc.indices = N.ptr(packed.indices, N.ctypes.c_uint32)
+    desc.material_ids = N.ptr(packed.material_ids, N.ctypes.c_uint32)
+    desc.material_count = len(packed.materials)
+    desc.materials = N.ptr(packed.materials)
+    desc.analytic_light_count = packed.analytic_light_count
+    desc.analytic_lights = N.ptr(packed.analytic_lights)
+    desc.area_light_count = packed.area_light_count
+    desc.area_lights = N.ptr(packed.area_lights)
+    desc.area_light_weight = packed.area_light_weight
+    desc.leaf_size = int(leaf_size)
+    return desc
+
+
+def _instanced_desc(p, leaf_size):
+    d = N.InstancedDesc()
+    d.resource_count = len(p.vertex_offsets) - 1
+    d.vertex_offsets = N.ptr(p.vertex_offsets, N.ctypes.c_uint32)
+    d.triangle_offsets = N.ptr(p.triangle_offsets, N.ctypes.c_uint32)
+    d.positions = N.ptr(p.positions)
+    d.normals = N.ptr(p.normals)
+    d.indices = N.ptr(p.indices, N.ctypes.c_uint32)
+    d.instance_count = len(p.instance_resource)
+    d.instance_resource = N.ptr(p.instance_resource, N.ctypes.c_uint32)
+    d.instance_material = N.ptr(p.instance_material, N.ctypes.c_uint32)
+    d.instance_triangle_offset = N.ptr(p.instance_triangle_offset, N.ctypes.c_uint32)
+    d.instance_id = N.ptr(p.instance_id, N.ctypes.c_uint32)
+    d.world_to_object = N.ptr(p.world_to_object)
+    d.normal_matrix = N.ptr(p.normal_matrix)
+    d.determinant = N.ptr(p.determinant)
+    d.world_bounds = N.ptr(p.world_bounds)
+    d.material_count = len(p.materials)
+    d.materials = N.ptr(p.materials)
+    d.analytic_light_count = p.analytic_light_count
+    d.analytic_lights = N.ptr(p.analytic_lights)
+    d.area_light_count = p.area_light_count
+    d.area_lights = N.ptr(p.area_lights)
+    d.area_light_weight = p.area_light_weight
+    d.leaf_size = int(leaf_size)
+    return d
+
+
+def _same(a, b):
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
 class DeviceScene:
     """A scene resident in HBM: packed arrays + the GPU-built BVH.
 
@@ -187,6 +238,7 @@ class DeviceScene:
     def __init__(self, context, scene, leaf_size=4, instanced=None):
         self._lib = N.library()
         self.context = context
+        self.leaf_size = int(leaf_size)
         if instanced is None:
             instanced = shares_resources(scene)
         self.instanced = bool(instanced)
@@ -197,21 +249,7 @@ class DeviceScene:
         packed = pack_scene(scene)
         self.packed = packed
         pack_ms = (time.perf_counter() - started) * 1000.0
-        desc = N.SceneDesc()
-        desc.vertex_count = len(packed.positions)
-        desc.positions = N.ptr(packed.positions)
-        desc.normals = N.ptr(packed.normals)
-        desc.triangle_count = len(packed.indices)
-        desc.indices = N.ptr(packed.indices, N.ctypes.c_uint32)
-        desc.material_ids = N.ptr(packed.material_ids, N.ctypes.c_uint32)
-        desc.material_count = len(packed.materials)
-        desc.materials = N.ptr(packed.materials)
-        desc.analytic_light_count = packed.analytic_light_count
-        desc.analytic_lights = N.ptr(packed.analytic_lights)
-        desc.area_light_count = packed.area_light_count
-        desc.area_lights = N.ptr(packed.area_lights)
-        desc.area_light_weight = packed.area_light_weight
-        desc.leaf_size = int(leaf_size)
+        desc = _scene_desc(packed, leaf_size)
         handle = N.ctypes.c_void_p()
         started = time.perf_counter()
         N.check(self._lib.olhip_scene_create(context.handle, N.ctypes.byref(desc), N.ctypes.byref(handle)))
@@ -227,30 +265,7 @@ class DeviceScene:
         p = pack_instanced_scene(scene)
         self.packed = p
         pack_ms = (time.perf_counter() - started) * 1000.0
-        d = N.InstancedDesc()
-        d.resource_count = len(p.vertex_offsets) - 1
-        d.vertex_offsets = N.ptr(p.vertex_offsets, N.ctypes.c_uint32)
-        d.triangle_offsets = N.ptr(p.triangle_offsets, N.ctypes.c_uint32)
-        d.positions = N.ptr(p.positions)
-        d.normals = N.ptr(p.normals)
-        d.indices = N.ptr(p.indices, N.ctypes.c_uint32)
-        d.instance_count = len(p.instance_resource)
-        d.instance_resource = N.ptr(p.instance_resource, N.ctypes.c_uint32)
-        d.instance_material = N.ptr(p.instance_material, N.ctypes.c_uint32)
-        d.instance_triangle_offset = N.ptr(p.instance_triangle_offset, N.ctypes.c_uint32)
-        d.instance_id = N.ptr(p.instance_id, N.ctypes.c_uint32)
-        d.world_to_object = N.ptr(p.world_to_object)
-        d.normal_matrix = N.ptr(p.normal_matrix)
-        d.determinant = N.ptr(p.determinant)
-        d.world_bounds = N.ptr(p.world_bounds)
-        d.material_count = len(p.materials)
-        d.materials = N.ptr(p.materials)
-        d.analytic_light_count = p.analytic_light_count
-        d.analytic_lights = N.ptr(p.analytic_lights)
-        d.area_light_count = p.area_light_count
-        d.area_lights = N.ptr(p.area_lights)
-        d.area_light_weight = p.area_light_weight
-        d.leaf_size = int(leaf_size)
+        d = _instanced_desc(p, leaf_size)
         handle = N.ctypes.c_void_p()
         started = time.perf_counter()
         N.check(self._lib.olhip_scene_create_instanced(self.context.handle, N.ctypes.byref(d), N.ctypes.byref(handle)))
@@ -260,6 +275,45 @@ class DeviceScene:
         N.check(self._lib.olhip_scene_info_get(handle, N.ctypes.byref(info)))
         self.info = info
         self.timings = {"scene_pack_ms": pack_ms, "scene_upload_ms": upload_ms, "bvh_ms": info.build_ms}
+
+    def update(self, scene):
+        """Bring the resident copy up to date with ``scene`` in place when its
+        triangles are unchanged (the reference's partial update,
+        ``targets/vulkan/core.py:4696-4870``): moved vertices refit the BVH,
+        moved instances rebuild the top level, materials and lights are
+        replaced.  Returns False, changing nothing, when the structure
+        differs and the scene must be rebuilt."""
+        started = time.perf_counter()
+        old = self.packed
+        if self.instanced:
+            if not shares_resources(scene):
+                return False
+            p = pack_instanced_scene(scene)
+            keys = ("vertex_offsets", "triangle_offsets", "indices", "instance_resource", "instance_triangle_offset")
+            if not all(_same(getattr(p, k), getattr(old, k)) for k in keys):
+                return False
+            d = _instanced_desc(p, self.leaf_size)
+            update = self._lib.olhip_scene_update_instanced
+        else:
+            p = pack_scene(scene)
+            if not (_same(p.indices, old.indices) and len(p.positions) == len(old.positions)):
+                return False
+            d = _scene_desc(p, self.leaf_size)
+            update = self._lib.olhip_scene_update
+        moved = not (_same(p.positions, old.positions) and _same(p.normals, old.normals))
+        if not moved:
+            d.positions = None
+            d.normals = None
+        pack_ms = (time.perf_counter() - started) * 1000.0
+        started = time.perf_counter()
+        N.check(update(self.context.handle, self.handle, N.ctypes.byref(d)))
+        self.packed = p
+        info = N.SceneInfo()
+        N.check(self._lib.olhip_scene_info_get(self.handle, N.ctypes.byref(info)))
+        self.info = info
+        self.timings = {"scene_pack_ms": pack_ms, "scene_update_ms": (time.perf_counter() - started) * 1000.0,
+                        "bvh_ms": info.build_ms if moved else 0.0, "bvh_refit": bool(moved)}
+        return True
 
     @property
     def triangle_count(self):
@@ -300,6 +354,7 @@ class _ResidentMixin:
         self._device_index = int(device_index)
         self._leaf_size = int(leaf_size)
         self._scene_key = None
+        self._scene_ref = None
         self._device_scene = None
         self._lock = threading.RLock()
         self._closed = False
@@ -311,6 +366,12 @@ class _ResidentMixin:
 
     def _resident(self, scene):
         key = (id(scene), getattr(scene, "revision", None))
+        if (self._device_scene is not None and self._scene_key != key and self._scene_key[0] == key[0]
+                and self._scene_ref is scene and hasattr(self._device_scene, "update")
+                and self._device_scene.update(scene)):
+            # Same scene object, same triangles: updated in place.
+            self._scene_key = key
+            return self._device_scene
         if self._device_scene is None or self._scene_key != key:
             previous = self._device_scene
             self._device_scene = DeviceScene(self._ensure_context(), scene, self._leaf_size)

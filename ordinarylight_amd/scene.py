@@ -495,6 +495,54 @@ class Scene:
 
     triangle_object_ids = triangle_instance_ids
 
+    # -- updates (scene/_core.py:1579-1640, :1809-1850) ------------------------
+    def update_mesh_resource(self, resource, *, vertices=None, normals=None):
+        """New vertex positions (same count) and normals of shared geometry;
+        normals are regenerated when only vertices are given."""
+        if resource not in self.mesh_resources:
+            raise KeyError("resource does not belong to this scene")
+        v = resource.vertices if vertices is None else np.ascontiguousarray(vertices, np.float32)
+        if v.shape != resource.vertices.shape:
+            raise ValueError("vertices must keep the resource's vertex count")
+        if not np.all(np.isfinite(v)):
+            raise ValueError("vertices must be finite")
+        if normals is None:
+            n = generate_normals(v, resource.indices) if vertices is not None else resource.normals
+        else:
+            n = np.ascontiguousarray(normals, np.float32)
+            if n.shape != v.shape:
+                raise ValueError("normals must have the same shape as vertices")
+            n = n / np.linalg.norm(n, axis=1, keepdims=True)
+        resource.vertices = v
+        resource.normals = n
+        self._changed()
+        return resource
+
+    def update_instance(self, instance, *, transform=None, material=None):
+        """New placement or material of an instance, keeping its id."""
+        instance = self.get_instance(instance)
+        if transform is not None:
+            instance.transform = transform if isinstance(transform, Transform) else Transform(transform)
+        if material is not None:
+            if not isinstance(material, Material):
+                raise TypeError("material must be a Material")
+            self._register_material(material)
+            instance.material = material
+        self._changed()
+        return instance
+
+    def update_instance_transforms(self, instances, transforms):
+        """Column update of many instances' transforms."""
+        instances = list(instances)
+        transforms = list(transforms)
+        if len(instances) != len(transforms):
+            raise ValueError("instances and transforms must have the same length")
+        for inst, t in zip(instances, transforms):
+            inst = self.get_instance(inst)
+            inst.transform = t if isinstance(t, Transform) else Transform(t)
+        self._changed()
+        return instances
+
     def get_instance(self, reference):
         """The instance with this stable id (or the instance itself)."""
         object_id = int(reference.id if hasattr(reference, "id") else reference)

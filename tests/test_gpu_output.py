@@ -45,14 +45,31 @@ def test_outputs_match_oracle(hip_device, exposure):
     r.close()
 
 
+def _hip_runtime():
+    """The HIP runtime the library itself uses (already loaded in this process)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    return hip
+
+
 def test_output_into_device_memory(hip_device):
-    import torch
+    import ctypes
     scene, cam = workloads.box_room(2)
     r = HipPathTracer(max_bounces=3, device_index=hip_device)
     W, H = 64, 48
     host = r.render_output(scene, cam, W, H, pixel_format="nv12")
-    dev = torch.zeros(H * 3 // 2 * W, dtype=torch.uint8, device=f"cuda:{hip_device}")
-    torch.cuda.synchronize()
-    assert r.render_output(scene, cam, W, H, pixel_format="nv12", device_out=dev.data_ptr()) is None
-    assert np.array_equal(dev.cpu().numpy().reshape(H * 3 // 2, W), host)
+    hip = _hip_runtime()
+    size = H * 3 // 2 * W
+    ptr = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(ptr), size) == 0
+    try:
+        assert r.render_output(scene, cam, W, H, pixel_format="nv12", device_out=ptr.value) is None
+        back = np.zeros((H * 3 // 2, W), np.uint8)
+        assert hip.hipMemcpy(back.ctypes.data_as(ctypes.c_void_p), ptr, size, 2) == 0  # device to host
+        assert np.array_equal(back, host)
+    finally:
+        hip.hipFree(ptr)
     r.close()
