@@ -274,6 +274,8 @@ def main(argv=None):
                         help="Poisson arrival rate as a fraction of the batched jobs/s")
     parser.add_argument("--no-autotune", action="store_true")
     parser.add_argument("--no-c3", action="store_true", help="skip the instanced two-level workload")
+    parser.add_argument("--c3-only", action="store_true",
+                        help="run only the instanced workload and print its record (profiling aid)")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -282,6 +284,13 @@ def main(argv=None):
 
     from ordinarylight_amd import workloads
     from ordinarylight_amd.renderers.hip import HipPathTracer
+
+    if args.c3_only:
+        from ordinarylight_amd.renderers.hip import DeviceContext
+        context = DeviceContext(0)
+        print(json.dumps({"c3": c3_measurement(context, steps=args.steps)}), flush=True)
+        context.close()
+        return 0
 
     t0 = time.perf_counter()
     scene, camera = workloads.terrain(0)
