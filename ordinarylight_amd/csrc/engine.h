@@ -142,9 +142,10 @@ class Engine {
   uint64_t steps_ = 0;
   int max_depth_ = 1;
   bool stk_needed_ = false;
+  bool restir_needed_ = false;
 
   // Device state.
-  DevBuf T_, L_, M_, stk_, qo_[2], qd_[2], qp_[2], hit_, hit_inst_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
+  DevBuf T_, L_, M_, stk_, rsamp_, qo_[2], qd_[2], qp_[2], hit_, hit_inst_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
       list_dev_, spill_;
   uint64_t slots_ = 0, eslots_ = 0;
   Wave wave_{};

@@ -163,6 +163,7 @@ void Engine::ensure_buffers() {
   ed_.reserve(16 * eslots_);
   ec_.reserve(16 * eslots_);
   if (stk_needed_) stk_.reserve(64 * slots_);
+  if (restir_needed_) rsamp_.reserve(48 * slots_);
   Wave w{};
   w.jobs = jobs_dev_.as<JobDev>();
   w.list = list_dev_.as<StepList>();
@@ -170,6 +171,7 @@ void Engine::ensure_buffers() {
   w.L = L_.as<float4>();
   w.M = M_.as<uint4>();
   w.stk = stk_needed_ ? stk_.as<float4>() : nullptr;
+  w.rsamp = restir_needed_ ? rsamp_.as<float4>() : nullptr;
   for (int q = 0; q < 2; ++q) {
     w.qo[q] = qo_[q].as<float4>();
     w.qd[q] = qd_[q].as<float4>();
@@ -360,6 +362,10 @@ bool Engine::admit(Job& j, int slot) {
       slot_job_[slot] = j.id;
       if (j.transmits && !stk_needed_) {
         stk_needed_ = true;
+        ensure_buffers();
+      }
+      if (d.settings.restir_candidates && !restir_needed_) {
+        restir_needed_ = true;
         ensure_buffers();
       }
       const int depth = (int)d.scene->bvh.depth;

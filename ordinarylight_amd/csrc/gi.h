@@ -132,6 +132,10 @@ struct Wave {
   float4* eo;   // shadow entry: origin, tmin
   float4* ed;   // direction, tmax
   float4* ec;   // contribution rgb, kind (0 point, 1 area, 2 environment)
+  // ReSTIR primary samples (null until a ReSTIR job is admitted), 3 x float4
+  // per path: origin and valid flag, direction and tmax, contribution and
+  // the path's random state after the reservoir pass.
+  float4* rsamp;
   uint32_t* ctr;
   unsigned long long* work;  // optional: closest nodes, closest tris, shadow nodes, shadow tris
 };

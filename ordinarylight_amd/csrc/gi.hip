@@ -869,6 +869,104 @@ OL_DEV void restir_nee(const GiParams& P, const NeeSample& chosen, uint32_t& rs,
   }
 }
 
+// Hit reconstruction (shadeLoadHit) and surface (shadeResolveSurface,
+// wave_shade_candidate.glsl:957-997, :1170-1215): position, geometric
+// normal, oriented shading normal, the world-space edge cross product, the
+// original triangle, its material and the entering flag.
+OL_DEV void resolve_surface(const Wave& W, const DScene& sc, uint32_t i, uint32_t slot, float4 ro, float4 rd, float4 hv,
+                            f3& hitp, f3& gn, f3& nrm, f3& cw, uint32_t& prim, uint32_t& mat_id, bool& entering) {
+  const float t = hv.x, bu = hv.y, bv = hv.z;
+  float4 ta, te1, te2;
+  uint4 vi;
+  f3 n0, n1, n2;
+  if (!sc.instanced) {
+    ta = sc.tri[3 * slot];
+    te1 = sc.tri[3 * slot + 1];
+    te2 = sc.tri[3 * slot + 2];
+    prim = __float_as_uint(ta.w);
+    vi = sc.tri_vidx[prim];
+    n0 = rgb(sc.normals[vi.x]);
+    n1 = rgb(sc.normals[vi.y]);
+    n2 = rgb(sc.normals[vi.z]);
+    cw = cross3(mk3(te1), mk3(te2));
+    mat_id = vi.w;
+  } else {
+    // Object-space triangle of the instance: normals and the edge cross
+    // product go to world space through the normal matrix; the cross
+    // product of world edges is det(A) * N * (e1 x e2).
+    const InstanceDev& I = sc.inst[W.hit_inst[i]];
+    const BlasDev& B = sc.blas[I.blas];
+    ta = B.tri[3 * slot];
+    te1 = B.tri[3 * slot + 1];
+    te2 = B.tri[3 * slot + 2];
+    const uint32_t local = __float_as_uint(ta.w);
+    prim = I.tri_offset + local;
+    vi = B.vidx[local];
+    n0 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.x])));
+    n1 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.y])));
+    n2 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.z])));
+    cw = xform_vector(I.nrm, cross3(mk3(te1), mk3(te2))) * I.nrm[0].w;
+    mat_id = I.material;
+  }
+  hitp = mk3(ro.x + t * rd.x, ro.y + t * rd.y, ro.z + t * rd.z);
+  gn = normalize3(cw);
+  const float wx = (1.0f - bu) - bv, wy = bu, wz = bv;
+  f3 sn = normalize3(((n0 * wx) + (n1 * wy)) + (n2 * wz));
+  if (dot3(sn, gn) < 0.0f) sn = -sn;
+  entering = dot3(rgb(rd), gn) < 0.0f;
+  nrm = entering ? sn : -sn;
+}
+
+// ReSTIR pass at primary hits, before shade: for each bounce-0 path of a
+// ReSTIR job, clears the pixel's stored reservoir and surface, and where
+// shade will sample lights (an opaque hit below the bounce limit, chosen for
+// next-event estimation) runs the reservoir pass with the path's random
+// state and leaves the selected light sample and the advanced state for
+// shade.  Kept out of k_shade so that its registers do not lower shade's
+// occupancy for every other job.
+__global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in) {
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
+  const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
+    const uint32_t cu = uniform(c);
+    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, counts, W.jobs, cu)
+                                        : locate_chunk(W.ctr + C_RP2, counts, W.jobs, cu - flat_chunks);
+    const JobDev& J = W.jobs[uniform(k.job)];
+    const GiParams& P = J.P;
+    if (!P.restir) continue;
+    if (lane_id() >= k.count) continue;
+    const uint32_t i = k.first + lane_id();
+    const uint32_t path = W.qp[in][i];
+    const float4 T = W.T[path];
+    if (T.w != 0.0f) continue;
+    const uint4 M = W.M[path];
+    J.res_out[M.x] = reservoir_store(empty_reservoir());
+    J.surf_out[2u * M.x] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    J.surf_out[2u * M.x + 1u] = make_float4(__uint_as_float(OL_NONE), 0.0f, 0.0f, 0.0f);
+    const float4 hv = W.hit[i];
+    const uint32_t slot = __float_as_uint(hv.w);
+    if (fmaxf(T.x, fmaxf(T.y, T.z)) < 0.0001f || slot == OL_NONE || 1u >= P.max_bounces) continue;
+    const DScene& sc = P.sc;
+    const float4 ro = W.qo[in][i], rd = W.qd[in][i];
+    f3 hitp, gn, nrm, cw;
+    uint32_t prim, mat_id;
+    bool entering;
+    resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
+    const Mat mat = load_mat(sc.materials, mat_id);
+    if (mat.at.w > 0.001f) continue;
+    if (!select_secondary_nee(clampf(P.nee_probability, 1e-6f, 1.0f), M.x, M.y, 1u)) continue;
+    uint32_t rs = M.z;
+    NeeSample s;
+    restir_primary(P, J, rs, hitp, nrm, mat, rgb(rd), hv.x, M.x, mat_id, s);
+    float4* out = W.rsamp + 3u * path;
+    out[0] = mk4(s.origin, s.valid ? 1.0f : 0.0f);
+    out[1] = mk4(s.dir, s.tmax);
+    out[2] = mk4(s.c, __uint_as_float(rs));
+  }
+}
+
 // ---- shade -----------------------------------------------------------------------------------
 // One wave per 64-ray chunk (grid-stride over chunks), so the job and its
 // parameters are wave-uniform.  Outputs go to the job's segment
@@ -906,12 +1004,6 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
       const float4 ro = W.qo[in][i], rd = W.qd[in][i], hv = W.hit[i];
       float4 T = W.T[path], L = W.L[path];
       uint4 M = W.M[path];
-      if (P.restir && T.w == 0.0f) {
-        // This pass's history for the pixel: empty unless ReSTIR runs below.
-        J.res_out[M.x] = reservoir_store(empty_reservoir());
-        J.surf_out[2u * M.x] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        J.surf_out[2u * M.x + 1u] = make_float4(__uint_as_float(OL_NONE), 0.0f, 0.0f, 0.0f);
-      }
       incoming = rgb(rd);
       const uint32_t slot = __float_as_uint(hv.w);
       do {
@@ -944,52 +1036,15 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
           break;
         }
         // Hit reconstruction (shadeLoadHit) and surface (shadeResolveSurface).
-        const float t = hv.x, bu = hv.y, bv = hv.z;
-        float4 ta, te1, te2;
-        uint32_t prim;
-        uint4 vi;
-        f3 n0, n1, n2, cw;  // vertex normals and the world-space edge cross product
-        uint32_t mat_id;
-        if (!sc.instanced) {
-          ta = sc.tri[3 * slot];
-          te1 = sc.tri[3 * slot + 1];
-          te2 = sc.tri[3 * slot + 2];
-          prim = __float_as_uint(ta.w);
-          vi = sc.tri_vidx[prim];
-          n0 = rgb(sc.normals[vi.x]);
-          n1 = rgb(sc.normals[vi.y]);
-          n2 = rgb(sc.normals[vi.z]);
-          cw = cross3(mk3(te1), mk3(te2));
-          mat_id = vi.w;
-        } else {
-          // Object-space triangle of the instance: normals and the edge cross
-          // product go to world space through the normal matrix; the cross
-          // product of world edges is det(A) * N * (e1 x e2).
-          const InstanceDev& I = sc.inst[W.hit_inst[i]];
-          const BlasDev& B = sc.blas[I.blas];
-          ta = B.tri[3 * slot];
-          te1 = B.tri[3 * slot + 1];
-          te2 = B.tri[3 * slot + 2];
-          const uint32_t local = __float_as_uint(ta.w);
-          prim = I.tri_offset + local;
-          vi = B.vidx[local];
-          n0 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.x])));
-          n1 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.y])));
-          n2 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.z])));
-          cw = xform_vector(I.nrm, cross3(mk3(te1), mk3(te2))) * I.nrm[0].w;
-          mat_id = I.material;
-        }
-        hitp = mk3(ro.x + t * rd.x, ro.y + t * rd.y, ro.z + t * rd.z);
-        const f3 gn = normalize3(cw);
+        const float t = hv.x;
+        f3 gn, cw;
+        uint32_t prim, mat_id;
+        bool entering;
+        resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
         if (record_primary) {
           J.geo[M.x] = make_float4(t, gn.x, gn.y, gn.z);
           J.prim[M.x] = prim;
         }
-        const float wx = (1.0f - bu) - bv, wy = bu, wz = bv;
-        f3 sn = normalize3(((n0 * wx) + (n1 * wy)) + (n2 * wz));
-        if (dot3(sn, gn) < 0.0f) sn = -sn;
-        const bool entering = dot3(incoming, gn) < 0.0f;
-        nrm = entering ? sn : -sn;
         mat = load_mat(sc.materials, mat_id);
         uint32_t rs = M.z;
         uint32_t depth = (M.w >> 8) > 1u ? (M.w >> 8) : 1u;
@@ -1008,7 +1063,6 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
             float lpdf = ((sel * t) * t) / fmaxf(lc * area, 1e-6f);
             float sampled = lpdf * (float)(P.secondary_area_samples > 1u ? P.secondary_area_samples : 1u);
             if (M.w & F_PREV_UNIFIED) sampled = lpdf * unified_area_probability(P);
-            // After ReSTIR the light technique's MIS counted its candidates.
             emis = power_heuristic(L.w, sampled);
           }
           f3 add = (rgb(T) * e) * emis;
@@ -1060,9 +1114,18 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
             nee_p = nee_prob;
             nee_rng = rs;
             if (P.restir && bounce == 0u) {
-              // Primary hit: point lights, the ReSTIR area-light sample, then
-              // the environment samples (wavefront_primary_impl.glsl:1527-1785).
-              restir_primary(P, J, rs, hitp, nrm, mat, incoming, t, M.x, mat_id, single);
+              // Primary hit: point lights, the ReSTIR area-light sample (from
+              // k_restir, with the random state it advanced to), then the
+              // environment samples (wavefront_primary_impl.glsl:1527-1785).
+              const float4* rsp = W.rsamp + 3u * path;
+              const float4 ra = rsp[0], rb = rsp[1], rc = rsp[2];
+              single.origin = rgb(ra);
+              single.valid = ra.w != 0.0f;
+              single.dir = rgb(rb);
+              single.tmax = rb.w;
+              single.c = rgb(rc);
+              single.kind = 0u;
+              rs = __float_as_uint(rc.w);
               nee_rng = rs;
               restir_nee(P, single, rs, hitp, nrm, mat, incoming, env_n, [&](const NeeSample&) { ++nee_count; });
               used_restir = true;
@@ -1350,6 +1413,10 @@ void wave_trace_kernels(const GiLaunch& l, const Wave& w, uint32_t in, hipStream
 }
 
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  if (w.rsamp) {
+    static int rocc = resident_blocks(k_restir, 256);
+    hipLaunchKernelGGL(k_restir, dim3((unsigned)(l.cu_count * rocc)), dim3(256), 0, st, w, in);
+  }
   static int occ = resident_blocks(k_shade, 256);
   hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in);
 }
