@@ -38,7 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_IN, HIT_OUT, OCC_OUT = 64, 48, 32, 16, 4
-TRAFFIC_FILE = os.path.join("profiles", "k_trace_closest_terrain1M_1080p_8spp_8b_traffic.txt")
+TRAFFIC_FILE = os.path.join("profiles", "k_closest_terrain1M_1080p_8spp_8b_traffic.txt")
 
 
 def log(msg):
@@ -354,7 +354,7 @@ def main(argv=None):
         traffic = measured_traffic(workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "k_trace_closest", "avg_launch_ms": round(avg_ms, 4),
+                    "kernel": "k_closest", "avg_launch_ms": round(avg_ms, 4),
                     "bytes_per_launch": int(per_launch_bytes),
                     "traffic_source": TRAFFIC_FILE if traffic is not None else None}
     line = {

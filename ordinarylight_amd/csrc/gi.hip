@@ -996,7 +996,7 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
     uint32_t path = 0, nee_count = 0, area_n = 0, env_n = 0, nee_rng = 0;
     f3 nee_T = mk3(0, 0, 0), hitp = nee_T, nrm = nee_T, incoming = nee_T;
     float nee_p = 1.0f;
-    Mat mat;
+    uint32_t nee_mat = 0;  // material of the hit, reloaded to replay its samples
     NeeSample single;
     single.valid = false;
     if (live) {
@@ -1045,7 +1045,8 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
           J.geo[M.x] = make_float4(t, gn.x, gn.y, gn.z);
           J.prim[M.x] = prim;
         }
-        mat = load_mat(sc.materials, mat_id);
+        const Mat mat = load_mat(sc.materials, mat_id);
+        nee_mat = mat_id;
         uint32_t rs = M.z;
         uint32_t depth = (M.w >> 8) > 1u ? (M.w >> 8) : 1u;
         const uint32_t bounce = (uint32_t)T.w;
@@ -1229,6 +1230,7 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
           uint32_t kk = 0;
           uint32_t rs2 = nee_rng;
           uint32_t en;
+          const Mat mat = load_mat(sc.materials, nee_mat);
           restir_nee(P, single, rs2, hitp, nrm, mat, incoming, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
             W.ed[eslot + kk] = mk4(s.dir, s.tmax);
@@ -1242,6 +1244,7 @@ __global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
         } else {
           uint32_t kk = 0, an, en;
           uint32_t rs2 = nee_rng;
+          const Mat mat = load_mat(sc.materials, nee_mat);
           for_each_nee(P, rs2, hitp, nrm, mat, incoming, an, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
             W.ed[eslot + kk] = mk4(s.dir, s.tmax);
