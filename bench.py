@@ -233,6 +233,12 @@ def c3_measurement(context, steps=2):
                             wavefront_restir_candidates=32, wavefront_restir_spatial_reuse=True,
                             wavefront_restir_spatial_neighbors=1)
     job = Job(ds, camera, 1920, 1080, 4, 0, cfg)
+    # Work-counting frame (not timed): nodes and triangles per traversal.
+    from ordinarylight_amd import _native
+    lib = _native.library()
+    _native.check(lib.olhip_set_work_counters(context.handle, 1))
+    _, work = plain_render(job, copy_image=False)
+    _native.check(lib.olhip_set_work_counters(context.handle, 0))
     plain_render(job, copy_image=False)  # warm-up
     rays, wall = 0, 0.0
     for _ in range(steps):
@@ -246,6 +252,12 @@ def c3_measurement(context, steps=2):
         "mrays_per_s": round(rays / wall / 1e6, 1), "ms_per_frame": round(wall * 1e3 / steps, 2),
         "rays_per_frame": int(rays / steps), "scene_generation_s": round(gen_s, 2),
         "scene_build_ms": round(ds.info.build_ms, 2), "effective_triangles": int(ds.triangle_count),
+        "nodes_per_closest_ray": round(work["closest_nodes"] / max(work["primary_rays"] + work["extension_rays"], 1), 2),
+        "tris_per_closest_ray": round(work["closest_tris"] / max(work["primary_rays"] + work["extension_rays"], 1), 2),
+        "nodes_per_shadow_ray": round(work["shadow_nodes"] / max(work["shadow_rays"], 1), 2),
+        "tris_per_shadow_ray": round(work["shadow_tris"] / max(work["shadow_rays"], 1), 2),
+        "primary_rays": int(work["primary_rays"]), "extension_rays": int(work["extension_rays"]),
+        "shadow_rays": int(work["shadow_rays"]),
         "direct_lighting": "ReSTIR DI at primary hits (32 candidates, temporal + 1 spatial neighbour, "
                            "history limit 20); light-CDF NEE at later bounces",
     }

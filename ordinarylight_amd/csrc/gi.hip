@@ -973,7 +973,9 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in) {
 // (chunk index within the job) % stripes: a job's chunks number at most
 // n_paths / 64 + stripes, so a segment of (ceil(n_paths / (64 * stripes)) + 1)
 // * 64 slots cannot overflow.
-__global__ void __launch_bounds__(256) k_shade(Wave W, uint32_t in) {
+// Four waves per SIMD: the register budget drops from 153 to 128 VGPRs with
+// a few spilled dwords, which measured 7 % faster than three waves.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Wave W, uint32_t in) {
   const uint32_t out = in ^ 1u;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t flat_chunks = W.ctr[C_RP + kRuns];

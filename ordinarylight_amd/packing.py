@@ -204,16 +204,26 @@ def pack_instanced_scene(scene):
     A = M[:, :3, :3]
     normal_matrix = np.transpose(np.linalg.inv(A), (0, 2, 1))
     det = np.linalg.det(A)
-    # World bounds: the 8 corners of each resource's object box, transformed
-    # in float64 and rounded outward to float32.
+    # World bounds, in float64 and rounded outward to float32: the box of the
+    # 8 transformed corners of each resource's object box, intersected with
+    # the box of its transformed bounding sphere (centre c, radius r: world
+    # axis i spans (A c + t)_i +- r |A_i|).  Both contain the instance; for
+    # rotated round meshes the sphere box is up to sqrt(3) narrower per axis.
     lo = np.stack([r.vertices.min(0) for r in resources]).astype(np.float64)
     hi = np.stack([r.vertices.max(0) for r in resources]).astype(np.float64)
+    centre = 0.5 * (lo + hi)
+    radius = np.array([np.sqrt(((r.vertices.astype(np.float64) - c) ** 2).sum(1).max())
+                       for r, c in zip(resources, centre)])
     ri = np.asarray(inst_res)
     corners = np.stack([np.where(np.array(bits, bool), hi[ri], lo[ri])
                         for bits in np.ndindex(2, 2, 2)], axis=1)      # [I, 8, 3]
     world = corners @ np.transpose(A, (0, 2, 1)) + M[:, None, :3, 3]
-    wmin = np.nextafter(world.min(1).astype(np.float32), np.float32(-np.inf))
-    wmax = np.nextafter(world.max(1).astype(np.float32), np.float32(np.inf))
+    sphere_c = np.einsum("kij,kj->ki", A, centre[ri]) + M[:, :3, 3]
+    sphere_h = radius[ri][:, None] * np.linalg.norm(A, axis=2)
+    bmin = np.maximum(world.min(1), sphere_c - sphere_h)
+    bmax = np.minimum(world.max(1), sphere_c + sphere_h)
+    wmin = np.nextafter(bmin.astype(np.float32), np.float32(-np.inf))
+    wmax = np.nextafter(bmax.astype(np.float32), np.float32(np.inf))
     if not materials:
         raise ValueError("no materials")
     analytic_count = int(getattr(scene, "analytic_light_count", 0))

@@ -69,3 +69,26 @@ def test_tessellated_room_size_and_shared_layout():
     on_box = (np.isclose(np.abs(v[:, 0]), 1) | np.isclose(v[:, 1], 0) | np.isclose(v[:, 1], 2)
               | np.isclose(np.abs(v[:, 2]), 1))
     assert on_box.all()
+
+
+def test_instanced_packing_bounds_contain_instances_and_flatten_in_order():
+    """World bounds of the two-level packing contain every transformed vertex
+    and are no larger than the transformed object-box corners; triangle
+    offsets follow the flattened packed order."""
+    from ordinarylight_amd import packing
+    scene, _ = workloads.instanced_spheres(3, meshes=3, subdivisions=2, instances=120, box=12.0, emitters=10)
+    p = packing.pack_instanced_scene(scene)
+    meshes = [m for m in scene.meshes if len(m.indices)]
+    assert len(p.instance_resource) == len(meshes)
+    offset = 0
+    for k, mesh in enumerate(meshes):
+        w = mesh.world_vertices
+        lo, hi = p.world_bounds[k, :3], p.world_bounds[k, 3:]
+        assert np.all(w >= lo) and np.all(w <= hi)
+        corners = np.array([[(mesh.vertices.max(0) if b else mesh.vertices.min(0))[a] for a, b in enumerate(bits)]
+                            for bits in np.ndindex(2, 2, 2)], np.float64)
+        world = corners @ mesh.transform.matrix[:3, :3].T + mesh.transform.matrix[:3, 3]
+        assert np.all(lo >= np.nextafter(world.min(0).astype(np.float32), -np.inf))
+        assert np.all(hi <= np.nextafter(world.max(0).astype(np.float32), np.inf))
+        assert p.instance_triangle_offset[k] == offset
+        offset += len(mesh.indices)
