@@ -111,6 +111,15 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* des
  * :4696-4870).  A different structure is rejected (OLHIP_ERR_VALUE).  Jobs
  * that use the scene must not be queued or running during an update. */
 int olhip_scene_update(olhip_ctx* ctx, olhip_scene* scene, const olhip_scene_desc* desc);
+
+/* Host-only: the binned-SAH binary hierarchy a two-level scene's top level is
+ * built from (no counterpart in the reference, whose top level is built by
+ * the driver).  boxes [n][6] (min xyz, max xyz), n >= 2; order [n] is the
+ * primitive of each leaf position; internal node i (0 = root, n - 1 nodes)
+ * has children left/right (internal index, or leaf position | 0x80000000)
+ * and leaf range [first, last]. */
+int olhip_sah_topology(const float* boxes, uint32_t n, uint32_t* order, uint32_t* left, uint32_t* right,
+                       uint32_t* first, uint32_t* last);
 int olhip_scene_update_instanced(olhip_ctx* ctx, olhip_scene* scene, const olhip_instanced_desc* desc);
 
 typedef struct olhip_scene_info {

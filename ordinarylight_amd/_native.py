@@ -188,6 +188,8 @@ _SIGNATURES = {
                                           c_u32]),
     "olhip_scene_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(SceneDesc)]),
     "olhip_scene_update_instanced": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(InstancedDesc)]),
+    "olhip_sah_topology": (ctypes.c_int, [P_f32, c_u32, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32),
+                                          ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "olhip_camera_rays": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Camera), c_u32, c_u32, c_u32, c_u32, c_u32,
                                          c_u32, c_u32, c_u32, P_f32, P_f32]),
     "olhip_e0_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(E0Camera), c_u32, c_u32,

@@ -315,7 +315,7 @@ void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t
   upload(sc->tlas_vidx, tvidx.data(), tvidx.size(), s);
   upload(sc->inst_table, inst.data(), inst.size(), s);
   OL_HIP_CHECK(hipStreamSynchronize(s));
-  BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u};
+  BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u, d->world_bounds};
   bvh_build(tin, sc->bvh, s, nullptr);
   sc->tlas_depth = sc->bvh.depth;
   // Stack sizing: both levels plus the sentinel.
@@ -521,6 +521,21 @@ int olhip_scene_update_instanced(olhip_ctx* ctx, olhip_scene* sc, const olhip_in
                    d->area_light_count, d->area_lights, d->area_light_weight, s);
     sc->point_count = d->analytic_light_count;
     build_top_level(sc, d, s);
+  });
+}
+
+int olhip_sah_topology(const float* boxes, uint32_t n, uint32_t* order, uint32_t* left, uint32_t* right,
+                       uint32_t* first, uint32_t* last) {
+  return guarded([&] {
+    require(boxes && order && left && right && first && last, "arguments must not be NULL");
+    require(n >= 2, "at least two primitives are required");
+    std::vector<uint32_t> o, l, r, f, la, pi, pl;
+    sah_topology(boxes, n, o, l, r, f, la, pi, pl);
+    std::copy(o.begin(), o.end(), order);
+    std::copy(l.begin(), l.end(), left);
+    std::copy(r.begin(), r.end(), right);
+    std::copy(f.begin(), f.end(), first);
+    std::copy(la.begin(), la.end(), last);
   });
 }
 

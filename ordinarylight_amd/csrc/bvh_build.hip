@@ -10,7 +10,9 @@
 // Replaces the driver acceleration-structure builds of the reference
 // (targets/vulkan/scene.py:1132-1194, :1480-1561).
 
+#include <algorithm>
 #include <chrono>
+#include <vector>
 
 #include "internal.h"
 
@@ -545,6 +547,124 @@ void build_boxes(const BvhInput& in, const BvhTopology& tp, BvhOutput& out, hipS
 
 }  // namespace
 
+// Binned SAH over primitive boxes, one primitive per leaf: a top-down split
+// of [b, e) at the best of 16 centroid bins on each axis (surface area x
+// count), the middle when every centroid coincides.  Internal nodes are
+// numbered in creation order from the root (0); leaves are the positions of
+// the final order, so every node covers a contiguous leaf range, as in the
+// Karras hierarchy.
+void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, std::vector<uint32_t>& left,
+                  std::vector<uint32_t>& right, std::vector<uint32_t>& rfirst, std::vector<uint32_t>& rlast,
+                  std::vector<uint32_t>& parent_int, std::vector<uint32_t>& parent_leaf) {
+  constexpr int kBins = 16;
+  order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  const uint32_t ni = n - 1;
+  left.assign(ni, 0);
+  right.assign(ni, 0);
+  rfirst.assign(ni, 0);
+  rlast.assign(ni, 0);
+  parent_int.assign(ni, OL_NONE);
+  parent_leaf.assign(n, OL_NONE);
+  auto centroid = [&](uint32_t p, int a) { return 0.5f * (box[6 * p + a] + box[6 * p + 3 + a]); };
+  auto area = [](const float* mn, const float* mx) {
+    const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    return dx * dy + dy * dz + dz * dx;
+  };
+  struct Task { uint32_t b, e, node; };
+  std::vector<Task> stack;
+  uint32_t next = 1;
+  stack.push_back({0, n, 0});
+  while (!stack.empty()) {
+    const Task t = stack.back();
+    stack.pop_back();
+    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = t.b; k < t.e; ++k)
+      for (int a = 0; a < 3; ++a) {
+        const float c = centroid(order[k], a);
+        cmin[a] = std::min(cmin[a], c);
+        cmax[a] = std::max(cmax[a], c);
+      }
+    float best_cost = INFINITY;
+    int best_axis = -1, best_bin = 0;
+    for (int a = 0; a < 3; ++a) {
+      const float ext = cmax[a] - cmin[a];
+      if (!(ext > 0.0f)) continue;
+      float bmn[kBins][3], bmx[kBins][3];
+      uint32_t cnt[kBins] = {0};
+      for (int i = 0; i < kBins; ++i)
+        for (int k = 0; k < 3; ++k) { bmn[i][k] = INFINITY; bmx[i][k] = -INFINITY; }
+      for (uint32_t k = t.b; k < t.e; ++k) {
+        const uint32_t p = order[k];
+        const int bin = std::min(kBins - 1, (int)((centroid(p, a) - cmin[a]) / ext * kBins));
+        ++cnt[bin];
+        for (int q = 0; q < 3; ++q) {
+          bmn[bin][q] = std::min(bmn[bin][q], box[6 * p + q]);
+          bmx[bin][q] = std::max(bmx[bin][q], box[6 * p + 3 + q]);
+        }
+      }
+      // Sweep: cost of splitting after bin i.
+      float rmn[kBins][3], rmx[kBins][3];
+      uint32_t rc[kBins];
+      float amn[3] = {INFINITY, INFINITY, INFINITY}, amx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      uint32_t acc = 0;
+      for (int i = kBins - 1; i >= 0; --i) {
+        for (int q = 0; q < 3; ++q) { amn[q] = std::min(amn[q], bmn[i][q]); amx[q] = std::max(amx[q], bmx[i][q]); }
+        acc += cnt[i];
+        for (int q = 0; q < 3; ++q) { rmn[i][q] = amn[q]; rmx[i][q] = amx[q]; }
+        rc[i] = acc;
+      }
+      float lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      uint32_t lc = 0;
+      for (int i = 0; i + 1 < kBins; ++i) {
+        for (int q = 0; q < 3; ++q) { lmn[q] = std::min(lmn[q], bmn[i][q]); lmx[q] = std::max(lmx[q], bmx[i][q]); }
+        lc += cnt[i];
+        if (lc == 0 || rc[i + 1] == 0) continue;
+        const float cost = area(lmn, lmx) * (float)lc + area(rmn[i + 1], rmx[i + 1]) * (float)rc[i + 1];
+        if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = i; }
+      }
+    }
+    uint32_t mid;
+    if (best_axis < 0) {
+      mid = t.b + (t.e - t.b) / 2;
+    } else {
+      const int a = best_axis;
+      const float ext = cmax[a] - cmin[a];
+      auto it = std::partition(order.begin() + t.b, order.begin() + t.e, [&](uint32_t p) {
+        return std::min(kBins - 1, (int)((centroid(p, a) - cmin[a]) / ext * kBins)) <= best_bin;
+      });
+      mid = (uint32_t)(it - order.begin());
+      if (mid == t.b || mid == t.e) mid = t.b + (t.e - t.b) / 2;
+    }
+    rfirst[t.node] = t.b;
+    rlast[t.node] = t.e - 1;
+    const uint32_t halves[2][2] = {{t.b, mid}, {mid, t.e}};
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t b = halves[h][0], e = halves[h][1];
+      uint32_t ref;
+      if (e - b == 1) {
+        ref = b | OL_LEAF_BIT;
+        parent_leaf[b] = t.node;
+      } else {
+        ref = next++;
+        parent_int[ref] = t.node;
+        stack.push_back({b, e, ref});
+      }
+      (h == 0 ? left : right)[t.node] = ref;
+    }
+  }
+}
+
+namespace {
+
+template <class T>
+void upload_vec(DevBuf& d, const std::vector<T>& v, hipStream_t s) {
+  d.reserve(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (!v.empty()) OL_HIP_CHECK(hipMemcpyAsync(d.ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
 void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debug, bool keep_topology) {
   auto t0 = std::chrono::steady_clock::now();
   const uint32_t n = in.tri_count;
@@ -560,6 +680,22 @@ void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debu
   auto tp = std::make_unique<BvhTopology>();
   tp->tri_count = n;
   tp->leaf_size = in.leaf_size == 0 ? 4u : (in.leaf_size > OL_LEAF_MAX ? OL_LEAF_MAX : in.leaf_size);
+  if (in.host_boxes && n > 1) {
+    std::vector<uint32_t> order, l, r, f, la, pi, pl;
+    sah_topology(in.host_boxes, n, order, l, r, f, la, pi, pl);
+    upload_vec(tp->sorted, order, s);
+    upload_vec(tp->left, l, s);
+    upload_vec(tp->right, r, s);
+    upload_vec(tp->rfirst, f, s);
+    upload_vec(tp->rlast, la, s);
+    upload_vec(tp->parent_int, pi, s);
+    upload_vec(tp->parent_leaf, pl, s);
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+    build_boxes(in, *tp, out, s, debug);
+    if (keep_topology) out.topology = std::move(tp);
+    out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return;
+  }
 
   // Steps 1-4: bounds, Morton codes, radix sort, Karras hierarchy.
   DevBuf tmin, tmax, cb, codes, codes_tmp, vals_tmp;

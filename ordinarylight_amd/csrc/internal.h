@@ -58,6 +58,10 @@ struct BvhInput {
   const uint4* tri_vidx;    // device, [T] (i0, i1, i2, material)
   uint32_t tri_count;
   uint32_t leaf_size;       // 1..16
+  // Optional host copy of the primitives' boxes (min xyz, max xyz per
+  // primitive): the binary hierarchy is then built on the host by binned SAH
+  // instead of Morton codes (small top levels), the boxes phase as usual.
+  const float* host_boxes = nullptr;
 };
 
 // Binary LBVH topology of a build (Morton order and Karras hierarchy), kept
@@ -87,6 +91,13 @@ struct BvhDebug {
   std::vector<uint32_t> left, right;  // [T-1] binary children (leaf refs | 0x80000000)
   std::vector<float> node_box;        // [T-1][8] binary internal boxes (min.xyzw, max.xyzw)
 };
+
+// Host binned-SAH binary hierarchy over n >= 2 primitive boxes (min xyz, max
+// xyz each), one primitive per leaf, in the Karras arrays' conventions: leaf
+// order, children (leaf refs | OL_LEAF_BIT), leaf ranges, parents.
+void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, std::vector<uint32_t>& left,
+                  std::vector<uint32_t>& right, std::vector<uint32_t>& rfirst, std::vector<uint32_t>& rlast,
+                  std::vector<uint32_t>& parent_int, std::vector<uint32_t>& parent_leaf);
 
 // keep_topology: retain the binary topology in out.topology for bvh_refit.
 void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t stream, BvhDebug* debug, bool keep_topology = false);

@@ -287,3 +287,60 @@ def test_hip_renderer_configuration_validation():
         r.render_frame(None, None, 4, 4)
     from ordinarylight_amd.renderers import RendererProtocol
     assert isinstance(r, RendererProtocol)
+
+
+def sah(boxes):
+    from ordinarylight_amd import _native as N
+    n = len(boxes)
+    boxes = np.ascontiguousarray(boxes, np.float32)
+    out = [np.zeros(n, np.uint32)] + [np.zeros(n - 1, np.uint32) for _ in range(4)]
+    N.check(N.library().olhip_sah_topology(N.ptr(boxes), n, *[o.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+                                                               for o in out]))
+    return out
+
+
+def test_sah_top_level_hierarchy_is_a_full_binary_tree_over_contiguous_ranges():
+    """The host SAH hierarchy of two-level scenes: every primitive sits in one
+    leaf, every internal node covers the union of its children's contiguous
+    ranges, and the tree is reachable from the root without cycles."""
+    rng = np.random.default_rng(5)
+    for n in (2, 3, 17, 1000):
+        c = rng.uniform(-50, 50, (n, 3))
+        h = rng.uniform(0.1, 3.0, (n, 3))
+        boxes = np.concatenate([c - h, c + h], 1)
+        if n == 17:
+            boxes[:] = boxes[0]  # coincident centroids: median splits
+        order, left, right, first, last = sah(boxes)
+        assert sorted(order.tolist()) == list(range(n))
+        LEAF = 0x80000000
+
+        def span(ref):
+            return (int(ref) & 0x7FFFFFFF, int(ref) & 0x7FFFFFFF) if ref & LEAF else (first[ref], last[ref])
+
+        seen_leaves, seen_nodes, stack = [], set(), [0]
+        while stack:
+            i = stack.pop()
+            assert i not in seen_nodes
+            seen_nodes.add(i)
+            (a0, a1), (b0, b1) = span(left[i]), span(right[i])
+            assert a0 == first[i] and a1 + 1 == b0 and b1 == last[i]
+            for ref in (left[i], right[i]):
+                if ref & LEAF:
+                    seen_leaves.append(int(ref) & 0x7FFFFFFF)
+                else:
+                    assert ref > i
+                    stack.append(ref)
+        assert seen_nodes == set(range(n - 1)) and sorted(seen_leaves) == list(range(n))
+
+
+def test_sah_separates_two_clusters_at_the_root():
+    """Two distant clusters: the root split is the gap between them."""
+    rng = np.random.default_rng(6)
+    a = rng.uniform(0, 1, (50, 3))
+    b = rng.uniform(100, 101, (70, 3))
+    c = np.concatenate([a, b])
+    boxes = np.concatenate([c - 0.01, c + 0.01], 1)
+    order, left, right, first, last = sah(boxes)
+    lo, hi = (int(left[0]) & 0x7FFFFFFF, int(right[0]) & 0x7FFFFFFF)
+    left_prims = order[first[lo]:last[lo] + 1] if not left[0] & 0x80000000 else order[lo:lo + 1]
+    assert set(left_prims.tolist()) in (set(range(50)), set(range(50, 120)))
