@@ -215,7 +215,7 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8):
     }
 
 
-def c3_measurement(context, steps=2):
+def c3_measurement(context, steps=2, builder="lbvh"):
     """Instanced two-level scene: 5 jittered icospheres (20,480 triangles each)
     placed 10,000 times (~205M effective triangles), 20 % glass, 100k small
     emitters sampled with ReSTIR DI (32 candidates, temporal reuse and one
@@ -228,7 +228,7 @@ def c3_measurement(context, steps=2):
     t0 = time.perf_counter()
     scene, camera = workloads.instanced_spheres(0)
     gen_s = time.perf_counter() - t0
-    ds = DeviceScene(context, scene, 2, instanced=True)
+    ds = DeviceScene(context, scene, 2, instanced=True, builder=builder)
     cfg = HipRendererConfig(max_bounces=16, samples_per_pixel=4, wavefront_restir_di=True,
                             wavefront_restir_candidates=32, wavefront_restir_spatial_reuse=True,
                             wavefront_restir_spatial_neighbors=1)
@@ -278,6 +278,7 @@ def main(argv=None):
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-profile", action="store_true")
     parser.add_argument("--leaf-size", type=int, default=2)
+    parser.add_argument("--builder", choices=("lbvh", "sah"), default="lbvh", help="BVH hierarchy of the terrain")
     parser.add_argument("--lds-stack", type=int, default=16)
     parser.add_argument("--block", type=int, default=128)
     parser.add_argument("--batch-jobs", type=int, default=64, help="batched workload size (0: skip)")
@@ -300,7 +301,7 @@ def main(argv=None):
     if args.c3_only:
         from ordinarylight_amd.renderers.hip import DeviceContext
         context = DeviceContext(0)
-        print(json.dumps({"c3": c3_measurement(context, steps=args.steps)}), flush=True)
+        print(json.dumps({"c3": c3_measurement(context, steps=args.steps, builder=args.builder)}), flush=True)
         context.close()
         return 0
 
@@ -309,7 +310,7 @@ def main(argv=None):
     log(f"scene: {scene.triangle_count} triangles, generated in {time.perf_counter() - t0:.1f} s")
     W, H, spp, B = args.width, args.height, args.spp, args.bounces
     renderer = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
-                             bvh_leaf_size=args.leaf_size, traversal_lds_stack=args.lds_stack,
+                             bvh_leaf_size=args.leaf_size, bvh_builder=args.builder, traversal_lds_stack=args.lds_stack,
                              traversal_block=args.block)
     # Scene upload + BVH build (reported separately, outside the timed region).
     renderer.replace_scene(scene)
@@ -407,7 +408,7 @@ def main(argv=None):
     if not args.no_c3:
         from ordinarylight_amd.renderers.hip import DeviceContext
         context = DeviceContext(0)
-        line["c3"] = c3_measurement(context)
+        line["c3"] = c3_measurement(context, builder=args.builder)
         log(f"c3: {line['c3']}")
         context.close()
 

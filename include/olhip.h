@@ -65,7 +65,13 @@ typedef struct olhip_scene_desc {
   const float* area_lights;    /* [area_light_count][5][4] */
   float area_light_weight;     /* Scene.emissive_light_weight */
   uint32_t leaf_size;          /* BVH leaf size 1..16, 0 = default (4) */
+  uint32_t builder;            /* OLHIP_BUILDER_LBVH (GPU Morton LBVH, default) or
+                                  OLHIP_BUILDER_SAH (host binned-SAH hierarchy,
+                                  GPU boxes and wide collapse) */
 } olhip_scene_desc;
+
+#define OLHIP_BUILDER_LBVH 0u
+#define OLHIP_BUILDER_SAH 1u
 
 /* Two-level scene: object-space mesh resources, each with its own
  * bottom-level BVH, placed by instances under affine transforms; a top-level
@@ -97,6 +103,9 @@ typedef struct olhip_instanced_desc {
   const float* area_lights;            /* world-space emissive triangles [L][5][4] */
   float area_light_weight;
   uint32_t leaf_size;                  /* bottom-level leaf size 1..16, 0 = 4 */
+  uint32_t builder;                    /* bottom-level hierarchy: OLHIP_BUILDER_LBVH or
+                                          OLHIP_BUILDER_SAH; the top level is always
+                                          host binned SAH */
 } olhip_instanced_desc;
 
 int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* desc, olhip_scene** out);

@@ -36,8 +36,11 @@ class SceneDesc(ctypes.Structure):
         ("material_count", c_u32), ("materials", P_f32),
         ("analytic_light_count", c_u32), ("analytic_lights", P_f32),
         ("area_light_count", c_u32), ("area_lights", P_f32),
-        ("area_light_weight", c_f32), ("leaf_size", c_u32),
+        ("area_light_weight", c_f32), ("leaf_size", c_u32), ("builder", c_u32),
     ]
+
+
+BUILDER_LBVH, BUILDER_SAH = 0, 1
 
 
 class InstancedDesc(ctypes.Structure):
@@ -50,7 +53,7 @@ class InstancedDesc(ctypes.Structure):
         ("material_count", c_u32), ("materials", P_f32),
         ("analytic_light_count", c_u32), ("analytic_lights", P_f32),
         ("area_light_count", c_u32), ("area_lights", P_f32),
-        ("area_light_weight", c_f32), ("leaf_size", c_u32),
+        ("area_light_weight", c_f32), ("leaf_size", c_u32), ("builder", c_u32),
     ]
 
 

@@ -151,6 +151,7 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     require(d->analytic_light_count == 0 || d->analytic_lights, "analytic lights missing");
     require(d->area_light_count == 0 || d->area_lights, "area lights missing");
     require(d->leaf_size <= 16, "leaf_size must be 0..16");
+    require(d->builder <= OLHIP_BUILDER_SAH, "unknown BVH builder");
     OL_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     auto sc = std::make_unique<olhip_scene>();
@@ -189,7 +190,22 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     OL_HIP_CHECK(hipStreamSynchronize(s));
     sc->structure_hash = fnv1a(d->indices, 12ull * d->triangle_count, kFnvBasis);
     sc->leaf_size = d->leaf_size;
-    BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, d->leaf_size};
+    std::vector<float> boxes;
+    if (d->builder == OLHIP_BUILDER_SAH && d->triangle_count > 1) {
+      boxes.resize(6ull * d->triangle_count);
+      for (uint32_t t = 0; t < d->triangle_count; ++t) {
+        const float4 a = pos[vidx[t].x], b = pos[vidx[t].y], c = pos[vidx[t].z];
+        float* o = &boxes[6ull * t];
+        o[0] = std::min(a.x, std::min(b.x, c.x));
+        o[1] = std::min(a.y, std::min(b.y, c.y));
+        o[2] = std::min(a.z, std::min(b.z, c.z));
+        o[3] = std::max(a.x, std::max(b.x, c.x));
+        o[4] = std::max(a.y, std::max(b.y, c.y));
+        o[5] = std::max(a.z, std::max(b.z, c.z));
+      }
+    }
+    BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, d->leaf_size,
+                boxes.empty() ? nullptr : boxes.data()};
     bvh_build(in, sc->bvh, s, nullptr, true);
     DScene& v = sc->view;
     v.nodes = sc->bvh.nodes.as<WideNode>();
@@ -345,6 +361,7 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, 
                 d->instance_id && d->world_to_object && d->normal_matrix && d->determinant && d->world_bounds,
             "instance arrays missing");
     require(d->leaf_size <= 16, "leaf_size must be 0..16");
+    require(d->builder <= OLHIP_BUILDER_SAH, "unknown BVH builder");
     const uint32_t R = d->resource_count, V = d->vertex_offsets[R], T = d->triangle_offsets[R];
     require(V == 0 || (d->positions && d->normals), "geometry arrays missing");
     require(T == 0 || d->indices, "indices missing");
@@ -390,7 +407,23 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, 
       const uint32_t vo = d->vertex_offsets[r], to = d->triangle_offsets[r];
       const uint32_t nt = d->triangle_offsets[r + 1] - to;
       if (nt) {
-        BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, d->leaf_size};
+        std::vector<float> boxes;
+        if (d->builder == OLHIP_BUILDER_SAH && nt > 1) {
+          boxes.resize(6ull * nt);
+          for (uint32_t t = 0; t < nt; ++t) {
+            const uint4 tv = vidx[to + t];
+            const float4 a = pos[vo + tv.x], b = pos[vo + tv.y], c = pos[vo + tv.z];
+            float* o = &boxes[6ull * t];
+            o[0] = std::min(a.x, std::min(b.x, c.x));
+            o[1] = std::min(a.y, std::min(b.y, c.y));
+            o[2] = std::min(a.z, std::min(b.z, c.z));
+            o[3] = std::max(a.x, std::max(b.x, c.x));
+            o[4] = std::max(a.y, std::max(b.y, c.y));
+            o[5] = std::max(a.z, std::max(b.z, c.z));
+          }
+        }
+        BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, d->leaf_size,
+                    boxes.empty() ? nullptr : boxes.data()};
         bvh_build(in, *out_r, s, nullptr, true);
       }
       blas[r] = BlasDev{out_r->nodes.as<WideNode>(), out_r->tri.as<float4>(), sc->vidx.as<uint4>() + to,

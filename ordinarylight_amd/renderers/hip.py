@@ -67,6 +67,9 @@ class HipRendererConfig:
     max_samples_per_call: int = 64
     device_index: int = 0
     bvh_leaf_size: int = 2
+    # Bottom-level hierarchy: "lbvh" (GPU Morton LBVH) or "sah" (host binned
+    # SAH, slower to build, fewer nodes visited per ray).
+    bvh_builder: str = "lbvh"
     # Traversal stack entries kept in LDS per lane, and the traversal block size.
     traversal_lds_stack: int = 16
     traversal_block: int = 128
@@ -90,6 +93,8 @@ class HipRendererConfig:
             raise ValueError("wavefront_tile_capacity must be positive")
         if not 0 <= int(self.bvh_leaf_size) <= 16:
             raise ValueError("bvh_leaf_size must be between 0 and 16")
+        if self.bvh_builder not in ("lbvh", "sah"):
+            raise ValueError("bvh_builder must be 'lbvh' or 'sah'")
         if not (math.isfinite(float(self.wavefront_exposure)) and float(self.wavefront_exposure) > 0.0):
             raise ValueError("wavefront_exposure must be positive")
         if not 1 <= int(self.wavefront_restir_history_limit) <= 64:
@@ -175,7 +180,10 @@ class DeviceContext:
             self.handle = None
 
 
-def _scene_desc(packed, leaf_size):
+BUILDERS = {"lbvh": N.BUILDER_LBVH, "sah": N.BUILDER_SAH}
+
+
+def _scene_desc(packed, leaf_size, builder="lbvh"):
     desc = N.SceneDesc()
     desc.vertex_count = len(packed.positions)
     desc.positions = N.ptr(packed.positions)
@@ -191,10 +199,11 @@ def _scene_desc(packed, leaf_size):
     desc.area_lights = N.ptr(packed.area_lights)
     desc.area_light_weight = packed.area_light_weight
     desc.leaf_size = int(leaf_size)
+    desc.builder = BUILDERS[builder]
     return desc
 
 
-def _instanced_desc(p, leaf_size):
+def _instanced_desc(p, leaf_size, builder="lbvh"):
     d = N.InstancedDesc()
     d.resource_count = len(p.vertex_offsets) - 1
     d.vertex_offsets = N.ptr(p.vertex_offsets, N.ctypes.c_uint32)
@@ -219,6 +228,7 @@ def _instanced_desc(p, leaf_size):
     d.area_lights = N.ptr(p.area_lights)
     d.area_light_weight = p.area_light_weight
     d.leaf_size = int(leaf_size)
+    d.builder = BUILDERS[builder]
     return d
 
 
@@ -235,10 +245,13 @@ class DeviceScene:
     than once.
     """
 
-    def __init__(self, context, scene, leaf_size=4, instanced=None):
+    def __init__(self, context, scene, leaf_size=4, instanced=None, builder="lbvh"):
         self._lib = N.library()
         self.context = context
         self.leaf_size = int(leaf_size)
+        if builder not in BUILDERS:
+            raise ValueError("builder must be 'lbvh' or 'sah'")
+        self.builder = builder
         if instanced is None:
             instanced = shares_resources(scene)
         self.instanced = bool(instanced)
@@ -249,7 +262,7 @@ class DeviceScene:
         packed = pack_scene(scene)
         self.packed = packed
         pack_ms = (time.perf_counter() - started) * 1000.0
-        desc = _scene_desc(packed, leaf_size)
+        desc = _scene_desc(packed, leaf_size, builder)
         handle = N.ctypes.c_void_p()
         started = time.perf_counter()
         N.check(self._lib.olhip_scene_create(context.handle, N.ctypes.byref(desc), N.ctypes.byref(handle)))
@@ -265,7 +278,7 @@ class DeviceScene:
         p = pack_instanced_scene(scene)
         self.packed = p
         pack_ms = (time.perf_counter() - started) * 1000.0
-        d = _instanced_desc(p, leaf_size)
+        d = _instanced_desc(p, leaf_size, self.builder)
         handle = N.ctypes.c_void_p()
         started = time.perf_counter()
         N.check(self._lib.olhip_scene_create_instanced(self.context.handle, N.ctypes.byref(d), N.ctypes.byref(handle)))
@@ -292,13 +305,13 @@ class DeviceScene:
             keys = ("vertex_offsets", "triangle_offsets", "indices", "instance_resource", "instance_triangle_offset")
             if not all(_same(getattr(p, k), getattr(old, k)) for k in keys):
                 return False
-            d = _instanced_desc(p, self.leaf_size)
+            d = _instanced_desc(p, self.leaf_size, self.builder)
             update = self._lib.olhip_scene_update_instanced
         else:
             p = pack_scene(scene)
             if not (_same(p.indices, old.indices) and len(p.positions) == len(old.positions)):
                 return False
-            d = _scene_desc(p, self.leaf_size)
+            d = _scene_desc(p, self.leaf_size, self.builder)
             update = self._lib.olhip_scene_update
         moved = not (_same(p.positions, old.positions) and _same(p.normals, old.normals))
         if not moved:
@@ -349,10 +362,11 @@ class DeviceScene:
 class _ResidentMixin:
     """Scene residency keyed by identity and revision (``core.py:4574-4634``)."""
 
-    def _init_resident(self, device_index, leaf_size):
+    def _init_resident(self, device_index, leaf_size, builder="lbvh"):
         self._context = None
         self._device_index = int(device_index)
         self._leaf_size = int(leaf_size)
+        self._builder = builder
         self._scene_key = None
         self._scene_ref = None
         self._device_scene = None
@@ -374,7 +388,8 @@ class _ResidentMixin:
             return self._device_scene
         if self._device_scene is None or self._scene_key != key:
             previous = self._device_scene
-            self._device_scene = DeviceScene(self._ensure_context(), scene, self._leaf_size)
+            self._device_scene = DeviceScene(self._ensure_context(), scene, self._leaf_size,
+                                             builder=self._builder)
             self._scene_key = key
             self._scene_ref = scene
             if previous is not None:
@@ -422,7 +437,7 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         self.config = config or HipRendererConfig(**options)
         if not isinstance(self.config, HipRendererConfig):
             raise TypeError("config must be a HipRendererConfig")
-        self._init_resident(self.config.device_index, self.config.bvh_leaf_size)
+        self._init_resident(self.config.device_index, self.config.bvh_leaf_size, self.config.bvh_builder)
         self.device = f"HIP device {self.config.device_index}"
         self.last_timings = {}
         self.last_stats = None
