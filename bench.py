@@ -296,7 +296,7 @@ def main(argv=None):
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the path shards by image tiles or jobs across GPUs; see the design notes)", file=sys.stderr)
+              "(the path shards by image tiles or whole jobs, one GPU per share)", file=sys.stderr)
         return 2
 
     from ordinarylight_amd import workloads

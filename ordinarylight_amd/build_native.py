@@ -24,7 +24,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OLHIP_ARCH", "gfx950")
 
 # -ffp-contract=off: every float expression rounds as written so the oracle
-# reproduces it; packed-fp32 instructions are disabled (see the build notes).
+# reproduces it; packed-fp32 instructions are disabled: some packed FP32 forms
+# have produced wrong results on this hardware now and then.
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
