@@ -11,7 +11,9 @@
 // (targets/vulkan/scene.py:1132-1194, :1480-1561).
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -571,13 +573,12 @@ void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, st
     const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
     return dx * dy + dy * dz + dz * dx;
   };
+  // Node ids follow the tree's pre-order, so they do not depend on the
+  // order in which subtrees are built: the node over [b, e) has id X, its
+  // left subtree (m leaves) the ids X + 1 .. X + m - 1, its right subtree
+  // those after.  Subtrees below the first levels are built in parallel.
   struct Task { uint32_t b, e, node; };
-  std::vector<Task> stack;
-  uint32_t next = 1;
-  stack.push_back({0, n, 0});
-  while (!stack.empty()) {
-    const Task t = stack.back();
-    stack.pop_back();
+  auto split = [&](const Task& t, std::vector<Task>& out) {
     float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t k = t.b; k < t.e; ++k)
       for (int a = 0; a < 3; ++a) {
@@ -639,6 +640,7 @@ void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, st
     rfirst[t.node] = t.b;
     rlast[t.node] = t.e - 1;
     const uint32_t halves[2][2] = {{t.b, mid}, {mid, t.e}};
+    const uint32_t ids[2] = {t.node + 1u, t.node + (mid - t.b)};
     for (int h = 0; h < 2; ++h) {
       const uint32_t b = halves[h][0], e = halves[h][1];
       uint32_t ref;
@@ -646,13 +648,47 @@ void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, st
         ref = b | OL_LEAF_BIT;
         parent_leaf[b] = t.node;
       } else {
-        ref = next++;
+        ref = ids[h];
         parent_int[ref] = t.node;
-        stack.push_back({b, e, ref});
+        out.push_back({b, e, ref});
       }
       (h == 0 ? left : right)[t.node] = ref;
     }
+  };
+  auto build_subtree = [&](Task root) {
+    std::vector<Task> stack{root};
+    while (!stack.empty()) {
+      const Task t = stack.back();
+      stack.pop_back();
+      split(t, stack);
+    }
+  };
+  const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<Task> front{{0, n, 0}};
+  // Split breadth-first until there is work for every thread.
+  while (!front.empty() && front.size() < 4 * threads) {
+    std::vector<Task> next;
+    for (const Task& t : front) {
+      if (t.e - t.b < 4096) next.push_back(t);
+      else split(t, next);
+    }
+    if (next.size() == front.size()) {
+      front.swap(next);
+      break;
+    }
+    front.swap(next);
   }
+  if (threads == 1 || front.size() < 2) {
+    for (const Task& t : front) build_subtree(t);
+    return;
+  }
+  std::atomic<size_t> cursor{0};
+  std::vector<std::thread> pool;
+  for (unsigned k = 0; k < threads; ++k)
+    pool.emplace_back([&] {
+      for (size_t i = cursor++; i < front.size(); i = cursor++) build_subtree(front[i]);
+    });
+  for (auto& th : pool) th.join();
 }
 
 namespace {

@@ -304,13 +304,16 @@ def test_sah_top_level_hierarchy_is_a_full_binary_tree_over_contiguous_ranges():
     leaf, every internal node covers the union of its children's contiguous
     ranges, and the tree is reachable from the root without cycles."""
     rng = np.random.default_rng(5)
-    for n in (2, 3, 17, 1000):
+    for n in (2, 3, 17, 1000, 300000):
         c = rng.uniform(-50, 50, (n, 3))
         h = rng.uniform(0.1, 3.0, (n, 3))
         boxes = np.concatenate([c - h, c + h], 1)
         if n == 17:
             boxes[:] = boxes[0]  # coincident centroids: median splits
         order, left, right, first, last = sah(boxes)
+        if n == 300000:
+            # Subtrees are built on several threads: the result must not depend on it.
+            assert all(np.array_equal(a, b) for a, b in zip((order, left, right, first, last), sah(boxes)))
         assert sorted(order.tolist()) == list(range(n))
         LEAF = 0x80000000
 
