@@ -261,21 +261,6 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
 }
 
 // ---- traversal ------------------------------------------------------------------------------
-// The same over two columns (items, entry distances) for closest-hit kernels:
-// lds holds 2 * LSTACK rows, the spill buffer 2 * spill_cap rows.
-template <int LSTACK, int BLOCK>
-OL_DEV DistStack<LSTACK> lane_dist_stack(uint32_t* lds, uint32_t* spill, uint32_t spill_cap, uint32_t* overflow) {
-  DistStack<LSTACK> st;
-  st.lds = lds + threadIdx.x;
-  st.lds_stride = BLOCK;
-  st.spill = spill ? spill + (blockIdx.x * BLOCK + threadIdx.x) : nullptr;
-  st.spill_stride = gridDim.x * BLOCK;
-  st.spill_cap = spill ? spill_cap : 0u;
-  st.overflow = overflow;
-  st.sp = 0;
-  return st;
-}
-
 template <int LSTACK, int BLOCK>
 OL_DEV TravStack<LSTACK> lane_stack(uint32_t* lds, uint32_t* spill, uint32_t spill_cap, uint32_t* overflow) {
   TravStack<LSTACK> st;
@@ -322,8 +307,8 @@ OL_DEV void pool_refill(Pool& p, const Wave& W, const uint32_t* prefix, const ui
 // Closest hit for the input ray queue: refilled lanes, split inner / leaf steps.
 template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_closest(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[2 * LSTACK * BLOCK];
-  DistStack<LSTACK> st = lane_dist_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
   const uint32_t* prefix = W.ctr + C_RP;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
@@ -529,8 +514,8 @@ OL_DEV void pool2_refill(Pool2& p, const Wave& W, const uint32_t* prefix, const 
 // scheduling of k_closest with the instance-entering inner step.
 template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_closest2(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[2 * LSTACK * BLOCK];
-  DistStack<LSTACK> st = lane_dist_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
   const uint32_t* prefix = W.ctr + C_RP2;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
@@ -687,8 +672,8 @@ template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_rays(DScene sc, const float4* o4, const float4* d4, uint32_t n,
                                                       float4* hits, uint32_t* overflow, unsigned long long* work,
                                                       uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[2 * LSTACK * BLOCK];
-  DistStack<LSTACK> st = lane_dist_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
   uint32_t nv = 0, nt = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     float4 o = o4[i], d = d4[i];
@@ -705,8 +690,8 @@ template <int LSTACK, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_rays2(DScene sc, const float4* o4, const float4* d4, uint32_t n,
                                                        float4* hits, uint32_t* overflow, unsigned long long* work,
                                                        uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[2 * LSTACK * BLOCK];
-  DistStack<LSTACK> st = lane_dist_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
   uint32_t nv = 0, nt = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     const float4 o = o4[i], d = d4[i];
@@ -1320,9 +1305,8 @@ uint32_t spill_cap_of(const GiLaunch& l) {
   return l.stack_depth > l.lds_stack ? (uint32_t)(l.stack_depth - l.lds_stack) : 0u;
 }
 
-// Closest-hit kernels keep two spill columns per lane (items, distances).
 void check_spill(const GiLaunch& l, uint64_t threads) {
-  if (2ull * spill_cap_of(l) * threads > l.spill_words)
+  if ((uint64_t)spill_cap_of(l) * threads > l.spill_words)
     throw OlError{-2, "traversal spill buffer too small for the launch"};
 }
 
@@ -1334,7 +1318,7 @@ uint64_t spill_words_for(const GiLaunch& l) {
                                        resident_blocks(k_shadow2<LSTACK, BLOCK>, BLOCK))),
                      std::max(resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK),
                               resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK)));
-  return 2ull * spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
+  return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
 
 template <int LSTACK, int BLOCK>

@@ -150,45 +150,6 @@ struct TravStack {
   OL_DEV bool empty() const { return sp == 0; }
 };
 
-// Closest-hit stack of (item, entry distance bits) pairs in two columns: a
-// popped entry that starts beyond the current best hit is dropped without
-// fetching its node (cull on pop).
-template <int LSTACK>
-struct DistStack {
-  uint32_t* lds;       // this lane's item column; distances LSTACK rows further
-  uint32_t lds_stride; // block size
-  uint32_t* spill;     // this lane's global item column; distances spill_cap rows further
-  uint32_t spill_stride;
-  uint32_t spill_cap;
-  uint32_t* overflow;
-  uint32_t sp;
-
-  OL_DEV void push(uint32_t v, uint32_t d) {
-    if (sp < (uint32_t)LSTACK) {
-      lds[sp * lds_stride] = v;
-      lds[(sp + LSTACK) * lds_stride] = d;
-    } else if (sp - LSTACK < spill_cap) {
-      spill[(sp - LSTACK) * spill_stride] = v;
-      spill[(sp - LSTACK + spill_cap) * spill_stride] = d;
-    } else {
-      atomicOr(overflow, 1u);
-      return;
-    }
-    ++sp;
-  }
-  OL_DEV void pop(uint32_t& v, uint32_t& d) {
-    --sp;
-    if (sp < (uint32_t)LSTACK) {
-      v = lds[sp * lds_stride];
-      d = lds[(sp + LSTACK) * lds_stride];
-    } else {
-      v = spill[(sp - LSTACK) * spill_stride];
-      d = spill[(sp - LSTACK + spill_cap) * spill_stride];
-    }
-  }
-  OL_DEV bool empty() const { return sp == 0; }
-};
-
 // Children of a node in child-word order.
 OL_DEV void node_children(const uint4 w2, const uint4 w3, uint32_t ch[4]) {
   ch[0] = w2.z;
@@ -212,6 +173,60 @@ OL_DEV void closest_begin(ClosestRay& q, f3 o, f3 d, float tmin, float tmax) {
   q.node = 0;
 }
 
+// Visits q.node: tests its leaf children (near to far), continues with the
+// nearest inner child and pushes the others far to near, dropping those
+// that start beyond the culling distance.  Returns true when the traversal
+// has finished (nothing left to visit).  nv / nt count nodes visited and
+// triangles tested (work counters).
+template <int LSTACK>
+OL_DEV bool closest_step(const WideNode* __restrict__ nodes, const float4* __restrict__ tri, ClosestRay& q,
+                         TravStack<LSTACK>& st, uint32_t& nv, uint32_t& nt) {
+  uint4 w0, w1, w2, w3;
+  load_node(nodes, q.node, w0, w1, w2, w3);
+  ++nv;
+  uint32_t ch[4], key[4];
+  node_children(w2, w3, ch);
+  // Cull against the current best distance widened like the slab test.
+  ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
+  sort4(key);
+  // The lexicographic minimum does not depend on the order; nearer leaves
+  // shrink the culling distance sooner.
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (key[k] == OL_NONE) continue;
+    const uint32_t c = ch[key[k] & 3u];
+    if (!child_is_leaf(c)) continue;
+    const uint32_t first = leaf_first(c), cnt = leaf_count(c);
+    nt += cnt;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t slot = first + j;
+      const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+      float t, u, v;
+      if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
+        const uint32_t orig = __float_as_uint(a.w);
+        if (t < q.best.t || (t == q.best.t && orig < q.best.orig)) q.best = TraceHit{t, u, v, slot, orig};
+      }
+    }
+  }
+  const float cull = cull_distance(q.best.t);
+  uint32_t next = OL_NONE;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (key[k] == OL_NONE) continue;
+    const uint32_t c = ch[key[k] & 3u];
+    if (child_is_leaf(c) || key_distance(key[k]) > cull) continue;
+    if (next != OL_NONE) st.push(next);
+    next = c;
+  }
+  if (next != OL_NONE) {
+    q.node = next;
+    return false;
+  }
+  if (st.empty()) return true;
+  q.node = st.pop();
+  return false;
+}
+
 // Split-step form used by the refilling scheduler.  q.node holds the next
 // item to visit: an inner node index, a leaf word (OL_LEAF_BIT set), or
 // OL_NONE once the traversal has finished.  A wave chooses per step whether
@@ -220,26 +235,13 @@ OL_DEV void closest_begin(ClosestRay& q, f3 o, f3 d, float tmin, float tmax) {
 // tests of the others.
 OL_DEV bool item_is_leaf(uint32_t item) { return child_is_leaf(item); }
 
-// Next stack entry that starts within the culling distance of the best hit
-// (entry distances are >= 0, so their bits order like the values).
 template <int LSTACK>
-OL_DEV uint32_t next_within(DistStack<LSTACK>& st, float best_t) {
-  const uint32_t cull = __float_as_uint(cull_distance(best_t));
-  while (!st.empty()) {
-    uint32_t v, d;
-    st.pop(v, d);
-    if (d <= cull) return v;
-  }
-  return OL_NONE;
-}
-
-template <int LSTACK>
-OL_DEV void closest_next(ClosestRay& q, DistStack<LSTACK>& st) { q.node = next_within(st, q.best.t); }
+OL_DEV void closest_next(ClosestRay& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
 
 // Inner step: continue with the nearest hit child (inner or leaf) and push the
 // others far to near.
 template <int LSTACK>
-OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, DistStack<LSTACK>& st, uint32_t& nv) {
+OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, TravStack<LSTACK>& st, uint32_t& nv) {
   uint4 w0, w1, w2, w3;
   load_node(nodes, q.node, w0, w1, w2, w3);
   ++nv;
@@ -247,13 +249,12 @@ OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, Dis
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
   sort4(key);
-  uint32_t next = OL_NONE, next_key = 0;
+  uint32_t next = OL_NONE;
 #pragma unroll
   for (int k = 3; k >= 0; --k) {
     if (key[k] == OL_NONE) continue;
-    if (next != OL_NONE) st.push(next, next_key & ~3u);
+    if (next != OL_NONE) st.push(next);
     next = ch[key[k] & 3u];
-    next_key = key[k];
   }
   if (next != OL_NONE) q.node = next;
   else closest_next(q, st);
@@ -261,7 +262,7 @@ OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, Dis
 
 // Leaf step: test the leaf's triangles, then take the next stack entry.
 template <int LSTACK>
-OL_DEV void closest_leaf(const float4* __restrict__ tri, ClosestRay& q, DistStack<LSTACK>& st, uint32_t& nt) {
+OL_DEV void closest_leaf(const float4* __restrict__ tri, ClosestRay& q, TravStack<LSTACK>& st, uint32_t& nt) {
   const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
   nt += cnt;
   for (uint32_t j = 0; j < cnt; ++j) {
@@ -339,15 +340,13 @@ OL_DEV void any_leaf(const float4* __restrict__ tri, AnyRay& q, TravStack<LSTACK
 // Closest hit of one ray, run to completion.
 template <int LSTACK>
 OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float4* __restrict__ tri,
-                                 uint32_t node_count, f3 o, f3 d, float tmin, float tmax, DistStack<LSTACK>& st,
+                                 uint32_t node_count, f3 o, f3 d, float tmin, float tmax, TravStack<LSTACK>& st,
                                  uint32_t& nv, uint32_t& nt) {
   ClosestRay q;
   closest_begin(q, o, d, tmin, tmax);
   if (node_count == 0) return q.best;
   st.sp = 0;
-  while (q.node != OL_NONE) {
-    if (item_is_leaf(q.node)) closest_leaf<LSTACK>(tri, q, st, nt);
-    else closest_inner<LSTACK>(nodes, q, st, nv);
+  while (!closest_step<LSTACK>(nodes, tri, q, st, nv, nt)) {
   }
   return q.best;
 }
@@ -447,12 +446,12 @@ OL_DEV void closest2_begin(Closest2& q, f3 o, f3 d, float tmin, float tmax) {
 OL_DEV bool closest2_at_leaf(const Closest2& q) { return q.inst != OL_NONE && child_is_leaf(q.node); }
 
 template <int LSTACK>
-OL_DEV void closest2_next(Closest2& q, DistStack<LSTACK>& st) { q.node = next_within(st, q.best.t); }
+OL_DEV void closest2_next(Closest2& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
 
 // Inner step: leave an instance (sentinel), enter one (top-level leaf), or
 // test a node's boxes at the current level.
 template <int LSTACK>
-OL_DEV void closest2_inner(const DScene& sc, Closest2& q, DistStack<LSTACK>& st, uint32_t& nv) {
+OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv) {
   if (q.node == OL_SENTINEL) {
     q.inst = OL_NONE;
     q.r = q.w;
@@ -467,7 +466,7 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, DistStack<LSTACK>& st,
       closest2_next(q, st);
       return;
     }
-    st.push(OL_SENTINEL, 0u);  // distance 0: the return to the top level is never culled
+    st.push(OL_SENTINEL);
     q.inst = k;
     q.offset = I.tri_offset;
     q.bnodes = B.nodes;
@@ -484,20 +483,19 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, DistStack<LSTACK>& st,
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
   sort4(key);
-  uint32_t next = OL_NONE, next_key = 0;
+  uint32_t next = OL_NONE;
 #pragma unroll
   for (int k = 3; k >= 0; --k) {
     if (key[k] == OL_NONE) continue;
-    if (next != OL_NONE) st.push(next, next_key & ~3u);
+    if (next != OL_NONE) st.push(next);
     next = ch[key[k] & 3u];
-    next_key = key[k];
   }
   if (next != OL_NONE) q.node = next;
   else closest2_next(q, st);
 }
 
 template <int LSTACK>
-OL_DEV void closest2_leaf(Closest2& q, DistStack<LSTACK>& st, uint32_t& nt) {
+OL_DEV void closest2_leaf(Closest2& q, TravStack<LSTACK>& st, uint32_t& nt) {
   const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
   nt += cnt;
   for (uint32_t j = 0; j < cnt; ++j) {
