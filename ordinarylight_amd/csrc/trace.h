@@ -351,50 +351,6 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
   return q.best;
 }
 
-// Any hit in [tmin, tmax]: returns true as soon as one triangle qualifies.
-template <int LSTACK>
-OL_DEV bool ol_trace_any(const WideNode* __restrict__ nodes, const float4* __restrict__ tri, uint32_t node_count,
-                         f3 o, f3 d, float tmin, float tmax, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& nt) {
-  if (node_count == 0) return false;
-  const RayPre r = ray_pre(o, d);
-  const float tcull = cull_distance(tmax);
-  st.sp = 0;
-  uint32_t node = 0;
-  while (true) {
-    uint4 w0, w1, w2, w3;
-    load_node(nodes, node, w0, w1, w2, w3);
-    ++nv;
-    uint32_t ch[4], key[4];
-    node_children(w2, w3, ch);
-    ol_node_keys(w0, w1, w2, r, tmin, tcull, key, ch);
-    uint32_t next = OL_NONE;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (key[c] == OL_NONE) continue;
-      if (child_is_leaf(ch[c])) {
-        const uint32_t first = leaf_first(ch[c]), cnt = leaf_count(ch[c]);
-        for (uint32_t k = 0; k < cnt; ++k) {
-          const uint32_t slot = first + k;
-          const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
-          float t, u, v;
-          ++nt;
-          if (ol_tri_test(o, d, a, e1, e2, t, u, v) && t >= tmin && t <= tmax) return true;
-        }
-      } else if (next == OL_NONE) {
-        next = ch[c];
-      } else {
-        st.push(ch[c]);
-      }
-    }
-    if (next != OL_NONE) {
-      node = next;
-      continue;
-    }
-    if (st.empty()) return false;
-    node = st.pop();
-  }
-}
-
 // ---- two-level traversal (top-level BVH over instances of bottom-level BVHs) -----------
 // The stack is shared by both levels: entering an instance pushes
 // OL_SENTINEL, and popping it returns to the top level.  The ray is
