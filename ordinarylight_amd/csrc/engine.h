@@ -120,8 +120,15 @@ class Engine {
   };
 
   void ensure_buffers();
-  void capture_graphs();
-  void launch_step(uint32_t in, bool timed);
+  // A step's shape: queue parity, STEP_* kinds of resident work, and whether
+  // it accumulates or generates (bits 1.. of the key).  Absent work is not
+  // launched; each shape is captured once as a hipGraph and replayed.
+  static uint32_t step_key(uint32_t in, uint32_t kinds, bool acc, bool gen) {
+    return in | (kinds << 1) | (acc ? 16u : 0u) | (gen ? 32u : 0u);
+  }
+  hipGraphExec_t graph_for(uint32_t key);
+  void clear_graphs();
+  void launch_step(uint32_t key, bool timed);
   bool admit(Job& j, int slot);
   GiLaunch make_launch();
   double now_ms() const;
@@ -152,7 +159,7 @@ class Engine {
   GiLaunch launch_{};
   bool counting_ = false;
   bool profiling_ = false;
-  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  std::unordered_map<uint32_t, hipGraphExec_t> graphs_;  // by step_key
   bool graph_valid_ = false;
   // Pinned host staging.
   JobDev* jobs_host_ = nullptr;

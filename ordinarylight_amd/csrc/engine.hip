@@ -131,8 +131,7 @@ Engine::~Engine() {
   stop_worker();
   (void)hipSetDevice(ctx_->device);
   (void)hipStreamSynchronize(ctx_->stream);
-  for (auto& g : graph_)
-    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   if (done_) (void)hipEventDestroy(done_);
   for (auto& x : prof_ev_)
     if (x) (void)hipEventDestroy(x);
@@ -382,50 +381,56 @@ bool Engine::admit(Job& j, int slot) {
   return false;
 }
 
-void Engine::launch_step(uint32_t in, bool timed) {
+void Engine::launch_step(uint32_t key, bool timed) {
   hipStream_t s = ctx_->stream;
-  wave_accumulate(wave_, ctx_->cu_count, s);
-  wave_generate(wave_, in, ctx_->cu_count, s);
+  const uint32_t in = key & 1u, kinds = (key >> 1) & 7u;
+  if (key & 16u) wave_accumulate(wave_, ctx_->cu_count, s);
+  if (key & 32u) wave_generate(wave_, in, ctx_->cu_count, s);
   if (!timed) {
-    wave_trace(launch_, wave_, in, s);
-    wave_shade(launch_, wave_, in, s);
-    wave_shadow(launch_, wave_, s);
+    wave_trace(launch_, wave_, in, kinds, s);
+    wave_shade(launch_, wave_, in, kinds, s);
+    wave_shadow(launch_, wave_, kinds, s);
   } else {
     // Read after the step's completion (no synchronisation inside the step).
     for (auto& x : prof_ev_)
       if (!x) OL_HIP_CHECK(hipEventCreate(&x));
     OL_HIP_CHECK(hipEventRecord(prof_ev_[0], s));
-    wave_trace(launch_, wave_, in, s);
+    wave_trace(launch_, wave_, in, kinds, s);
     OL_HIP_CHECK(hipEventRecord(prof_ev_[1], s));
-    wave_shade(launch_, wave_, in, s);
+    wave_shade(launch_, wave_, in, kinds, s);
     OL_HIP_CHECK(hipEventRecord(prof_ev_[2], s));
-    wave_shadow(launch_, wave_, s);
+    wave_shadow(launch_, wave_, kinds, s);
     OL_HIP_CHECK(hipEventRecord(prof_ev_[3], s));
   }
   wave_epilogue(wave_, in, s);
 }
 
-void Engine::capture_graphs() {
-  hipStream_t s = ctx_->stream;
-  for (uint32_t in = 0; in < 2; ++in) {
-    if (graph_[in]) {
-      OL_HIP_CHECK(hipGraphExecDestroy(graph_[in]));
-      graph_[in] = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    OL_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      launch_step(in, false);
-    } catch (...) {
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
-    }
-    OL_HIP_CHECK(hipStreamEndCapture(s, &g));
-    OL_HIP_CHECK(hipGraphInstantiate(&graph_[in], g, nullptr, nullptr, 0));
-    OL_HIP_CHECK(hipGraphDestroy(g));
-  }
+void Engine::clear_graphs() {
+  for (auto& kv : graphs_) OL_HIP_CHECK(hipGraphExecDestroy(kv.second));
+  graphs_.clear();
   graph_valid_ = true;
+}
+
+hipGraphExec_t Engine::graph_for(uint32_t key) {
+  if (!graph_valid_) clear_graphs();
+  auto it = graphs_.find(key);
+  if (it != graphs_.end()) return it->second;
+  hipStream_t s = ctx_->stream;
+  hipGraph_t g = nullptr;
+  OL_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    launch_step(key, false);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  OL_HIP_CHECK(hipStreamEndCapture(s, &g));
+  hipGraphExec_t exec = nullptr;
+  OL_HIP_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  OL_HIP_CHECK(hipGraphDestroy(g));
+  graphs_[key] = exec;
+  return exec;
 }
 
 uint32_t Engine::step() {
@@ -433,6 +438,7 @@ uint32_t Engine::step() {
   OL_HIP_CHECK(hipSetDevice(ctx_->device));
   const double t_begin = now_ms();
   bool any_resident = false;
+  uint32_t kinds = 0;
   std::vector<uint64_t> finals;  // jobs whose last chunk is accumulated in this step
   {
     std::lock_guard<std::mutex> g(mu_);
@@ -469,7 +475,13 @@ uint32_t Engine::step() {
     finals = pending_acc_jobs_;
     pending_acc_.clear();
     pending_acc_jobs_.clear();
-    for (uint32_t k = 0; k < kMaxJobs; ++k) any_resident = any_resident || slot_job_[k] != 0;
+    for (uint32_t k = 0; k < kMaxJobs; ++k) {
+      if (slot_job_[k] == 0) continue;
+      any_resident = true;
+      const olhip_job_desc& d = jobs_[slot_job_[k]]->desc;
+      kinds |= d.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
+      if (d.settings.restir_candidates) kinds |= STEP_RESTIR;
+    }
     totals_.resident_jobs = 0;
     for (uint32_t k = 0; k < kMaxJobs; ++k) totals_.resident_jobs += slot_job_[k] != 0;
     totals_.max_resident_jobs = std::max(totals_.max_resident_jobs, totals_.resident_jobs);
@@ -482,14 +494,14 @@ uint32_t Engine::step() {
   OL_HIP_CHECK(hipMemcpyAsync(list_dev_.ptr, list_host_, sizeof(StepList), hipMemcpyHostToDevice, s));
   const uint32_t in = parity_;
   const bool timed = any_resident && profiling_;
+  const uint32_t key = step_key(in, kinds, Ls.n_acc > 0, Ls.n_gen > 0);
   if (!any_resident) {
     // Only accumulation left: no traversal launches.
     wave_accumulate(wave_, ctx_->cu_count, s);
   } else if (profiling_ || !cfg_.use_graph || steps_ == 0) {
-    launch_step(in, profiling_);
+    launch_step(key, profiling_);
   } else {
-    if (!graph_valid_) capture_graphs();
-    OL_HIP_CHECK(hipGraphLaunch(graph_[in], s));
+    OL_HIP_CHECK(hipGraphLaunch(graph_for(key), s));
   }
   OL_HIP_CHECK(hipGetLastError());
   OL_HIP_CHECK(hipMemcpyAsync(report_host_, ctr_.as<uint32_t>() + C_OVERFLOW, 4ull * (C_WORDS - C_OVERFLOW),

@@ -158,12 +158,14 @@ int stack_depth_for(uint32_t bvh_depth);
 // and report per-job counts.  `in` is the parity of the input ray queue.
 // With `marks`, events are recorded around the closest-hit, shade and shadow
 // launches (profiling).
+// `kinds` says which resident work a step holds (STEP_* bits); the launches
+// for absent kinds are left out (their chunk totals stay zero).
+constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u;
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
 void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
-void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
-void wave_trace_kernels(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
-void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
-void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st);
+void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
+void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
+void wave_shadow(const GiLaunch& l, const Wave& w, uint32_t kinds, hipStream_t st);
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
 void wave_clear(const Wave& w, hipStream_t st);
 

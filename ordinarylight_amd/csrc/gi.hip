@@ -1283,6 +1283,14 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
     W.ctr[C_E + r] = 0u;
   }
   if (t < 4 * kWorkStripes) W.ctr[C_W + t] = 0u;
+  // Chunk totals of the four plans: a plan left out of the next step (no
+  // resident job of its kind) must read as empty.
+  if (t == 0) {
+    W.ctr[C_RP + kRuns] = 0u;
+    W.ctr[C_GP + kRuns] = 0u;
+    W.ctr[C_RP2 + kRuns] = 0u;
+    W.ctr[C_GP2 + kRuns] = 0u;
+  }
   __syncthreads();
   if (t < 4 * kMaxJobs) W.ctr[C_REPORT + t] = rep[t];
 }
@@ -1406,19 +1414,19 @@ void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st) {
   hipLaunchKernelGGL(k_generate, dim3((unsigned)(cu_count * 8)), dim3(256), 0, st, w, in);
 }
 
-void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP, w.jobs, 0u);
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP2, w.jobs, 1u);
-  wave_trace_kernels(l, w, in, st);
+void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
+  if (kinds & STEP_FLAT) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP, w.jobs, 0u);
+    [&] { OL_STACK_DISPATCH(launch_closest, l, w, in, st); }();
+  }
+  if (kinds & STEP_INSTANCED) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP2, w.jobs, 1u);
+    [&] { OL_STACK_DISPATCH(launch_closest2, l, w, in, st); }();
+  }
 }
 
-void wave_trace_kernels(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  [&] { OL_STACK_DISPATCH(launch_closest, l, w, in, st); }();
-  OL_STACK_DISPATCH(launch_closest2, l, w, in, st);
-}
-
-void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  if (w.rsamp) {
+void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
+  if (w.rsamp && (kinds & STEP_RESTIR)) {
     static int rocc = resident_blocks(k_restir, 256);
     hipLaunchKernelGGL(k_restir, dim3((unsigned)(l.cu_count * rocc)), dim3(256), 0, st, w, in);
   }
@@ -1426,11 +1434,15 @@ void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
   hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in);
 }
 
-void wave_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP, w.jobs, 0u);
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP2, w.jobs, 1u);
-  [&] { OL_STACK_DISPATCH(launch_shadow, l, w, st); }();
-  OL_STACK_DISPATCH(launch_shadow2, l, w, st);
+void wave_shadow(const GiLaunch& l, const Wave& w, uint32_t kinds, hipStream_t st) {
+  if (kinds & STEP_FLAT) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP, w.jobs, 0u);
+    [&] { OL_STACK_DISPATCH(launch_shadow, l, w, st); }();
+  }
+  if (kinds & STEP_INSTANCED) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP2, w.jobs, 1u);
+    [&] { OL_STACK_DISPATCH(launch_shadow2, l, w, st); }();
+  }
 }
 
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {
