@@ -268,6 +268,72 @@ def c3_measurement(context, steps=2, builder="lbvh"):
     return out
 
 
+def c1_measurement(repeats=5):
+    """Box room of 36 triangles (five walls, two boxes rotated about y, one
+    emissive quad of radiance 15), 256x256, 4 spp, 4 bounces, seed 0, frame 0:
+    the reference's CPU estimator (``CpuReferenceRenderer`` semantics, PCG64
+    stream drawn on the host) run on the GPU, best of ``repeats`` frames after
+    one warm-up, beside the NumPy restatement of the same estimator timed the
+    same way on one host thread."""
+    from oracle import e0_numpy
+    from ordinarylight_amd import workloads
+    from ordinarylight_amd.renderers.hip import HipReferenceRenderer
+    scene, camera = workloads.box_room(0)
+    W = H = 256
+    r = HipReferenceRenderer(samples_per_pixel=4, max_bounces=4, seed=0)
+    r.render_frame(scene, camera, W, H)  # warm-up: scene upload, first launches
+    best = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        img = r.render_frame(scene, camera, W, H)
+        best = min(best, time.perf_counter() - t0)
+    rays = r.last_timings["rays"]
+    r.close()
+    e0_numpy.render_scene(scene, camera, W, H, samples=4, max_bounces=4, seed=0)
+    cpu_best = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ref = e0_numpy.render_scene(scene, camera, W, H, samples=4, max_bounces=4, seed=0)
+        cpu_best = min(cpu_best, time.perf_counter() - t0)
+    a, b = img[..., :3], ref[..., :3]
+    close = np.all(np.abs(a - b) <= 1e-6 + 1e-3 * np.maximum(np.abs(a), np.abs(b)), axis=-1)
+    return {
+        "workload": "box room 36 triangles, 256x256, 4 spp, 4 bounces, seed 0 (reference CPU estimator)",
+        "ms_per_frame": round(best * 1e3, 2), "mrays_per_s": round(rays / best / 1e6, 2),
+        "rays_per_frame": int(rays),
+        "cpu_numpy_ms_per_frame": round(cpu_best * 1e3, 1),
+        "cpu_numpy_mrays_per_s": round(rays / cpu_best / 1e6, 3), "cpu_cores": 1,
+        "pixels_within_1e-3": round(float(close.mean()), 6),
+        "timing": f"best of {repeats} after one warm-up, host PCG64 draws included",
+    }
+
+
+def builder_comparison(scene, camera, W, H, spp, B, leaf, builder, frames=3):
+    """The same frame with the scene's BVH built by the other builder: build
+    time, nodes per closest-hit ray and Mrays/s over ``frames`` frames."""
+    from ordinarylight_amd.renderers.hip import HipPathTracer
+    r = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
+                      bvh_leaf_size=leaf, bvh_builder=builder)
+    r.replace_scene(scene)
+    build_ms = r._device_scene.info.build_ms
+    r.reconfigure(work_counters=True)
+    r._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+    work = r.last_stats
+    r.reconfigure(work_counters=False)
+    r._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+    rays = 0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        r._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+        st = r.last_stats
+        rays += st.primary_rays + st.extension_rays + st.shadow_rays
+    wall = time.perf_counter() - t0
+    r.close()
+    closest = work.primary_rays + work.extension_rays
+    return {"builder": builder, "mrays_per_s": round(rays / wall / 1e6, 1), "bvh_build_ms": round(build_ms, 2),
+            "nodes_per_closest_ray": round(work.closest_nodes / max(closest, 1), 2)}
+
+
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -291,6 +357,9 @@ def main(argv=None):
                         help="Poisson arrival rate as a fraction of the batched jobs/s")
     parser.add_argument("--no-autotune", action="store_true")
     parser.add_argument("--no-c3", action="store_true", help="skip the instanced two-level workload")
+    parser.add_argument("--no-c1", action="store_true", help="skip the reference-estimator box room")
+    parser.add_argument("--no-builder-comparison", action="store_true",
+                        help="skip the plain frame with the other BVH builder")
     parser.add_argument("--c3-only", action="store_true",
                         help="run only the instanced workload and print its record (profiling aid)")
     args = parser.parse_args(argv)
@@ -406,9 +475,16 @@ def main(argv=None):
             "nodes_per_shadow_ray": round(work.shadow_nodes / max(work.shadow_rays, 1), 2),
         },
     }
+    renderer.close()
+    if not args.no_builder_comparison:
+        other = "lbvh" if args.builder == "sah" else "sah"
+        line["extra"]["other_builder"] = builder_comparison(scene, camera, W, H, spp, B, args.leaf_size, other)
+        log(f"other builder: {line['extra']['other_builder']}")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(scene, camera, W, H, spp, B, args.cpu_seconds)
-    renderer.close()
+    if not args.no_c1:
+        line["c1"] = c1_measurement()
+        log(f"c1: {line['c1']}")
 
     if not args.no_c3:
         from ordinarylight_amd.renderers.hip import DeviceContext

@@ -5,9 +5,11 @@
 // contiguous slot range for their next sample chunk is free (admission by
 // memory), advance together one wavefront step at a time, and release
 // their ranges when their paths have terminated.  A step is a fixed
-// sequence of launches, captured once per queue parity as a hipGraph and
-// replayed.  A worker thread can drive steps while other threads submit and
-// wait (serving).
+// sequence of launches, captured once per step shape as a hipGraph and
+// replayed.  Steps are pipelined: the next step is queued on the stream
+// before the host reads the previous step's report, so report handling and
+// launch overlap device work.  A worker thread can drive steps while other
+// threads submit and wait (serving).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -72,8 +74,11 @@ class Engine {
   ~Engine();
 
   uint64_t submit(const olhip_job_desc& job);
-  // One step.  Returns the number of jobs not yet done (queued, resident or
-  // accumulating).  Callers hold no lock; the engine serialises steps.
+  // Queues the next step when there is work for it (keeping at most two in
+  // flight), then waits for the oldest queued step and applies its report.
+  // Returns the number of jobs not yet done (queued, resident or
+  // accumulating) plus the steps still in flight.  Callers hold no lock; the
+  // engine serialises steps.
   uint32_t step();
   void run_until_idle();
   int state(uint64_t id);
@@ -111,7 +116,9 @@ class Engine {
     uint32_t chunk_spp = 0, chunk_base_sample = 0;
     uint64_t region = 0, span = 0, ebase = 0, espan = 0;
     bool first_chunk = true;
-    uint32_t steps_in_chunk = 0;
+    uint32_t steps_in_chunk = 0;   // steps of the resident chunk whose reports were applied
+    uint32_t steps_launched = 0;   // steps of the resident chunk queued so far
+    uint64_t admit_seq = 0;        // step that generates the resident chunk
     uint32_t passed = 0;        // admission rounds in which this job did not fit
     int status = OLHIP_JOB_QUEUED;
     DevBuf acc, image, geo, prim, lum, variance, res, surf;
@@ -128,7 +135,20 @@ class Engine {
   }
   hipGraphExec_t graph_for(uint32_t key);
   void clear_graphs();
-  void launch_step(uint32_t key, bool timed);
+  void launch_step(uint32_t key, bool timed, int buf);
+  // A step queued on the stream whose report has not been applied yet.
+  struct InFlight {
+    uint64_t seq = 0;
+    bool traced = false;  // traversal ran (otherwise accumulation only)
+    bool timed = false;
+    int buf = 0;          // staging, report and event set of the step
+    std::vector<uint64_t> finals;  // jobs whose last chunk this step accumulates
+  };
+  // Builds and queues the next step; false when there is nothing to do.
+  bool launch_next();
+  void retire_front();
+  // Waits for the device before buffers that queued steps use are replaced.
+  void quiesce();
   bool admit(Job& j, int slot);
   GiLaunch make_launch();
   double now_ms() const;
@@ -147,6 +167,10 @@ class Engine {
   uint64_t next_id_ = 1;
   uint32_t parity_ = 0;           // input queue of the next step
   uint64_t steps_ = 0;
+  uint64_t seq_ = 0;              // sequence number of the next step queued
+  std::deque<InFlight> inflight_;
+  std::atomic<uint32_t> inflight_count_{0};  // inflight_.size() for readers without step_mu_
+  uint64_t overflow_checked_from_ = 0;       // first step whose stack-overflow flag is still unhandled
   int max_depth_ = 1;
   bool stk_needed_ = false;
   bool restir_needed_ = false;
@@ -161,12 +185,15 @@ class Engine {
   bool profiling_ = false;
   std::unordered_map<uint32_t, hipGraphExec_t> graphs_;  // by step_key
   bool graph_valid_ = false;
-  // Pinned host staging.
-  JobDev* jobs_host_ = nullptr;
-  StepList* list_host_ = nullptr;
-  uint32_t* report_host_ = nullptr;
-  hipEvent_t done_ = nullptr;
-  hipEvent_t prof_ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // Job table (host copy; the device copy is refreshed when it changes) and
+  // pinned staging, one set per in-flight step.
+  std::vector<JobDev> jobs_state_;
+  bool jobs_dirty_ = true;
+  JobDev* jobs_host_[2] = {nullptr, nullptr};
+  StepList* list_host_[2] = {nullptr, nullptr};
+  uint32_t* report_host_[2] = {nullptr, nullptr};
+  hipEvent_t done_[2] = {nullptr, nullptr};
+  hipEvent_t prof_ev_[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
   double t0_ms_ = 0;
   // Engine totals.
   olhip_engine_stats totals_{};

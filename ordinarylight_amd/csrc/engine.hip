@@ -1,5 +1,6 @@
-// Batched render engine: admission, sample chunking, step launch (direct or
-// captured hipGraph), per-job reports and the serving worker thread.
+// Batched render engine: admission, sample chunking, pipelined step launch
+// (direct or captured hipGraph), per-job reports and the serving worker
+// thread.
 
 #include <algorithm>
 #include <chrono>
@@ -109,10 +110,13 @@ Engine::Engine(olhip_ctx* ctx, const olhip_engine_config& cfg) : ctx_(ctx), cfg_
   slot_job_.assign(kMaxJobs, 0);
   paths_.reset(slots_);
   entries_.reset(eslots_);
-  jobs_host_ = pinned<JobDev>(kMaxJobs);
-  list_host_ = pinned<StepList>(1);
-  report_host_ = pinned<uint32_t>(C_WORDS - C_OVERFLOW);
-  OL_HIP_CHECK(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+  jobs_state_.assign(kMaxJobs, JobDev{});
+  for (int b = 0; b < 2; ++b) {
+    jobs_host_[b] = pinned<JobDev>(kMaxJobs);
+    list_host_[b] = pinned<StepList>(1);
+    report_host_[b] = pinned<uint32_t>(C_WORDS - C_OVERFLOW);
+    OL_HIP_CHECK(hipEventCreateWithFlags(&done_[b], hipEventDisableTiming));
+  }
   t0_ms_ = 0;
   t0_ms_ = now_ms();
   ctr_.reserve(4ull * C_WORDS);
@@ -132,12 +136,18 @@ Engine::~Engine() {
   (void)hipSetDevice(ctx_->device);
   (void)hipStreamSynchronize(ctx_->stream);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
-  if (done_) (void)hipEventDestroy(done_);
-  for (auto& x : prof_ev_)
-    if (x) (void)hipEventDestroy(x);
-  if (jobs_host_) (void)hipHostFree(jobs_host_);
-  if (list_host_) (void)hipHostFree(list_host_);
-  if (report_host_) (void)hipHostFree(report_host_);
+  for (int b = 0; b < 2; ++b) {
+    if (done_[b]) (void)hipEventDestroy(done_[b]);
+    for (auto& x : prof_ev_[b])
+      if (x) (void)hipEventDestroy(x);
+    if (jobs_host_[b]) (void)hipHostFree(jobs_host_[b]);
+    if (list_host_[b]) (void)hipHostFree(list_host_[b]);
+    if (report_host_[b]) (void)hipHostFree(report_host_[b]);
+  }
+}
+
+void Engine::quiesce() {
+  if (!inflight_.empty()) OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
 }
 
 double Engine::now_ms() const {
@@ -146,6 +156,7 @@ double Engine::now_ms() const {
 }
 
 void Engine::ensure_buffers() {
+  quiesce();
   T_.reserve(16 * slots_);
   L_.reserve(16 * slots_);
   M_.reserve(16 * slots_);
@@ -190,6 +201,7 @@ void Engine::ensure_buffers() {
 }
 
 GiLaunch Engine::make_launch() {
+  quiesce();
   GiLaunch l{};
   l.cu_count = ctx_->cu_count;
   l.stack_depth = (int)(3u * (uint32_t)std::max(max_depth_, 1) + 1u);
@@ -290,7 +302,10 @@ bool Engine::admit(Job& j, int slot) {
       j.next_sample += (uint32_t)spp;
       j.slot = slot;
       j.steps_in_chunk = 0;
-      JobDev& J = jobs_host_[slot];
+      j.steps_launched = 0;
+      j.admit_seq = seq_;
+      jobs_dirty_ = true;
+      JobDev& J = jobs_state_[slot];
       std::memset(&J, 0, sizeof(J));
       GiParams& P = J.P;
       P.sc = d.scene->view;
@@ -381,7 +396,7 @@ bool Engine::admit(Job& j, int slot) {
   return false;
 }
 
-void Engine::launch_step(uint32_t key, bool timed) {
+void Engine::launch_step(uint32_t key, bool timed, int buf) {
   hipStream_t s = ctx_->stream;
   const uint32_t in = key & 1u, kinds = (key >> 1) & 7u;
   if (key & 16u) wave_accumulate(wave_, ctx_->cu_count, s);
@@ -391,21 +406,23 @@ void Engine::launch_step(uint32_t key, bool timed) {
     wave_shade(launch_, wave_, in, kinds, s);
     wave_shadow(launch_, wave_, kinds, s);
   } else {
-    // Read after the step's completion (no synchronisation inside the step).
-    for (auto& x : prof_ev_)
-      if (!x) OL_HIP_CHECK(hipEventCreate(&x));
-    OL_HIP_CHECK(hipEventRecord(prof_ev_[0], s));
+    // Read when the step is retired (no synchronisation inside the step).
+    hipEvent_t* ev = prof_ev_[buf];
+    for (int k = 0; k < 4; ++k)
+      if (!ev[k]) OL_HIP_CHECK(hipEventCreate(&ev[k]));
+    OL_HIP_CHECK(hipEventRecord(ev[0], s));
     wave_trace(launch_, wave_, in, kinds, s);
-    OL_HIP_CHECK(hipEventRecord(prof_ev_[1], s));
+    OL_HIP_CHECK(hipEventRecord(ev[1], s));
     wave_shade(launch_, wave_, in, kinds, s);
-    OL_HIP_CHECK(hipEventRecord(prof_ev_[2], s));
+    OL_HIP_CHECK(hipEventRecord(ev[2], s));
     wave_shadow(launch_, wave_, kinds, s);
-    OL_HIP_CHECK(hipEventRecord(prof_ev_[3], s));
+    OL_HIP_CHECK(hipEventRecord(ev[3], s));
   }
   wave_epilogue(wave_, in, s);
 }
 
 void Engine::clear_graphs() {
+  quiesce();
   for (auto& kv : graphs_) OL_HIP_CHECK(hipGraphExecDestroy(kv.second));
   graphs_.clear();
   graph_valid_ = true;
@@ -417,9 +434,10 @@ hipGraphExec_t Engine::graph_for(uint32_t key) {
   if (it != graphs_.end()) return it->second;
   hipStream_t s = ctx_->stream;
   hipGraph_t g = nullptr;
+  quiesce();
   OL_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    launch_step(key, false);
+    launch_step(key, false, 0);
   } catch (...) {
     (void)hipStreamEndCapture(s, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -437,12 +455,57 @@ uint32_t Engine::step() {
   std::lock_guard<std::mutex> step_guard(step_mu_);
   OL_HIP_CHECK(hipSetDevice(ctx_->device));
   const double t_begin = now_ms();
-  bool any_resident = false;
-  uint32_t kinds = 0;
-  std::vector<uint64_t> finals;  // jobs whose last chunk is accumulated in this step
+  try {
+    // While no job waits for admission, one step stays queued behind the
+    // one the host waits for, so the report handling and the next launch
+    // overlap device work.  Jobs waiting for slots are better served by
+    // building each step after the previous report, which frees the slots
+    // of finished chunks one step earlier.
+    bool waiting;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      waiting = !queue_.empty();
+    }
+    const size_t depth = waiting ? 1u : 2u;
+    while (inflight_.size() < depth && launch_next()) {
+    }
+    if (!inflight_.empty()) retire_front();
+  } catch (...) {
+    // The queued steps' reports can no longer be matched to jobs.
+    (void)hipStreamSynchronize(ctx_->stream);
+    inflight_.clear();
+    inflight_count_ = 0;
+    throw;
+  }
+  inflight_count_ = (uint32_t)inflight_.size();
+  uint32_t pending = 0;
   {
     std::lock_guard<std::mutex> g(mu_);
-    StepList& Ls = *list_host_;
+    totals_.steps = steps_;
+    totals_.busy_ms += now_ms() - t_begin;
+    for (const auto& kv : jobs_)
+      if (kv.second->status == OLHIP_JOB_QUEUED || kv.second->status == OLHIP_JOB_RUNNING) ++pending;
+    cv_.notify_all();
+  }
+  return pending + (uint32_t)inflight_.size();
+}
+
+// Builds the next step from the state the applied reports leave: admission,
+// the chunks to accumulate, and the resident chunks that may still hold live
+// paths.  A chunk's paths end by its max_bounces-th step, so steps queued
+// ahead of a report are only issued for chunks that can still need them; a
+// step that finds no paths of a chunk (they all ended earlier) costs only
+// empty launches.
+bool Engine::launch_next() {
+  const int b = (int)(seq_ & 1u);
+  StepList& Ls = *list_host_[b];
+  bool traced = false;
+  uint32_t kinds = 0;
+  InFlight f;
+  f.seq = seq_;
+  f.buf = b;
+  {
+    std::lock_guard<std::mutex> g(mu_);
     Ls.n_gen = 0;
     Ls.gen_prefix[0] = 0;
     // Admission: first-fit over the queue; a job passed over for 8 steps
@@ -458,7 +521,7 @@ uint32_t Engine::step() {
       Job& j = *jobs_[*it];
       if (admit(j, slot)) {
         Ls.gen_job[Ls.n_gen] = (uint32_t)slot;
-        Ls.gen_prefix[Ls.n_gen + 1] = Ls.gen_prefix[Ls.n_gen] + jobs_host_[slot].n_paths;
+        Ls.gen_prefix[Ls.n_gen + 1] = Ls.gen_prefix[Ls.n_gen] + jobs_state_[slot].n_paths;
         ++Ls.n_gen;
         it = queue_.erase(it);
         continue;
@@ -472,85 +535,116 @@ uint32_t Engine::step() {
       Ls.acc[k] = pending_acc_[k];
       Ls.acc_prefix[k + 1] = Ls.acc_prefix[k] + pending_acc_[k].np;
     }
-    finals = pending_acc_jobs_;
+    f.finals = pending_acc_jobs_;
     pending_acc_.clear();
     pending_acc_jobs_.clear();
     for (uint32_t k = 0; k < kMaxJobs; ++k) {
       if (slot_job_[k] == 0) continue;
-      any_resident = true;
-      const olhip_job_desc& d = jobs_[slot_job_[k]]->desc;
-      kinds |= d.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
-      if (d.settings.restir_candidates) kinds |= STEP_RESTIR;
+      Job& j = *jobs_[slot_job_[k]];
+      if (j.steps_launched >= j.desc.settings.max_bounces) continue;
+      ++j.steps_launched;
+      traced = true;
+      kinds |= j.desc.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
+      if (j.desc.settings.restir_candidates) kinds |= STEP_RESTIR;
     }
     totals_.resident_jobs = 0;
     for (uint32_t k = 0; k < kMaxJobs; ++k) totals_.resident_jobs += slot_job_[k] != 0;
     totals_.max_resident_jobs = std::max(totals_.max_resident_jobs, totals_.resident_jobs);
     totals_.queued_jobs = (uint32_t)queue_.size();
   }
-  const StepList& Ls = *list_host_;
-  if (!any_resident && Ls.n_acc == 0) return 0;
+  if (!traced && Ls.n_acc == 0) return false;
   hipStream_t s = ctx_->stream;
-  OL_HIP_CHECK(hipMemcpyAsync(jobs_dev_.ptr, jobs_host_, sizeof(JobDev) * kMaxJobs, hipMemcpyHostToDevice, s));
-  OL_HIP_CHECK(hipMemcpyAsync(list_dev_.ptr, list_host_, sizeof(StepList), hipMemcpyHostToDevice, s));
+  if (jobs_dirty_) {
+    std::memcpy(jobs_host_[b], jobs_state_.data(), sizeof(JobDev) * kMaxJobs);
+    OL_HIP_CHECK(hipMemcpyAsync(jobs_dev_.ptr, jobs_host_[b], sizeof(JobDev) * kMaxJobs, hipMemcpyHostToDevice, s));
+    jobs_dirty_ = false;
+  }
+  // Only accumulation and generation read the step list.
+  if (Ls.n_acc || Ls.n_gen)
+    OL_HIP_CHECK(hipMemcpyAsync(list_dev_.ptr, list_host_[b], sizeof(StepList), hipMemcpyHostToDevice, s));
   const uint32_t in = parity_;
-  const bool timed = any_resident && profiling_;
+  f.traced = traced;
+  f.timed = traced && profiling_;
   const uint32_t key = step_key(in, kinds, Ls.n_acc > 0, Ls.n_gen > 0);
-  if (!any_resident) {
+  if (!traced) {
     // Only accumulation left: no traversal launches.
     wave_accumulate(wave_, ctx_->cu_count, s);
-  } else if (profiling_ || !cfg_.use_graph || steps_ == 0) {
-    launch_step(key, profiling_);
+  } else if (profiling_ || !cfg_.use_graph || seq_ == 0) {
+    launch_step(key, profiling_, b);
   } else {
     OL_HIP_CHECK(hipGraphLaunch(graph_for(key), s));
   }
   OL_HIP_CHECK(hipGetLastError());
-  OL_HIP_CHECK(hipMemcpyAsync(report_host_, ctr_.as<uint32_t>() + C_OVERFLOW, 4ull * (C_WORDS - C_OVERFLOW),
+  OL_HIP_CHECK(hipMemcpyAsync(report_host_[b], ctr_.as<uint32_t>() + C_OVERFLOW, 4ull * (C_WORDS - C_OVERFLOW),
                               hipMemcpyDeviceToHost, s));
-  OL_HIP_CHECK(hipEventRecord(done_, s));
-  OL_HIP_CHECK(hipEventSynchronize(done_));
-  if (timed) {
+  OL_HIP_CHECK(hipEventRecord(done_[b], s));
+  if (traced) parity_ ^= 1u;
+  ++seq_;
+  inflight_.push_back(std::move(f));
+  return true;
+}
+
+// Waits for the oldest queued step and applies its report.  Reports are
+// matched to the chunks that were resident when the step was built: a slot
+// released and re-admitted since then belongs to a later chunk, whose
+// admission step is later than the report's.
+void Engine::retire_front() {
+  InFlight f = std::move(inflight_.front());
+  inflight_.pop_front();
+  OL_HIP_CHECK(hipEventSynchronize(done_[f.buf]));
+  if (f.timed) {
     float a = 0, b = 0, c = 0;
-    OL_HIP_CHECK(hipEventElapsedTime(&a, prof_ev_[0], prof_ev_[1]));
-    OL_HIP_CHECK(hipEventElapsedTime(&b, prof_ev_[1], prof_ev_[2]));
-    OL_HIP_CHECK(hipEventElapsedTime(&c, prof_ev_[2], prof_ev_[3]));
+    OL_HIP_CHECK(hipEventElapsedTime(&a, prof_ev_[f.buf][0], prof_ev_[f.buf][1]));
+    OL_HIP_CHECK(hipEventElapsedTime(&b, prof_ev_[f.buf][1], prof_ev_[f.buf][2]));
+    OL_HIP_CHECK(hipEventElapsedTime(&c, prof_ev_[f.buf][2], prof_ev_[f.buf][3]));
     trace_ms += a;
     shade_ms += b;
     shadow_ms += c;
     ++trace_launches;
   }
-  uint32_t pending = 0;
+  bool reset_overflow = false;
   {
     std::lock_guard<std::mutex> g(mu_);
     const double t = now_ms();
-    for (uint64_t id : finals) {
+    for (uint64_t id : f.finals) {
       auto it = jobs_.find(id);
       if (it == jobs_.end()) continue;
       it->second->status = OLHIP_JOB_DONE;
       it->second->stats.done_ms = t;
       ++totals_.jobs_done;
     }
-    if (any_resident) {
+    if (f.traced) {
       ++steps_;
-      const uint32_t* rep = report_host_ + (C_REPORT - C_OVERFLOW);
-      const bool stack_overflow = report_host_[C_STACK - C_OVERFLOW] != 0;
-      totals_.queue_overflow = report_host_[0];
+      const uint32_t* report = report_host_[f.buf];
+      const uint32_t* rep = report + (C_REPORT - C_OVERFLOW);
+      // The overflow flag is sticky until reset; a reset queued after an
+      // overflow follows the steps already in flight, whose reports still
+      // show it.  Those steps' jobs were failed with the first report.
+      const bool stack_overflow = report[C_STACK - C_OVERFLOW] != 0 && f.seq >= overflow_checked_from_;
+      totals_.queue_overflow = report[0];
       for (uint32_t k = 0; k < kMaxJobs; ++k) {
         if (slot_job_[k] == 0) continue;
         Job& j = *jobs_[slot_job_[k]];
+        // A traversal stack overflow fails every resident chunk, including
+        // those of the step queued behind this one.
+        if (j.admit_seq > f.seq && !stack_overflow) continue;
         const uint32_t rays = rep[4 * k], shadow = rep[4 * k + 1], live = rep[4 * k + 2];
-        if (j.steps_in_chunk == 0) j.stats.primary_rays += rays;
-        else j.stats.extension_rays += rays;
-        j.stats.shadow_rays += shadow;
-        totals_.rays += (uint64_t)rays + shadow;
-        ++j.steps_in_chunk;
-        ++j.stats.steps;
+        if (j.admit_seq <= f.seq) {
+          if (j.steps_in_chunk == 0) j.stats.primary_rays += rays;
+          else j.stats.extension_rays += rays;
+          j.stats.shadow_rays += shadow;
+          totals_.rays += (uint64_t)rays + shadow;
+          ++j.steps_in_chunk;
+          ++j.stats.steps;
+        }
         if (stack_overflow) {
           j.status = OLHIP_JOB_FAILED;
           j.error = "traversal stack overflow";
         }
-        if (live == 0 || stack_overflow) {
-          // Chunk finished: accumulate next step, release its slots now (the
-          // accumulation precedes any generate that reuses them).
+        if ((live == 0 && j.admit_seq <= f.seq) || stack_overflow) {
+          // Chunk finished: accumulate in the next step built, release its
+          // slots now (the accumulation precedes any generate that reuses
+          // them, and steps already queued hold no paths of this chunk).
           AccItem a{};
           a.base = (uint32_t)j.region;
           a.np = j.desc.width * j.desc.height;
@@ -574,19 +668,16 @@ uint32_t Engine::step() {
           if (!last && !stack_overflow) queue_.push_front(j.id);
         }
       }
-      parity_ ^= 1u;
       if (stack_overflow) {
-        OL_HIP_CHECK(hipMemsetAsync(ctr_.as<uint32_t>() + C_STACK, 0, 4, ctx_->stream));
-        OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+        reset_overflow = true;
+        overflow_checked_from_ = seq_;
       }
     }
-    totals_.steps = steps_;
-    totals_.busy_ms += now_ms() - t_begin;
-    for (const auto& kv : jobs_)
-      if (kv.second->status == OLHIP_JOB_QUEUED || kv.second->status == OLHIP_JOB_RUNNING) ++pending;
-    cv_.notify_all();
   }
-  return pending;
+  if (reset_overflow) {
+    OL_HIP_CHECK(hipMemsetAsync(ctr_.as<uint32_t>() + C_STACK, 0, 4, ctx_->stream));
+    OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+  }
 }
 
 void Engine::run_until_idle() {
@@ -698,7 +789,7 @@ void Engine::start_worker() {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&] {
           if (stop_) return true;
-          if (!queue_.empty() || !pending_acc_.empty()) return true;
+          if (!queue_.empty() || !pending_acc_.empty() || inflight_count_ > 0) return true;
           for (uint32_t k = 0; k < kMaxJobs; ++k)
             if (slot_job_[k]) return true;
           return false;

@@ -264,9 +264,9 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
 template <int LSTACK, int BLOCK>
 OL_DEV TravStack<LSTACK> lane_stack(uint32_t* lds, uint32_t* spill, uint32_t spill_cap, uint32_t* overflow) {
   TravStack<LSTACK> st;
-  st.lds = lds + threadIdx.x;
+  st.lds = (lds_word*)lds + threadIdx.x;
   st.lds_stride = BLOCK;
-  st.spill = spill ? spill + (blockIdx.x * BLOCK + threadIdx.x) : nullptr;
+  st.spill = spill ? (global_word*)spill + (blockIdx.x * BLOCK + threadIdx.x) : nullptr;
   st.spill_stride = gridDim.x * BLOCK;
   st.spill_cap = spill ? spill_cap : 0u;
   st.overflow = overflow;

@@ -108,25 +108,51 @@ OL_DEV void sort4(uint32_t k[4]) {
 }
 OL_DEV float key_distance(uint32_t key) { return __uint_as_float(key & ~3u); }
 
+// Child word a key refers to (its low two bits are the child's position).
+OL_DEV uint32_t key_child(const uint32_t ch[4], uint32_t key) {
+  const bool odd = (key & 1u) != 0u;
+  const uint32_t lo = odd ? ch[1] : ch[0], hi = odd ? ch[3] : ch[2];
+  return (key & 2u) ? hi : lo;
+}
+
 // Culling distance for a current best hit at t: boxes entered beyond it are
 // skipped.  The 2^-16 relative slack absorbs the rounding of the
 // Moller-Trumbore distance itself, so a triangle that ties the best hit
 // (equal float t, lower index) is never culled.
 OL_DEV float cull_distance(float t) { return t * 1.0000152587890625f; }
 
+// Pointers with their address space spelled out, so that stack, node and
+// triangle accesses compile to LDS (ds_*) and global (global_*) instructions
+// rather than flat ones, which also wait on the other memory's counter.
+typedef __attribute__((address_space(3))) uint32_t lds_word;
+typedef __attribute__((address_space(1))) uint32_t global_word;
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4_t global_u32x4;
+typedef __attribute__((address_space(1))) const f32x4_t global_f32x4;
+
 OL_DEV void load_node(const WideNode* nodes, uint32_t index, uint4& w0, uint4& w1, uint4& w2, uint4& w3) {
-  const uint4* p = reinterpret_cast<const uint4*>(nodes + index);
-  w0 = p[0];
-  w1 = p[1];
-  w2 = p[2];
-  w3 = p[3];
+  global_u32x4* p = (global_u32x4*)(nodes + index);
+  const u32x4_t a = p[0], b = p[1], c = p[2], d = p[3];
+  w0 = make_uint4(a.x, a.y, a.z, a.w);
+  w1 = make_uint4(b.x, b.y, b.z, b.w);
+  w2 = make_uint4(c.x, c.y, c.z, c.w);
+  w3 = make_uint4(d.x, d.y, d.z, d.w);
+}
+
+OL_DEV void load_tri(const float4* tri, uint32_t slot, float4& a, float4& e1, float4& e2) {
+  global_f32x4* p = (global_f32x4*)(tri + 3u * slot);
+  const f32x4_t x = p[0], y = p[1], z = p[2];
+  a = make_float4(x.x, x.y, x.z, x.w);
+  e1 = make_float4(y.x, y.y, y.z, y.w);
+  e2 = make_float4(z.x, z.y, z.z, z.w);
 }
 
 template <int LSTACK>
 struct TravStack {
-  uint32_t* lds;       // this lane's LDS column
-  uint32_t lds_stride; // block size
-  uint32_t* spill;     // this lane's global column (may be null when spill_cap == 0)
+  lds_word* lds;        // this lane's LDS column
+  uint32_t lds_stride;  // block size
+  global_word* spill;   // this lane's global column (may be null when spill_cap == 0)
   uint32_t spill_stride;
   uint32_t spill_cap;
   uint32_t* overflow;
@@ -143,9 +169,28 @@ struct TravStack {
     }
     ++sp;
   }
+  // Pushes m (0..3) entries: c3 (m = 3), c2 (m >= 2), c1 (m >= 1), in that
+  // order, so c1 ends on top.  When three entries fit in LDS the lane
+  // writes three words unconditionally (those above the new top are
+  // scratch) instead of branching per entry.
+  OL_DEV void push_upto3(uint32_t c1, uint32_t c2, uint32_t c3, uint32_t m) {
+    if (sp + 3u <= (uint32_t)LSTACK) {
+      const uint32_t v0 = m >= 3u ? c3 : (m == 2u ? c2 : c1);
+      const uint32_t v1 = m >= 3u ? c2 : c1;
+      lds[sp * lds_stride] = v0;
+      lds[(sp + 1u) * lds_stride] = v1;
+      lds[(sp + 2u) * lds_stride] = c1;
+      sp += m;
+    } else {
+      if (m >= 3u) push(c3);
+      if (m >= 2u) push(c2);
+      if (m >= 1u) push(c1);
+    }
+  }
   OL_DEV uint32_t pop() {
     --sp;
-    return sp < (uint32_t)LSTACK ? lds[sp * lds_stride] : spill[(sp - LSTACK) * spill_stride];
+    if (sp < (uint32_t)LSTACK) return lds[sp * lds_stride];
+    return spill[(sp - LSTACK) * spill_stride];
   }
   OL_DEV bool empty() const { return sp == 0; }
 };
@@ -200,7 +245,8 @@ OL_DEV bool closest_step(const WideNode* __restrict__ nodes, const float4* __res
     nt += cnt;
     for (uint32_t j = 0; j < cnt; ++j) {
       const uint32_t slot = first + j;
-      const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+      float4 a, e1, e2;
+      load_tri(tri, slot, a, e1, e2);
       float t, u, v;
       if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
         const uint32_t orig = __float_as_uint(a.w);
@@ -227,6 +273,30 @@ OL_DEV bool closest_step(const WideNode* __restrict__ nodes, const float4* __res
   return false;
 }
 
+// Any hit ends at the first occluder found, so its children need no
+// distance order: they are taken in build order (the first hit child
+// continues), which has measured faster for shadow rays than near to far.
+// The keys become the hit children's positions, ascending, misses last.
+OL_DEV void build_order(uint32_t key[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) key[k] = key[k] == OL_NONE ? OL_NONE : (uint32_t)k;
+  sort4(key);
+}
+
+// With the hit children's keys first and in visiting order (sort4 or
+// build_order): continues with the first and pushes the others in reverse,
+// so the second is on top; with no child hit, takes the next stack entry.
+template <int LSTACK>
+OL_DEV void descend(TravStack<LSTACK>& st, const uint32_t ch[4], const uint32_t key[4], uint32_t& node) {
+  if (key[0] == OL_NONE) {
+    node = st.empty() ? OL_NONE : st.pop();
+    return;
+  }
+  const uint32_t m = (key[1] != OL_NONE ? 1u : 0u) + (key[2] != OL_NONE ? 1u : 0u) + (key[3] != OL_NONE ? 1u : 0u);
+  st.push_upto3(key_child(ch, key[1]), key_child(ch, key[2]), key_child(ch, key[3]), m);
+  node = key_child(ch, key[0]);
+}
+
 // Split-step form used by the refilling scheduler.  q.node holds the next
 // item to visit: an inner node index, a leaf word (OL_LEAF_BIT set), or
 // OL_NONE once the traversal has finished.  A wave chooses per step whether
@@ -249,15 +319,7 @@ OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, Tra
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
   sort4(key);
-  uint32_t next = OL_NONE;
-#pragma unroll
-  for (int k = 3; k >= 0; --k) {
-    if (key[k] == OL_NONE) continue;
-    if (next != OL_NONE) st.push(next);
-    next = ch[key[k] & 3u];
-  }
-  if (next != OL_NONE) q.node = next;
-  else closest_next(q, st);
+  descend(st, ch, key, q.node);
 }
 
 // Leaf step: test the leaf's triangles, then take the next stack entry.
@@ -267,7 +329,8 @@ OL_DEV void closest_leaf(const float4* __restrict__ tri, ClosestRay& q, TravStac
   nt += cnt;
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t slot = first + j;
-    const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+    float4 a, e1, e2;
+    load_tri(tri, slot, a, e1, e2);
     float t, u, v;
     if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
       const uint32_t orig = __float_as_uint(a.w);
@@ -307,17 +370,8 @@ OL_DEV void any_inner(const WideNode* __restrict__ nodes, AnyRay& q, TravStack<L
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
-  // Any hit ends at the first occluder found, so no distance order: children
-  // are taken in build order (the first one continues).
-  uint32_t next = OL_NONE;
-#pragma unroll
-  for (int k = 3; k >= 0; --k) {
-    if (key[k] == OL_NONE) continue;
-    if (next != OL_NONE) st.push(next);
-    next = ch[k];
-  }
-  if (next != OL_NONE) q.node = next;
-  else any_next(q, st);
+  build_order(key);
+  descend(st, ch, key, q.node);
 }
 
 template <int LSTACK>
@@ -325,7 +379,8 @@ OL_DEV void any_leaf(const float4* __restrict__ tri, AnyRay& q, TravStack<LSTACK
   const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t slot = first + j;
-    const float4 a = tri[3 * slot], e1 = tri[3 * slot + 1], e2 = tri[3 * slot + 2];
+    float4 a, e1, e2;
+    load_tri(tri, slot, a, e1, e2);
     float t, u, v;
     ++nt;
     if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin && t <= q.tmax) {
@@ -439,15 +494,7 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st,
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
   sort4(key);
-  uint32_t next = OL_NONE;
-#pragma unroll
-  for (int k = 3; k >= 0; --k) {
-    if (key[k] == OL_NONE) continue;
-    if (next != OL_NONE) st.push(next);
-    next = ch[key[k] & 3u];
-  }
-  if (next != OL_NONE) q.node = next;
-  else closest2_next(q, st);
+  descend(st, ch, key, q.node);
 }
 
 template <int LSTACK>
@@ -456,7 +503,8 @@ OL_DEV void closest2_leaf(Closest2& q, TravStack<LSTACK>& st, uint32_t& nt) {
   nt += cnt;
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t slot = first + j;
-    const float4 a = q.btri[3 * slot], e1 = q.btri[3 * slot + 1], e2 = q.btri[3 * slot + 2];
+    float4 a, e1, e2;
+    load_tri(q.btri, slot, a, e1, e2);
     float t, u, v;
     if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin) {
       const uint32_t orig = q.offset + __float_as_uint(a.w);
@@ -527,15 +575,8 @@ OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
-  uint32_t next = OL_NONE;
-#pragma unroll
-  for (int k = 3; k >= 0; --k) {
-    if (key[k] == OL_NONE) continue;
-    if (next != OL_NONE) st.push(next);
-    next = ch[k];
-  }
-  if (next != OL_NONE) q.node = next;
-  else any2_next(q, st);
+  build_order(key);
+  descend(st, ch, key, q.node);
 }
 
 template <int LSTACK>
@@ -543,7 +584,8 @@ OL_DEV void any2_leaf(Any2& q, TravStack<LSTACK>& st, uint32_t& nt) {
   const uint32_t first = leaf_first(q.node), cnt = leaf_count(q.node);
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t slot = first + j;
-    const float4 a = q.btri[3 * slot], e1 = q.btri[3 * slot + 1], e2 = q.btri[3 * slot + 2];
+    float4 a, e1, e2;
+    load_tri(q.btri, slot, a, e1, e2);
     float t, u, v;
     ++nt;
     if (ol_tri_test(q.r.o, q.r.d, a, e1, e2, t, u, v) && t >= q.tmin && t <= q.tmax) {

@@ -768,6 +768,7 @@ class HipReferenceRenderer(_ResidentMixin, RendererImplementation):
             n_hit = N.ctypes.c_uint32(0)
             n_active = N.ctypes.c_uint32(0)
             primary = None
+            rays = 0
             for sample in range(samples):
                 jx = rng.random((height, width), dtype=np.float32)
                 jy = rng.random((height, width), dtype=np.float32)
@@ -776,6 +777,7 @@ class HipReferenceRenderer(_ResidentMixin, RendererImplementation):
                     N.check(lib.olhip_e0_trace(context.handle, N.ctypes.byref(n_hit), N.ctypes.byref(n_active)))
                     if n_active.value == 0:
                         break
+                    rays += n_active.value
                     if capture_primary_hits and sample == 0 and bounce == 0:
                         primary = np.empty(n_active.value, np.int32)
                         N.check(lib.olhip_e0_last_hits(context.handle, N.ptr(primary, N.ctypes.c_int32),
@@ -790,7 +792,8 @@ class HipReferenceRenderer(_ResidentMixin, RendererImplementation):
             out = np.empty((height, width, 4), np.float32)
             N.check(lib.olhip_e0_end(context.handle, samples, N.ptr(out)))
             self.last_primary_hits = primary
-            self.last_timings = {"total_ms": (time.perf_counter() - started) * 1000.0, **device_scene.timings}
+            self.last_timings = {"total_ms": (time.perf_counter() - started) * 1000.0, "rays": rays,
+                                 **device_scene.timings}
             return out
 
     render_wavefront = render_frame
