@@ -108,11 +108,22 @@ OL_DEV void sort4(uint32_t k[4]) {
 }
 OL_DEV float key_distance(uint32_t key) { return __uint_as_float(key & ~3u); }
 
-// Child word a key refers to (its low two bits are the child's position).
-OL_DEV uint32_t key_child(const uint32_t ch[4], uint32_t key) {
-  const bool odd = (key & 1u) != 0u;
-  const uint32_t lo = odd ? ch[1] : ch[0], hi = odd ? ch[3] : ch[2];
-  return (key & 2u) ? hi : lo;
+// Ascending sort of four keys carrying their child words along (five
+// compare-exchanges of pairs), so no child needs selecting by key afterwards.
+OL_DEV void pair_swap(uint32_t& ka, uint32_t& kb, uint32_t& ca, uint32_t& cb) {
+  const bool s = kb < ka;
+  const uint32_t k0 = s ? kb : ka, k1 = s ? ka : kb, c0 = s ? cb : ca, c1 = s ? ca : cb;
+  ka = k0;
+  kb = k1;
+  ca = c0;
+  cb = c1;
+}
+OL_DEV void sort4_pairs(uint32_t k[4], uint32_t c[4]) {
+  pair_swap(k[0], k[1], c[0], c[1]);
+  pair_swap(k[2], k[3], c[2], c[3]);
+  pair_swap(k[0], k[2], c[0], c[2]);
+  pair_swap(k[1], k[3], c[1], c[3]);
+  pair_swap(k[1], k[2], c[1], c[2]);
 }
 
 // Culling distance for a current best hit at t: boxes entered beyond it are
@@ -277,15 +288,16 @@ OL_DEV bool closest_step(const WideNode* __restrict__ nodes, const float4* __res
 // distance order: they are taken in build order (the first hit child
 // continues), which has measured faster for shadow rays than near to far.
 // The keys become the hit children's positions, ascending, misses last.
-OL_DEV void build_order(uint32_t key[4]) {
+OL_DEV void build_order(uint32_t key[4], uint32_t ch[4]) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) key[k] = key[k] == OL_NONE ? OL_NONE : (uint32_t)k;
-  sort4(key);
+  sort4_pairs(key, ch);
 }
 
-// With the hit children's keys first and in visiting order (sort4 or
-// build_order): continues with the first and pushes the others in reverse,
-// so the second is on top; with no child hit, takes the next stack entry.
+// With the hit children first and in visiting order, keys and child words
+// sorted together (sort4_pairs or build_order): continues with the first
+// and pushes the others in reverse, so the second is on top; with no child
+// hit, takes the next stack entry.
 template <int LSTACK>
 OL_DEV void descend(TravStack<LSTACK>& st, const uint32_t ch[4], const uint32_t key[4], uint32_t& node) {
   if (key[0] == OL_NONE) {
@@ -293,8 +305,8 @@ OL_DEV void descend(TravStack<LSTACK>& st, const uint32_t ch[4], const uint32_t 
     return;
   }
   const uint32_t m = (key[1] != OL_NONE ? 1u : 0u) + (key[2] != OL_NONE ? 1u : 0u) + (key[3] != OL_NONE ? 1u : 0u);
-  st.push_upto3(key_child(ch, key[1]), key_child(ch, key[2]), key_child(ch, key[3]), m);
-  node = key_child(ch, key[0]);
+  st.push_upto3(ch[1], ch[2], ch[3], m);
+  node = ch[0];
 }
 
 // Split-step form used by the refilling scheduler.  q.node holds the next
@@ -318,7 +330,7 @@ OL_DEV void closest_inner(const WideNode* __restrict__ nodes, ClosestRay& q, Tra
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
-  sort4(key);
+  sort4_pairs(key, ch);
   descend(st, ch, key, q.node);
 }
 
@@ -370,7 +382,7 @@ OL_DEV void any_inner(const WideNode* __restrict__ nodes, AnyRay& q, TravStack<L
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
-  build_order(key);
+  build_order(key, ch);
   descend(st, ch, key, q.node);
 }
 
@@ -493,7 +505,7 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st,
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
-  sort4(key);
+  sort4_pairs(key, ch);
   descend(st, ch, key, q.node);
 }
 
@@ -575,7 +587,7 @@ OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_
   uint32_t ch[4], key[4];
   node_children(w2, w3, ch);
   ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
-  build_order(key);
+  build_order(key, ch);
   descend(st, ch, key, q.node);
 }
 
