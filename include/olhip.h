@@ -106,6 +106,9 @@ typedef struct olhip_instanced_desc {
   uint32_t builder;                    /* bottom-level hierarchy: OLHIP_BUILDER_LBVH or
                                           OLHIP_BUILDER_SAH; the top level is always
                                           host binned SAH */
+  const float* world_spheres;          /* [I][4] centre xyz, radius of a sphere containing
+                                          the instance, or NULL; rays whose line misses
+                                          it skip the instance */
 } olhip_instanced_desc;
 
 int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* desc, olhip_scene** out);

@@ -54,6 +54,7 @@ class InstancedDesc(ctypes.Structure):
         ("analytic_light_count", c_u32), ("analytic_lights", P_f32),
         ("area_light_count", c_u32), ("area_lights", P_f32),
         ("area_light_weight", c_f32), ("leaf_size", c_u32), ("builder", c_u32),
+        ("world_spheres", P_f32),
     ]
 
 

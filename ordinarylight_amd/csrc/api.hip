@@ -322,6 +322,13 @@ void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t
     x.id = d->instance_id[k];
     const float* b = d->world_bounds + 6ull * k;
     for (int a = 0; a < 6; ++a) require(std::isfinite(b[a]), "instance bounds must be finite");
+    x.sphere = make_float4(0.0f, 0.0f, 0.0f, INFINITY);
+    if (d->world_spheres) {
+      const float* sp = d->world_spheres + 4ull * k;
+      require(std::isfinite(sp[0]) && std::isfinite(sp[1]) && std::isfinite(sp[2]) && sp[3] >= 0.0f,
+              "instance spheres need a finite centre and a radius >= 0");
+      x.sphere = make_float4(sp[0], sp[1], sp[2], sp[3]);
+    }
     tpos[2 * k] = make_float4(b[0], b[1], b[2], 0.0f);
     tpos[2 * k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
     tvidx[k] = make_uint4(2 * k, 2 * k + 1, 2 * k, 0u);

@@ -125,6 +125,7 @@ struct InstanceDev {
   uint32_t material;    // material record index
   uint32_t tri_offset;  // first triangle of this instance in packed (reference) triangle order
   uint32_t id;          // scene instance id
+  float4 sphere;        // world bounding sphere: centre xyz, radius (+inf: none)
 };
 
 // Bottom-level BVH of one mesh resource, object space.

@@ -439,6 +439,24 @@ OL_DEV f3 xform_vector(const float4* rows, f3 v) {
              (rows[2].x * v.x + rows[2].y * v.y) + rows[2].z * v.z);
 }
 
+OL_DEV float max_abs3(f3 a) { return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z)); }
+
+// Whether the ray's line passes within an instance's world bounding sphere
+// (centre xyz, radius w); an instance whose sphere it misses cannot be hit.
+// v is the offset of the centre from the computed closest point of the line:
+// its rounding (a few ulps of the operands) and the closest point's shift
+// along the line (which only lengthens v, by ~1e-7 of the magnitudes) stay
+// far inside the 1e-5 allowance, so a reachable instance is never skipped.
+OL_DEV bool sphere_reachable(const RayPre& r, float4 s) {
+  const f3 c = mk3(s.x, s.y, s.z);
+  const f3 oc = c - r.o;
+  const float tc = dot3(oc, r.d) / dot3(r.d, r.d);
+  const f3 v = oc - r.d * tc;
+  const float scale = fmaxf(fmaxf(max_abs3(r.o), max_abs3(c)), fabsf(tc) * max_abs3(r.d));
+  const float reach = s.w + 1e-5f * scale;
+  return dot3(v, v) <= reach * reach;
+}
+
 struct Closest2 {
   RayPre w;                // world ray
   RayPre r;                // current ray: world at the top level, object space inside an instance
@@ -485,7 +503,7 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st,
     const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
     const InstanceDev& I = sc.inst[k];
     const BlasDev& B = sc.blas[I.blas];
-    if (B.node_count == 0) {
+    if (B.node_count == 0 || !sphere_reachable(q.w, I.sphere)) {
       closest2_next(q, st);
       return;
     }
@@ -568,7 +586,7 @@ OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_
     const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
     const InstanceDev& I = sc.inst[k];
     const BlasDev& B = sc.blas[I.blas];
-    if (B.node_count == 0) {
+    if (B.node_count == 0 || !sphere_reachable(q.w, I.sphere)) {
       any2_next(q, st);
       return;
     }

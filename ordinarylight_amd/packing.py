@@ -130,6 +130,7 @@ class PackedInstancedScene:
     normal_matrix: np.ndarray       # float32 [I, 3, 3]
     determinant: np.ndarray         # float32 [I]
     world_bounds: np.ndarray        # float32 [I, 6]
+    world_spheres: np.ndarray       # float32 [I, 4] centre xyz, radius (contains the instance)
     materials: np.ndarray
     analytic_lights: np.ndarray
     analytic_light_count: int
@@ -224,6 +225,14 @@ def pack_instanced_scene(scene):
     bmax = np.minimum(world.max(1), sphere_c + sphere_h)
     wmin = np.nextafter(bmin.astype(np.float32), np.float32(-np.inf))
     wmax = np.nextafter(bmax.astype(np.float32), np.float32(np.inf))
+    # World bounding spheres: the transformed centre, rounded to float32, and
+    # the radius scaled by the largest singular value of the linear part, grown
+    # by the centre's rounding and rounded up.
+    centre32 = sphere_c.astype(np.float32)
+    grow = np.linalg.norm(centre32.astype(np.float64) - sphere_c, axis=1)
+    reach = radius[ri] * np.linalg.norm(A, ord=2, axis=(1, 2)) * (1.0 + 1e-6) + grow
+    reach32 = np.nextafter(reach.astype(np.float32), np.float32(np.inf))
+    spheres = np.concatenate((centre32, reach32[:, None]), 1)
     if not materials:
         raise ValueError("no materials")
     analytic_count = int(getattr(scene, "analytic_light_count", 0))
@@ -237,6 +246,7 @@ def pack_instanced_scene(scene):
         normal_matrix=np.ascontiguousarray(normal_matrix, np.float32),
         determinant=np.ascontiguousarray(det, np.float32),
         world_bounds=np.ascontiguousarray(np.concatenate((wmin, wmax), 1), np.float32),
+        world_spheres=np.ascontiguousarray(spheres, np.float32),
         materials=np.ascontiguousarray(np.stack(materials), np.float32),
         analytic_lights=np.ascontiguousarray(scene.analytic_light_data(), np.float32),
         analytic_light_count=analytic_count,

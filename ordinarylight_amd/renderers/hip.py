@@ -220,6 +220,7 @@ def _instanced_desc(p, leaf_size, builder="lbvh"):
     d.normal_matrix = N.ptr(p.normal_matrix)
     d.determinant = N.ptr(p.determinant)
     d.world_bounds = N.ptr(p.world_bounds)
+    d.world_spheres = N.ptr(p.world_spheres)
     d.material_count = len(p.materials)
     d.materials = N.ptr(p.materials)
     d.analytic_light_count = p.analytic_light_count

@@ -92,3 +92,22 @@ def test_instanced_packing_bounds_contain_instances_and_flatten_in_order():
         assert np.all(hi <= np.nextafter(world.max(0).astype(np.float32), np.inf))
         assert p.instance_triangle_offset[k] == offset
         offset += len(mesh.indices)
+
+
+def test_instanced_packing_spheres_contain_instances():
+    """Each instance's world bounding sphere (float32 centre and radius)
+    contains every transformed vertex and is tight: its radius exceeds the
+    largest vertex distance from the centre by a relative 1e-4 at most."""
+    from ordinarylight_amd import packing
+    scene, _ = workloads.instanced_spheres(5, meshes=3, subdivisions=2, instances=200, box=12.0, emitters=10)
+    p = packing.pack_instanced_scene(scene)
+    meshes = [m for m in scene.meshes if len(m.indices)]
+    assert p.world_spheres.shape == (len(meshes), 4) and p.world_spheres.dtype == np.float32
+    for k, mesh in enumerate(meshes):
+        m = np.asarray(mesh.transform.matrix, np.float64)
+        w = mesh.vertices.astype(np.float64) @ m[:3, :3].T + m[:3, 3]
+        c = p.world_spheres[k, :3].astype(np.float64)
+        r = float(p.world_spheres[k, 3])
+        far = np.sqrt(((w - c) ** 2).sum(1)).max()
+        assert far <= r
+        assert r <= far * (1.0 + 1e-4) + 1e-6
