@@ -195,12 +195,13 @@ typedef struct olhip_stats {
   uint64_t shadow_rays;
   uint64_t queue_overflow;
   double total_ms;         /* device time of the render (events) */
-  double trace_ms;         /* closest-hit kernels (when profiling) */
+  double trace_ms;         /* traversal launches (when profiling): each traces a step's
+                              closest hits and the previous step's shadow rays */
   double shade_ms;
-  double shadow_ms;
+  double shadow_ms;        /* 0: shadow rays are traced within the traversal launches */
   uint32_t trace_launches;
   uint32_t shade_launches;
-  uint32_t shadow_launches;
+  uint32_t shadow_launches; /* 0, as shadow_ms */
   uint32_t chunks;
   /* Work counters (only with olhip_set_work_counters(ctx, 1)). */
   uint64_t closest_nodes;  /* 64-byte nodes visited by closest-hit rays */

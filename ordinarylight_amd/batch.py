@@ -227,7 +227,7 @@ def run_batch(engine: BatchEngine, jobs, copy_images=False):
     return wall, [engine.result(i, copy_image=copy_images) for i in ids]
 
 
-def autotune(context: DeviceContext, jobs, *, slots=(1 << 22, 1 << 23, 1 << 24, 1 << 25),
+def autotune(context: DeviceContext, jobs, *, slots=(1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27),
              shapes=((16, 128), (8, 128), (16, 256), (32, 128)), repeats=1, measure=None):
     """Choose path-slot capacity and (LDS stack entries, block size) for a job mix.
 

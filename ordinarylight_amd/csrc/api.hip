@@ -682,7 +682,7 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
     Engine& eng = *ctx->frame_engine;
     eng.set_profiling(ctx->profiling != 0);
     eng.set_work_counters(ctx->counting != 0);
-    eng.trace_ms = eng.shade_ms = eng.shadow_ms = 0.0;
+    eng.trace_ms = eng.shade_ms = 0.0;
     eng.trace_launches = 0;
     olhip_job_desc d{};
     d.scene = sc;
@@ -728,10 +728,10 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
       o.total_ms = elapsed(e_start, e_end);
       o.trace_ms = eng.trace_ms;
       o.shade_ms = eng.shade_ms;
-      o.shadow_ms = eng.shadow_ms;
+      o.shadow_ms = 0.0;
       o.trace_launches = eng.trace_launches;
       o.shade_launches = eng.trace_launches;
-      o.shadow_launches = eng.trace_launches;
+      o.shadow_launches = 0;
       o.chunks = js.chunks;
       if (ctx->counting) {
         unsigned long long w[4];

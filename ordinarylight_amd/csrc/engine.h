@@ -98,11 +98,12 @@ class Engine {
 
   olhip_ctx* ctx() const { return ctx_; }
   const olhip_engine_config& config() const { return cfg_; }
-  // Per-step kernel timing (events around the traversal and shade launches).
+  // Per-step kernel timing (events around the traversal launch, closest hits
+  // and the previous step's shadow rays, and around shading).
   void set_profiling(bool on) { profiling_ = on; }
   void set_work_counters(bool on);
   void read_work(unsigned long long out[4]);
-  double trace_ms = 0, shade_ms = 0, shadow_ms = 0;
+  double trace_ms = 0, shade_ms = 0;
   uint32_t trace_launches = 0;
 
  private:
@@ -193,7 +194,7 @@ class Engine {
   StepList* list_host_[2] = {nullptr, nullptr};
   uint32_t* report_host_[2] = {nullptr, nullptr};
   hipEvent_t done_[2] = {nullptr, nullptr};
-  hipEvent_t prof_ev_[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+  hipEvent_t prof_ev_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double t0_ms_ = 0;
   // Engine totals.
   olhip_engine_stats totals_{};

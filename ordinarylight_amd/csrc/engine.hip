@@ -404,19 +404,16 @@ void Engine::launch_step(uint32_t key, bool timed, int buf) {
   if (!timed) {
     wave_trace(launch_, wave_, in, kinds, s);
     wave_shade(launch_, wave_, in, kinds, s);
-    wave_shadow(launch_, wave_, kinds, s);
   } else {
     // Read when the step is retired (no synchronisation inside the step).
     hipEvent_t* ev = prof_ev_[buf];
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 3; ++k)
       if (!ev[k]) OL_HIP_CHECK(hipEventCreate(&ev[k]));
     OL_HIP_CHECK(hipEventRecord(ev[0], s));
     wave_trace(launch_, wave_, in, kinds, s);
     OL_HIP_CHECK(hipEventRecord(ev[1], s));
     wave_shade(launch_, wave_, in, kinds, s);
     OL_HIP_CHECK(hipEventRecord(ev[2], s));
-    wave_shadow(launch_, wave_, kinds, s);
-    OL_HIP_CHECK(hipEventRecord(ev[3], s));
   }
   wave_epilogue(wave_, in, s);
 }
@@ -593,13 +590,11 @@ void Engine::retire_front() {
   inflight_.pop_front();
   OL_HIP_CHECK(hipEventSynchronize(done_[f.buf]));
   if (f.timed) {
-    float a = 0, b = 0, c = 0;
+    float a = 0, b = 0;
     OL_HIP_CHECK(hipEventElapsedTime(&a, prof_ev_[f.buf][0], prof_ev_[f.buf][1]));
     OL_HIP_CHECK(hipEventElapsedTime(&b, prof_ev_[f.buf][1], prof_ev_[f.buf][2]));
-    OL_HIP_CHECK(hipEventElapsedTime(&c, prof_ev_[f.buf][2], prof_ev_[f.buf][3]));
     trace_ms += a;
     shade_ms += b;
-    shadow_ms += c;
     ++trace_launches;
   }
   bool reset_overflow = false;

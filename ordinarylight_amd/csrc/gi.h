@@ -49,12 +49,14 @@ constexpr uint32_t kRuns = kMaxJobs * kJobStripes;
 constexpr uint32_t kChunk = 64;
 constexpr uint32_t kWorkStripes = 32;  // work cursors of the refilling traversal kernels
 
-// Counter block (uint32 words).
-constexpr uint32_t C_Q0 = 0, C_Q1 = kRuns, C_G = 2 * kRuns, C_E = 3 * kRuns;
+// Counter block (uint32 words).  Ray queues (C_Q0/1) alternate by step
+// parity; so do the shadow groups and entries (C_G0/1, C_E0/1), because a
+// step's shadow rays are traced in the next step, beside its closest hits.
+constexpr uint32_t C_Q0 = 0, C_Q1 = kRuns, C_G0 = 2 * kRuns, C_G1 = 3 * kRuns, C_E0 = 4 * kRuns, C_E1 = 5 * kRuns;
 // Chunk prefixes are kept separately for jobs of flat scenes (C_RP, C_GP)
 // and of two-level scenes (C_RP2, C_GP2), so each traversal kernel variant
 // sees one kind only.
-constexpr uint32_t C_RP = 4 * kRuns;             // kRuns + 1: chunk prefix of the input ray queue (flat)
+constexpr uint32_t C_RP = 6 * kRuns;             // kRuns + 1: chunk prefix of the input ray queue (flat)
 constexpr uint32_t C_GP = C_RP + kRuns + 1;      // kRuns + 1: chunk prefix of the shadow groups (flat)
 constexpr uint32_t C_RP2 = C_GP + kRuns + 1;     // the same for two-level scenes
 constexpr uint32_t C_GP2 = C_RP2 + kRuns + 1;
@@ -66,6 +68,8 @@ constexpr uint32_t C_OVERFLOW = C_WS2 + kWorkStripes, C_STACK = C_OVERFLOW + 1;
 constexpr uint32_t C_REPORT = C_OVERFLOW + 8;    // kMaxJobs x 4: rays traced, shadow rays, live paths, shaded hits
 constexpr uint32_t C_WORDS = C_REPORT + 4 * kMaxJobs;
 OL_HD constexpr uint32_t c_queue(uint32_t q) { return q ? C_Q1 : C_Q0; }
+OL_HD constexpr uint32_t c_groups(uint32_t p) { return p ? C_G1 : C_G0; }
+OL_HD constexpr uint32_t c_entries(uint32_t p) { return p ? C_E1 : C_E0; }
 
 // Device job record (read wave-uniformly by the kernels).
 struct JobDev {
@@ -154,10 +158,10 @@ uint64_t trace_spill_words(const GiLaunch& l);
 int stack_depth_for(uint32_t bvh_depth);
 
 // One step of the batched wavefront: accumulate finished chunks, generate
-// newly admitted jobs, then trace / shade / shadow every resident path once,
-// and report per-job counts.  `in` is the parity of the input ray queue.
-// With `marks`, events are recorded around the closest-hit, shade and shadow
-// launches (profiling).
+// newly admitted jobs, then trace every resident path once together with
+// the shadow rays the previous step's shading emitted, shade, and report
+// per-job counts.  `in` is the parity of the input ray queue and of the
+// shadow groups this step's shading emits.
 // `kinds` says which resident work a step holds (STEP_* bits); the launches
 // for absent kinds are left out (their chunk totals stay zero).
 constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u;
@@ -165,7 +169,6 @@ void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
 void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
-void wave_shadow(const GiLaunch& l, const Wave& w, uint32_t kinds, hipStream_t st);
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
 void wave_clear(const Wave& w, hipStream_t st);
 

@@ -37,13 +37,21 @@ using namespace olshade;
 constexpr uint32_t kRefillMin = 16;
 
 // ---- chunk plan ----------------------------------------------------------------------
-// Exclusive prefix of ceil(count / kChunk) over the runs of jobs whose scene
-// is flat (kind 0) or two-level (kind 1); out[kRuns] = total.
-__global__ void __launch_bounds__(1024) k_plan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
-                                               const JobDev* __restrict__ jobs, uint32_t kind) {
+// Up to four plans in one launch, one block each: block b computes the
+// exclusive prefix of ceil(count / kChunk) over the runs of counter block
+// counts[b] whose jobs' scenes are flat (kind 0) or two-level (kind 1), into
+// out[b] (out[b][kRuns] = total).  Offsets are counter-block words.
+struct PlanSet {
+  uint32_t counts[4], out[4], kind[4];
+};
+
+__global__ void __launch_bounds__(1024) k_plan(uint32_t* __restrict__ ctr, const JobDev* __restrict__ jobs, PlanSet ps) {
   static_assert(kRuns == 2 * 1024 && kJobStripes % 2 == 0, "two runs of one job per thread");
   __shared__ uint32_t wave_tot[16];
   const uint32_t t = threadIdx.x;
+  const uint32_t* __restrict__ counts = ctr + ps.counts[blockIdx.x];
+  uint32_t* __restrict__ out = ctr + ps.out[blockIdx.x];
+  const uint32_t kind = ps.kind[blockIdx.x];
   const bool mine = (jobs[(2 * t) / kJobStripes].P.sc.instanced != 0u) == (kind != 0u);
   const uint32_t a = mine ? (counts[2 * t] + kChunk - 1u) / kChunk : 0u;
   const uint32_t b = mine ? (counts[2 * t + 1] + kChunk - 1u) / kChunk : 0u;
@@ -305,10 +313,8 @@ OL_DEV void pool_refill(Pool& p, const Wave& W, const uint32_t* prefix, const ui
 }
 
 // Closest hit for the input ray queue: refilled lanes, split inner / leaf steps.
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_closest(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+template <int LSTACK>
+OL_DEV void closest_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* prefix = W.ctr + C_RP;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
@@ -397,14 +403,13 @@ OL_DEV void shadow_resolve(const Wave& W, uint32_t g, f3 direct, f3 area_direct,
   W.L[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, L.w);
 }
 
-// Shadow groups: a lane owns one group and traces its entries in order
-// (any hit), summing the visible contributions as the inline form does.
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_shadow(Wave W, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+// Shadow groups the previous step's shading emitted (parity in ^ 1): a lane
+// owns one group and traces its entries in order (any hit), summing the
+// visible contributions as the inline form does.
+template <int LSTACK>
+OL_DEV void shadow_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* prefix = W.ctr + C_GP;
-  const uint32_t* counts = W.ctr + C_G;
+  const uint32_t* counts = W.ctr + c_groups(in ^ 1u);
   const uint32_t chunks = prefix[kRuns];
   if (chunks == 0) return;
   const uint32_t ws = blockIdx.x % kWorkStripes;
@@ -511,11 +516,9 @@ OL_DEV void pool2_refill(Pool2& p, const Wave& W, const uint32_t* prefix, const 
 }
 
 // Closest hit in two-level scenes (top-level BVH over instances): the
-// scheduling of k_closest with the instance-entering inner step.
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_closest2(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+// scheduling of closest_pass with the instance-entering inner step.
+template <int LSTACK>
+OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* prefix = W.ctr + C_RP2;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
@@ -582,12 +585,10 @@ __global__ void __launch_bounds__(BLOCK) k_closest2(Wave W, uint32_t in, uint32_
 }
 
 // Shadow groups in two-level scenes.
-template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_shadow2(Wave W, uint32_t* spill, uint32_t spill_cap) {
-  __shared__ uint32_t lds[LSTACK * BLOCK];
-  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+template <int LSTACK>
+OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* prefix = W.ctr + C_GP2;
-  const uint32_t* counts = W.ctr + C_G;
+  const uint32_t* counts = W.ctr + c_groups(in ^ 1u);
   const uint32_t chunks = prefix[kRuns];
   if (chunks == 0) return;
   const uint32_t ws = blockIdx.x % kWorkStripes;
@@ -666,6 +667,33 @@ __global__ void __launch_bounds__(BLOCK) k_shadow2(Wave W, uint32_t* spill, uint
     atomicAdd(&W.work[2], (unsigned long long)nv);
     atomicAdd(&W.work[3], (unsigned long long)nt);
   }
+}
+
+// A step's traversal: closest hits of the input ray queue, then the shadow
+// rays of the previous step's shading.  Waves move on to shadow rays as the
+// closest-hit chunks of their work stripe run out, so each pass's last few
+// long rays overlap the other pass's work instead of ending a launch alone.
+// Neither pass writes what the other reads: closest hit writes hit records,
+// shadow rays add to path radiance, which only the shading that follows
+// reads.
+// Six waves per SIMD (80 VGPRs): the closest-hit pass alone needs no more,
+// and the shadow pass was measured no slower held to it.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6)))
+k_trace(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  closest_pass<LSTACK>(W, in, st);
+  shadow_pass<LSTACK>(W, in, st);
+}
+
+// The same for jobs of two-level scenes.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace2(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  closest2_pass<LSTACK>(W, in, st);
+  shadow2_pass<LSTACK>(W, in, st);
 }
 
 template <int LSTACK, int BLOCK>
@@ -1220,8 +1248,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     // Shadow group + entries.
     const bool has_group = nee && nee_count > 0u;
-    const uint32_t gl = wave_reserve(&W.ctr[C_G + run], has_group);
-    const uint32_t el = wave_reserve_n(&W.ctr[C_E + run], has_group ? nee_count : 0u);
+    const uint32_t gl = wave_reserve(&W.ctr[c_groups(in) + run], has_group);
+    const uint32_t el = wave_reserve_n(&W.ctr[c_entries(in) + run], has_group ? nee_count : 0u);
     if (has_group) {
       if (gl < J.seg && el + nee_count <= J.eseg) {
         const uint32_t gslot = J.base + so * J.seg + gl, eslot = J.ebase + so * J.eseg + el;
@@ -1262,9 +1290,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 }
 
 // ---- step epilogue -----------------------------------------------------------------------------
-// Per-job report (rays traced from the input queue, shadow rays, live paths
-// in the output queue, shaded hits = groups) and reset of every consumed
-// counter, so the input queue can serve as the next step's output.
+// Per-job report: rays traced from the input queue, shadow rays traced (the
+// previous step's entries), work left (paths in the output queue plus the
+// shadow groups this step's shading emitted, which the next step traces)
+// and shaded hits (those groups); then a reset of every consumed counter,
+// so the input queue and the traced groups can serve the next step's output.
 __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
   __shared__ uint32_t rep[4 * kMaxJobs];
   const uint32_t t = threadIdx.x;
@@ -1272,15 +1302,15 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
   __syncthreads();
   for (uint32_t r = t; r < kRuns; r += blockDim.x) {
     const uint32_t job = r / kJobStripes;
-    const uint32_t a = W.ctr[c_queue(in) + r], e = W.ctr[C_E + r], l = W.ctr[c_queue(in ^ 1u) + r],
-                   g = W.ctr[C_G + r];
+    const uint32_t a = W.ctr[c_queue(in) + r], e = W.ctr[c_entries(in ^ 1u) + r],
+                   l = W.ctr[c_queue(in ^ 1u) + r], g = W.ctr[c_groups(in) + r];
     if (a) atomicAdd(&rep[4 * job + 0], a);
     if (e) atomicAdd(&rep[4 * job + 1], e);
-    if (l) atomicAdd(&rep[4 * job + 2], l);
+    if (l + g) atomicAdd(&rep[4 * job + 2], l + g);
     if (g) atomicAdd(&rep[4 * job + 3], g);
     W.ctr[c_queue(in) + r] = 0u;
-    W.ctr[C_G + r] = 0u;
-    W.ctr[C_E + r] = 0u;
+    W.ctr[c_groups(in ^ 1u) + r] = 0u;
+    W.ctr[c_entries(in ^ 1u) + r] = 0u;
   }
   if (t < 4 * kWorkStripes) W.ctr[C_W + t] = 0u;
   // Chunk totals of the four plans: a plan left out of the next step (no
@@ -1320,45 +1350,26 @@ void check_spill(const GiLaunch& l, uint64_t threads) {
 
 template <int LSTACK, int BLOCK>
 uint64_t spill_words_for(const GiLaunch& l) {
-  int occ = std::max(std::max(std::max(resident_blocks(k_closest<LSTACK, BLOCK>, BLOCK),
-                                       resident_blocks(k_shadow<LSTACK, BLOCK>, BLOCK)),
-                              std::max(resident_blocks(k_closest2<LSTACK, BLOCK>, BLOCK),
-                                       resident_blocks(k_shadow2<LSTACK, BLOCK>, BLOCK))),
+  int occ = std::max(std::max(resident_blocks(k_trace<LSTACK, BLOCK>, BLOCK), resident_blocks(k_trace2<LSTACK, BLOCK>, BLOCK)),
                      std::max(resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK),
                               resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK)));
   return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
 
 template <int LSTACK, int BLOCK>
-void launch_closest(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  static int occ = resident_blocks(k_closest<LSTACK, BLOCK>, BLOCK);
+void launch_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  static int occ = resident_blocks(k_trace<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
   check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_closest<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
+  hipLaunchKernelGGL((k_trace<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
-void launch_closest2(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  static int occ = resident_blocks(k_closest2<LSTACK, BLOCK>, BLOCK);
+void launch_trace2(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  static int occ = resident_blocks(k_trace2<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
   check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_closest2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
-}
-
-template <int LSTACK, int BLOCK>
-void launch_shadow2(const GiLaunch& l, const Wave& w, hipStream_t st) {
-  static int occ = resident_blocks(k_shadow2<LSTACK, BLOCK>, BLOCK);
-  dim3 grid((unsigned)(l.cu_count * occ));
-  check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_shadow2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, l.spill, spill_cap_of(l));
-}
-
-template <int LSTACK, int BLOCK>
-void launch_shadow(const GiLaunch& l, const Wave& w, hipStream_t st) {
-  static int occ = resident_blocks(k_shadow<LSTACK, BLOCK>, BLOCK);
-  dim3 grid((unsigned)(l.cu_count * occ));
-  check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_shadow<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, l.spill, spill_cap_of(l));
+  hipLaunchKernelGGL((k_trace2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
@@ -1415,14 +1426,28 @@ void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st) {
 }
 
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
+  // One launch plans the input ray queue and the previous step's shadow
+  // groups for each scene kind present.
+  PlanSet ps{};
+  uint32_t n = 0;
+  auto add = [&](uint32_t counts, uint32_t out, uint32_t kind) {
+    ps.counts[n] = counts;
+    ps.out[n] = out;
+    ps.kind[n] = kind;
+    ++n;
+  };
   if (kinds & STEP_FLAT) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP, w.jobs, 0u);
-    [&] { OL_STACK_DISPATCH(launch_closest, l, w, in, st); }();
+    add(c_queue(in), C_RP, 0u);
+    add(c_groups(in ^ 1u), C_GP, 0u);
   }
   if (kinds & STEP_INSTANCED) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + c_queue(in), w.ctr + C_RP2, w.jobs, 1u);
-    [&] { OL_STACK_DISPATCH(launch_closest2, l, w, in, st); }();
+    add(c_queue(in), C_RP2, 1u);
+    add(c_groups(in ^ 1u), C_GP2, 1u);
   }
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_plan, dim3(n), dim3(1024), 0, st, w.ctr, w.jobs, ps);
+  if (kinds & STEP_FLAT) [&] { OL_STACK_DISPATCH(launch_trace, l, w, in, st); }();
+  if (kinds & STEP_INSTANCED) [&] { OL_STACK_DISPATCH(launch_trace2, l, w, in, st); }();
 }
 
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
@@ -1432,17 +1457,6 @@ void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, h
   }
   static int occ = resident_blocks(k_shade, 256);
   hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in);
-}
-
-void wave_shadow(const GiLaunch& l, const Wave& w, uint32_t kinds, hipStream_t st) {
-  if (kinds & STEP_FLAT) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP, w.jobs, 0u);
-    [&] { OL_STACK_DISPATCH(launch_shadow, l, w, st); }();
-  }
-  if (kinds & STEP_INSTANCED) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, w.ctr + C_G, w.ctr + C_GP2, w.jobs, 1u);
-    [&] { OL_STACK_DISPATCH(launch_shadow2, l, w, st); }();
-  }
 }
 
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {

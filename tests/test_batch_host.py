@@ -33,13 +33,13 @@ def test_autotune_coordinate_search_picks_minimum():
     # Synthetic cost: best shape (8, 128), best capacity 2^23.
     def cost(s):
         shape = {(16, 128): 1.0, (8, 128): 0.8, (16, 256): 1.2, (32, 128): 1.1}[(s.lds_stack, s.trace_block)]
-        size = {1 << 22: 1.3, 1 << 23: 1.0, 1 << 24: 1.05, 1 << 25: 1.1}[s.path_slots]
+        size = {1 << 22: 1.3, 1 << 23: 1.0, 1 << 24: 1.05, 1 << 25: 1.1, 1 << 26: 1.15, 1 << 27: 1.2}[s.path_slots]
         return shape * size
 
     best, timings = autotune(None, [object()], measure=cost)
     assert (best.lds_stack, best.trace_block, best.path_slots) == (8, 128, 1 << 23)
-    # Shape search at the largest capacity (4) + capacity search at the best shape (3 new).
-    assert len(timings) == 7
+    # Shape search at the largest capacity (4) + capacity search at the best shape (5 new).
+    assert len(timings) == 9
     assert all(isinstance(k, EngineSettings) for k in timings)
 
 
