@@ -107,19 +107,30 @@ class Engine {
   uint32_t trace_launches = 0;
 
  private:
+  // A sample chunk of a job: resident in a device job slot while its paths
+  // are traced, then finished until its accumulation is queued, in sample
+  // order, together with the release of its slot ranges.
+  struct Chunk {
+    int slot = -1;
+    uint32_t spp = 0, base_sample = 0;
+    uint64_t region = 0, span = 0, ebase = 0, espan = 0;
+    uint32_t steps_in_chunk = 0;  // steps whose reports were applied
+    uint32_t steps_launched = 0;  // steps queued so far
+    uint64_t admit_seq = 0;       // step that generates the chunk
+  };
   struct Job {
     uint64_t id = 0;
     olhip_job_desc desc{};
     uint32_t per_hit = 1;
     bool transmits = false;
+    // ReSTIR passes are admitted one per step while earlier passes are still
+    // resident (a pass only reads the previous pass's reservoirs, written in
+    // that pass's first step); other jobs keep one chunk resident at a time.
+    bool stagger = false;
+    bool buffers = false;       // accumulation and product buffers allocated
     uint32_t next_sample = 0;   // first sample not yet admitted
-    int slot = -1;              // device job slot of the resident chunk
-    uint32_t chunk_spp = 0, chunk_base_sample = 0;
-    uint64_t region = 0, span = 0, ebase = 0, espan = 0;
-    bool first_chunk = true;
-    uint32_t steps_in_chunk = 0;   // steps of the resident chunk whose reports were applied
-    uint32_t steps_launched = 0;   // steps of the resident chunk queued so far
-    uint64_t admit_seq = 0;        // step that generates the resident chunk
+    uint32_t acc_next = 0;      // first sample whose accumulation is not yet queued
+    std::vector<Chunk> resident, finished;
     uint32_t passed = 0;        // admission rounds in which this job did not fit
     int status = OLHIP_JOB_QUEUED;
     DevBuf acc, image, geo, prim, lum, variance, res, surf;
@@ -151,6 +162,8 @@ class Engine {
   // Waits for the device before buffers that queued steps use are replaced.
   void quiesce();
   bool admit(Job& j, int slot);
+  Chunk* chunk_in_slot(Job& j, int slot);
+  void drop_chunks(Job& j);
   GiLaunch make_launch();
   double now_ms() const;
 
@@ -162,8 +175,7 @@ class Engine {
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs_;
   std::deque<uint64_t> queue_;    // waiting for admission (FIFO)
   std::vector<uint64_t> slot_job_;  // kMaxJobs: job id per device slot (0 = free)
-  std::vector<AccItem> pending_acc_;
-  std::vector<uint64_t> pending_acc_jobs_;  // jobs whose final accumulation is pending
+  std::vector<uint64_t> acc_jobs_;  // jobs with finished chunks not yet accumulated
   RangeAllocator paths_, entries_;
   uint64_t next_id_ = 1;
   uint32_t parity_ = 0;           // input queue of the next step

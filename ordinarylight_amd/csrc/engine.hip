@@ -256,6 +256,7 @@ uint64_t Engine::submit(const olhip_job_desc& d) {
   j->desc = d;
   j->per_hit = job_entries_per_hit(d.scene, s);
   j->transmits = d.scene->has_transmission;
+  j->stagger = s.restir_candidates > 0;
   // One sample chunk must fit an empty engine.
   const uint64_t n = (uint64_t)d.width * d.height;
   const uint32_t S = stripes_for(n);
@@ -293,17 +294,17 @@ bool Engine::admit(Job& j, int slot) {
       const uint64_t base = paths_.allocate(span);
       const uint64_t ebase = entries_.allocate(espan);
       if (base == UINT64_MAX || ebase == UINT64_MAX) throw OlError{OLHIP_ERR_RUNTIME, "slot allocator inconsistency"};
-      j.region = base;
-      j.span = span;
-      j.ebase = ebase;
-      j.espan = espan;
-      j.chunk_spp = (uint32_t)spp;
-      j.chunk_base_sample = j.next_sample;
+      Chunk c;
+      c.slot = slot;
+      c.region = base;
+      c.span = span;
+      c.ebase = ebase;
+      c.espan = espan;
+      c.spp = (uint32_t)spp;
+      c.base_sample = j.next_sample;
+      c.admit_seq = seq_;
       j.next_sample += (uint32_t)spp;
-      j.slot = slot;
-      j.steps_in_chunk = 0;
-      j.steps_launched = 0;
-      j.admit_seq = seq_;
+      j.resident.push_back(c);
       jobs_dirty_ = true;
       JobDev& J = jobs_state_[slot];
       std::memset(&J, 0, sizeof(J));
@@ -321,7 +322,7 @@ bool Engine::admit(Job& j, int slot) {
       P.spp = (uint32_t)spp;
       // The reference stores the frame index as float32 in the camera block.
       P.frame = (uint32_t)((float)d.frame_index + 0.5f);
-      P.sample_base = j.chunk_base_sample;
+      P.sample_base = c.base_sample;
       P.max_bounces = d.settings.max_bounces;
       P.point_count = d.scene->point_count;
       P.area_count = d.scene->view.area_count;
@@ -339,7 +340,7 @@ bool Engine::admit(Job& j, int slot) {
       J.seg = seg;
       J.ebase = (uint32_t)ebase;
       J.eseg = seg * j.per_hit;
-      if (j.first_chunk) {
+      if (!j.buffers) {
         j.acc.reserve(sizeof(double4) * np);
         j.image.reserve(sizeof(float4) * np);
         if (d.products & OLHIP_PRODUCT_PRIMARY) {
@@ -353,11 +354,11 @@ bool Engine::admit(Job& j, int slot) {
         j.stats.admit_ms = now_ms();
       }
       if (d.settings.restir_candidates) {
-        if (j.first_chunk) {
+        if (!j.buffers) {
           j.res.reserve(sizeof(uint4) * 2 * np);
           j.surf.reserve(sizeof(float4) * 4 * np);
         }
-        const uint32_t pass = j.chunk_base_sample;
+        const uint32_t pass = c.base_sample;
         P.restir = d.settings.restir_candidates;
         P.restir_spatial = d.settings.restir_spatial_neighbors;
         P.restir_radius = d.settings.restir_spatial_radius;
@@ -369,6 +370,7 @@ bool Engine::admit(Job& j, int slot) {
         J.surf_out = j.surf.as<float4>() + (pass & 1u) * 2 * np;
         J.surf_in = j.surf.as<float4>() + ((pass + 1u) & 1u) * 2 * np;
       }
+      j.buffers = true;
       J.geo = (d.products & OLHIP_PRODUCT_PRIMARY) ? j.geo.as<float4>() : nullptr;
       J.prim = (d.products & OLHIP_PRODUCT_PRIMARY) ? j.prim.as<uint32_t>() : nullptr;
       j.status = OLHIP_JOB_RUNNING;
@@ -394,6 +396,26 @@ bool Engine::admit(Job& j, int slot) {
     spp = std::min(spp - 1, fit);
   }
   return false;
+}
+
+Engine::Chunk* Engine::chunk_in_slot(Job& j, int slot) {
+  for (Chunk& c : j.resident)
+    if (c.slot == slot) return &c;
+  throw OlError{OLHIP_ERR_RUNTIME, "engine slot table inconsistency"};
+}
+
+// Releases the slot ranges of a failed job's chunks and forgets them.
+void Engine::drop_chunks(Job& j) {
+  for (const std::vector<Chunk>* v : {&j.resident, &j.finished})
+    for (const Chunk& c : *v) {
+      paths_.release(c.region, c.span);
+      entries_.release(c.ebase, c.espan);
+      if (c.slot >= 0) slot_job_[c.slot] = 0;
+    }
+  j.resident.clear();
+  j.finished.clear();
+  queue_.erase(std::remove(queue_.begin(), queue_.end(), j.id), queue_.end());
+  acc_jobs_.erase(std::remove(acc_jobs_.begin(), acc_jobs_.end(), j.id), acc_jobs_.end());
 }
 
 void Engine::launch_step(uint32_t key, bool timed, int buf) {
@@ -503,10 +525,49 @@ bool Engine::launch_next() {
   f.buf = b;
   {
     std::lock_guard<std::mutex> g(mu_);
+    // Accumulation: for each job, the finished chunk next in sample order,
+    // at most one per job and step (two items of one job would race on its
+    // pixel sums).  Its slot ranges are released here, before admission: a
+    // chunk generated in this step may reuse them, since accumulation runs
+    // first.
+    Ls.n_acc = 0;
+    Ls.acc_prefix[0] = 0;
+    for (auto it = acc_jobs_.begin(); it != acc_jobs_.end();) {
+      Job& j = *jobs_[*it];
+      auto c = std::find_if(j.finished.begin(), j.finished.end(),
+                            [&](const Chunk& x) { return x.base_sample == j.acc_next; });
+      if (c == j.finished.end() || Ls.n_acc == kMaxJobs) {
+        ++it;
+        continue;
+      }
+      const uint64_t np = (uint64_t)j.desc.width * j.desc.height;
+      AccItem a{};
+      a.base = (uint32_t)c->region;
+      a.np = (uint32_t)np;
+      a.spp = c->spp;
+      a.first_chunk = c->base_sample == 0 ? 1u : 0u;
+      const bool last = c->base_sample + c->spp >= j.desc.samples;
+      a.final_samples = last ? j.desc.samples : 0u;
+      a.acc = j.acc.as<double4>();
+      a.image = j.image.as<float4>();
+      a.lum = (j.desc.products & OLHIP_PRODUCT_VARIANCE) ? j.lum.as<double2>() : nullptr;
+      a.variance = (j.desc.products & OLHIP_PRODUCT_VARIANCE) ? j.variance.as<float>() : nullptr;
+      Ls.acc[Ls.n_acc] = a;
+      Ls.acc_prefix[Ls.n_acc + 1] = Ls.acc_prefix[Ls.n_acc] + a.np;
+      ++Ls.n_acc;
+      paths_.release(c->region, c->span);
+      entries_.release(c->ebase, c->espan);
+      j.acc_next += c->spp;
+      j.finished.erase(c);
+      if (last) f.finals.push_back(j.id);
+      if (j.finished.empty()) it = acc_jobs_.erase(it);
+      else ++it;
+    }
     Ls.n_gen = 0;
     Ls.gen_prefix[0] = 0;
     // Admission: first-fit over the queue; a job passed over for 8 steps
-    // blocks the ones behind it so it cannot starve.
+    // blocks the ones behind it so it cannot starve.  A staggered job stays
+    // queued until its last pass is admitted, one pass per step.
     for (auto it = queue_.begin(); it != queue_.end();) {
       int slot = -1;
       for (uint32_t k = 0; k < kMaxJobs; ++k)
@@ -520,26 +581,21 @@ bool Engine::launch_next() {
         Ls.gen_job[Ls.n_gen] = (uint32_t)slot;
         Ls.gen_prefix[Ls.n_gen + 1] = Ls.gen_prefix[Ls.n_gen] + jobs_state_[slot].n_paths;
         ++Ls.n_gen;
-        it = queue_.erase(it);
+        if (j.stagger && j.next_sample < j.desc.samples) ++it;
+        else it = queue_.erase(it);
         continue;
       }
-      if (++j.passed > 8) break;
+      // A staggered job waiting for memory its own passes hold does not
+      // block the jobs behind it.
+      if (j.resident.empty() && ++j.passed > 8) break;
       ++it;
     }
-    Ls.n_acc = (uint32_t)pending_acc_.size();
-    Ls.acc_prefix[0] = 0;
-    for (uint32_t k = 0; k < Ls.n_acc; ++k) {
-      Ls.acc[k] = pending_acc_[k];
-      Ls.acc_prefix[k + 1] = Ls.acc_prefix[k] + pending_acc_[k].np;
-    }
-    f.finals = pending_acc_jobs_;
-    pending_acc_.clear();
-    pending_acc_jobs_.clear();
     for (uint32_t k = 0; k < kMaxJobs; ++k) {
       if (slot_job_[k] == 0) continue;
       Job& j = *jobs_[slot_job_[k]];
-      if (j.steps_launched >= j.desc.settings.max_bounces) continue;
-      ++j.steps_launched;
+      Chunk& c = *chunk_in_slot(j, (int)k);
+      if (c.steps_launched >= j.desc.settings.max_bounces) continue;
+      ++c.steps_launched;
       traced = true;
       kinds |= j.desc.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
       if (j.desc.settings.restir_candidates) kinds |= STEP_RESTIR;
@@ -620,47 +676,33 @@ void Engine::retire_front() {
       for (uint32_t k = 0; k < kMaxJobs; ++k) {
         if (slot_job_[k] == 0) continue;
         Job& j = *jobs_[slot_job_[k]];
-        // A traversal stack overflow fails every resident chunk, including
-        // those of the step queued behind this one.
-        if (j.admit_seq > f.seq && !stack_overflow) continue;
-        const uint32_t rays = rep[4 * k], shadow = rep[4 * k + 1], live = rep[4 * k + 2];
-        if (j.admit_seq <= f.seq) {
-          if (j.steps_in_chunk == 0) j.stats.primary_rays += rays;
-          else j.stats.extension_rays += rays;
-          j.stats.shadow_rays += shadow;
-          totals_.rays += (uint64_t)rays + shadow;
-          ++j.steps_in_chunk;
-          ++j.stats.steps;
-        }
+        Chunk& c = *chunk_in_slot(j, (int)k);
+        // A traversal stack overflow fails every resident chunk's job,
+        // including chunks of the step queued behind this one.
         if (stack_overflow) {
           j.status = OLHIP_JOB_FAILED;
           j.error = "traversal stack overflow";
+          drop_chunks(j);
+          continue;
         }
-        if ((live == 0 && j.admit_seq <= f.seq) || stack_overflow) {
-          // Chunk finished: accumulate in the next step built, release its
-          // slots now (the accumulation precedes any generate that reuses
-          // them, and steps already queued hold no paths of this chunk).
-          AccItem a{};
-          a.base = (uint32_t)j.region;
-          a.np = j.desc.width * j.desc.height;
-          a.spp = j.chunk_spp;
-          a.first_chunk = j.first_chunk ? 1u : 0u;
-          const bool last = j.next_sample >= j.desc.samples;
-          a.final_samples = last ? j.desc.samples : 0u;
-          a.acc = j.acc.as<double4>();
-          a.image = j.image.as<float4>();
-          a.lum = (j.desc.products & OLHIP_PRODUCT_VARIANCE) ? j.lum.as<double2>() : nullptr;
-          a.variance = (j.desc.products & OLHIP_PRODUCT_VARIANCE) ? j.variance.as<float>() : nullptr;
-          if (!stack_overflow) {
-            pending_acc_.push_back(a);
-            if (last) pending_acc_jobs_.push_back(j.id);
-          }
-          paths_.release(j.region, j.span);
-          entries_.release(j.ebase, j.espan);
+        if (c.admit_seq > f.seq) continue;
+        const uint32_t rays = rep[4 * k], shadow = rep[4 * k + 1], live = rep[4 * k + 2];
+        if (c.steps_in_chunk == 0) j.stats.primary_rays += rays;
+        else j.stats.extension_rays += rays;
+        j.stats.shadow_rays += shadow;
+        totals_.rays += (uint64_t)rays + shadow;
+        ++c.steps_in_chunk;
+        ++j.stats.steps;
+        if (live == 0) {
+          // Chunk finished: its device slot is free now; its slot ranges are
+          // released when its accumulation is queued, in sample order.
+          Chunk done = c;
+          done.slot = -1;
+          j.resident.erase(j.resident.begin() + (&c - j.resident.data()));
           slot_job_[k] = 0;
-          j.slot = -1;
-          j.first_chunk = false;
-          if (!last && !stack_overflow) queue_.push_front(j.id);
+          if (j.finished.empty()) acc_jobs_.push_back(j.id);
+          j.finished.push_back(done);
+          if (!j.stagger && j.next_sample < j.desc.samples) queue_.push_front(j.id);
         }
       }
       if (stack_overflow) {
@@ -784,7 +826,7 @@ void Engine::start_worker() {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&] {
           if (stop_) return true;
-          if (!queue_.empty() || !pending_acc_.empty() || inflight_count_ > 0) return true;
+          if (!queue_.empty() || !acc_jobs_.empty() || inflight_count_ > 0) return true;
           for (uint32_t k = 0; k < kMaxJobs; ++k)
             if (slot_job_[k]) return true;
           return false;
@@ -801,6 +843,7 @@ void Engine::start_worker() {
             kv.second->error = e.message;
           }
         queue_.clear();
+        acc_jobs_.clear();
         std::fill(slot_job_.begin(), slot_job_.end(), 0);
         cv_.notify_all();
         break;
