@@ -63,6 +63,26 @@ float elapsed(const Event& a, const Event& b) {
 }
 
 constexpr uint64_t kFnvBasis = 1469598103934665603ull;
+// Emitter selection guide (DScene::area_guide): entry j is the first emitter
+// whose cumulative weight (record word 5 * i + 4 .x) is >= j / kAreaGuide,
+// clamped to the last emitter; entry kAreaGuide closes the last bucket.
+void upload_area_guide(olhip_scene* sc, const float* area_lights, uint32_t area_count, hipStream_t s) {
+  if (area_count == 0) {
+    sc->view.area_guide = nullptr;
+    return;
+  }
+  std::vector<uint32_t> guide(kAreaGuide + 1u);
+  uint32_t i = 0;
+  for (uint32_t j = 0; j <= kAreaGuide; ++j) {
+    const float x = (float)j / (float)kAreaGuide;
+    while (i + 1u < area_count && !(x <= area_lights[20ull * i + 16])) ++i;
+    guide[j] = i;
+  }
+  upload(sc->area_guide, guide.data(), guide.size(), s);
+  OL_HIP_CHECK(hipStreamSynchronize(s));
+  sc->view.area_guide = sc->area_guide.as<uint32_t>();
+}
+
 uint64_t fnv1a(const void* data, size_t bytes, uint64_t h) {
   const unsigned char* p = static_cast<const unsigned char*>(data);
   for (size_t i = 0; i < bytes; ++i) h = (h ^ p[i]) * 1099511628211ull;
@@ -187,6 +207,7 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     std::vector<float4> area(5ull * std::max<uint32_t>(d->area_light_count, 1), make_float4(0, 0, 0, 0));
     if (d->area_light_count) std::memcpy(area.data(), d->area_lights, sizeof(float) * 20ull * d->area_light_count);
     upload(sc->area, area.data(), area.size(), s);
+    upload_area_guide(sc.get(), d->area_lights, d->area_light_count, s);
     OL_HIP_CHECK(hipStreamSynchronize(s));
     sc->structure_hash = fnv1a(d->indices, 12ull * d->triangle_count, kFnvBasis);
     sc->leaf_size = d->leaf_size;
@@ -258,6 +279,7 @@ void upload_shading(olhip_scene* sc, uint32_t material_count, const float* mater
   std::vector<float4> area(5ull * std::max<uint32_t>(area_count, 1), make_float4(0, 0, 0, 0));
   if (area_count) std::memcpy(area.data(), area_lights, sizeof(float) * 20ull * area_count);
   upload(sc->area, area.data(), area.size(), s);
+  upload_area_guide(sc, area_lights, area_count, s);
   DScene& v = sc->view;
   v.materials = sc->materials.as<float4>();
   v.analytic = sc->analytic.as<float4>();

@@ -31,7 +31,7 @@
 
 struct olhip_scene {
   olhip_ctx* ctx = nullptr;
-  DevBuf positions, normals, vidx, materials, analytic, area;
+  DevBuf positions, normals, vidx, materials, analytic, area, area_guide;
   BvhOutput bvh;  // the scene's BVH, or the top-level BVH over instances
   // Two-level scenes: bottom-level BVHs per resource, instance and BLAS tables.
   std::vector<std::unique_ptr<BvhOutput>> blas;

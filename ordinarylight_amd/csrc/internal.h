@@ -155,4 +155,9 @@ struct DScene {
   uint32_t instanced, instance_count;
   const InstanceDev* inst;
   const BlasDev* blas;
+  // Emitter selection guide (null without area lights): entry j is the
+  // first emitter whose cumulative weight reaches j / kAreaGuide (clamped to
+  // the last), so a selection in bucket j lies between entries j and j + 1.
+  const uint32_t* area_guide;
 };
+constexpr uint32_t kAreaGuide = 4096;

@@ -161,12 +161,7 @@ OL_DEV void candidate_generate(const DScene& sc, uint32_t& rs, f3 hit, f3 n, f3 
   if (sc.area_count == 0u) return;
   rs = pcg_step(rs);
   const float selection = ((float)k + pcg_value(rs)) / (float)count;
-  uint32_t lower = 0u, upper = sc.area_count - 1u;
-  for (uint32_t step = 0; step < 32u && lower < upper; ++step) {
-    const uint32_t middle = lower + (upper - lower) / 2u;
-    if (selection <= sc.area[5u * middle + 4u].x) upper = middle;
-    else lower = middle + 1u;
-  }
+  const uint32_t lower = select_area_light(sc, selection);
   rs = pcg_step(rs);
   const float root_u = sqrtf(pcg_value(rs));
   rs = pcg_step(rs);

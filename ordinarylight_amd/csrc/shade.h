@@ -220,8 +220,18 @@ OL_DEV NeeSample point_sample(const DScene& sc, uint32_t li, f3 hit, f3 n, const
   return s;
 }
 
+// First emitter whose cumulative weight reaches the selection (the last
+// if none does), by bisection of the CDF (shadeSelectAreaLight,
+// wavefront_shade_candidate.glsl:1621-1645).  The guide narrows the range
+// to the selection's bucket first; the bisection answer does not depend on
+// the starting range as long as it contains that answer.
 OL_DEV uint32_t select_area_light(const DScene& sc, float selection) {
   uint32_t lower = 0, upper = sc.area_count - 1u;
+  if (sc.area_guide) {
+    const uint32_t b = min((uint32_t)(selection * (float)kAreaGuide), kAreaGuide - 1u);
+    lower = sc.area_guide[b];
+    upper = sc.area_guide[b + 1u];
+  }
   for (int step = 0; step < 32; ++step) {
     if (lower >= upper) break;
     uint32_t middle = lower + (upper - lower) / 2u;
