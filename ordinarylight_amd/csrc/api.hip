@@ -690,7 +690,11 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
     const uint64_t capacity = std::min<uint64_t>(set->path_capacity ? set->path_capacity : (1ull << 25),
                                                  (1ull << 28) / per_hit);
     const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(samples, capacity / np));
-    const uint64_t need = std::max<uint64_t>(4096, chunk * np + 2ull * kChunk * kJobStripes);
+    // A chunk's queue segments round its span up by at most two 64-path
+    // chunks per segment.  ReSTIR passes are resident together, one span
+    // each; other jobs keep one chunk of `chunk` samples resident.
+    const uint64_t pad = 2ull * kChunk * kJobStripes;
+    const uint64_t need = std::max<uint64_t>(4096, set->restir_candidates ? chunk * (np + pad) : chunk * np + pad);
     if (!ctx->frame_engine || ctx->frame_engine_slots < need || ctx->frame_engine_entries < per_hit) {
       ctx->frame_engine.reset();
       olhip_engine_config cfg{};
