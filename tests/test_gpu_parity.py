@@ -183,6 +183,28 @@ def test_gi_lights_transmission_matches_oracle(hip_device):
         assert ok.mean() >= 0.999
 
 
+def test_gi_many_emitters_matches_oracle(hip_device):
+    """3,000 emissive triangles of unequal power: the device narrows the
+    emitter bisection with its selection guide, the oracle bisects the whole
+    CDF, and both must pick the same emitters."""
+    from ordinarylight_amd.scene import Material
+    scene, cam = workloads.box_room(4)
+    rng = np.random.default_rng(11)
+    quad = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)  # facing down
+    for _ in range(1500):
+        x, z = rng.uniform(-0.95, 0.9, 2)
+        s = rng.uniform(0.01, 0.05)
+        v = np.array([[x, 1.999, z], [x + s, 1.999, z], [x + s, 1.999, z + s], [x, 1.999, z + s]], np.float32)
+        e = float(rng.uniform(0.5, 30.0))
+        scene.add_mesh(v, quad, Material(base_color=(0.5, 0.5, 0.5), emission=(e, 0.9 * e, 0.8 * e)))
+    log("gi many emitters")
+    img, ref, stats = gi_compare(scene, cam, 80, 60, 2, 4)
+    ok = rel_close(img[..., :3], ref[..., :3])
+    log(f"  within tol {ok.mean():.6f} shadow rays {stats.shadow_rays}")
+    assert stats.shadow_rays > 0
+    assert ok.mean() >= 0.999
+
+
 def test_gi_tile_capacity_invariance(hip_device):
     """Chunked residency gives the identical image (tests/gates/tile_quality.py)."""
     scene, cam = workloads.box_room(1)
