@@ -425,7 +425,7 @@ void Engine::launch_step(uint32_t key, bool timed, int buf) {
   if (key & 32u) wave_generate(wave_, in, ctx_->cu_count, s);
   if (!timed) {
     wave_trace(launch_, wave_, in, kinds, s);
-    wave_shade(launch_, wave_, in, kinds, s);
+    wave_shade(launch_, wave_, in, kinds, (key & 32u) != 0u, s);
   } else {
     // Read when the step is retired (no synchronisation inside the step).
     hipEvent_t* ev = prof_ev_[buf];
@@ -434,7 +434,7 @@ void Engine::launch_step(uint32_t key, bool timed, int buf) {
     OL_HIP_CHECK(hipEventRecord(ev[0], s));
     wave_trace(launch_, wave_, in, kinds, s);
     OL_HIP_CHECK(hipEventRecord(ev[1], s));
-    wave_shade(launch_, wave_, in, kinds, s);
+    wave_shade(launch_, wave_, in, kinds, (key & 32u) != 0u, s);
     OL_HIP_CHECK(hipEventRecord(ev[2], s));
   }
   wave_epilogue(wave_, in, s);
@@ -565,6 +565,7 @@ bool Engine::launch_next() {
     }
     Ls.n_gen = 0;
     Ls.gen_prefix[0] = 0;
+    Ls.fresh = 0ull;
     // Admission: first-fit over the queue; a job passed over for 8 steps
     // blocks the ones behind it so it cannot starve.  A staggered job stays
     // queued until its last pass is admitted, one pass per step.
@@ -579,6 +580,7 @@ bool Engine::launch_next() {
       Job& j = *jobs_[*it];
       if (admit(j, slot)) {
         Ls.gen_job[Ls.n_gen] = (uint32_t)slot;
+        Ls.fresh |= 1ull << slot;
         Ls.gen_prefix[Ls.n_gen + 1] = Ls.gen_prefix[Ls.n_gen] + jobs_state_[slot].n_paths;
         ++Ls.n_gen;
         if (j.stagger && j.next_sample < j.desc.samples) ++it;

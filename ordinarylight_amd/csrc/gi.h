@@ -112,6 +112,10 @@ struct AccItem {
 // Per-step work lists written by the host before each step.
 struct StepList {
   uint32_t n_gen, n_acc;
+  // Job slots generated this step (bit k: slot k): their paths are at bounce
+  // 0 with unit throughput and no radiance, which the generate kernel leaves
+  // unwritten and the shading kernels take without loading.
+  unsigned long long fresh;
   uint32_t gen_job[kMaxJobs];          // job slots whose primary rays are generated this step
   uint32_t gen_prefix[kMaxJobs + 1];   // exclusive prefix of their n_paths
   uint32_t acc_prefix[kMaxJobs + 1];   // exclusive prefix of the accumulation items' pixel counts
@@ -168,7 +172,8 @@ constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u;
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
 void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
-void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
+// `gen`: the step generates (its step list holds the fresh slots).
+void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, bool gen, hipStream_t st);
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
 void wave_clear(const Wave& w, hipStream_t st);
 

@@ -261,8 +261,8 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
     W.qo[in][slot] = mk4(org, 0.001f);
     W.qd[in][slot] = mk4(dir, 1e30f);
     W.qp[in][slot] = path;
-    W.T[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    W.L[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // T = (1, 1, 1, bounce 0) and L = 0 are implied for a fresh slot
+    // (StepList::fresh) and first written by shade.
     W.M[path] = make_uint4(pixel, (P.frame << 8) | (sample & 255u), rng, 257u);
     if (W.stk) W.stk[4u * path] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
@@ -952,7 +952,7 @@ OL_DEV void resolve_surface(const Wave& W, const DScene& sc, uint32_t i, uint32_
 // state and leaves the selected light sample and the advanced state for
 // shade.  Kept out of k_shade so that its registers do not lower shade's
 // occupancy for every other job.
-__global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in) {
+__global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t gen) {
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
   const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
@@ -964,11 +964,12 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in) {
     const JobDev& J = W.jobs[uniform(k.job)];
     const GiParams& P = J.P;
     if (!P.restir) continue;
+    // Primary hits only: the chunks generated this step.
+    if (!gen || !((W.list->fresh >> uniform(k.job)) & 1ull)) continue;
     if (lane_id() >= k.count) continue;
     const uint32_t i = k.first + lane_id();
     const uint32_t path = W.qp[in][i];
-    const float4 T = W.T[path];
-    if (T.w != 0.0f) continue;
+    const float4 T = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     const uint4 M = W.M[path];
     J.res_out[M.x] = reservoir_store(empty_reservoir());
     J.surf_out[2u * M.x] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
@@ -1003,7 +1004,8 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in) {
 // * 64 slots cannot overflow.
 // Four waves per SIMD: the register budget drops from 153 to 128 VGPRs with
 // a few spilled dwords, which measured 7 % faster than three waves.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Wave W, uint32_t in) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Wave W, uint32_t in,
+                                                                                       uint32_t gen) {
   const uint32_t out = in ^ 1u;
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
@@ -1032,7 +1034,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (live) {
       path = W.qp[in][i];
       const float4 ro = W.qo[in][i], rd = W.qd[in][i], hv = W.hit[i];
-      float4 T = W.T[path], L = W.L[path];
+      const bool fresh = gen && ((W.list->fresh >> job) & 1ull);
+      float4 T = fresh ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : W.T[path];
+      float4 L = fresh ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : W.L[path];
       uint4 M = W.M[path];
       incoming = rgb(rd);
       const uint32_t slot = __float_as_uint(hv.w);
@@ -1450,13 +1454,13 @@ void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, h
   if (kinds & STEP_INSTANCED) [&] { OL_STACK_DISPATCH(launch_trace2, l, w, in, st); }();
 }
 
-void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
-  if (w.rsamp && (kinds & STEP_RESTIR)) {
+void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, bool gen, hipStream_t st) {
+  if (w.rsamp && (kinds & STEP_RESTIR) && gen) {
     static int rocc = resident_blocks(k_restir, 256);
-    hipLaunchKernelGGL(k_restir, dim3((unsigned)(l.cu_count * rocc)), dim3(256), 0, st, w, in);
+    hipLaunchKernelGGL(k_restir, dim3((unsigned)(l.cu_count * rocc)), dim3(256), 0, st, w, in, 1u);
   }
   static int occ = resident_blocks(k_shade, 256);
-  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in);
+  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in, gen ? 1u : 0u);
 }
 
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {
