@@ -733,15 +733,15 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
       eng.take_result(id, nullptr, nullptr);  // throws with the failure reason
       throw OlError{OLHIP_ERR_RUNTIME, "render did not complete"};
     }
-    ctx->image.reserve(sizeof(float4) * np);
-    OL_HIP_CHECK(hipMemcpyAsync(ctx->image.ptr, eng.device_image(id), sizeof(float4) * np, hipMemcpyDeviceToDevice,
-                                ctx->stream));
-    OL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    // The frame stays in HBM for olhip_frame_output / olhip_frame_download:
+    // the job's image buffer becomes the context's (no copy).
+    eng.swap_image(id, ctx->image);
     ctx->image_pixels = (uint32_t)np;
     ctx->image_width = width;
     ctx->image_height = height;
     if (d.products) eng.products(id, depth, normal, primitive, variance);
-    eng.take_result(id, out_rgba, &js);
+    if (out_rgba) OL_HIP_CHECK(hipMemcpy(out_rgba, ctx->image.ptr, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    eng.take_result(id, nullptr, &js);
     if (stats_out) {
       olhip_stats& o = *stats_out;
       std::memset(&o, 0, sizeof(o));

@@ -89,6 +89,9 @@ class Engine {
   void engine_stats(olhip_engine_stats* out);
   // Device image of a done job (valid until take_result).
   const float4* device_image(uint64_t id);
+  // Exchanges a done job's image buffer with `buf` (no copy); the job keeps
+  // the previous contents of `buf` as its image.
+  void swap_image(uint64_t id, DevBuf& buf);
 
   void start_worker();
   void stop_worker();
@@ -162,6 +165,10 @@ class Engine {
   // Waits for the device before buffers that queued steps use are replaced.
   void quiesce();
   bool admit(Job& j, int slot);
+  // Per-job device buffers come from, and return to, a pool of buffers of
+  // forgotten jobs: consecutive frames of one size then allocate nothing.
+  void take_buffer(DevBuf& dst, size_t bytes);
+  void keep_buffers(Job& j);
   Chunk* chunk_in_slot(Job& j, int slot);
   void drop_chunks(Job& j);
   GiLaunch make_launch();
@@ -176,6 +183,9 @@ class Engine {
   std::deque<uint64_t> queue_;    // waiting for admission (FIFO)
   std::vector<uint64_t> slot_job_;  // kMaxJobs: job id per device slot (0 = free)
   std::vector<uint64_t> acc_jobs_;  // jobs with finished chunks not yet accumulated
+  std::vector<DevBuf> spare_;       // buffers of forgotten jobs (kept under kSpareBytes)
+  size_t spare_bytes_ = 0;
+  static constexpr size_t kSpareBytes = size_t(4) << 30;
   RangeAllocator paths_, entries_;
   uint64_t next_id_ = 1;
   uint32_t parity_ = 0;           // input queue of the next step

@@ -341,22 +341,22 @@ bool Engine::admit(Job& j, int slot) {
       J.ebase = (uint32_t)ebase;
       J.eseg = seg * j.per_hit;
       if (!j.buffers) {
-        j.acc.reserve(sizeof(double4) * np);
-        j.image.reserve(sizeof(float4) * np);
+        take_buffer(j.acc, sizeof(double4) * np);
+        take_buffer(j.image, sizeof(float4) * np);
         if (d.products & OLHIP_PRODUCT_PRIMARY) {
-          j.geo.reserve(sizeof(float4) * np);
-          j.prim.reserve(sizeof(uint32_t) * np);
+          take_buffer(j.geo, sizeof(float4) * np);
+          take_buffer(j.prim, sizeof(uint32_t) * np);
         }
         if (d.products & OLHIP_PRODUCT_VARIANCE) {
-          j.lum.reserve(sizeof(double2) * np);
-          j.variance.reserve(sizeof(float) * np);
+          take_buffer(j.lum, sizeof(double2) * np);
+          take_buffer(j.variance, sizeof(float) * np);
         }
         j.stats.admit_ms = now_ms();
       }
       if (d.settings.restir_candidates) {
         if (!j.buffers) {
-          j.res.reserve(sizeof(uint4) * 2 * np);
-          j.surf.reserve(sizeof(float4) * 4 * np);
+          take_buffer(j.res, sizeof(uint4) * 2 * np);
+          take_buffer(j.surf, sizeof(float4) * 4 * np);
         }
         const uint32_t pass = c.base_sample;
         P.restir = d.settings.restir_candidates;
@@ -396,6 +396,36 @@ bool Engine::admit(Job& j, int slot) {
     spp = std::min(spp - 1, fit);
   }
   return false;
+}
+
+// The smallest spare buffer that holds `bytes` (its contents are not
+// preserved), else a new allocation.  Called with mu_ held.
+void Engine::take_buffer(DevBuf& dst, size_t bytes) {
+  size_t best = spare_.size();
+  for (size_t k = 0; k < spare_.size(); ++k)
+    if (spare_[k].bytes >= bytes && (best == spare_.size() || spare_[k].bytes < spare_[best].bytes)) best = k;
+  if (best == spare_.size()) {
+    dst.reserve(bytes);
+    return;
+  }
+  spare_bytes_ -= spare_[best].bytes;
+  dst = std::move(spare_[best]);
+  spare_.erase(spare_.begin() + best);
+}
+
+// Keeps a forgotten job's buffers for later jobs, evicting the oldest spare
+// buffers beyond kSpareBytes.  Called with mu_ held, once the job's last
+// step has completed.
+void Engine::keep_buffers(Job& j) {
+  for (DevBuf* b : {&j.acc, &j.image, &j.geo, &j.prim, &j.lum, &j.variance, &j.res, &j.surf}) {
+    if (!b->ptr) continue;
+    spare_bytes_ += b->bytes;
+    spare_.push_back(std::move(*b));
+  }
+  while (spare_bytes_ > kSpareBytes && !spare_.empty()) {
+    spare_bytes_ -= spare_.front().bytes;
+    spare_.erase(spare_.begin());
+  }
 }
 
 Engine::Chunk* Engine::chunk_in_slot(Job& j, int slot) {
@@ -743,6 +773,13 @@ int Engine::wait(uint64_t id, double timeout_s) {
   return it == jobs_.end() ? OLHIP_JOB_UNKNOWN : it->second->status;
 }
 
+void Engine::swap_image(uint64_t id, DevBuf& buf) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end() || it->second->status != OLHIP_JOB_DONE) throw OlError{OLHIP_ERR_RUNTIME, "job is not done"};
+  it->second->image.swap(buf);
+}
+
 const float4* Engine::device_image(uint64_t id) {
   std::lock_guard<std::mutex> g(mu_);
   auto it = jobs_.find(id);
@@ -779,6 +816,8 @@ void Engine::take_result(uint64_t id, float* out_rgba, olhip_job_stats* stats, b
   if (out_rgba)
     OL_HIP_CHECK(hipMemcpy(out_rgba, j->image.ptr, sizeof(float4) * j->desc.width * j->desc.height,
                            hipMemcpyDeviceToHost));
+  std::lock_guard<std::mutex> g(mu_);
+  keep_buffers(*j);
 }
 
 void Engine::products(uint64_t id, float* depth, float* normal, uint32_t* primitive, float* variance) {

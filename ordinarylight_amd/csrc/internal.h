@@ -30,6 +30,24 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  void swap(DevBuf& o) noexcept {
+    std::swap(ptr, o.ptr);
+    std::swap(bytes, o.bytes);
+  }
   ~DevBuf() { release(); }
   void release() {
     if (ptr) (void)hipFree(ptr);
