@@ -96,14 +96,22 @@ def camera_projection(camera):
     return PROJECTION_PERSPECTIVE
 
 
+def _cross3(a, b):
+    """float32 cross product of two 3-vectors, rounded as ``np.cross`` rounds
+    (each product, then the difference), without its per-call overhead."""
+    a0, a1, a2 = a
+    b0, b1, b2 = b
+    return np.array((a1 * b2 - a2 * b1, a2 * b0 - a0 * b2, a0 * b1 - a1 * b0), dtype=np.float32)
+
+
 def camera_basis(camera):
     """float32 ``(position, forward, right, up)`` with scaled right/up."""
     position = np.asarray(camera.position, dtype=np.float32)
     forward = np.asarray(camera.target, dtype=np.float32) - position
     forward /= np.linalg.norm(forward)
-    right = np.cross(forward, np.asarray(camera.up, dtype=np.float32))
+    right = _cross3(forward, np.asarray(camera.up, dtype=np.float32))
     right /= np.linalg.norm(right)
-    up = np.cross(right, forward)
+    up = _cross3(right, forward)
     projection = camera_projection(camera)
     if projection == PROJECTION_ORTHOGRAPHIC:
         scale = float(camera.vertical_size) * 0.5

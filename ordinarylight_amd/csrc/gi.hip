@@ -111,9 +111,20 @@ OL_DEV uint32_t wave_sum(uint32_t v) {
 }
 
 // ---- accumulate + generate (host lists) ------------------------------------------------------
-// Index of the list item owning global thread index g (prefix[n] = total).
+// Index of the list item owning global thread index g (prefix[n] = total):
+// the last k < n with prefix[k] <= g.  The wave's first active lane (the
+// lowest g) binary-searches with wave-uniform loads; every lane then steps
+// forward from that item, which already holds it unless the wave straddles
+// an item boundary.
 OL_DEV uint32_t list_item(const uint32_t* prefix, uint32_t n, uint32_t g) {
-  uint32_t k = 0;
+  const uint32_t g0 = uniform(g);
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g0) lo = mid;
+    else hi = mid;
+  }
+  uint32_t k = lo;
   while (k + 1u < n && prefix[k + 1u] <= g) ++k;
   return k;
 }
@@ -142,7 +153,9 @@ __global__ void __launch_bounds__(256) k_accumulate(Wave W) {
         a.w += 1.0;
       }
     }
-    it.acc[pixel] = a;
+    // The running sums are read only by the job's later chunks.
+    const bool only_chunk = it.first_chunk && it.final_samples;
+    if (!only_chunk) it.acc[pixel] = a;
     if (it.final_samples) {
       const double ns = (double)it.final_samples;
       it.image[pixel] = make_float4((float)(a.x / ns), (float)(a.y / ns), (float)(a.z / ns), (float)(a.w / ns));
@@ -161,7 +174,7 @@ __global__ void __launch_bounds__(256) k_accumulate(Wave W) {
           m.y += y * y;
         }
       }
-      it.lum[pixel] = m;
+      if (!only_chunk) it.lum[pixel] = m;
       if (it.final_samples) {
         const double n = (double)it.final_samples;
         double v = 0.0;
@@ -172,13 +185,18 @@ __global__ void __launch_bounds__(256) k_accumulate(Wave W) {
   }
 }
 
+// Initial random state of the camera path of (pixel, sample).
+OL_DEV uint32_t primary_seed(uint32_t frame, uint32_t pixel, uint32_t sample) {
+  return wave_hash((pixel ^ wave_hash(frame)) ^ wave_hash(sample + 1u));
+}
+
 // Camera ray of one (pixel, sample) with the hash jitter of
 // wavefront_generate.comp:115-149; rng is the path's initial state.
 OL_DEV void primary_ray(const GiParams& P, uint32_t px, uint32_t py, uint32_t sample, f3& org, f3& dir,
                         uint32_t& rng) {
   const uint32_t Wd = P.width, H = P.height;
   const uint32_t pixel = py * Wd + px;
-  rng = wave_hash((pixel ^ wave_hash(P.frame)) ^ wave_hash(sample + 1u));
+  rng = primary_seed(P.frame, pixel, sample);
   rng = wave_hash(rng);
   const float jx = u32_to_unit(rng);
   rng = wave_hash(rng);
@@ -203,6 +221,19 @@ OL_DEV void primary_ray(const GiParams& P, uint32_t px, uint32_t py, uint32_t sa
   } else {
     dir = normalize3((fw + ((ndx * aspect) * rt)) - (ndy * up));
   }
+}
+
+// Path state (M) of a camera path before its first shading: pixel,
+// frame << 8 | sample, the random state primary_ray leaves after its two
+// jitter draws, active at medium depth 1.  Generated slots store no copy
+// (like their unit throughput and zero radiance): the shading kernels rebuild
+// it from the path's offset in its chunk, local sample * pixels + pixel.
+OL_DEV uint4 fresh_path_state(const GiParams& P, uint32_t rel) {
+  const uint32_t np = P.width * P.height;
+  const uint32_t local = rel / np, pixel = rel - local * np;
+  const uint32_t sample = P.sample_base + local;
+  const uint32_t rng = wave_hash(wave_hash(primary_seed(P.frame, pixel, sample)));
+  return make_uint4(pixel, (P.frame << 8) | (sample & 255u), rng, F_ACTIVE | (1u << 8));
 }
 
 // Primary rays of a pixel tile [x0, x0 + tw) x [y0, y0 + th), row-major
@@ -261,9 +292,8 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
     W.qo[in][slot] = mk4(org, 0.001f);
     W.qd[in][slot] = mk4(dir, 1e30f);
     W.qp[in][slot] = path;
-    // T = (1, 1, 1, bounce 0) and L = 0 are implied for a fresh slot
-    // (StepList::fresh) and first written by shade.
-    W.M[path] = make_uint4(pixel, (P.frame << 8) | (sample & 255u), rng, 257u);
+    // T = (1, 1, 1, bounce 0), L = 0 and M = fresh_path_state are implied
+    // for a fresh slot (StepList::fresh) and first written by shade.
     if (W.stk) W.stk[4u * path] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   }
 }
@@ -970,7 +1000,7 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t ge
     const uint32_t i = k.first + lane_id();
     const uint32_t path = W.qp[in][i];
     const float4 T = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    const uint4 M = W.M[path];
+    const uint4 M = fresh_path_state(P, path - J.base);
     J.res_out[M.x] = reservoir_store(empty_reservoir());
     J.surf_out[2u * M.x] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     J.surf_out[2u * M.x + 1u] = make_float4(__uint_as_float(OL_NONE), 0.0f, 0.0f, 0.0f);
@@ -1037,7 +1067,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       const bool fresh = gen && ((W.list->fresh >> job) & 1ull);
       float4 T = fresh ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : W.T[path];
       float4 L = fresh ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : W.L[path];
-      uint4 M = W.M[path];
+      uint4 M = fresh ? fresh_path_state(P, path - J.base) : W.M[path];
       incoming = rgb(rd);
       const uint32_t slot = __float_as_uint(hv.w);
       do {
@@ -1233,9 +1263,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         co = mk4(hitp + dir * 0.002f, 0.001f);
         cd = mk4(dir, 1e30f);
       } while (false);
-      W.T[path] = T;
+      // Throughput and state are read only by the path's next shading.
+      if (cont) {
+        W.T[path] = T;
+        W.M[path] = M;
+      }
       W.L[path] = L;
-      W.M[path] = M;
     }
     // Continuation slot in the job's output segment (one atomic per wave).
     const uint32_t cl = wave_reserve(&W.ctr[c_queue(out) + run], cont);
