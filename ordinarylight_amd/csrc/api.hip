@@ -228,9 +228,11 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, d->leaf_size,
                 boxes.empty() ? nullptr : boxes.data()};
     bvh_build(in, sc->bvh, s, nullptr, true);
+    build_shade_records(sc->bvh, sc->vidx.as<uint4>(), sc->normals.as<float4>(), d->triangle_count, true, s);
     DScene& v = sc->view;
     v.nodes = sc->bvh.nodes.as<WideNode>();
     v.tri = sc->bvh.tri.as<float4>();
+    v.shade = sc->bvh.shade.as<float4>();
     v.tri_vidx = sc->vidx.as<uint4>();
     v.normals = sc->normals.as<float4>();
     v.materials = sc->materials.as<float4>();
@@ -454,9 +456,10 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, 
         BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, d->leaf_size,
                     boxes.empty() ? nullptr : boxes.data()};
         bvh_build(in, *out_r, s, nullptr, true);
+        build_shade_records(*out_r, sc->vidx.as<uint4>() + to, sc->normals.as<float4>() + vo, nt, false, s);
       }
       blas[r] = BlasDev{out_r->nodes.as<WideNode>(), out_r->tri.as<float4>(), sc->vidx.as<uint4>() + to,
-                        sc->normals.as<float4>() + vo, out_r->node_count, nt, 0u, 0u};
+                        sc->normals.as<float4>() + vo, out_r->shade.as<float4>(), out_r->node_count, nt, 0u, 0u};
       blas_depth = std::max(blas_depth, out_r->depth);
       sc->blas.push_back(std::move(out_r));
     }
@@ -513,6 +516,9 @@ int olhip_scene_update(olhip_ctx* ctx, olhip_scene* sc, const olhip_scene_desc* 
       v.normals = sc->normals.as<float4>();
       v.node_count = sc->bvh.node_count;
     }
+    // Normals or material ids may have changed.
+    build_shade_records(sc->bvh, sc->vidx.as<uint4>(), sc->normals.as<float4>(), d->triangle_count, true, s);
+    sc->view.shade = sc->bvh.shade.as<float4>();
     sc->has_transmission = false;
     upload_shading(sc, d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
                    d->area_light_count, d->area_lights, d->area_light_weight, s);
@@ -570,9 +576,10 @@ int olhip_scene_update_instanced(olhip_ctx* ctx, olhip_scene* sc, const olhip_in
         if (nt) {
           BvhInput in{sc->positions.as<float4>() + vo, sc->vidx.as<uint4>() + to, nt, sc->leaf_size};
           bvh_refit(in, b, s);
+          build_shade_records(b, sc->vidx.as<uint4>() + to, sc->normals.as<float4>() + vo, nt, false, s);
         }
         blas[r] = BlasDev{b.nodes.as<WideNode>(), b.tri.as<float4>(), sc->vidx.as<uint4>() + to,
-                          sc->normals.as<float4>() + vo, b.node_count, nt, 0u, 0u};
+                          sc->normals.as<float4>() + vo, b.shade.as<float4>(), b.node_count, nt, 0u, 0u};
         blas_depth = std::max(blas_depth, b.depth);
       }
       upload(sc->blas_table, blas.data(), blas.size(), s);

@@ -169,6 +169,10 @@ int stack_depth_for(uint32_t bvh_depth);
 // `kinds` says which resident work a step holds (STEP_* bits); the launches
 // for absent kinds are left out (their chunk totals stay zero).
 constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u;
+// Fills b.shade from the hierarchy's leaf triangles, the triangles' vertex
+// indices (material in .w when `material`) and vertex normals.
+void build_shade_records(BvhOutput& b, const uint4* vidx, const float4* normals, uint32_t count, bool material,
+                         hipStream_t st);
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
 void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);

@@ -927,6 +927,23 @@ OL_DEV void restir_nee(const GiParams& P, const NeeSample& chosen, uint32_t& rs,
   }
 }
 
+// Shading record of each leaf triangle (BvhOutput::shade): the vertex
+// normals, material, original triangle id and edge cross product that
+// resolve_surface needs, gathered once per hierarchy instead of per hit.
+__global__ void __launch_bounds__(256) k_shade_records(const float4* __restrict__ tri, const uint4* __restrict__ vidx,
+                                                       const float4* __restrict__ normals, uint32_t count,
+                                                       uint32_t material, float4* __restrict__ rec) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  const uint32_t id = __float_as_uint(tri[3u * s].w);
+  const uint4 vi = vidx[id];
+  const f3 cw = cross3(mk3(tri[3u * s + 1u]), mk3(tri[3u * s + 2u]));
+  rec[4u * s] = mk4(rgb(normals[vi.x]), __uint_as_float(material ? vi.w : 0u));
+  rec[4u * s + 1u] = mk4(rgb(normals[vi.y]), __uint_as_float(id));
+  rec[4u * s + 2u] = mk4(rgb(normals[vi.z]), 0.0f);
+  rec[4u * s + 3u] = mk4(cw, 0.0f);
+}
+
 // Hit reconstruction (shadeLoadHit) and surface (shadeResolveSurface,
 // wave_shade_candidate.glsl:957-997, :1170-1215): position, geometric
 // normal, oriented shading normal, the world-space edge cross product, the
@@ -934,36 +951,28 @@ OL_DEV void restir_nee(const GiParams& P, const NeeSample& chosen, uint32_t& rs,
 OL_DEV void resolve_surface(const Wave& W, const DScene& sc, uint32_t i, uint32_t slot, float4 ro, float4 rd, float4 hv,
                             f3& hitp, f3& gn, f3& nrm, f3& cw, uint32_t& prim, uint32_t& mat_id, bool& entering) {
   const float t = hv.x, bu = hv.y, bv = hv.z;
-  float4 ta, te1, te2;
-  uint4 vi;
   f3 n0, n1, n2;
   if (!sc.instanced) {
-    ta = sc.tri[3 * slot];
-    te1 = sc.tri[3 * slot + 1];
-    te2 = sc.tri[3 * slot + 2];
-    prim = __float_as_uint(ta.w);
-    vi = sc.tri_vidx[prim];
-    n0 = rgb(sc.normals[vi.x]);
-    n1 = rgb(sc.normals[vi.y]);
-    n2 = rgb(sc.normals[vi.z]);
-    cw = cross3(mk3(te1), mk3(te2));
-    mat_id = vi.w;
+    const float4* r = sc.shade + 4u * slot;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    n0 = rgb(r0);
+    n1 = rgb(r1);
+    n2 = rgb(r2);
+    cw = rgb(r3);
+    mat_id = __float_as_uint(r0.w);
+    prim = __float_as_uint(r1.w);
   } else {
     // Object-space triangle of the instance: normals and the edge cross
     // product go to world space through the normal matrix; the cross
     // product of world edges is det(A) * N * (e1 x e2).
     const InstanceDev& I = sc.inst[W.hit_inst[i]];
-    const BlasDev& B = sc.blas[I.blas];
-    ta = B.tri[3 * slot];
-    te1 = B.tri[3 * slot + 1];
-    te2 = B.tri[3 * slot + 2];
-    const uint32_t local = __float_as_uint(ta.w);
-    prim = I.tri_offset + local;
-    vi = B.vidx[local];
-    n0 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.x])));
-    n1 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.y])));
-    n2 = normalize3(xform_vector(I.nrm, rgb(B.normals[vi.z])));
-    cw = xform_vector(I.nrm, cross3(mk3(te1), mk3(te2))) * I.nrm[0].w;
+    const float4* r = sc.blas[I.blas].shade + 4u * slot;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    prim = I.tri_offset + __float_as_uint(r1.w);
+    n0 = normalize3(xform_vector(I.nrm, rgb(r0)));
+    n1 = normalize3(xform_vector(I.nrm, rgb(r1)));
+    n2 = normalize3(xform_vector(I.nrm, rgb(r2)));
+    cw = xform_vector(I.nrm, rgb(r3)) * I.nrm[0].w;
     mat_id = I.material;
   }
   hitp = mk3(ro.x + t * rd.x, ro.y + t * rd.y, ro.z + t * rd.z);
@@ -1453,6 +1462,15 @@ int stack_depth_for(uint32_t bvh_depth) {
 uint64_t trace_spill_words(const GiLaunch& l) { OL_STACK_DISPATCH(spill_words_for, l); }
 
 void wave_clear(const Wave& w, hipStream_t st) { hipLaunchKernelGGL(k_clear, dim3(1), dim3(1024), 0, st, w); }
+
+void build_shade_records(BvhOutput& b, const uint4* vidx, const float4* normals, uint32_t count, bool material,
+                         hipStream_t st) {
+  if (count == 0) return;
+  b.shade.reserve(sizeof(float4) * 4ull * count);
+  hipLaunchKernelGGL(k_shade_records, dim3((count + 255u) / 256u), dim3(256), 0, st, b.tri.as<float4>(), vidx, normals,
+                     count, material ? 1u : 0u, b.shade.as<float4>());
+  OL_HIP_CHECK(hipGetLastError());
+}
 
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st) {
   hipLaunchKernelGGL(k_accumulate, dim3((unsigned)(cu_count * 4)), dim3(256), 0, st, w);

@@ -96,6 +96,10 @@ struct BvhTopology {
 struct BvhOutput {
   DevBuf nodes;      // WideNode[node_count]
   DevBuf tri;        // float4[3 * T]: (v0, orig id bits), e1, e2 in leaf order
+  // float4[4 * T] shading records in leaf order (triangle hierarchies of
+  // shaded scenes): (n0, material bits), (n1, triangle id bits), (n2, 0),
+  // (e1 x e2, 0), so a hit's surface is one 64-byte read.
+  DevBuf shade;
   uint32_t node_count = 0;
   uint32_t depth = 0;      // wide levels (root = level 1)
   double build_ms = 0.0;
@@ -152,6 +156,7 @@ struct BlasDev {
   const float4* tri;      // leaf-order (v0, local id bits), e1, e2
   const uint4* vidx;      // local triangle -> (i0, i1, i2, unused), indices into `normals`
   const float4* normals;  // object-space vertex normals of this resource
+  const float4* shade;    // leaf-order shading records (BvhOutput::shade; material bits 0)
   uint32_t node_count, tri_count, pad0, pad1;
 };
 
@@ -160,6 +165,7 @@ struct DScene {
   const float4* tri;        // leaf-order traversal triangles
   const uint4* tri_vidx;    // original order (i0, i1, i2, material)
   const float4* normals;    // [V]
+  const float4* shade;      // leaf-order shading records (BvhOutput::shade)
   const float4* materials;  // [M * 12]
   const float4* analytic;   // [P * 4]
   const float4* area;       // [L * 5]
