@@ -352,7 +352,7 @@ OL_DEV void closest_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
-  uint32_t* cursor = &W.ctr[C_W + ws];
+  uint32_t* cursor = &W.ctr[C_W + ws * kCursorStride];
   const float4* __restrict__ qo = W.qo[in];
   const float4* __restrict__ qd = W.qd[in];
   Pool pool{0u, 0u, nullptr, nullptr, 0u, false};
@@ -445,7 +445,7 @@ OL_DEV void shadow_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
-  uint32_t* cursor = &W.ctr[C_WS + ws];
+  uint32_t* cursor = &W.ctr[C_WS + ws * kCursorStride];
   Pool pool{0u, 0u, nullptr, nullptr, 0u, false};
   AnyRay q;
   q.node = OL_NONE;
@@ -556,7 +556,7 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
-  uint32_t* cursor = &W.ctr[C_W2 + ws];
+  uint32_t* cursor = &W.ctr[C_W2 + ws * kCursorStride];
   const float4* __restrict__ qo = W.qo[in];
   const float4* __restrict__ qd = W.qd[in];
   Pool2 pool{0u, 0u, 0u, false};
@@ -624,7 +624,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
-  uint32_t* cursor = &W.ctr[C_WS2 + ws];
+  uint32_t* cursor = &W.ctr[C_WS2 + ws * kCursorStride];
   Pool2 pool{0u, 0u, 0u, false};
   Any2 q;
   q.node = OL_NONE;
@@ -1358,7 +1358,7 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
     W.ctr[c_groups(in ^ 1u) + r] = 0u;
     W.ctr[c_entries(in ^ 1u) + r] = 0u;
   }
-  if (t < 4 * kWorkStripes) W.ctr[C_W + t] = 0u;
+  if (t < 4 * kWorkStripes) W.ctr[C_W + t * kCursorStride] = 0u;
   // Chunk totals of the four plans: a plan left out of the next step (no
   // resident job of its kind) must read as empty.
   if (t == 0) {
