@@ -75,6 +75,10 @@ constexpr uint32_t C_WORDS = C_REPORT + 4 * kMaxJobs;
 OL_HD constexpr uint32_t c_queue(uint32_t q) { return q ? C_Q1 : C_Q0; }
 OL_HD constexpr uint32_t c_groups(uint32_t p) { return p ? C_G1 : C_G0; }
 OL_HD constexpr uint32_t c_entries(uint32_t p) { return p ? C_E1 : C_E0; }
+// Word of run r (job * kJobStripes + stripe) in a run-counter block,
+// stripe-major: the stripes of one job, which the waves of its chunks
+// reserve queue space from with atomics, lie in different 256-byte lines.
+OL_HD constexpr uint32_t run_word(uint32_t r) { return (r % kJobStripes) * kMaxJobs + r / kJobStripes; }
 
 // Device job record (read wave-uniformly by the kernels).
 struct JobDev {

@@ -53,8 +53,8 @@ __global__ void __launch_bounds__(1024) k_plan(uint32_t* __restrict__ ctr, const
   uint32_t* __restrict__ out = ctr + ps.out[blockIdx.x];
   const uint32_t kind = ps.kind[blockIdx.x];
   const bool mine = (jobs[(2 * t) / kJobStripes].P.sc.instanced != 0u) == (kind != 0u);
-  const uint32_t a = mine ? (counts[2 * t] + kChunk - 1u) / kChunk : 0u;
-  const uint32_t b = mine ? (counts[2 * t + 1] + kChunk - 1u) / kChunk : 0u;
+  const uint32_t a = mine ? (counts[run_word(2 * t)] + kChunk - 1u) / kChunk : 0u;
+  const uint32_t b = mine ? (counts[run_word(2 * t + 1)] + kChunk - 1u) / kChunk : 0u;
   const uint32_t sum = a + b;
   uint32_t incl = sum;
   for (int off = 1; off < 64; off <<= 1) {
@@ -97,7 +97,7 @@ OL_DEV ChunkLoc locate_chunk(const uint32_t* __restrict__ prefix, const uint32_t
   k.job = r / kJobStripes;
   k.stripe = r % kJobStripes;
   const uint32_t off = (c - prefix[r]) * kChunk;
-  k.count = min(kChunk, counts[r] - off);
+  k.count = min(kChunk, counts[run_word(r)] - off);
   k.first = jobs[k.job].base + k.stripe * jobs[k.job].seg + off;
   k.chunk_in_job = c - prefix[k.job * kJobStripes];
   return k;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
     if (i == 0) {
       for (uint32_t s = 0; s < J.stripes; ++s) {
         const uint32_t lo = s * J.seg;
-        W.ctr[c_queue(in) + job * kJobStripes + s] = J.n_paths > lo ? min(J.seg, J.n_paths - lo) : 0u;
+        W.ctr[c_queue(in) + run_word(job * kJobStripes + s)] = J.n_paths > lo ? min(J.seg, J.n_paths - lo) : 0u;
       }
     }
     // i -> (sample, 8x8 tile, position in tile) in tile order.
@@ -1280,7 +1280,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       W.L[path] = L;
     }
     // Continuation slot in the job's output segment (one atomic per wave).
-    const uint32_t cl = wave_reserve(&W.ctr[c_queue(out) + run], cont);
+    const uint32_t cl = wave_reserve(&W.ctr[c_queue(out) + run_word(run)], cont);
     if (cont) {
       if (cl < J.seg) {
         const uint32_t cslot = J.base + so * J.seg + cl;
@@ -1294,8 +1294,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     // Shadow group + entries.
     const bool has_group = nee && nee_count > 0u;
-    const uint32_t gl = wave_reserve(&W.ctr[c_groups(in) + run], has_group);
-    const uint32_t el = wave_reserve_n(&W.ctr[c_entries(in) + run], has_group ? nee_count : 0u);
+    const uint32_t gl = wave_reserve(&W.ctr[c_groups(in) + run_word(run)], has_group);
+    const uint32_t el = wave_reserve_n(&W.ctr[c_entries(in) + run_word(run)], has_group ? nee_count : 0u);
     if (has_group) {
       if (gl < J.seg && el + nee_count <= J.eseg) {
         const uint32_t gslot = J.base + so * J.seg + gl, eslot = J.ebase + so * J.eseg + el;
@@ -1348,15 +1348,15 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
   __syncthreads();
   for (uint32_t r = t; r < kRuns; r += blockDim.x) {
     const uint32_t job = r / kJobStripes;
-    const uint32_t a = W.ctr[c_queue(in) + r], e = W.ctr[c_entries(in ^ 1u) + r],
-                   l = W.ctr[c_queue(in ^ 1u) + r], g = W.ctr[c_groups(in) + r];
+    const uint32_t a = W.ctr[c_queue(in) + run_word(r)], e = W.ctr[c_entries(in ^ 1u) + run_word(r)],
+                   l = W.ctr[c_queue(in ^ 1u) + run_word(r)], g = W.ctr[c_groups(in) + run_word(r)];
     if (a) atomicAdd(&rep[4 * job + 0], a);
     if (e) atomicAdd(&rep[4 * job + 1], e);
     if (l + g) atomicAdd(&rep[4 * job + 2], l + g);
     if (g) atomicAdd(&rep[4 * job + 3], g);
-    W.ctr[c_queue(in) + r] = 0u;
-    W.ctr[c_groups(in ^ 1u) + r] = 0u;
-    W.ctr[c_entries(in ^ 1u) + r] = 0u;
+    W.ctr[c_queue(in) + run_word(r)] = 0u;
+    W.ctr[c_groups(in ^ 1u) + run_word(r)] = 0u;
+    W.ctr[c_entries(in ^ 1u) + run_word(r)] = 0u;
   }
   if (t < 4 * kWorkStripes) W.ctr[C_W + t * kCursorStride] = 0u;
   // Chunk totals of the four plans: a plan left out of the next step (no
