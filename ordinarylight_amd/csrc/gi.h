@@ -47,7 +47,11 @@ constexpr uint32_t kMaxJobs = 64;
 constexpr uint32_t kJobStripes = 32;
 constexpr uint32_t kRuns = kMaxJobs * kJobStripes;
 constexpr uint32_t kChunk = 64;
-constexpr uint32_t kWorkStripes = 32;  // work cursors of the refilling traversal kernels
+// Work cursors of the refilling traversal kernels: block b takes chunks from
+// stripe b % kWorkStripes, and workgroups are dealt to the eight XCDs round
+// robin, so with eight stripes each XCD's blocks share one dynamic pool (C2
+// traversal 5.59 ms per frame with 32 stripes, 4.93 with 16, 4.77 with 8).
+constexpr uint32_t kWorkStripes = 8;
 
 // Counter block (uint32 words).  Ray queues (C_Q0/1) alternate by step
 // parity; so do the shadow groups and entries (C_G0/1, C_E0/1), because a
