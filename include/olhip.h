@@ -237,6 +237,12 @@ typedef struct olhip_engine_config {
   int32_t lds_stack;      /* traversal stack entries in LDS; 0 = context setting */
   int32_t trace_block;    /* traversal block size; 0 = context setting */
   int32_t use_graph;      /* replay each step as a captured hipGraph (1) or launch directly (0) */
+  /* Failure-path testing: most continuations and shadow groups one queue
+   * segment accepts per step (0 = the segment's size, which cannot
+   * overflow).  A smaller value makes shading drop paths; the jobs resident
+   * in that step then fail with "ray queue overflow". */
+  uint32_t append_limit;
+  uint32_t pad;
 } olhip_engine_config;
 
 /* Named products beside the color image (render_wavefront_outputs,

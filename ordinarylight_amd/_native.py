@@ -100,7 +100,8 @@ class Stats(ctypes.Structure):
 class EngineConfig(ctypes.Structure):
     _fields_ = [
         ("path_slots", ctypes.c_uint64), ("entry_factor", c_u32), ("lds_stack", ctypes.c_int32),
-        ("trace_block", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+        ("trace_block", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("append_limit", c_u32),
+        ("pad", c_u32),
     ]
 
 

@@ -46,6 +46,9 @@ class EngineSettings:
     lds_stack: int = 16          # traversal stack entries kept in LDS
     trace_block: int = 128       # traversal workgroup size
     use_graph: bool = True       # replay each step as a captured hipGraph
+    # Failure-path testing only: continuations a queue segment accepts per
+    # step (0 = its full size, which cannot overflow).
+    append_limit: int = 0
 
     def native(self):
         c = N.EngineConfig()
@@ -54,6 +57,7 @@ class EngineSettings:
         c.lds_stack = int(self.lds_stack)
         c.trace_block = int(self.trace_block)
         c.use_graph = int(bool(self.use_graph))
+        c.append_limit = int(self.append_limit)
         return c
 
 
@@ -215,6 +219,8 @@ def plain_render(job: Job, copy_image=True):
                                 int(job.height), int(job.samples), int(job.frame_index),
                                 N.ctypes.byref(settings), N.ptr(out) if out is not None else None,
                                 N.ctypes.byref(stats)))
+    if stats.queue_overflow:
+        raise RuntimeError(f"ray queue overflow: {stats.queue_overflow} continuations or shadow groups dropped")
     return out, stats.as_dict()
 
 

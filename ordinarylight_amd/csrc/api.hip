@@ -729,6 +729,8 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
     d.products = ((depth || normal || primitive) ? OLHIP_PRODUCT_PRIMARY : 0u) | (variance ? OLHIP_PRODUCT_VARIANCE : 0u);
     unsigned long long work0[4] = {0, 0, 0, 0};
     if (ctx->counting) eng.read_work(work0);
+    olhip_engine_stats es0{};
+    eng.engine_stats(&es0);
     Event e_start, e_end;
     OL_HIP_CHECK(hipEventRecord(e_start.e, ctx->stream));
     const uint64_t id = eng.submit(d);
@@ -757,7 +759,7 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
       o.shadow_rays = js.shadow_rays;
       olhip_engine_stats es{};
       eng.engine_stats(&es);
-      o.queue_overflow = es.queue_overflow;
+      o.queue_overflow = es.queue_overflow - es0.queue_overflow;
       o.total_ms = elapsed(e_start, e_end);
       o.trace_ms = eng.trace_ms;
       o.shade_ms = eng.shade_ms;

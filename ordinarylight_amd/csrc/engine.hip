@@ -340,6 +340,7 @@ bool Engine::admit(Job& j, int slot) {
       J.seg = seg;
       J.ebase = (uint32_t)ebase;
       J.eseg = seg * j.per_hit;
+      J.append = cfg_.append_limit ? std::min<uint32_t>(seg, cfg_.append_limit) : seg;
       if (!j.buffers) {
         take_buffer(j.acc, sizeof(double4) * np);
         take_buffer(j.image, sizeof(float4) * np);
@@ -691,7 +692,7 @@ void Engine::retire_front() {
     const double t = now_ms();
     for (uint64_t id : f.finals) {
       auto it = jobs_.find(id);
-      if (it == jobs_.end()) continue;
+      if (it == jobs_.end() || it->second->status == OLHIP_JOB_FAILED) continue;
       it->second->status = OLHIP_JOB_DONE;
       it->second->stats.done_ms = t;
       ++totals_.jobs_done;
@@ -700,20 +701,24 @@ void Engine::retire_front() {
       ++steps_;
       const uint32_t* report = report_host_[f.buf];
       const uint32_t* rep = report + (C_REPORT - C_OVERFLOW);
-      // The overflow flag is sticky until reset; a reset queued after an
+      // The overflow flags are sticky until reset; a reset queued after an
       // overflow follows the steps already in flight, whose reports still
       // show it.  Those steps' jobs were failed with the first report.
-      const bool stack_overflow = report[C_STACK - C_OVERFLOW] != 0 && f.seq >= overflow_checked_from_;
-      totals_.queue_overflow = report[0];
+      const bool fresh = f.seq >= overflow_checked_from_;
+      const bool stack_overflow = report[C_STACK - C_OVERFLOW] != 0 && fresh;
+      const bool queue_overflow = report[0] != 0 && fresh;
+      if (queue_overflow) totals_.queue_overflow += report[0];
       for (uint32_t k = 0; k < kMaxJobs; ++k) {
         if (slot_job_[k] == 0) continue;
         Job& j = *jobs_[slot_job_[k]];
         Chunk& c = *chunk_in_slot(j, (int)k);
-        // A traversal stack overflow fails every resident chunk's job,
+        // A traversal stack overflow, or a shading step that had to drop
+        // paths or shadow rays for want of queue space, fails every resident
+        // chunk's job (the dropped work is not attributed to a job),
         // including chunks of the step queued behind this one.
-        if (stack_overflow) {
+        if (stack_overflow || queue_overflow) {
           j.status = OLHIP_JOB_FAILED;
-          j.error = "traversal stack overflow";
+          j.error = stack_overflow ? "traversal stack overflow" : "ray queue overflow: shading dropped paths";
           drop_chunks(j);
           continue;
         }
@@ -737,14 +742,20 @@ void Engine::retire_front() {
           if (!j.stagger && j.next_sample < j.desc.samples) queue_.push_front(j.id);
         }
       }
-      if (stack_overflow) {
+      if (stack_overflow || queue_overflow) {
         reset_overflow = true;
         overflow_checked_from_ = seq_;
       }
     }
   }
   if (reset_overflow) {
-    OL_HIP_CHECK(hipMemsetAsync(ctr_.as<uint32_t>() + C_STACK, 0, 4, ctx_->stream));
+    // Every resident job has failed.  Once the steps in flight have run, the
+    // counter block returns to its initial state: the failed jobs' queue and
+    // group counts (which a job admitted later into the same slot would
+    // otherwise read as its own) and both flags.
+    OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    wave_clear(wave_, ctx_->stream);
+    OL_HIP_CHECK(hipGetLastError());
     OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
   }
 }

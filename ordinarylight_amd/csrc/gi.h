@@ -93,7 +93,8 @@ struct JobDev {
   uint32_t seg;      // slots per segment: segment s holds slots base + s * seg ...
   uint32_t ebase;    // first shadow-entry slot
   uint32_t eseg;     // shadow-entry slots per segment
-  uint32_t pad[2];
+  uint32_t append;   // continuations / shadow groups a segment accepts per step (<= seg)
+  uint32_t pad;
   // Primary-hit products of sample 0 (null unless requested): per pixel
   // (t or +inf, geometric normal) and the original triangle index (or
   // OL_NONE) (wavefront_intersect.comp:317-356, targets/vulkan/core.py:4500-4536).

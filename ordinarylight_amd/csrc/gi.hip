@@ -1286,7 +1286,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     // Continuation slot in the job's output segment (one atomic per wave).
     const uint32_t cl = wave_reserve(&W.ctr[c_queue(out) + run_word(run)], cont);
     if (cont) {
-      if (cl < J.seg) {
+      if (cl < J.append) {
         const uint32_t cslot = J.base + so * J.seg + cl;
         W.qo[out][cslot] = co;
         W.qd[out][cslot] = cd;
@@ -1301,7 +1301,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const uint32_t gl = wave_reserve(&W.ctr[c_groups(in) + run_word(run)], has_group);
     const uint32_t el = wave_reserve_n(&W.ctr[c_entries(in) + run_word(run)], has_group ? nee_count : 0u);
     if (has_group) {
-      if (gl < J.seg && el + nee_count <= J.eseg) {
+      if (gl < J.append && el + nee_count <= J.eseg) {
         const uint32_t gslot = J.base + so * J.seg + gl, eslot = J.ebase + so * J.eseg + el;
         W.gT[gslot] = mk4(nee_T, nee_p);
         W.gI[gslot] = make_uint4(path, eslot, nee_count,
@@ -1352,8 +1352,19 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
   __syncthreads();
   for (uint32_t r = t; r < kRuns; r += blockDim.x) {
     const uint32_t job = r / kJobStripes;
-    const uint32_t a = W.ctr[c_queue(in) + run_word(r)], e = W.ctr[c_entries(in ^ 1u) + run_word(r)],
-                   l = W.ctr[c_queue(in ^ 1u) + run_word(r)], g = W.ctr[c_groups(in) + run_word(r)];
+    const uint32_t a = W.ctr[c_queue(in) + run_word(r)], e = W.ctr[c_entries(in ^ 1u) + run_word(r)];
+    uint32_t l = W.ctr[c_queue(in ^ 1u) + run_word(r)], g = W.ctr[c_groups(in) + run_word(r)];
+    // A reservation past the segment's capacity was counted but not written
+    // (shade flagged the overflow): the next step enumerates written entries only.
+    const uint32_t cap = W.jobs[job].append;
+    if (l > cap) {
+      l = cap;
+      W.ctr[c_queue(in ^ 1u) + run_word(r)] = cap;
+    }
+    if (g > cap) {
+      g = cap;
+      W.ctr[c_groups(in) + run_word(r)] = cap;
+    }
     if (a) atomicAdd(&rep[4 * job + 0], a);
     if (e) atomicAdd(&rep[4 * job + 1], e);
     if (l + g) atomicAdd(&rep[4 * job + 2], l + g);
@@ -1398,6 +1409,12 @@ void check_spill(const GiLaunch& l, uint64_t threads) {
     throw OlError{-2, "traversal spill buffer too small for the launch"};
 }
 
+// Block b of a refilling traversal launch serves work stripe b % kWorkStripes;
+// a grid of fewer blocks would leave whole stripes untraced.
+void check_stripes(dim3 grid) {
+  if (grid.x < kWorkStripes) throw OlError{-2, "traversal grid smaller than the number of work stripes"};
+}
+
 template <int LSTACK, int BLOCK>
 uint64_t spill_words_for(const GiLaunch& l) {
   int occ = std::max(std::max(resident_blocks(k_trace<LSTACK, BLOCK>, BLOCK), resident_blocks(k_trace2<LSTACK, BLOCK>, BLOCK)),
@@ -1410,6 +1427,7 @@ template <int LSTACK, int BLOCK>
 void launch_trace(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
   static int occ = resident_blocks(k_trace<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
+  check_stripes(grid);
   check_spill(l, (uint64_t)grid.x * BLOCK);
   hipLaunchKernelGGL((k_trace<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }
@@ -1418,6 +1436,7 @@ template <int LSTACK, int BLOCK>
 void launch_trace2(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
   static int occ = resident_blocks(k_trace2<LSTACK, BLOCK>, BLOCK);
   dim3 grid((unsigned)(l.cu_count * occ));
+  check_stripes(grid);
   check_spill(l, (uint64_t)grid.x * BLOCK);
   hipLaunchKernelGGL((k_trace2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }

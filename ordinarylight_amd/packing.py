@@ -75,24 +75,33 @@ def pack_scene(scene):
         indices.append(tri + np.uint32(offset))
         offset += len(world)
         material = mesh.material
+        if any(getattr(material, name, None) is not None for name in _TEXTURE_FIELDS):
+            textured = True
+        faces = getattr(mesh, "face_materials", None)
+        if faces is not None:
+            # One record per triangle, appended in triangle order.
+            block = faces.records(material)
+            block[:, 5, 3] = 1.0
+            first = sum(len(m) for m in materials)
+            materials.append(block)
+            material_ids.append(np.arange(first, first + len(tri), dtype=np.uint32))
+            continue
         key = id(material)
         if key not in material_index:
-            material_index[key] = len(materials)
-            if any(getattr(material, name, None) is not None for name in _TEXTURE_FIELDS):
-                textured = True
+            material_index[key] = sum(len(m) for m in materials)
             record = material_record(material)
             # Resolved occlusion factor of an untextured material: the value
             # the reference's textured path computes when no occlusion map is
             # bound (mix(1, 1, strength) = 1).
             record[5, 3] = 1.0
-            materials.append(record)
+            materials.append(record[None])
         material_ids.append(np.full(len(tri), material_index[key], np.uint32))
     if not materials:
         record = material_record(type("Default", (), {
             "base_color": (0.8, 0.8, 0.8), "emission": (0.0, 0.0, 0.0),
             "roughness": 1.0, "metallic": 0.0})())
         record[5, 3] = 1.0
-        materials.append(record)
+        materials.append(record[None])
     cat = lambda parts, shape, dtype: (np.ascontiguousarray(np.concatenate(parts), dtype)  # noqa: E731
                                        if parts else np.zeros(shape, dtype))
     analytic_count = int(getattr(scene, "analytic_light_count", 0))
@@ -102,7 +111,7 @@ def pack_scene(scene):
         normals=cat(normals, (0, 3), np.float32),
         indices=cat(indices, (0, 3), np.uint32),
         material_ids=cat(material_ids, (0,), np.uint32),
-        materials=np.ascontiguousarray(np.stack(materials), np.float32),
+        materials=np.ascontiguousarray(np.concatenate(materials), np.float32),
         analytic_lights=np.ascontiguousarray(scene.analytic_light_data(), np.float32),
         analytic_light_count=analytic_count,
         area_lights=np.ascontiguousarray(scene.emissive_triangle_data(), np.float32),
@@ -175,6 +184,8 @@ def pack_instanced_scene(scene):
     inst_res, inst_mat, inst_off, inst_id, mats = [], [], [], [], []
     offset = 0
     for mesh in meshes:
+        if getattr(mesh, "face_materials", None) is not None:
+            raise ValueError("per-triangle materials need a flat scene (instanced=False)")
         resource = mesh.resource
         if id(resource) not in res_index:
             res_index[id(resource)] = len(resources)
@@ -247,7 +258,7 @@ def pack_instanced_scene(scene):
         determinant=np.ascontiguousarray(det, np.float32),
         world_bounds=np.ascontiguousarray(np.concatenate((wmin, wmax), 1), np.float32),
         world_spheres=np.ascontiguousarray(spheres, np.float32),
-        materials=np.ascontiguousarray(np.stack(materials), np.float32),
+        materials=np.ascontiguousarray(np.concatenate(materials), np.float32),
         analytic_lights=np.ascontiguousarray(scene.analytic_light_data(), np.float32),
         analytic_light_count=analytic_count,
         area_lights=np.ascontiguousarray(scene.emissive_triangle_data(), np.float32),

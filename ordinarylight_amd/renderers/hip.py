@@ -510,6 +510,10 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
                 *(N.ptr(products[k], N.ctypes.c_uint32 if k == "primitive" else N.ctypes.c_float)
                   if k in products else None for k in ("depth", "normal", "primitive", "variance")),
                 N.ctypes.byref(stats)))
+        if stats.queue_overflow:
+            # The engine fails such a job itself; a render never returns an
+            # image from which paths or shadow rays were dropped.
+            raise RuntimeError(f"ray queue overflow: {stats.queue_overflow} continuations or shadow groups dropped")
         total_ms = (time.perf_counter() - started) * 1000.0
         rays = stats.primary_rays + stats.extension_rays + stats.shadow_rays
         self.last_stats = stats
@@ -534,6 +538,8 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         unsupported = set(outputs) - set(self.available_outputs)
         if not outputs or unsupported:
             raise ValueError(f"unsupported render outputs: {tuple(sorted(unsupported))}")
+        if len(set(outputs)) != len(outputs):
+            raise ValueError(f"render outputs listed twice: {outputs}")
         width, height = _check_size(width, height)
         ids = {"instance_id", "object_id", "material_id"} & set(outputs)
         buffers = {}

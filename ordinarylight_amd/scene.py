@@ -192,6 +192,43 @@ def material_record(material, program_id=0):
     ), dtype=np.float32)
 
 
+@dataclass(frozen=True, eq=False)
+class FaceMaterials:
+    """Per-triangle GGX parameters of one mesh (an extension of the
+    reference's one-material-per-mesh model, for workloads that draw a
+    material per triangle): triangle k is shaded with the mesh material's
+    record in which base colour, roughness and metallic factor are replaced
+    by row k.  Emission and every other field stay the mesh material's."""
+
+    base_color: np.ndarray   # float32 [T, 3] in [0, 1]
+    roughness: np.ndarray    # float32 [T] in [0, 1]
+    metallic: np.ndarray     # float32 [T] in [0, 1]
+
+    def __post_init__(self):
+        base = np.ascontiguousarray(self.base_color, np.float32)
+        rough = np.ascontiguousarray(self.roughness, np.float32).reshape(-1)
+        metal = np.ascontiguousarray(self.metallic, np.float32).reshape(-1)
+        if base.ndim != 2 or base.shape[1] != 3 or not len(base) == len(rough) == len(metal):
+            raise ValueError("face materials need base_color [T, 3], roughness [T] and metallic [T]")
+        for name, a in (("base_color", base), ("roughness", rough), ("metallic", metal)):
+            if not np.all(np.isfinite(a)) or a.min(initial=0.0) < 0.0 or a.max(initial=0.0) > 1.0:
+                raise ValueError(f"face {name} values must lie in [0, 1]")
+        object.__setattr__(self, "base_color", base)
+        object.__setattr__(self, "roughness", rough)
+        object.__setattr__(self, "metallic", metal)
+
+    def __len__(self):
+        return len(self.roughness)
+
+    def records(self, material):
+        """float32 [T, 12, 4] material records over ``material``'s record."""
+        out = np.repeat(material_record(material)[None], len(self), axis=0)
+        out[:, 0, :3] = self.base_color
+        out[:, 0, 3] = self.roughness
+        out[:, 1, 3] = self.metallic
+        return out
+
+
 def generate_normals(vertices, indices):
     """Area-weighted vertex normals, welded by identical position.
 
@@ -263,11 +300,14 @@ class Mesh:
     visible: bool = True
     name: str | None = None
     metadata: dict = field(default_factory=dict, repr=False)
+    face_materials: FaceMaterials | None = field(default=None, repr=False)
     id: int | None = field(default=None, init=False)
 
     def __post_init__(self):
         if not isinstance(self.transform, Transform):
             self.transform = Transform(self.transform)
+        if self.face_materials is not None and len(self.face_materials) != len(self.resource.indices):
+            raise ValueError("face_materials must hold one row per triangle of the mesh")
         if not isinstance(self.visible, bool):
             raise TypeError("visible must be a bool")
         self.metadata = MappingProxyType(dict(self.metadata))
@@ -388,13 +428,14 @@ class Scene:
     create_mesh_resource = create_mesh
 
     def add_instance(self, resource, *, transform=None, material=None,
-                     visible=True, name=None, metadata=None):
+                     visible=True, name=None, metadata=None, face_materials=None):
         """Place one independently shaded occurrence of ``resource``."""
         if resource not in self.mesh_resources:
             raise KeyError("mesh resource is not part of this scene")
         mesh = Mesh(resource, resource.material if material is None else material,
                     Transform() if transform is None else transform,
-                    visible=visible, name=name, metadata=metadata or {})
+                    visible=visible, name=name, metadata=metadata or {},
+                    face_materials=face_materials)
         return self._attach_mesh(mesh)
 
     def add_instances(self, resource, transforms, *, materials=None, visible=True):
@@ -410,11 +451,11 @@ class Scene:
         )
 
     def add_mesh(self, vertices, indices, material=None, *, normals=None,
-                 transform=None, name=None, metadata=None):
+                 transform=None, name=None, metadata=None, face_materials=None):
         resource = self.create_mesh(vertices, indices, material,
                                     normals=normals, name=name, metadata=metadata)
         return self.add_instance(resource, transform=transform, name=name,
-                                 metadata=metadata)
+                                 metadata=metadata, face_materials=face_materials)
 
     def add_light(self, light):
         if not isinstance(light, LIGHT_TYPES):

@@ -1,49 +1,65 @@
-"""Picking types: the options, viewport mapping and result of
-``ordinarylight/selection/_core.py`` (``PickOptions`` :15-37,
-``ViewportMapping`` :40-86, ``PickResult`` :89-99), same fields and checks."""
+"""Picking records of the reference's selection API.
+
+The field names and meanings are the interface ``HipPathTracer.pick``
+answers to (``ordinarylight/selection/_core.py``: ``PickOptions`` :15-37,
+``ViewportMapping`` :40-86, ``PickResult`` :89-99).  A viewport pixel maps
+through the displayed content rectangle onto the framebuffer or render
+grid; everything else here is validation of those records.
+"""
 
 from __future__ import annotations
 
 from dataclasses import dataclass
 import math
 
-import numpy as np
+_TRANSMISSIVE = ("surface", "through")
+_VOLUMES = ("include", "exclude", "only")
 
 
-def _size(value, name):
-    if len(value) != 2:
-        raise ValueError(f"{name} must contain width and height")
-    out = tuple(int(v) for v in value)
-    if min(out) < 1:
-        raise ValueError(f"{name} dimensions must be positive")
+def _two_positive_ints(value, what):
+    pair = tuple(value)
+    if len(pair) != 2:
+        raise ValueError(f"{what}: expected (width, height), got {len(pair)} values")
+    pair = tuple(int(v) for v in pair)
+    if pair[0] < 1 or pair[1] < 1:
+        raise ValueError(f"{what}: width and height must be at least 1, got {pair}")
+    return pair
+
+
+def _finite(values, what):
+    out = tuple(float(v) for v in values)
+    if not all(math.isfinite(v) for v in out):
+        raise ValueError(f"{what}: values must be finite, got {out}")
     return out
 
 
 @dataclass(frozen=True)
 class PickOptions:
-    """Which intersections may be selected: ``transmissive`` "surface" or
-    "through", the transmission threshold, and ``volumes`` "include",
-    "exclude" or "only"."""
+    """Which surfaces a pick may return: glass like any surface
+    (``transmissive="surface"``) or skipped above ``transmission_threshold``
+    (``"through"``); volume proxies included, excluded or alone."""
 
     transmissive: str = "surface"
     transmission_threshold: float = 0.001
     volumes: str = "include"
 
     def __post_init__(self):
-        if self.transmissive not in ("surface", "through"):
-            raise ValueError("transmissive must be 'surface' or 'through'")
-        threshold = float(self.transmission_threshold)
-        if not math.isfinite(threshold) or not 0.0 <= threshold <= 1.0:
-            raise ValueError("transmission_threshold must be in [0, 1]")
-        if self.volumes not in ("include", "exclude", "only"):
-            raise ValueError("volumes must be 'include', 'exclude', or 'only'")
+        if self.transmissive not in _TRANSMISSIVE:
+            raise ValueError(f"transmissive: one of {_TRANSMISSIVE}, got {self.transmissive!r}")
+        if self.volumes not in _VOLUMES:
+            raise ValueError(f"volumes: one of {_VOLUMES}, got {self.volumes!r}")
+        (threshold,) = _finite((self.transmission_threshold,), "transmission_threshold")
+        if threshold < 0.0 or threshold > 1.0:
+            raise ValueError(f"transmission_threshold: expected a value in [0, 1], got {threshold}")
         object.__setattr__(self, "transmission_threshold", threshold)
 
 
 @dataclass(frozen=True)
 class ViewportMapping:
-    """UI pixel -> render pixel through a displayed content rectangle
-    ``(x, y, width, height)``, a framebuffer scale and a render size."""
+    """A UI viewport showing the render inside ``content_rect`` (x, y, width,
+    height in viewport pixels, e.g. letterboxed); the framebuffer may be
+    scaled for high DPI and the render grid may differ again.  Omitted sizes
+    default to the previous one in that chain."""
 
     viewport_size: tuple
     framebuffer_size: tuple | None = None
@@ -51,36 +67,39 @@ class ViewportMapping:
     content_rect: tuple | None = None
 
     def __post_init__(self):
-        viewport = _size(self.viewport_size, "viewport_size")
-        framebuffer = _size(self.framebuffer_size or viewport, "framebuffer_size")
-        render = _size(self.render_size or framebuffer, "render_size")
-        rect = tuple(float(v) for v in (self.content_rect or (0.0, 0.0, float(viewport[0]), float(viewport[1]))))
-        if len(rect) != 4 or not np.isfinite(rect).all():
-            raise ValueError("content_rect must contain four finite values")
-        if rect[2] <= 0.0 or rect[3] <= 0.0:
-            raise ValueError("content_rect dimensions must be positive")
-        object.__setattr__(self, "viewport_size", viewport)
-        object.__setattr__(self, "framebuffer_size", framebuffer)
-        object.__setattr__(self, "render_size", render)
-        object.__setattr__(self, "content_rect", rect)
+        viewport = _two_positive_ints(self.viewport_size, "viewport_size")
+        framebuffer = _two_positive_ints(self.framebuffer_size or viewport, "framebuffer_size")
+        render = _two_positive_ints(self.render_size or framebuffer, "render_size")
+        rect = self.content_rect if self.content_rect is not None else (0, 0, viewport[0], viewport[1])
+        rect = _finite(rect, "content_rect")
+        if len(rect) != 4:
+            raise ValueError(f"content_rect: expected (x, y, width, height), got {len(rect)} values")
+        if min(rect[2], rect[3]) <= 0.0:
+            raise ValueError(f"content_rect: width and height must be positive, got {rect[2:]}")
+        for name, value in (("viewport_size", viewport), ("framebuffer_size", framebuffer),
+                            ("render_size", render), ("content_rect", rect)):
+            object.__setattr__(self, name, value)
 
     def map_pixel(self, pixel, *, target="render"):
-        """The mapped pixel, or None outside the displayed content."""
-        if target not in ("framebuffer", "render"):
-            raise ValueError("target must be 'framebuffer' or 'render'")
-        x, y = (float(v) for v in pixel)
-        if not np.isfinite((x, y)).all():
-            raise ValueError("pixel coordinates must be finite")
-        left, top, width, height = self.content_rect
-        if x < left or y < top or x >= left + width or y >= top + height:
+        """``pixel`` (viewport coordinates) on the ``target`` grid
+        ("render" or "framebuffer"), or None outside the content rectangle."""
+        grids = {"render": self.render_size, "framebuffer": self.framebuffer_size}
+        if target not in grids:
+            raise ValueError(f"target: one of {tuple(grids)}, got {target!r}")
+        px, py = _finite(pixel, "pixel")
+        x0, y0, w, h = self.content_rect
+        inside = x0 <= px < x0 + w and y0 <= py < y0 + h
+        if not inside:
             return None
-        size = self.framebuffer_size if target == "framebuffer" else self.render_size
-        return (x - left) / width * size[0], (y - top) / height * size[1]
+        gw, gh = grids[target]
+        return (px - x0) / w * gw, (py - y0) / h * gh
 
 
 @dataclass(frozen=True)
 class PickResult:
-    """The closest selectable object hit by a pixel's primary ray."""
+    """The closest selectable surface under a pixel: the scene object, its
+    id, the distance and position along the pixel's primary ray, the global
+    and per-object triangle index and the hit's barycentric weights."""
 
     object: object
     object_id: int

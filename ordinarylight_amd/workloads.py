@@ -7,10 +7,10 @@ workload definitions of this package, all deterministic in their seed:
   boxes with a seeded rotation about +Y (24), one emissive ceiling quad (2)
   of radiance 15.  Albedos are uniform in [0.1, 0.9]^3.  Pinhole camera.
 * :func:`terrain` -- fBm-displaced heightfield of ``(nx-1)*(nz-1)*2``
-  triangles (1001 x 501 vertices give 1,000,000), split into chunk
-  meshes that each carry one GGX material (roughness ~ U[0.05, 1],
-  metallic ~ Bernoulli(0.3)), plus 64 emissive triangles and a constant
-  environment of radiance 0.5.
+  triangles (1001 x 501 vertices give 1,000,000) in chunk meshes, with a
+  GGX material drawn per triangle (albedo ~ U[0.1, 0.9]^3, roughness ~
+  U[0.05, 1], metallic ~ Bernoulli(0.3)), plus 64 emissive triangles and a
+  constant environment of radiance 0.5.
 * :func:`tessellated_room`, :func:`mixed_scene`, :func:`mixed_jobs` -- the
   batched/served workload: eight scenes of 10k to 2M triangles from the two
   generators and a seeded list of jobs over them.
@@ -24,7 +24,7 @@ import numpy as np
 
 from .cameras import PerspectiveCamera
 from .lights import EnvironmentLight
-from .scene import Material, Scene, Transform
+from .scene import FaceMaterials, Material, Scene, Transform
 
 
 def _quad(p0, p1, p2, p3):
@@ -111,8 +111,14 @@ def _fbm(x, z, rng, octaves=6):
     return height
 
 
-def terrain(seed=0, nx=1001, nz=501, chunk=50, emissive_quads=32, height_scale=60.0):
-    """fBm heightfield terrain with chunked GGX materials, emitters and sky."""
+def terrain(seed=0, nx=1001, nz=501, chunk=50, emissive_quads=32, height_scale=60.0, materials="triangle"):
+    """fBm heightfield terrain with GGX materials, emitters and sky.
+
+    ``materials="triangle"`` draws every triangle's base colour (U[0.1,0.9]^3),
+    roughness (U[0.05,1]) and metallic factor (Bernoulli(0.3)) on its own;
+    ``"chunk"`` draws one material per chunk mesh."""
+    if materials not in ("triangle", "chunk"):
+        raise ValueError("materials must be 'triangle' or 'chunk'")
     rng = np.random.default_rng(seed)
     xs = np.arange(nx, dtype=np.float64)
     zs = np.arange(nz, dtype=np.float64)
@@ -134,6 +140,12 @@ def terrain(seed=0, nx=1001, nz=501, chunk=50, emissive_quads=32, height_scale=6
     normals = normals.reshape(nx, nz, 3)
 
     scene = Scene()
+    n_tri = (nx - 1) * (nz - 1) * 2
+    if materials == "triangle":
+        face_base = rng.uniform(0.1, 0.9, (n_tri, 3)).astype(np.float32)
+        face_rough = rng.uniform(0.05, 1.0, n_tri).astype(np.float32)
+        face_metal = (rng.random(n_tri) < 0.3).astype(np.float32)
+    first = 0
     for x0 in range(0, nx - 1, chunk):
         for z0 in range(0, nz - 1, chunk):
             x1, z1 = min(x0 + chunk, nx - 1), min(z0 + chunk, nz - 1)
@@ -147,6 +159,12 @@ def terrain(seed=0, nx=1001, nz=501, chunk=50, emissive_quads=32, height_scale=6
             indices = np.empty((len(quads) * 2, 3), np.uint32)
             indices[0::2] = quads[:, (0, 1, 2)]
             indices[1::2] = quads[:, (0, 2, 3)]
+            if materials == "triangle":
+                rows = slice(first, first + len(indices))
+                first += len(indices)
+                scene.add_mesh(verts, indices, Material(), normals=norms,
+                               face_materials=FaceMaterials(face_base[rows], face_rough[rows], face_metal[rows]))
+                continue
             material = Material(
                 base_color=tuple(float(v) for v in rng.uniform(0.1, 0.9, 3)),
                 roughness=float(rng.uniform(0.05, 1.0)),

@@ -14,6 +14,7 @@ import time
 import numpy as np
 import pytest
 
+from ordinarylight_amd import _native as N
 from ordinarylight_amd import workloads
 from ordinarylight_amd.batch import BatchEngine, EngineSettings, Job, RenderServer, plain_render, run_batch
 from ordinarylight_amd.renderers.hip import DeviceContext, DeviceScene, HipRendererConfig
@@ -98,6 +99,7 @@ def test_batch_mix_equals_alone(ctx, scenes, slots, graph):
         assert np.array_equal(img, r), f"job {k} differs from its single-frame render"
         assert js["primary_rays"] == jobs[k].width * jobs[k].height * jobs[k].samples
     assert stats["jobs_done"] == len(jobs)
+    assert stats["queue_overflow"] == 0
     if slots == 16384:
         assert max(js["chunks"] for _, js in results) > 1
     else:
@@ -117,6 +119,7 @@ def test_jobs_join_mid_flight(ctx, scenes):
         stats = eng.stats()
         results = [eng.result(i) for i in first + later]
     assert stats["max_resident_jobs"] >= 4
+    assert stats["queue_overflow"] == 0
     for k, ((img, _), r) in enumerate(zip(results, ref)):
         assert np.array_equal(img, r), f"job {k} differs"
 
@@ -169,3 +172,27 @@ def test_batch_products_equal_plain(ctx, scenes):
             single.result(k)
         for name in ("depth", "normal", "primitive", "variance"):
             assert np.array_equal(g[name], ref[name]), name
+
+
+def test_forced_queue_overflow_fails_the_job_and_the_engine_recovers(ctx, scenes):
+    """A queue segment held (from the host) below the no-overflow bound makes
+    shading drop continuations: the job fails with the reason instead of
+    returning a biased image, and a job admitted afterwards into the same
+    slots renders exactly as alone (the failed job's queue counts are gone)."""
+    ds, cam = scenes["terrain"]
+    cfg = HipRendererConfig(max_bounces=4, samples_per_pixel=1)
+    big = Job(ds, cam, 96, 64, 2, 0, cfg)      # 12,288 paths in one segment
+    small = Job(ds, cam, 16, 16, 1, 0, cfg)    # 256 paths: at most 256 continuations
+    ref = plain_render(small)[0]
+    log("forced overflow: append_limit=256")
+    with BatchEngine(ctx, EngineSettings(path_slots=1 << 20, append_limit=256)) as eng:
+        k = eng.submit(big)
+        eng.run()
+        assert eng.poll(k) == N.JOB_FAILED
+        with pytest.raises(RuntimeError, match="ray queue overflow"):
+            eng.result(k)
+        assert eng.stats()["queue_overflow"] > 0
+        k2 = eng.submit(small)
+        eng.run()
+        img, _ = eng.result(k2)
+    assert np.array_equal(img, ref)

@@ -4,20 +4,20 @@
   (tests/test_renderer.py:149-171 of the reference).
 * Every symbol declared in include/olhip.h is exported by libolhip.so and
   bound by ordinarylight_amd._native (no GPU call is made).
-* Renderer front end: frame-index reservation, out= reuse, reset_sequence,
-  replace_scene, reconfigure and validation (the reference's
-  tests/test_high_level_renderer.py:317-393), against a fake backend.
+* The implementation contract the reference's own ``Renderer`` relies on:
+  call signatures, capabilities, validation and close semantics (the
+  frame-index sequence itself is the caller's; its effect on the image is
+  a GPU test).
 """
 
 import ctypes
+import inspect
 import pathlib
 import re
-import threading
 
 import numpy as np
 import pytest
 
-import ordinarylight_amd as ol
 from ordinarylight_amd import cameras, packing, workloads
 from ordinarylight_amd.scene import Material, Scene
 
@@ -84,8 +84,18 @@ def test_terrain_workload_shape():
     assert scene.triangle_count == 100 * 50 * 2 + 64
     assert scene.emissive_triangle_count == 64
     assert scene.environment.intensity == 0.5
-    roughness = [m.material.roughness for m in scene.meshes[:-1]]
-    assert min(roughness) >= 0.05 and max(roughness) <= 1.0
+    faces = [m.face_materials for m in scene.meshes[:-1]]
+    rough = np.concatenate([f.roughness for f in faces])
+    metal = np.concatenate([f.metallic for f in faces])
+    assert len(rough) == 100 * 50 * 2
+    assert rough.min() >= 0.05 and rough.max() <= 1.0 and set(np.unique(metal)) == {0.0, 1.0}
+    p = packing.pack_scene(scene)
+    # One material record per terrain triangle (+ the emitters' material).
+    assert len(p.materials) == 100 * 50 * 2 + 1
+    np.testing.assert_array_equal(p.materials[p.material_ids[:len(rough)], 0, 3], rough)
+    np.testing.assert_array_equal(p.materials[p.material_ids[:len(rough)], 1, 3], metal)
+    chunked, _ = workloads.terrain(0, nx=101, nz=51, chunk=25, emissive_quads=32, materials="chunk")
+    assert len(packing.pack_scene(chunked).materials) == 4 * 2 + 1
 
 
 def test_full_terrain_has_one_million_triangles():
@@ -128,138 +138,61 @@ def test_library_reports_errors_without_device():
 
 
 # -- renderer contract ------------------------------------------------------------------
-class FakeBackend:
-    available_outputs = ("color", "depth")
-
-    def __init__(self):
-        self.config = {"samples_per_pixel": 1}
-        self.device = "test-device"
-        self.last_timings = {}
-        self.calls = []
-        self.scene_replacements = []
-        self.setting_changes = []
-        self.closed = 0
-
-    def render_frame(self, scene, camera, width, height, *, samples=None, frame_index=0):
-        self.calls.append((scene, camera, width, height, samples, frame_index))
-        self.last_timings = {"gpu_ms": 2.5}
-        return np.full((height, width, 4), float(frame_index), np.float32)
-
-    def render_products(self, scene, camera, width, height, *, outputs, samples=None, frame_index=0):
-        out = {"color": self.render_frame(scene, camera, width, height, samples=samples, frame_index=frame_index)}
-        if "depth" in outputs:
-            out["depth"] = np.ones((height, width), np.float32)
-        return out
-
-    def replace_scene(self, scene):
-        self.scene_replacements.append(scene)
-
-    def reconfigure(self, **changes):
-        self.setting_changes.append(changes)
-        return changes
-
-    def close(self):
-        self.closed += 1
+def _keyword_only(fn):
+    return {n: p.default for n, p in inspect.signature(fn).parameters.items() if p.kind is p.KEYWORD_ONLY}
 
 
-class BlockingBackend(FakeBackend):
-    def __init__(self):
-        super().__init__()
-        self.release = threading.Event()
-        self.started = threading.Event()
+def test_protocol_signatures_match_the_reference_contract():
+    """The reference's Renderer calls implementations as
+    render_frame(scene, camera, width, height, *, samples, frame_index) and
+    render_products(..., *, outputs, samples, frame_index)
+    (renderer/_core.py:757-766, renderers/base.py:41-71)."""
+    from ordinarylight_amd.renderers.hip import HipPathTracer, HipReferenceRenderer
+    for cls in (HipPathTracer, HipReferenceRenderer):
+        params = list(inspect.signature(cls.render_frame).parameters)
+        assert params[:5] == ["self", "scene", "camera", "width", "height"], cls
+        kw = _keyword_only(cls.render_frame)
+        assert kw["samples"] is None and kw["frame_index"] == 0, cls
+    kw = _keyword_only(HipPathTracer.render_products)
+    assert set(kw) >= {"outputs", "samples", "frame_index"}
+    assert kw["samples"] is None and kw["frame_index"] == 0
 
-    def render_frame(self, *args, **kwargs):
-        self.started.set()
-        self.release.wait(5)
-        return super().render_frame(*args, **kwargs)
 
-
-def fixture():
-    return workloads.box_room(0)
-
-
-def test_renderer_advances_sequence_and_reports_statistics():
-    backend = FakeBackend()
-    r = ol.Renderer(implementation=backend)
-    scene, cam = fixture()
-    first = r.render(scene, cam, (5, 3), samples=2)
-    second = r.render(scene, cam, (5, 3))
-    assert first.shape == (3, 5, 4) and first.dtype == np.float32
-    assert np.all(first == 0.0) and np.all(second == 1.0)
-    assert backend.calls[0][2:] == (5, 3, 2, 0)
-    assert r.frame_index == 2 and r.device == "test-device"
-    assert r.last_timings == {"gpu_ms": 2.5}
-    assert r.last_statistics.as_dict() == {"frame_index": 1, "width": 5, "height": 3, "samples": None,
-                                           "total_ms": None, "gpu_ms": 2.5}
+def test_close_is_idempotent_and_final():
+    from ordinarylight_amd.renderers.hip import HipPathTracer
+    scene, cam = workloads.box_room(0)
+    r = HipPathTracer()      # no device is touched before the first render
     r.close()
-    assert backend.closed == 1
     r.close()
-    assert backend.closed == 1
+    for call in (lambda: r.render_frame(scene, cam, 4, 4), lambda: r.replace_scene(scene),
+                 lambda: r.render_products(scene, cam, 4, 4, outputs=("color",))):
+        with pytest.raises(RuntimeError):
+            call()
 
 
-def test_explicit_frame_and_caller_owned_output():
-    r = ol.Renderer(implementation=FakeBackend())
-    scene, cam = fixture()
-    out = np.empty((2, 4, 4), np.float32)
-    result = r.render(scene, cam, (4, 2), frame_index=7, out=out)
-    assert result is out and np.all(out == 7.0)
-    assert r.frame_index == 8
-    r.reset_sequence()
-    assert r.frame_index == 0
+def test_capabilities_are_a_plain_mapping():
+    """The reference normalises a dict of capabilities itself
+    (capabilities/_core.py:78-99): outputs and limits must be consistent."""
+    from ordinarylight_amd.renderers.hip import HipPathTracer, HipReferenceRenderer
+    for cls in (HipPathTracer, HipReferenceRenderer):
+        r = cls()
+        caps = r.capabilities
+        assert isinstance(caps, dict)
+        assert tuple(caps["outputs"]) == tuple(r.available_outputs)
+        assert caps["renderer"] == r.implementation.name
+        assert caps["limits"]["max_bounces"] == 64
+        r.close()
+
+
+def test_render_products_validates_outputs_before_rendering():
+    from ordinarylight_amd.renderers.hip import HipPathTracer
+    scene, cam = workloads.box_room(0)
+    r = HipPathTracer()
+    for bad in ((), ("color", "motion"), ("albedo",)):
+        with pytest.raises(ValueError):
+            r.render_products(scene, cam, 4, 4, outputs=bad)
     with pytest.raises(ValueError):
-        r.render(scene, cam, (4, 2), out=np.empty((3, 4, 4), np.float32))
-    with pytest.raises(TypeError):
-        r.render(scene, cam, (4, 2), out=np.empty((2, 4, 4), np.float64))
-    r.close()
-
-
-def test_replace_scene_and_reconfigure_are_ordered():
-    backend = FakeBackend()
-    r = ol.Renderer(implementation=backend)
-    a, cam = fixture()
-    b, _ = fixture()
-    r.render(a, cam, (2, 2))
-    assert r.replace_scene(b) is b
-    assert backend.scene_replacements == [b] and r.frame_index == 0 and r.last_statistics is None
-    assert r.reconfigure(samples_per_pixel=2, max_bounces=8) == {"samples_per_pixel": 2, "max_bounces": 8}
-    r.close()
-    with pytest.raises(RuntimeError):
-        r.render(a, cam, (2, 2))
-
-
-def test_frame_indices_are_reserved_at_submission_and_cancel_leaves_gaps():
-    backend = BlockingBackend()
-    r = ol.Renderer(implementation=backend)
-    scene, cam = fixture()
-    first = r.render_async(scene, cam, (2, 2))
-    backend.started.wait(5)
-    second = r.render_async(scene, cam, (2, 2))
-    third = r.render_async(scene, cam, (2, 2))
-    assert (first.frame_index, second.frame_index, third.frame_index) == (0, 1, 2)
-    assert second.cancel()
-    backend.release.set()
-    assert np.all(third.result() == 2.0)
-    assert second.cancelled() and r.frame_index == 3
-    assert [c[-1] for c in backend.calls] == [0, 2]
-    r.close()
-
-
-def test_named_products_and_validation():
-    r = ol.Renderer(implementation=FakeBackend())
-    scene, cam = fixture()
-    frame = r.render(scene, cam, (3, 2), outputs=("color", "depth"))
-    assert frame.color.shape == (2, 3, 4) and frame["depth"].shape == (2, 3)
-    assert frame.metadata["frame_index"] == 0
-    with pytest.raises(ValueError):
-        r.render(scene, cam, (3, 2), outputs=("normal",))
-    with pytest.raises(ValueError):
-        r.render(scene, cam, (3, 2), outputs=("color", "color"))
-    with pytest.raises(ValueError):
-        r.render(scene, cam, (0, 2))
-    with pytest.raises(TypeError):
-        r.render(scene, object(), (3, 2))
-    with pytest.raises(ValueError):
-        r.render(scene, cam, (3, 2), frame_index=-1)
+        r.render_products(scene, cam, 0, 4, outputs=("color",))
     r.close()
 
 
