@@ -44,6 +44,22 @@
 
 typedef OL_REAL R;
 
+/* Planted mistakes for the tests' mutation checks: a build with
+ * -DOL_MUTANT=k (oracle/Makefile, libol_oracle_m<k>.so) breaks exactly one
+ * term, so that the closed-form checks can show they catch it.  The default
+ * build (OL_MUTANT 0) has none.
+ *   1 refraction eta inverted          7 exit pops to air, not the outer medium
+ *   2 point light without 1/d^2        8 area-light pdf without d^2
+ *   3 point light without cos          9 environment NEE without its MIS weight
+ *   4 roulette survivors not 1/p      10 spot cone linear instead of smoothstep
+ *   5 Beer absorption on the wrong    11 emission hit without its MIS weight
+ *     segment                         12 GGX specular lobe without D
+ *   6 total internal reflection passes straight on */
+#ifndef OL_MUTANT
+#define OL_MUTANT 0
+#endif
+#define OL_MUT(k) (OL_MUTANT == (k))
+
 /* Float32 Moller-Trumbore accepts points a few ulps of the scene scale outside
  * the exact triangle; node boxes are padded by this fraction of the largest
  * coordinate so that the exact box test never culls such a hit. */
@@ -467,7 +483,7 @@ static V eval_pbr(const Mat* m, V n, V view, V out) {
   V f0 = f0_of(m);
   V fr = vadd(f0, vs(vsub(v3(1, 1, 1), f0), k));
   R er = m->rough * ((R)1 - (R)0.25 * absr(clampr(m->aniso, (R)-1, (R)1)));
-  R D = ggx_d(nh, er);
+  R D = OL_MUT(12) ? (R)1 : ggx_d(nh, er);
   R G = ggx_g(nv, er) * ggx_g(nl, er);
   R den = rmax(((R)4 * nv) * nl, (R)1e-6);
   V spec = vdiv(vs(vs(fr, D), G), den);
@@ -564,7 +580,7 @@ static int area_sample(const Scene* s, uint32_t* rs, V hit, V n, const Mat* m, V
   R lc = rmax(raw, (R)0);
   if (A[18] > 0.5f) lc = absr(raw);
   if (scos <= (R)0 || lc <= (R)1e-6) return 0;
-  R lpdf = ((R)A[17] * d2) / rmax(lc * (R)A[15], (R)1e-6);
+  R lpdf = ((R)A[17] * (OL_MUT(8) ? (R)1 : d2)) / rmax(lc * (R)A[15], (R)1e-6);
   R eff = lpdf * rmax(tp, (R)1e-6);
   if (occluded(s, vadd(hit, vs(n, (R)0.002)), dir, (R)0.001, rmax(dl - (R)0.004, (R)0.001))) return 0;
   V view = vneg(inc);
@@ -585,7 +601,7 @@ static int env_sample(const Scene* s, uint32_t* rs, V hit, V n, const Mat* m, V 
   R pdf = c / PI_R, eff = pdf * rmax(tp, (R)1e-6);
   if (!(c > (R)0)) return 0;
   if (occluded(s, vadd(hit, vs(n, (R)0.002)), dir, (R)0.001, (R)1e30)) return 0;
-  R mis = power_h(eff * (R)sc, c / PI_R);
+  R mis = OL_MUT(9) ? (R)1 : power_h(eff * (R)sc, c / PI_R);
   R mw = mixr(m->occl, (R)1, m->metal);
   *out = vs(vdiv(vs(vs(vs(vmul(eval_pbr(m, n, vneg(inc), dir), env_rad(s, dir)), c), mis), (R)1), rmax(eff, (R)1e-6)),
             mw);
@@ -606,11 +622,11 @@ static int point_sample(const Scene* s, uint32_t li, V hit, V n, const Mat* m, V
     dist = rsqrt_(d2);
     dir = vdiv(off, dist);
     if (L[7] > 0.0f && dist > (R)L[7]) return 0;
-    att = (R)1 / d2;
+    att = OL_MUT(2) ? (R)1 : (R)1 / d2;
     if (type == 2) {
       R cone = dot(norm(v3(L[4], L[5], L[6])), vneg(dir));
       R amount = clampr((cone - (R)L[13]) / rmax((R)L[12] - (R)L[13], (R)1e-6), (R)0, (R)1);
-      R spot = (amount * amount) * ((R)3 - (R)2 * amount);
+      R spot = OL_MUT(10) ? amount : (amount * amount) * ((R)3 - (R)2 * amount);
       if (spot <= (R)0) return 0;
       att = att * spot;
     }
@@ -621,7 +637,7 @@ static int point_sample(const Scene* s, uint32_t li, V hit, V n, const Mat* m, V
   if (type == 1) sd = 10000;
   if (occluded(s, vadd(hit, vs(n, (R)0.002)), dir, (R)0.001, sd)) return 0;
   V incident = vs(vs(v3(L[8], L[9], L[10]), (R)L[11]), att);
-  *out = vs(vmul(vmul(eval_pbr(m, n, vneg(inc), dir), incident), v3(1, 1, 1)), c);
+  *out = vs(vmul(vmul(eval_pbr(m, n, vneg(inc), dir), incident), v3(1, 1, 1)), OL_MUT(3) ? (R)1 : c);
   return 1;
 }
 
@@ -1054,7 +1070,7 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
         R lp = ((sel * t) * t) / rmax(lc * area, (R)1e-6);
         R sp = lp * (R)(st->sec_area > 1u ? st->sec_area : 1u);
         if (flags & 4u) sp = lp * unified_prob(s, st);
-        emis = power_h(prev_pdf, sp);
+        emis = OL_MUT(11) ? (R)1 : power_h(prev_pdf, sp);
       }
       Lr = vadd(Lr, vs(vmul(T, em), emis));
     }
@@ -1066,19 +1082,19 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
     if (trans > (R)0.001) {
       R cur = medium_get(stack, depth - 1u), target = 1;
       if (entering) target = rmax(m.ior, (R)1.0001);
-      else if (depth > 1u) target = medium_get(stack, depth - 2u);
-      R eta = cur / target, ndi = dot(n, inc);
+      else if (depth > 1u && !OL_MUT(7)) target = medium_get(stack, depth - 2u);
+      R eta = OL_MUT(1) ? target / cur : cur / target, ndi = dot(n, inc);
       R k = (R)1 - eta * eta * ((R)1 - ndi * ndi);
       V rd = k < (R)0 ? v3(0, 0, 0) : vsub(sv(eta, inc), sv(eta * ndi + rsqrt_(k), n));
       if (dot(rd, rd) < (R)0.01) {
-        rd = reflect_(inc, n);
+        rd = OL_MUT(6) ? inc : reflect_(inc, n);
       } else if (entering && depth < 16u) {
         stack[depth] = target;
         depth++;
       } else if (!entering && depth > 1u) {
         depth--;
       }
-      R optical = entering ? (R)0 : t;
+      R optical = (entering != OL_MUT(5)) ? (R)0 : t;
       if (m.thin > (R)0.5) optical = m.thick;
       R ex = optical / rmax(m.adist, (R)1e-6);
       V ab;
@@ -1175,7 +1191,7 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
       R surv = clampr(rmax(T.x, rmax(T.y, T.z)), (R)st->rr_min, (R)0.95);
       rs = pcg(rs);
       if (pcg_unit(rs) >= surv) break;
-      T = vdiv(T, surv);
+      if (!OL_MUT(4)) T = vdiv(T, surv);
     }
     d = norm(nd);
     bounce = nb;

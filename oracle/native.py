@@ -17,24 +17,26 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "libol_oracle.so"
 
-_lib = None
+_libs = {}
 P_f32 = ctypes.POINTER(ctypes.c_float)
 P_u32 = ctypes.POINTER(ctypes.c_uint32)
 P_u64 = ctypes.POINTER(ctypes.c_uint64)
 
 
 def build():
-    """Compile the oracle library (gcc, OpenMP)."""
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    """Compile the oracle library and its planted-mistake builds (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-j8", "-C", str(HERE)], check=True)
     return LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
+def lib(mutant=0):
+    """The oracle library; ``mutant=k`` loads the build with planted mistake k
+    (gi_oracle.c, OL_MUTANT), for the tests that show a check catches it."""
+    if mutant not in _libs:
+        path = LIB_PATH if mutant == 0 else HERE / "_build" / f"libol_oracle_m{int(mutant)}.so"
+        if not path.exists():
             build()
-        L = ctypes.CDLL(str(LIB_PATH))
+        L = ctypes.CDLL(str(path))
         for sfx in ("_f32", "_f64"):
             getattr(L, "olgi_scene_new" + sfx).restype = ctypes.c_void_p
             getattr(L, "olgi_scene_new" + sfx).argtypes = [
@@ -77,10 +79,11 @@ def lib():
             getattr(L, "olgi_eval_pbr" + sfx).argtypes = [P_f32, P_f32, P_f32, P_f32, P_f32, P_f32]
             getattr(L, "olgi_sample_pbr" + sfx).restype = ctypes.c_uint32
             getattr(L, "olgi_sample_pbr" + sfx).argtypes = [P_f32, P_f32, P_f32, ctypes.c_uint32, P_f32, P_f32, P_f32]
-        L.ol_lbvh_build.argtypes = [P_f32, P_u32, ctypes.c_uint32, ctypes.c_uint32, P_u32, P_u32, P_u32, P_u32,
-                                    P_f32, ctypes.POINTER(ctypes.c_uint8), P_u32, P_u32, P_f32]
-        _lib = L
-    return _lib
+        if mutant == 0:
+            L.ol_lbvh_build.argtypes = [P_f32, P_u32, ctypes.c_uint32, ctypes.c_uint32, P_u32, P_u32, P_u32, P_u32,
+                                        P_f32, ctypes.POINTER(ctypes.c_uint8), P_u32, P_u32, P_f32]
+        _libs[mutant] = L
+    return _libs[mutant]
 
 
 def _p(a, t=ctypes.c_float):
@@ -90,10 +93,11 @@ def _p(a, t=ctypes.c_float):
 class OracleScene:
     """Packed scene (ordinarylight_amd.packing.PackedScene) in the C oracle."""
 
-    def __init__(self, packed, precision="f32"):
+    def __init__(self, packed, precision="f32", mutant=0):
         self.sfx = "_" + precision
         self.packed = packed
-        L = lib()
+        self.mutant = int(mutant)
+        L = lib(self.mutant)
         self._arrays = [np.ascontiguousarray(a) for a in (
             packed.positions.astype(np.float32), packed.normals.astype(np.float32),
             packed.indices.astype(np.uint32), packed.material_ids.astype(np.uint32),
@@ -107,7 +111,7 @@ class OracleScene:
 
     def close(self):
         if self.handle:
-            getattr(lib(), "olgi_scene_free" + self.sfx)(self.handle)
+            getattr(lib(self.mutant), "olgi_scene_free" + self.sfx)(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -124,7 +128,7 @@ class OracleScene:
         out = np.zeros((height, width, 4), np.float32)
         counts = np.zeros(3, np.uint64)
         r0, r1 = rows if rows is not None else (0, height)
-        getattr(lib(), "olgi_render" + self.sfx)(
+        getattr(lib(self.mutant), "olgi_render" + self.sfx)(
             self.handle, _p(cam), projection, width, height, samples, frame, _p(iset, ctypes.c_uint32),
             _p(fset), int(bool(unified)), r0, r1, _p(out), _p(counts, ctypes.c_uint64))
         self.last_counts = counts
@@ -144,7 +148,7 @@ class OracleScene:
                           np.uint32)
         out = np.zeros((height, width, 4), np.float32)
         counts = np.zeros(3, np.uint64)
-        rc = getattr(lib(), "olgi_render_restir" + self.sfx)(
+        rc = getattr(lib(self.mutant), "olgi_render_restir" + self.sfx)(
             self.handle, _p(cam), projection, width, height, samples, frame, _p(iset, ctypes.c_uint32),
             _p(fset), int(bool(unified)), _p(rset, ctypes.c_uint32), _p(out), _p(counts, ctypes.c_uint64))
         if rc != 0:
@@ -162,7 +166,7 @@ class OracleScene:
         fset = np.asarray((roulette_min_survival, nee_probability), np.float32)
         out = {"depth": np.zeros((height, width), np.float32), "normal": np.zeros((height, width, 3), np.float32),
                "primitive": np.zeros((height, width), np.uint32), "variance": np.zeros((height, width), np.float32)}
-        getattr(lib(), "olgi_render_products" + self.sfx)(
+        getattr(lib(self.mutant), "olgi_render_products" + self.sfx)(
             self.handle, _p(cam), projection, width, height, samples, frame, _p(iset, ctypes.c_uint32), _p(fset),
             int(bool(unified)), _p(out["depth"]), _p(out["normal"]), _p(out["primitive"], ctypes.c_uint32),
             _p(out["variance"]))
@@ -175,7 +179,7 @@ class OracleScene:
         o[:, :3], o[:, 3] = origins, tmin
         d[:, :3], d[:, 3] = directions, tmax
         hits = np.empty((n, 4), np.float32)
-        getattr(lib(), "olgi_trace" + self.sfx)(self.handle, _p(o), _p(d), n, _p(hits))
+        getattr(lib(self.mutant), "olgi_trace" + self.sfx)(self.handle, _p(o), _p(d), n, _p(hits))
         return hits[:, 0], hits[:, 1], hits[:, 2], hits[:, 3].view(np.uint32)
 
 
@@ -312,13 +316,13 @@ def sincos(x):
     return float(s[0]), float(c[0])
 
 
-def eval_pbr(record, n, view, out, precision="f64"):
+def eval_pbr(record, n, view, out, precision="f64", mutant=0):
     """(f rgb, pdf) of the PBR BSDF for one material record (12 x 4 floats)."""
     rec = np.ascontiguousarray(record, np.float32).reshape(48)
     f = np.zeros(3, np.float32)
     pdf = np.zeros(1, np.float32)
     a = [np.ascontiguousarray(x, np.float32) for x in (n, view, out)]
-    getattr(lib(), "olgi_eval_pbr_" + precision)(_p(rec), _p(a[0]), _p(a[1]), _p(a[2]), _p(f), _p(pdf))
+    getattr(lib(mutant), "olgi_eval_pbr_" + precision)(_p(rec), _p(a[0]), _p(a[1]), _p(a[2]), _p(f), _p(pdf))
     return f, float(pdf[0])
 
 

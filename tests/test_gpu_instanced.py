@@ -39,9 +39,11 @@ def ctx(hip_device):
     c.close()
 
 
-def test_two_level_closest_hit_matches_flattened_oracle(ctx):
+@pytest.mark.parametrize("builder", ["lbvh", "sah"])
+def test_two_level_closest_hit_matches_flattened_oracle(ctx, builder):
     scene, cam = small_instanced(1)
-    ds = DeviceScene(ctx, scene, 2, instanced=True)
+    log(f"two-level closest hit, {builder} bottom levels")
+    ds = DeviceScene(ctx, scene, 2, instanced=True, builder=builder)
     assert ds.instanced and ds.triangle_count == scene.triangle_count
     orc = oracle.OracleScene(packing.pack_scene(scene))
     rng = np.random.default_rng(7)
@@ -67,10 +69,11 @@ def test_two_level_closest_hit_matches_flattened_oracle(ctx):
     ds.close()
 
 
-def render(ctx, scene, cam, instanced, w=96, h=64, spp=3, bounces=8):
-    ds = DeviceScene(ctx, scene, 2, instanced=instanced)
+def render(ctx, scene, cam, instanced, w=96, h=64, spp=3, bounces=8, builder="lbvh"):
+    ds = DeviceScene(ctx, scene, 2, instanced=instanced, builder=builder)
     img, st = plain_render(Job(ds, cam, w, h, spp, 0, HipRendererConfig(max_bounces=bounces)))
     ds.close()
+    assert st["queue_overflow"] == 0
     return img, st
 
 
@@ -78,10 +81,12 @@ def rel_close(a, b, rel=1e-3, abs_=1e-6):
     return np.all(np.abs(a - b) <= abs_ + rel * np.maximum(np.abs(a), np.abs(b)), axis=-1)
 
 
-def test_two_level_gi_matches_flat_and_oracle(ctx):
+@pytest.mark.parametrize("builder", ["lbvh", "sah"])
+def test_two_level_gi_matches_flat_and_oracle(ctx, builder):
     scene, cam = small_instanced(2)
-    two, st2 = render(ctx, scene, cam, True)
-    flat, st1 = render(ctx, scene, cam, False)
+    log(f"two-level GI, {builder} hierarchies")
+    two, st2 = render(ctx, scene, cam, True, builder=builder)
+    flat, st1 = render(ctx, scene, cam, False, builder=builder)
     assert st2["primary_rays"] == st1["primary_rays"]
     ok = rel_close(two[..., :3], flat[..., :3])
     log(f"two-level vs flat within tol {ok.mean():.5f}")
