@@ -1448,6 +1448,10 @@ void launch_rays(const GiLaunch& l, const DScene& sc, const float4* o, const flo
                             resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK));
   uint32_t need = (n + BLOCK - 1) / BLOCK;
   uint32_t cap = (uint32_t)(l.cu_count * occ);
+  // The grid strides over the rays, so a smaller spill buffer only bounds
+  // the lanes in flight.
+  if (spill_cap_of(l)) cap = (uint32_t)std::min<uint64_t>(cap, l.spill_words / ((uint64_t)spill_cap_of(l) * BLOCK));
+  if (cap == 0) throw OlError{-2, "traversal spill buffer too small for one block"};
   dim3 grid(need < cap ? (need ? need : 1) : cap);
   check_spill(l, (uint64_t)grid.x * BLOCK);
   if (sc.instanced)
