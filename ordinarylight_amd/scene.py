@@ -511,13 +511,18 @@ class Scene:
 
     # -- packers (reference layouts) ------------------------------------------
     def triangles(self):
-        """World-space triangles, per-triangle base colour and emission."""
+        """World-space triangles, per-triangle base colour (the face material's
+        when the mesh has one) and emission."""
         tris, colors, emissions = [], [], []
         for mesh in self.visible_meshes:
             world = mesh.world_vertices[mesh.indices]
             tris.append(world)
             count = len(world)
-            colors.append(np.broadcast_to(np.asarray(mesh.material.base_color, np.float32), (count, 3)))
+            faces = mesh.face_materials
+            if faces is not None:
+                colors.append(faces.base_color)
+            else:
+                colors.append(np.broadcast_to(np.asarray(mesh.material.base_color, np.float32), (count, 3)))
             emissions.append(np.broadcast_to(np.asarray(mesh.material.emission, np.float32), (count, 3)))
         if not tris:
             empty3 = np.empty((0, 3), np.float32)
