@@ -368,6 +368,22 @@ int olhip_camera_rays(olhip_ctx* ctx, const olhip_camera* cam, uint32_t width, u
 int olhip_trace(olhip_ctx* ctx, const olhip_scene* scene, const float* origins, const float* directions, uint32_t n,
                 float* hits);
 
+/* olhip_trace for rays already in device memory, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  d_origins, d_directions, d_hits: device float4[n] with
+ * olhip_trace's layout; d_status: one device word, cleared by the call and
+ * set non-zero when a traversal stack overflowed (the hits are then not
+ * valid).  stream: a hipStream_t, or NULL for the context's stream; the
+ * call only enqueues work on it.  The traversal stack's spill lives in a
+ * fixed per-device array of OLHIP_TRACE_SCRATCH_BYTES: scenes whose
+ * hierarchy needs more spill than that for one workgroup are refused
+ * (OLHIP_ERR_VALUE).  Every call on a GPU shares that array, so calls that
+ * overlap on one GPU (other streams, contexts or threads) race. */
+#define OLHIP_TRACE_SCRATCH_BYTES (16u << 20)
+int olhip_trace_device(olhip_ctx* ctx, const olhip_scene* scene, const float* d_origins, const float* d_directions,
+                       uint32_t n, float* d_hits, uint32_t* d_status, void* stream);
+
 /* ---- reference-estimator mode (renderers/reference/path_tracer.py) ----
  * The host owns the PCG64 stream; the device runs the per-bounce work.
  * Sequence per frame: e0_begin; per sample { e0_sample(jx, jy);

@@ -20,6 +20,7 @@ OLHIP_ERR_VALUE = -1
 OLHIP_ERR_RUNTIME = -2
 OLHIP_ERR_MEMORY = -3
 OLHIP_ERR_DEVICE = -4
+OLHIP_TRACE_SCRATCH_BYTES = 16 << 20
 
 c_u32 = ctypes.c_uint32
 c_f32 = ctypes.c_float
@@ -189,6 +190,8 @@ _SIGNATURES = {
     "olhip_engine_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EngineStats)]),
     "olhip_set_work_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "olhip_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P_f32, P_f32, c_u32, P_f32]),
+    "olhip_trace_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_u32,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "olhip_frame_output": (ctypes.c_int, [ctypes.c_void_p, c_u32, ctypes.c_float, c_u32, c_u32, ctypes.c_void_p,
                                           c_u32]),
     "olhip_scene_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(SceneDesc)]),

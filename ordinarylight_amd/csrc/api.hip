@@ -5,7 +5,9 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/olhip.h"
 #include "e0.h"
@@ -14,7 +16,31 @@
 #include "internal.h"
 #include "output.h"
 
+// Traversal-stack spill of olhip_trace_device, which allocates nothing: one
+// array per device for the life of the process.  Host code never uses this
+// name as an address; it uses the per-device address that
+// trace_scratch_address() takes once per device with that device current.
+static __device__ __align__(256) uint32_t scratch[OLHIP_TRACE_SCRATCH_BYTES / 4u];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<uint32_t*> g_scratch_ptr;  // by device index; null until taken
+
+// Device address of `scratch` on `device` (which must be current).
+uint32_t* trace_scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr.size() <= (size_t)device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    OL_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw OlError{OLHIP_ERR_DEVICE, "trace scratch array has no device address"};
+    g_scratch_ptr[device] = static_cast<uint32_t*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+__global__ void k_clear_word(uint32_t* word) { *word = 0u; }
 
 thread_local std::string g_last_error;
 
@@ -119,6 +145,7 @@ int olhip_create(int device, olhip_ctx** out) {
     hipDeviceProp_t prop;
     OL_HIP_CHECK(hipGetDeviceProperties(&prop, device));
     ctx->cu_count = prop.multiProcessorCount;
+    ctx->trace_scratch = trace_scratch_address(device);
     *out = ctx.release();
   });
 }
@@ -989,6 +1016,37 @@ int olhip_trace(olhip_ctx* ctx, const olhip_scene* sc, const float* origins, con
     OL_HIP_CHECK(hipMemcpy(&overflow, ov.ptr, 4, hipMemcpyDeviceToHost));
     if (overflow) throw OlError{OLHIP_ERR_RUNTIME, "traversal stack overflow"};
     OL_HIP_CHECK(hipMemcpy(hits, h.ptr, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int olhip_trace_device(olhip_ctx* ctx, const olhip_scene* sc, const float* d_origins, const float* d_directions,
+                       uint32_t n, float* d_hits, uint32_t* d_status, void* stream) {
+  return guarded([&] {
+    require(ctx && sc && d_origins && d_directions && d_hits && d_status, "arguments must not be NULL");
+    require(sc->ctx == ctx, "scene belongs to another context");
+    require(ctx->trace_scratch != nullptr, "context has no trace scratch array");
+    OL_HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    // The launch shape of make_launch, with the spill columns in the fixed
+    // array; the grid is capped so that its lanes' columns fit there.
+    GiLaunch l{};
+    l.cu_count = ctx->cu_count;
+    l.stack_depth = (int)(3u * std::max<uint32_t>(sc->bvh.depth, 1) + 1u);
+    l.lds_stack = ctx->lds_stack;
+    l.trace_block = ctx->trace_block;
+    l.spill = ctx->trace_scratch;
+    l.spill_words = OLHIP_TRACE_SCRATCH_BYTES / 4u;
+    const uint64_t per_block =
+        (uint64_t)std::max(l.stack_depth - l.lds_stack, 0) * (uint64_t)std::max(l.trace_block, 1);
+    require(per_block <= l.spill_words,
+            "hierarchy too deep for the fixed trace scratch array (" + std::to_string(4 * per_block) +
+                " bytes of stack spill per workgroup, at most " + std::to_string(OLHIP_TRACE_SCRATCH_BYTES) + ")");
+    hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(1), 0, s, d_status);
+    OL_HIP_CHECK(hipGetLastError());
+    if (n == 0) return;
+    trace_rays(l, sc->view, reinterpret_cast<const float4*>(d_origins), reinterpret_cast<const float4*>(d_directions),
+               n, reinterpret_cast<float4*>(d_hits), d_status, nullptr, s);
+    OL_HIP_CHECK(hipGetLastError());
   });
 }
 

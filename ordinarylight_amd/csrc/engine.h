@@ -245,6 +245,8 @@ struct olhip_ctx {
   DevBuf image;  // last single frame (olhip_frame_download, olhip_frame_output)
   uint32_t image_pixels = 0, image_width = 0, image_height = 0;
   DevBuf spill;  // olhip_trace
+  // Device address of the fixed traversal spill array (olhip_trace_device).
+  uint32_t* trace_scratch = nullptr;
   // Reference-estimator frame.
   struct E0Frame {
     DevBuf o, d, T, rad, acc, active, flags, scan, ht, htri, hitlist, hit_t, hit_tri, jx, jy, gauss, env, ctr, out;
