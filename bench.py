@@ -13,9 +13,11 @@
    [128, 1024], eight resident scenes of 10k-2M triangles, spp in
    {1,2,4,8,16}, bounces in {4,8}; one engine run (autotuned capacity and
    launch shape) against the same jobs rendered back-to-back by the plain path.
-3. Served (``served``): Poisson arrivals of such jobs at a fixed fraction of
-   the batched throughput; continuous-batching server against a plain
-   one-at-a-time server on the same arrivals; p50/p99 latency and jobs/s.
+3. Served (``served``): Poisson arrivals of such jobs at twice the plain
+   path's back-to-back throughput; continuous-batching server against a
+   plain one-at-a-time server on the same arrivals; p50/p99 latency and
+   jobs/s; then a sweep of arrival rates at 0.5, 1 and 1.5 times the batched
+   throughput.
 4. Instanced (``c3``): a two-level scene of 10,000 instances of five
    20,480-triangle icospheres (~205M effective triangles), 20 % glass,
    100k emitters sampled with ReSTIR DI (32 candidates, temporal reuse + one
@@ -40,7 +42,21 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_IN, HIT_OUT, OCC_OUT = 64, 48, 32, 16, 4
-TRAFFIC_FILE = os.path.join("profiles", "k_trace_terrain1M_1080p_8spp_8b_traffic.txt")
+# Instance record read when a ray enters an instance: world-to-object rows,
+# normal matrix, bounding sphere, bottom-level index and offsets (128 B).
+INST_BYTES = 128
+# Shading, per closest-hit ray: ray record (origin, direction, path id) and
+# hit record in, radiance out; per non-fresh path: throughput, radiance and
+# state in (a fresh path's state is rebuilt from its slot); per hit: the
+# 64-byte shading record and the 192-byte material; per continuation:
+# throughput and state out plus the next ray record; per shadow ray: its
+# 48-byte entry and a 32-byte group out (one entry per group without point
+# lights); per area-light sample: the 80-byte emitter record.
+SHADE_RAY, SHADE_STATE, SHADE_RADIANCE = 36 + 16, 48, 16
+SHADE_RECORD, SHADE_MATERIAL, SHADE_CONT = 64, 192, 32 + 36
+SHADE_ENTRY, SHADE_GROUP, SHADE_AREA = 48, 32, 80
+TRAFFIC_C2 = os.path.join("profiles", "traffic_terrain1M_1080p_8spp_8b.txt")
+TRAFFIC_C3 = os.path.join("profiles", "traffic_instanced205M_1080p_4spp_16b.txt")
 
 
 def log(msg):
@@ -58,17 +74,78 @@ def cpu_model():
     return platform.processor()
 
 
-def measured_traffic(workload):
-    """HBM bytes per closest-hit dispatch from the committed rocprofv3 FETCH_SIZE /
+def host_cpu():
+    """lscpu's model, logical CPUs, threads per core and NUMA nodes."""
+    import subprocess
+    info = {"model": cpu_model(), "cpus": os.cpu_count(), "threads_per_core": None, "numa_nodes": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    fields = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    info["model"] = fields.get("Model name", info["model"])
+    for key, name in (("CPU(s)", "cpus"), ("Thread(s) per core", "threads_per_core"), ("NUMA node(s)", "numa_nodes"),
+                      ("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket")):
+        try:
+            info[name] = int(fields[key])
+        except (KeyError, ValueError):
+            pass
+    return info
+
+
+def measured_traffic(path, workload, kernel):
+    """HBM bytes per dispatch of ``kernel`` from a committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, when it was measured on this same workload."""
     try:
-        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
-            fields = dict(line.split(":", 1) for line in f if line.startswith("traffic_"))
+        with open(os.path.join(ROOT, path)) as f:
+            fields = dict((k.strip(), v.strip()) for k, v in
+                          (line.split(":", 1) for line in f if line.startswith("traffic ")))
     except OSError:
         return None
-    if fields.get("traffic_workload", "").strip() != workload:
+    if fields.get("traffic workload") != workload or f"traffic {kernel}" not in fields:
         return None
-    return int(fields["traffic_bytes_per_launch"])
+    return int(fields[f"traffic {kernel}"])
+
+
+def traversal_bytes(w):
+    """Algorithmic bytes of a frame's traversal launches from its work
+    counters: node, triangle and instance-record fetches, ray records in,
+    hit records or occlusion bits out."""
+    closest = w["primary_rays"] + w["extension_rays"]
+    return (w["closest_nodes"] * NODE_BYTES + w["closest_tris"] * TRI_BYTES + w["closest_instances"] * INST_BYTES
+            + closest * (RAY_IN + HIT_OUT)
+            + w["shadow_nodes"] * NODE_BYTES + w["shadow_tris"] * TRI_BYTES + w["shadow_instances"] * INST_BYTES
+            + w["shadow_rays"] * (RAY_IN + OCC_OUT))
+
+
+def shade_bytes(w, instanced=False):
+    """Algorithmic bytes of a frame's shading launches (constants above)."""
+    closest = w["primary_rays"] + w["extension_rays"]
+    hit = SHADE_RECORD + SHADE_MATERIAL + (INST_BYTES if instanced else 0)
+    return (closest * (SHADE_RAY + SHADE_RADIANCE) + w["extension_rays"] * (SHADE_STATE + SHADE_CONT)
+            + w["shaded_hits"] * hit + w["shadow_rays"] * (SHADE_ENTRY + SHADE_GROUP)
+            + w["area_samples"] * SHADE_AREA)
+
+
+def roofline_of(kernel, bytes_per_launch, avg_ms, traffic, traffic_source):
+    """HBM roofline of one kernel from its algorithmic bytes per launch and
+    its average launch time (HIP events on the render stream); ``traffic`` is
+    the counter-measured HBM bytes per launch (or None)."""
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+           "basis": "algorithmic bytes per launch / average launch time",
+           "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bytes_per_launch),
+           "traffic_source": traffic_source if traffic is not None else None}
+    if traffic is not None:
+        gbs = traffic / (avg_ms * 1e-3) / 1e9
+        out["traffic_gbs"] = round(gbs, 1)
+        out["traffic_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    return out
 
 
 def cpu_baseline(scene, camera, width, height, spp, bounces, target_seconds):
@@ -98,11 +175,13 @@ def cpu_baseline(scene, camera, width, height, spp, bounces, target_seconds):
             break
         rows = min(height, max(rows * 2, int(rows * target_seconds / max(elapsed, 1e-3))))
     orc.close()
+    host = host_cpu()
     return {
         "value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
         "kind": "port",
         "sample": f"rows {r0}-{r1} of {width}x{height}, {spp} spp, {bounces} bounces "
-                  f"({rays} rays incl. shadow in {elapsed:.1f} s; oracle BVH build {build_s:.1f} s; {cpu_model()})",
+                  f"({rays} rays incl. shadow in {elapsed:.1f} s; oracle BVH build {build_s:.1f} s)",
+        "host": host,
     }
 
 
@@ -165,9 +244,9 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
     }
 
 
-def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8):
+def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8, compare_plain=True):
     """Poisson arrivals of mixed jobs: continuous-batching server vs a plain
-    one-job-at-a-time server on the same arrival sequence."""
+    one-job-at-a-time server on the same arrival sequence (``compare_plain``)."""
     from ordinarylight_amd import batch, workloads
     specs = workloads.mixed_jobs(seed, n_jobs)
     jobs = make_jobs(specs, scenes)
@@ -196,6 +275,14 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8):
     engine.close()
     served = batch.latency_percentiles(lat)
     equal = all(np.array_equal(img, batch.plain_render(j)[0]) for img, j in zip(results, jobs[:check]))
+    out = {
+        "jobs": n_jobs, "rate_per_s": rate,
+        "jobs_per_s": round(n_jobs / ((last_done - first_submit) * 1e-3), 2),
+        "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
+        "results_equal_plain": bool(equal), "checked_jobs": check,
+    }
+    if not compare_plain:
+        return out
     # Plain server: jobs run one at a time in arrival order.
     plain_lat = []
     t0 = time.perf_counter()
@@ -208,14 +295,12 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8):
         clock = time.perf_counter() - t0
         plain_lat.append((clock - at) * 1e3)
     plain = batch.latency_percentiles(plain_lat)
-    return {
-        "jobs": n_jobs, "rate_per_s": rate,
-        "jobs_per_s": round(n_jobs / ((last_done - first_submit) * 1e-3), 2),
-        "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
+    out.update({
         "plain_jobs_per_s": round(n_jobs / clock, 2),
         "plain_p50_ms": round(plain["p50_ms"], 2), "plain_p99_ms": round(plain["p99_ms"], 2),
-        "results_equal_plain": bool(equal), "checked_jobs": check,
-    }
+    })
+    out["speedup_vs_plain"] = round(out["jobs_per_s"] / out["plain_jobs_per_s"], 3)
+    return out
 
 
 def c3_measurement(context, steps=2, builder="lbvh"):
@@ -242,13 +327,27 @@ def c3_measurement(context, steps=2, builder="lbvh"):
     _native.check(lib.olhip_set_work_counters(context.handle, 1))
     _, work = plain_render(job, copy_image=False)
     _native.check(lib.olhip_set_work_counters(context.handle, 0))
+    _native.check(lib.olhip_set_profiling(context.handle, 1))
     plain_render(job, copy_image=False)  # warm-up
     rays, wall = 0, 0.0
+    trace_ms = shade_ms = 0.0
+    launches = 0
     for _ in range(steps):
         t1 = time.perf_counter()
         _, st = plain_render(job, copy_image=False)
         wall += time.perf_counter() - t1
         rays += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+        trace_ms += st["trace_ms"]
+        shade_ms += st["shade_ms"]
+        launches += st["trace_launches"]
+    _native.check(lib.olhip_set_profiling(context.handle, 0))
+    workload = "instanced 205M triangles, 1920x1080, 4 spp, 16 bounces, ReSTIR DI"
+    roofline = None
+    if launches and trace_ms > 0:
+        per_frame = launches / steps
+        roofline = roofline_of("k_trace2", traversal_bytes(work) / per_frame, trace_ms / launches,
+                               measured_traffic(TRAFFIC_C3, workload, "k_trace2"), TRAFFIC_C3)
+    closest = max(work["primary_rays"] + work["extension_rays"], 1)
     out = {
         "workload": "5 icospheres x 10,000 instances (204,900,000 triangles), 20% glass, 100k emitters, "
                     "1920x1080, 4 spp, 16 bounces",
@@ -259,8 +358,12 @@ def c3_measurement(context, steps=2, builder="lbvh"):
         "tris_per_closest_ray": round(work["closest_tris"] / max(work["primary_rays"] + work["extension_rays"], 1), 2),
         "nodes_per_shadow_ray": round(work["shadow_nodes"] / max(work["shadow_rays"], 1), 2),
         "tris_per_shadow_ray": round(work["shadow_tris"] / max(work["shadow_rays"], 1), 2),
+        "instances_per_closest_ray": round(work["closest_instances"] / closest, 3),
+        "instances_per_shadow_ray": round(work["shadow_instances"] / max(work["shadow_rays"], 1), 3),
         "primary_rays": int(work["primary_rays"]), "extension_rays": int(work["extension_rays"]),
         "shadow_rays": int(work["shadow_rays"]),
+        "trace_ms_per_frame": round(trace_ms / steps, 3), "shade_ms_per_frame": round(shade_ms / steps, 3),
+        "roofline": roofline, "traffic_workload": workload,
         "direct_lighting": "ReSTIR DI at primary hits (32 candidates, temporal + 1 spatial neighbour, "
                            "history limit 20); light-CDF NEE at later bounces",
     }
@@ -353,8 +456,8 @@ def main(argv=None):
     parser.add_argument("--block", type=int, default=128)
     parser.add_argument("--batch-jobs", type=int, default=64, help="batched workload size (0: skip)")
     parser.add_argument("--serve-jobs", type=int, default=96, help="served workload size (0: skip)")
-    parser.add_argument("--serve-load", type=float, default=0.7,
-                        help="Poisson arrival rate as a fraction of the batched jobs/s")
+    parser.add_argument("--serve-load", type=float, default=2.0,
+                        help="Poisson arrival rate as a multiple of the plain path's back-to-back jobs/s")
     parser.add_argument("--no-autotune", action="store_true")
     parser.add_argument("--no-c3", action="store_true", help="skip the instanced two-level workload")
     parser.add_argument("--no-c1", action="store_true", help="skip the reference-estimator box room")
@@ -431,22 +534,16 @@ def main(argv=None):
     # step's shadow rays: its algorithmic bytes are both rays' node and
     # triangle fetches, ray records in, hit records or occlusion bits out.
     closest_rays = work.primary_rays + work.extension_rays
-    alg_bytes = (work.closest_nodes * NODE_BYTES + work.closest_tris * TRI_BYTES
-                 + closest_rays * (RAY_IN + HIT_OUT)
-                 + work.shadow_nodes * NODE_BYTES + work.shadow_tris * TRI_BYTES
-                 + work.shadow_rays * (RAY_IN + OCC_OUT))
-    roofline = None
+    w = work.as_dict()
+    roofline = shade_roofline = None
+    workload = (f"terrain 1M triangles, per-triangle GGX, {W}x{H}, {spp} spp, {B} bounces, "
+                f"leaf {args.leaf_size}, {args.builder}")
     if trace_ms > 0 and trace_launches:
-        per_launch_bytes = alg_bytes / (trace_launches / args.steps)
-        avg_ms = trace_ms / trace_launches
-        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-        workload = f"terrain 1M triangles, {W}x{H}, {spp} spp, {B} bounces, leaf {args.leaf_size}, {args.builder}"
-        traffic = measured_traffic(workload)
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "k_trace", "avg_launch_ms": round(avg_ms, 4),
-                    "bytes_per_launch": int(per_launch_bytes),
-                    "traffic_source": TRAFFIC_FILE if traffic is not None else None}
+        launches_per_frame = trace_launches / args.steps
+        roofline = roofline_of("k_trace", traversal_bytes(w) / launches_per_frame, trace_ms / trace_launches,
+                               measured_traffic(TRAFFIC_C2, workload, "k_trace"), TRAFFIC_C2)
+        shade_roofline = roofline_of("k_shade", shade_bytes(w) / launches_per_frame, shade_ms / trace_launches,
+                                     measured_traffic(TRAFFIC_C2, workload, "k_shade"), TRAFFIC_C2)
     line = {
         "metric": "Mrays/s at 1080p 8-bounce; batched jobs/s + p50/p99 latency vs plain GPU",
         "value": round(value, 2),
@@ -460,7 +557,8 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded fBm terrain)",
-        "config": {"workload": "terrain 1M triangles, 1920x1080, 8 spp, 8 bounces, 64 emissive tris + env 0.5",
+        "config": {"workload": "terrain 1M triangles, per-triangle GGX materials, 1920x1080, 8 spp, 8 bounces, "
+                               "64 emissive tris + env 0.5",
                    "bvh_builder": args.builder,
                    "width": W, "height": H, "spp": spp, "max_bounces": B, "triangles": int(ds.info.triangle_count)},
         "roofline": roofline,
@@ -478,6 +576,9 @@ def main(argv=None):
             "nodes_per_shadow_ray": round(work.shadow_nodes / max(work.shadow_rays, 1), 2),
             "tris_per_shadow_ray": round(work.shadow_tris / max(work.shadow_rays, 1), 2),
             "shadow_rays_per_frame": int(work.shadow_rays),
+            "shaded_hits_per_frame": int(work.shaded_hits),
+            "shade_roofline": shade_roofline,
+            "traffic_workload": workload,
         },
     }
     renderer.close()
@@ -511,10 +612,23 @@ def main(argv=None):
             log(f"batched: {line['batched']}")
         if args.serve_jobs > 0:
             from ordinarylight_amd.batch import EngineSettings
-            rate = args.serve_load * (line["batched"]["jobs_per_s"] if "batched" in line else 20.0)
+            # Headline: arrivals at --serve-load times the plain path's
+            # back-to-back capacity, against a plain one-at-a-time server on
+            # the same arrivals; then a load sweep around the batched capacity.
+            plain_cap = line["batched"]["plain_jobs_per_s"] if "batched" in line else 20.0
+            batched_cap = line["batched"]["jobs_per_s"] if "batched" in line else 40.0
             line["served"] = served_measurement(context, scenes, settings or EngineSettings(), args.serve_jobs,
-                                                round(rate, 2), 1)
+                                                round(args.serve_load * plain_cap, 2), 1)
+            line["served"]["load_vs_plain_capacity"] = args.serve_load
             log(f"served: {line['served']}")
+            sweep = []
+            for load in (0.5, 1.0, 1.5):
+                r = served_measurement(context, scenes, settings or EngineSettings(), 2 * args.serve_jobs,
+                                       round(load * batched_cap, 2), 2, check=2, compare_plain=False)
+                r["load_vs_batched_capacity"] = load
+                sweep.append(r)
+                log(f"served sweep: {r}")
+            line["served"]["sweep"] = sweep
         for ds, _ in scenes.values():
             ds.close()
         context.close()

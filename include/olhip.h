@@ -208,6 +208,10 @@ typedef struct olhip_stats {
   uint64_t closest_tris;   /* 48-byte triangles tested by closest-hit rays */
   uint64_t shadow_nodes;
   uint64_t shadow_tris;
+  uint64_t closest_instances; /* two-level scenes: instances entered by closest-hit rays */
+  uint64_t shadow_instances;  /* and by shadow rays */
+  uint64_t shaded_hits;       /* hits whose shading record and material were read */
+  uint64_t area_samples;      /* area-light records read by next-event estimation */
 } olhip_stats;
 
 /* One GI frame; replaces _VulkanGlobalIlluminationEngine.render_frame /

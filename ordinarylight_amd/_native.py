@@ -92,6 +92,8 @@ class Stats(ctypes.Structure):
         ("trace_launches", c_u32), ("shade_launches", c_u32), ("shadow_launches", c_u32),
         ("chunks", c_u32), ("closest_nodes", ctypes.c_uint64), ("closest_tris", ctypes.c_uint64),
         ("shadow_nodes", ctypes.c_uint64), ("shadow_tris", ctypes.c_uint64),
+        ("closest_instances", ctypes.c_uint64), ("shadow_instances", ctypes.c_uint64),
+        ("shaded_hits", ctypes.c_uint64), ("area_samples", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -754,7 +754,7 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
     d.settings = *set;
     d.settings.path_capacity = (uint32_t)std::min<uint64_t>(capacity, 0xFFFFFFFFull);
     d.products = ((depth || normal || primitive) ? OLHIP_PRODUCT_PRIMARY : 0u) | (variance ? OLHIP_PRODUCT_VARIANCE : 0u);
-    unsigned long long work0[4] = {0, 0, 0, 0};
+    unsigned long long work0[kWorkWords] = {};
     if (ctx->counting) eng.read_work(work0);
     olhip_engine_stats es0{};
     eng.engine_stats(&es0);
@@ -796,12 +796,16 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
       o.shadow_launches = 0;
       o.chunks = js.chunks;
       if (ctx->counting) {
-        unsigned long long w[4];
+        unsigned long long w[kWorkWords];
         eng.read_work(w);
         o.closest_nodes = w[0] - work0[0];
         o.closest_tris = w[1] - work0[1];
         o.shadow_nodes = w[2] - work0[2];
         o.shadow_tris = w[3] - work0[3];
+        o.closest_instances = w[4] - work0[4];
+        o.shadow_instances = w[5] - work0[5];
+        o.shaded_hits = w[6] - work0[6];
+        o.area_samples = w[7] - work0[7];
       }
     }
   });

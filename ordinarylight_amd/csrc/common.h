@@ -121,6 +121,12 @@ OL_DEV uint32_t wave_reserve(uint32_t* counter, bool want) {
   base = __shfl(base, (int)leader, 64);
   return base + lanes_below(mask);
 }
+// Adds the number of active lanes to a 64-bit counter with one atomic per
+// wave (work counters).
+OL_DEV void count_active(unsigned long long* counter) {
+  const uint64_t active = __ballot(true);
+  if (lanes_below(active) == 0u) atomicAdd(counter, (unsigned long long)__popcll(active));
+}
 // Reserve `n` consecutive slots per lane (wave prefix sum, one atomic per wave).
 OL_DEV uint32_t wave_reserve_n(uint32_t* counter, uint32_t n) {
   uint32_t incl = n;

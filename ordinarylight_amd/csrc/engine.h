@@ -105,7 +105,7 @@ class Engine {
   // and the previous step's shadow rays, and around shading).
   void set_profiling(bool on) { profiling_ = on; }
   void set_work_counters(bool on);
-  void read_work(unsigned long long out[4]);
+  void read_work(unsigned long long out[kWorkWords]);
   double trace_ms = 0, shade_ms = 0;
   uint32_t trace_launches = 0;
 

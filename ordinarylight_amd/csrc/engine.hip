@@ -219,13 +219,13 @@ void Engine::set_work_counters(bool on) {
   std::lock_guard<std::mutex> g(step_mu_);
   if (on == counting_) return;
   counting_ = on;
-  OL_HIP_CHECK(hipMemsetAsync(work_.ptr, 0, 32, ctx_->stream));
+  OL_HIP_CHECK(hipMemsetAsync(work_.ptr, 0, 8 * kWorkWords, ctx_->stream));
   ensure_buffers();
 }
 
-void Engine::read_work(unsigned long long out[4]) {
+void Engine::read_work(unsigned long long out[kWorkWords]) {
   OL_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
-  OL_HIP_CHECK(hipMemcpy(out, work_.ptr, 32, hipMemcpyDeviceToHost));
+  OL_HIP_CHECK(hipMemcpy(out, work_.ptr, 8 * kWorkWords, hipMemcpyDeviceToHost));
 }
 
 uint64_t Engine::submit(const olhip_job_desc& d) {

@@ -44,6 +44,7 @@ struct GiParams {
 // 64-entry chunks through a prefix over all runs, so each chunk lies in one
 // job and a wave that takes a chunk is job-uniform.
 constexpr uint32_t kMaxJobs = 64;
+constexpr uint32_t kWorkWords = 8;
 constexpr uint32_t kJobStripes = 32;
 constexpr uint32_t kRuns = kMaxJobs * kJobStripes;
 constexpr uint32_t kChunk = 64;
@@ -159,7 +160,10 @@ struct Wave {
   // the path's random state after the reservoir pass.
   float4* rsamp;
   uint32_t* ctr;
-  unsigned long long* work;  // optional: closest nodes, closest tris, shadow nodes, shadow tris
+  // Optional work counters (kWorkWords): closest nodes, closest tris, shadow
+  // nodes, shadow tris, closest instance entries, shadow instance entries,
+  // shaded hits, area-light records read by NEE.
+  unsigned long long* work;
 };
 
 // Traversal launch shape.

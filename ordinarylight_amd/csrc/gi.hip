@@ -569,7 +569,7 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   q.inst = OL_NONE;
   uint32_t job = 0, slot = OL_NONE;
   bool busy = false;
-  uint32_t nv = 0, nt = 0;
+  uint32_t nv = 0, nt = 0, ni = 0;
   while (true) {
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
@@ -604,7 +604,7 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
     if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
       if (at_leaf) closest2_leaf<LSTACK>(q, st, nt);
     } else if (busy && !at_leaf) {
-      closest2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv);
+      closest2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv, ni);
     }
     if (busy && q.node == OL_NONE) {
       W.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
@@ -612,9 +612,10 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       busy = false;
     }
   }
-  if (W.work && (nv | nt)) {
+  if (W.work && (nv | nt | ni)) {
     atomicAdd(&W.work[0], (unsigned long long)nv);
     atomicAdd(&W.work[1], (unsigned long long)nt);
+    atomicAdd(&W.work[4], (unsigned long long)ni);
   }
 }
 
@@ -637,7 +638,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   uint32_t job = 0, g = 0, e = 0, e_end = 0, mode = 0;
   f3 direct = mk3(0.0f, 0.0f, 0.0f), area_direct = direct, env_direct = direct;
   bool busy = false;
-  uint32_t nv = 0, nt = 0;
+  uint32_t nv = 0, nt = 0, ni = 0;
   while (true) {
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
@@ -682,7 +683,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
     if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
       if (at_leaf) any2_leaf<LSTACK>(q, st, nt);
     } else if (busy && !at_leaf) {
-      any2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv);
+      any2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv, ni);
     }
     if (busy && q.node == OL_NONE) {
       if (!q.occluded) shadow_add(mode, W.ec[e], direct, area_direct, env_direct);
@@ -697,9 +698,10 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       }
     }
   }
-  if (W.work && (nv | nt)) {
+  if (W.work && (nv | nt | ni)) {
     atomicAdd(&W.work[2], (unsigned long long)nv);
     atomicAdd(&W.work[3], (unsigned long long)nt);
+    atomicAdd(&W.work[5], (unsigned long long)ni);
   }
 }
 
@@ -754,7 +756,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays2(DScene sc, const float4* 
                                                        uint32_t* spill, uint32_t spill_cap) {
   __shared__ uint32_t lds[LSTACK * BLOCK];
   TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, overflow);
-  uint32_t nv = 0, nt = 0;
+  uint32_t nv = 0, nt = 0, ni = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     const float4 o = o4[i], d = d4[i];
     Closest2 q;
@@ -763,7 +765,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays2(DScene sc, const float4* 
     if (sc.node_count == 0) q.node = OL_NONE;
     while (q.node != OL_NONE) {
       if (closest2_at_leaf(q)) closest2_leaf<LSTACK>(q, st, nt);
-      else closest2_inner<LSTACK>(sc, q, st, nv);
+      else closest2_inner<LSTACK>(sc, q, st, nv, ni);
     }
     hits[i] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.orig));
   }
@@ -1118,6 +1120,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         uint32_t prim, mat_id;
         bool entering;
         resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
+        if (W.work) count_active(&W.work[6]);
         if (record_primary) {
           J.geo[M.x] = make_float4(t, gn.x, gn.y, gn.z);
           J.prim[M.x] = prim;
@@ -1213,8 +1216,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 single = s;
                 nee_count = 1u;
               });
+              if (W.work && area_n) atomicAdd(&W.work[7], (unsigned long long)area_n);
             } else {
               for_each_nee(P, rs, hitp, nrm, mat, incoming, area_n, env_n, [&](const NeeSample&) { ++nee_count; });
+              if (W.work && area_n) atomicAdd(&W.work[7], (unsigned long long)area_n);
             }
           }
           // shadeScatterOpaquePath / shadeSamplePbr
@@ -1389,7 +1394,7 @@ __global__ void __launch_bounds__(1024) k_epilogue(Wave W, uint32_t in) {
 // Zeroes the whole counter block (engine start).
 __global__ void __launch_bounds__(1024) k_clear(Wave W) {
   for (uint32_t i = threadIdx.x; i < C_WORDS; i += blockDim.x) W.ctr[i] = 0u;
-  if (W.work && threadIdx.x < 4) W.work[threadIdx.x] = 0ull;
+  if (W.work && threadIdx.x < kWorkWords) W.work[threadIdx.x] = 0ull;
 }
 
 // ---- launch helpers ------------------------------------------------------------------------------

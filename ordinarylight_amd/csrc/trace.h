@@ -457,6 +457,27 @@ OL_DEV bool sphere_reachable(const RayPre& r, float4 s) {
   return dot3(v, v) <= reach * reach;
 }
 
+// Whether the ray can meet an instance's world bounding sphere at a
+// parameter t <= tmax: the line passes within the sphere (sphere_reachable)
+// and its entry into the sphere, grown by the same allowance, lies no later
+// than tmax plus a further allowance of 1e-4 of the magnitudes.  Every point
+// of the instance lies in the sphere, so an instance failing the test holds
+// no hit at t <= tmax: a closest-hit ray skips it once a nearer hit is known.
+OL_DEV bool sphere_before(const RayPre& r, float4 s, float tmax) {
+  const f3 c = mk3(s.x, s.y, s.z);
+  const f3 oc = c - r.o;
+  const float dd = dot3(r.d, r.d);
+  const float tc = dot3(oc, r.d) / dd;
+  const f3 v = oc - r.d * tc;
+  const float scale = fmaxf(fmaxf(max_abs3(r.o), max_abs3(c)), fabsf(tc) * max_abs3(r.d));
+  const float reach = s.w + 1e-5f * scale;
+  const float h2 = reach * reach - dot3(v, v);
+  if (!(h2 >= 0.0f)) return false;
+  const float inv_len = 1.0f / sqrtf(dd);
+  const float t_in = tc - sqrtf(h2) * inv_len;
+  return t_in <= tmax + 1e-4f * (s.w + scale) * inv_len;
+}
+
 struct Closest2 {
   RayPre w;                // world ray
   RayPre r;                // current ray: world at the top level, object space inside an instance
@@ -492,7 +513,7 @@ OL_DEV void closest2_next(Closest2& q, TravStack<LSTACK>& st) { q.node = st.empt
 // Inner step: leave an instance (sentinel), enter one (top-level leaf), or
 // test a node's boxes at the current level.
 template <int LSTACK>
-OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv) {
+OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& ni) {
   if (q.node == OL_SENTINEL) {
     q.inst = OL_NONE;
     q.r = q.w;
@@ -503,10 +524,11 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st,
     const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
     const InstanceDev& I = sc.inst[k];
     const BlasDev& B = sc.blas[I.blas];
-    if (B.node_count == 0 || !sphere_reachable(q.w, I.sphere)) {
+    if (B.node_count == 0 || !sphere_before(q.w, I.sphere, q.best.t)) {
       closest2_next(q, st);
       return;
     }
+    ++ni;
     st.push(OL_SENTINEL);
     q.inst = k;
     q.offset = I.tri_offset;
@@ -575,7 +597,7 @@ template <int LSTACK>
 OL_DEV void any2_next(Any2& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
 
 template <int LSTACK>
-OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& nv) {
+OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& ni) {
   if (q.node == OL_SENTINEL) {
     q.inst = OL_NONE;
     q.r = q.w;
@@ -590,6 +612,7 @@ OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_
       any2_next(q, st);
       return;
     }
+    ++ni;
     st.push(OL_SENTINEL);
     q.inst = k;
     q.bnodes = B.nodes;
