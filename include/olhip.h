@@ -68,6 +68,22 @@ typedef struct olhip_scene_desc {
   uint32_t builder;            /* OLHIP_BUILDER_LBVH (GPU Morton LBVH, default) or
                                   OLHIP_BUILDER_SAH (host binned-SAH hierarchy,
                                   GPU boxes and wide collapse) */
+  /* Textures (all NULL / 0 without): the packed word array of
+   * Scene.texture_data (scene/_core.py:3005-3043: count, 8 metadata words per
+   * texture, sRGB and linear RGBA8 mip chains), binding records of
+   * texture_binding_data (:2984-3003), and for textured materials the
+   * per-vertex inputs of triangle_attribute_data (:2042-2069), indexed.
+   * Materials with bindings carry 1.0 in record word [8].w (a padding word
+   * of the reference record) and their transmission binding in [5].w.  An
+   * image environment's record holds (texture, rotation, log range) in row 3
+   * (analytic_light_data, :2613-2629). */
+  uint32_t texture_word_count;
+  const uint32_t* texture_words;
+  uint32_t binding_count;
+  const float* bindings;       /* [binding_count][2][4] */
+  const float* uvs;            /* [vertex_count][4] uv0.xy, uv1.xy (textured materials) */
+  const float* tangents;       /* [vertex_count][4] world tangent, handedness */
+  const float* uv_density;     /* [triangle_count] sqrt(uv0 area / world area) */
 } olhip_scene_desc;
 
 #define OLHIP_BUILDER_LBVH 0u
@@ -109,6 +125,10 @@ typedef struct olhip_instanced_desc {
   const float* world_spheres;          /* [I][4] centre xyz, radius of a sphere containing
                                           the instance, or NULL; rays whose line misses
                                           it skip the instance */
+  uint32_t texture_word_count;         /* an image environment's texture words (as in
+                                          olhip_scene_desc), or 0; materials with texture
+                                          bindings need a flat scene */
+  const uint32_t* texture_words;
 } olhip_instanced_desc;
 
 int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* desc, olhip_scene** out);

@@ -54,7 +54,11 @@ typedef OL_REAL R;
  *   4 roulette survivors not 1/p      10 spot cone linear instead of smoothstep
  *   5 Beer absorption on the wrong    11 emission hit without its MIS weight
  *     segment                         12 GGX specular lobe without D
- *   6 total internal reflection passes straight on */
+ *   6 total internal reflection passes straight on
+ *  13 bilinear filtering without the half-texel shift
+ *  14 sRGB texels not decoded to linear
+ *  15 mip level chosen without the binding's uv scale
+ *  16 normal map without the tangent frame's handedness */
 #ifndef OL_MUTANT
 #define OL_MUTANT 0
 #endif
@@ -179,6 +183,19 @@ typedef struct {
   uint32_t node_count;
   uint32_t* order; /* leaf order -> triangle */
   double pad;      /* node boxes are grown by this much on every side */
+  /* Textures (olgi_scene_textures; NULL / 0 without): packed texture words,
+   * binding records (8 floats each), per-vertex uv0.xy uv1.xy and world
+   * tangents, per-triangle uv0 density; the image environment (texture,
+   * rotation, log range from the environment record's row 3). */
+  uint32_t* words;
+  uint32_t nwords;
+  float* bind;
+  uint32_t B;
+  float* uvs;
+  float* tans;
+  float* dens;
+  int env_tex;
+  float env_rot, env_log, env_col[3], env_int;
 } Scene;
 
 static void tri_box(const Scene* s, uint32_t t, double mn[3], double mx[3]) {
@@ -275,6 +292,7 @@ void* API(olgi_scene_new)(const float* pos, const float* nrm, uint32_t V, const 
   if (P) memcpy(s->ana, ana, 64ull * P);
   if (L) memcpy(s->area, area, 80ull * L);
   s->weight = weight;
+  s->env_tex = -1;
   for (uint32_t l = 0; l < P; ++l) {
     const float* a = s->ana + 16ull * l;
     if ((int)(a[3] + 0.5f) == 3) {
@@ -282,6 +300,13 @@ void* API(olgi_scene_new)(const float* pos, const float* nrm, uint32_t V, const 
       s->env[0] = (1.0f * a[8]) * a[11];
       s->env[1] = (1.0f * a[9]) * a[11];
       s->env[2] = (1.0f * a[10]) * a[11];
+      s->env_col[0] = a[8];
+      s->env_col[1] = a[9];
+      s->env_col[2] = a[10];
+      s->env_int = a[11];
+      s->env_tex = a[12] >= 0.0f ? (int)a[12] : -1;
+      s->env_rot = a[13];
+      s->env_log = a[14];
       break;
     }
   }
@@ -316,7 +341,37 @@ void API(olgi_scene_free)(void* h) {
   free(s->area);
   free(s->nodes);
   free(s->order);
+  free(s->words);
+  free(s->bind);
+  free(s->uvs);
+  free(s->tans);
+  free(s->dens);
   free(s);
+}
+
+/* Attaches texture data to a scene (copied).  uvs / tangents / density may
+ * be NULL when no material has texture bindings. */
+void API(olgi_scene_textures)(void* h, const uint32_t* words, uint32_t nwords, const float* bind, uint32_t B,
+                              const float* uvs, const float* tans, const float* dens) {
+  Scene* s = (Scene*)h;
+  if (nwords) {
+    s->words = (uint32_t*)malloc(4ull * nwords);
+    memcpy(s->words, words, 4ull * nwords);
+    s->nwords = nwords;
+  }
+  if (B) {
+    s->bind = (float*)malloc(32ull * B);
+    memcpy(s->bind, bind, 32ull * B);
+    s->B = B;
+  }
+  if (uvs) {
+    s->uvs = (float*)malloc(16ull * s->V + 4);
+    s->tans = (float*)malloc(16ull * s->V + 4);
+    s->dens = (float*)malloc(4ull * s->T + 4);
+    memcpy(s->uvs, uvs, 16ull * s->V);
+    memcpy(s->tans, tans, 16ull * s->V);
+    memcpy(s->dens, dens, 4ull * s->T);
+  }
 }
 
 /* Moller-Trumbore in R, expressions as written. */
@@ -535,8 +590,117 @@ static R power_h(R a, R b) {
   R a2 = a * a, b2 = b * b;
   return a2 / rmax(a2 + b2, (R)1e-6);
 }
+/* ---- texture sampling (wavefront_shade_candidate.glsl:678-881) ----------- */
+static R r_floor(R x) { return sizeof(R) == 4 ? (R)floorf((float)x) : (R)floor((double)x); }
+static R r_pow(R x, R y) { return sizeof(R) == 4 ? (R)powf((float)x, (float)y) : (R)pow((double)x, (double)y); }
+static R r_log2(R x) { return sizeof(R) == 4 ? (R)log2f((float)x) : (R)log2((double)x); }
+static R r_exp2(R x) { return sizeof(R) == 4 ? (R)exp2f((float)x) : (R)exp2((double)x); }
+static R r_atan2(R y, R x) { return sizeof(R) == 4 ? (R)atan2f((float)y, (float)x) : (R)atan2((double)y, (double)x); }
+static R r_acos(R x) { return sizeof(R) == 4 ? (R)acosf((float)x) : (R)acos((double)x); }
+
+typedef struct {
+  R x, y, z, w;
+} V4;
+static V4 v4mix(V4 a, V4 b, R t) {
+  V4 r = {mixr(a.x, b.x, t), mixr(a.y, b.y, t), mixr(a.z, b.z, t), mixr(a.w, b.w, t)};
+  return r;
+}
+/* shadeSrgbChannel :678 */
+static R srgb_ch(R v) { return v <= (R)0.04045 ? v / (R)12.92 : r_pow((v + (R)0.055) / (R)1.055, (R)2.4); }
+/* shadeWrapTextureCoordinate :687 */
+static R wrap_c(R v, uint32_t mode) {
+  if (mode == 1u) return clampr(v, (R)0, (R)0.99999994f);
+  if (mode == 2u) {
+    R period = v - r_floor(v / (R)2) * (R)2;
+    if (period < (R)0) period = period + (R)2;
+    return period <= (R)1 ? period : (R)2 - period;
+  }
+  return v - r_floor(v);
+}
+/* shadeWrapTextureIndex :708 */
+static int wrap_i(int v, int size, uint32_t mode) {
+  if (mode == 1u) return v < 0 ? 0 : (v > size - 1 ? size - 1 : v);
+  int period = mode == 2u ? size * 2 : size;
+  int w = v % period;
+  if (w < 0) w += period;
+  if (mode == 2u && w >= size) return (period - 1) - w;
+  return w;
+}
+/* shadeDecodeTextureTexel / shadeFetchTextureTexel :730-747 */
+static V4 fetch(const Scene* s, uint32_t off, int w, int h, int x, int y, uint32_t wx, uint32_t wy, int srgb) {
+  uint32_t pk = s->words[off + (uint32_t)(wrap_i(y, h, wy) * w + wrap_i(x, w, wx))];
+  V4 c = {(R)(float)(pk & 255u) / (R)255, (R)(float)((pk >> 8) & 255u) / (R)255,
+          (R)(float)((pk >> 16) & 255u) / (R)255, (R)(float)(pk >> 24) / (R)255};
+  if (srgb && !OL_MUT(14)) {
+    c.x = srgb_ch(c.x);
+    c.y = srgb_ch(c.y);
+    c.z = srgb_ch(c.z);
+  }
+  return c;
+}
+/* shadeTextureMipOffset / shadeSampleTextureLevel :749-782 */
+static V4 sample_lv(const Scene* s, uint32_t base, int bw, int bh, uint32_t level, R u, R v, uint32_t wx, uint32_t wy,
+                    int srgb, int linear) {
+  uint32_t off = base;
+  int w = bw, h = bh;
+  for (uint32_t k = 0; k < level && k < 32u; ++k) {
+    off += (uint32_t)(w * h);
+    w = (w + 1) / 2 > 1 ? (w + 1) / 2 : 1;
+    h = (h + 1) / 2 > 1 ? (h + 1) / 2 : 1;
+  }
+  R wu = wrap_c(u, wx), wv = wrap_c(v, wy);
+  if (!linear) return fetch(s, off, w, h, (int)r_floor(wu * (R)w), (int)r_floor(wv * (R)h), wx, wy, srgb);
+  R half = OL_MUT(13) ? (R)0 : (R)0.5;
+  R tx = wu * (R)w - half, ty = wv * (R)h - half;
+  int x0 = (int)r_floor(tx), y0 = (int)r_floor(ty);
+  R fx = tx - r_floor(tx), fy = ty - r_floor(ty);
+  V4 top = v4mix(fetch(s, off, w, h, x0, y0, wx, wy, srgb), fetch(s, off, w, h, x0 + 1, y0, wx, wy, srgb), fx);
+  V4 bot = v4mix(fetch(s, off, w, h, x0, y0 + 1, wx, wy, srgb), fetch(s, off, w, h, x0 + 1, y0 + 1, wx, wy, srgb), fx);
+  return v4mix(top, bot, fy);
+}
+/* shadeSampleMaterialTexture :784-827 */
+static V4 sample_mat_tex(const Scene* s, R binding, R u0, R v0, R u1, R v1, int srgb, R fp0, R fp1) {
+  V4 white = {1, 1, 1, 1};
+  int b = (int)binding;
+  if (b < 0) return white;
+  const float* r = s->bind + 8ull * (uint32_t)b;
+  int t = (int)r[0];
+  if (t < 0 || (uint32_t)t >= s->words[0]) return white;
+  int use1 = r[3] > 0.5f;
+  R u = use1 ? u1 : u0, v = use1 ? v1 : v0, fp = use1 ? fp1 : fp0;
+  R cs = r[1], sn = r[2];
+  R su = u * (R)r[6], sv_ = v * (R)r[7];
+  u = (R)r[4] + (cs * su - sn * sv_);
+  v = (R)r[5] + (sn * su + cs * sv_);
+  if (!OL_MUT(15)) fp = fp * rmax(absr((R)r[6]), absr((R)r[7]));
+  uint32_t meta = 1u + (uint32_t)t * 8u;
+  uint32_t off = s->words[meta + (srgb ? 0u : 1u)];
+  int w = (int)s->words[meta + 2u], h = (int)s->words[meta + 3u];
+  uint32_t flags = s->words[meta + 4u], levels = s->words[meta + 5u];
+  R lod = clampr(r_log2(rmax(fp * (R)(w > h ? w : h), (R)1)), (R)0, (R)((levels > 1u ? levels : 1u) - 1u));
+  uint32_t lower = (uint32_t)r_floor(lod);
+  int linear = (flags & 16u) != 0u;
+  V4 first = sample_lv(s, off, w, h, lower, u, v, flags & 3u, (flags >> 2) & 3u, srgb, linear);
+  if (!linear || lower + 1u >= levels) return first;
+  V4 second = sample_lv(s, off, w, h, lower + 1u, u, v, flags & 3u, (flags >> 2) & 3u, srgb, 1);
+  return v4mix(first, second, lod - r_floor(lod));
+}
+
 static V env_rad(const Scene* s, V d) {
   if (!s->env_valid) return mixv(v3((R)0.018, (R)0.022, (R)0.032), v3((R)0.32, (R)0.46, (R)0.72), rmax(d.y, (R)0));
+  if (s->env_tex >= 0 && s->words && (uint32_t)s->env_tex < s->words[0]) {
+    /* shadeEnvironmentUv, shadeSamplePackedSceneTexture, shadeDecodeEnvironmentRadiance :1835-1894 */
+    R lon = r_atan2(d.z, d.x) + (R)s->env_rot;
+    R x = lon * (R)0.15915494309189535 + (R)0.5;
+    R u = x - r_floor(x);
+    R v = r_acos(clampr(d.y, (R)-1, (R)1)) * (R)0.3183098861837907;
+    uint32_t meta = 1u + (uint32_t)s->env_tex * 8u, flags = s->words[meta + 4u];
+    V4 c = sample_lv(s, s->words[meta + 1u], (int)s->words[meta + 2u], (int)s->words[meta + 3u], 0u, u, v, flags & 3u,
+                     (flags >> 2) & 3u, 0, (flags & 16u) != 0u);
+    V rad = v3(r_exp2(c.x * (R)s->env_log) - (R)1, r_exp2(c.y * (R)s->env_log) - (R)1,
+               r_exp2(c.z * (R)s->env_log) - (R)1);
+    return vs(vmul(rad, v3(s->env_col[0], s->env_col[1], s->env_col[2])), (R)s->env_int);
+  }
   return v3(s->env[0], s->env[1], s->env[2]);
 }
 
@@ -1001,6 +1165,136 @@ static int restir_primary(const Scene* s, const Restir* rc, uint32_t* rs, V hit,
   return 1;
 }
 
+/* Probes of the sampler for the closed-form tests. */
+void API(olgi_texture_sample)(void* h, float binding, float u0, float v0, float u1, float v1, int srgb, float fp0,
+                              float fp1, float out[4]) {
+  V4 c = sample_mat_tex((const Scene*)h, binding, u0, v0, u1, v1, srgb, fp0, fp1);
+  out[0] = (float)c.x;
+  out[1] = (float)c.y;
+  out[2] = (float)c.z;
+  out[3] = (float)c.w;
+}
+void API(olgi_env_radiance)(void* h, const float d[3], float out[3]) {
+  V c = env_rad((const Scene*)h, v3(d[0], d[1], d[2]));
+  out[0] = (float)c.x;
+  out[1] = (float)c.y;
+  out[2] = (float)c.z;
+}
+
+/* ---- textured surfaces (shadeResolveSurface :957-997) -------------------- */
+static int mat_textured(const Scene* s, uint32_t m) { return s->mat[48ull * m + 35] == 1.0f; }
+
+/* shadeTriangleUvDensity :872-879 */
+static R uv_dens(V a, V b, V c, R ua, R va, R ub, R vb, R uc, R vc) {
+  R fx = ub - ua, fy = vb - va, sx = uc - ua, sy = vc - va;
+  R uv_area = absr(fx * sy - fy * sx);
+  R world = len(cross(vsub(b, a), vsub(c, a)));
+  return rsqrt_(uv_area / rmax(world, (R)1e-8));
+}
+
+/* Material textures and normal map of a hit (shadeApplyMaterialTextures
+ * :915-934, shadeTriangleTangent :889-913, shadeApplyNormalTexture :936-955):
+ * updates m (base colour, roughness, metallic, transmission, occlusion), the
+ * emission *em and the shading normal *sn (on the geometric side). */
+static void tex_surface(const Scene* s, uint32_t mid, uint32_t prim, R bu, R bv, V gn, R cone_w, Mat* m, V* em,
+                        V* sn) {
+  const float* rec = s->mat + 48ull * mid;
+  const uint32_t* tv = s->idx + 3ull * prim;
+  R wx = ((R)1 - bu) - bv, wy = bu, wz = bv;
+  const float *qa = s->uvs + 4ull * tv[0], *qb = s->uvs + 4ull * tv[1], *qc = s->uvs + 4ull * tv[2];
+  R u0 = (((R)qa[0] * wx) + ((R)qb[0] * wy)) + ((R)qc[0] * wz);
+  R v0 = (((R)qa[1] * wx) + ((R)qb[1] * wy)) + ((R)qc[1] * wz);
+  R u1 = 0, v1 = 0, fp1 = 0;
+  R fp0 = cone_w * (R)s->dens[prim];
+  const float *pa = s->pos + 3ull * tv[0], *pb = s->pos + 3ull * tv[1], *pc = s->pos + 3ull * tv[2];
+  V A = v3(pa[0], pa[1], pa[2]), B = v3(pb[0], pb[1], pb[2]), C = v3(pc[0], pc[1], pc[2]);
+  R idz = rec[14];
+  if (idz - r_floor(idz) > (R)0.125) {
+    u1 = (((R)qa[2] * wx) + ((R)qb[2] * wy)) + ((R)qc[2] * wz);
+    v1 = (((R)qa[3] * wx) + ((R)qb[3] * wy)) + ((R)qc[3] * wz);
+    fp1 = cone_w * uv_dens(A, B, C, qa[2], qa[3], qb[2], qb[3], qc[2], qc[3]);
+  }
+  m->trans = m->trans * sample_mat_tex(s, rec[23], u0, v0, u1, v1, 0, fp0, fp1).x;
+  V4 base = sample_mat_tex(s, rec[16], u0, v0, u1, v1, 1, fp0, fp1);
+  m->base = vmul(m->base, v3(base.x, base.y, base.z));
+  V4 emit = sample_mat_tex(s, rec[18], u0, v0, u1, v1, 1, fp0, fp1);
+  *em = vmul(*em, v3(emit.x, emit.y, emit.z));
+  if (!(m->trans > (R)0.001)) {
+    V4 mr = sample_mat_tex(s, rec[17], u0, v0, u1, v1, 0, fp0, fp1);
+    m->rough = m->rough * mr.y;
+    m->metal = m->metal * mr.z;
+    R occ = sample_mat_tex(s, rec[21], u0, v0, u1, v1, 0, fp0, fp1).x;
+    m->occl = mixr((R)1, occ, (R)rec[22]);
+  }
+  const float *ta = s->tans + 4ull * tv[0], *tb = s->tans + 4ull * tv[1], *tc = s->tans + 4ull * tv[2];
+  V tan = v3((((R)ta[0] * wx) + ((R)tb[0] * wy)) + ((R)tc[0] * wz), (((R)ta[1] * wx) + ((R)tb[1] * wy)) + ((R)tc[1] * wz),
+             (((R)ta[2] * wx) + ((R)tb[2] * wy)) + ((R)tc[2] * wz));
+  R hand = (((R)ta[3] * wx) + ((R)tb[3] * wy)) + ((R)tc[3] * wz);
+  int nb = (int)rec[19];
+  V n = *sn;
+  if (nb >= 0 && s->bind[8ull * (uint32_t)nb + 3] > 0.5f) {
+    R dax = (R)qb[2] - (R)qa[2], day = (R)qb[3] - (R)qa[3], dbx = (R)qc[2] - (R)qa[2], dby = (R)qc[3] - (R)qa[3];
+    V ea = vsub(B, A), eb = vsub(C, A);
+    R det = dax * dby - day * dbx;
+    if (absr(det) < (R)1e-8) {
+      V fb = cross(n, v3(0, 1, 0));
+      if (absr(n.z) < (R)0.999) fb = cross(n, v3(0, 0, 1));
+      tan = norm(fb);
+      hand = 1;
+    } else {
+      R inv = (R)1 / det;
+      V t = norm(vs(vsub(vs(ea, dby), vs(eb, day)), inv));
+      V bt = norm(vs(vsub(vs(eb, dax), vs(ea, dbx)), inv));
+      tan = t;
+      hand = dot(cross(n, t), bt) < (R)0 ? (R)-1 : (R)1;
+    }
+  }
+  if (nb >= 0) {
+    V4 t4 = sample_mat_tex(s, rec[19], u0, v0, u1, v1, 0, fp0, fp1);
+    V tn = v3(t4.x * (R)2 - (R)1, t4.y * (R)2 - (R)1, t4.z * (R)2 - (R)1);
+    tn.x = tn.x * (R)rec[20];
+    tn.y = tn.y * (R)rec[20];
+    tn = norm(tn);
+    V t = norm(vsub(tan, vs(n, dot(n, tan))));
+    V bt = vs(cross(n, t), OL_MUT(16) ? (R)1 : hand);
+    const float* br = s->bind + 8ull * (uint32_t)nb;
+    R cs = br[1], snn = br[2];
+    R ix = (R)1 / (R)br[6], iy = (R)1 / (R)br[7];
+    V tt = norm(vadd(vs(t, cs * ix), vs(bt, (-snn) * iy)));
+    V tbn = norm(vadd(vs(t, snn * ix), vs(bt, cs * iy)));
+    n = norm(vadd(vadd(vs(tt, tn.x), vs(tbn, tn.y)), vs(n, tn.z)));
+  }
+  if (dot(n, gn) < (R)0) n = vneg(n);
+  *sn = n;
+}
+
+/* Probe for the tests: the material and shading normal a hit on triangle
+ * `prim` at barycentrics (bu, bv) shades with, for a ray arriving along
+ * `dir` with cone width `cone_w` at the hit. out: shading normal xyz, base
+ * rgb, roughness, metallic, transmission, occlusion, emission rgb. */
+void API(olgi_textured_surface)(void* h, uint32_t prim, float bu, float bv, const float dir[3], float cone_w,
+                                float out[14]) {
+  const Scene* s = (const Scene*)h;
+  const uint32_t* tv = s->idx + 3ull * prim;
+  const float *pa = s->pos + 3ull * tv[0], *pb = s->pos + 3ull * tv[1], *pc = s->pos + 3ull * tv[2];
+  V e1 = v3((R)(pb[0] - pa[0]), (R)(pb[1] - pa[1]), (R)(pb[2] - pa[2]));
+  V e2 = v3((R)(pc[0] - pa[0]), (R)(pc[1] - pa[1]), (R)(pc[2] - pa[2]));
+  V gn = norm(cross(e1, e2));
+  R wx = ((R)1 - bu) - bv, wy = bu, wz = bv;
+  const float *n0 = s->nrm + 3ull * tv[0], *n1 = s->nrm + 3ull * tv[1], *n2 = s->nrm + 3ull * tv[2];
+  V sn = norm(vadd(vadd(vs(v3(n0[0], n0[1], n0[2]), wx), vs(v3(n1[0], n1[1], n1[2]), wy)),
+                   vs(v3(n2[0], n2[1], n2[2]), wz)));
+  if (dot(sn, gn) < (R)0) sn = vneg(sn);
+  (void)dir;
+  Mat m = load_mat(s, s->mid[prim]);
+  V em = emission_of(s, s->mid[prim]);
+  if (s->uvs && mat_textured(s, s->mid[prim])) tex_surface(s, s->mid[prim], prim, bu, bv, gn, cone_w, &m, &em, &sn);
+  float vals[14] = {(float)sn.x, (float)sn.y, (float)sn.z, (float)m.base.x, (float)m.base.y, (float)m.base.z,
+                    (float)m.rough, (float)m.metal, (float)m.trans, (float)m.occl, (float)em.x, (float)em.y,
+                    (float)em.z, 0.0f};
+  memcpy(out, vals, sizeof(vals));
+}
+
 static R medium_get(const R st[16], uint32_t i) { return st[i]; }
 
 /* One path; returns radiance (rgb) as floats of R.  counts[3] += rays. */
@@ -1020,6 +1314,7 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
   }
   V o = org, d = dir;
   R tmin = (R)0.001, tmax = (R)1e30;
+  R cone_w = 0, cone_s = 0; /* primary rays carry a zero cone (wavefront_generate.comp:153-155) */
   const uint32_t frame_sample = (frame << 8) | (sample & 255u);
   counts[0]++;
   /* This pass's history for the pixel: empty unless ReSTIR runs below. */
@@ -1056,6 +1351,12 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
     V n = entering ? sn : vneg(sn);
     Mat m = load_mat(s, s->mid[prim]);
     V em = emission_of(s, s->mid[prim]);
+    /* The ray cone at the hit (shadeRayCone :1217-1222). */
+    R cone_hit = cone_w + t * cone_s;
+    if (s->uvs && mat_textured(s, s->mid[prim])) {
+      tex_surface(s, s->mid[prim], prim, bu, bv, gn, cone_hit, &m, &em, &sn);
+      n = entering ? sn : vneg(sn);
+    }
     uint32_t depth = (flags >> 8) > 1u ? (flags >> 8) : 1u;
     uint32_t rs = rng;
     if (entering || m.two_sided > (R)0.5) {
@@ -1195,6 +1496,10 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
     }
     d = norm(nd);
     bounce = nb;
+    /* The continuation's cone: the width at this hit, the spread widened by
+     * the roughness after an opaque scatter (:1551, :1373). */
+    cone_w = cone_hit;
+    if (!(trans > (R)0.001)) cone_s = cone_s + m.rough * (R)0.25;
     flags = 1u | (depth << 8);
     if (trans <= (R)0.001) {
       flags |= 2u;

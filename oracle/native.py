@@ -43,6 +43,14 @@ def lib(mutant=0):
                 P_f32, P_f32, ctypes.c_uint32, P_u32, P_u32, ctypes.c_uint32, P_f32, ctypes.c_uint32,
                 P_f32, ctypes.c_uint32, P_f32, ctypes.c_uint32, ctypes.c_float]
             getattr(L, "olgi_scene_free" + sfx).argtypes = [ctypes.c_void_p]
+            getattr(L, "olgi_texture_sample" + sfx).argtypes = [
+                ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                ctypes.c_int, ctypes.c_float, ctypes.c_float, P_f32]
+            getattr(L, "olgi_env_radiance" + sfx).argtypes = [ctypes.c_void_p, P_f32, P_f32]
+            getattr(L, "olgi_textured_surface" + sfx).argtypes = [
+                ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, P_f32, ctypes.c_float, P_f32]
+            getattr(L, "olgi_scene_textures" + sfx).argtypes = [
+                ctypes.c_void_p, P_u32, ctypes.c_uint32, P_f32, ctypes.c_uint32, P_f32, P_f32, P_f32]
             getattr(L, "olgi_render" + sfx).argtypes = [
                 ctypes.c_void_p, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                 ctypes.c_uint32, P_u32, P_f32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P_f32, P_u64]
@@ -108,6 +116,43 @@ class OracleScene:
             _p(pos), _p(nrm), len(pos), _p(idx, ctypes.c_uint32), _p(mid, ctypes.c_uint32), len(idx),
             _p(mat), len(mat), _p(ana), packed.analytic_light_count, _p(area), packed.area_light_count,
             packed.area_light_weight)
+        words = getattr(packed, "texture_words", None)
+        if words is not None:
+            tex = [np.ascontiguousarray(words, np.uint32)]
+            bind = getattr(packed, "bindings", None)
+            uvs = getattr(packed, "uvs", None)
+            tex.append(np.ascontiguousarray(bind, np.float32) if bind is not None else np.zeros((1, 2, 4), np.float32))
+            if uvs is not None:
+                tex += [np.ascontiguousarray(uvs, np.float32), np.ascontiguousarray(packed.tangents, np.float32),
+                        np.ascontiguousarray(packed.uv_density, np.float32)]
+            self._arrays += tex
+            getattr(L, "olgi_scene_textures" + self.sfx)(
+                self.handle, _p(tex[0], ctypes.c_uint32), len(tex[0]), _p(tex[1]), len(tex[1]) if bind is not None else 0,
+                _p(tex[2]) if uvs is not None else None, _p(tex[3]) if uvs is not None else None,
+                _p(tex[4]) if uvs is not None else None)
+
+    def texture_sample(self, binding, uv0, uv1=(0.0, 0.0), srgb=False, footprint0=0.0, footprint1=0.0):
+        """RGBA of material-texture binding ``binding`` at the given uvs."""
+        out = np.zeros(4, np.float32)
+        getattr(lib(self.mutant), "olgi_texture_sample" + self.sfx)(
+            self.handle, float(binding), float(uv0[0]), float(uv0[1]), float(uv1[0]), float(uv1[1]), int(srgb),
+            float(footprint0), float(footprint1), _p(out))
+        return out
+
+    def textured_surface(self, prim, bu, bv, cone_width=0.0):
+        """dict of the shading normal and textured material of a hit."""
+        out = np.zeros(14, np.float32)
+        d = np.zeros(3, np.float32)
+        getattr(lib(self.mutant), "olgi_textured_surface" + self.sfx)(
+            self.handle, int(prim), float(bu), float(bv), _p(d), float(cone_width), _p(out))
+        return {"normal": out[0:3], "base": out[3:6], "roughness": out[6], "metallic": out[7],
+                "transmission": out[8], "occlusion": out[9], "emission": out[10:13]}
+
+    def environment_radiance(self, direction):
+        d = np.ascontiguousarray(direction, np.float32)
+        out = np.zeros(3, np.float32)
+        getattr(lib(self.mutant), "olgi_env_radiance" + self.sfx)(self.handle, _p(d), _p(out))
+        return out
 
     def close(self):
         if self.handle:

@@ -38,6 +38,8 @@ class SceneDesc(ctypes.Structure):
         ("analytic_light_count", c_u32), ("analytic_lights", P_f32),
         ("area_light_count", c_u32), ("area_lights", P_f32),
         ("area_light_weight", c_f32), ("leaf_size", c_u32), ("builder", c_u32),
+        ("texture_word_count", c_u32), ("texture_words", P_u32), ("binding_count", c_u32),
+        ("bindings", P_f32), ("uvs", P_f32), ("tangents", P_f32), ("uv_density", P_f32),
     ]
 
 
@@ -55,7 +57,7 @@ class InstancedDesc(ctypes.Structure):
         ("analytic_light_count", c_u32), ("analytic_lights", P_f32),
         ("area_light_count", c_u32), ("area_lights", P_f32),
         ("area_light_weight", c_f32), ("leaf_size", c_u32), ("builder", c_u32),
-        ("world_spheres", P_f32),
+        ("world_spheres", P_f32), ("texture_word_count", c_u32), ("texture_words", P_u32),
     ]
 
 

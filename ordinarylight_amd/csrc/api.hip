@@ -115,6 +115,108 @@ uint64_t fnv1a(const void* data, size_t bytes, uint64_t h) {
   return h;
 }
 
+bool material_textured(const float* record) { return record[8 * 4 + 3] == 1.0f; }
+
+// Checks a texture word array (header, metadata, both mip chains inside the
+// array) so that every fetch the sampler can make stays in bounds.
+void validate_texture_words(const uint32_t* w, uint32_t n) {
+  require(w != nullptr && n >= 1, "texture words missing");
+  const uint64_t count = w[0];
+  require(1 + 8 * count <= n, "texture word header exceeds the array");
+  for (uint64_t k = 0; k < count; ++k) {
+    const uint32_t* m = w + 1 + 8 * k;
+    const uint64_t width = m[2], height = m[3], levels = m[5];
+    require(width >= 1 && height >= 1 && width <= 65536 && height <= 65536, "texture size out of range");
+    uint64_t need = 0, lw = width, lh = height, expect = 1;
+    while (lw > 1 || lh > 1) {
+      lw = std::max<uint64_t>((lw + 1) / 2, 1);
+      lh = std::max<uint64_t>((lh + 1) / 2, 1);
+      ++expect;
+    }
+    require(levels == expect, "texture mip level count does not match its size");
+    lw = width;
+    lh = height;
+    for (uint64_t l = 0; l < levels; ++l) {
+      need += lw * lh;
+      lw = std::max<uint64_t>((lw + 1) / 2, 1);
+      lh = std::max<uint64_t>((lh + 1) / 2, 1);
+    }
+    require((uint64_t)m[0] + need <= n && (uint64_t)m[1] + need <= n, "texture mip chain exceeds the word array");
+  }
+}
+
+// Texture words (a flat scene's material textures and an image environment,
+// or a two-level scene's image environment) and, when some material has
+// bindings, the bindings and per-vertex inputs.  Returns the texture count.
+uint32_t upload_textures(olhip_scene* sc, uint32_t word_count, const uint32_t* words, uint32_t material_count,
+                         const float* materials, const olhip_scene_desc* flat, hipStream_t s) {
+  DScene& v = sc->view;
+  uint32_t tex_count = 0;
+  bool textured = false;
+  for (uint32_t m = 0; m < material_count; ++m) textured = textured || material_textured(materials + 48ull * m);
+  if (word_count) {
+    validate_texture_words(words, word_count);
+    tex_count = words[0];
+    upload(sc->tex_words, words, word_count, s);
+  }
+  v.tex_words = word_count ? sc->tex_words.as<uint32_t>() : nullptr;
+  v.textured = textured ? 1u : 0u;
+  v.bindings = nullptr;
+  v.uvs = v.tangents = nullptr;
+  v.uv_density = nullptr;
+  v.positions = sc->positions.as<float4>();
+  if (!textured) return tex_count;
+  require(flat != nullptr, "materials with texture bindings need a flat scene");
+  const olhip_scene_desc* d = flat;
+  require(d->binding_count >= 1 && d->bindings, "texture bindings missing");
+  require(d->uvs && d->tangents && d->uv_density, "textured materials need uvs, tangents and uv density");
+  for (uint32_t b = 0; b < d->binding_count; ++b) {
+    const float t = d->bindings[8ull * b];
+    require(t < 0.0f || (uint32_t)t < tex_count, "binding texture index out of range");
+  }
+  for (uint32_t m = 0; m < material_count; ++m) {
+    const float* r = materials + 48ull * m;
+    if (!material_textured(r)) continue;
+    for (const float b : {r[16], r[17], r[18], r[19], r[21], r[23]})
+      require(b < 0.0f || (uint32_t)b < d->binding_count, "material texture binding out of range");
+  }
+  upload(sc->bindings, reinterpret_cast<const float4*>(d->bindings), 2ull * d->binding_count, s);
+  upload(sc->uvs, reinterpret_cast<const float4*>(d->uvs), d->vertex_count, s);
+  upload(sc->tangents, reinterpret_cast<const float4*>(d->tangents), d->vertex_count, s);
+  upload(sc->uv_density, d->uv_density, d->triangle_count, s);
+  v.bindings = sc->bindings.as<float4>();
+  v.uvs = sc->uvs.as<float4>();
+  v.tangents = sc->tangents.as<float4>();
+  v.uv_density = sc->uv_density.as<float>();
+  return tex_count;
+}
+
+// Environment light of the analytic records (type 3): constant radiance, or
+// an image when row 3 names a texture of `texture_count`.
+void set_environment(DScene& v, const float* analytic, uint32_t count, uint32_t texture_count) {
+  v.env_valid = 0;
+  v.env_texture = -1;
+  for (uint32_t l = 0; l < count; ++l) {
+    const float* L = analytic + 16ull * l;
+    if ((int)(L[3] + 0.5f) != 3) continue;
+    v.env_valid = 1;
+    v.env_r = (1.0f * L[8]) * L[11];
+    v.env_g = (1.0f * L[9]) * L[11];
+    v.env_b = (1.0f * L[10]) * L[11];
+    v.env_color_r = L[8];
+    v.env_color_g = L[9];
+    v.env_color_b = L[10];
+    v.env_intensity = L[11];
+    if (L[12] >= 0.0f) {
+      require((uint32_t)L[12] < texture_count, "environment texture index out of range");
+      v.env_texture = (int32_t)L[12];
+      v.env_rotation = L[13];
+      v.env_log_range = L[14];
+    }
+    return;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -271,18 +373,10 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     v.analytic_count = d->analytic_light_count;
     v.area_count = d->area_light_count;
     v.area_weight = d->area_light_weight;
-    v.env_valid = 0;
-    for (uint32_t l = 0; l < d->analytic_light_count; ++l) {
-      const float* L = d->analytic_lights + 16ull * l;
-      if ((int)(L[3] + 0.5f) == 3) {
-        require(L[12] < 0.0f, "image environments are not supported by the GI estimator");
-        v.env_valid = 1;
-        v.env_r = (1.0f * L[8]) * L[11];
-        v.env_g = (1.0f * L[9]) * L[11];
-        v.env_b = (1.0f * L[10]) * L[11];
-        break;
-      }
-    }
+    const uint32_t tex_count = upload_textures(sc.get(), d->texture_word_count, d->texture_words, d->material_count,
+                                               d->materials, d, s);
+    set_environment(v, d->analytic_lights, d->analytic_light_count, tex_count);
+    OL_HIP_CHECK(hipStreamSynchronize(s));
     sc->point_count = d->analytic_light_count;
     *out = sc.release();
   });
@@ -317,18 +411,6 @@ void upload_shading(olhip_scene* sc, uint32_t material_count, const float* mater
   v.analytic_count = analytic_count;
   v.area_count = area_count;
   v.area_weight = area_weight;
-  v.env_valid = 0;
-  for (uint32_t l = 0; l < analytic_count; ++l) {
-    const float* L = analytic_lights + 16ull * l;
-    if ((int)(L[3] + 0.5f) == 3) {
-      require(L[12] < 0.0f, "image environments are not supported by the GI estimator");
-      v.env_valid = 1;
-      v.env_r = (1.0f * L[8]) * L[11];
-      v.env_g = (1.0f * L[9]) * L[11];
-      v.env_b = (1.0f * L[10]) * L[11];
-      break;
-    }
-  }
   sc->point_count = analytic_count;
 }
 
@@ -451,6 +533,9 @@ int olhip_scene_create_instanced(olhip_ctx* ctx, const olhip_instanced_desc* d, 
     upload(sc->vidx, vidx.data(), vidx.size(), s);
     upload_shading(sc.get(), d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
                    d->area_light_count, d->area_lights, d->area_light_weight, s);
+    set_environment(sc->view, d->analytic_lights, d->analytic_light_count,
+                    upload_textures(sc.get(), d->texture_word_count, d->texture_words, d->material_count,
+                                    d->materials, nullptr, s));
     sc->leaf_size = d->leaf_size;
     sc->vertex_offsets.assign(d->vertex_offsets, d->vertex_offsets + R + 1);
     sc->triangle_offsets.assign(d->triangle_offsets, d->triangle_offsets + R + 1);
@@ -549,6 +634,9 @@ int olhip_scene_update(olhip_ctx* ctx, olhip_scene* sc, const olhip_scene_desc* 
     sc->has_transmission = false;
     upload_shading(sc, d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
                    d->area_light_count, d->area_lights, d->area_light_weight, s);
+    set_environment(sc->view, d->analytic_lights, d->analytic_light_count,
+                    upload_textures(sc, d->texture_word_count, d->texture_words, d->material_count, d->materials,
+                                    d, s));
     sc->point_count = d->analytic_light_count;
     OL_HIP_CHECK(hipStreamSynchronize(s));
   });
@@ -615,6 +703,9 @@ int olhip_scene_update_instanced(olhip_ctx* ctx, olhip_scene* sc, const olhip_in
     sc->has_transmission = false;
     upload_shading(sc, d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
                    d->area_light_count, d->area_lights, d->area_light_weight, s);
+    set_environment(sc->view, d->analytic_lights, d->analytic_light_count,
+                    upload_textures(sc, d->texture_word_count, d->texture_words, d->material_count, d->materials,
+                                    nullptr, s));
     sc->point_count = d->analytic_light_count;
     build_top_level(sc, d, s);
   });

@@ -32,6 +32,7 @@
 struct olhip_scene {
   olhip_ctx* ctx = nullptr;
   DevBuf positions, normals, vidx, materials, analytic, area, area_guide;
+  DevBuf tex_words, bindings, uvs, tangents, uv_density;  // textured scenes
   BvhOutput bvh;  // the scene's BVH, or the top-level BVH over instances
   // Two-level scenes: bottom-level BVHs per resource, instance and BLAS tables.
   std::vector<std::unique_ptr<BvhOutput>> blas;
@@ -146,8 +147,9 @@ class Engine {
   // it accumulates or generates (bits 1.. of the key).  Absent work is not
   // launched; each shape is captured once as a hipGraph and replayed.
   static uint32_t step_key(uint32_t in, uint32_t kinds, bool acc, bool gen) {
-    return in | (kinds << 1) | (acc ? 16u : 0u) | (gen ? 32u : 0u);
+    return in | (kinds << 1) | (acc ? kKeyAcc : 0u) | (gen ? kKeyGen : 0u);
   }
+  static constexpr uint32_t kKeyAcc = 32u, kKeyGen = 64u;
   hipGraphExec_t graph_for(uint32_t key);
   void clear_graphs();
   void launch_step(uint32_t key, bool timed, int buf);
@@ -197,9 +199,10 @@ class Engine {
   int max_depth_ = 1;
   bool stk_needed_ = false;
   bool restir_needed_ = false;
+  bool tex_needed_ = false;
 
   // Device state.
-  DevBuf T_, L_, M_, stk_, rsamp_, qo_[2], qd_[2], qp_[2], hit_, hit_inst_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
+  DevBuf T_, L_, M_, stk_, rsamp_, tex_, cone_, qo_[2], qd_[2], qp_[2], hit_, hit_inst_, gT_, gI_, eo_, ed_, ec_, ctr_, work_, jobs_dev_,
       list_dev_, spill_;
   uint64_t slots_ = 0, eslots_ = 0;
   Wave wave_{};

@@ -174,6 +174,10 @@ void Engine::ensure_buffers() {
   ec_.reserve(16 * eslots_);
   if (stk_needed_) stk_.reserve(64 * slots_);
   if (restir_needed_) rsamp_.reserve(48 * slots_);
+  if (tex_needed_) {
+    tex_.reserve(64 * slots_);
+    cone_.reserve(8 * slots_);
+  }
   Wave w{};
   w.jobs = jobs_dev_.as<JobDev>();
   w.list = list_dev_.as<StepList>();
@@ -182,6 +186,8 @@ void Engine::ensure_buffers() {
   w.M = M_.as<uint4>();
   w.stk = stk_needed_ ? stk_.as<float4>() : nullptr;
   w.rsamp = restir_needed_ ? rsamp_.as<float4>() : nullptr;
+  w.tex = tex_needed_ ? tex_.as<float4>() : nullptr;
+  w.cone = tex_needed_ ? cone_.as<float2>() : nullptr;
   for (int q = 0; q < 2; ++q) {
     w.qo[q] = qo_[q].as<float4>();
     w.qd[q] = qd_[q].as<float4>();
@@ -385,6 +391,10 @@ bool Engine::admit(Job& j, int slot) {
         restir_needed_ = true;
         ensure_buffers();
       }
+      if (d.scene->view.textured && !tex_needed_) {
+        tex_needed_ = true;
+        ensure_buffers();
+      }
       const int depth = (int)d.scene->bvh.depth;
       if (depth > max_depth_) {
         max_depth_ = depth;
@@ -451,12 +461,13 @@ void Engine::drop_chunks(Job& j) {
 
 void Engine::launch_step(uint32_t key, bool timed, int buf) {
   hipStream_t s = ctx_->stream;
-  const uint32_t in = key & 1u, kinds = (key >> 1) & 7u;
-  if (key & 16u) wave_accumulate(wave_, ctx_->cu_count, s);
-  if (key & 32u) wave_generate(wave_, in, ctx_->cu_count, s);
+  const uint32_t in = key & 1u, kinds = (key >> 1) & 15u;
+  const bool gen = (key & kKeyGen) != 0u;
+  if (key & kKeyAcc) wave_accumulate(wave_, ctx_->cu_count, s);
+  if (gen) wave_generate(wave_, in, ctx_->cu_count, s);
   if (!timed) {
     wave_trace(launch_, wave_, in, kinds, s);
-    wave_shade(launch_, wave_, in, kinds, (key & 32u) != 0u, s);
+    wave_shade(launch_, wave_, in, kinds, gen, s);
   } else {
     // Read when the step is retired (no synchronisation inside the step).
     hipEvent_t* ev = prof_ev_[buf];
@@ -465,7 +476,7 @@ void Engine::launch_step(uint32_t key, bool timed, int buf) {
     OL_HIP_CHECK(hipEventRecord(ev[0], s));
     wave_trace(launch_, wave_, in, kinds, s);
     OL_HIP_CHECK(hipEventRecord(ev[1], s));
-    wave_shade(launch_, wave_, in, kinds, (key & 32u) != 0u, s);
+    wave_shade(launch_, wave_, in, kinds, gen, s);
     OL_HIP_CHECK(hipEventRecord(ev[2], s));
   }
   wave_epilogue(wave_, in, s);
@@ -632,6 +643,7 @@ bool Engine::launch_next() {
       traced = true;
       kinds |= j.desc.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
       if (j.desc.settings.restir_candidates) kinds |= STEP_RESTIR;
+      if (j.desc.scene->view.textured) kinds |= STEP_TEXTURED;
     }
     totals_.resident_jobs = 0;
     for (uint32_t k = 0; k < kMaxJobs; ++k) totals_.resident_jobs += slot_job_[k] != 0;

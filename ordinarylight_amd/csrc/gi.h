@@ -145,6 +145,13 @@ struct Wave {
   float4* L;    // radiance rgb, previous BSDF pdf
   uint4* M;     // pixel, frame << 8 | sample, rng state, flags | medium depth << 8
   float4* stk;  // 4 x float4 medium IOR stack per path (null when no resident job transmits)
+  // Textured scenes (null when no resident job samples textures): per input
+  // queue slot, the hit's material after its textures (base colour and
+  // roughness, emission and metallic, transmission and resolved occlusion)
+  // and its shading normal; per path, the ray cone (width, spread) of the
+  // path's next ray.
+  float4* tex;
+  float2* cone;
   float4* qo[2];
   float4* qd[2];
   uint32_t* qp[2];
@@ -186,7 +193,7 @@ int stack_depth_for(uint32_t bvh_depth);
 // shadow groups this step's shading emits.
 // `kinds` says which resident work a step holds (STEP_* bits); the launches
 // for absent kinds are left out (their chunk totals stay zero).
-constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u;
+constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u, STEP_TEXTURED = 8u;
 // Fills b.shade from the hierarchy's leaf triangles, the triangles' vertex
 // indices (material in .w when `material`) and vertex normals.
 void build_shade_records(BvhOutput& b, const uint4* vidx, const float4* normals, uint32_t count, bool material,

@@ -990,6 +990,152 @@ OL_DEV void resolve_surface(const Wave& W, const DScene& sc, uint32_t i, uint32_
   nrm = entering ? sn : -sn;
 }
 
+// ---- textured surfaces ------------------------------------------------------------------------
+struct f2 {
+  float x, y;
+};
+OL_DEV f2 mk2(float x, float y) { return f2{x, y}; }
+
+// sqrt(uv area / world area) of a triangle (shadeTriangleUvDensity, :872-879).
+OL_DEV float uv_density(f3 a, f3 b, f3 c, f2 ua, f2 ub, f2 uc) {
+  const f2 e1 = mk2(ub.x - ua.x, ub.y - ua.y), e2 = mk2(uc.x - ua.x, uc.y - ua.y);
+  const float uv_area = fabsf(e1.x * e2.y - e1.y * e2.x);
+  const float world_area = length3(cross3(b - a, c - a));
+  return sqrtf(uv_area / fmaxf(world_area, 1e-8f));
+}
+
+// Tangent frame from a triangle's uv1 gradients (shadeTriangleTangent, :889-913).
+OL_DEV float4 triangle_tangent(f3 a, f3 b, f3 c, f2 ua, f2 ub, f2 uc, f3 n) {
+  const f3 ea = b - a, eb = c - a;
+  const f2 da = mk2(ub.x - ua.x, ub.y - ua.y), db = mk2(uc.x - ua.x, uc.y - ua.y);
+  const float det = da.x * db.y - da.y * db.x;
+  if (fabsf(det) < 1e-8f) {
+    f3 fb = cross3(n, mk3(0.0f, 1.0f, 0.0f));
+    if (fabsf(n.z) < 0.999f) fb = cross3(n, mk3(0.0f, 0.0f, 1.0f));
+    return mk4(normalize3(fb), 1.0f);
+  }
+  const float inv = 1.0f / det;
+  const f3 t = normalize3((ea * db.y - eb * da.y) * inv);
+  const f3 bt = normalize3((eb * da.x - ea * db.x) * inv);
+  return mk4(t, dot3(cross3(n, t), bt) < 0.0f ? -1.0f : 1.0f);
+}
+
+// Material textures and normal map of a hit on a flat scene
+// (shadeResolveSurface's textured branch, :971-993; shadeApplyMaterialTextures
+// :915-934; shadeApplyNormalTexture :936-955).  `sn` is the shading normal
+// on the geometric side; it returns the mapped one, also on that side.
+OL_DEV void texture_surface(const DScene& sc, uint32_t mat_id, uint32_t prim, float bu, float bv, f3 gn,
+                            float cone_width, Mat& mat, f3& sn) {
+  const float4 ti = sc.materials[12u * mat_id + 4u];
+  const uint4 vi = sc.tri_vidx[prim];
+  const float wx = (1.0f - bu) - bv, wy = bu, wz = bv;
+  const float4 qa = sc.uvs[vi.x], qb = sc.uvs[vi.y], qc = sc.uvs[vi.z];
+  const float u0 = ((qa.x * wx) + (qb.x * wy)) + (qc.x * wz);
+  const float v0 = ((qa.y * wx) + (qb.y * wy)) + (qc.y * wz);
+  float u1 = 0.0f, v1 = 0.0f, fp1 = 0.0f;
+  const float fp0 = cone_width * sc.uv_density[prim];
+  const f3 a = rgb(sc.positions[vi.x]), b = rgb(sc.positions[vi.y]), c = rgb(sc.positions[vi.z]);
+  const f2 a1 = mk2(qa.z, qa.w), b1 = mk2(qb.z, qb.w), c1 = mk2(qc.z, qc.w);
+  if (mat.id.z - floorf(mat.id.z) > 0.125f) {
+    u1 = ((qa.z * wx) + (qb.z * wy)) + (qc.z * wz);
+    v1 = ((qa.w * wx) + (qb.w * wy)) + (qc.w * wz);
+    fp1 = cone_width * uv_density(a, b, c, a1, b1, c1);
+  }
+  const float tr = sample_material_texture(sc, mat.tp.w, u0, v0, u1, v1, false, fp0, fp1).x;
+  mat.at.w = mat.at.w * tr;
+  const float4 base = sample_material_texture(sc, ti.x, u0, v0, u1, v1, true, fp0, fp1);
+  mat.br.x = mat.br.x * base.x;
+  mat.br.y = mat.br.y * base.y;
+  mat.br.z = mat.br.z * base.z;
+  const float4 emit = sample_material_texture(sc, ti.z, u0, v0, u1, v1, true, fp0, fp1);
+  mat.em.x = mat.em.x * emit.x;
+  mat.em.y = mat.em.y * emit.y;
+  mat.em.z = mat.em.z * emit.z;
+  if (!(mat.at.w > 0.001f)) {
+    const float4 mr = sample_material_texture(sc, ti.y, u0, v0, u1, v1, false, fp0, fp1);
+    mat.br.w = mat.br.w * mr.y;
+    mat.em.w = mat.em.w * mr.z;
+    const float occ = sample_material_texture(sc, mat.tp.y, u0, v0, u1, v1, false, fp0, fp1).x;
+    mat.tp.w = mixf(1.0f, occ, mat.tp.z);
+  }
+  // Interpolated vertex tangents, or the uv1 triangle frame when the normal
+  // map reads uv1.
+  const float4 ta = sc.tangents[vi.x], tb = sc.tangents[vi.y], tc = sc.tangents[vi.z];
+  float4 tan = make_float4(((ta.x * wx) + (tb.x * wy)) + (tc.x * wz), ((ta.y * wx) + (tb.y * wy)) + (tc.y * wz),
+                           ((ta.z * wx) + (tb.z * wy)) + (tc.z * wz), ((ta.w * wx) + (tb.w * wy)) + (tc.w * wz));
+  const int nb = (int)ti.w;
+  if (nb >= 0 && sc.bindings[2 * nb].w > 0.5f) tan = triangle_tangent(a, b, c, a1, b1, c1, sn);
+  if (nb >= 0) {
+    const float4 s4 = sample_material_texture(sc, ti.w, u0, v0, u1, v1, false, fp0, fp1);
+    f3 tn = mk3(s4.x * 2.0f - 1.0f, s4.y * 2.0f - 1.0f, s4.z * 2.0f - 1.0f);
+    tn.x = tn.x * mat.tp.x;
+    tn.y = tn.y * mat.tp.x;
+    tn = normalize3(tn);
+    const f3 td = rgb(tan);
+    const f3 t = normalize3(td - sn * dot3(sn, td));
+    const f3 bt = cross3(sn, t) * tan.w;
+    const float4 r0 = sc.bindings[2 * nb], r1 = sc.bindings[2 * nb + 1];
+    const float cs = r0.y, sns = r0.z;
+    const float ix = 1.0f / r1.z, iy = 1.0f / r1.w;
+    const f3 tt = normalize3(t * (cs * ix) + bt * ((-sns) * iy));
+    const f3 tbn = normalize3(t * (sns * ix) + bt * (cs * iy));
+    sn = normalize3(((tt * tn.x) + (tbn * tn.y)) + (sn * tn.z));
+  }
+  if (dot3(sn, gn) < 0.0f) sn = -sn;
+}
+
+// Texture pass of the jobs whose scenes have textured materials, before the
+// ReSTIR pass and shade: for every hit, the material after its textures and
+// the shading normal (per input queue slot), and the cone of the path's next
+// ray (shadeRayCone :1217-1222 at the hit; the spread widened by the
+// roughness after an opaque scatter, :1551).  Primary rays start with a zero
+// cone, as wavefront_generate.comp:153-155 leaves it.
+__global__ void __launch_bounds__(256) k_texture(Wave W, uint32_t in, uint32_t gen) {
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < flat_chunks; c += waves) {
+    const ChunkLoc k = locate_chunk(W.ctr + C_RP, counts, W.jobs, uniform(c));
+    const uint32_t job = uniform(k.job);
+    const JobDev& J = W.jobs[job];
+    const DScene& sc = J.P.sc;
+    if (!sc.textured) continue;
+    if (lane_id() >= k.count) continue;
+    const uint32_t i = k.first + lane_id();
+    const float4 hv = W.hit[i];
+    const uint32_t slot = __float_as_uint(hv.w);
+    if (slot == OL_NONE) continue;
+    const uint32_t path = W.qp[in][i];
+    const bool fresh = gen && ((W.list->fresh >> job) & 1ull);
+    const float2 cone = fresh ? make_float2(0.0f, 0.0f) : W.cone[path];
+    const float4 ro = W.qo[in][i], rd = W.qd[in][i];
+    f3 hitp, gn, nrm, cw;
+    uint32_t prim, mat_id;
+    bool entering;
+    resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
+    f3 sn = entering ? nrm : -nrm;
+    Mat mat = load_mat(sc.materials, mat_id);
+    const float width = cone.x + hv.x * cone.y;
+    if (sc.materials[12u * mat_id + 8u].w == 1.0f) texture_surface(sc, mat_id, prim, hv.y, hv.z, gn, width, mat, sn);
+    float4* out = W.tex + 4u * i;
+    out[0] = mat.br;
+    out[1] = mat.em;
+    out[2] = make_float4(mat.at.w, mat.tp.w, 0.0f, 0.0f);
+    out[3] = mk4(sn, 0.0f);
+    W.cone[path] = make_float2(width, mat.at.w > 0.001f ? cone.y : cone.y + mat.br.w * 0.25f);
+  }
+}
+
+// A textured job's hit: the material after its textures (from k_texture).
+OL_DEV void textured_material(const Wave& W, uint32_t i, Mat& mat) {
+  const float4* r = W.tex + 4u * i;
+  mat.br = r[0];
+  mat.em = r[1];
+  const float4 a = r[2];
+  mat.at.w = a.x;
+  mat.tp.w = a.y;
+}
+
 // ReSTIR pass at primary hits, before shade: for each bounce-0 path of a
 // ReSTIR job, clears the pixel's stored reservoir and surface, and where
 // shade will sample lights (an opaque hit below the bounce limit, chosen for
@@ -1028,7 +1174,12 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t ge
     uint32_t prim, mat_id;
     bool entering;
     resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
-    const Mat mat = load_mat(sc.materials, mat_id);
+    Mat mat = load_mat(sc.materials, mat_id);
+    if (sc.textured) {
+      textured_material(W, i, mat);
+      const f3 sn = rgb(W.tex[4u * i + 3u]);
+      nrm = entering ? sn : -sn;
+    }
     if (mat.at.w > 0.001f) continue;
     if (!select_secondary_nee(clampf(P.nee_probability, 1e-6f, 1.0f), M.x, M.y, 1u)) continue;
     uint32_t rs = M.z;
@@ -1125,7 +1276,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           J.geo[M.x] = make_float4(t, gn.x, gn.y, gn.z);
           J.prim[M.x] = prim;
         }
-        const Mat mat = load_mat(sc.materials, mat_id);
+        Mat mat = load_mat(sc.materials, mat_id);
+        if (sc.textured) {
+          textured_material(W, i, mat);
+          const f3 sn = rgb(W.tex[4u * i + 3u]);
+          nrm = entering ? sn : -sn;
+        }
         nee_mat = mat_id;
         uint32_t rs = M.z;
         uint32_t depth = (M.w >> 8) > 1u ? (M.w >> 8) : 1u;
@@ -1315,7 +1471,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           uint32_t kk = 0;
           uint32_t rs2 = nee_rng;
           uint32_t en;
-          const Mat mat = load_mat(sc.materials, nee_mat);
+          Mat mat = load_mat(sc.materials, nee_mat);
+          if (sc.textured) textured_material(W, i, mat);
           restir_nee(P, single, rs2, hitp, nrm, mat, incoming, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
             W.ed[eslot + kk] = mk4(s.dir, s.tmax);
@@ -1329,7 +1486,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         } else {
           uint32_t kk = 0, an, en;
           uint32_t rs2 = nee_rng;
-          const Mat mat = load_mat(sc.materials, nee_mat);
+          Mat mat = load_mat(sc.materials, nee_mat);
+          if (sc.textured) textured_material(W, i, mat);
           for_each_nee(P, rs2, hitp, nrm, mat, incoming, an, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
             W.ed[eslot + kk] = mk4(s.dir, s.tmax);
@@ -1538,6 +1696,10 @@ void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, h
 }
 
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, bool gen, hipStream_t st) {
+  if (w.tex && (kinds & STEP_TEXTURED)) {
+    static int tocc = resident_blocks(k_texture, 256);
+    hipLaunchKernelGGL(k_texture, dim3((unsigned)(l.cu_count * tocc)), dim3(256), 0, st, w, in, gen ? 1u : 0u);
+  }
   if (w.rsamp && (kinds & STEP_RESTIR) && gen) {
     static int rocc = resident_blocks(k_restir, 256);
     hipLaunchKernelGGL(k_restir, dim3((unsigned)(l.cu_count * rocc)), dim3(256), 0, st, w, in, 1u);

@@ -183,5 +183,18 @@ struct DScene {
   // first emitter whose cumulative weight reaches j / kAreaGuide (clamped to
   // the last), so a selection in bucket j lies between entries j and j + 1.
   const uint32_t* area_guide;
+  // Textures (null / -1 without): the texture word array, binding records
+  // (2 float4 each), per-vertex uv0.xy uv1.xy and world tangents, per-triangle
+  // uv0 density and world positions (flat scenes only).
+  const uint32_t* tex_words;
+  const float4* bindings;
+  const float4* uvs;
+  const float4* tangents;
+  const float* uv_density;
+  const float4* positions;
+  uint32_t textured;       // some material has texture bindings
+  int32_t env_texture;     // image environment's texture, or -1
+  float env_rotation, env_log_range;
+  float env_color_r, env_color_g, env_color_b, env_intensity;
 };
 constexpr uint32_t kAreaGuide = 4096;

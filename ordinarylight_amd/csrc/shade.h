@@ -7,6 +7,7 @@
 #pragma once
 
 #include "gi.h"
+#include "texture.h"
 #include "trace.h"
 
 namespace olshade {
@@ -152,6 +153,7 @@ OL_DEV f3 analytic_environment(f3 d) {
 
 OL_DEV f3 environment_radiance(const DScene& sc, f3 d) {
   if (!sc.env_valid) return analytic_environment(d);
+  if (sc.env_texture >= 0) return environment_image(sc, d);
   return mk3(sc.env_r, sc.env_g, sc.env_b);
 }
 

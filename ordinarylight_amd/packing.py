@@ -22,14 +22,112 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import textures as tx
 from .scene import material_record
 
-_TEXTURE_FIELDS = (
-    "base_color_texture", "metallic_roughness_texture", "emissive_texture",
-    "normal_texture", "occlusion_texture", "transmission_texture",
-    "thickness_texture", "clearcoat_texture", "sheen_texture",
-    "anisotropy_texture", "subsurface_texture",
+# (texture, transform) fields in the reference's binding order
+# (Scene.textures / texture_bindings, scene/_core.py:2927-2976); the last four
+# share the base-colour transform there.  Only the first six are sampled by
+# the GI shading; the rest keep binding indices equal to the reference's.
+_BINDING_SLOTS = (
+    ("base_color_texture", "base_color_transform"),
+    ("metallic_roughness_texture", "metallic_roughness_transform"),
+    ("emissive_texture", "emissive_transform"),
+    ("normal_texture", "normal_transform"),
+    ("occlusion_texture", "occlusion_transform"),
+    ("transmission_texture", "transmission_transform"),
+    ("thickness_texture", "thickness_transform"),
+    ("clearcoat_texture", "base_color_transform"),
+    ("sheen_texture", "base_color_transform"),
+    ("anisotropy_texture", "base_color_transform"),
+    ("subsurface_texture", "base_color_transform"),
 )
+_TEXTURE_FIELDS = tuple(name for name, _ in _BINDING_SLOTS)
+# Material record word (row 8 .w, after the sheen colour: a padding word of the reference record)
+# that marks a material with texture bindings for the device.
+TEXTURED_FLAG = (8, 3)
+
+
+def _slot_pairs(material):
+    for tex_name, tf_name in _BINDING_SLOTS:
+        tex = getattr(material, tex_name, None)
+        if tex is not None:
+            yield tex, getattr(material, tf_name)
+
+
+def is_textured(material):
+    return any(True for _ in _slot_pairs(material))
+
+
+class _Bindings:
+    """Textures and (texture, transform) bindings in first-use order."""
+
+    def __init__(self):
+        self.textures, self.bindings = [], []
+
+    def add(self, material):
+        for tex, tf in _slot_pairs(material):
+            if all(tex is not t for t in self.textures):
+                self.textures.append(tex)
+            if all(tex is not t or tf != f for t, f in self.bindings):
+                self.bindings.append((tex, tf))
+
+    def index(self, material, tex_name, tf_name):
+        tex = getattr(material, tex_name, None)
+        if tex is None:
+            return -1.0
+        tf = getattr(material, tf_name)
+        return float(next(i for i, (t, f) in enumerate(self.bindings) if t is tex and f == tf))
+
+
+def _material_block(material, count, faces, bindings):
+    """Material record(s): one, or one per triangle over face materials,
+    with texture binding indices (Scene.triangle_material_data,
+    scene/_core.py:2816-2924).  Untextured records hold the resolved
+    occlusion factor 1 in row 5 .w (the value the reference's textured path
+    computes when no occlusion map is bound); textured ones keep the
+    transmission binding there, as the reference does, and set the flag."""
+    block = faces.records(material) if faces is not None else material_record(material)[None]
+    if not is_textured(material):
+        block[:, 5, 3] = 1.0
+        return block
+    idx = lambda slot: bindings.index(material, *_BINDING_SLOTS[slot])  # noqa: E731
+    uses_uv1 = any(tf.texcoord_set == 1 for _, tf in _slot_pairs(material))
+    block[:, 3, 2] = block[:, 3, 2] + (0.25 if uses_uv1 else 0.0)
+    block[:, 4] = (idx(0), idx(1), idx(2), idx(3))
+    block[:, 5, 1] = idx(4)
+    block[:, 5, 3] = idx(5)
+    block[:, 10] = (idx(7), idx(8), idx(9), idx(10))
+    block[:, 11, 0] = idx(6)
+    block[:, TEXTURED_FLAG[0], TEXTURED_FLAG[1]] = 1.0
+    return block
+
+
+def _environment_words(environment, analytic, analytic_count, texture_list):
+    """Texture words for ``texture_list`` plus an image environment's texture
+    (appended last, as Scene.textures does); the environment record's row 3
+    becomes (texture index, rotation, log range, 0) (analytic_light_data,
+    scene/_core.py:2613-2629).  None when there is nothing to sample."""
+    env_texture, log_range = tx.environment_texture(environment)
+    if env_texture is not None:
+        texture_list.append(env_texture)
+        for k in range(min(analytic_count, len(analytic))):
+            if int(analytic[k, 0, 3] + 0.5) == 3:
+                analytic[k, 3, 0] = float(len(texture_list) - 1)
+                analytic[k, 3, 2] = log_range
+    return tx.texture_words(texture_list) if texture_list else None
+
+
+def _uv_density(world, tri, uv):
+    """sqrt(uv area / world area) per triangle (triangle_attribute_data,
+    scene/_core.py:2042-2069)."""
+    p = world[tri]
+    w = uv[tri]
+    e = w[:, 1:] - w[:, :1]
+    uv_area = np.abs(e[:, 0, 0] * e[:, 1, 1] - e[:, 0, 1] * e[:, 1, 0])
+    we = p[:, 1:] - p[:, :1]
+    world_area = np.linalg.norm(np.cross(we[:, 0], we[:, 1]), axis=1)
+    return np.sqrt(uv_area / np.maximum(world_area, np.float32(1e-8))).astype(np.float32)
 
 
 @dataclass
@@ -46,6 +144,12 @@ class PackedScene:
     area_light_weight: float
     environment: object
     has_textures: bool
+    # Textured scenes only (None otherwise):
+    uvs: np.ndarray | None = None            # float32 [V, 4] uv0.xy, uv1.xy
+    tangents: np.ndarray | None = None       # float32 [V, 4] world tangent, handedness
+    uv_density: np.ndarray | None = None     # float32 [T] uv0 texels-per-world-unit scale
+    texture_words: np.ndarray | None = None  # uint32 (textures.texture_words)
+    bindings: np.ndarray | None = None       # float32 [B, 2, 4]
 
     @property
     def triangle_count(self):
@@ -64,37 +168,36 @@ def pack_scene(scene):
     positions, normals, indices, material_ids = [], [], [], []
     materials, material_index = [], {}
     offset = 0
-    textured = False
-    for mesh in _meshes(scene):
+    meshes = [m for m in _meshes(scene) if len(m.indices)]
+    bindings = _Bindings()
+    for mesh in meshes:
+        bindings.add(mesh.material)
+    textured = bool(bindings.textures) or getattr(getattr(scene, "environment", None), "image", None) is not None
+    uvs, tangents, density = [], [], []
+    for mesh in meshes:
         tri = np.asarray(mesh.indices, dtype=np.uint32)
-        if not len(tri):
-            continue
         world = np.asarray(mesh.world_vertices, dtype=np.float32)
         positions.append(world)
         normals.append(np.asarray(mesh.world_normals, dtype=np.float32))
         indices.append(tri + np.uint32(offset))
         offset += len(world)
+        if bindings.textures:
+            uv0 = np.asarray(mesh.texcoords, np.float32)
+            uvs.append(np.column_stack((uv0, np.asarray(mesh.texcoords1, np.float32))))
+            tangents.append(np.asarray(mesh.world_tangents, np.float32))
+            density.append(_uv_density(world, tri, uv0))
         material = mesh.material
-        if any(getattr(material, name, None) is not None for name in _TEXTURE_FIELDS):
-            textured = True
         faces = getattr(mesh, "face_materials", None)
         if faces is not None:
             # One record per triangle, appended in triangle order.
-            block = faces.records(material)
-            block[:, 5, 3] = 1.0
             first = sum(len(m) for m in materials)
-            materials.append(block)
+            materials.append(_material_block(material, len(tri), faces, bindings))
             material_ids.append(np.arange(first, first + len(tri), dtype=np.uint32))
             continue
         key = id(material)
         if key not in material_index:
             material_index[key] = sum(len(m) for m in materials)
-            record = material_record(material)
-            # Resolved occlusion factor of an untextured material: the value
-            # the reference's textured path computes when no occlusion map is
-            # bound (mix(1, 1, strength) = 1).
-            record[5, 3] = 1.0
-            materials.append(record[None])
+            materials.append(_material_block(material, 1, None, bindings))
         material_ids.append(np.full(len(tri), material_index[key], np.uint32))
     if not materials:
         record = material_record(type("Default", (), {
@@ -106,19 +209,30 @@ def pack_scene(scene):
                                        if parts else np.zeros(shape, dtype))
     analytic_count = int(getattr(scene, "analytic_light_count", 0))
     area_count = int(getattr(scene, "emissive_triangle_count", 0))
+    analytic = np.array(scene.analytic_light_data(), np.float32).reshape(-1, 4, 4)
+    words = binding = None
+    if textured:
+        texture_list = list(bindings.textures)
+        words = _environment_words(getattr(scene, "environment", None), analytic, analytic_count, texture_list)
+        binding = tx.binding_records(bindings.bindings, texture_list)
     return PackedScene(
         positions=cat(positions, (0, 3), np.float32),
         normals=cat(normals, (0, 3), np.float32),
         indices=cat(indices, (0, 3), np.uint32),
         material_ids=cat(material_ids, (0,), np.uint32),
         materials=np.ascontiguousarray(np.concatenate(materials), np.float32),
-        analytic_lights=np.ascontiguousarray(scene.analytic_light_data(), np.float32),
+        analytic_lights=np.ascontiguousarray(analytic, np.float32),
         analytic_light_count=analytic_count,
         area_lights=np.ascontiguousarray(scene.emissive_triangle_data(), np.float32),
         area_light_count=area_count,
         area_light_weight=float(scene.emissive_light_weight),
         environment=getattr(scene, "environment", None),
         has_textures=textured,
+        uvs=cat(uvs, (0, 4), np.float32) if bindings.textures else None,
+        tangents=cat(tangents, (0, 4), np.float32) if bindings.textures else None,
+        uv_density=cat(density, (0,), np.float32) if bindings.textures else None,
+        texture_words=words,
+        bindings=binding,
     )
 
 
@@ -147,12 +261,18 @@ class PackedInstancedScene:
     area_light_count: int
     area_light_weight: float
     environment: object
-    has_textures: bool
+    has_textures: bool                       # some material has texture bindings
+    texture_words: np.ndarray | None = None  # an image environment's texture
 
     @property
     def triangle_count(self):
         return int(self.instance_triangle_offset[-1] + (self.triangle_offsets[self.instance_resource[-1] + 1]
                                                         - self.triangle_offsets[self.instance_resource[-1]]))
+
+
+def has_textured_materials(scene):
+    """True when some rendered mesh's material has texture bindings."""
+    return any(is_textured(m.material) for m in _meshes(scene) if len(m.indices))
 
 
 def shares_resources(scene):
@@ -193,11 +313,11 @@ def pack_instanced_scene(scene):
         material = mesh.material
         if id(material) not in material_index:
             material_index[id(material)] = len(materials)
-            if any(getattr(material, name, None) is not None for name in _TEXTURE_FIELDS):
-                textured = True
+            if is_textured(material):
+                raise ValueError("materials with texture bindings need a flat scene (instanced=False)")
             record = material_record(material)
             record[5, 3] = 1.0  # resolved occlusion factor of an untextured material
-            materials.append(record)
+            materials.append(record[None])
         inst_res.append(res_index[id(resource)])
         inst_mat.append(material_index[id(material)])
         inst_off.append(offset)
@@ -248,6 +368,8 @@ def pack_instanced_scene(scene):
         raise ValueError("no materials")
     analytic_count = int(getattr(scene, "analytic_light_count", 0))
     area_count = int(getattr(scene, "emissive_triangle_count", 0))
+    analytic = np.array(scene.analytic_light_data(), np.float32).reshape(-1, 4, 4)
+    words = _environment_words(getattr(scene, "environment", None), analytic, analytic_count, [])
     return PackedInstancedScene(
         vertex_offsets=vertex_offsets, triangle_offsets=triangle_offsets,
         positions=positions, normals=normals, indices=indices,
@@ -259,14 +381,16 @@ def pack_instanced_scene(scene):
         world_bounds=np.ascontiguousarray(np.concatenate((wmin, wmax), 1), np.float32),
         world_spheres=np.ascontiguousarray(spheres, np.float32),
         materials=np.ascontiguousarray(np.concatenate(materials), np.float32),
-        analytic_lights=np.ascontiguousarray(scene.analytic_light_data(), np.float32),
+        analytic_lights=np.ascontiguousarray(analytic, np.float32),
         analytic_light_count=analytic_count,
         area_lights=np.ascontiguousarray(scene.emissive_triangle_data(), np.float32),
         area_light_count=area_count,
         area_light_weight=float(scene.emissive_light_weight),
         environment=getattr(scene, "environment", None),
         has_textures=textured,
+        texture_words=words,
     )
 
 
-__all__ = ["PackedScene", "PackedInstancedScene", "pack_scene", "pack_instanced_scene", "shares_resources"]
+__all__ = ["PackedScene", "PackedInstancedScene", "has_textured_materials", "pack_scene", "pack_instanced_scene",
+           "shares_resources"]

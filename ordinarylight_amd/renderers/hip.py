@@ -25,7 +25,7 @@ import numpy as np
 
 from .. import _native as N
 from ..cameras import camera_basis, camera_projection
-from ..packing import pack_instanced_scene, pack_scene, shares_resources
+from ..packing import has_textured_materials, pack_instanced_scene, pack_scene, shares_resources
 from ..selection import PickOptions, PickResult, ViewportMapping
 from .base import RendererImplementation, RendererImplementationInfo
 
@@ -200,6 +200,16 @@ def _scene_desc(packed, leaf_size, builder="lbvh"):
     desc.area_light_weight = packed.area_light_weight
     desc.leaf_size = int(leaf_size)
     desc.builder = BUILDERS[builder]
+    if packed.texture_words is not None:
+        desc.texture_word_count = len(packed.texture_words)
+        desc.texture_words = N.ptr(packed.texture_words, N.ctypes.c_uint32)
+    if packed.bindings is not None:
+        desc.binding_count = len(packed.bindings)
+        desc.bindings = N.ptr(packed.bindings)
+    if packed.uvs is not None:
+        desc.uvs = N.ptr(packed.uvs)
+        desc.tangents = N.ptr(packed.tangents)
+        desc.uv_density = N.ptr(packed.uv_density)
     return desc
 
 
@@ -230,6 +240,9 @@ def _instanced_desc(p, leaf_size, builder="lbvh"):
     d.area_light_weight = p.area_light_weight
     d.leaf_size = int(leaf_size)
     d.builder = BUILDERS[builder]
+    if p.texture_words is not None:
+        d.texture_word_count = len(p.texture_words)
+        d.texture_words = N.ptr(p.texture_words, N.ctypes.c_uint32)
     return d
 
 
@@ -254,7 +267,8 @@ class DeviceScene:
             raise ValueError("builder must be 'lbvh' or 'sah'")
         self.builder = builder
         if instanced is None:
-            instanced = shares_resources(scene)
+            # Textured materials are shaded on flat scenes only.
+            instanced = shares_resources(scene) and not has_textured_materials(scene)
         self.instanced = bool(instanced)
         if self.instanced:
             self._init_instanced(scene, leaf_size)
@@ -476,13 +490,8 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         frame_index = int(frame_index)
         if frame_index < 0:
             raise ValueError("frame_index cannot be negative")
-        environment = getattr(scene, "environment", None)
-        if environment is not None and getattr(environment, "image", None) is not None:
-            raise RuntimeError("image environments are not supported by the HIP GI renderer")
         started = time.perf_counter()
         device_scene = self._resident(scene)
-        if device_scene.packed.has_textures:
-            raise RuntimeError("textured materials are not supported by the HIP GI renderer")
         context = self._ensure_context()
         lib = N.library()
         N.check(lib.olhip_set_profiling(context.handle, int(bool(self.config.profiling))))

@@ -104,6 +104,77 @@ class Transform:
         return Transform(self.matrix @ other.matrix)
 
 
+_WRAP_MODES = {"repeat": 0, "clamp": 1, "mirror": 2}
+
+
+@dataclass(frozen=True)
+class Texture:
+    """RGBA8 image with glTF sampler state (the reference's ``Texture``,
+    ``scene/_core.py:116-135``): wrap per axis and bilinear/nearest filter.
+    Textures are bound by identity, as the reference binds them."""
+
+    pixels: np.ndarray = field(compare=False, repr=False)
+    wrap_s: str = "repeat"
+    wrap_t: str = "repeat"
+    linear_filter: bool = True
+
+    def __post_init__(self):
+        pixels = np.ascontiguousarray(self.pixels, dtype=np.uint8)
+        if pixels.ndim != 3 or pixels.shape[2] != 4:
+            raise ValueError("texture pixels must have shape (height, width, 4)")
+        if pixels.shape[0] < 1 or pixels.shape[1] < 1:
+            raise ValueError("texture dimensions must be positive")
+        for name in ("wrap_s", "wrap_t"):
+            if getattr(self, name) not in _WRAP_MODES:
+                raise ValueError(f"{name} must be repeat, clamp, or mirror")
+        if not isinstance(self.linear_filter, bool):
+            raise TypeError("linear_filter must be a bool")
+        pixels.flags.writeable = False
+        object.__setattr__(self, "pixels", pixels)
+
+
+@dataclass(frozen=True)
+class TextureTransform:
+    """Affine UV mapping of one binding (KHR_texture_transform, as the
+    reference's ``TextureTransform``, ``scene/_core.py:403-425``): scale, then
+    rotate, then offset, on texture coordinate set 0 or 1."""
+
+    offset: tuple = (0.0, 0.0)
+    scale: tuple = (1.0, 1.0)
+    rotation: float = 0.0
+    texcoord_set: int = 0
+
+    def __post_init__(self):
+        if len(self.offset) != 2 or len(self.scale) != 2:
+            raise ValueError("texture transform offset and scale must be vec2 values")
+        offset = tuple(float(v) for v in self.offset)
+        scale = tuple(float(v) for v in self.scale)
+        rotation = float(self.rotation)
+        if int(self.texcoord_set) not in (0, 1):
+            raise ValueError("texcoord_set must be zero or one")
+        if not all(math.isfinite(v) for v in (*offset, *scale, rotation)):
+            raise ValueError("texture transform values must be finite")
+        if abs(scale[0]) < 1e-8 or abs(scale[1]) < 1e-8:
+            raise ValueError("texture transform scale components cannot be zero")
+        object.__setattr__(self, "offset", offset)
+        object.__setattr__(self, "scale", scale)
+        object.__setattr__(self, "rotation", rotation)
+        object.__setattr__(self, "texcoord_set", int(self.texcoord_set))
+
+
+# Texture slots the GI shading applies (shadeApplyMaterialTextures and
+# shadeApplyNormalTexture, wavefront_shade_candidate.glsl:915-955), each with
+# its transform field.
+MATERIAL_TEXTURE_SLOTS = (
+    ("base_color_texture", "base_color_transform"),
+    ("metallic_roughness_texture", "metallic_roughness_transform"),
+    ("emissive_texture", "emissive_transform"),
+    ("normal_texture", "normal_transform"),
+    ("occlusion_texture", "occlusion_transform"),
+    ("transmission_texture", "transmission_transform"),
+)
+
+
 @dataclass(frozen=True)
 class Material:
     """Physically based material; defaults follow the reference material."""
@@ -132,8 +203,25 @@ class Material:
     emission_two_sided: bool = False
     normal_scale: float = 1.0
     occlusion_strength: float = 1.0
+    base_color_texture: Texture | None = None
+    base_color_transform: TextureTransform = TextureTransform()
+    metallic_roughness_texture: Texture | None = None
+    metallic_roughness_transform: TextureTransform = TextureTransform()
+    emissive_texture: Texture | None = None
+    emissive_transform: TextureTransform = TextureTransform()
+    normal_texture: Texture | None = None
+    normal_transform: TextureTransform = TextureTransform()
+    occlusion_texture: Texture | None = None
+    occlusion_transform: TextureTransform = TextureTransform()
+    transmission_texture: Texture | None = None
+    transmission_transform: TextureTransform = TextureTransform()
 
     def __post_init__(self):
+        for texture_name, transform_name in MATERIAL_TEXTURE_SLOTS:
+            if getattr(self, texture_name) is not None and not isinstance(getattr(self, texture_name), Texture):
+                raise TypeError(f"{texture_name} must be a Texture or None")
+            if not isinstance(getattr(self, transform_name), TextureTransform):
+                raise TypeError(f"{transform_name} must be a TextureTransform")
         color = _vec3(self.base_color, "base_color")
         emission = _vec3(self.emission, "emission")
         attenuation = _vec3(self.attenuation_color, "attenuation_color")
@@ -256,9 +344,53 @@ def generate_normals(vertices, indices):
     return np.ascontiguousarray(normals, dtype=np.float32)
 
 
+def generate_tangents(vertices, indices, normals, texcoords):
+    """Per-vertex tangents (xyz, handedness) from texture-coordinate
+    gradients, the definition of ``Mesh._generate_tangents``
+    (``scene/_core.py:740-772``): per triangle the uv-gradient tangent and
+    bitangent (triangles with |det| < 1e-10 skipped), summed into each
+    corner, Gram-Schmidt against the normal (a fallback axis where the sum
+    vanishes), handedness from the bitangent sum."""
+    vertices = np.asarray(vertices, np.float32)
+    indices = np.asarray(indices, np.int64)
+    normals = np.asarray(normals, np.float32)
+    uv = np.asarray(texcoords, np.float32)
+    t_sum = np.zeros_like(vertices)
+    b_sum = np.zeros_like(vertices)
+    if len(indices):
+        p = vertices[indices]
+        w = uv[indices]
+        ea, eb = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]
+        ua, ub = w[:, 1] - w[:, 0], w[:, 2] - w[:, 0]
+        det = ua[:, 0] * ub[:, 1] - ua[:, 1] * ub[:, 0]
+        ok = np.abs(det.astype(np.float64)) >= 1e-10
+        inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0).astype(np.float32)
+        tan = (ea * ub[:, 1:2] - eb * ua[:, 1:2]) * inv[:, None]
+        bit = (eb * ua[:, 0:1] - ea * ub[:, 0:1]) * inv[:, None]
+        for corner in range(3):
+            np.add.at(t_sum, indices[ok, corner], tan[ok])
+            np.add.at(b_sum, indices[ok, corner], bit[ok])
+    tangent = t_sum - normals * np.sum(normals * t_sum, axis=1, keepdims=True)
+    length = np.linalg.norm(tangent, axis=1)
+    small = length < 1e-8
+    if np.any(small):
+        n = normals[small]
+        axis = np.where(np.abs(n[:, 2:3]) < 0.999, np.array([[0.0, 0.0, 1.0]], np.float32),
+                        np.array([[0.0, 1.0, 0.0]], np.float32))
+        tangent[small] = np.cross(n, axis)
+        length[small] = np.linalg.norm(tangent[small], axis=1)
+    tangent = tangent / np.maximum(length, 1e-8)[:, None]
+    hand = np.where(np.sum(np.cross(normals, tangent) * b_sum, axis=1) < 0.0, -1.0, 1.0)
+    return np.ascontiguousarray(np.column_stack((tangent, hand)), dtype=np.float32)
+
+
 @dataclass(eq=False)
 class MeshResource:
-    """Reusable object-space geometry shared by any number of instances."""
+    """Reusable object-space geometry shared by any number of instances.
+
+    ``texcoords`` / ``texcoords1`` default to zeros and ``tangents`` are
+    generated from ``texcoords`` when not given (``scene/_core.py:561-637``).
+    """
 
     vertices: np.ndarray
     indices: np.ndarray
@@ -266,6 +398,9 @@ class MeshResource:
     normals: np.ndarray | None = None
     name: str | None = None
     metadata: dict = field(default_factory=dict, repr=False)
+    texcoords: np.ndarray | None = field(default=None, repr=False)
+    texcoords1: np.ndarray | None = field(default=None, repr=False)
+    tangents: np.ndarray | None = field(default=None, repr=False)
     id: int | None = field(default=None, init=False)
 
     def __post_init__(self):
@@ -287,7 +422,31 @@ class MeshResource:
             if np.any(lengths < 1e-8):
                 raise ValueError("normals cannot contain zero vectors")
             self.normals = normals / lengths[:, None]
+        count = len(self.vertices)
+        for name in ("texcoords", "texcoords1"):
+            value = getattr(self, name)
+            value = np.zeros((count, 2), np.float32) if value is None else np.ascontiguousarray(value, np.float32)
+            if value.shape != (count, 2):
+                raise ValueError(f"{name} must have shape (vertex_count, 2)")
+            setattr(self, name, value)
+        if self.tangents is not None:
+            tangents = np.array(self.tangents, dtype=np.float32)
+            if tangents.shape != (count, 4):
+                raise ValueError("tangents must have shape (vertex_count, 4)")
+            lengths = np.linalg.norm(tangents[:, :3], axis=1)
+            if np.any(lengths < 1e-8):
+                raise ValueError("tangent directions cannot contain zero vectors")
+            tangents[:, :3] /= lengths[:, None]
+            tangents[:, 3] = np.where(tangents[:, 3] < 0.0, -1.0, 1.0)
+            self.tangents = np.ascontiguousarray(tangents)
         self.metadata = MappingProxyType(dict(self.metadata))
+
+    def tangent_frame(self):
+        """The given tangents, or ones generated from ``texcoords`` on first
+        use (untextured scenes never need them) and kept in ``tangents``."""
+        if self.tangents is None:
+            self.tangents = generate_tangents(self.vertices, self.indices, self.normals, self.texcoords)
+        return self.tangents
 
 
 @dataclass(eq=False)
@@ -340,6 +499,33 @@ class Mesh:
         result = self.normals @ normal_matrix.T
         result /= np.linalg.norm(result, axis=1, keepdims=True)
         return np.ascontiguousarray(result, dtype=np.float32)
+
+    @property
+    def texcoords(self):
+        return self.resource.texcoords
+
+    @property
+    def texcoords1(self):
+        return self.resource.texcoords1
+
+    @property
+    def tangents(self):
+        return self.resource.tangent_frame()
+
+    @property
+    def world_tangents(self):
+        """Tangents through the linear part, made orthogonal to the world
+        normal and renormalised; handedness flips with a mirroring transform
+        (``scene/_core.py:701-714``)."""
+        linear = self.transform.matrix[:3, :3]
+        directions = self.tangents[:, :3] @ linear.T
+        normals = self.world_normals
+        directions -= normals * np.sum(directions * normals, axis=1)[:, None]
+        directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+        handedness = self.tangents[:, 3].copy()
+        if np.linalg.det(linear) < 0.0:
+            handedness *= -1.0
+        return np.ascontiguousarray(np.column_stack((directions, handedness)), dtype=np.float32)
 
 
 Instance = Mesh
@@ -417,11 +603,12 @@ class Scene:
 
     # -- construction ----------------------------------------------------
     def create_mesh(self, vertices, indices, material=None, *, normals=None,
-                    name=None, metadata=None):
+                    name=None, metadata=None, texcoords=None, texcoords1=None, tangents=None):
         """Create reusable object-space geometry without placing it."""
         resource = MeshResource(vertices, indices, material or Material(),
                                 normals=normals, name=name,
-                                metadata=metadata or {})
+                                metadata=metadata or {}, texcoords=texcoords,
+                                texcoords1=texcoords1, tangents=tangents)
         self._attach_resource(resource)
         return resource
 
@@ -451,9 +638,11 @@ class Scene:
         )
 
     def add_mesh(self, vertices, indices, material=None, *, normals=None,
-                 transform=None, name=None, metadata=None, face_materials=None):
+                 transform=None, name=None, metadata=None, face_materials=None,
+                 texcoords=None, texcoords1=None, tangents=None):
         resource = self.create_mesh(vertices, indices, material,
-                                    normals=normals, name=name, metadata=metadata)
+                                    normals=normals, name=name, metadata=metadata,
+                                    texcoords=texcoords, texcoords1=texcoords1, tangents=tangents)
         return self.add_instance(resource, transform=transform, name=name,
                                  metadata=metadata, face_materials=face_materials)
 
@@ -722,6 +911,7 @@ class Scene:
 
 
 __all__ = [
+    "Texture", "TextureTransform", "MATERIAL_TEXTURE_SLOTS", "generate_tangents",
     "Instance", "Material", "Mesh", "MeshResource", "Scene", "Transform",
     "generate_normals", "material_record",
 ]

@@ -24,7 +24,7 @@ import numpy as np
 
 from .cameras import PerspectiveCamera
 from .lights import EnvironmentLight
-from .scene import FaceMaterials, Material, Scene, Transform
+from .scene import FaceMaterials, Material, Scene, Texture, TextureTransform, Transform
 
 
 def _quad(p0, p1, p2, p3):
@@ -83,6 +83,92 @@ def box_room(seed=0):
                                               emission=(15.0, 15.0, 15.0)))
     camera = PerspectiveCamera(position=(0.0, 1.0, 3.4), target=(0.0, 1.0, 0.0),
                                vertical_fov_degrees=40.0)
+    return scene, camera
+
+
+def _texture(rng, h, w, kind, **sampler):
+    """Seeded RGBA8 test textures: checker colours, random bytes, or a
+    normal map of random tilts (tangent-space, z up)."""
+    if kind == "checker":
+        a, b = rng.integers(20, 236, 3), rng.integers(20, 236, 3)
+        yy, xx = np.mgrid[0:h, 0:w]
+        px = np.where(((yy // 4 + xx // 4) % 2)[..., None] == 0, a, b)
+        px = np.concatenate((px, np.full((h, w, 1), 255)), axis=2)
+    elif kind == "normal":
+        v = rng.normal(0.0, 0.35, (h, w, 3))
+        v[..., 2] = 1.0
+        v /= np.linalg.norm(v, axis=2, keepdims=True)
+        px = np.concatenate((np.round((v * 0.5 + 0.5) * 255), np.full((h, w, 1), 255)), axis=2)
+    else:
+        px = rng.integers(0, 256, (h, w, 4))
+    return Texture(np.clip(px, 0, 255).astype(np.uint8), **sampler)
+
+
+def _planar_uv(vertices, scale):
+    """Texture coordinates by planar projection (x + z, y), scaled."""
+    v = np.asarray(vertices, np.float32)
+    return np.ascontiguousarray(np.column_stack((v[:, 0] + v[:, 2], v[:, 1])) * np.float32(scale), np.float32)
+
+
+def textured_room(seed=0, open_sky=False):
+    """The box room with textured materials: base colour, metallic-roughness,
+    emissive, normal, occlusion and transmission maps; repeat, clamp and
+    mirror wrapping, nearest and bilinear filtering, transformed bindings and
+    a second texture-coordinate set.  ``open_sky`` keeps the floor and the
+    boxes under an image environment instead of the walls and ceiling."""
+    rng = np.random.default_rng(seed)
+    scene = Scene()
+    floor_v, floor_i = _quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1))
+    floor = Material(
+        base_color=(0.9, 0.85, 0.8), roughness=0.8, metallic=0.6,
+        base_color_texture=_texture(rng, 32, 32, "checker"),
+        base_color_transform=TextureTransform(offset=(0.1, 0.2), scale=(3.0, 2.5), rotation=0.3),
+        metallic_roughness_texture=_texture(rng, 16, 16, "random"),
+        normal_texture=_texture(rng, 24, 24, "normal"), normal_scale=0.8)
+    uv = _planar_uv(floor_v[:, [0, 2, 1]], 0.9)
+    scene.add_mesh(floor_v, floor_i, floor, texcoords=uv, texcoords1=uv[:, ::-1] * 1.7)
+    if not open_sky:
+        back_v, back_i = _quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1))
+        back = Material(
+            base_color=(0.8, 0.8, 0.8), roughness=0.5,
+            base_color_texture=_texture(rng, 16, 24, "checker", wrap_s="mirror", wrap_t="repeat"),
+            base_color_transform=TextureTransform(scale=(2.2, 1.5), texcoord_set=1),
+            normal_texture=_texture(rng, 16, 16, "normal"), normal_transform=TextureTransform(texcoord_set=1))
+        uv = _planar_uv(back_v, 1.0)
+        scene.add_mesh(back_v, back_i, back, texcoords=uv, texcoords1=uv * np.float32(1.3) + np.float32(0.2))
+        for corners, texture in (
+                (((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)),
+                 _texture(rng, 8, 8, "checker", wrap_s="clamp", wrap_t="clamp", linear_filter=False)),
+                (((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)), _texture(rng, 12, 20, "checker", wrap_t="mirror")),
+                (((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)), None)):
+            v, i = _quad(*corners)
+            mat = Material(base_color=tuple(float(c) for c in rng.uniform(0.3, 0.9, 3)),
+                           base_color_texture=texture,
+                           base_color_transform=TextureTransform(offset=(-0.3, 0.15), scale=(1.6, 1.4)))
+            scene.add_mesh(v, i, mat, texcoords=_planar_uv(v, 1.1))
+        light_v, light_i = _quad((-0.25, 1.99, -0.25), (0.25, 1.99, -0.25), (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25))
+        scene.add_mesh(light_v, light_i, Material(base_color=(0.8, 0.8, 0.8), emission=(12.0, 12.0, 12.0),
+                                                  emissive_texture=_texture(rng, 8, 8, "checker")),
+                       texcoords=_planar_uv(light_v[:, [0, 2, 1]], 2.0))
+    box_v, box_i = _box((0.55, 1.2, 0.55), rng)
+    occl = Material(base_color=(0.7, 0.6, 0.5), roughness=0.4,
+                    base_color_texture=_texture(rng, 16, 16, "checker"),
+                    occlusion_texture=_texture(rng, 8, 8, "random"), occlusion_strength=0.7)
+    scene.add_mesh(box_v, box_i, occl, texcoords=_planar_uv(box_v, 1.5),
+                   transform=Transform.translation((-0.38, 0.0, -0.3)) @ Transform.rotation((0, 1, 0), 0.4))
+    box_v, box_i = _box((0.55, 0.6, 0.55), rng)
+    glass = Material(base_color=(0.9, 0.95, 1.0), transmission=1.0, ior=1.5,
+                     transmission_texture=_texture(rng, 8, 8, "random"),
+                     base_color_texture=_texture(rng, 8, 8, "checker"))
+    scene.add_mesh(box_v, box_i, glass, texcoords=_planar_uv(box_v, 1.2),
+                   transform=Transform.translation((0.4, 0.0, 0.35)) @ Transform.rotation((0, 1, 0), -0.3))
+    if open_sky:
+        yy, xx = np.mgrid[0:16, 0:32]
+        sky = np.stack((0.4 + 0.3 * np.sin(xx / 5.0), 0.5 + 0.2 * np.cos(yy / 3.0), 0.9 + 0.0 * xx), axis=2)
+        sky[2:5, 6:9] = (40.0, 36.0, 30.0)  # a bright sun
+        scene.set_environment(EnvironmentLight(image=sky.astype(np.float32), color=(1.0, 0.95, 0.9),
+                                               intensity=0.8, rotation=0.7))
+    camera = PerspectiveCamera(position=(0.0, 1.0, 3.4), target=(0.0, 0.7, 0.0), vertical_fov_degrees=42.0)
     return scene, camera
 
 

@@ -280,3 +280,41 @@ def test_sah_separates_two_clusters_at_the_root():
     lo, hi = (int(left[0]) & 0x7FFFFFFF, int(right[0]) & 0x7FFFFFFF)
     left_prims = order[first[lo]:last[lo] + 1] if not left[0] & 0x80000000 else order[lo:lo + 1]
     assert set(left_prims.tolist()) in (set(range(50)), set(range(50, 120)))
+
+
+def test_packed_material_arrays_hold_one_record_per_material():
+    """Both packers hand the C ABI [M, 12, 4] records, M = the material count
+    it reads (48 floats each), and every material index lies below M."""
+    from ordinarylight_amd import packing
+    inst, _ = workloads.instanced_spheres(3, meshes=2, subdivisions=1, instances=12, box=6.0, emitters=4,
+                                          emitter_size=0.3)
+    p = packing.pack_instanced_scene(inst)
+    assert p.materials.ndim == 3 and p.materials.shape[1:] == (12, 4)
+    assert int(p.instance_material.max()) < len(p.materials)
+    flat = packing.pack_scene(workloads.box_room(0)[0])
+    assert flat.materials.ndim == 3 and flat.materials.shape[1:] == (12, 4)
+    assert int(flat.material_ids.max()) < len(flat.materials)
+
+
+def test_textured_scene_packing():
+    """Textured materials: binding indices in record rows 4 / 5, the textured
+    flag word, per-vertex uvs and world tangents (unit, handedness +-1), and
+    an image environment's texture index in its analytic record."""
+    from ordinarylight_amd import packing
+    scene, _ = workloads.textured_room(0, open_sky=True)
+    p = packing.pack_scene(scene)
+    m = p.materials
+    assert p.texture_words[0] == len(p.texture_words[1::8][:p.texture_words[0]])
+    flags = m[:, 8, 3]
+    assert set(flags.tolist()) <= {0.0, 1.0} and flags.sum() >= 2
+    textured = m[flags == 1.0]
+    for r in textured:
+        for b in (*r[4], r[5, 1], r[5, 3]):
+            assert b == -1.0 or 0 <= b < len(p.bindings)
+    assert p.uvs.shape == (len(p.positions), 4) and p.tangents.shape == (len(p.positions), 4)
+    assert np.allclose(np.linalg.norm(p.tangents[:, :3], axis=1), 1.0, atol=1e-5)
+    assert set(np.unique(p.tangents[:, 3]).tolist()) <= {-1.0, 1.0}
+    env = [r for r in p.analytic_lights[:p.analytic_light_count] if int(r[0, 3] + 0.5) == 3]
+    assert len(env) == 1 and env[0][3, 0] == p.texture_words[0] - 1 and env[0][3, 2] > 0.0
+    with pytest.raises(ValueError):
+        packing.pack_instanced_scene(scene)
