@@ -452,7 +452,7 @@ void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t
     x.blas = d->instance_resource[k];
     x.material = d->instance_material[k];
     x.tri_offset = d->instance_triangle_offset[k];
-    x.id = d->instance_id[k];
+    x.root = 0u;
     const float* b = d->world_bounds + 6ull * k;
     for (int a = 0; a < 6; ++a) require(std::isfinite(b[a]), "instance bounds must be finite");
     x.sphere = make_float4(0.0f, 0.0f, 0.0f, INFINITY);
@@ -469,10 +469,25 @@ void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t
   }
   upload(sc->tlas_pos, tpos.data(), tpos.size(), s);
   upload(sc->tlas_vidx, tvidx.data(), tvidx.size(), s);
-  upload(sc->inst_table, inst.data(), inst.size(), s);
   OL_HIP_CHECK(hipStreamSynchronize(s));
   BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u, d->world_bounds};
   bvh_build(tin, sc->bvh, s, nullptr);
+  // The instance table in top-level leaf order: a top-level leaf's first
+  // slot is then its instance's index, and entering an instance needs no
+  // read of the leaf record.
+  {
+    std::vector<float4> leaves(3ull * I);
+    OL_HIP_CHECK(hipMemcpy(leaves.data(), sc->bvh.tri.ptr, sizeof(float4) * leaves.size(), hipMemcpyDeviceToHost));
+    std::vector<InstanceDev> ordered(I);
+    for (uint32_t j = 0; j < I; ++j) {
+      uint32_t k;
+      std::memcpy(&k, &leaves[3ull * j].w, sizeof(k));
+      require(k < I, "top-level leaf names no instance");
+      ordered[j] = inst[k];
+    }
+    upload(sc->inst_table, ordered.data(), ordered.size(), s);
+    OL_HIP_CHECK(hipStreamSynchronize(s));
+  }
   sc->tlas_depth = sc->bvh.depth;
   // Stack sizing: both levels plus the sentinel.
   sc->bvh.depth = sc->tlas_depth + sc->blas_depth + 1u;

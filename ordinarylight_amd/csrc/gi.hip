@@ -595,16 +595,25 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       pool.n -= take;
     }
     const bool at_leaf = busy && closest2_at_leaf(q);
+    const bool at_entry = busy && closest2_at_entry(q);
     const uint64_t leafy = __ballot(at_leaf);
-    const uint64_t inner = __ballot(busy && !at_leaf);
-    if ((leafy | inner) == 0) {
+    const uint64_t entries = __ballot(at_entry);
+    const uint64_t inner = __ballot(busy && !at_leaf && !at_entry);
+    const uint32_t n_busy = (uint32_t)__popcll(leafy | entries | inner);
+    if (n_busy == 0) {
       if (pool.exhausted && pool.n == 0) break;
       continue;
     }
-    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+    // Leaf, entry or node step: a class runs when it holds a third of the
+    // busy lanes or nothing else is left (leaves first, then entries).
+    if (leafy && ((entries | inner) == 0 || 3u * (uint32_t)__popcll(leafy) >= n_busy)) {
       if (at_leaf) closest2_leaf<LSTACK>(q, st, nt);
-    } else if (busy && !at_leaf) {
-      closest2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv, ni);
+    } else if (entries && (inner == 0 || 3u * (uint32_t)__popcll(entries) >= n_busy)) {
+      if (at_entry) closest2_enter<LSTACK>(W.jobs[job].P.sc, q, st, ni);
+    } else if (inner) {
+      if (busy && !at_leaf && !at_entry) closest2_node<LSTACK>(W.jobs[job].P.sc, q, st, nv);
+    } else {
+      if (at_leaf) closest2_leaf<LSTACK>(q, st, nt);
     }
     if (busy && q.node == OL_NONE) {
       W.hit[slot] = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
@@ -674,16 +683,23 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       pool.n -= take;
     }
     const bool at_leaf = busy && any2_at_leaf(q);
+    const bool at_entry = busy && any2_at_entry(q);
     const uint64_t leafy = __ballot(at_leaf);
-    const uint64_t inner = __ballot(busy && !at_leaf);
-    if ((leafy | inner) == 0) {
+    const uint64_t entries = __ballot(at_entry);
+    const uint64_t inner = __ballot(busy && !at_leaf && !at_entry);
+    const uint32_t n_busy = (uint32_t)__popcll(leafy | entries | inner);
+    if (n_busy == 0) {
       if (pool.exhausted && pool.n == 0) break;
       continue;
     }
-    if (inner == 0 || 3u * (uint32_t)__popcll(leafy) >= (uint32_t)__popcll(leafy | inner)) {
+    if (leafy && ((entries | inner) == 0 || 3u * (uint32_t)__popcll(leafy) >= n_busy)) {
       if (at_leaf) any2_leaf<LSTACK>(q, st, nt);
-    } else if (busy && !at_leaf) {
-      any2_inner<LSTACK>(W.jobs[job].P.sc, q, st, nv, ni);
+    } else if (entries && (inner == 0 || 3u * (uint32_t)__popcll(entries) >= n_busy)) {
+      if (at_entry) any2_enter<LSTACK>(W.jobs[job].P.sc, q, st, ni);
+    } else if (inner) {
+      if (busy && !at_leaf && !at_entry) any2_node<LSTACK>(W.jobs[job].P.sc, q, st, nv);
+    } else {
+      if (at_leaf) any2_leaf<LSTACK>(q, st, nt);
     }
     if (busy && q.node == OL_NONE) {
       if (!q.occluded) shadow_add(mode, W.ec[e], direct, area_direct, env_direct);

@@ -146,7 +146,7 @@ struct InstanceDev {
   uint32_t blas;        // bottom-level structure
   uint32_t material;    // material record index
   uint32_t tri_offset;  // first triangle of this instance in packed (reference) triangle order
-  uint32_t id;          // scene instance id
+  uint32_t root;        // child word of `blas` where traversal starts (0: its root node)
   float4 sphere;        // world bounding sphere: centre xyz, radius (+inf: none)
 };
 

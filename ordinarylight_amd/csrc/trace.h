@@ -55,6 +55,21 @@ OL_DEV RayPre ray_pre(f3 o, f3 d) {
   return r;
 }
 
+// The same constants for a ray entering an instance, with reciprocals from
+// v_rcp_f32 (within 1 ulp) instead of correctly rounded divisions: the node
+// boxes' build-time growth (2^-17 of the largest root coordinate) is far
+// wider than the extra half ulp, and the triangle test does not use them,
+// so hits do not change.
+OL_DEV RayPre ray_pre_fast(f3 o, f3 d) {
+  auto safe = [](float x) { return fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x); };
+  RayPre r;
+  r.o = o;
+  r.d = d;
+  r.inv = mk3(__builtin_amdgcn_rcpf(safe(d.x)), __builtin_amdgcn_rcpf(safe(d.y)), __builtin_amdgcn_rcpf(safe(d.z)));
+  r.neg = (r.inv.x < 0.0f ? 1u : 0u) | (r.inv.y < 0.0f ? 2u : 0u) | (r.inv.z < 0.0f ? 4u : 0u);
+  return r;
+}
+
 OL_DEV float byte_f(uint32_t w, int c) { return (float)((w >> (8 * c)) & 255u); }
 
 // Slab test of the four quantised child boxes of a node.
@@ -450,7 +465,7 @@ OL_DEV float max_abs3(f3 a) { return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(
 OL_DEV bool sphere_reachable(const RayPre& r, float4 s) {
   const f3 c = mk3(s.x, s.y, s.z);
   const f3 oc = c - r.o;
-  const float tc = dot3(oc, r.d) / dot3(r.d, r.d);
+  const float tc = dot3(oc, r.d) * __builtin_amdgcn_rcpf(dot3(r.d, r.d));
   const f3 v = oc - r.d * tc;
   const float scale = fmaxf(fmaxf(max_abs3(r.o), max_abs3(c)), fabsf(tc) * max_abs3(r.d));
   const float reach = s.w + 1e-5f * scale;
@@ -463,18 +478,20 @@ OL_DEV bool sphere_reachable(const RayPre& r, float4 s) {
 // than tmax plus a further allowance of 1e-4 of the magnitudes.  Every point
 // of the instance lies in the sphere, so an instance failing the test holds
 // no hit at t <= tmax: a closest-hit ray skips it once a nearer hit is known.
+// The divisions and square roots use the hardware approximations (within an
+// ulp or two): the allowances above are orders of magnitude wider.
 OL_DEV bool sphere_before(const RayPre& r, float4 s, float tmax) {
   const f3 c = mk3(s.x, s.y, s.z);
   const f3 oc = c - r.o;
   const float dd = dot3(r.d, r.d);
-  const float tc = dot3(oc, r.d) / dd;
+  const float tc = dot3(oc, r.d) * __builtin_amdgcn_rcpf(dd);
   const f3 v = oc - r.d * tc;
   const float scale = fmaxf(fmaxf(max_abs3(r.o), max_abs3(c)), fabsf(tc) * max_abs3(r.d));
   const float reach = s.w + 1e-5f * scale;
   const float h2 = reach * reach - dot3(v, v);
   if (!(h2 >= 0.0f)) return false;
-  const float inv_len = 1.0f / sqrtf(dd);
-  const float t_in = tc - sqrtf(h2) * inv_len;
+  const float inv_len = __builtin_amdgcn_rsqf(dd);
+  const float t_in = tc - __builtin_amdgcn_sqrtf(h2) * inv_len;
   return t_in <= tmax + 1e-4f * (s.w + scale) * inv_len;
 }
 
@@ -510,32 +527,43 @@ OL_DEV bool closest2_at_leaf(const Closest2& q) { return q.inst != OL_NONE && ch
 template <int LSTACK>
 OL_DEV void closest2_next(Closest2& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
 
-// Inner step: leave an instance (sentinel), enter one (top-level leaf), or
-// test a node's boxes at the current level.
+// Entry step: a top-level leaf names an instance (an entry of one); the ray
+// enters it, in object space, unless its bounding sphere starts beyond the
+// best hit.  Kept apart from node steps, so that a wave runs the entry code
+// only for lanes that all enter (it is several times a node test's cost).
+OL_DEV bool closest2_at_entry(const Closest2& q) { return q.inst == OL_NONE && child_is_leaf(q.node); }
+
 template <int LSTACK>
-OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& ni) {
+OL_DEV void closest2_enter(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& ni) {
+  const uint32_t k = leaf_first(q.node);  // the instance table is in top-level leaf order
+  const InstanceDev& I = sc.inst[k];
+  if (!sphere_before(q.w, I.sphere, q.best.t)) {
+    closest2_next(q, st);
+    return;
+  }
+  const BlasDev& B = sc.blas[I.blas];
+  if (B.node_count == 0) {
+    closest2_next(q, st);
+    return;
+  }
+  ++ni;
+  st.push(OL_SENTINEL);
+  q.inst = k;
+  q.offset = I.tri_offset;
+  q.bnodes = B.nodes;
+  q.btri = B.tri;
+  q.r = ray_pre_fast(xform_point(I.w2o, q.w.o), xform_vector(I.w2o, q.w.d));
+  q.node = I.root;
+}
+
+// Node step: leave an instance (sentinel) or test a node's boxes at the
+// current level.
+template <int LSTACK>
+OL_DEV void closest2_node(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv) {
   if (q.node == OL_SENTINEL) {
     q.inst = OL_NONE;
     q.r = q.w;
     closest2_next(q, st);
-    return;
-  }
-  if (q.inst == OL_NONE && child_is_leaf(q.node)) {
-    const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
-    const InstanceDev& I = sc.inst[k];
-    const BlasDev& B = sc.blas[I.blas];
-    if (B.node_count == 0 || !sphere_before(q.w, I.sphere, q.best.t)) {
-      closest2_next(q, st);
-      return;
-    }
-    ++ni;
-    st.push(OL_SENTINEL);
-    q.inst = k;
-    q.offset = I.tri_offset;
-    q.bnodes = B.nodes;
-    q.btri = B.tri;
-    q.r = ray_pre(xform_point(I.w2o, q.w.o), xform_vector(I.w2o, q.w.d));
-    q.node = 0;
     return;
   }
   const WideNode* nodes = q.inst == OL_NONE ? sc.nodes : q.bnodes;
@@ -547,6 +575,13 @@ OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st,
   ol_node_keys(w0, w1, w2, q.r, q.tmin, cull_distance(q.best.t), key, ch);
   sort4_pairs(key, ch);
   descend(st, ch, key, q.node);
+}
+
+// Entry or node step, whichever the lane is at.
+template <int LSTACK>
+OL_DEV void closest2_inner(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& ni) {
+  if (closest2_at_entry(q)) closest2_enter<LSTACK>(sc, q, st, ni);
+  else closest2_node<LSTACK>(sc, q, st, nv);
 }
 
 template <int LSTACK>
@@ -596,29 +631,36 @@ OL_DEV bool any2_at_leaf(const Any2& q) { return q.inst != OL_NONE && child_is_l
 template <int LSTACK>
 OL_DEV void any2_next(Any2& q, TravStack<LSTACK>& st) { q.node = st.empty() ? OL_NONE : st.pop(); }
 
+OL_DEV bool any2_at_entry(const Any2& q) { return q.inst == OL_NONE && child_is_leaf(q.node); }
+
 template <int LSTACK>
-OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& ni) {
+OL_DEV void any2_enter(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& ni) {
+  const uint32_t k = leaf_first(q.node);  // the instance table is in top-level leaf order
+  const InstanceDev& I = sc.inst[k];
+  if (!sphere_before(q.w, I.sphere, q.tmax)) {
+    any2_next(q, st);
+    return;
+  }
+  const BlasDev& B = sc.blas[I.blas];
+  if (B.node_count == 0) {
+    any2_next(q, st);
+    return;
+  }
+  ++ni;
+  st.push(OL_SENTINEL);
+  q.inst = k;
+  q.bnodes = B.nodes;
+  q.btri = B.tri;
+  q.r = ray_pre_fast(xform_point(I.w2o, q.w.o), xform_vector(I.w2o, q.w.d));
+  q.node = I.root;
+}
+
+template <int LSTACK>
+OL_DEV void any2_node(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& nv) {
   if (q.node == OL_SENTINEL) {
     q.inst = OL_NONE;
     q.r = q.w;
     any2_next(q, st);
-    return;
-  }
-  if (q.inst == OL_NONE && child_is_leaf(q.node)) {
-    const uint32_t k = __float_as_uint(sc.tri[3u * leaf_first(q.node)].w);
-    const InstanceDev& I = sc.inst[k];
-    const BlasDev& B = sc.blas[I.blas];
-    if (B.node_count == 0 || !sphere_reachable(q.w, I.sphere)) {
-      any2_next(q, st);
-      return;
-    }
-    ++ni;
-    st.push(OL_SENTINEL);
-    q.inst = k;
-    q.bnodes = B.nodes;
-    q.btri = B.tri;
-    q.r = ray_pre(xform_point(I.w2o, q.w.o), xform_vector(I.w2o, q.w.d));
-    q.node = 0;
     return;
   }
   const WideNode* nodes = q.inst == OL_NONE ? sc.nodes : q.bnodes;
@@ -630,6 +672,12 @@ OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_
   ol_node_keys(w0, w1, w2, q.r, q.tmin, q.tcull, key, ch);
   build_order(key, ch);
   descend(st, ch, key, q.node);
+}
+
+template <int LSTACK>
+OL_DEV void any2_inner(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& nv, uint32_t& ni) {
+  if (any2_at_entry(q)) any2_enter<LSTACK>(sc, q, st, ni);
+  else any2_node<LSTACK>(sc, q, st, nv);
 }
 
 template <int LSTACK>
