@@ -122,10 +122,12 @@ def traversal_bytes(w):
             + w["shadow_rays"] * (RAY_IN + OCC_OUT))
 
 
-def shade_bytes(w, instanced=False):
-    """Algorithmic bytes of a frame's shading launches (constants above)."""
+def shade_bytes(w, instanced=False, face_params=False):
+    """Algorithmic bytes of a frame's shading launches (constants above);
+    ``face_params``: each hit also reads its triangle's 16-byte base colour
+    and roughness."""
     closest = w["primary_rays"] + w["extension_rays"]
-    hit = SHADE_RECORD + SHADE_MATERIAL + (INST_BYTES if instanced else 0)
+    hit = SHADE_RECORD + SHADE_MATERIAL + (INST_BYTES if instanced else 0) + (16 if face_params else 0)
     return (closest * (SHADE_RAY + SHADE_RADIANCE) + w["extension_rays"] * (SHADE_STATE + SHADE_CONT)
             + w["shaded_hits"] * hit + w["shadow_rays"] * (SHADE_ENTRY + SHADE_GROUP)
             + w["area_samples"] * SHADE_AREA)
@@ -542,7 +544,9 @@ def main(argv=None):
         launches_per_frame = trace_launches / args.steps
         roofline = roofline_of("k_trace", traversal_bytes(w) / launches_per_frame, trace_ms / trace_launches,
                                measured_traffic(TRAFFIC_C2, workload, "k_trace"), TRAFFIC_C2)
-        shade_roofline = roofline_of("k_shade", shade_bytes(w) / launches_per_frame, shade_ms / trace_launches,
+        faces = ds.packed.face_params is not None
+        shade_roofline = roofline_of("k_shade", shade_bytes(w, face_params=faces) / launches_per_frame,
+                                     shade_ms / trace_launches,
                                      measured_traffic(TRAFFIC_C2, workload, "k_shade"), TRAFFIC_C2)
     line = {
         "metric": "Mrays/s at 1080p 8-bounce; batched jobs/s + p50/p99 latency vs plain GPU",

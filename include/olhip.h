@@ -84,6 +84,12 @@ typedef struct olhip_scene_desc {
   const float* uvs;            /* [vertex_count][4] uv0.xy, uv1.xy (textured materials) */
   const float* tangents;       /* [vertex_count][4] world tangent, handedness */
   const float* uv_density;     /* [triangle_count] sqrt(uv0 area / world area) */
+  /* Per-triangle material parameters (NULL without): base colour rgb,
+   * roughness and metallic factor of each triangle, replacing those words of
+   * its material record -- the per-triangle records of
+   * Scene.triangle_material_data (scene/_core.py:2816-2924) stored as the
+   * five words that differ between the triangles of a mesh. */
+  const float* face_params;    /* [triangle_count][5] */
 } olhip_scene_desc;
 
 #define OLHIP_BUILDER_LBVH 0u

@@ -40,6 +40,7 @@ class SceneDesc(ctypes.Structure):
         ("area_light_weight", c_f32), ("leaf_size", c_u32), ("builder", c_u32),
         ("texture_word_count", c_u32), ("texture_words", P_u32), ("binding_count", c_u32),
         ("bindings", P_f32), ("uvs", P_f32), ("tangents", P_f32), ("uv_density", P_f32),
+        ("face_params", P_f32),
     ]
 
 

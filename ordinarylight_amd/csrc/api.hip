@@ -191,6 +191,27 @@ uint32_t upload_textures(olhip_scene* sc, uint32_t word_count, const uint32_t* w
   return tex_count;
 }
 
+// Per-triangle material parameters of a flat scene (original order on the
+// device: base colour and roughness as float4, metallic as float), or none.
+void upload_faces(olhip_scene* sc, const olhip_scene_desc* d, hipStream_t s) {
+  if (!d->face_params) {
+    sc->face_orig.release();
+    sc->face_metal.release();
+    return;
+  }
+  const uint32_t T = d->triangle_count;
+  std::vector<float4> f(T);
+  std::vector<float> m(T);
+  for (uint32_t t = 0; t < T; ++t) {
+    const float* q = d->face_params + 5ull * t;
+    for (int k = 0; k < 5; ++k) require(std::isfinite(q[k]), "per-triangle material parameters must be finite");
+    f[t] = make_float4(q[0], q[1], q[2], q[3]);
+    m[t] = q[4];
+  }
+  upload(sc->face_orig, f.data(), f.size(), s);
+  upload(sc->face_metal, m.data(), m.size(), s);
+}
+
 // Environment light of the analytic records (type 3): constant radiance, or
 // an image when row 3 names a texture of `texture_count`.
 void set_environment(DScene& v, const float* analytic, uint32_t count, uint32_t texture_count) {
@@ -357,11 +378,16 @@ int olhip_scene_create(olhip_ctx* ctx, const olhip_scene_desc* d, olhip_scene** 
     BvhInput in{sc->positions.as<float4>(), sc->vidx.as<uint4>(), d->triangle_count, d->leaf_size,
                 boxes.empty() ? nullptr : boxes.data()};
     bvh_build(in, sc->bvh, s, nullptr, true);
-    build_shade_records(sc->bvh, sc->vidx.as<uint4>(), sc->normals.as<float4>(), d->triangle_count, true, s);
+    upload_faces(sc.get(), d, s);
+    build_shade_records(sc->bvh, sc->vidx.as<uint4>(), sc->normals.as<float4>(), d->triangle_count, true, s,
+                        d->face_params ? sc->face_orig.as<float4>() : nullptr,
+                        d->face_params ? sc->face_metal.as<float>() : nullptr);
     DScene& v = sc->view;
     v.nodes = sc->bvh.nodes.as<WideNode>();
     v.tri = sc->bvh.tri.as<float4>();
     v.shade = sc->bvh.shade.as<float4>();
+    v.face = d->face_params ? sc->bvh.face.as<float4>() : nullptr;
+    v.face_orig = d->face_params ? sc->face_orig.as<float4>() : nullptr;
     v.tri_vidx = sc->vidx.as<uint4>();
     v.normals = sc->normals.as<float4>();
     v.materials = sc->materials.as<float4>();
@@ -643,9 +669,14 @@ int olhip_scene_update(olhip_ctx* ctx, olhip_scene* sc, const olhip_scene_desc* 
       v.normals = sc->normals.as<float4>();
       v.node_count = sc->bvh.node_count;
     }
-    // Normals or material ids may have changed.
-    build_shade_records(sc->bvh, sc->vidx.as<uint4>(), sc->normals.as<float4>(), d->triangle_count, true, s);
+    // Normals, material ids or per-triangle parameters may have changed.
+    upload_faces(sc, d, s);
+    build_shade_records(sc->bvh, sc->vidx.as<uint4>(), sc->normals.as<float4>(), d->triangle_count, true, s,
+                        d->face_params ? sc->face_orig.as<float4>() : nullptr,
+                        d->face_params ? sc->face_metal.as<float>() : nullptr);
     sc->view.shade = sc->bvh.shade.as<float4>();
+    sc->view.face = d->face_params ? sc->bvh.face.as<float4>() : nullptr;
+    sc->view.face_orig = d->face_params ? sc->face_orig.as<float4>() : nullptr;
     sc->has_transmission = false;
     upload_shading(sc, d->material_count, d->materials, d->analytic_light_count, d->analytic_lights,
                    d->area_light_count, d->area_lights, d->area_light_weight, s);

@@ -287,7 +287,8 @@ __global__ void __launch_bounds__(256) k_e0_shade(E0Params P, E0State S, uint32_
     ny = ny * -1.0f;
     nz = nz * -1.0f;
   }
-  const float4 em = P.sc.materials[12u * vi.w + 1u], co = P.sc.materials[12u * vi.w];
+  const float4 em = P.sc.materials[12u * vi.w + 1u];
+  const float4 co = P.sc.face_orig ? P.sc.face_orig[tri] : P.sc.materials[12u * vi.w];
   const float emis[3] = {em.x, em.y, em.z}, col[3] = {co.x, co.y, co.z};
   const float nrm[3] = {nx, ny, nz};
   for (int q = 0; q < 3; ++q) {

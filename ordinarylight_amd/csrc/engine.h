@@ -33,6 +33,7 @@ struct olhip_scene {
   olhip_ctx* ctx = nullptr;
   DevBuf positions, normals, vidx, materials, analytic, area, area_guide;
   DevBuf tex_words, bindings, uvs, tangents, uv_density;  // textured scenes
+  DevBuf face_orig, face_metal;  // per-triangle material parameters, original order
   BvhOutput bvh;  // the scene's BVH, or the top-level BVH over instances
   // Two-level scenes: bottom-level BVHs per resource, instance and BLAS tables.
   std::vector<std::unique_ptr<BvhOutput>> blas;

@@ -196,8 +196,11 @@ int stack_depth_for(uint32_t bvh_depth);
 constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u, STEP_TEXTURED = 8u;
 // Fills b.shade from the hierarchy's leaf triangles, the triangles' vertex
 // indices (material in .w when `material`) and vertex normals.
+// face_in / metal_in (original order, optional): per-triangle base colour
+// and roughness, and metallic factor, placed in leaf order (b.face, record
+// word 11).
 void build_shade_records(BvhOutput& b, const uint4* vidx, const float4* normals, uint32_t count, bool material,
-                         hipStream_t st);
+                         hipStream_t st, const float4* face_in = nullptr, const float* metal_in = nullptr);
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
 void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);

@@ -954,7 +954,9 @@ OL_DEV void restir_nee(const GiParams& P, const NeeSample& chosen, uint32_t& rs,
 // resolve_surface needs, gathered once per hierarchy instead of per hit.
 __global__ void __launch_bounds__(256) k_shade_records(const float4* __restrict__ tri, const uint4* __restrict__ vidx,
                                                        const float4* __restrict__ normals, uint32_t count,
-                                                       uint32_t material, float4* __restrict__ rec) {
+                                                       uint32_t material, float4* __restrict__ rec,
+                                                       const float4* __restrict__ face_in,
+                                                       const float* __restrict__ metal_in, float4* __restrict__ face) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= count) return;
   const uint32_t id = __float_as_uint(tri[3u * s].w);
@@ -962,8 +964,16 @@ __global__ void __launch_bounds__(256) k_shade_records(const float4* __restrict_
   const f3 cw = cross3(mk3(tri[3u * s + 1u]), mk3(tri[3u * s + 2u]));
   rec[4u * s] = mk4(rgb(normals[vi.x]), __uint_as_float(material ? vi.w : 0u));
   rec[4u * s + 1u] = mk4(rgb(normals[vi.y]), __uint_as_float(id));
-  rec[4u * s + 2u] = mk4(rgb(normals[vi.z]), 0.0f);
+  rec[4u * s + 2u] = mk4(rgb(normals[vi.z]), face_in ? metal_in[id] : 0.0f);
   rec[4u * s + 3u] = mk4(cw, 0.0f);
+  if (face_in) face[s] = face_in[id];
+}
+
+// A hit's per-triangle material parameters (scenes with them): base colour,
+// roughness and metallic factor replace the record's.
+OL_DEV void apply_face(const DScene& sc, uint32_t slot, Mat& mat) {
+  mat.br = sc.face[slot];
+  mat.em.w = sc.shade[4u * slot + 2u].w;
 }
 
 // Hit reconstruction (shadeLoadHit) and surface (shadeResolveSurface,
@@ -1131,6 +1141,7 @@ __global__ void __launch_bounds__(256) k_texture(Wave W, uint32_t in, uint32_t g
     resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
     f3 sn = entering ? nrm : -nrm;
     Mat mat = load_mat(sc.materials, mat_id);
+    if (sc.face) apply_face(sc, slot, mat);
     const float width = cone.x + hv.x * cone.y;
     if (sc.materials[12u * mat_id + 8u].w == 1.0f) texture_surface(sc, mat_id, prim, hv.y, hv.z, gn, width, mat, sn);
     float4* out = W.tex + 4u * i;
@@ -1191,6 +1202,7 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t ge
     bool entering;
     resolve_surface(W, sc, i, slot, ro, rd, hv, hitp, gn, nrm, cw, prim, mat_id, entering);
     Mat mat = load_mat(sc.materials, mat_id);
+    if (sc.face) apply_face(sc, slot, mat);
     if (sc.textured) {
       textured_material(W, i, mat);
       const f3 sn = rgb(W.tex[4u * i + 3u]);
@@ -1293,6 +1305,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           J.prim[M.x] = prim;
         }
         Mat mat = load_mat(sc.materials, mat_id);
+        if (sc.face) apply_face(sc, slot, mat);
         if (sc.textured) {
           textured_material(W, i, mat);
           const f3 sn = rgb(W.tex[4u * i + 3u]);
@@ -1488,6 +1501,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           uint32_t rs2 = nee_rng;
           uint32_t en;
           Mat mat = load_mat(sc.materials, nee_mat);
+          if (sc.face) apply_face(sc, __float_as_uint(W.hit[i].w), mat);
           if (sc.textured) textured_material(W, i, mat);
           restir_nee(P, single, rs2, hitp, nrm, mat, incoming, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
@@ -1503,6 +1517,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           uint32_t kk = 0, an, en;
           uint32_t rs2 = nee_rng;
           Mat mat = load_mat(sc.materials, nee_mat);
+          if (sc.face) apply_face(sc, __float_as_uint(W.hit[i].w), mat);
           if (sc.textured) textured_material(W, i, mat);
           for_each_nee(P, rs2, hitp, nrm, mat, incoming, an, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
@@ -1670,11 +1685,13 @@ uint64_t trace_spill_words(const GiLaunch& l) { OL_STACK_DISPATCH(spill_words_fo
 void wave_clear(const Wave& w, hipStream_t st) { hipLaunchKernelGGL(k_clear, dim3(1), dim3(1024), 0, st, w); }
 
 void build_shade_records(BvhOutput& b, const uint4* vidx, const float4* normals, uint32_t count, bool material,
-                         hipStream_t st) {
+                         hipStream_t st, const float4* face_in, const float* metal_in) {
   if (count == 0) return;
   b.shade.reserve(sizeof(float4) * 4ull * count);
+  if (face_in) b.face.reserve(sizeof(float4) * count);
   hipLaunchKernelGGL(k_shade_records, dim3((count + 255u) / 256u), dim3(256), 0, st, b.tri.as<float4>(), vidx, normals,
-                     count, material ? 1u : 0u, b.shade.as<float4>());
+                     count, material ? 1u : 0u, b.shade.as<float4>(), face_in, metal_in,
+                     face_in ? b.face.as<float4>() : nullptr);
   OL_HIP_CHECK(hipGetLastError());
 }
 

@@ -100,6 +100,10 @@ struct BvhOutput {
   // shaded scenes): (n0, material bits), (n1, triangle id bits), (n2, 0),
   // (e1 x e2, 0), so a hit's surface is one 64-byte read.
   DevBuf shade;
+  // float4[T] per-triangle base colour and roughness in leaf order (scenes
+  // with per-triangle material parameters; the metallic factor is word 11 of
+  // the shading record).
+  DevBuf face;
   uint32_t node_count = 0;
   uint32_t depth = 0;      // wide levels (root = level 1)
   double build_ms = 0.0;
@@ -183,6 +187,11 @@ struct DScene {
   // first emitter whose cumulative weight reaches j / kAreaGuide (clamped to
   // the last), so a selection in bucket j lies between entries j and j + 1.
   const uint32_t* area_guide;
+  // Per-triangle material parameters (null without): base colour and
+  // roughness per leaf slot (the metallic factor is the shading record's
+  // word 11), and per original triangle (the reference estimator's colours).
+  const float4* face;
+  const float4* face_orig;
   // Textures (null / -1 without): the texture word array, binding records
   // (2 float4 each), per-vertex uv0.xy uv1.xy and world tangents, per-triangle
   // uv0 density and world positions (flat scenes only).

@@ -80,14 +80,14 @@ class _Bindings:
         return float(next(i for i, (t, f) in enumerate(self.bindings) if t is tex and f == tf))
 
 
-def _material_block(material, count, faces, bindings):
-    """Material record(s): one, or one per triangle over face materials,
-    with texture binding indices (Scene.triangle_material_data,
-    scene/_core.py:2816-2924).  Untextured records hold the resolved
-    occlusion factor 1 in row 5 .w (the value the reference's textured path
-    computes when no occlusion map is bound); textured ones keep the
-    transmission binding there, as the reference does, and set the flag."""
-    block = faces.records(material) if faces is not None else material_record(material)[None]
+def _material_block(material, bindings):
+    """One material record with texture binding indices
+    (Scene.triangle_material_data, scene/_core.py:2816-2924).  Untextured
+    records hold the resolved occlusion factor 1 in row 5 .w (the value the
+    reference's textured path computes when no occlusion map is bound);
+    textured ones keep the transmission binding there, as the reference
+    does, and set the flag."""
+    block = material_record(material)[None]
     if not is_textured(material):
         block[:, 5, 3] = 1.0
         return block
@@ -118,6 +118,20 @@ def _environment_words(environment, analytic, analytic_count, texture_list):
     return tx.texture_words(texture_list) if texture_list else None
 
 
+def expanded_materials(packed):
+    """(records [T, 12, 4], ids [T]): one full material record per triangle
+    with its per-triangle parameters written in, the form the reference's
+    Scene.triangle_material_data has; (materials, material_ids) unchanged
+    when the scene has none."""
+    if packed.face_params is None:
+        return packed.materials, packed.material_ids
+    records = packed.materials[packed.material_ids].copy()
+    records[:, 0, :3] = packed.face_params[:, :3]
+    records[:, 0, 3] = packed.face_params[:, 3]
+    records[:, 1, 3] = packed.face_params[:, 4]
+    return np.ascontiguousarray(records, np.float32), np.arange(len(records), dtype=np.uint32)
+
+
 def _uv_density(world, tri, uv):
     """sqrt(uv area / world area) per triangle (triangle_attribute_data,
     scene/_core.py:2042-2069)."""
@@ -144,6 +158,12 @@ class PackedScene:
     area_light_weight: float
     environment: object
     has_textures: bool
+    # Per-triangle GGX parameters (meshes with face materials; None when no
+    # mesh has them): base colour rgb, roughness, metallic replacing those
+    # words of the triangle's material record; other triangles repeat their
+    # record's values.  expanded_materials() gives one full record per
+    # triangle instead.
+    face_params: np.ndarray | None = None    # float32 [T, 5]
     # Textured scenes only (None otherwise):
     uvs: np.ndarray | None = None            # float32 [V, 4] uv0.xy, uv1.xy
     tangents: np.ndarray | None = None       # float32 [V, 4] world tangent, handedness
@@ -174,6 +194,8 @@ def pack_scene(scene):
         bindings.add(mesh.material)
     textured = bool(bindings.textures) or getattr(getattr(scene, "environment", None), "image", None) is not None
     uvs, tangents, density = [], [], []
+    any_faces = any(getattr(m, "face_materials", None) is not None for m in meshes)
+    face_params = []
     for mesh in meshes:
         tri = np.asarray(mesh.indices, dtype=np.uint32)
         world = np.asarray(mesh.world_vertices, dtype=np.float32)
@@ -187,18 +209,18 @@ def pack_scene(scene):
             tangents.append(np.asarray(mesh.world_tangents, np.float32))
             density.append(_uv_density(world, tri, uv0))
         material = mesh.material
-        faces = getattr(mesh, "face_materials", None)
-        if faces is not None:
-            # One record per triangle, appended in triangle order.
-            first = sum(len(m) for m in materials)
-            materials.append(_material_block(material, len(tri), faces, bindings))
-            material_ids.append(np.arange(first, first + len(tri), dtype=np.uint32))
-            continue
         key = id(material)
         if key not in material_index:
             material_index[key] = sum(len(m) for m in materials)
-            materials.append(_material_block(material, 1, None, bindings))
+            materials.append(_material_block(material, bindings))
         material_ids.append(np.full(len(tri), material_index[key], np.uint32))
+        if any_faces:
+            faces = getattr(mesh, "face_materials", None)
+            if faces is not None:
+                face_params.append(np.column_stack((faces.base_color, faces.roughness, faces.metallic)))
+            else:
+                rec = materials[material_index[key]][0]
+                face_params.append(np.broadcast_to(np.array((*rec[0], rec[1, 3]), np.float32), (len(tri), 5)))
     if not materials:
         record = material_record(type("Default", (), {
             "base_color": (0.8, 0.8, 0.8), "emission": (0.0, 0.0, 0.0),
@@ -228,6 +250,7 @@ def pack_scene(scene):
         area_light_weight=float(scene.emissive_light_weight),
         environment=getattr(scene, "environment", None),
         has_textures=textured,
+        face_params=cat(face_params, (0, 5), np.float32) if any_faces else None,
         uvs=cat(uvs, (0, 4), np.float32) if bindings.textures else None,
         tangents=cat(tangents, (0, 4), np.float32) if bindings.textures else None,
         uv_density=cat(density, (0,), np.float32) if bindings.textures else None,
@@ -392,5 +415,5 @@ def pack_instanced_scene(scene):
     )
 
 
-__all__ = ["PackedScene", "PackedInstancedScene", "has_textured_materials", "pack_scene", "pack_instanced_scene",
-           "shares_resources"]
+__all__ = ["PackedScene", "PackedInstancedScene", "expanded_materials", "has_textured_materials", "pack_scene",
+           "pack_instanced_scene", "shares_resources"]

@@ -210,6 +210,8 @@ def _scene_desc(packed, leaf_size, builder="lbvh"):
         desc.uvs = N.ptr(packed.uvs)
         desc.tangents = N.ptr(packed.tangents)
         desc.uv_density = N.ptr(packed.uv_density)
+    if packed.face_params is not None:
+        desc.face_params = N.ptr(packed.face_params)
     return desc
 
 

@@ -90,10 +90,20 @@ def test_terrain_workload_shape():
     assert len(rough) == 100 * 50 * 2
     assert rough.min() >= 0.05 and rough.max() <= 1.0 and set(np.unique(metal)) == {0.0, 1.0}
     p = packing.pack_scene(scene)
-    # One material record per terrain triangle (+ the emitters' material).
-    assert len(p.materials) == 100 * 50 * 2 + 1
-    np.testing.assert_array_equal(p.materials[p.material_ids[:len(rough)], 0, 3], rough)
-    np.testing.assert_array_equal(p.materials[p.material_ids[:len(rough)], 1, 3], metal)
+    # Per-triangle parameters over the terrain chunks' shared records; the
+    # expanded form has one full record per triangle.
+    assert p.face_params.shape == (scene.triangle_count, 5)
+    np.testing.assert_array_equal(p.face_params[:len(rough), 3], rough)
+    np.testing.assert_array_equal(p.face_params[:len(rough), 4], metal)
+    full, ids = packing.expanded_materials(p)
+    assert len(full) == scene.triangle_count
+    np.testing.assert_array_equal(full[ids[:len(rough)], 0, 3], rough)
+    np.testing.assert_array_equal(full[ids[:len(rough)], 1, 3], metal)
+    base = np.concatenate([f.base_color for f in faces])
+    np.testing.assert_array_equal(full[:len(rough), 0, :3], base)
+    # Triangles without face materials keep their record's values.
+    emit = p.materials[p.material_ids[-1]]
+    np.testing.assert_array_equal(full[-1], emit)
     chunked, _ = workloads.terrain(0, nx=101, nz=51, chunk=25, emissive_quads=32, materials="chunk")
     assert len(packing.pack_scene(chunked).materials) == 4 * 2 + 1
 
