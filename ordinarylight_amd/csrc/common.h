@@ -121,6 +121,37 @@ OL_DEV uint32_t wave_reserve(uint32_t* counter, bool want) {
   base = __shfl(base, (int)leader, 64);
   return base + lanes_below(mask);
 }
+// Shading's three reservations at once: one slot per lane with `a` (counter
+// ca), one per lane with `b` (cb) and `n` consecutive slots per lane (cn).
+// The leader issues the three atomics back to back, so their round trips
+// overlap instead of following each other.  Must be reached by every active
+// lane of the wave.
+struct Reserve3 {
+  uint32_t a, b, n;
+};
+OL_DEV Reserve3 wave_reserve3(uint32_t* ca, bool a, uint32_t* cb, bool b, uint32_t* cn, uint32_t n) {
+  const uint64_t ma = __ballot(a), mb = __ballot(b);
+  uint32_t incl = n;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t v = __shfl_up(incl, (unsigned)off, 64);
+    if ((int)lane_id() >= off) incl += v;
+  }
+  const uint64_t active = __ballot(true);
+  const uint32_t last = 63u - (uint32_t)__clzll((long long)active);
+  const uint32_t total = __shfl(incl, (int)last, 64);
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)active) - 1u;
+  uint32_t base_a = 0, base_b = 0, base_n = 0;
+  if (lane_id() == leader) {
+    if (ma != 0) base_a = atomicAdd(ca, (uint32_t)__popcll(ma));
+    if (mb != 0) base_b = atomicAdd(cb, (uint32_t)__popcll(mb));
+    if (total != 0) base_n = atomicAdd(cn, total);
+  }
+  Reserve3 r;
+  r.a = __shfl(base_a, (int)leader, 64) + lanes_below(ma);
+  r.b = __shfl(base_b, (int)leader, 64) + lanes_below(mb);
+  r.n = __shfl(base_n, (int)leader, 64) + incl - n;
+  return r;
+}
 // Adds the number of active lanes to a 64-bit counter with one atomic per
 // wave (work counters).
 OL_DEV void count_active(unsigned long long* counter) {

@@ -1427,7 +1427,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           }
           o = normalize3(o);
           const float pdf = fmaxf(pdf_pbr(mat, nrm, view, o), 1e-6f);
-          f3 w = (eval_pbr(mat, nrm, view, o) * fmaxf(dot3(nrm, o), 0.0f)) / pdf;
+          f3 w = fdiv3(eval_pbr(mat, nrm, view, o) * fmaxf(dot3(nrm, o), 0.0f), pdf);
           w = w * mixf(mat.tp.w, 1.0f, mat.em.w);
           T.x = T.x * w.x;
           T.y = T.y * w.y;
@@ -1473,8 +1473,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       }
       W.L[path] = L;
     }
-    // Continuation slot in the job's output segment (one atomic per wave).
-    const uint32_t cl = wave_reserve(&W.ctr[c_queue(out) + run_word(run)], cont);
+    // Continuation slot in the job's output segment, the shadow group's slot
+    // and its entries' slots (three atomics per wave, issued together).
+    const bool has_group = nee && nee_count > 0u;
+    const Reserve3 rv = wave_reserve3(&W.ctr[c_queue(out) + run_word(run)], cont,
+                                      &W.ctr[c_groups(in) + run_word(run)], has_group,
+                                      &W.ctr[c_entries(in) + run_word(run)], has_group ? nee_count : 0u);
+    const uint32_t cl = rv.a;
     if (cont) {
       if (cl < J.append) {
         const uint32_t cslot = J.base + so * J.seg + cl;
@@ -1487,9 +1492,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       }
     }
     // Shadow group + entries.
-    const bool has_group = nee && nee_count > 0u;
-    const uint32_t gl = wave_reserve(&W.ctr[c_groups(in) + run_word(run)], has_group);
-    const uint32_t el = wave_reserve_n(&W.ctr[c_entries(in) + run_word(run)], has_group ? nee_count : 0u);
+    const uint32_t gl = rv.b, el = rv.n;
     if (has_group) {
       if (gl < J.append && el + nee_count <= J.eseg) {
         const uint32_t gslot = J.base + so * J.seg + gl, eslot = J.ebase + so * J.eseg + el;

@@ -57,6 +57,16 @@ OL_DEV f3 cosine_hemisphere(f3 n, float u, float v) {
   return normalize3(((tangent * (radius * cp)) + (bitangent * (radius * sp))) + (n * sqrtf(fmaxf(0.0f, 1.0f - u))));
 }
 
+// Division inside the BSDF and light-sample weights: a * rcp(b), within 2
+// ulps as GLSL allows its divisions (2.5 ulp), instead of a correctly
+// rounded sequence.  Sampled directions and the decisions drawn against
+// random numbers keep exact divisions.
+OL_DEV float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+OL_DEV f3 fdiv3(f3 a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  return mk3(a.x * r, a.y * r, a.z * r);
+}
+
 OL_DEV f3 fresnel_schlick(f3 f0, float c) {
   float k = pow5(1.0f - clampf(c, 0.0f, 1.0f));
   return f0 + (mk3(1.0f, 1.0f, 1.0f) - f0) * k;
@@ -66,13 +76,13 @@ OL_DEV float ggx_d(float nh, float roughness) {
   float alpha = fmaxf(roughness * roughness, 0.0009f);
   float a2 = alpha * alpha;
   float den = ((nh * nh) * (a2 - 1.0f)) + 1.0f;
-  return a2 / fmaxf((OL_PI * den) * den, 1e-6f);
+  return fdiv(a2, fmaxf((OL_PI * den) * den, 1e-6f));
 }
 
 OL_DEV float ggx_g1(float nd, float roughness) {
   float alpha = fmaxf(roughness * roughness, 0.0009f);
   float a2 = alpha * alpha;
-  return (2.0f * nd) / fmaxf(nd + sqrtf(a2 + ((1.0f - a2) * nd) * nd), 1e-6f);
+  return fdiv(2.0f * nd, fmaxf(nd + sqrtf(a2 + ((1.0f - a2) * nd) * nd), 1e-6f));
 }
 
 OL_DEV f3 f0_of(const Mat& m) { return mix3(mk3(0.04f, 0.04f, 0.04f), rgb(m.br), m.em.w); }
@@ -97,8 +107,8 @@ OL_DEV f3 eval_pbr(const Mat& m, f3 n, f3 view, f3 out) {
   float D = ggx_d(nh, eff_r);
   float G = ggx_g1(nv, eff_r) * ggx_g1(nl, eff_r);
   float den = fmaxf((4.0f * nv) * nl, 1e-6f);
-  f3 spec = ((fr * D) * G) / den;
-  f3 diffuse = (((mk3(1.0f, 1.0f, 1.0f) - fr) * (1.0f - metallic)) * base) / OL_PI;
+  f3 spec = fdiv3((fr * D) * G, den);
+  f3 diffuse = fdiv3(((mk3(1.0f, 1.0f, 1.0f) - fr) * (1.0f - metallic)) * base, OL_PI);
   float subsurface = clampf(m.a1.x, 0.0f, 1.0f);
   diffuse = mix3(diffuse, diffuse * rgb(m.ss), subsurface);
   float sv = pow5(1.0f - vh);
@@ -108,7 +118,7 @@ OL_DEV f3 eval_pbr(const Mat& m, f3 n, f3 view, f3 out) {
   float cD = ggx_d(nh, coat_r);
   float cG = ggx_g1(nv, coat_r) * ggx_g1(nl, coat_r);
   float cF = 0.04f + 0.96f * sv;
-  float coat = (((clearcoat * cD) * cG) * cF) / den;
+  float coat = fdiv(((clearcoat * cD) * cG) * cF, den);
   float layer = 1.0f - clearcoat * cF;
   f3 d = diffuse * layer;
   f3 s = ((spec + sheen) * layer) + mk3(coat, coat, coat);
@@ -121,8 +131,8 @@ OL_DEV float pdf_pbr(const Mat& m, f3 n, f3 view, f3 out) {
   f3 h = normalize3(view + out);
   float nh = fmaxf(dot3(n, h), 0.0f);
   float vh = fmaxf(dot3(view, h), 1e-6f);
-  float sp = (ggx_d(nh, m.br.w) * nh) / (4.0f * vh);
-  float dp = nl / OL_PI;
+  float sp = fdiv(ggx_d(nh, m.br.w) * nh, 4.0f * vh);
+  float dp = fdiv(nl, OL_PI);
   return mixf(dp, sp, spec_probability(m));
 }
 
@@ -143,7 +153,7 @@ OL_DEV f3 ggx_half(f3 n, float roughness, float u, float v) {
 
 OL_DEV float power_heuristic(float a, float b) {
   float a2 = a * a, b2 = b * b;
-  return a2 / fmaxf(a2 + b2, 1e-6f);
+  return fdiv(a2, fmaxf(a2 + b2, 1e-6f));
 }
 
 OL_DEV f3 analytic_environment(f3 d) {
@@ -269,7 +279,7 @@ OL_DEV NeeSample area_sample(const DScene& sc, uint32_t& rs, f3 hit, f3 n, const
   float lc = fmaxf(raw, 0.0f);
   if (dist.z > 0.5f) lc = fabsf(raw);
   if (sc_cos <= 0.0f || lc <= 1e-6f) return s;
-  float light_pdf = (dist.y * d2) / fmaxf(lc * ea.w, 1e-6f);
+  float light_pdf = fdiv(dist.y * d2, fmaxf(lc * ea.w, 1e-6f));
   float eff = light_pdf * fmaxf(technique, 1e-6f);
   s.origin = hit + n * 0.002f;
   s.dir = dir;
@@ -278,7 +288,7 @@ OL_DEV NeeSample area_sample(const DScene& sc, uint32_t& rs, f3 hit, f3 n, const
   f3 view = -incoming;
   float bp = pdf_pbr(m, n, view, dir);
   float mis = power_heuristic(eff * (float)sample_count, bp);
-  s.c = ((((eval_pbr(m, n, view, dir) * rgb(ea)) * sc_cos) * mis) * mk3(1.0f, 1.0f, 1.0f)) / fmaxf(eff, 1e-6f);
+  s.c = fdiv3((((eval_pbr(m, n, view, dir) * rgb(ea)) * sc_cos) * mis) * mk3(1.0f, 1.0f, 1.0f), fmaxf(eff, 1e-6f));
   s.valid = true;
   return s;
 }
@@ -293,7 +303,7 @@ OL_DEV NeeSample env_sample(const DScene& sc, uint32_t& rs, f3 hit, f3 n, const 
   float v = pcg_value(rs);
   f3 dir = cosine_hemisphere(n, u, v);
   float cosine = fmaxf(dot3(n, dir), 0.0f);
-  float pdf = cosine / OL_PI;
+  float pdf = fdiv(cosine, OL_PI);
   float eff = pdf * fmaxf(technique, 1e-6f);
   s.origin = hit + n * 0.002f;
   s.dir = dir;
@@ -301,11 +311,11 @@ OL_DEV NeeSample env_sample(const DScene& sc, uint32_t& rs, f3 hit, f3 n, const 
   s.valid = cosine > 0.0f;
   if (!s.valid) return s;
   // shadeEnvironmentContribution with unit volume transmittance.
-  float pdf2 = cosine / OL_PI;
+  float pdf2 = fdiv(cosine, OL_PI);
   float mis = power_heuristic(eff * (float)sample_count, pdf2);
   float mw = mixf(m.tp.w, 1.0f, m.em.w);
-  s.c = (((((eval_pbr(m, n, -incoming, dir) * environment_radiance(sc, dir)) * cosine) * mis) * mk3(1.0f, 1.0f, 1.0f)) /
-         fmaxf(eff, 1e-6f)) * mw;
+  s.c = fdiv3((((eval_pbr(m, n, -incoming, dir) * environment_radiance(sc, dir)) * cosine) * mis) *
+                  mk3(1.0f, 1.0f, 1.0f), fmaxf(eff, 1e-6f)) * mw;
   return s;
 }
 
