@@ -227,6 +227,7 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
         settings, tuned = batch.EngineSettings(), {}
     with batch.BatchEngine(context, settings) as engine:
         batch.run_batch(engine, jobs)  # warm-up
+        steps0 = engine.stats()["steps"]
         batched_s, results = batch.run_batch(engine, jobs)
         est = engine.stats()
     batched_rays = sum(st["primary_rays"] + st["extension_rays"] + st["shadow_rays"] for _, st in results)
@@ -240,6 +241,7 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
         "mrays_per_s": round(batched_rays / batched_s / 1e6, 1),
         "plain_mrays_per_s": round(rays / plain_s / 1e6, 1),
         "max_resident_jobs": int(est["max_resident_jobs"]),
+        "engine_steps": int(est["steps"] - steps0),
         "settings": {"path_slots": settings.path_slots, "lds_stack": settings.lds_stack,
                      "trace_block": settings.trace_block},
         "autotune_ms": tuned,
@@ -274,6 +276,7 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8, c
         last_done = st["done_ms"] if last_done is None else max(last_done, st["done_ms"])
         if k < check:
             results.append(img)
+    est = engine.stats()
     engine.close()
     served = batch.latency_percentiles(lat)
     equal = all(np.array_equal(img, batch.plain_render(j)[0]) for img, j in zip(results, jobs[:check]))
@@ -282,6 +285,8 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8, c
         "jobs_per_s": round(n_jobs / ((last_done - first_submit) * 1e-3), 2),
         "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
         "results_equal_plain": bool(equal), "checked_jobs": check,
+        "engine_steps": int(est["steps"]), "engine_busy_ms": round(est["busy_ms"], 2),
+        "max_resident_jobs": int(est["max_resident_jobs"]),
     }
     if not compare_plain:
         return out
