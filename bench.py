@@ -13,10 +13,12 @@
    [128, 1024], eight resident scenes of 10k-2M triangles, spp in
    {1,2,4,8,16}, bounces in {4,8}; one engine run (autotuned capacity and
    launch shape) against the same jobs rendered back-to-back by the plain path.
-3. Served (``served``): Poisson arrivals of such jobs at twice the plain
-   path's back-to-back throughput; continuous-batching server against a
-   plain one-at-a-time server on the same arrivals; p50/p99 latency and
-   jobs/s; then a sweep of arrival rates at 0.5, 1 and 1.5 times the batched
+3. Served (``served``): 256 such jobs (another seed).  Their plain
+   back-to-back throughput and the engine's throughput with every job queued
+   at once are measured first; then Poisson arrivals at twice the plain
+   throughput through the continuous-batching server (jobs/s, p50/p99
+   latency; a plain one-at-a-time server on the same arrivals beside it),
+   and a sweep of arrival rates at 0.5, 1 and 1.5 times the all-queued
    throughput.
 4. Instanced (``c3``): a two-level scene of 10,000 instances of five
    20,480-triangle icospheres (~205M effective triangles), 20 % glass,
@@ -227,7 +229,8 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
         settings, tuned = batch.EngineSettings(), {}
     with batch.BatchEngine(context, settings) as engine:
         batch.run_batch(engine, jobs)  # warm-up
-        steps0 = engine.stats()["steps"]
+        st0 = engine.stats()
+        steps0, dev0 = st0["steps"], st0["device_ms"]
         batched_s, results = batch.run_batch(engine, jobs)
         est = engine.stats()
     batched_rays = sum(st["primary_rays"] + st["extension_rays"] + st["shadow_rays"] for _, st in results)
@@ -242,18 +245,19 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
         "plain_mrays_per_s": round(rays / plain_s / 1e6, 1),
         "max_resident_jobs": int(est["max_resident_jobs"]),
         "engine_steps": int(est["steps"] - steps0),
+        "device_busy_frac": round((est["device_ms"] - dev0) / (batched_s * 1e3), 3),
         "settings": {"path_slots": settings.path_slots, "lds_stack": settings.lds_stack,
                      "trace_block": settings.trace_block},
         "autotune_ms": tuned,
     }
 
 
-def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8, compare_plain=True):
-    """Poisson arrivals of mixed jobs: continuous-batching server vs a plain
-    one-job-at-a-time server on the same arrival sequence (``compare_plain``)."""
-    from ordinarylight_amd import batch, workloads
-    specs = workloads.mixed_jobs(seed, n_jobs)
-    jobs = make_jobs(specs, scenes)
+def served_run(context, jobs, settings, rate, seed, check=8, plain_server=False):
+    """Poisson arrivals of ``jobs`` at ``rate``/s through the continuous-batching
+    server; with ``plain_server`` also a plain one-job-at-a-time server on the
+    same arrivals."""
+    from ordinarylight_amd import batch
+    n_jobs = len(jobs)
     arrivals = batch.poisson_arrivals(rate, n_jobs, seed)
     engine = batch.BatchEngine(context, settings)
     engine.start()
@@ -281,32 +285,64 @@ def served_measurement(context, scenes, settings, n_jobs, rate, seed, check=8, c
     served = batch.latency_percentiles(lat)
     equal = all(np.array_equal(img, batch.plain_render(j)[0]) for img, j in zip(results, jobs[:check]))
     out = {
-        "jobs": n_jobs, "rate_per_s": rate,
+        "jobs": n_jobs, "rate_per_s": round(rate, 2),
         "jobs_per_s": round(n_jobs / ((last_done - first_submit) * 1e-3), 2),
         "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
         "results_equal_plain": bool(equal), "checked_jobs": check,
-        "engine_steps": int(est["steps"]), "engine_busy_ms": round(est["busy_ms"], 2),
+        "engine_steps": int(est["steps"]), "device_busy_frac": round(est["device_ms"] / (last_done - first_submit), 3),
         "max_resident_jobs": int(est["max_resident_jobs"]),
     }
-    if not compare_plain:
-        return out
-    # Plain server: jobs run one at a time in arrival order.
-    plain_lat = []
-    t0 = time.perf_counter()
-    clock = 0.0
-    for j, at in zip(jobs, arrivals):
-        now = time.perf_counter() - t0
-        if now < at:
-            time.sleep(at - now)
+    if plain_server:
+        plain_lat = []
+        t0 = time.perf_counter()
+        clock = 0.0
+        for j, at in zip(jobs, arrivals):
+            now = time.perf_counter() - t0
+            if now < at:
+                time.sleep(at - now)
+            batch.plain_render(j, copy_image=False)
+            clock = time.perf_counter() - t0
+            plain_lat.append((clock - at) * 1e3)
+        plain = batch.latency_percentiles(plain_lat)
+        out.update({"plain_server_jobs_per_s": round(n_jobs / clock, 2),
+                    "plain_server_p50_ms": round(plain["p50_ms"], 2),
+                    "plain_server_p99_ms": round(plain["p99_ms"], 2)})
+    return out
+
+
+def served_measurement(context, scenes, settings, n_jobs, seed, load, sweep_loads=(0.5, 1.0, 1.5)):
+    """The served workload on one job list: its plain back-to-back capacity
+    and the engine's capacity with every job queued at once, then Poisson
+    arrivals at ``load`` times the plain capacity (headline, with a plain
+    server on the same arrivals) and a sweep of rates around the engine's
+    capacity."""
+    from ordinarylight_amd import batch, workloads
+    jobs = make_jobs(workloads.mixed_jobs(seed, n_jobs), scenes)
+    for j in sorted(jobs, key=lambda j: -j.width * j.height * j.samples)[:2]:
         batch.plain_render(j, copy_image=False)
-        clock = time.perf_counter() - t0
-        plain_lat.append((clock - at) * 1e3)
-    plain = batch.latency_percentiles(plain_lat)
+    t0 = time.perf_counter()
+    for j in jobs:
+        batch.plain_render(j, copy_image=False)
+    plain_cap = n_jobs / (time.perf_counter() - t0)
+    with batch.BatchEngine(context, settings) as eng:
+        batch.run_batch(eng, jobs[:8])
+        queued_s, _ = batch.run_batch(eng, jobs)
+    queued_cap = n_jobs / queued_s
+    out = served_run(context, jobs, settings, load * plain_cap, seed, plain_server=True)
     out.update({
-        "plain_jobs_per_s": round(n_jobs / clock, 2),
-        "plain_p50_ms": round(plain["p50_ms"], 2), "plain_p99_ms": round(plain["p99_ms"], 2),
+        "load_vs_plain_capacity": load,
+        "plain_jobs_per_s": round(plain_cap, 2),
+        "all_queued_jobs_per_s": round(queued_cap, 2),
+        "speedup_vs_plain": round(out["jobs_per_s"] / plain_cap, 3),
+        "all_queued_speedup_vs_plain": round(queued_cap / plain_cap, 3),
     })
-    out["speedup_vs_plain"] = round(out["jobs_per_s"] / out["plain_jobs_per_s"], 3)
+    sweep = []
+    for s_load in sweep_loads:
+        r = served_run(context, jobs, settings, s_load * queued_cap, seed + 1, check=2)
+        r["load_vs_all_queued_capacity"] = s_load
+        sweep.append(r)
+        log(f"served sweep: {r}")
+    out["sweep"] = sweep
     return out
 
 
@@ -462,7 +498,7 @@ def main(argv=None):
     parser.add_argument("--lds-stack", type=int, default=16)
     parser.add_argument("--block", type=int, default=128)
     parser.add_argument("--batch-jobs", type=int, default=64, help="batched workload size (0: skip)")
-    parser.add_argument("--serve-jobs", type=int, default=96, help="served workload size (0: skip)")
+    parser.add_argument("--serve-jobs", type=int, default=256, help="served workload size (0: skip)")
     parser.add_argument("--serve-load", type=float, default=2.0,
                         help="Poisson arrival rate as a multiple of the plain path's back-to-back jobs/s")
     parser.add_argument("--no-autotune", action="store_true")
@@ -621,23 +657,12 @@ def main(argv=None):
             log(f"batched: {line['batched']}")
         if args.serve_jobs > 0:
             from ordinarylight_amd.batch import EngineSettings
-            # Headline: arrivals at --serve-load times the plain path's
-            # back-to-back capacity, against a plain one-at-a-time server on
-            # the same arrivals; then a load sweep around the batched capacity.
-            plain_cap = line["batched"]["plain_jobs_per_s"] if "batched" in line else 20.0
-            batched_cap = line["batched"]["jobs_per_s"] if "batched" in line else 40.0
-            line["served"] = served_measurement(context, scenes, settings or EngineSettings(), args.serve_jobs,
-                                                round(args.serve_load * plain_cap, 2), 1)
-            line["served"]["load_vs_plain_capacity"] = args.serve_load
+            # One job list: arrivals at --serve-load times its plain
+            # back-to-back capacity (headline), then a load sweep around the
+            # engine's capacity on the same list.
+            line["served"] = served_measurement(context, scenes, settings or EngineSettings(), args.serve_jobs, 1,
+                                                args.serve_load)
             log(f"served: {line['served']}")
-            sweep = []
-            for load in (0.5, 1.0, 1.5):
-                r = served_measurement(context, scenes, settings or EngineSettings(), 2 * args.serve_jobs,
-                                       round(load * batched_cap, 2), 2, check=2, compare_plain=False)
-                r["load_vs_batched_capacity"] = load
-                sweep.append(r)
-                log(f"served sweep: {r}")
-            line["served"]["sweep"] = sweep
         for ds, _ in scenes.values():
             ds.close()
         context.close()

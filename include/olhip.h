@@ -310,6 +310,7 @@ typedef struct olhip_engine_stats {
   double busy_ms;  /* host time spent inside steps */
   uint32_t resident_jobs, queued_jobs, max_resident_jobs, pad;
   uint64_t path_slots, entry_slots;
+  double device_ms;  /* stream time of the retired steps, first launch to report copy */
 } olhip_engine_stats;
 
 int olhip_engine_create(olhip_ctx* ctx, const olhip_engine_config* config, olhip_engine** out);

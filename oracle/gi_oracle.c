@@ -150,10 +150,42 @@ static uint32_t hash32(uint32_t v) {
 }
 static R unit(uint32_t w) { return (R)w * (R)(1.0 / 4294967296.0); }
 static uint32_t pcg(uint32_t s) { return s * 747796405u + 2891336453u; }
+
+/* Rounding-size perturbation (for the tests' mismatch classification, not
+ * part of the estimator): with a nonzero seed every uniform sample moves
+ * `ulps` units in the last place, up or down by a hash of its state and the
+ * seed (staying in [0, 1)).  Pixels whose value changes under such moves sit
+ * on a near-tie decision (a selection threshold, a triangle edge, a
+ * closest-hit tie), where a rounding difference between two float32
+ * evaluations of the same path can send it elsewhere. */
+static uint32_t g_jitter = 0u, g_jitter_ulps = 1u, g_jitter_t_ulps = 0u;
+void API(olgi_set_jitter)(uint32_t seed, uint32_t ulps, uint32_t t_ulps) {
+  g_jitter = seed;
+  g_jitter_ulps = ulps;
+  g_jitter_t_ulps = t_ulps;
+}
+
+/* The same for path-tracing hit distances: `t_ulps` units in the last place
+ * of t (the size of the rounding difference between intersecting in world
+ * space and in an instance's object space). */
+static R jitter_t(R t, uint32_t salt) {
+  if (g_jitter == 0u || g_jitter_t_ulps == 0u) return t;
+  const R up = sizeof(R) == sizeof(float) ? (R)nextafterf((float)t, 3.0e38f) : (R)nextafter((double)t, 1.0e300);
+  const R step = (up - t) * (R)g_jitter_t_ulps;
+  return (hash32(salt ^ g_jitter) & 1u) ? t + step : t - step;
+}
+
 static R pcg_unit(uint32_t s) {
   uint32_t w = ((s >> ((s >> 28) + 4u)) ^ s) * 277803737u;
   w = (w >> 22) ^ w;
-  return unit(w);
+  R u = unit(w);
+  if (g_jitter != 0u && g_jitter_ulps != 0u) {
+    const R up = sizeof(R) == sizeof(float) ? (R)nextafterf((float)u, 1.0f) : (R)nextafter((double)u, 1.0);
+    const R step = (up - u) * (R)g_jitter_ulps;
+    const R moved = (hash32(s ^ g_jitter) & 1u) ? u + step : u - step;
+    u = moved < (R)0.0 ? (R)0.0 : (moved >= (R)1.0 ? u : moved);
+  }
+  return u;
 }
 static uint32_t brev(uint32_t v) {
   uint32_t r = 0;
@@ -1323,6 +1355,7 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
     R t = 0, bu = 0, bv = 0;
     uint32_t prim;
     int hit = closest(s, o, d, tmin, tmax, &t, &bu, &bv, &prim);
+    if (hit) t = jitter_t(t, hash32(pixel ^ hash32(frame_sample)) ^ iter);
     V inc = d;
     if (rmax(T.x, rmax(T.y, T.z)) < (R)0.0001) break;
     if (!hit) {

@@ -81,6 +81,7 @@ def lib(mutant=0):
             getattr(L, "olgi_wave_hash" + sfx).argtypes = [ctypes.c_uint32]
             getattr(L, "olgi_pcg_step" + sfx).restype = ctypes.c_uint32
             getattr(L, "olgi_pcg_step" + sfx).argtypes = [ctypes.c_uint32]
+            getattr(L, "olgi_set_jitter" + sfx).argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
             getattr(L, "olgi_pcg_value" + sfx).restype = ctypes.c_float
             getattr(L, "olgi_pcg_value" + sfx).argtypes = [ctypes.c_uint32]
             getattr(L, "olgi_sincos" + sfx).argtypes = [ctypes.c_float, P_f32, P_f32]
@@ -135,6 +136,28 @@ class OracleScene:
                 self.handle, _p(tex[0], ctypes.c_uint32), len(tex[0]), _p(tex[1]), len(tex[1]) if bind is not None else 0,
                 _p(tex[2]) if uvs is not None else None, _p(tex[3]) if uvs is not None else None,
                 _p(tex[4]) if uvs is not None else None)
+
+    def set_jitter(self, seed, ulps=1, t_ulps=0):
+        """Perturbation in later renders (seed 0 = off): every uniform sample
+        moves ``ulps`` units in the last place and every path hit distance
+        ``t_ulps``, up or down by a hash of the sample state and ``seed``."""
+        getattr(lib(self.mutant), "olgi_set_jitter" + self.sfx)(int(seed), int(ulps), int(t_ulps))
+
+    def fragile_pixels(self, render, seeds=tuple(range(1, 33)), ulps=1, t_ulps=0, rel=1e-3, abs_=1e-6):
+        """(H, W) mask of the pixels of ``render()`` (a call of this scene's
+        render or render_restir) that move beyond ``rel`` under the
+        perturbations of :meth:`set_jitter`, over ``seeds``: the pixels whose
+        path crosses a decision that close to a tie."""
+        base = render()[..., :3]
+        mask = np.zeros(base.shape[:2], bool)
+        try:
+            for seed in seeds:
+                self.set_jitter(seed, ulps, t_ulps)
+                alt = render()[..., :3]
+                mask |= ~np.all(np.abs(alt - base) <= abs_ + rel * np.maximum(np.abs(alt), np.abs(base)), axis=-1)
+        finally:
+            self.set_jitter(0)
+        return mask
 
     def texture_sample(self, binding, uv0, uv1=(0.0, 0.0), srgb=False, footprint0=0.0, footprint1=0.0):
         """RGBA of material-texture binding ``binding`` at the given uvs."""

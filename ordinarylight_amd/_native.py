@@ -133,7 +133,7 @@ class EngineStats(ctypes.Structure):
         ("steps", ctypes.c_uint64), ("jobs_done", ctypes.c_uint64), ("rays", ctypes.c_uint64),
         ("queue_overflow", ctypes.c_uint64), ("busy_ms", c_f64), ("resident_jobs", c_u32),
         ("queued_jobs", c_u32), ("max_resident_jobs", c_u32), ("pad", c_u32),
-        ("path_slots", ctypes.c_uint64), ("entry_slots", ctypes.c_uint64),
+        ("path_slots", ctypes.c_uint64), ("entry_slots", ctypes.c_uint64), ("device_ms", c_f64),
     ]
 
     def as_dict(self):

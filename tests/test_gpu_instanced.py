@@ -4,9 +4,14 @@ The oracle flattens the instances into world-space triangles.  The device
 intersects in object space (the ray is transformed, not the triangles), so
 distances agree to float32 rounding rather than bit for bit: closest-hit
 triangle ids must agree on >= 99.9 % of rays and distances to 1e-4
-relative + 2e-5 absolute on >= 99.99 % of them (1e-3 + 2e-4 at grazing
-incidence); images must agree with the flattened render within the GI
-tolerance on >= 99 % of pixels.
+relative + 2e-5 absolute on >= 99.99 % of them.  Every distance obeys the
+rounding bound of a plane intersection, err <= K eps (|o| + t + 12) / |cos|
+(eps = 2^-24; 12 bounds the instance translations, |cos| the angle between
+the ray and the triangle), so the rest are grazing hits.  Images agree with
+the flattened render within the GI tolerance on >= 99.75 % of pixels, and
+every pixel outside it lies in the oracle's fragile set: the pixels that
+move beyond 1e-3 when the oracle's hit distances move by 16 ulps, the size
+of the object-space rounding (32 perturbation seeds).
 """
 
 import sys
@@ -25,6 +30,12 @@ pytestmark = pytest.mark.gpu
 def log(msg):
     print(msg, flush=True)
     sys.stdout.flush()
+
+
+# Rounding constant of the plane-intersection distance bound: a few roundings
+# of the transformed origin and direction (largest measured ratio 2.5 on 100k
+# rays).
+DISTANCE_K = 8.0
 
 
 def small_instanced(seed=0):
@@ -63,8 +74,17 @@ def test_two_level_closest_hit_matches_flattened_oracle(ctx, builder):
     err = np.abs(g[0][both] - c[0][both])
     t = np.abs(c[0][both])
     tight = err <= 2e-5 + 1e-4 * t
-    log(f"  distances within 2e-5 + 1e-4 t: {tight.mean():.6f}, max error {float(err.max()):.3g}")
-    assert tight.mean() >= 0.9999 and np.all(err <= 2e-4 + 1e-3 * t), float(err.max())
+    packed = packing.pack_scene(scene)
+    tri = packed.indices[c[3][both]]
+    a, b, e = (packed.positions[tri[:, k]].astype(np.float64) for k in range(3))
+    n = np.cross(b - a, e - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    cos = np.abs(np.sum(n * d[both], axis=1))
+    scale = np.abs(o[both]).max(axis=1) + t + 12.0
+    ratio = err * np.maximum(cos, 1e-6) / (2.0 ** -24 * scale)
+    log(f"  distances within 2e-5 + 1e-4 t: {tight.mean():.6f}, max error {float(err.max()):.3g}; "
+        f"err |cos| / (eps scale) max {float(ratio.max()):.1f}, p99.99 {float(np.quantile(ratio, 0.9999)):.1f}")
+    assert tight.mean() >= 0.9999 and float(ratio.max()) <= DISTANCE_K, float(ratio.max())
     orc.close()
     ds.close()
 
@@ -90,12 +110,17 @@ def test_two_level_gi_matches_flat_and_oracle(ctx, builder):
     assert st2["primary_rays"] == st1["primary_rays"]
     ok = rel_close(two[..., :3], flat[..., :3])
     log(f"two-level vs flat within tol {ok.mean():.5f}")
-    assert np.isfinite(two).all() and ok.mean() >= 0.99
+    assert np.isfinite(two).all() and ok.mean() >= 0.9975
     orc = oracle.OracleScene(packing.pack_scene(scene))
-    ref = orc.render(cameras.camera_basis(cam), cameras.camera_projection(cam), 96, 64, 3, 0, max_bounces=8)
+
+    def oracle_render():
+        return orc.render(cameras.camera_basis(cam), cameras.camera_projection(cam), 96, 64, 3, 0, max_bounces=8)
+
+    ref = oracle_render()
+    near_tie = orc.fragile_pixels(oracle_render, ulps=0, t_ulps=16)
     orc.close()
     ok2 = rel_close(two[..., :3], ref[..., :3])
-    log(f"two-level vs oracle within tol {ok2.mean():.5f}")
-    assert ok2.mean() >= 0.99
-    # Converged means agree closely even where single paths diverge.
-    assert abs(two[..., :3].mean() / ref[..., :3].mean() - 1.0) < 0.01
+    log(f"two-level vs oracle within tol {ok2.mean():.5f}; outside: {int((~ok2).sum())} px, of them outside the "
+        f"fragile set ({int(near_tie.sum())} px): {int((~ok2 & ~near_tie).sum())}")
+    assert ok2.mean() >= 0.9975 and not np.any(~ok2 & ~near_tie)
+    assert abs(two[..., :3].mean() / ref[..., :3].mean() - 1.0) < 0.002

@@ -2,11 +2,23 @@
 
 Both sides run the same sample streams: a job's samples are successive
 one-sample passes, pass s reusing pass s - 1's reservoirs at the reprojected
-pixel (temporal) and around it (spatial).  Reservoir choices are discrete, so
-a float32 rounding difference can flip a selection in a rare pixel: images
-must agree within the GI tolerance (1e-3 relative) on >= 99 % of pixels and
-in mean within 1 %.  With one candidate and no history ReSTIR is one-sample
-area NEE, so it must converge to the plain estimator's image.
+pixel (temporal) and around it (spatial).  Images must agree within the GI
+tolerance (1e-3 relative) on >= 99.9 % of pixels, and the pixels outside it
+fall into two measured classes:
+
+* value-level: the stored reservoir keeps w_sum and the target in half
+  precision (10 mantissa bits); a float32 rounding difference upstream can
+  flip one half rounding, which moves w_sum / target by up to 2^-10 each,
+  so <= 2e-3 relative.  Every pixel outside the oracle's fragile set must
+  agree within 2e-3.
+* divergent: a path crosses a decision within rounding distance of its
+  threshold (a reservoir pick, a lobe choice, an edge).  Every pixel beyond
+  2e-3 must lie in the oracle's fragile set: the pixels that move beyond
+  1e-3 when the oracle's uniform samples move by 128 ulps or its hit
+  distances by 16 ulps (32 perturbation seeds each).
+
+With one candidate and no history ReSTIR is one-sample area NEE, so it must
+converge to the plain estimator's image.
 """
 
 import sys
@@ -49,6 +61,10 @@ def rel_close(a, b, rel=1e-3, abs_=1e-6):
     return np.all(np.abs(a - b) <= abs_ + rel * np.maximum(np.abs(a), np.abs(b)), axis=-1)
 
 
+def fragile(orc, render):
+    return orc.fragile_pixels(render, ulps=128) | orc.fragile_pixels(render, ulps=0, t_ulps=16)
+
+
 CASES = [
     # candidates, spatial neighbours, radius, pairwise, history limit, environment samples, spp
     (1, 0, 4, True, 20, 1, 3),
@@ -72,14 +88,21 @@ def test_restir_matches_oracle(ctx, spheres, case):
                             wavefront_restir_pairwise_mis=pairwise, wavefront_restir_history_limit=limit)
     img, st = plain_render(Job(ds, cam, w, h, spp, frame, cfg))
     assert st["chunks"] == spp and st["primary_rays"] == w * h * spp
-    ref = orc.render_restir(cameras.camera_basis(cam), cameras.camera_projection(cam), w, h, spp, frame,
-                            candidates=cand, spatial_neighbors=spatial, spatial_radius=radius, pairwise=pairwise,
-                            history_limit=limit, max_bounces=bounces, environment_samples=env)
+    def oracle_render():
+        return orc.render_restir(cameras.camera_basis(cam), cameras.camera_projection(cam), w, h, spp, frame,
+                                 candidates=cand, spatial_neighbors=spatial, spatial_radius=radius,
+                                 pairwise=pairwise, history_limit=limit, max_bounces=bounces, environment_samples=env)
+
+    ref = oracle_render()
     ok = rel_close(img[..., :3], ref[..., :3])
     ratio = img[..., :3].mean() / ref[..., :3].mean()
-    log(f"  within tol {ok.mean():.5f}, mean ratio {ratio:.5f}")
-    assert np.isfinite(img).all() and ok.mean() >= 0.99
-    assert abs(ratio - 1.0) < 0.01
+    near_tie = fragile(orc, oracle_render)
+    wide = ~rel_close(img[..., :3], ref[..., :3], rel=2e-3)
+    log(f"  within tol {ok.mean():.5f}, mean ratio {ratio:.5f}; beyond 1e-3: {int((~ok).sum())} px, beyond 2e-3: "
+        f"{int(wide.sum())} px, of them outside the fragile set ({int(near_tie.sum())} px): {int((wide & ~near_tie).sum())}")
+    assert np.isfinite(img).all() and ok.mean() >= 0.999
+    assert not np.any(wide & ~near_tie)
+    assert abs(ratio - 1.0) < 0.001
 
 
 def test_restir_single_candidate_converges_to_plain_nee(ctx, spheres):

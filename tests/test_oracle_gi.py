@@ -274,3 +274,30 @@ def test_oracle_closest_hit_equals_brute_force():
         assert tri[k] == best, (k, tri[k], best)
         if best != 0xFFFFFFFF:
             assert t[k] == best_t
+
+
+def test_rounding_perturbation_probe():
+    """The perturbation the GPU tests use to classify mismatched pixels: off
+    is the estimator itself, bit for bit; on, it moves few pixels (those at
+    near-tie decisions), more as the perturbation grows, and leaves the image
+    mean unchanged to well under the tolerance of a converged comparison."""
+    scene, cam = workloads.box_room(2)
+    orc = O.OracleScene(packing.pack_scene(scene))
+
+    def render():
+        return orc.render(cameras.camera_basis(cam), 0, 48, 32, 4, 0, max_bounces=5)
+
+    base = render()
+    orc.set_jitter(7, 1)
+    moved = render()
+    orc.set_jitter(0)
+    assert np.array_equal(render(), base)
+    assert not np.array_equal(moved, base)
+    assert abs(moved[..., :3].mean() / base[..., :3].mean() - 1.0) < 1e-3
+    small = orc.fragile_pixels(render, seeds=range(1, 9), ulps=1)
+    large = orc.fragile_pixels(render, seeds=range(1, 9), ulps=4096)
+    hit_t = orc.fragile_pixels(render, seeds=range(1, 9), ulps=0, t_ulps=4096)
+    assert np.array_equal(render(), base)
+    assert 0 < small.sum() <= large.sum() and small.mean() < 0.05
+    assert hit_t.sum() > 0
+    orc.close()
