@@ -31,6 +31,7 @@ Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GP
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -234,6 +235,13 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
         batched_s, results = batch.run_batch(engine, jobs)
         est = engine.stats()
     batched_rays = sum(st["primary_rays"] + st["extension_rays"] + st["shadow_rays"] for _, st in results)
+    # The same batch under the hybrid strategy (its first step, all camera
+    # rays, runs as one traversal-and-shading kernel).
+    hybrid = dataclasses.replace(settings, fused_primary=True)
+    with batch.BatchEngine(context, hybrid) as engine:
+        batch.run_batch(engine, jobs)
+        hybrid_s, _ = batch.run_batch(engine, jobs)
+        fused_steps = engine.stats()["fused_steps"]
     return settings, {
         "jobs": n_jobs,
         "jobs_per_s": round(n_jobs / batched_s, 2),
@@ -246,6 +254,7 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
         "max_resident_jobs": int(est["max_resident_jobs"]),
         "engine_steps": int(est["steps"] - steps0),
         "device_busy_frac": round((est["device_ms"] - dev0) / (batched_s * 1e3), 3),
+        "hybrid_jobs_per_s": round(n_jobs / hybrid_s, 2), "hybrid_fused_steps": int(fused_steps),
         "settings": {"path_slots": settings.path_slots, "lds_stack": settings.lds_stack,
                      "trace_block": settings.trace_block},
         "autotune_ms": tuned,
@@ -480,6 +489,32 @@ def builder_comparison(scene, camera, W, H, spp, B, leaf, builder, frames=3):
             "nodes_per_closest_ray": round(work.closest_nodes / max(closest, 1), 2)}
 
 
+def strategy_comparison(scene, camera, W, H, spp, B, leaf, builder, strategy, reference_image, frames=3):
+    """The same frame under the other execution strategy (``wavefront``:
+    every bounce traced and shaded by separate kernels; ``hybrid``: the
+    camera-ray bounce traced and shaded by one kernel): Mrays/s and device
+    ms per frame over ``frames`` frames, trace/shade split, and whether its
+    image equals the headline strategy's bit for bit."""
+    from ordinarylight_amd.renderers.hip import HipPathTracer
+    r = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
+                      bvh_leaf_size=leaf, bvh_builder=builder, wavefront_execution_strategy=strategy, profiling=True)
+    r.replace_scene(scene)
+    image = r.render_frame(scene, camera, W, H)
+    rays, dev, trace, shade = 0, 0.0, 0.0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        r._render(scene, camera, W, H, spp, 0, keep_on_device=True)
+        st = r.last_stats
+        rays += st.primary_rays + st.extension_rays + st.shadow_rays
+        dev, trace, shade = dev + st.total_ms, trace + st.trace_ms, shade + st.shade_ms
+    wall = time.perf_counter() - t0
+    r.close()
+    return {"strategy": strategy, "mrays_per_s": round(rays / wall / 1e6, 1),
+            "device_ms_per_frame": round(dev / frames, 3), "trace_ms_per_frame": round(trace / frames, 3),
+            "shade_ms_per_frame": round(shade / frames, 3),
+            "image_equal": bool(reference_image is not None and np.array_equal(image, reference_image))}
+
+
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -506,6 +541,10 @@ def main(argv=None):
     parser.add_argument("--no-c1", action="store_true", help="skip the reference-estimator box room")
     parser.add_argument("--no-builder-comparison", action="store_true",
                         help="skip the plain frame with the other BVH builder")
+    parser.add_argument("--strategy", choices=("wavefront", "hybrid"), default="wavefront",
+                        help="execution strategy of the plain frame (the other one is measured beside it)")
+    parser.add_argument("--no-strategy-comparison", action="store_true",
+                        help="skip the plain frame under the other execution strategy")
     parser.add_argument("--c3-only", action="store_true",
                         help="run only the instanced workload and print its record (profiling aid)")
     args = parser.parse_args(argv)
@@ -530,7 +569,7 @@ def main(argv=None):
     W, H, spp, B = args.width, args.height, args.spp, args.bounces
     renderer = HipPathTracer(max_bounces=B, samples_per_pixel=spp, max_samples_per_call=max(64, spp),
                              bvh_leaf_size=args.leaf_size, bvh_builder=args.builder, traversal_lds_stack=args.lds_stack,
-                             traversal_block=args.block)
+                             traversal_block=args.block, wavefront_execution_strategy=args.strategy)
     # Scene upload + BVH build (reported separately, outside the timed region).
     renderer.replace_scene(scene)
     ds = renderer._device_scene
@@ -570,7 +609,7 @@ def main(argv=None):
 
     # PCIe-inclusive: one frame with the image copied back to the host.
     t1 = time.perf_counter()
-    renderer.render_frame(scene, camera, W, H)
+    headline_image = renderer.render_frame(scene, camera, W, H)
     pcie_ms = (time.perf_counter() - t1) * 1000.0
 
     # The traversal launch of a step traces its closest hits and the previous
@@ -604,7 +643,7 @@ def main(argv=None):
         "data": "synthetic (seeded fBm terrain)",
         "config": {"workload": "terrain 1M triangles, per-triangle GGX materials, 1920x1080, 8 spp, 8 bounces, "
                                "64 emissive tris + env 0.5",
-                   "bvh_builder": args.builder,
+                   "bvh_builder": args.builder, "execution_strategy": args.strategy,
                    "width": W, "height": H, "spp": spp, "max_bounces": B, "triangles": int(ds.info.triangle_count)},
         "roofline": roofline,
         "extra": {
@@ -627,6 +666,11 @@ def main(argv=None):
         },
     }
     renderer.close()
+    if not args.no_strategy_comparison:
+        other = "hybrid" if args.strategy == "wavefront" else "wavefront"
+        line["extra"]["other_strategy"] = strategy_comparison(scene, camera, W, H, spp, B, args.leaf_size,
+                                                              args.builder, other, headline_image)
+        log(f"other strategy: {line['extra']['other_strategy']}")
     if not args.no_builder_comparison:
         other = "lbvh" if args.builder == "sah" else "sah"
         line["extra"]["other_builder"] = builder_comparison(scene, camera, W, H, spp, B, args.leaf_size, other)

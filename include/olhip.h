@@ -272,7 +272,12 @@ typedef struct olhip_engine_config {
    * overflow).  A smaller value makes shading drop paths; the jobs resident
    * in that step then fail with "ray queue overflow". */
   uint32_t append_limit;
-  uint32_t pad;
+  /* Execution strategy of a step whose every ray is a fresh camera ray of a
+   * flat, untextured scene without ReSTIR: 0 traces and shades in separate
+   * kernels (wavefront), 1 runs both in one kernel (hybrid: the
+   * coherent primary bounce inline, later bounces through the queues).  Both
+   * give the same images bit for bit. */
+  uint32_t fused_primary;
 } olhip_engine_config;
 
 /* Named products beside the color image (render_wavefront_outputs,
@@ -308,7 +313,8 @@ typedef struct olhip_job_stats {
 typedef struct olhip_engine_stats {
   uint64_t steps, jobs_done, rays, queue_overflow;
   double busy_ms;  /* host time spent inside steps */
-  uint32_t resident_jobs, queued_jobs, max_resident_jobs, pad;
+  uint32_t resident_jobs, queued_jobs, max_resident_jobs;
+  uint32_t fused_steps;  /* steps run as one traversal-and-shading kernel (hybrid strategy) */
   uint64_t path_slots, entry_slots;
   double device_ms;  /* stream time of the retired steps, first launch to report copy */
 } olhip_engine_stats;
@@ -361,6 +367,12 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled);
  * size.  Supported: (8|12|24|32|48, 128) and (16, 64|128|256).  Results do
  * not depend on these settings. */
 int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block);
+/* Execution strategy of the single-frame entry points' primary step
+ * (olhip_engine_config.fused_primary; 0 = wavefront, 1 = hybrid).  Replaces
+ * the reference's wavefront_execution_strategy choice
+ * (targets/vulkan/api.py:158, :611-616; core.py:886) for the two strategies
+ * built here. */
+int olhip_set_fused_primary(olhip_ctx* ctx, int enabled);
 
 /* Output formats of olhip_frame_output. */
 #define OLHIP_OUTPUT_RGBA8 0u

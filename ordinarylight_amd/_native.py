@@ -107,7 +107,7 @@ class EngineConfig(ctypes.Structure):
     _fields_ = [
         ("path_slots", ctypes.c_uint64), ("entry_factor", c_u32), ("lds_stack", ctypes.c_int32),
         ("trace_block", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("append_limit", c_u32),
-        ("pad", c_u32),
+        ("fused_primary", c_u32),
     ]
 
 
@@ -132,7 +132,7 @@ class EngineStats(ctypes.Structure):
     _fields_ = [
         ("steps", ctypes.c_uint64), ("jobs_done", ctypes.c_uint64), ("rays", ctypes.c_uint64),
         ("queue_overflow", ctypes.c_uint64), ("busy_ms", c_f64), ("resident_jobs", c_u32),
-        ("queued_jobs", c_u32), ("max_resident_jobs", c_u32), ("pad", c_u32),
+        ("queued_jobs", c_u32), ("max_resident_jobs", c_u32), ("fused_steps", c_u32),
         ("path_slots", ctypes.c_uint64), ("entry_slots", ctypes.c_uint64), ("device_ms", c_f64),
     ]
 
@@ -176,6 +176,7 @@ _SIGNATURES = {
     "olhip_bandwidth_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32, ctypes.POINTER(c_f64)]),
     "olhip_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "olhip_set_traversal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "olhip_set_fused_primary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "olhip_engine_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EngineConfig),
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "olhip_engine_destroy": (ctypes.c_int, [ctypes.c_void_p]),

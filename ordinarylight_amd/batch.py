@@ -49,6 +49,9 @@ class EngineSettings:
     # Failure-path testing only: continuations a queue segment accepts per
     # step (0 = its full size, which cannot overflow).
     append_limit: int = 0
+    # A step of fresh camera rays only (flat, untextured scenes without
+    # ReSTIR) traces and shades them in one kernel (the hybrid strategy).
+    fused_primary: bool = False
 
     def native(self):
         c = N.EngineConfig()
@@ -58,6 +61,7 @@ class EngineSettings:
         c.trace_block = int(self.trace_block)
         c.use_graph = int(bool(self.use_graph))
         c.append_limit = int(self.append_limit)
+        c.fused_primary = int(bool(self.fused_primary))
         return c
 
 

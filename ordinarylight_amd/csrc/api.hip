@@ -298,6 +298,15 @@ int olhip_set_work_counters(olhip_ctx* ctx, int enabled) {
   });
 }
 
+int olhip_set_fused_primary(olhip_ctx* ctx, int enabled) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx must not be NULL");
+    const int on = enabled ? 1 : 0;
+    if (ctx->fused_primary != on) ctx->frame_engine.reset();
+    ctx->fused_primary = on;
+  });
+}
+
 int olhip_set_traversal(olhip_ctx* ctx, int lds_entries, int block) {
   return guarded([&] {
     require(ctx != nullptr, "ctx must not be NULL");
@@ -872,6 +881,7 @@ int olhip_render_products(olhip_ctx* ctx, const olhip_scene* sc, const olhip_cam
       cfg.path_slots = need;
       cfg.entry_factor = per_hit;
       cfg.use_graph = 1;
+      cfg.fused_primary = (uint32_t)ctx->fused_primary;
       ctx->frame_engine = std::make_unique<Engine>(ctx, cfg);
       ctx->frame_engine_slots = need;
       ctx->frame_engine_entries = per_hit;

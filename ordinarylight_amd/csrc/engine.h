@@ -147,10 +147,10 @@ class Engine {
   // A step's shape: queue parity, STEP_* kinds of resident work, and whether
   // it accumulates or generates (bits 1.. of the key).  Absent work is not
   // launched; each shape is captured once as a hipGraph and replayed.
-  static uint32_t step_key(uint32_t in, uint32_t kinds, bool acc, bool gen) {
-    return in | (kinds << 1) | (acc ? kKeyAcc : 0u) | (gen ? kKeyGen : 0u);
+  static uint32_t step_key(uint32_t in, uint32_t kinds, bool acc, bool gen, bool fused) {
+    return in | (kinds << 1) | (acc ? kKeyAcc : 0u) | (gen ? kKeyGen : 0u) | (fused ? kKeyFused : 0u);
   }
-  static constexpr uint32_t kKeyAcc = 32u, kKeyGen = 64u;
+  static constexpr uint32_t kKeyAcc = 32u, kKeyGen = 64u, kKeyFused = 128u;
   hipGraphExec_t graph_for(uint32_t key);
   void clear_graphs();
   void launch_step(uint32_t key, bool timed, int buf);
@@ -243,6 +243,7 @@ struct olhip_ctx {
   int counting = 0;
   int lds_stack = 16;
   int trace_block = 128;
+  int fused_primary = 0;  // olhip_set_fused_primary
   // Engine of the single-frame entry point (created on first use).
   std::unique_ptr<Engine> frame_engine;
   uint64_t frame_engine_slots = 0;

@@ -466,8 +466,21 @@ void Engine::launch_step(uint32_t key, bool timed, int buf) {
   const uint32_t in = key & 1u, kinds = (key >> 1) & 15u;
   const bool gen = (key & kKeyGen) != 0u;
   if (key & kKeyAcc) wave_accumulate(wave_, ctx_->cu_count, s);
-  if (gen) wave_generate(wave_, in, ctx_->cu_count, s);
-  if (!timed) {
+  if (gen) wave_generate(wave_, in, ctx_->cu_count, s, (key & kKeyFused) == 0u);
+  if (key & kKeyFused) {
+    // Traversal and shading in one kernel, timed as traversal.
+    hipEvent_t* ev = prof_ev_[buf];
+    if (timed) {
+      for (int k = 0; k < 3; ++k)
+        if (!ev[k]) OL_HIP_CHECK(hipEventCreate(&ev[k]));
+      OL_HIP_CHECK(hipEventRecord(ev[0], s));
+    }
+    wave_primary(launch_, wave_, in, s);
+    if (timed) {
+      OL_HIP_CHECK(hipEventRecord(ev[1], s));
+      OL_HIP_CHECK(hipEventRecord(ev[2], s));
+    }
+  } else if (!timed) {
     wave_trace(launch_, wave_, in, kinds, s);
     wave_shade(launch_, wave_, in, kinds, gen, s);
   } else {
@@ -559,7 +572,7 @@ uint32_t Engine::step() {
 bool Engine::launch_next() {
   const int b = (int)(seq_ & 1u);
   StepList& Ls = *list_host_[b];
-  bool traced = false;
+  bool traced = false, all_fresh = true;
   uint32_t kinds = 0;
   InFlight f;
   f.seq = seq_;
@@ -640,6 +653,7 @@ bool Engine::launch_next() {
       if (c.steps_launched >= j.desc.settings.max_bounces) continue;
       ++c.steps_launched;
       traced = true;
+      if (!((Ls.fresh >> k) & 1ull)) all_fresh = false;
       kinds |= j.desc.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
       if (j.desc.settings.restir_candidates) kinds |= STEP_RESTIR;
       if (j.desc.scene->view.textured) kinds |= STEP_TEXTURED;
@@ -663,7 +677,14 @@ bool Engine::launch_next() {
   const uint32_t in = parity_;
   f.traced = traced;
   f.timed = traced && profiling_;
-  const uint32_t key = step_key(in, kinds, Ls.n_acc > 0, Ls.n_gen > 0);
+  // Hybrid strategy: a step of fresh camera rays only, of flat untextured
+  // scenes without ReSTIR, runs traversal and shading as one kernel.
+  const bool fused = cfg_.fused_primary != 0u && traced && all_fresh && kinds == STEP_FLAT;
+  if (fused) {
+    std::lock_guard<std::mutex> g(mu_);
+    ++totals_.fused_steps;
+  }
+  const uint32_t key = step_key(in, kinds, Ls.n_acc > 0, Ls.n_gen > 0, fused);
   if (!traced) {
     // Only accumulation left: no traversal launches.
     wave_accumulate(wave_, ctx_->cu_count, s);

@@ -202,10 +202,16 @@ constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u, STEP_T
 void build_shade_records(BvhOutput& b, const uint4* vidx, const float4* normals, uint32_t count, bool material,
                          hipStream_t st, const float4* face_in = nullptr, const float* metal_in = nullptr);
 void wave_accumulate(const Wave& w, int cu_count, hipStream_t st);
-void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st);
+// rays = false: queue counts and medium stacks only (the hybrid primary
+// kernel computes the camera rays itself).
+void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st, bool rays = true);
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
 // `gen`: the step generates (its step list holds the fresh slots).
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, bool gen, hipStream_t st);
+// Hybrid primary step: plans the input queue and traces and shades it in one
+// kernel (every ray a fresh camera ray of a flat, untextured scene without
+// ReSTIR).
+void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
 void wave_clear(const Wave& w, hipStream_t st);
 

@@ -254,11 +254,33 @@ __global__ void __launch_bounds__(256) k_camera_rays(GiParams P, uint32_t x0, ui
   d[i] = mk4(dir, 1e30f);
 }
 
+// The camera ray of a job's i-th generated queue slot: slots follow 8x8
+// pixel tiles (sample-major), so a wave's rays are coherent.
+OL_DEV void generated_ray(const JobDev& J, uint32_t i, f3& org, f3& dir, uint32_t& path) {
+  const GiParams& P = J.P;
+  const uint32_t Wd = P.width, H = P.height, np = Wd * H;
+  // i -> (sample, 8x8 tile, position in tile) in tile order.
+  const uint32_t local = i / np, rem = i - local * np;
+  const uint32_t ty = rem / (8u * Wd);            // tile row
+  const uint32_t th = min(8u, H - 8u * ty);       // rows in this tile row
+  const uint32_t in_row = rem - ty * 8u * Wd;     // index inside the tile row
+  const uint32_t tx = in_row / (8u * th);         // full-height tiles to the left are 8 wide
+  const uint32_t tw = min(8u, Wd - 8u * tx);
+  const uint32_t in_tile = in_row - tx * 8u * th;
+  const uint32_t ly = in_tile / tw, lx = in_tile - ly * tw;
+  const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
+  const uint32_t sample = P.sample_base + local;
+  path = J.base + local * np + py * Wd + px;
+  uint32_t rng;
+  primary_ray(P, px, py, sample, org, dir, rng);
+}
+
 // Camera rays and path initialisation of the listed jobs
-// (wavefront_generate.comp:86-167).  Queue slots follow 8x8 pixel tiles, so a
-// wave's rays are coherent; the job's queue segments are filled in order
-// (slot base + i for the i-th ray in tile order).
-__global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
+// (wavefront_generate.comp:86-167): the job's queue segments are filled in
+// order (slot base + i for the i-th ray in tile order).  With `rays` 0 only
+// the queue counts and medium stacks are written: the hybrid primary kernel
+// computes each slot's ray itself.
+__global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in, uint32_t rays) {
   const StepList& Ls = *W.list;
   const uint32_t n = Ls.n_gen;
   if (n == 0) return;
@@ -267,35 +289,23 @@ __global__ void __launch_bounds__(256) k_generate(Wave W, uint32_t in) {
     const uint32_t k = list_item(Ls.gen_prefix, n, g);
     const uint32_t job = Ls.gen_job[k];
     const JobDev& J = W.jobs[job];
-    const GiParams& P = J.P;
     const uint32_t i = g - Ls.gen_prefix[k];
-    const uint32_t Wd = P.width, H = P.height, np = Wd * H;
     if (i == 0) {
       for (uint32_t s = 0; s < J.stripes; ++s) {
         const uint32_t lo = s * J.seg;
         W.ctr[c_queue(in) + run_word(job * kJobStripes + s)] = J.n_paths > lo ? min(J.seg, J.n_paths - lo) : 0u;
       }
     }
-    // i -> (sample, 8x8 tile, position in tile) in tile order.
-    const uint32_t local = i / np, rem = i - local * np;
-    const uint32_t ty = rem / (8u * Wd);            // tile row
-    const uint32_t th = min(8u, H - 8u * ty);       // rows in this tile row
-    const uint32_t in_row = rem - ty * 8u * Wd;     // index inside the tile row
-    const uint32_t tx = in_row / (8u * th);         // full-height tiles to the left are 8 wide
-    const uint32_t tw = min(8u, Wd - 8u * tx);
-    const uint32_t in_tile = in_row - tx * 8u * th;
-    const uint32_t ly = in_tile / tw, lx = in_tile - ly * tw;
-    const uint32_t px = tx * 8u + lx, py = ty * 8u + ly;
-    const uint32_t sample = P.sample_base + local;
-    const uint32_t pixel = py * Wd + px;
-    const uint32_t path = J.base + local * np + pixel;
-    const uint32_t slot = J.base + i;
-    f3 org, dir;
-    uint32_t rng;
-    primary_ray(P, px, py, sample, org, dir, rng);
-    W.qo[in][slot] = mk4(org, 0.001f);
-    W.qd[in][slot] = mk4(dir, 1e30f);
-    W.qp[in][slot] = path;
+    // Without rays, the medium stacks of the same set of paths, in slot order.
+    uint32_t path = J.base + i;
+    if (rays) {
+      f3 org, dir;
+      generated_ray(J, i, org, dir, path);
+      const uint32_t slot = J.base + i;
+      W.qo[in][slot] = mk4(org, 0.001f);
+      W.qd[in][slot] = mk4(dir, 1e30f);
+      W.qp[in][slot] = path;
+    }
     // T = (1, 1, 1, bounce 0), L = 0 and M = fresh_path_state are implied
     // for a fresh slot (StepList::fresh) and first written by shade.
     if (W.stk) W.stk[4u * path] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
@@ -1228,17 +1238,13 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t ge
 // * 64 slots cannot overflow.
 // Four waves per SIMD: the register budget drops from 153 to 128 VGPRs with
 // a few spilled dwords, which measured 7 % faster than three waves.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Wave W, uint32_t in,
-                                                                                       uint32_t gen) {
+// Shading of one chunk of the input queue by one wave: lanes below k.count
+// hold entry k.first + lane, with its path slot, ray (origin / tmin,
+// direction / tmax) and hit record (t, u, v, triangle slot bits).
+OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc& k, uint32_t path_in, float4 ro,
+                        float4 rd, float4 hv) {
   const uint32_t out = in ^ 1u;
-  const uint32_t* counts = W.ctr + c_queue(in);
-  const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
-  const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
-  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
-  for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
-    const uint32_t cu = uniform(c);
-    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, counts, W.jobs, cu)
-                                        : locate_chunk(W.ctr + C_RP2, counts, W.jobs, cu - flat_chunks);
+  {
     const uint32_t job = uniform(k.job);
     const JobDev& J = W.jobs[job];
     const GiParams& P = J.P;
@@ -1256,8 +1262,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     NeeSample single;
     single.valid = false;
     if (live) {
-      path = W.qp[in][i];
-      const float4 ro = W.qo[in][i], rd = W.qd[in][i], hv = W.hit[i];
+      path = path_in;
       const bool fresh = gen && ((W.list->fresh >> job) & 1ull);
       float4 T = fresh ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : W.T[path];
       float4 L = fresh ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : W.L[path];
@@ -1504,7 +1509,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           uint32_t rs2 = nee_rng;
           uint32_t en;
           Mat mat = load_mat(sc.materials, nee_mat);
-          if (sc.face) apply_face(sc, __float_as_uint(W.hit[i].w), mat);
+          if (sc.face) apply_face(sc, __float_as_uint(hv.w), mat);
           if (sc.textured) textured_material(W, i, mat);
           restir_nee(P, single, rs2, hitp, nrm, mat, incoming, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
@@ -1520,7 +1525,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           uint32_t kk = 0, an, en;
           uint32_t rs2 = nee_rng;
           Mat mat = load_mat(sc.materials, nee_mat);
-          if (sc.face) apply_face(sc, __float_as_uint(W.hit[i].w), mat);
+          if (sc.face) apply_face(sc, __float_as_uint(hv.w), mat);
           if (sc.textured) textured_material(W, i, mat);
           for_each_nee(P, rs2, hitp, nrm, mat, incoming, an, en, [&](const NeeSample& s) {
             W.eo[eslot + kk] = mk4(s.origin, 0.001f);
@@ -1533,6 +1538,76 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         atomicAdd(&W.ctr[C_OVERFLOW], 1u);
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Wave W, uint32_t in,
+                                                                                       uint32_t gen) {
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
+  const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
+    const uint32_t cu = uniform(c);
+    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, counts, W.jobs, cu)
+                                        : locate_chunk(W.ctr + C_RP2, counts, W.jobs, cu - flat_chunks);
+    const uint32_t i = k.first + lane_id();
+    uint32_t path = 0;
+    float4 ro = make_float4(0, 0, 0, 0), rd = ro, hv = ro;
+    if (lane_id() < k.count) {
+      path = W.qp[in][i];
+      ro = W.qo[in][i];
+      rd = W.qd[in][i];
+      hv = W.hit[i];
+    }
+    shade_chunk(W, in, gen, k, path, ro, rd, hv);
+  }
+}
+
+// ---- fused primary step ------------------------------------------------------------------------
+// The camera rays of a step whose every chunk is fresh (flat, untextured
+// scenes without ReSTIR), traced and shaded by one kernel: each wave takes a
+// chunk, traverses its rays to their closest hits (one ray per lane, no
+// refill; camera rays of an 8x8 tile are coherent) and shades them in
+// registers, so the hit records never go through memory and the step runs
+// one kernel instead of a traversal and a shading launch
+// (processPrimaryPixel, wavefront_primary_impl.glsl:1097).  Hits and shading
+// are those of the split kernels, bit for bit.
+template <int LSTACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_primary(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
+  __shared__ uint32_t lds[LSTACK * BLOCK];
+  TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
+  const uint32_t* counts = W.ctr + c_queue(in);
+  const uint32_t chunks = W.ctr[C_RP + kRuns];
+  const uint32_t waves = gridDim.x * (BLOCK / 64u);
+  uint32_t nv = 0, nt = 0;
+  for (uint32_t c = blockIdx.x * (BLOCK / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
+    const ChunkLoc k = locate_chunk(W.ctr + C_RP, counts, W.jobs, uniform(c));
+    const DScene& sc = W.jobs[uniform(k.job)].P.sc;
+    const uint32_t i = k.first + lane_id();
+    uint32_t path = 0;
+    float4 ro = make_float4(0, 0, 0, 0), rd = ro, hv = ro;
+    if (lane_id() < k.count) {
+      f3 org, dir;
+      generated_ray(W.jobs[uniform(k.job)], i - W.jobs[uniform(k.job)].base, org, dir, path);
+      ro = mk4(org, 0.001f);
+      rd = mk4(dir, 1e30f);
+      hv = make_float4(rd.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
+      if (sc.node_count != 0u) {
+        ClosestRay q;
+        closest_begin(q, mk3(ro), mk3(rd), ro.w, rd.w);
+        st.sp = 0;
+        while (!closest_step<LSTACK>(sc.nodes, sc.tri, q, st, nv, nt)) {
+        }
+        hv = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
+      }
+    }
+    shade_chunk(W, in, 1u, k, path, ro, rd, hv);
+  }
+  if (W.work && (nv | nt)) {
+    atomicAdd(&W.work[0], (unsigned long long)nv);
+    atomicAdd(&W.work[1], (unsigned long long)nt);
   }
 }
 
@@ -1617,6 +1692,7 @@ uint64_t spill_words_for(const GiLaunch& l) {
   int occ = std::max(std::max(resident_blocks(k_trace<LSTACK, BLOCK>, BLOCK), resident_blocks(k_trace2<LSTACK, BLOCK>, BLOCK)),
                      std::max(resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK),
                               resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK)));
+  occ = std::max(occ, resident_blocks(k_primary<LSTACK, BLOCK>, BLOCK));
   return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
 
@@ -1636,6 +1712,14 @@ void launch_trace2(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st
   check_stripes(grid);
   check_spill(l, (uint64_t)grid.x * BLOCK);
   hipLaunchKernelGGL((k_trace2<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
+}
+
+template <int LSTACK, int BLOCK>
+void launch_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  static int occ = resident_blocks(k_primary<LSTACK, BLOCK>, BLOCK);
+  dim3 grid((unsigned)(l.cu_count * occ));
+  check_spill(l, (uint64_t)grid.x * BLOCK);
+  hipLaunchKernelGGL((k_primary<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
@@ -1702,8 +1786,8 @@ void wave_accumulate(const Wave& w, int cu_count, hipStream_t st) {
   hipLaunchKernelGGL(k_accumulate, dim3((unsigned)(cu_count * 4)), dim3(256), 0, st, w);
 }
 
-void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st) {
-  hipLaunchKernelGGL(k_generate, dim3((unsigned)(cu_count * 8)), dim3(256), 0, st, w, in);
+void wave_generate(const Wave& w, uint32_t in, int cu_count, hipStream_t st, bool rays) {
+  hipLaunchKernelGGL(k_generate, dim3((unsigned)(cu_count * 8)), dim3(256), 0, st, w, in, rays ? 1u : 0u);
 }
 
 void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
@@ -1729,6 +1813,18 @@ void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, h
   hipLaunchKernelGGL(k_plan, dim3(n), dim3(1024), 0, st, w.ctr, w.jobs, ps);
   if (kinds & STEP_FLAT) [&] { OL_STACK_DISPATCH(launch_trace, l, w, in, st); }();
   if (kinds & STEP_INSTANCED) [&] { OL_STACK_DISPATCH(launch_trace2, l, w, in, st); }();
+}
+
+void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+  PlanSet ps{};
+  ps.counts[0] = c_queue(in);
+  ps.out[0] = C_RP;
+  ps.kind[0] = 0u;
+  ps.counts[1] = c_groups(in ^ 1u);
+  ps.out[1] = C_GP;
+  ps.kind[1] = 0u;
+  hipLaunchKernelGGL(k_plan, dim3(2), dim3(1024), 0, st, w.ctr, w.jobs, ps);
+  OL_STACK_DISPATCH(launch_primary, l, w, in, st);
 }
 
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, bool gen, hipStream_t st) {

@@ -75,6 +75,15 @@ class HipRendererConfig:
     traversal_block: int = 128
     profiling: bool = False
     work_counters: bool = False
+    # Execution strategy (targets/vulkan/api.py:158, :611-616): "wavefront"
+    # traces and shades every bounce in separate kernels; "hybrid" runs the
+    # coherent primary bounce (fresh camera rays of flat, untextured scenes
+    # without ReSTIR) as one traversal-and-shading kernel and later bounces
+    # through the queues.  Both give the same image bit for bit; "auto" is
+    # "wavefront", the faster of the two on the measured workloads.  The
+    # reference's "megakernel", "persistent" and "ser" strategies are not
+    # built.
+    wavefront_execution_strategy: str = "wavefront"
 
     def __post_init__(self):
         if not 1 <= int(self.max_bounces) <= 64:
@@ -108,6 +117,13 @@ class HipRendererConfig:
             raise ValueError("wavefront_restir_spatial_neighbors must be between 1 and 8")
         if not 1 <= int(self.wavefront_restir_spatial_radius) <= 32:
             raise ValueError("wavefront_restir_spatial_radius must be between 1 and 32")
+        if self.wavefront_execution_strategy not in ("wavefront", "hybrid", "auto"):
+            raise ValueError("wavefront_execution_strategy must be 'wavefront', 'hybrid' or 'auto' "
+                             "(the megakernel, persistent and ser strategies are not built)")
+
+    @property
+    def fused_primary(self):
+        return self.wavefront_execution_strategy == "hybrid"
 
     def gi_settings(self):
         s = N.GiSettings()
@@ -137,6 +153,7 @@ HOT_SETTINGS = frozenset({
     "wavefront_russian_roulette", "wavefront_russian_roulette_start",
     "wavefront_russian_roulette_min_survival", "wavefront_tile_capacity",
     "max_samples_per_call", "traversal_lds_stack", "traversal_block", "profiling", "work_counters",
+    "wavefront_execution_strategy",
     "wavefront_exposure", "wavefront_restir_di", "wavefront_restir_candidates", "wavefront_restir_history_limit",
     "wavefront_restir_spatial_reuse", "wavefront_restir_spatial_neighbors", "wavefront_restir_spatial_radius",
     "wavefront_restir_pairwise_mis",
@@ -500,6 +517,7 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         N.check(lib.olhip_set_traversal(context.handle, int(self.config.traversal_lds_stack),
                                         int(self.config.traversal_block)))
         N.check(lib.olhip_set_work_counters(context.handle, int(bool(self.config.work_counters))))
+        N.check(lib.olhip_set_fused_primary(context.handle, int(self.config.fused_primary)))
         position, forward, right, up = camera_basis(camera)
         cam = N.Camera()
         cam.position[:] = position.tolist()
