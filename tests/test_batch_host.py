@@ -111,3 +111,17 @@ def test_instanced_packing_spheres_contain_instances():
         far = np.sqrt(((w - c) ** 2).sum(1)).max()
         assert far <= r
         assert r <= far * (1.0 + 1e-4) + 1e-6
+
+
+def test_execution_strategy_settings():
+    """Strategies: wavefront (default) and hybrid reach the C ABI as the
+    fused-primary flag; the reference's other strategies are refused."""
+    from ordinarylight_amd.renderers.hip import HipRendererConfig
+    assert not HipRendererConfig().fused_primary
+    assert HipRendererConfig(wavefront_execution_strategy="hybrid").fused_primary
+    assert not HipRendererConfig(wavefront_execution_strategy="auto").fused_primary
+    for name in ("megakernel", "persistent", "ser", "fused"):
+        with pytest.raises(ValueError):
+            HipRendererConfig(wavefront_execution_strategy=name)
+    assert EngineSettings().native().fused_primary == 0
+    assert EngineSettings(fused_primary=True).native().fused_primary == 1
