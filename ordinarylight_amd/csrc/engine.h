@@ -150,7 +150,7 @@ class Engine {
   static uint32_t step_key(uint32_t in, uint32_t kinds, bool acc, bool gen, bool fused) {
     return in | (kinds << 1) | (acc ? kKeyAcc : 0u) | (gen ? kKeyGen : 0u) | (fused ? kKeyFused : 0u);
   }
-  static constexpr uint32_t kKeyAcc = 32u, kKeyGen = 64u, kKeyFused = 128u;
+  static constexpr uint32_t kKeyAcc = 64u, kKeyGen = 128u, kKeyFused = 256u;
   hipGraphExec_t graph_for(uint32_t key);
   void clear_graphs();
   void launch_step(uint32_t key, bool timed, int buf);

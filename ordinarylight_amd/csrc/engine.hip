@@ -463,7 +463,7 @@ void Engine::drop_chunks(Job& j) {
 
 void Engine::launch_step(uint32_t key, bool timed, int buf) {
   hipStream_t s = ctx_->stream;
-  const uint32_t in = key & 1u, kinds = (key >> 1) & 15u;
+  const uint32_t in = key & 1u, kinds = (key >> 1) & kStepKinds;
   const bool gen = (key & kKeyGen) != 0u;
   if (key & kKeyAcc) wave_accumulate(wave_, ctx_->cu_count, s);
   if (gen) wave_generate(wave_, in, ctx_->cu_count, s, (key & kKeyFused) == 0u);
@@ -475,7 +475,7 @@ void Engine::launch_step(uint32_t key, bool timed, int buf) {
         if (!ev[k]) OL_HIP_CHECK(hipEventCreate(&ev[k]));
       OL_HIP_CHECK(hipEventRecord(ev[0], s));
     }
-    wave_primary(launch_, wave_, in, s);
+    wave_primary(launch_, wave_, in, kinds, s);
     if (timed) {
       OL_HIP_CHECK(hipEventRecord(ev[1], s));
       OL_HIP_CHECK(hipEventRecord(ev[2], s));
@@ -657,6 +657,7 @@ bool Engine::launch_next() {
       kinds |= j.desc.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
       if (j.desc.settings.restir_candidates) kinds |= STEP_RESTIR;
       if (j.desc.scene->view.textured) kinds |= STEP_TEXTURED;
+      if (jobs_state_[k].P.two_pass) kinds |= STEP_TWO_PASS;
     }
     totals_.resident_jobs = 0;
     for (uint32_t k = 0; k < kMaxJobs; ++k) totals_.resident_jobs += slot_job_[k] != 0;
@@ -679,7 +680,7 @@ bool Engine::launch_next() {
   f.timed = traced && profiling_;
   // Hybrid strategy: a step of fresh camera rays only, of flat untextured
   // scenes without ReSTIR, runs traversal and shading as one kernel.
-  const bool fused = cfg_.fused_primary != 0u && traced && all_fresh && kinds == STEP_FLAT;
+  const bool fused = cfg_.fused_primary != 0u && traced && all_fresh && (kinds & ~STEP_TWO_PASS) == STEP_FLAT;
   if (fused) {
     std::lock_guard<std::mutex> g(mu_);
     ++totals_.fused_steps;

@@ -193,7 +193,11 @@ int stack_depth_for(uint32_t bvh_depth);
 // shadow groups this step's shading emits.
 // `kinds` says which resident work a step holds (STEP_* bits); the launches
 // for absent kinds are left out (their chunk totals stay zero).
-constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u, STEP_TEXTURED = 8u;
+// STEP_TWO_PASS: some job evaluates more than one next-event sample per hit
+// (point lights beside the environment, or per-domain NEE); shading without
+// it, ReSTIR or textures runs a variant compiled without those paths.
+constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u, STEP_TEXTURED = 8u, STEP_TWO_PASS = 16u;
+constexpr uint32_t kStepKinds = 31u;
 // Fills b.shade from the hierarchy's leaf triangles, the triangles' vertex
 // indices (material in .w when `material`) and vertex normals.
 // face_in / metal_in (original order, optional): per-triangle base colour
@@ -211,7 +215,7 @@ void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, b
 // Hybrid primary step: plans the input queue and traces and shades it in one
 // kernel (every ray a fresh camera ray of a flat, untextured scene without
 // ReSTIR).
-void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st);
+void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st);
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st);
 void wave_clear(const Wave& w, hipStream_t st);
 

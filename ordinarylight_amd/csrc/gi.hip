@@ -1240,7 +1240,10 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t ge
 // a few spilled dwords, which measured 7 % faster than three waves.
 // Shading of one chunk of the input queue by one wave: lanes below k.count
 // hold entry k.first + lane, with its path slot, ray (origin / tmin,
-// direction / tmax) and hit record (t, u, v, triangle slot bits).
+// direction / tmax) and hit record (t, u, v, triangle slot bits).  kFull =
+// false compiles out the ReSTIR primary hit, textured materials and the
+// multi-sample next-event replay, for steps whose jobs use none of them.
+template <bool kFull>
 OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc& k, uint32_t path_in, float4 ro,
                         float4 rd, float4 hv) {
   const uint32_t out = in ^ 1u;
@@ -1311,7 +1314,7 @@ OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc
         }
         Mat mat = load_mat(sc.materials, mat_id);
         if (sc.face) apply_face(sc, slot, mat);
-        if (sc.textured) {
+        if (kFull && sc.textured) {
           textured_material(W, i, mat);
           const f3 sn = rgb(W.tex[4u * i + 3u]);
           nrm = entering ? sn : -sn;
@@ -1384,7 +1387,7 @@ OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc
             nee_T = rgb(T);
             nee_p = nee_prob;
             nee_rng = rs;
-            if (P.restir && bounce == 0u) {
+            if (kFull && P.restir && bounce == 0u) {
               // Primary hit: point lights, the ReSTIR area-light sample (from
               // k_restir, with the random state it advanced to), then the
               // environment samples (wavefront_primary_impl.glsl:1527-1785).
@@ -1400,7 +1403,7 @@ OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc
               nee_rng = rs;
               restir_nee(P, single, rs, hitp, nrm, mat, incoming, env_n, [&](const NeeSample&) { ++nee_count; });
               used_restir = true;
-            } else if (!P.two_pass) {
+            } else if (!kFull || !P.two_pass) {
               // At most one sample: evaluate once and keep it in registers.
               for_each_nee(P, rs, hitp, nrm, mat, incoming, area_n, env_n, [&](const NeeSample& s) {
                 single = s;
@@ -1504,7 +1507,7 @@ OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc
         W.gT[gslot] = mk4(nee_T, nee_p);
         W.gI[gslot] = make_uint4(path, eslot, nee_count,
                                  area_n | (env_n << 8) | ((P.unified && !used_restir ? 0u : 1u) << 16));
-        if (used_restir) {
+        if (kFull && used_restir) {
           uint32_t kk = 0;
           uint32_t rs2 = nee_rng;
           uint32_t en;
@@ -1517,7 +1520,7 @@ OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc
             W.ec[eslot + kk] = mk4(s.c, (float)s.kind);
             ++kk;
           });
-        } else if (!P.two_pass) {
+        } else if (!kFull || !P.two_pass) {
           W.eo[eslot] = mk4(single.origin, 0.001f);
           W.ed[eslot] = mk4(single.dir, single.tmax);
           W.ec[eslot] = mk4(single.c, (float)single.kind);
@@ -1541,6 +1544,7 @@ OL_DEV void shade_chunk(const Wave& W, uint32_t in, uint32_t gen, const ChunkLoc
   }
 }
 
+template <bool kFull>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Wave W, uint32_t in,
                                                                                        uint32_t gen) {
   const uint32_t* counts = W.ctr + c_queue(in);
@@ -1560,7 +1564,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       rd = W.qd[in][i];
       hv = W.hit[i];
     }
-    shade_chunk(W, in, gen, k, path, ro, rd, hv);
+    shade_chunk<kFull>(W, in, gen, k, path, ro, rd, hv);
   }
 }
 
@@ -1573,7 +1577,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // one kernel instead of a traversal and a shading launch
 // (processPrimaryPixel, wavefront_primary_impl.glsl:1097).  Hits and shading
 // are those of the split kernels, bit for bit.
-template <int LSTACK, int BLOCK>
+template <int LSTACK, int BLOCK, bool kFull>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_primary(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
   __shared__ uint32_t lds[LSTACK * BLOCK];
@@ -1603,7 +1607,7 @@ k_primary(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
         hv = make_float4(q.best.t, q.best.u, q.best.v, __uint_as_float(q.best.slot));
       }
     }
-    shade_chunk(W, in, 1u, k, path, ro, rd, hv);
+    shade_chunk<kFull>(W, in, 1u, k, path, ro, rd, hv);
   }
   if (W.work && (nv | nt)) {
     atomicAdd(&W.work[0], (unsigned long long)nv);
@@ -1692,7 +1696,8 @@ uint64_t spill_words_for(const GiLaunch& l) {
   int occ = std::max(std::max(resident_blocks(k_trace<LSTACK, BLOCK>, BLOCK), resident_blocks(k_trace2<LSTACK, BLOCK>, BLOCK)),
                      std::max(resident_blocks(k_trace_rays<LSTACK, BLOCK>, BLOCK),
                               resident_blocks(k_trace_rays2<LSTACK, BLOCK>, BLOCK)));
-  occ = std::max(occ, resident_blocks(k_primary<LSTACK, BLOCK>, BLOCK));
+  occ = std::max(occ, std::max(resident_blocks(k_primary<LSTACK, BLOCK, true>, BLOCK),
+                               resident_blocks(k_primary<LSTACK, BLOCK, false>, BLOCK)));
   return (uint64_t)spill_cap_of(l) * (uint64_t)l.cu_count * (uint64_t)occ * BLOCK;
 }
 
@@ -1715,11 +1720,15 @@ void launch_trace2(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st
 }
 
 template <int LSTACK, int BLOCK>
-void launch_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
-  static int occ = resident_blocks(k_primary<LSTACK, BLOCK>, BLOCK);
-  dim3 grid((unsigned)(l.cu_count * occ));
+void launch_primary(const GiLaunch& l, const Wave& w, uint32_t in, bool full, hipStream_t st) {
+  static int occ_full = resident_blocks(k_primary<LSTACK, BLOCK, true>, BLOCK);
+  static int occ_one = resident_blocks(k_primary<LSTACK, BLOCK, false>, BLOCK);
+  dim3 grid((unsigned)(l.cu_count * (full ? occ_full : occ_one)));
   check_spill(l, (uint64_t)grid.x * BLOCK);
-  hipLaunchKernelGGL((k_primary<LSTACK, BLOCK>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
+  if (full)
+    hipLaunchKernelGGL((k_primary<LSTACK, BLOCK, true>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
+  else
+    hipLaunchKernelGGL((k_primary<LSTACK, BLOCK, false>), grid, dim3(BLOCK), 0, st, w, in, l.spill, spill_cap_of(l));
 }
 
 template <int LSTACK, int BLOCK>
@@ -1815,7 +1824,7 @@ void wave_trace(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, h
   if (kinds & STEP_INSTANCED) [&] { OL_STACK_DISPATCH(launch_trace2, l, w, in, st); }();
 }
 
-void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st) {
+void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, hipStream_t st) {
   PlanSet ps{};
   ps.counts[0] = c_queue(in);
   ps.out[0] = C_RP;
@@ -1824,7 +1833,7 @@ void wave_primary(const GiLaunch& l, const Wave& w, uint32_t in, hipStream_t st)
   ps.out[1] = C_GP;
   ps.kind[1] = 0u;
   hipLaunchKernelGGL(k_plan, dim3(2), dim3(1024), 0, st, w.ctr, w.jobs, ps);
-  OL_STACK_DISPATCH(launch_primary, l, w, in, st);
+  OL_STACK_DISPATCH(launch_primary, l, w, in, (kinds & STEP_TWO_PASS) != 0u, st);
 }
 
 void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, bool gen, hipStream_t st) {
@@ -1836,8 +1845,13 @@ void wave_shade(const GiLaunch& l, const Wave& w, uint32_t in, uint32_t kinds, b
     static int rocc = resident_blocks(k_restir, 256);
     hipLaunchKernelGGL(k_restir, dim3((unsigned)(l.cu_count * rocc)), dim3(256), 0, st, w, in, 1u);
   }
-  static int occ = resident_blocks(k_shade, 256);
-  hipLaunchKernelGGL(k_shade, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in, gen ? 1u : 0u);
+  if (kinds & (STEP_RESTIR | STEP_TEXTURED | STEP_TWO_PASS)) {
+    static int occ = resident_blocks(k_shade<true>, 256);
+    hipLaunchKernelGGL(k_shade<true>, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in, gen ? 1u : 0u);
+  } else {
+    static int occ = resident_blocks(k_shade<false>, 256);
+    hipLaunchKernelGGL(k_shade<false>, dim3((unsigned)(l.cu_count * occ)), dim3(256), 0, st, w, in, gen ? 1u : 0u);
+  }
 }
 
 void wave_epilogue(const Wave& w, uint32_t in, hipStream_t st) {
