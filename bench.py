@@ -33,6 +33,7 @@ from __future__ import annotations
 import argparse
 import dataclasses
 import json
+import math
 import os
 import platform
 import sys
@@ -187,6 +188,41 @@ def cpu_baseline(scene, camera, width, height, spp, bounces, target_seconds):
         "sample": f"rows {r0}-{r1} of {width}x{height}, {spp} spp, {bounces} bounces "
                   f"({rays} rays incl. shadow in {elapsed:.1f} s; oracle BVH build {build_s:.1f} s)",
         "host": host,
+    }
+
+
+def c3_cpu_baseline(scene, camera, target_seconds):
+    """Time the float32 C oracle's two-level path (instances intersected in
+    object space, as the device does) with the C3 settings on all host cores,
+    at a reduced 16:9 resolution sized to about ``target_seconds``: ReSTIR
+    passes need whole images, so the sample is fewer pixels rather than a
+    band of rows."""
+    from ordinarylight_amd import cameras, packing
+    from oracle import native as oracle
+    threads = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    started = time.perf_counter()
+    orc = oracle.OracleScene(packing.pack_instanced_scene(scene))
+    build_s = time.perf_counter() - started
+    basis, proj = cameras.camera_basis(camera), cameras.camera_projection(camera)
+
+    def run(k):
+        t0 = time.perf_counter()
+        orc.render_restir(basis, proj, 16 * k, 9 * k, 4, 0, candidates=32, spatial_neighbors=1, spatial_radius=4,
+                          pairwise=True, history_limit=20, max_bounces=16, environment_samples=1)
+        return time.perf_counter() - t0, int(orc.last_counts.sum())
+
+    k = 8
+    elapsed, rays = run(k)
+    k = int(min(120, max(k, k * math.sqrt(target_seconds / max(elapsed, 1e-3)))))
+    elapsed, rays = run(k)
+    orc.close()
+    return {
+        "value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+        "kind": "port",
+        "sample": f"{16 * k}x{9 * k} (C3 scene, camera and settings: 4 ReSTIR passes, 16 bounces; {rays} rays "
+                  f"incl. shadow in {elapsed:.1f} s; oracle build {build_s:.1f} s)",
+        "host": host_cpu(),
     }
 
 
@@ -355,7 +391,7 @@ def served_measurement(context, scenes, settings, n_jobs, seed, load, sweep_load
     return out
 
 
-def c3_measurement(context, steps=2, builder="lbvh"):
+def c3_measurement(context, steps=2, builder="lbvh", cpu_seconds=0.0):
     """Instanced two-level scene: 5 jittered icospheres (20,480 triangles each)
     placed 10,000 times (~205M effective triangles), 20 % glass, 100k small
     emitters sampled with ReSTIR DI (32 candidates, temporal reuse and one
@@ -420,6 +456,8 @@ def c3_measurement(context, steps=2, builder="lbvh"):
                            "history limit 20); light-CDF NEE at later bounces",
     }
     ds.close()
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = c3_cpu_baseline(scene, camera, cpu_seconds)
     return out
 
 
@@ -559,7 +597,8 @@ def main(argv=None):
     if args.c3_only:
         from ordinarylight_amd.renderers.hip import DeviceContext
         context = DeviceContext(0)
-        print(json.dumps({"c3": c3_measurement(context, steps=args.steps, builder=args.builder)}), flush=True)
+        print(json.dumps({"c3": c3_measurement(context, steps=args.steps, builder=args.builder,
+                                               cpu_seconds=0.0 if args.no_cpu else args.cpu_seconds)}), flush=True)
         context.close()
         return 0
 
@@ -684,7 +723,7 @@ def main(argv=None):
     if not args.no_c3:
         from ordinarylight_amd.renderers.hip import DeviceContext
         context = DeviceContext(0)
-        line["c3"] = c3_measurement(context, builder=args.builder)
+        line["c3"] = c3_measurement(context, builder=args.builder, cpu_seconds=0.0 if args.no_cpu else args.cpu_seconds)
         log(f"c3: {line['c3']}")
         context.close()
 

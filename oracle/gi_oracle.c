@@ -199,7 +199,7 @@ typedef struct {
   uint32_t left, right; /* leaf: right = count | 0x80000000, left = first */
 } BNode;
 
-typedef struct {
+typedef struct Scene_ {
   uint32_t V, T, M, P, L;
   float* pos;      /* V*3 */
   float* nrm;      /* V*3 */
@@ -228,9 +228,32 @@ typedef struct {
   float* dens;
   int env_tex;
   float env_rot, env_log, env_col[3], env_int;
+  /* Primitives that are boxes (6 floats: min xyz, max xyz) instead of
+   * triangles: the top level of a two-level scene. */
+  float* boxes;
+  /* Two-level scenes (olgi_scene_new_instanced): I instances of R object-
+   * space resource scenes (positions, normals, indices, their own BVH); the
+   * instance arrays hold world-to-object rows (12 floats), normal-matrix
+   * rows (9), determinant, resource, material and first global triangle; a
+   * BVH over the instances' world boxes (`top`).  Rays are intersected in
+   * each instance's object space with the device's arithmetic (the ray is
+   * transformed, not the triangles), and ids are global: the instance's
+   * first triangle plus the local id.  T counts the instanced triangles. */
+  uint32_t I, R_count;
+  struct Scene_* res;
+  struct Scene_* top;
+  float *w2o, *nmx, *det;
+  uint32_t *ires, *imat, *ioff;
 } Scene;
 
 static void tri_box(const Scene* s, uint32_t t, double mn[3], double mx[3]) {
+  if (s->boxes) {
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = s->boxes[6ull * t + k];
+      mx[k] = s->boxes[6ull * t + 3 + k];
+    }
+    return;
+  }
   for (int k = 0; k < 3; ++k) {
     mn[k] = INFINITY;
     mx[k] = -INFINITY;
@@ -300,6 +323,31 @@ static uint32_t build_rec(Scene* s, uint32_t first, uint32_t count) {
   return id;
 }
 
+/* Median-split BVH2 over the scene's T primitives (triangles or boxes). */
+static void build_bvh(Scene* s) {
+  const uint32_t T = s->T;
+  s->nodes = (BNode*)malloc(sizeof(BNode) * (2ull * T + 2));
+  s->order = (uint32_t*)malloc(4ull * T + 4);
+  g_cent = (double*)malloc(24ull * T + 8);
+  double scale = 0.0;
+  for (uint32_t t = 0; t < T; ++t) {
+    double a[3], b[3];
+    tri_box(s, t, a, b);
+    for (int k = 0; k < 3; ++k) {
+      g_cent[3ull * t + k] = 0.5 * (a[k] + b[k]);
+      if (s->boxes) scale = fmax(scale, fmax(fabs(a[k]), fabs(b[k])));
+    }
+    s->order[t] = t;
+  }
+  if (!s->boxes)
+    for (uint64_t i = 0; i < 3ull * s->V; ++i) scale = fmax(scale, fabs((double)s->pos[i]));
+  s->pad = OL_BOX_PAD * (1.0 + scale);
+  g_sort_scene = s;
+  if (T) build_rec(s, 0, T);
+  free(g_cent);
+  g_cent = NULL;
+}
+
 void* API(olgi_scene_new)(const float* pos, const float* nrm, uint32_t V, const uint32_t* idx, const uint32_t* mid,
                           uint32_t T, const float* mat, uint32_t M, const float* ana, uint32_t P, const float* area,
                           uint32_t L, float weight) {
@@ -342,22 +390,56 @@ void* API(olgi_scene_new)(const float* pos, const float* nrm, uint32_t V, const 
       break;
     }
   }
-  s->nodes = (BNode*)malloc(sizeof(BNode) * (2ull * T + 2));
-  s->order = (uint32_t*)malloc(4ull * T + 4);
-  g_cent = (double*)malloc(24ull * T + 8);
-  for (uint32_t t = 0; t < T; ++t) {
-    double a[3], b[3];
-    tri_box(s, t, a, b);
-    for (int k = 0; k < 3; ++k) g_cent[3ull * t + k] = 0.5 * (a[k] + b[k]);
-    s->order[t] = t;
+  build_bvh(s);
+  return s;
+}
+
+/* A two-level scene: R object-space resources (vertices pos/nrm split by
+ * voff[R + 1], local indices split by toff[R + 1]) placed by I instances
+ * (resource, material, first global triangle, world-to-object rows, normal
+ * matrix rows, determinant, world box min/max), with materials and lights
+ * as olgi_scene_new takes them (emitters as world-space records). */
+void* API(olgi_scene_new_instanced)(const float* pos, const float* nrm, const uint32_t* voff, const uint32_t* idx,
+                                    const uint32_t* toff, uint32_t R_count, uint32_t I, const uint32_t* ires,
+                                    const uint32_t* imat, const uint32_t* ioff, const float* w2o, const float* nmx,
+                                    const float* det, const float* wbox, const float* mat, uint32_t M,
+                                    const float* ana, uint32_t P, const float* area, uint32_t L, float weight) {
+  static const float none[4] = {0, 0, 0, 0};
+  static const uint32_t none_u[4] = {0, 0, 0, 0};
+  Scene* s = (Scene*)API(olgi_scene_new)(none, none, 0, none_u, none_u, 0, mat, M, ana, P, area, L, weight);
+  s->R_count = R_count;
+  s->res = (Scene*)calloc(R_count ? R_count : 1, sizeof(Scene));
+  for (uint32_t r = 0; r < R_count; ++r) {
+    Scene* q = &s->res[r];
+    q->V = voff[r + 1] - voff[r];
+    q->T = toff[r + 1] - toff[r];
+    q->pos = (float*)malloc(12ull * q->V + 4);
+    q->nrm = (float*)malloc(12ull * q->V + 4);
+    q->idx = (uint32_t*)malloc(12ull * q->T + 4);
+    memcpy(q->pos, pos + 3ull * voff[r], 12ull * q->V);
+    memcpy(q->nrm, nrm + 3ull * voff[r], 12ull * q->V);
+    memcpy(q->idx, idx + 3ull * toff[r], 12ull * q->T);
+    build_bvh(q);
   }
-  double scale = 0.0;
-  for (uint64_t i = 0; i < 3ull * V; ++i) scale = fmax(scale, fabs((double)s->pos[i]));
-  s->pad = OL_BOX_PAD * (1.0 + scale);
-  g_sort_scene = s;
-  if (T) build_rec(s, 0, T);
-  free(g_cent);
-  g_cent = NULL;
+  s->I = I;
+  s->ires = (uint32_t*)malloc(4ull * I + 4);
+  s->imat = (uint32_t*)malloc(4ull * I + 4);
+  s->ioff = (uint32_t*)malloc(4ull * I + 4);
+  s->w2o = (float*)malloc(48ull * I + 4);
+  s->nmx = (float*)malloc(36ull * I + 4);
+  s->det = (float*)malloc(4ull * I + 4);
+  memcpy(s->ires, ires, 4ull * I);
+  memcpy(s->imat, imat, 4ull * I);
+  memcpy(s->ioff, ioff, 4ull * I);
+  memcpy(s->w2o, w2o, 48ull * I);
+  memcpy(s->nmx, nmx, 36ull * I);
+  memcpy(s->det, det, 4ull * I);
+  s->T = I ? ioff[I - 1] + s->res[ires[I - 1]].T : 0u;
+  s->top = (Scene*)calloc(1, sizeof(Scene));
+  s->top->T = I;
+  s->top->boxes = (float*)malloc(24ull * I + 4);
+  memcpy(s->top->boxes, wbox, 24ull * I);
+  build_bvh(s->top);
   return s;
 }
 
@@ -378,6 +460,23 @@ void API(olgi_scene_free)(void* h) {
   free(s->uvs);
   free(s->tans);
   free(s->dens);
+  free(s->boxes);
+  if (s->top) API(olgi_scene_free)(s->top);
+  for (uint32_t r = 0; r < s->R_count; ++r) {
+    Scene* q = &s->res[r];
+    free(q->pos);
+    free(q->nrm);
+    free(q->idx);
+    free(q->nodes);
+    free(q->order);
+  }
+  free(s->res);
+  free(s->w2o);
+  free(s->nmx);
+  free(s->det);
+  free(s->ires);
+  free(s->imat);
+  free(s->ioff);
   free(s);
 }
 
@@ -446,7 +545,7 @@ static int box_hit(const BNode* n, const double o[3], const double inv[3], doubl
 }
 
 /* Closest hit in [tmin, tmax]: lowest t, ties to the lowest triangle index. */
-static int closest(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv, uint32_t* bi) {
+static int closest1(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv, uint32_t* bi) {
   *bt = tmax;
   *bi = 0xFFFFFFFFu;
   if (!s->T) return 0;
@@ -482,8 +581,7 @@ static int closest(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv
 
 static _Thread_local uint64_t g_shadow_rays;
 
-static int occluded(const Scene* s, V o, V d, R tmin, R tmax) {
-  ++g_shadow_rays;
+static int occluded1(const Scene* s, V o, V d, R tmin, R tmax) {
   if (!s->T) return 0;
   double od[3] = {o.x, o.y, o.z}, inv[3], dd[3] = {d.x, d.y, d.z};
   for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (fabs(dd[k]) > 1e-300 ? dd[k] : copysign(1e-300, dd[k]));
@@ -505,6 +603,135 @@ static int occluded(const Scene* s, V o, V d, R tmin, R tmax) {
     }
   }
   return 0;
+}
+
+/* Instance transforms, each component summed left to right as the device
+ * does: world -> object point and vector (rows of 4: x y z translation),
+ * object -> world normal (rows of 3). */
+static V xf_point(const float* r, V p) {
+  return v3((((R)r[0] * p.x + (R)r[1] * p.y) + (R)r[2] * p.z) + (R)r[3],
+            (((R)r[4] * p.x + (R)r[5] * p.y) + (R)r[6] * p.z) + (R)r[7],
+            (((R)r[8] * p.x + (R)r[9] * p.y) + (R)r[10] * p.z) + (R)r[11]);
+}
+static V xf_vector(const float* r, V v) {
+  return v3(((R)r[0] * v.x + (R)r[1] * v.y) + (R)r[2] * v.z, ((R)r[4] * v.x + (R)r[5] * v.y) + (R)r[6] * v.z,
+            ((R)r[8] * v.x + (R)r[9] * v.y) + (R)r[10] * v.z);
+}
+static V xf_normal(const float* m, V v) {
+  return v3(((R)m[0] * v.x + (R)m[1] * v.y) + (R)m[2] * v.z, ((R)m[3] * v.x + (R)m[4] * v.y) + (R)m[5] * v.z,
+            ((R)m[6] * v.x + (R)m[7] * v.y) + (R)m[8] * v.z);
+}
+
+/* Two-level closest hit: the top level's boxes in world space, each reached
+ * instance's resource in its object space (the direction is not
+ * renormalised, so t stays the world distance); ties to the lowest global
+ * triangle id.  *inst = the hit's instance. */
+static int closest2(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv, uint32_t* bi, uint32_t* inst) {
+  const Scene* tl = s->top;
+  *bt = tmax;
+  *bi = 0xFFFFFFFFu;
+  *inst = 0xFFFFFFFFu;
+  double od[3] = {o.x, o.y, o.z}, inv[3], dd[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (fabs(dd[k]) > 1e-300 ? dd[k] : copysign(1e-300, dd[k]));
+  uint32_t stack[256];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode* n = &tl->nodes[stack[--sp]];
+    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)*bt * (1.0 + 1e-5) + 1e-9)) continue;
+    if (n->right & 0x80000000u) {
+      const uint32_t cnt = n->right & 0x7FFFFFFFu;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t i = tl->order[n->left + k];
+        const float* w = s->w2o + 12ull * i;
+        R tt, uu, vv;
+        uint32_t local;
+        if (!closest1(&s->res[s->ires[i]], xf_point(w, o), xf_vector(w, d), tmin, *bt, &tt, &uu, &vv, &local))
+          continue;
+        const uint32_t id = s->ioff[i] + local;
+        if (tt < *bt || (tt == *bt && id < *bi)) {
+          *bt = tt;
+          *bu = uu;
+          *bv = vv;
+          *bi = id;
+          *inst = i;
+        }
+      }
+    } else {
+      stack[sp++] = n->left;
+      stack[sp++] = n->right;
+    }
+  }
+  return *bi != 0xFFFFFFFFu;
+}
+
+static int occluded2(const Scene* s, V o, V d, R tmin, R tmax) {
+  const Scene* tl = s->top;
+  double od[3] = {o.x, o.y, o.z}, inv[3], dd[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (fabs(dd[k]) > 1e-300 ? dd[k] : copysign(1e-300, dd[k]));
+  uint32_t stack[256];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode* n = &tl->nodes[stack[--sp]];
+    if (!box_hit(n, od, inv, (double)tmin - 1e-9, (double)tmax * (1.0 + 1e-5) + 1e-9)) continue;
+    if (n->right & 0x80000000u) {
+      const uint32_t cnt = n->right & 0x7FFFFFFFu;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t i = tl->order[n->left + k];
+        const float* w = s->w2o + 12ull * i;
+        if (occluded1(&s->res[s->ires[i]], xf_point(w, o), xf_vector(w, d), tmin, tmax)) return 1;
+      }
+    } else {
+      stack[sp++] = n->left;
+      stack[sp++] = n->right;
+    }
+  }
+  return 0;
+}
+
+static int closest(const Scene* s, V o, V d, R tmin, R tmax, R* bt, R* bu, R* bv, uint32_t* bi, uint32_t* inst) {
+  if (s->I) return closest2(s, o, d, tmin, tmax, bt, bu, bv, bi, inst);
+  *inst = 0xFFFFFFFFu;
+  return closest1(s, o, d, tmin, tmax, bt, bu, bv, bi);
+}
+
+static int occluded(const Scene* s, V o, V d, R tmin, R tmax) {
+  ++g_shadow_rays;
+  if (s->I) return occluded2(s, o, d, tmin, tmax);
+  return occluded1(s, o, d, tmin, tmax);
+}
+
+/* A hit's edge cross product (world space), interpolated shading normal
+ * (not normalised) and material: from the flat triangle, or from the
+ * instance's object-space triangle through its normal matrix and
+ * determinant (the device's shading-record path). */
+static void hit_frame(const Scene* s, uint32_t prim, uint32_t inst, R bu, R bv, V* cw, V* sn, uint32_t* mat) {
+  const Scene* g = s;
+  uint32_t t = prim;
+  if (s->I) {
+    g = &s->res[s->ires[inst]];
+    t = prim - s->ioff[inst];
+  }
+  const uint32_t* tv = g->idx + 3ull * t;
+  const float *pa = g->pos + 3ull * tv[0], *pb = g->pos + 3ull * tv[1], *pc = g->pos + 3ull * tv[2];
+  V e1 = v3((R)(pb[0] - pa[0]), (R)(pb[1] - pa[1]), (R)(pb[2] - pa[2]));
+  V e2 = v3((R)(pc[0] - pa[0]), (R)(pc[1] - pa[1]), (R)(pc[2] - pa[2]));
+  const float *q0 = g->nrm + 3ull * tv[0], *q1 = g->nrm + 3ull * tv[1], *q2 = g->nrm + 3ull * tv[2];
+  V n0 = v3(q0[0], q0[1], q0[2]), n1 = v3(q1[0], q1[1], q1[2]), n2 = v3(q2[0], q2[1], q2[2]);
+  *cw = cross(e1, e2);
+  if (s->I) {
+    const float* m = s->nmx + 9ull * inst;
+    *cw = vs(xf_normal(m, *cw), (R)s->det[inst]);
+    n0 = norm(xf_normal(m, n0));
+    n1 = norm(xf_normal(m, n1));
+    n2 = norm(xf_normal(m, n2));
+    *mat = s->imat[inst];
+  } else {
+    *mat = s->mid[prim];
+  }
+  const R wx = ((R)1 - bu) - bv, wy = bu, wz = bv;
+  *sn = vadd(vadd(vs(n0, wx), vs(n1, wy)), vs(n2, wz));
 }
 
 /* ---- materials and BSDF ----------------------------------------------------- */
@@ -1353,8 +1580,8 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
   if (rc) restir_clear_pixel(rc, pixel);
   for (uint32_t iter = 0; iter < st->max_bounces; ++iter) {
     R t = 0, bu = 0, bv = 0;
-    uint32_t prim;
-    int hit = closest(s, o, d, tmin, tmax, &t, &bu, &bv, &prim);
+    uint32_t prim, inst;
+    int hit = closest(s, o, d, tmin, tmax, &t, &bu, &bv, &prim, &inst);
     if (hit) t = jitter_t(t, hash32(pixel ^ hash32(frame_sample)) ^ iter);
     V inc = d;
     if (rmax(T.x, rmax(T.y, T.z)) < (R)0.0001) break;
@@ -1370,24 +1597,20 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
       break;
     }
     V hp = vadd(o, sv(t, d));
-    const uint32_t* tv = s->idx + 3ull * prim;
-    const float *pa = s->pos + 3ull * tv[0], *pb = s->pos + 3ull * tv[1], *pc = s->pos + 3ull * tv[2];
-    V e1 = v3((R)(pb[0] - pa[0]), (R)(pb[1] - pa[1]), (R)(pb[2] - pa[2]));
-    V e2 = v3((R)(pc[0] - pa[0]), (R)(pc[1] - pa[1]), (R)(pc[2] - pa[2]));
-    V gn = norm(cross(e1, e2));
-    R wx = ((R)1 - bu) - bv, wy = bu, wz = bv;
-    const float *n0 = s->nrm + 3ull * tv[0], *n1 = s->nrm + 3ull * tv[1], *n2 = s->nrm + 3ull * tv[2];
-    V sn = norm(vadd(vadd(vs(v3(n0[0], n0[1], n0[2]), wx), vs(v3(n1[0], n1[1], n1[2]), wy)),
-                     vs(v3(n2[0], n2[1], n2[2]), wz)));
+    V cw, sn_sum;
+    uint32_t mat_id;
+    hit_frame(s, prim, inst, bu, bv, &cw, &sn_sum, &mat_id);
+    V gn = norm(cw);
+    V sn = norm(sn_sum);
     if (dot(sn, gn) < (R)0) sn = vneg(sn);
     int entering = dot(inc, gn) < (R)0;
     V n = entering ? sn : vneg(sn);
-    Mat m = load_mat(s, s->mid[prim]);
-    V em = emission_of(s, s->mid[prim]);
+    Mat m = load_mat(s, mat_id);
+    V em = emission_of(s, mat_id);
     /* The ray cone at the hit (shadeRayCone :1217-1222). */
     R cone_hit = cone_w + t * cone_s;
-    if (s->uvs && mat_textured(s, s->mid[prim])) {
-      tex_surface(s, s->mid[prim], prim, bu, bv, gn, cone_hit, &m, &em, &sn);
+    if (s->uvs && mat_textured(s, mat_id)) {
+      tex_surface(s, mat_id, prim, bu, bv, gn, cone_hit, &m, &em, &sn);
       n = entering ? sn : vneg(sn);
     }
     uint32_t depth = (flags >> 8) > 1u ? (flags >> 8) : 1u;
@@ -1395,7 +1618,7 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
     if (entering || m.two_sided > (R)0.5) {
       R emis = 1;
       if ((flags & 2u) && dot(em, em) > (R)0) {
-        R area = (R)0.5 * len(cross(e1, e2));
+        R area = (R)0.5 * len(cw);
         R raw = dot(gn, vneg(inc));
         R lc = rmax(raw, (R)0);
         if (m.two_sided > (R)0.5) lc = absr(raw);
@@ -1461,7 +1684,7 @@ static void trace_path(const Scene* s, const Settings* st, const Restir* rc, uin
           V rdir;
           R rtmax;
           used_restir = 1;
-          if (restir_primary(s, rc, &rs, hp, n, &m, inc, t, pixel, s->mid[prim], &c, &rdir, &rtmax) &&
+          if (restir_primary(s, rc, &rs, hp, n, &m, inc, t, pixel, mat_id, &c, &rdir, &rtmax) &&
               !occluded(s, vadd(hp, vs(n, (R)0.002)), rdir, (R)0.001, rtmax))
             direct = vadd(direct, c);
           uint32_t ec = st->env_samples < 4u ? st->env_samples : 4u;
@@ -1658,16 +1881,14 @@ int API(olgi_render_products)(void* h, const float cam[12], uint32_t proj, uint3
         V o, d;
         camera_ray(cam, proj, W, H, px, (uint32_t)py, frame, 0, &o, &d);
         R t, u, v;
-        uint32_t tri;
-        const int hit = closest(s, o, d, (R)0.001, (R)1e30, &t, &u, &v, &tri);
+        uint32_t tri, inst;
+        const int hit = closest(s, o, d, (R)0.001, (R)1e30, &t, &u, &v, &tri, &inst);
         V n = v3(0, 0, 0);
         if (hit) {
-          const float* a = s->pos + 3ull * s->idx[3ull * tri];
-          const float* b = s->pos + 3ull * s->idx[3ull * tri + 1];
-          const float* c = s->pos + 3ull * s->idx[3ull * tri + 2];
-          V e1 = v3((R)(b[0] - a[0]), (R)(b[1] - a[1]), (R)(b[2] - a[2]));
-          V e2 = v3((R)(c[0] - a[0]), (R)(c[1] - a[1]), (R)(c[2] - a[2]));
-          n = norm(cross(e1, e2));
+          V cw, sn_sum;
+          uint32_t mat_id;
+          hit_frame(s, tri, inst, u, v, &cw, &sn_sum, &mat_id);
+          n = norm(cw);
         }
         if (depth) depth[pix] = hit ? (float)t : INFINITY;
         if (normal) {
@@ -1776,9 +1997,9 @@ int API(olgi_trace)(void* h, const float* o, const float* d, uint32_t n, float* 
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < (int64_t)n; ++i) {
     R t, u = 0, v = 0;
-    uint32_t prim;
+    uint32_t prim, inst;
     V oo = v3(o[4 * i], o[4 * i + 1], o[4 * i + 2]), dd = v3(d[4 * i], d[4 * i + 1], d[4 * i + 2]);
-    int hit = closest(s, oo, dd, (R)o[4 * i + 3], (R)d[4 * i + 3], &t, &u, &v, &prim);
+    int hit = closest(s, oo, dd, (R)o[4 * i + 3], (R)d[4 * i + 3], &t, &u, &v, &prim, &inst);
     hits[4 * i] = (float)t;
     hits[4 * i + 1] = (float)u;
     hits[4 * i + 2] = (float)v;

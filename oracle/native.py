@@ -43,6 +43,11 @@ def lib(mutant=0):
                 P_f32, P_f32, ctypes.c_uint32, P_u32, P_u32, ctypes.c_uint32, P_f32, ctypes.c_uint32,
                 P_f32, ctypes.c_uint32, P_f32, ctypes.c_uint32, ctypes.c_float]
             getattr(L, "olgi_scene_free" + sfx).argtypes = [ctypes.c_void_p]
+            getattr(L, "olgi_scene_new_instanced" + sfx).restype = ctypes.c_void_p
+            getattr(L, "olgi_scene_new_instanced" + sfx).argtypes = [
+                P_f32, P_f32, P_u32, P_u32, P_u32, ctypes.c_uint32, ctypes.c_uint32, P_u32, P_u32, P_u32,
+                P_f32, P_f32, P_f32, P_f32, P_f32, ctypes.c_uint32, P_f32, ctypes.c_uint32, P_f32, ctypes.c_uint32,
+                ctypes.c_float]
             getattr(L, "olgi_texture_sample" + sfx).argtypes = [
                 ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                 ctypes.c_int, ctypes.c_float, ctypes.c_float, P_f32]
@@ -107,6 +112,9 @@ class OracleScene:
         self.packed = packed
         self.mutant = int(mutant)
         L = lib(self.mutant)
+        if hasattr(packed, "instance_resource"):
+            self._init_instanced(L, packed)
+            return
         # Per-triangle material parameters become full records per triangle.
         materials, material_ids = packed.materials, packed.material_ids
         if getattr(packed, "face_params", None) is not None:
@@ -136,6 +144,24 @@ class OracleScene:
                 self.handle, _p(tex[0], ctypes.c_uint32), len(tex[0]), _p(tex[1]), len(tex[1]) if bind is not None else 0,
                 _p(tex[2]) if uvs is not None else None, _p(tex[3]) if uvs is not None else None,
                 _p(tex[4]) if uvs is not None else None)
+
+    def _init_instanced(self, L, p):
+        """Two-level scene (ordinarylight_amd.packing.PackedInstancedScene):
+        object-space resources and instances, intersected in object space."""
+        w2o = np.ascontiguousarray(p.world_to_object, np.float32).reshape(-1, 12)
+        self._arrays = [np.ascontiguousarray(a) for a in (
+            p.positions.astype(np.float32), p.normals.astype(np.float32), p.vertex_offsets.astype(np.uint32),
+            p.indices.astype(np.uint32), p.triangle_offsets.astype(np.uint32), p.instance_resource.astype(np.uint32),
+            p.instance_material.astype(np.uint32), p.instance_triangle_offset.astype(np.uint32), w2o,
+            np.asarray(p.normal_matrix, np.float32).reshape(-1, 9), p.determinant.astype(np.float32),
+            p.world_bounds.astype(np.float32), p.materials.astype(np.float32), p.analytic_lights.astype(np.float32),
+            p.area_lights.astype(np.float32))]
+        pos, nrm, voff, idx, toff, ires, imat, ioff, w2o, nmx, det, box, mat, ana, area = self._arrays
+        u32 = ctypes.c_uint32
+        self.handle = getattr(L, "olgi_scene_new_instanced" + self.sfx)(
+            _p(pos), _p(nrm), _p(voff, u32), _p(idx, u32), _p(toff, u32), len(voff) - 1, len(ires), _p(ires, u32),
+            _p(imat, u32), _p(ioff, u32), _p(w2o), _p(nmx), _p(det), _p(box), _p(mat), len(mat), _p(ana),
+            p.analytic_light_count, _p(area), p.area_light_count, p.area_light_weight)
 
     def set_jitter(self, seed, ulps=1, t_ulps=0):
         """Perturbation in later renders (seed 0 = off): every uniform sample
