@@ -186,9 +186,7 @@ class Engine {
   std::deque<uint64_t> queue_;    // waiting for admission (FIFO)
   std::vector<uint64_t> slot_job_;  // kMaxJobs: job id per device slot (0 = free)
   std::vector<uint64_t> acc_jobs_;  // jobs with finished chunks not yet accumulated
-  std::vector<DevBuf> spare_;       // buffers of forgotten jobs (kept under kSpareBytes)
-  size_t spare_bytes_ = 0;
-  static constexpr size_t kSpareBytes = size_t(4) << 30;
+  static constexpr size_t kSpareBytes = size_t(4) << 30;  // the context's pool of spare job buffers
   RangeAllocator paths_, entries_;
   uint64_t next_id_ = 1;
   uint32_t parity_ = 0;           // input queue of the next step
@@ -244,6 +242,13 @@ struct olhip_ctx {
   int lds_stack = 16;
   int trace_block = 128;
   int fused_primary = 0;  // olhip_set_fused_primary
+  // Device buffers of forgotten jobs, shared by the context's engines (an
+  // engine created later reuses what earlier ones released, so a new server
+  // allocates nothing for jobs of sizes already seen), kept under
+  // Engine::kSpareBytes.  Declared before the engines it outlives.
+  std::mutex spare_mu;
+  std::vector<DevBuf> spare;
+  size_t spare_bytes = 0;
   // Engine of the single-frame entry point (created on first use).
   std::unique_ptr<Engine> frame_engine;
   uint64_t frame_engine_slots = 0;

@@ -136,6 +136,8 @@ Engine::~Engine() {
   stop_worker();
   (void)hipSetDevice(ctx_->device);
   (void)hipStreamSynchronize(ctx_->stream);
+  // Jobs never taken leave their buffers to the context's pool.
+  for (auto& kv : jobs_) keep_buffers(*kv.second);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (int b = 0; b < 2; ++b) {
     if (done_[b]) (void)hipEventDestroy(done_[b]);
@@ -411,33 +413,38 @@ bool Engine::admit(Job& j, int slot) {
   return false;
 }
 
-// The smallest spare buffer that holds `bytes` (its contents are not
-// preserved), else a new allocation.  Called with mu_ held.
+// The smallest spare buffer of the context's pool that holds `bytes` (its
+// contents are not preserved), else a new allocation.  Called with mu_ held.
 void Engine::take_buffer(DevBuf& dst, size_t bytes) {
-  size_t best = spare_.size();
-  for (size_t k = 0; k < spare_.size(); ++k)
-    if (spare_[k].bytes >= bytes && (best == spare_.size() || spare_[k].bytes < spare_[best].bytes)) best = k;
-  if (best == spare_.size()) {
-    dst.reserve(bytes);
-    return;
+  {
+    std::lock_guard<std::mutex> g(ctx_->spare_mu);
+    std::vector<DevBuf>& spare = ctx_->spare;
+    size_t best = spare.size();
+    for (size_t k = 0; k < spare.size(); ++k)
+      if (spare[k].bytes >= bytes && (best == spare.size() || spare[k].bytes < spare[best].bytes)) best = k;
+    if (best != spare.size()) {
+      ctx_->spare_bytes -= spare[best].bytes;
+      dst = std::move(spare[best]);
+      spare.erase(spare.begin() + best);
+      return;
+    }
   }
-  spare_bytes_ -= spare_[best].bytes;
-  dst = std::move(spare_[best]);
-  spare_.erase(spare_.begin() + best);
+  dst.reserve(bytes);
 }
 
-// Keeps a forgotten job's buffers for later jobs, evicting the oldest spare
-// buffers beyond kSpareBytes.  Called with mu_ held, once the job's last
-// step has completed.
+// Keeps a forgotten job's buffers in the context's pool for later jobs,
+// evicting the oldest beyond kSpareBytes.  Called once the job's last step
+// has completed (no queued work uses the buffers).
 void Engine::keep_buffers(Job& j) {
+  std::lock_guard<std::mutex> g(ctx_->spare_mu);
   for (DevBuf* b : {&j.acc, &j.image, &j.geo, &j.prim, &j.lum, &j.variance, &j.res, &j.surf}) {
     if (!b->ptr) continue;
-    spare_bytes_ += b->bytes;
-    spare_.push_back(std::move(*b));
+    ctx_->spare_bytes += b->bytes;
+    ctx_->spare.push_back(std::move(*b));
   }
-  while (spare_bytes_ > kSpareBytes && !spare_.empty()) {
-    spare_bytes_ -= spare_.front().bytes;
-    spare_.erase(spare_.begin());
+  while (ctx_->spare_bytes > kSpareBytes && !ctx_->spare.empty()) {
+    ctx_->spare_bytes -= ctx_->spare.front().bytes;
+    ctx_->spare.erase(ctx_->spare.begin());
   }
 }
 

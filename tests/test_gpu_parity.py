@@ -300,16 +300,19 @@ def test_e0_projections_match_numpy_oracle(hip_device, kind):
 
 
 def test_e0_bvh_path_matches_numpy_oracle(hip_device):
-    """Larger scene through the BVH traversal path (lexicographic ties)."""
+    """Larger scene through the BVH traversal path (lexicographic ties):
+    primary hits equal the oracle's brute-force scan on every pixel, and
+    every pixel agrees within the tolerance (the BVH only orders the same
+    candidate tests, so nothing may differ)."""
     scene, cam = workloads.terrain(4, nx=41, nz=21, chunk=20, emissive_quads=3)
     r = HipReferenceRenderer(samples_per_pixel=1, max_bounces=3, seed=2)
-    img = r.render_frame(scene, cam, 40, 24, capture_primary_hits=True)
+    img = r.render_frame(scene, cam, 80, 48, capture_primary_hits=True)
     trace = []
-    ref = e0_numpy.render_scene(scene, cam, 40, 24, samples=1, max_bounces=3, seed=2, trace=trace)
-    assert np.mean(r.last_primary_hits == trace[0][3]) >= 0.999
+    ref = e0_numpy.render_scene(scene, cam, 80, 48, samples=1, max_bounces=3, seed=2, trace=trace)
+    same = r.last_primary_hits == trace[0][3]
     ok = rel_close(img[..., :3], ref[..., :3])
-    log(f"  e0 bvh within tol {ok.mean():.6f}")
-    assert ok.mean() >= 0.995
+    log(f"  e0 bvh within tol {ok.mean():.6f}, primary hits equal {same.mean():.6f}")
+    assert same.all() and ok.all()
     r.close()
 
 
@@ -345,8 +348,10 @@ def test_named_products_match_oracle(hip_device, case):
     exp_mat[hit] = tri_mat[prim[hit]]
     same = got["instance_id"] == exp_inst
     log(f"products {case}: hit {hit.mean():.3f} instance ids equal {same.mean():.5f}")
-    assert same.mean() >= 0.999 and np.array_equal(got["instance_id"], got["object_id"])
-    assert (got["material_id"] == exp_mat).mean() >= 0.999
+    # Both sides resolve the same camera rays by the same closest-hit rule
+    # (lowest distance, then lowest triangle index), so every pixel agrees.
+    assert same.all() and np.array_equal(got["instance_id"], got["object_id"])
+    assert np.array_equal(got["material_id"], exp_mat)
     ok = same & (got["material_id"] == exp_mat)
     assert np.array_equal(got["depth"][ok].view(np.uint32), ref["depth"][ok].view(np.uint32))
     assert np.array_equal(got["normal"][ok].view(np.uint32), ref["normal"][ok].view(np.uint32))
