@@ -450,6 +450,71 @@ class _ResidentMixin:
                 self._context = None
 
 
+def capture_motion_state(scene, camera, size):
+    """What the next frame's motion output needs of this one: the scene
+    object, camera, image size and each visible mesh's vertices, indices
+    and transform (copies), keyed by mesh id."""
+    return {"scene": scene, "camera": camera, "size": tuple(size),
+            "meshes": {mesh.id: (np.array(mesh.vertices, copy=True), np.array(mesh.indices, copy=True),
+                                 np.array(mesh.transform.matrix, copy=True)) for mesh in scene.visible_meshes}}
+
+
+def project_positions(positions, camera, width, height):
+    """Pixel coordinates of world points under a look-at camera with a
+    vertical field of view, and whether each lies in front of it (float64,
+    as the reference's host projection, targets/vulkan/api.py:1118-1137)."""
+    origin = np.asarray(camera.position, np.float64)
+    forward = np.asarray(camera.target, np.float64) - origin
+    forward = forward / np.linalg.norm(forward)
+    right = np.cross(forward, np.asarray(camera.up, np.float64))
+    right = right / np.linalg.norm(right)
+    up = np.cross(right, forward)
+    rel = np.asarray(positions, np.float64) - origin
+    depth = rel @ forward
+    half = math.tan(math.radians(float(camera.vertical_fov_degrees)) * 0.5)
+    z = np.maximum(depth, 1e-12)
+    x = (rel @ right) / (z * half * (width / height))
+    y = -(rel @ up) / (z * half)
+    return np.stack(((x + 1.0) * 0.5 * width, (y + 1.0) * 0.5 * height), axis=1), depth > 1e-8
+
+
+def motion_vectors(scene, camera, width, height, primitive, position, barycentric, previous):
+    """Per-pixel motion of sample 0's primary hit since the previous frame
+    (the reference's opt-in ``motion`` product, targets/vulkan/api.py:1151-1200):
+    the hit's barycentric point on its triangle as the previous frame's mesh
+    vertices and transform placed it, projected by the previous camera, is
+    subtracted from the hit's projection now.  Zero without a previous frame
+    of the same scene object and size, on misses, for meshes whose triangles
+    changed, and where either projection is behind its camera."""
+    motion = np.zeros((height, width, 2), np.float32)
+    if previous is None or previous["scene"] is not scene or previous["size"] != (width, height):
+        return motion
+    prim = np.asarray(primitive).reshape(-1)
+    pos = np.asarray(position).reshape(-1, 3)
+    bary = np.asarray(barycentric).reshape(-1, 2)
+    flat = motion.reshape(-1, 2)
+    first = 0
+    for mesh in scene.visible_meshes:
+        count = len(mesh.indices)
+        rows = np.flatnonzero((prim >= first) & (prim < first + count))
+        old = previous["meshes"].get(mesh.id)
+        if rows.size and old is not None:
+            vertices, indices, transform = old
+            if indices.shape == np.shape(mesh.indices) and np.array_equal(indices, mesh.indices) and \
+                    vertices.shape == np.shape(mesh.vertices):
+                corners = vertices[indices[prim[rows].astype(np.int64) - first]]
+                u, v = bary[rows, 0], bary[rows, 1]
+                weights = np.stack((1.0 - u - v, u, v), axis=1)
+                local = np.einsum("nk,nkc->nc", weights, corners)
+                world = local @ transform[:3, :3].T + transform[:3, 3]
+                now, ok_now = project_positions(pos[rows], camera, width, height)
+                then, ok_then = project_positions(world, previous["camera"], width, height)
+                ok = ok_now & ok_then
+                flat[rows[ok]] = (now[ok] - then[ok]).astype(np.float32)
+        first += count
+    return motion
+
+
 def _check_size(width, height):
     width, height = int(width), int(height)
     if width < 1 or height < 1:
@@ -460,7 +525,7 @@ def _check_size(width, height):
 class HipPathTracer(_ResidentMixin, RendererImplementation):
     """GI wavefront path tracer on one MI355X (drop-in for the Vulkan GI path)."""
 
-    available_outputs = ("color", "variance", "depth", "normal", "instance_id", "object_id", "material_id")
+    available_outputs = ("color", "variance", "depth", "normal", "instance_id", "object_id", "material_id", "motion")
     implementation = RendererImplementationInfo(
         name="hip-wavefront-gi", family="global_illumination", graphics_api="hip",
     )
@@ -475,6 +540,15 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
         self.device = f"HIP device {self.config.device_index}"
         self.last_timings = {}
         self.last_stats = None
+        self._output_history = None
+
+    def reset_output_history(self):
+        """Discard the previous frame's state that the motion output uses."""
+        self._output_history = None
+
+    def replace_scene(self, scene):
+        _ResidentMixin.replace_scene(self, scene)
+        self.reset_output_history()
 
     @property
     def capabilities(self):
@@ -582,7 +656,19 @@ class HipPathTracer(_ResidentMixin, RendererImplementation):
             buffers["variance"] = np.empty((height, width), np.float32)
         with self._lock:
             color = self._render(scene, camera, width, height, samples, frame_index, products=buffers)
+            if "motion" in outputs:
+                # Sample 0's primary hits with their barycentrics: the render's
+                # own camera rays against the resident hierarchy (the same
+                # closest-hit rule, so the same hits as the primitive product).
+                o, d = self._ensure_context().camera_rays(camera, width, height, int(frame_index), 0)
+                t, u, v, prim = self._resident(scene).trace(o[:, :3], d[:, :3], float(o[0, 3]), float(d[0, 3]))
+                position = o[:, :3] + t[:, None] * d[:, :3]
+                motion = motion_vectors(scene, camera, width, height, prim, position, np.stack((u, v), axis=1),
+                                        self._output_history)
+                self._output_history = capture_motion_state(scene, camera, (width, height))
         result = {}
+        if "motion" in outputs:
+            result["motion"] = motion
         if "color" in outputs:
             result["color"] = color
         for name in ("depth", "normal", "variance"):

@@ -433,3 +433,33 @@ def test_e0_image_environment_matches_numpy_oracle(hip_device):
         ok = rel_close(img[..., :3], ref[..., :3])
         log(f"  within tol {ok.mean():.6f} exact {np.all(img == ref, axis=-1).mean():.6f}")
         assert ok.mean() >= 0.999
+
+
+def test_motion_product_matches_oracle_hits(hip_device):
+    """The motion output over two frames of a static scene seen by a moving
+    camera: zero on the first frame; on the second, the projection of each
+    pixel's primary hit under the new camera minus its projection under the
+    old one, with the hits taken from the oracle's traversal of the same
+    camera rays (the device's hits equal them bit for bit)."""
+    from ordinarylight_amd.cameras import PerspectiveCamera
+    from ordinarylight_amd.renderers.hip import motion_vectors, capture_motion_state
+    scene, cam0 = workloads.box_room(5)
+    cam1 = PerspectiveCamera(position=tuple(np.asarray(cam0.position) + (0.05, 0.02, 0.0)),
+                             target=tuple(np.asarray(cam0.target) + (0.02, 0.0, 0.0)), up=cam0.up,
+                             vertical_fov_degrees=cam0.vertical_fov_degrees)
+    w, h = 64, 48
+    r = HipPathTracer(max_bounces=3, samples_per_pixel=2)
+    first = r.render_products(scene, cam0, w, h, outputs=("motion", "color"), frame_index=0)["motion"]
+    second = r.render_products(scene, cam1, w, h, outputs=("motion",), frame_index=1)["motion"]
+    r.close()
+    assert first.shape == (h, w, 2) and not first.any()
+    orc = oracle.OracleScene(packing.pack_scene(scene))
+    o, d = oracle.camera_rays(cameras.camera_basis(cam1), cameras.camera_projection(cam1), w, h, 1, 0)
+    t, u, v, prim = orc.trace(o[:, :3], d[:, :3], float(o[0, 3]), float(d[0, 3]))
+    orc.close()
+    position = o[:, :3] + t[:, None] * d[:, :3]
+    expect = motion_vectors(scene, cam1, w, h, prim, position, np.stack((u, v), axis=1),
+                            capture_motion_state(scene, cam0, (w, h)))
+    log(f"motion: max |m| {np.abs(second).max():.3f} px, max error {np.abs(second - expect).max():.2e}")
+    assert np.abs(expect).max() > 0.5
+    assert np.allclose(second, expect, atol=1e-4)

@@ -198,7 +198,7 @@ def test_render_products_validates_outputs_before_rendering():
     from ordinarylight_amd.renderers.hip import HipPathTracer
     scene, cam = workloads.box_room(0)
     r = HipPathTracer()
-    for bad in ((), ("color", "motion"), ("albedo",)):
+    for bad in ((), ("color", "albedo"), ("albedo",), ("color", "color")):
         with pytest.raises(ValueError):
             r.render_products(scene, cam, 4, 4, outputs=bad)
     with pytest.raises(ValueError):
@@ -328,3 +328,60 @@ def test_textured_scene_packing():
     assert len(env) == 1 and env[0][3, 0] == p.texture_words[0] - 1 and env[0][3, 2] > 0.0
     with pytest.raises(ValueError):
         packing.pack_instanced_scene(scene)
+
+
+def test_motion_vectors_closed_forms():
+    """The motion output's host math (the reference's opt-in product):
+    zero without history, zero for a static scene and camera, the
+    projection difference for a moved camera, the mesh's own displacement
+    for a moved mesh, and zero where a mesh's triangles changed."""
+    from ordinarylight_amd import workloads
+    from ordinarylight_amd.cameras import PerspectiveCamera
+    from ordinarylight_amd.renderers.hip import capture_motion_state, motion_vectors, project_positions
+    from ordinarylight_amd.scene import Transform
+    scene, cam = workloads.box_room(0)
+    w, h = 16, 12
+    packed = packing.pack_scene(scene)
+    rng = np.random.default_rng(3)
+    tri = rng.integers(0, len(packed.indices), (h, w)).astype(np.uint32)
+    bary = rng.uniform(0, 0.5, (h, w, 2)).astype(np.float32)
+    corners = packed.positions[packed.indices[tri.reshape(-1)]].astype(np.float64)
+    weights = np.stack((1 - bary[..., 0].reshape(-1) - bary[..., 1].reshape(-1), bary[..., 0].reshape(-1),
+                        bary[..., 1].reshape(-1)), 1)
+    position = np.einsum("nk,nkc->nc", weights, corners).astype(np.float32).reshape(h, w, 3)
+    assert not motion_vectors(scene, cam, w, h, tri, position, bary, None).any()
+    state = capture_motion_state(scene, cam, (w, h))
+    still = motion_vectors(scene, cam, w, h, tri, position, bary, state)
+    assert np.abs(still).max() < 1e-3
+    moved = PerspectiveCamera(position=tuple(np.asarray(cam.position) + (0.1, 0.05, 0.0)),
+                              target=tuple(np.asarray(cam.target) + (0.1, 0.05, 0.0)), up=cam.up,
+                              vertical_fov_degrees=cam.vertical_fov_degrees)
+    m = motion_vectors(scene, moved, w, h, tri, position, bary, state)
+    now, ok1 = project_positions(position.reshape(-1, 3), moved, w, h)
+    then, ok2 = project_positions(position.reshape(-1, 3), cam, w, h)
+    expect = np.where((ok1 & ok2)[:, None], now - then, 0.0).reshape(h, w, 2)
+    assert np.allclose(m, expect, atol=1e-3) and np.abs(expect).max() > 0.1
+    # A mesh moved by a translation: its pixels move by the projection of
+    # the displacement, other meshes' pixels do not move.
+    mesh = scene.visible_meshes[0]
+    first = 0
+    count = len(mesh.indices)
+    old = mesh.transform.matrix.copy()
+    mesh.transform = Transform(np.asarray(old) @ np.array(
+        [[1, 0, 0, 0.2], [0, 1, 0, 0.0], [0, 0, 1, 0.0], [0, 0, 0, 1.0]]))
+    shifted = position.copy()
+    on_mesh = (tri >= first) & (tri < first + count)
+    shifted[on_mesh] += np.float32(0.2) * np.asarray(old)[:3, 0].astype(np.float32)
+    m2 = motion_vectors(scene, cam, w, h, tri, shifted, bary, state)
+    now, a = project_positions(shifted.reshape(-1, 3), cam, w, h)
+    then, b = project_positions(position.reshape(-1, 3), cam, w, h)
+    exp2 = np.where((a & b)[:, None], now - then, 0.0).reshape(h, w, 2)
+    assert on_mesh.any() and np.allclose(m2[on_mesh], exp2[on_mesh], atol=2e-3)
+    assert np.abs(m2[~on_mesh]).max() < 1e-3
+    # Changed triangles: no motion for that mesh.
+    state2 = capture_motion_state(scene, cam, (w, h))
+    key = mesh.id
+    v, i, t = state2["meshes"][key]
+    state2["meshes"][key] = (v, i[::-1].copy(), t)
+    m3 = motion_vectors(scene, moved, w, h, tri, shifted, bary, state2)
+    assert not m3[on_mesh].any()
