@@ -61,16 +61,23 @@ constexpr uint32_t C_Q0 = 0, C_Q1 = kRuns, C_G0 = 2 * kRuns, C_G1 = 3 * kRuns, C
 // Chunk prefixes are kept separately for jobs of flat scenes (C_RP, C_GP)
 // and of two-level scenes (C_RP2, C_GP2), so each traversal kernel variant
 // sees one kind only.
-constexpr uint32_t C_RP = 6 * kRuns;             // kRuns + 1: chunk prefix of the input ray queue (flat)
-constexpr uint32_t C_GP = C_RP + kRuns + 1;      // kRuns + 1: chunk prefix of the shadow groups (flat)
-constexpr uint32_t C_RP2 = C_GP + kRuns + 1;     // the same for two-level scenes
-constexpr uint32_t C_GP2 = C_RP2 + kRuns + 1;
+// A plan: the exclusive prefix of chunk counts over the runs (kRuns + 1
+// words, the last the total), then a bucket index that takes a chunk to its
+// run in a few steps: the chunks are cut into at most kPlanBuckets buckets
+// of 2^shift chunks, and bucket b names the run holding its first chunk.
+constexpr uint32_t kPlanBuckets = 1024;
+constexpr uint32_t P_SHIFT = kRuns + 1, P_BUCKETS = kRuns + 2, P_LAST = kRuns + 3, P_INDEX = kRuns + 4;
+constexpr uint32_t kPlanWords = P_INDEX + kPlanBuckets;
+constexpr uint32_t C_RP = 6 * kRuns;             // chunk plan of the input ray queue (flat)
+constexpr uint32_t C_GP = C_RP + kPlanWords;     // chunk plan of the shadow groups (flat)
+constexpr uint32_t C_RP2 = C_GP + kPlanWords;    // the same for two-level scenes
+constexpr uint32_t C_GP2 = C_RP2 + kPlanWords;
 // Work cursors of the refilling traversal kernels, one per 256-byte line:
 // every wave's chunk claim is an atomic on its stripe's cursor, and cursors
 // sharing a line serialise there (C2 traversal 6.12 -> 5.60 ms per frame,
 // small single jobs far more).
 constexpr uint32_t kCursorStride = 64;
-constexpr uint32_t C_W = ((C_GP2 + kRuns + 1 + 63) / 64) * 64;  // closest-hit cursors (flat)
+constexpr uint32_t C_W = ((C_GP2 + kPlanWords + 63) / 64) * 64;  // closest-hit cursors (flat)
 constexpr uint32_t C_WS = C_W + kWorkStripes * kCursorStride;   // shadow cursors (flat)
 constexpr uint32_t C_W2 = C_WS + kWorkStripes * kCursorStride;  // closest-hit cursors (two-level)
 constexpr uint32_t C_WS2 = C_W2 + kWorkStripes * kCursorStride; // shadow cursors (two-level)

@@ -44,14 +44,15 @@ constexpr uint32_t kRefillMin2 = 8;
 // Up to four plans in one launch, one block each: block b computes the
 // exclusive prefix of ceil(count / kChunk) over the runs of counter block
 // counts[b] whose jobs' scenes are flat (kind 0) or two-level (kind 1), into
-// out[b] (out[b][kRuns] = total).  Offsets are counter-block words.
+// out[b] (out[b][kRuns] = total), and the plan's bucket index (P_SHIFT ..
+// P_INDEX).  Offsets are counter-block words.
 struct PlanSet {
   uint32_t counts[4], out[4], kind[4];
 };
 
 __global__ void __launch_bounds__(1024) k_plan(uint32_t* __restrict__ ctr, const JobDev* __restrict__ jobs, PlanSet ps) {
   static_assert(kRuns == 2 * 1024 && kJobStripes % 2 == 0, "two runs of one job per thread");
-  __shared__ uint32_t wave_tot[16];
+  __shared__ uint32_t wave_tot[16], wave_last[16], shift_s;
   const uint32_t t = threadIdx.x;
   const uint32_t* __restrict__ counts = ctr + ps.counts[blockIdx.x];
   uint32_t* __restrict__ out = ctr + ps.out[blockIdx.x];
@@ -65,8 +66,12 @@ __global__ void __launch_bounds__(1024) k_plan(uint32_t* __restrict__ ctr, const
     const uint32_t o = __shfl_up(incl, (unsigned)off, 64);
     if ((int)lane_id() >= off) incl += o;
   }
+  // The last run holding chunks bounds the consumers' run search.
+  uint32_t last = b ? 2u * t + 1u : (a ? 2u * t : 0u);
+  for (int off = 32; off; off >>= 1) last = max(last, (uint32_t)__shfl_xor((int)last, off, 64));
   const uint32_t wave = t / 64u;
   if (lane_id() == 63u) wave_tot[wave] = incl;
+  if (lane_id() == 0u) wave_last[wave] = last;
   __syncthreads();
   uint32_t base = 0, total = 0;
   for (uint32_t w = 0; w < 16u; ++w) {
@@ -76,16 +81,50 @@ __global__ void __launch_bounds__(1024) k_plan(uint32_t* __restrict__ ctr, const
   const uint32_t excl = base + incl - sum;
   out[2 * t] = excl;
   out[2 * t + 1] = excl + a;
-  if (t == 0) out[kRuns] = total;
+  if (t == 0) {
+    uint32_t shift = 0;
+    while (((total + (1u << shift) - 1u) >> shift) > kPlanBuckets) ++shift;
+    uint32_t top = 0;
+    for (uint32_t w = 0; w < 16u; ++w) top = max(top, wave_last[w]);
+    out[kRuns] = total;
+    out[P_SHIFT] = shift;
+    out[P_BUCKETS] = (total + (1u << shift) - 1u) >> shift;
+    out[P_LAST] = top;
+    shift_s = shift;
+  }
+  __syncthreads();
+  // Bucket k starts at chunk k << shift; the run whose chunks hold that
+  // start names it (runs without chunks name none).
+  const uint32_t shift = shift_s;
+  const uint32_t firsts[2] = {excl, excl + a}, ns[2] = {a, b};
+  for (int h = 0; h < 2; ++h) {
+    if (ns[h] == 0u) continue;
+    const uint32_t p = firsts[h];
+    for (uint32_t k = (p + (1u << shift) - 1u) >> shift; k <= (p + ns[h] - 1u) >> shift; ++k) out[P_INDEX + k] = 2u * t + (uint32_t)h;
+  }
 }
 
-// Run holding chunk c: the largest r with prefix[r] <= c.
-OL_DEV uint32_t find_run(const uint32_t* __restrict__ prefix, uint32_t c) {
-  uint32_t r = 0;
-#pragma unroll
-  for (uint32_t step = kRuns / 2u; step; step >>= 1)
-    if (prefix[r + step] <= c) r += step;
-  return r;
+// A plan's bucket parameters, read once per kernel.
+struct PlanHead {
+  uint32_t shift, buckets, last;
+};
+OL_DEV PlanHead plan_head(const uint32_t* __restrict__ prefix) {
+  return PlanHead{prefix[P_SHIFT], prefix[P_BUCKETS], prefix[P_LAST]};
+}
+
+// Run holding chunk c: the largest r with prefix[r] <= c.  It lies between
+// the run of c's bucket and the run of the next bucket (or the last run
+// holding chunks): a bisection over that range, usually a step or two.
+OL_DEV uint32_t find_run(const uint32_t* __restrict__ prefix, const PlanHead& h, uint32_t c) {
+  const uint32_t k = c >> h.shift;
+  uint32_t lo = prefix[P_INDEX + k];
+  uint32_t hi = k + 1u < h.buckets ? prefix[P_INDEX + k + 1u] : h.last;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (prefix[mid] <= c) lo = mid;
+    else hi = mid - 1u;
+  }
+  return lo;
 }
 
 // A chunk located in its run: job slot, segment and the first queue slot.
@@ -94,10 +133,10 @@ struct ChunkLoc {
   uint32_t chunk_in_job;               // chunk index counted from the job's first run
 };
 
-OL_DEV ChunkLoc locate_chunk(const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ counts,
+OL_DEV ChunkLoc locate_chunk(const uint32_t* __restrict__ prefix, const PlanHead& top, const uint32_t* __restrict__ counts,
                              const JobDev* __restrict__ jobs, uint32_t c) {
   ChunkLoc k;
-  const uint32_t r = find_run(prefix, c);
+  const uint32_t r = find_run(prefix, top, c);
   k.job = r / kJobStripes;
   k.stripe = r % kJobStripes;
   const uint32_t off = (c - prefix[r]) * kChunk;
@@ -337,8 +376,8 @@ struct Pool {
 };
 
 // Takes the next chunk of this block's work stripe when the pool is empty.
-OL_DEV void pool_refill(Pool& p, const Wave& W, const uint32_t* prefix, const uint32_t* counts, uint32_t* cursor,
-                        uint32_t c_begin, uint32_t c_end) {
+OL_DEV void pool_refill(Pool& p, const Wave& W, const uint32_t* prefix, const PlanHead& top, const uint32_t* counts,
+                        uint32_t* cursor, uint32_t c_begin, uint32_t c_end) {
   if (p.n != 0 || p.exhausted) return;
   uint32_t c = 0;
   if (lane_id() == 0) c = atomicAdd(cursor, 1u);
@@ -347,7 +386,7 @@ OL_DEV void pool_refill(Pool& p, const Wave& W, const uint32_t* prefix, const ui
     p.exhausted = true;
     return;
   }
-  const ChunkLoc k = locate_chunk(prefix, counts, W.jobs, c_begin + c);
+  const ChunkLoc k = locate_chunk(prefix, top, counts, W.jobs, c_begin + c);
   const DScene& sc = W.jobs[uniform(k.job)].P.sc;
   p.slot = uniform(k.first);
   p.n = uniform(k.count);
@@ -363,6 +402,7 @@ OL_DEV void closest_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
   if (chunks == 0) return;
+  const PlanHead top = plan_head(prefix);
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
@@ -381,7 +421,7 @@ OL_DEV void closest_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= kRefillMin) {
-      pool_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      pool_refill(pool, W, prefix, top, counts, cursor, c_begin, c_end);
       const uint32_t take = min(n_idle, pool.n);
       if (!busy) {
         const uint32_t rank = lanes_below(idle);
@@ -456,6 +496,7 @@ OL_DEV void shadow_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* counts = W.ctr + c_groups(in ^ 1u);
   const uint32_t chunks = prefix[kRuns];
   if (chunks == 0) return;
+  const PlanHead top = plan_head(prefix);
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
@@ -474,7 +515,7 @@ OL_DEV void shadow_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= kRefillMin) {
-      pool_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      pool_refill(pool, W, prefix, top, counts, cursor, c_begin, c_end);
       const uint32_t take = min(n_idle, pool.n);
       if (!busy) {
         const uint32_t rank = lanes_below(idle);
@@ -543,8 +584,8 @@ struct Pool2 {
   bool exhausted;
 };
 
-OL_DEV void pool2_refill(Pool2& p, const Wave& W, const uint32_t* prefix, const uint32_t* counts, uint32_t* cursor,
-                         uint32_t c_begin, uint32_t c_end) {
+OL_DEV void pool2_refill(Pool2& p, const Wave& W, const uint32_t* prefix, const PlanHead& top, const uint32_t* counts,
+                         uint32_t* cursor, uint32_t c_begin, uint32_t c_end) {
   if (p.n != 0 || p.exhausted) return;
   uint32_t c = 0;
   if (lane_id() == 0) c = atomicAdd(cursor, 1u);
@@ -553,7 +594,7 @@ OL_DEV void pool2_refill(Pool2& p, const Wave& W, const uint32_t* prefix, const 
     p.exhausted = true;
     return;
   }
-  const ChunkLoc k = locate_chunk(prefix, counts, W.jobs, c_begin + c);
+  const ChunkLoc k = locate_chunk(prefix, top, counts, W.jobs, c_begin + c);
   p.slot = uniform(k.first);
   p.n = uniform(k.count);
   p.job = uniform(k.job);
@@ -567,6 +608,7 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = prefix[kRuns];
   if (chunks == 0) return;
+  const PlanHead top = plan_head(prefix);
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
@@ -584,7 +626,7 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= kRefillMin2) {
-      pool2_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      pool2_refill(pool, W, prefix, top, counts, cursor, c_begin, c_end);
       const uint32_t take = min(n_idle, pool.n);
       if (!busy) {
         const uint32_t rank = lanes_below(idle);
@@ -645,6 +687,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
   const uint32_t* counts = W.ctr + c_groups(in ^ 1u);
   const uint32_t chunks = prefix[kRuns];
   if (chunks == 0) return;
+  const PlanHead top = plan_head(prefix);
   const uint32_t ws = blockIdx.x % kWorkStripes;
   const uint32_t c_begin = (uint32_t)((uint64_t)chunks * ws / kWorkStripes);
   const uint32_t c_end = (uint32_t)((uint64_t)chunks * (ws + 1u) / kWorkStripes);
@@ -662,7 +705,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
     const uint64_t idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (n_idle >= kRefillMin2) {
-      pool2_refill(pool, W, prefix, counts, cursor, c_begin, c_end);
+      pool2_refill(pool, W, prefix, top, counts, cursor, c_begin, c_end);
       const uint32_t take = min(n_idle, pool.n);
       if (!busy) {
         const uint32_t rank = lanes_below(idle);
@@ -1129,9 +1172,10 @@ OL_DEV void texture_surface(const DScene& sc, uint32_t mat_id, uint32_t prim, fl
 __global__ void __launch_bounds__(256) k_texture(Wave W, uint32_t in, uint32_t gen) {
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
+  const PlanHead top = plan_head(W.ctr + C_RP);
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
   for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < flat_chunks; c += waves) {
-    const ChunkLoc k = locate_chunk(W.ctr + C_RP, counts, W.jobs, uniform(c));
+    const ChunkLoc k = locate_chunk(W.ctr + C_RP, top, counts, W.jobs, uniform(c));
     const uint32_t job = uniform(k.job);
     const JobDev& J = W.jobs[job];
     const DScene& sc = J.P.sc;
@@ -1184,11 +1228,12 @@ __global__ void __launch_bounds__(256) k_restir(Wave W, uint32_t in, uint32_t ge
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
   const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
+  const PlanHead top = plan_head(W.ctr + C_RP), top2 = plan_head(W.ctr + C_RP2);
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
   for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
     const uint32_t cu = uniform(c);
-    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, counts, W.jobs, cu)
-                                        : locate_chunk(W.ctr + C_RP2, counts, W.jobs, cu - flat_chunks);
+    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, top, counts, W.jobs, cu)
+                                        : locate_chunk(W.ctr + C_RP2, top2, counts, W.jobs, cu - flat_chunks);
     const JobDev& J = W.jobs[uniform(k.job)];
     const GiParams& P = J.P;
     if (!P.restir) continue;
@@ -1550,11 +1595,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t flat_chunks = W.ctr[C_RP + kRuns];
   const uint32_t chunks = flat_chunks + W.ctr[C_RP2 + kRuns];
+  const PlanHead top = plan_head(W.ctr + C_RP), top2 = plan_head(W.ctr + C_RP2);
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
   for (uint32_t c = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
     const uint32_t cu = uniform(c);
-    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, counts, W.jobs, cu)
-                                        : locate_chunk(W.ctr + C_RP2, counts, W.jobs, cu - flat_chunks);
+    const ChunkLoc k = cu < flat_chunks ? locate_chunk(W.ctr + C_RP, top, counts, W.jobs, cu)
+                                        : locate_chunk(W.ctr + C_RP2, top2, counts, W.jobs, cu - flat_chunks);
     const uint32_t i = k.first + lane_id();
     uint32_t path = 0;
     float4 ro = make_float4(0, 0, 0, 0), rd = ro, hv = ro;
@@ -1584,10 +1630,11 @@ k_primary(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
   TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
   const uint32_t* counts = W.ctr + c_queue(in);
   const uint32_t chunks = W.ctr[C_RP + kRuns];
+  const PlanHead top = plan_head(W.ctr + C_RP);
   const uint32_t waves = gridDim.x * (BLOCK / 64u);
   uint32_t nv = 0, nt = 0;
   for (uint32_t c = blockIdx.x * (BLOCK / 64u) + threadIdx.x / 64u; c < chunks; c += waves) {
-    const ChunkLoc k = locate_chunk(W.ctr + C_RP, counts, W.jobs, uniform(c));
+    const ChunkLoc k = locate_chunk(W.ctr + C_RP, top, counts, W.jobs, uniform(c));
     const DScene& sc = W.jobs[uniform(k.job)].P.sc;
     const uint32_t i = k.first + lane_id();
     uint32_t path = 0;
