@@ -152,7 +152,8 @@ int olhip_scene_update(olhip_ctx* ctx, olhip_scene* scene, const olhip_scene_des
 
 /* Host-only: the binned-SAH binary hierarchy a two-level scene's top level is
  * built from (no counterpart in the reference, whose top level is built by
- * the driver).  boxes [n][6] (min xyz, max xyz), n >= 2; order [n] is the
+ * the driver): centroid-binned splits, and the largest box split off alone
+ * where that prices lower.  boxes [n][6] (min xyz, max xyz), n >= 2; order [n] is the
  * primitive of each leaf position; internal node i (0 = root, n - 1 nodes)
  * has children left/right (internal index, or leaf position | 0x80000000)
  * and leaf range [first, last]. */

@@ -467,12 +467,38 @@ uint64_t instanced_hash(const olhip_instanced_desc* d, const std::vector<uint4>&
 // Instance records and the top-level BVH over the instances' world bounds
 // (a box is the bounds of the degenerate triangle (min, max, min)); sets the
 // scene view.  The bottom levels must be built.
+// An instance whose box has at least this fraction of the scene box's
+// surface area (and is not the only one) is entered after the rest of the
+// scene: a ray reaches such a box at its entry into the scene, before any
+// other instance has given it a best hit, and would otherwise traverse it
+// along its whole length.
+#ifndef OL_LATER_AREA
+#define OL_LATER_AREA 0.25
+#endif
+constexpr double kLaterArea = OL_LATER_AREA;
+#ifndef OL_ISOLATE_LARGE
+#define OL_ISOLATE_LARGE 1
+#endif
+constexpr bool kIsolateLarge = OL_ISOLATE_LARGE != 0;
+
+double box_area(const float* b) {
+  const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
 void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t s) {
   const uint32_t R = d->resource_count, I = d->instance_count;
   std::vector<InstanceDev> inst(I);
   std::vector<float4> tpos(2ull * I);
   std::vector<uint4> tvidx(I);
   uint64_t tri_total = 0;
+  float scene_box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t k = 0; k < I; ++k)
+    for (int a = 0; a < 3; ++a) {
+      scene_box[a] = std::min(scene_box[a], d->world_bounds[6ull * k + a]);
+      scene_box[3 + a] = std::max(scene_box[3 + a], d->world_bounds[6ull * k + 3 + a]);
+    }
+  const double later_area = kLaterArea * box_area(scene_box);
   for (uint32_t k = 0; k < I; ++k) {
     require(d->instance_resource[k] < R, "instance resource out of range");
     require(d->instance_material[k] < d->material_count, "instance material out of range");
@@ -497,6 +523,8 @@ void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t
               "instance spheres need a finite centre and a radius >= 0");
       x.sphere = make_float4(sp[0], sp[1], sp[2], sp[3]);
     }
+    // The "later" flag is the sign bit of the sphere radius (traversal reads |r|).
+    if (I > 1 && box_area(b) >= later_area) x.sphere.w = std::copysign(x.sphere.w, -1.0f);
     tpos[2 * k] = make_float4(b[0], b[1], b[2], 0.0f);
     tpos[2 * k + 1] = make_float4(b[3], b[4], b[5], 0.0f);
     tvidx[k] = make_uint4(2 * k, 2 * k + 1, 2 * k, 0u);
@@ -505,7 +533,7 @@ void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t
   upload(sc->tlas_pos, tpos.data(), tpos.size(), s);
   upload(sc->tlas_vidx, tvidx.data(), tvidx.size(), s);
   OL_HIP_CHECK(hipStreamSynchronize(s));
-  BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u, d->world_bounds};
+  BvhInput tin{sc->tlas_pos.as<float4>(), sc->tlas_vidx.as<uint4>(), I, 1u, d->world_bounds, kIsolateLarge};
   bvh_build(tin, sc->bvh, s, nullptr);
   // The instance table in top-level leaf order: a top-level leaf's first
   // slot is then its instance's index, and entering an instance needs no
@@ -772,7 +800,7 @@ int olhip_sah_topology(const float* boxes, uint32_t n, uint32_t* order, uint32_t
     require(boxes && order && left && right && first && last, "arguments must not be NULL");
     require(n >= 2, "at least two primitives are required");
     std::vector<uint32_t> o, l, r, f, la, pi, pl;
-    sah_topology(boxes, n, o, l, r, f, la, pi, pl);
+    sah_topology(boxes, n, o, l, r, f, la, pi, pl, kIsolateLarge);
     std::copy(o.begin(), o.end(), order);
     std::copy(l.begin(), l.end(), left);
     std::copy(r.begin(), r.end(), right);

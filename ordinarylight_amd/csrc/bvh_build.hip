@@ -551,13 +551,17 @@ void build_boxes(const BvhInput& in, const BvhTopology& tp, BvhOutput& out, hipS
 
 // Binned SAH over primitive boxes, one primitive per leaf: a top-down split
 // of [b, e) at the best of 16 centroid bins on each axis (surface area x
-// count), the middle when every centroid coincides.  Internal nodes are
-// numbered in creation order from the root (0); leaves are the positions of
-// the final order, so every node covers a contiguous leaf range, as in the
-// Karras hierarchy.
+// count), the middle when every centroid coincides.  With `isolate`, one more
+// candidate is priced the same way: the primitive of largest box alone
+// against the rest.  Centroid bins cannot separate a box that spans much of
+// its node from the small boxes around its centre, so without it every node
+// on such a primitive's path takes its size.  Internal nodes are numbered in
+// creation order from the root (0); leaves are the positions of the final
+// order, so every node covers a contiguous leaf range, as in the Karras
+// hierarchy.
 void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, std::vector<uint32_t>& left,
                   std::vector<uint32_t>& right, std::vector<uint32_t>& rfirst, std::vector<uint32_t>& rlast,
-                  std::vector<uint32_t>& parent_int, std::vector<uint32_t>& parent_leaf) {
+                  std::vector<uint32_t>& parent_int, std::vector<uint32_t>& parent_leaf, bool isolate) {
   constexpr int kBins = 16;
   order.resize(n);
   for (uint32_t i = 0; i < n; ++i) order[i] = i;
@@ -625,8 +629,29 @@ void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, st
         if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = i; }
       }
     }
+    // The largest box alone against the rest (when centroid bins gave a
+    // candidate: boxes of one centroid keep the middle split).
+    uint32_t big = OL_NONE;
+    if (isolate && best_axis >= 0 && t.e - t.b > 2) {
+      float big_area = -1.0f;
+      for (uint32_t k = t.b; k < t.e; ++k) {
+        const float a = area(box + 6 * order[k], box + 6 * order[k] + 3);
+        if (a > big_area) { big_area = a; big = k; }
+      }
+      float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t k = t.b; k < t.e; ++k) {
+        if (k == big) continue;
+        const float* pb = box + 6 * order[k];
+        for (int q = 0; q < 3; ++q) { mn[q] = std::min(mn[q], pb[q]); mx[q] = std::max(mx[q], pb[3 + q]); }
+      }
+      const float cost = big_area + area(mn, mx) * (float)(t.e - t.b - 1);
+      if (!(cost < best_cost)) big = OL_NONE;
+    }
     uint32_t mid;
-    if (best_axis < 0) {
+    if (big != OL_NONE) {
+      std::swap(order[big], order[t.e - 1]);
+      mid = t.e - 1;
+    } else if (best_axis < 0) {
       mid = t.b + (t.e - t.b) / 2;
     } else {
       const int a = best_axis;
@@ -718,7 +743,7 @@ void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t s, BvhDebug* debu
   tp->leaf_size = in.leaf_size == 0 ? 4u : (in.leaf_size > OL_LEAF_MAX ? OL_LEAF_MAX : in.leaf_size);
   if (in.host_boxes && n > 1) {
     std::vector<uint32_t> order, l, r, f, la, pi, pl;
-    sah_topology(in.host_boxes, n, order, l, r, f, la, pi, pl);
+    sah_topology(in.host_boxes, n, order, l, r, f, la, pi, pl, in.isolate_large);
     upload_vec(tp->sorted, order, s);
     upload_vec(tp->left, l, s);
     upload_vec(tp->right, r, s);

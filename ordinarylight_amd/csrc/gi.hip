@@ -638,7 +638,7 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
           if (W.jobs[job].P.sc.node_count == 0) {
             W.hit[slot] = make_float4(d.w, 0.0f, 0.0f, __uint_as_float(OL_NONE));
           } else {
-            st.sp = 0;
+            two_level_start(st);
             busy = true;
           }
         }
@@ -725,7 +725,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
           } else if (e < e_end) {
             const float4 o = W.eo[e], d = W.ed[e];
             any2_begin(q, mk3(o), mk3(d), o.w, d.w);
-            st.sp = 0;
+            two_level_start(st);
             busy = true;
           } else {
             shadow_resolve(W, g, direct, area_direct, env_direct);
@@ -760,7 +760,7 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       if (e < e_end) {
         const float4 o = W.eo[e], d = W.ed[e];
         any2_begin(q, mk3(o), mk3(d), o.w, d.w);
-        st.sp = 0;
+        two_level_start(st);
       } else {
         shadow_resolve(W, g, direct, area_direct, env_direct);
         busy = false;
@@ -830,7 +830,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_rays2(DScene sc, const float4* 
     const float4 o = o4[i], d = d4[i];
     Closest2 q;
     closest2_begin(q, mk3(o), mk3(d), o.w, d.w);
-    st.sp = 0;
+    two_level_start(st);
     if (sc.node_count == 0) q.node = OL_NONE;
     while (q.node != OL_NONE) {
       if (closest2_at_leaf(q)) closest2_leaf<LSTACK>(q, st, nt);

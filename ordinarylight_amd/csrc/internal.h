@@ -80,6 +80,7 @@ struct BvhInput {
   // primitive): the binary hierarchy is then built on the host by binned SAH
   // instead of Morton codes (small top levels), the boxes phase as usual.
   const float* host_boxes = nullptr;
+  bool isolate_large = false;  // sah_topology's `isolate` (top levels over instances)
 };
 
 // Binary LBVH topology of a build (Morton order and Karras hierarchy), kept
@@ -120,10 +121,11 @@ struct BvhDebug {
 
 // Host binned-SAH binary hierarchy over n >= 2 primitive boxes (min xyz, max
 // xyz each), one primitive per leaf, in the Karras arrays' conventions: leaf
-// order, children (leaf refs | OL_LEAF_BIT), leaf ranges, parents.
+// order, children (leaf refs | OL_LEAF_BIT), leaf ranges, parents.  isolate:
+// also price splitting off the largest box alone (top levels over instances).
 void sah_topology(const float* box, uint32_t n, std::vector<uint32_t>& order, std::vector<uint32_t>& left,
                   std::vector<uint32_t>& right, std::vector<uint32_t>& rfirst, std::vector<uint32_t>& rlast,
-                  std::vector<uint32_t>& parent_int, std::vector<uint32_t>& parent_leaf);
+                  std::vector<uint32_t>& parent_int, std::vector<uint32_t>& parent_leaf, bool isolate = false);
 
 // keep_topology: retain the binary topology in out.topology for bvh_refit.
 void bvh_build(const BvhInput& in, BvhOutput& out, hipStream_t stream, BvhDebug* debug, bool keep_topology = false);

@@ -219,6 +219,10 @@ struct TravStack {
     return spill[(sp - LSTACK) * spill_stride];
   }
   OL_DEV bool empty() const { return sp == 0; }
+  // The bottom entry (always in LDS), for the two-level traversal's
+  // held-back instance.
+  OL_DEV uint32_t bottom() const { return lds[0]; }
+  OL_DEV void set_bottom(uint32_t v) { lds[0] = v; }
 };
 
 // Children of a node in child-word order.
@@ -441,7 +445,36 @@ OL_DEV TraceHit ol_trace_closest(const WideNode* __restrict__ nodes, const float
 // instances compare directly.  Ties resolve to the lowest packed-order
 // triangle index (instance triangle offset + local index), the order of the
 // flattened scene.
+//
+// An instance flagged "later" (the sign bit of its sphere radius; set on the
+// host for instances whose box spans a large part of the scene) is not
+// entered when the top-level traversal reaches it, but once everything else
+// has been visited, with the best hit of the rest of the scene known, so its
+// nodes beyond that hit are culled (any hit: an occluder found elsewhere ends
+// the ray without entering it).  A two-level traversal starts with OL_NONE
+// pushed at the bottom of the stack (two_level_start), which ends it when
+// popped; holding an instance back overwrites that bottom entry with the
+// instance's leaf word, so it is popped last, at no cost to the other steps.
+// One instance per ray is held back (the bottom entry reads OL_NONE only
+// while the slot is free); another reached later is entered at once.  The
+// visiting order does not change the result.
 constexpr uint32_t OL_SENTINEL = 0x7FFFFFFFu;
+
+OL_DEV bool instance_later(const float4& sphere) { return (__float_as_uint(sphere.w) >> 31) != 0u; }
+
+template <int LSTACK>
+OL_DEV void two_level_start(TravStack<LSTACK>& st) {
+  st.sp = 0;
+  st.push(OL_NONE);
+}
+
+// Holds the instance at leaf word `word` back when the slot is free.
+template <int LSTACK>
+OL_DEV bool hold_back(TravStack<LSTACK>& st, uint32_t word) {
+  if (st.sp == 0 || st.bottom() != OL_NONE) return false;
+  st.set_bottom(word);
+  return true;
+}
 
 OL_DEV f3 xform_point(const float4* rows, f3 p) {
   return mk3(((rows[0].x * p.x + rows[0].y * p.y) + rows[0].z * p.z) + rows[0].w,
@@ -468,7 +501,7 @@ OL_DEV bool sphere_reachable(const RayPre& r, float4 s) {
   const float tc = dot3(oc, r.d) * __builtin_amdgcn_rcpf(dot3(r.d, r.d));
   const f3 v = oc - r.d * tc;
   const float scale = fmaxf(fmaxf(max_abs3(r.o), max_abs3(c)), fabsf(tc) * max_abs3(r.d));
-  const float reach = s.w + 1e-5f * scale;
+  const float reach = fabsf(s.w) + 1e-5f * scale;
   return dot3(v, v) <= reach * reach;
 }
 
@@ -479,7 +512,8 @@ OL_DEV bool sphere_reachable(const RayPre& r, float4 s) {
 // of the instance lies in the sphere, so an instance failing the test holds
 // no hit at t <= tmax: a closest-hit ray skips it once a nearer hit is known.
 // The divisions and square roots use the hardware approximations (within an
-// ulp or two): the allowances above are orders of magnitude wider.
+// ulp or two): the allowances above are orders of magnitude wider.  The
+// radius's sign bit is the "later" flag, not part of the radius.
 OL_DEV bool sphere_before(const RayPre& r, float4 s, float tmax) {
   const f3 c = mk3(s.x, s.y, s.z);
   const f3 oc = c - r.o;
@@ -487,12 +521,13 @@ OL_DEV bool sphere_before(const RayPre& r, float4 s, float tmax) {
   const float tc = dot3(oc, r.d) * __builtin_amdgcn_rcpf(dd);
   const f3 v = oc - r.d * tc;
   const float scale = fmaxf(fmaxf(max_abs3(r.o), max_abs3(c)), fabsf(tc) * max_abs3(r.d));
-  const float reach = s.w + 1e-5f * scale;
+  const float radius = fabsf(s.w);
+  const float reach = radius + 1e-5f * scale;
   const float h2 = reach * reach - dot3(v, v);
   if (!(h2 >= 0.0f)) return false;
   const float inv_len = __builtin_amdgcn_rsqf(dd);
   const float t_in = tc - __builtin_amdgcn_sqrtf(h2) * inv_len;
-  return t_in <= tmax + 1e-4f * (s.w + scale) * inv_len;
+  return t_in <= tmax + 1e-4f * (radius + scale) * inv_len;
 }
 
 struct Closest2 {
@@ -537,7 +572,7 @@ template <int LSTACK>
 OL_DEV void closest2_enter(const DScene& sc, Closest2& q, TravStack<LSTACK>& st, uint32_t& ni) {
   const uint32_t k = leaf_first(q.node);  // the instance table is in top-level leaf order
   const InstanceDev& I = sc.inst[k];
-  if (!sphere_before(q.w, I.sphere, q.best.t)) {
+  if (!sphere_before(q.w, I.sphere, q.best.t) || (instance_later(I.sphere) && hold_back(st, q.node))) {
     closest2_next(q, st);
     return;
   }
@@ -637,7 +672,7 @@ template <int LSTACK>
 OL_DEV void any2_enter(const DScene& sc, Any2& q, TravStack<LSTACK>& st, uint32_t& ni) {
   const uint32_t k = leaf_first(q.node);  // the instance table is in top-level leaf order
   const InstanceDev& I = sc.inst[k];
-  if (!sphere_before(q.w, I.sphere, q.tmax)) {
+  if (!sphere_before(q.w, I.sphere, q.tmax) || (instance_later(I.sphere) && hold_back(st, q.node))) {
     any2_next(q, st);
     return;
   }

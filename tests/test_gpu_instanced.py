@@ -39,6 +39,10 @@ DISTANCE_K = 8.0
 
 
 def small_instanced(seed=0):
+    """60 instances of 3 icospheres in a 12-unit cube and 40 emitters
+    scattered through it: the emitter mesh's box spans the scene, so its
+    instance is split off alone near the top-level root and entered after
+    the rest of the scene (the held-back entry), as in the full-size scene."""
     return workloads.instanced_spheres(seed, meshes=3, subdivisions=2, instances=60, box=12.0, dielectric=0.2,
                                        emitters=40, emitter_size=0.3)
 

@@ -292,6 +292,36 @@ def test_sah_separates_two_clusters_at_the_root():
     assert set(left_prims.tolist()) in (set(range(50)), set(range(50, 120)))
 
 
+def test_sah_splits_off_a_box_spanning_the_field():
+    """A box as large as the whole field (a mesh of scattered emitters, say)
+    among 2,000 small ones: centroid bins alone would keep it with the small
+    boxes round the centre on every level, each node on its path taking its
+    size; the top-level build splits it off alone within two levels of the
+    root."""
+    rng = np.random.default_rng(8)
+    c = rng.uniform(-50, 50, (2000, 3))
+    boxes = np.concatenate([c - 0.5, c + 0.5], 1)
+    boxes = np.concatenate([boxes, [[-50, -50, -50, 50, 50, 50]]])
+    order, left, right, first, last = sah(boxes)
+    LEAF = 0x80000000
+    leaf_of = {int(order[k]): k for k in range(len(order))}
+    depth, node, target = 0, 0, leaf_of[2000]
+    while True:
+        nxt = None
+        for ref in (left[node], right[node]):
+            if ref & LEAF:
+                if int(ref) & 0x7FFFFFFF == target:
+                    nxt = -1
+            elif first[ref] <= target <= last[ref]:
+                nxt = int(ref)
+        assert nxt is not None
+        depth += 1
+        if nxt < 0:
+            break
+        node = nxt
+    assert depth <= 2
+
+
 def test_packed_material_arrays_hold_one_record_per_material():
     """Both packers hand the C ABI [M, 12, 4] records, M = the material count
     it reads (48 floats each), and every material index lies below M."""
