@@ -656,11 +656,13 @@ OL_DEV void closest2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       if (pool.exhausted && pool.n == 0) break;
       continue;
     }
-    // Leaf, entry or node step: a class runs when it holds a third of the
-    // busy lanes or nothing else is left (leaves first, then entries).
-    if (leafy && ((entries | inner) == 0 || 3u * (uint32_t)__popcll(leafy) >= n_busy)) {
+    // Leaf, entry or node step: a class runs when it holds a quarter of the
+    // busy lanes or nothing else is left (leaves first, then entries).  A
+    // quarter measured faster than a third or a fifth for both classes on
+    // the instanced sphere field (the flat kernels keep a third).
+    if (leafy && ((entries | inner) == 0 || 4u * (uint32_t)__popcll(leafy) >= n_busy)) {
       if (at_leaf) closest2_leaf<LSTACK>(q, st, nt);
-    } else if (entries && (inner == 0 || 3u * (uint32_t)__popcll(entries) >= n_busy)) {
+    } else if (entries && (inner == 0 || 4u * (uint32_t)__popcll(entries) >= n_busy)) {
       if (at_entry) closest2_enter<LSTACK>(W.jobs[job].P.sc, q, st, ni);
     } else if (inner) {
       if (busy && !at_leaf && !at_entry) closest2_node<LSTACK>(W.jobs[job].P.sc, q, st, nv);
@@ -745,9 +747,9 @@ OL_DEV void shadow2_pass(const Wave& W, uint32_t in, TravStack<LSTACK>& st) {
       if (pool.exhausted && pool.n == 0) break;
       continue;
     }
-    if (leafy && ((entries | inner) == 0 || 3u * (uint32_t)__popcll(leafy) >= n_busy)) {
+    if (leafy && ((entries | inner) == 0 || 4u * (uint32_t)__popcll(leafy) >= n_busy)) {
       if (at_leaf) any2_leaf<LSTACK>(q, st, nt);
-    } else if (entries && (inner == 0 || 3u * (uint32_t)__popcll(entries) >= n_busy)) {
+    } else if (entries && (inner == 0 || 4u * (uint32_t)__popcll(entries) >= n_busy)) {
       if (at_entry) any2_enter<LSTACK>(W.jobs[job].P.sc, q, st, ni);
     } else if (inner) {
       if (busy && !at_leaf && !at_entry) any2_node<LSTACK>(W.jobs[job].P.sc, q, st, nv);
