@@ -794,9 +794,15 @@ k_trace(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
   shadow_pass<LSTACK>(W, in, st);
 }
 
-// The same for jobs of two-level scenes.
+// The same for jobs of two-level scenes.  Five waves per SIMD (96 VGPRs, a
+// few registers of the shadow pass spilled) instead of the four that the
+// shadow pass's 111 VGPRs allow: dependent node and instance loads need the
+// extra wave more than the spilled registers cost (instanced sphere field:
+// traversal 73 -> 65.5 ms per frame; six waves spill 104 bytes per lane and
+// are slower than four).
 template <int LSTACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace2(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace2(Wave W, uint32_t in, uint32_t* spill, uint32_t spill_cap) {
   __shared__ uint32_t lds[LSTACK * BLOCK];
   TravStack<LSTACK> st = lane_stack<LSTACK, BLOCK>(lds, spill, spill_cap, &W.ctr[C_STACK]);
   closest2_pass<LSTACK>(W, in, st);
