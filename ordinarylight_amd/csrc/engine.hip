@@ -662,7 +662,9 @@ bool Engine::launch_next() {
       traced = true;
       if (!((Ls.fresh >> k) & 1ull)) all_fresh = false;
       kinds |= j.desc.scene->view.instanced ? STEP_INSTANCED : STEP_FLAT;
-      if (j.desc.settings.restir_candidates) kinds |= STEP_RESTIR;
+      // The reservoir pass and its replay in shading run at primary hits
+      // only: the step that generated the chunk.
+      if (j.desc.settings.restir_candidates && ((Ls.fresh >> k) & 1ull)) kinds |= STEP_RESTIR;
       if (j.desc.scene->view.textured) kinds |= STEP_TEXTURED;
       if (jobs_state_[k].P.two_pass) kinds |= STEP_TWO_PASS;
     }

@@ -200,9 +200,11 @@ int stack_depth_for(uint32_t bvh_depth);
 // shadow groups this step's shading emits.
 // `kinds` says which resident work a step holds (STEP_* bits); the launches
 // for absent kinds are left out (their chunk totals stay zero).
-// STEP_TWO_PASS: some job evaluates more than one next-event sample per hit
-// (point lights beside the environment, or per-domain NEE); shading without
-// it, ReSTIR or textures runs a variant compiled without those paths.
+// STEP_RESTIR: a ReSTIR job's chunk is at its primary hits (generated this
+// step).  STEP_TWO_PASS: some job evaluates more than one next-event sample
+// per hit (point lights beside the environment, or per-domain NEE); shading
+// without it, ReSTIR or textures runs a variant compiled without those
+// paths.
 constexpr uint32_t STEP_FLAT = 1u, STEP_INSTANCED = 2u, STEP_RESTIR = 4u, STEP_TEXTURED = 8u, STEP_TWO_PASS = 16u;
 constexpr uint32_t kStepKinds = 31u;
 // Fills b.shade from the hierarchy's leaf triangles, the triangles' vertex
