@@ -90,23 +90,28 @@ float elapsed(const Event& a, const Event& b) {
 
 constexpr uint64_t kFnvBasis = 1469598103934665603ull;
 // Emitter selection guide (DScene::area_guide): entry j is the first emitter
-// whose cumulative weight (record word 5 * i + 4 .x) is >= j / kAreaGuide,
-// clamped to the last emitter; entry kAreaGuide closes the last bucket.
+// whose cumulative weight (record word 5 * i + 4 .x) is >= j / size, clamped
+// to the last emitter; entry `size` closes the last bucket.  j / size is
+// exact in float32 for a power-of-two size.
 void upload_area_guide(olhip_scene* sc, const float* area_lights, uint32_t area_count, hipStream_t s) {
   if (area_count == 0) {
     sc->view.area_guide = nullptr;
+    sc->view.area_guide_size = 0;
     return;
   }
-  std::vector<uint32_t> guide(kAreaGuide + 1u);
+  uint32_t size = kAreaGuideMin;
+  while (size < kAreaGuideMax && size < 2ull * area_count) size <<= 1;
+  std::vector<uint32_t> guide(size + 1u);
   uint32_t i = 0;
-  for (uint32_t j = 0; j <= kAreaGuide; ++j) {
-    const float x = (float)j / (float)kAreaGuide;
+  for (uint32_t j = 0; j <= size; ++j) {
+    const float x = (float)j / (float)size;
     while (i + 1u < area_count && !(x <= area_lights[20ull * i + 16])) ++i;
     guide[j] = i;
   }
   upload(sc->area_guide, guide.data(), guide.size(), s);
   OL_HIP_CHECK(hipStreamSynchronize(s));
   sc->view.area_guide = sc->area_guide.as<uint32_t>();
+  sc->view.area_guide_size = size;
 }
 
 uint64_t fnv1a(const void* data, size_t bytes, uint64_t h) {

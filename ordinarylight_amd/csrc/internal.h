@@ -185,10 +185,13 @@ struct DScene {
   uint32_t instanced, instance_count;
   const InstanceDev* inst;
   const BlasDev* blas;
-  // Emitter selection guide (null without area lights): entry j is the
-  // first emitter whose cumulative weight reaches j / kAreaGuide (clamped to
-  // the last), so a selection in bucket j lies between entries j and j + 1.
+  // Emitter selection guide (null without area lights): area_guide_size
+  // buckets (a power of two, so bucket j = floor(selection * size) is exact),
+  // entry j the first emitter whose cumulative weight reaches j / size
+  // (clamped to the last), so a selection in bucket j lies between entries j
+  // and j + 1.
   const uint32_t* area_guide;
+  uint32_t area_guide_size;
   // Per-triangle material parameters (null without): base colour and
   // roughness per leaf slot (the metallic factor is the shading record's
   // word 11), and per original triangle (the reference estimator's colours).
@@ -208,4 +211,7 @@ struct DScene {
   float env_rotation, env_log_range;
   float env_color_r, env_color_g, env_color_b, env_intensity;
 };
-constexpr uint32_t kAreaGuide = 4096;
+// Guide buckets: the power of two at or above twice the emitter count, at
+// least kAreaGuideMin and at most kAreaGuideMax, so that a bucket holds
+// about half an emitter and the bisection inside it rarely takes a step.
+constexpr uint32_t kAreaGuideMin = 4096, kAreaGuideMax = 1u << 22;

@@ -240,7 +240,7 @@ OL_DEV NeeSample point_sample(const DScene& sc, uint32_t li, f3 hit, f3 n, const
 OL_DEV uint32_t select_area_light(const DScene& sc, float selection) {
   uint32_t lower = 0, upper = sc.area_count - 1u;
   if (sc.area_guide) {
-    const uint32_t b = min((uint32_t)(selection * (float)kAreaGuide), kAreaGuide - 1u);
+    const uint32_t b = min((uint32_t)(selection * (float)sc.area_guide_size), sc.area_guide_size - 1u);
     lower = sc.area_guide[b];
     upper = sc.area_guide[b + 1u];
   }
