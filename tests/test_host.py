@@ -320,6 +320,18 @@ def test_sah_splits_off_a_box_spanning_the_field():
             break
         node = nxt
     assert depth <= 2
+    # Many nested boxes of slowly shrinking size, each spanning most of the
+    # field: the binned splits price at least as low as splitting off the
+    # largest, so no chain of single-box levels forms (one level per box would
+    # be 512 deep).
+    c = rng.uniform(-1, 1, (512, 3))
+    h = 50.0 * 0.995 ** np.arange(512)[:, None]
+    boxes = np.concatenate([c - h, c + h], 1)
+    order, left, right, first, last = sah(boxes)
+
+    def depth_of(ref):
+        return 0 if ref & LEAF else 1 + max(depth_of(left[ref]), depth_of(right[ref]))
+    assert depth_of(0) <= 40
 
 
 def test_packed_material_arrays_hold_one_record_per_material():
