@@ -469,28 +469,23 @@ uint64_t instanced_hash(const olhip_instanced_desc* d, const std::vector<uint4>&
   return h;
 }
 
-// Instance records and the top-level BVH over the instances' world bounds
-// (a box is the bounds of the degenerate triangle (min, max, min)); sets the
-// scene view.  The bottom levels must be built.
 // An instance whose box has at least this fraction of the scene box's
 // surface area (and is not the only one) is entered after the rest of the
 // scene: a ray reaches such a box at its entry into the scene, before any
 // other instance has given it a best hit, and would otherwise traverse it
 // along its whole length.
-#ifndef OL_LATER_AREA
-#define OL_LATER_AREA 0.25
-#endif
-constexpr double kLaterArea = OL_LATER_AREA;
-#ifndef OL_ISOLATE_LARGE
-#define OL_ISOLATE_LARGE 1
-#endif
-constexpr bool kIsolateLarge = OL_ISOLATE_LARGE != 0;
+constexpr double kLaterArea = 0.25;
+// Top levels price splitting off their largest box (sah_topology).
+constexpr bool kIsolateLarge = true;
 
 double box_area(const float* b) {
   const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
   return dx * dy + dy * dz + dz * dx;
 }
 
+// Instance records and the top-level BVH over the instances' world bounds
+// (a box is the bounds of the degenerate triangle (min, max, min)); sets the
+// scene view.  The bottom levels must be built.
 void build_top_level(olhip_scene* sc, const olhip_instanced_desc* d, hipStream_t s) {
   const uint32_t R = d->resource_count, I = d->instance_count;
   std::vector<InstanceDev> inst(I);
