@@ -297,6 +297,31 @@ def batched_measurement(context, scenes, n_jobs, seed, autotune_on=True):
     }
 
 
+def c4_phase(context, scenes, n_jobs, phase):
+    """One path of the mixed workload alone (profiling aid): the 64 jobs back
+    to back on the plain path, or in one batched engine run with the settings
+    the autotuner picks for them on this device (2^27 path slots, 16 LDS stack
+    entries, 256-thread traversal blocks); one untimed warm-up pass first."""
+    from ordinarylight_amd import batch, workloads
+    jobs = make_jobs(workloads.mixed_jobs(0, n_jobs), scenes)
+    if phase == "plain":
+        for j in jobs:
+            batch.plain_render(j, copy_image=False)
+        t0 = time.perf_counter()
+        for j in jobs:
+            batch.plain_render(j, copy_image=False)
+        return {"phase": phase, "jobs": n_jobs, "jobs_per_s": round(n_jobs / (time.perf_counter() - t0), 2)}
+    settings = batch.EngineSettings(path_slots=1 << 27, lds_stack=16, trace_block=256)
+    with batch.BatchEngine(context, settings) as engine:
+        batch.run_batch(engine, jobs)
+        st0 = engine.stats()
+        seconds, _ = batch.run_batch(engine, jobs)
+        est = engine.stats()
+    return {"phase": phase, "jobs": n_jobs, "jobs_per_s": round(n_jobs / seconds, 2),
+            "engine_steps": int(est["steps"] - st0["steps"]),
+            "device_busy_frac": round((est["device_ms"] - st0["device_ms"]) / (seconds * 1e3), 3)}
+
+
 def served_run(context, jobs, settings, rate, seed, check=8, plain_server=False):
     """Poisson arrivals of ``jobs`` at ``rate``/s through the continuous-batching
     server; with ``plain_server`` also a plain one-job-at-a-time server on the
@@ -590,6 +615,8 @@ def main(argv=None):
                         help="skip the plain frame under the other execution strategy")
     parser.add_argument("--c3-only", action="store_true",
                         help="run only the instanced workload and print its record (profiling aid)")
+    parser.add_argument("--c4-only", choices=("plain", "batched"), default=None,
+                        help="run only the mixed 64-job workload on one path and print its record (profiling aid)")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -598,6 +625,16 @@ def main(argv=None):
 
     from ordinarylight_amd import workloads
     from ordinarylight_amd.renderers.hip import HipPathTracer
+
+    if args.c4_only:
+        from ordinarylight_amd.renderers.hip import DeviceContext
+        context = DeviceContext(0)
+        scenes = build_mixed_scenes(context, builder=args.builder)
+        print(json.dumps({"c4": c4_phase(context, scenes, args.batch_jobs or 64, args.c4_only)}), flush=True)
+        for ds, _ in scenes.values():
+            ds.close()
+        context.close()
+        return 0
 
     if args.c3_only:
         from ordinarylight_amd.renderers.hip import DeviceContext
